@@ -1,0 +1,228 @@
+/*
+ * pixels.h - C ABI of the MI355X (gfx950) brick-map voxel library.
+ *
+ * The library owns one HIP stream and all device memory of a world: the brick
+ * grid, the voxel pool, the super grid, the material table, the control
+ * counters and the edit-chain heap.  Every entry point returns an int status
+ * (PX_OK = 0, negative = error); px_last_error() gives the message.  No C++
+ * exception crosses this boundary.  A context is used from one host thread at
+ * a time (the reference drives its GPUDevice from one JS thread as well).
+ *
+ * Citations are into the reference repository (Crunchy000/pixels).
+ */
+#ifndef PIXELS_H
+#define PIXELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PX_OK 0
+#define PX_ERR_ARG (-1)      /* bad argument (null pointer, size, kind) */
+#define PX_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define PX_ERR_CAPACITY (-3) /* a size exceeds a configured limit */
+#define PX_ERR_STATE (-4)    /* call not valid in the current state */
+#define PX_ERR_NODEVICE (-5) /* no HIP device */
+
+/* ---- encoding constants (src/world/VoxelWorld.ts:4-13, shared.wgsl:44) -- */
+#define PX_NONE 0xFFFFFFFFu
+#define PX_UNIFORM_BIT 0x80000000u
+#define PX_BRICK 8
+#define PX_BRICK_VOXELS 512
+#define PX_UNIFORM_FLOATS 52 /* src/gpu/GpuWorld.ts:8, shared.wgsl:5-19 */
+#define PX_MAT_FLOATS 20     /* src/world/materials.ts:193 */
+#define PX_MAT_COUNT 256
+#define PX_CTRL_WORDS 36     /* layout.ts:48-58 rounded up to 4, layout.ts:67 */
+#define PX_ARGS_WORDS 44     /* layout.ts:41-45 */
+#define PX_CAMERA_FLOATS 64  /* GpuWorld.ts:10 (256 bytes) */
+
+/* Control-counter word indices, in the order of CTRL_NAMES (layout.ts:48-58). */
+enum px_ctrl_word {
+  PX_C_FREE_TOP = 0, PX_C_IMPACTS, PX_C_JOBS, PX_C_ALLOCS, PX_C_ALLOC_N,
+  PX_C_FLOOD_CUR, PX_C_FLOOD_NEXT, PX_C_FLOOD_PAR,
+  PX_C_S_STATE, PX_C_S_CUR, PX_C_S_NEXT, PX_C_S_PAR, PX_C_NODES, PX_C_PENDING,
+  PX_C_S_ITERS, PX_C_SEED_N, PX_C_CHUNKS, PX_C_ARENA, PX_C_WRITES, PX_C_WRITE_N,
+  PX_C_DIRTY, PX_C_PCURSOR, PX_C_FLASH, PX_C_SPAWNED,
+  PX_S_BILL, PX_S_DESTROYED, PX_S_ALLOC_FAIL, PX_S_FALLING, PX_S_SEARCHES, PX_S_ISLANDS,
+  PX_S_SEED_DROP, PX_S_BUDGET, PX_S_HASH_FULL,
+  PX_CTRL_USED
+};
+
+/* Face ids follow dirOf() order (sim.wgsl:27-32): the face is k with
+ * normal == dirOf(k).  6 = zero normal (the ray started inside the volume and
+ * stopped in its first cell), 0xFF = miss, 0xFE = shadow ray not cast. */
+#define PX_FACE_PX 0
+#define PX_FACE_NX 1
+#define PX_FACE_PY 2
+#define PX_FACE_NY 3
+#define PX_FACE_PZ 4
+#define PX_FACE_NZ 5
+#define PX_FACE_ZERO 6
+#define PX_FACE_NOT_CAST 0xFE
+#define PX_FACE_MISS 0xFF
+
+/* ---- ray queries ---------------------------------------------------------
+ * PRIMARY: march(ro, rd, maxT, shadow=false)      raymarch.wgsl:44-162
+ * SHADOW : march(ro, rd, maxT, shadow=true)       raymarch.wgsl:326,354
+ * SHOOT  : traceSolid(ro, rd, maxT)               sim.wgsl:296-357
+ * SEGMENT: particle segment DDA from pos=ro along the displacement rd
+ *          (max_t unused)                         sim.wgsl:1110-1145       */
+#define PX_RAY_PRIMARY 0u
+#define PX_RAY_SHADOW 1u
+#define PX_RAY_SHOOT 2u
+#define PX_RAY_SEGMENT 3u
+#define PX_RAY_KIND_MASK 3u
+
+/* 32-byte ray record: origin and direction in voxel units, FP32. `flags`
+ * carries the query kind in its low 2 bits for the batched entry point. */
+typedef struct px_ray {
+  float ro[3];
+  float rd[3];
+  float max_t;
+  uint32_t flags;
+} px_ray;
+
+/* 16-byte result: voxel key x + DX*(y + DY*z) (sim.wgsl:50) or PX_NONE, t of
+ * the hit (0 on a miss, as the zero-initialised Hit of raymarch.wgsl:45),
+ * step count (gOuterSteps + gInnerSteps, raymarch.wgsl:18-19,84,93,107),
+ * face id, and the low byte of the hit voxel's material id. */
+typedef struct px_hit {
+  uint32_t key;
+  float t;
+  uint32_t steps;
+  uint8_t face;
+  uint8_t mat;
+  uint16_t value_hi; /* high byte of the 16-bit voxel value ("shade") */
+} px_hit;
+
+/* ---- limits (layout.ts:11-33) ------------------------------------------- */
+typedef struct px_limits {
+  uint32_t max_dirty, max_alloc, max_writes, max_impacts, max_jobs, max_flood, max_seeds;
+  uint32_t hash_size, max_chunks, max_arena, max_lights, max_anchors, max_flashes;
+  uint32_t spawn_budget, component_budget, search_iters_per_frame, flood_iters_per_frame;
+} px_limits;
+
+/* ---- world description: what `new GpuWorld(device, world, scene)` takes
+ * (GpuWorld.ts:46-137).  All host pointers; the library copies them. */
+typedef struct px_world_desc {
+  uint32_t dims[3];        /* voxel dims, multiples of 8 (config.ts:26-30) */
+  const uint32_t* grid;    /* (dx/8)*(dy/8)*(dz/8) entries, VoxelWorld.ts:6-8 */
+  const uint16_t* pool;    /* pool_used * 512 voxels, local index x|y<<3|z<<6 */
+  uint32_t pool_used;      /* bricks present in `pool` */
+  const uint32_t* holes;   /* free bricks inside [0, pool_used) (VoxelWorld.freeBricks) */
+  uint32_t n_holes;
+  const float* mats;       /* 256 * 20 floats, packMaterials() layout */
+  uint32_t pool_capacity;  /* 0 = min(limit, max(used + 98304, 2 * used)), GpuWorld.ts:60 */
+  const px_limits* limits; /* NULL = the reference LIMITS */
+  float voxel_size;        /* metres per voxel, U.dims.w */
+  int32_t device;          /* HIP device ordinal */
+} px_world_desc;
+
+typedef struct px_ctx px_ctx;
+
+/* Fills *l with the reference LIMITS (layout.ts:11-33). */
+void px_default_limits(px_limits* l);
+
+/* Replaces `new GpuWorld(device, world, scene)` (GpuWorld.ts:46-137): sizes
+ * the pool, builds the super-grid counts (:90-100) and the free list
+ * (:103-113), writes C_FREE_TOP (:111-113) and uploads everything once. */
+int px_create(const px_world_desc* desc, px_ctx** out);
+/* Frees every device buffer of the context. */
+int px_destroy(px_ctx* ctx);
+/* Message of the last failed call on ctx (or of the last failed px_create when
+ * ctx is NULL).  Never NULL. */
+const char* px_last_error(const px_ctx* ctx);
+
+/* Replaces the per-frame `writeBuffer(uniformBuf, 208 B)` (main.ts:218-237);
+ * same field order as Uniforms (shared.wgsl:5-19). */
+int px_set_uniforms(px_ctx* ctx, const float u[PX_UNIFORM_FLOATS]);
+/* Replaces the renderer's `lights` buffer (render_common.wgsl:8-9,16-18):
+ * lights[0].x = count; light i at [1+2i] = (pos m, radius m), [2+2i] =
+ * (colour, shadow flag).  n_vec4 = 1 + 2*count. */
+int px_set_lights(px_ctx* ctx, const float* lights_vec4, uint32_t n_vec4);
+
+/* Blocks until all work queued on the context's stream has finished. */
+int px_sync(px_ctx* ctx);
+/* The context's hipStream_t, for callers that order their own work with it. */
+void* px_stream(px_ctx* ctx);
+
+/* Ray queries over host buffers (blocking).  kind = PX_RAY_*.  New entry the
+ * hot path needs: the reference has rays only inside fsMain/shoot/particleSim
+ * (raymarch.wgsl:388-398, sim.wgsl:380, sim.wgsl:1119-1145). */
+int px_trace(px_ctx* ctx, uint32_t kind, const px_ray* rays, uint32_t n, px_hit* out);
+/* Same over device buffers, asynchronous on the context's stream. */
+int px_trace_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, uint32_t n, px_hit* d_out);
+
+/* One frame of the fragment shader's ray work (fsMain raymarch.wgsl:398 and
+ * the shadow rays of shade() :323-356, quality 1): the primary march of every
+ * ray, then for every primary hit on a non-emissive voxel the sun shadow ray
+ * (if dot(N, sunDir) > 0 and sun intensity > 0) and one shadow ray per light
+ * (if within its radius, dot(N, L) > 0, the light casts shadows and
+ * U.flags.y > 0.5).  Device buffers; d_shadow_out holds (1 + n_lights) slots
+ * per primary ray, slot 0 = sun, slot 1+i = light i; slots not cast read
+ * key PX_NONE, face PX_FACE_NOT_CAST.  *n_shadow_cast (host pointer, may be
+ * NULL) receives the number of shadow rays actually traced (blocking then). */
+int px_trace_frame_device(px_ctx* ctx, const px_ray* d_primary, uint32_t n, px_hit* d_primary_out,
+                          px_hit* d_shadow_out, uint32_t* n_shadow_cast);
+/* Slots per primary ray that px_trace_frame_device writes (1 + light count). */
+uint32_t px_frame_slots(const px_ctx* ctx);
+
+/* Replaces Simulation.probe (Simulation.ts:161-196, probe.wgsl:25-32):
+ * decoded voxel values of the box [o, o+s), x fastest.  Host buffer. */
+int px_probe(px_ctx* ctx, const int32_t o[3], const int32_t s[3], uint32_t* out);
+/* Same into a device buffer, asynchronous. */
+int px_probe_device(px_ctx* ctx, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
+
+/* Replaces afterSubmit's readback (Simulation.ts:275-322): ctrl counters in
+ * CTRL_NAMES order, indirect args in ARGS_NAMES order and the 64-float camera
+ * block.  Any pointer may be NULL.  Blocking. */
+int px_read_ctrl(px_ctx* ctx, uint32_t* ctrl, uint32_t* args, float* cam);
+/* Copies the brick grid (cells) and the super-grid counts to host buffers. */
+int px_read_grid(px_ctx* ctx, uint32_t* grid, uint32_t* super_grid);
+/* Copies pool bricks [first, first+count) as 256 words each. */
+int px_read_pool(px_ctx* ctx, uint32_t first, uint32_t count, uint32_t* words);
+/* World geometry: dims[3], bricks[3], supers[3], pool capacity. */
+int px_world_info(const px_ctx* ctx, uint32_t dims[3], uint32_t bricks[3], uint32_t supers[3],
+                  uint32_t* pool_capacity);
+
+/* Queue impacts for the next edit pass: n records of 16 floats in the heap
+ * layout of pushImpact (sim.wgsl:203-211): c.xyz, r, fr, impulse, n.xyz,
+ * dir.xyz, 4 unused.  Fails with PX_ERR_CAPACITY beyond max_impacts. */
+int px_push_impacts(px_ctx* ctx, const float* impacts16, uint32_t n);
+
+/* The island pass of the edit chain, run to quiescence on the context's
+ * stream: impactJobs -> allocBricks -> carve (sim.wgsl:408-584), the glass
+ * pane flood to completion (:586-618), maint (:1205-1274), then a 6-connected
+ * labelling of every non-air, non-rubble, non-structural voxel; a component
+ * is supported if it holds a voxel with y == 0 or touches a structural voxel
+ * (the anchor rules of visit/searchStep, :767-827).  Asynchronous; labels are
+ * read with px_island_labels. */
+int px_island_pass(px_ctx* ctx);
+/* Unsupported components of the last island pass, sorted by label: the
+ * canonical label (minimum voxel key of the component) and its voxel count.
+ * *n receives the total count even when it exceeds cap.  Blocking. */
+int px_island_labels(px_ctx* ctx, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+
+/* ---- profiling (no counterpart in the reference, which has no GPU timers;
+ * for callers that report per-kernel time and the roofline) -------------
+ * px_profile_begin starts recording HIP events around every kernel the
+ * library launches on its stream; px_profile_end waits for them and writes,
+ * per kernel slot (PX_PROF_*), out[2*k] = summed milliseconds and
+ * out[2*k+1] = number of launches, then stops recording. */
+#define PX_PROF_PRIMARY 0  /* frame: primary march + shadow-ray generation */
+#define PX_PROF_SHADOW 1   /* frame: shadow-ray march */
+#define PX_PROF_TRACE 2    /* px_trace / px_trace_device */
+#define PX_PROF_EDIT 3     /* island pass, all kernels */
+#define PX_PROF_SLOTS 8
+int px_profile_begin(px_ctx* ctx);
+int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIXELS_H */

@@ -1,0 +1,161 @@
+"""CPU oracle of the brick-map hot path - TEST INFRASTRUCTURE ONLY.
+
+A plain-C restatement (px_oracle.c) of the reference's WGSL: march(),
+traceSolid(), the particle segment DDA, the shade() shadow-ray setup and the
+carve / pane-flood / maint chain, plus a textbook breadth-first labeller for
+the islands.  Only the tests, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py use it, always as the checker of the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+HIT_DTYPE = np.dtype([("key", "<u4"), ("t", "<f4"), ("steps", "<u4"), ("face", "u1"), ("mat", "u1"),
+                      ("value_hi", "<u2")])
+RAY_DTYPE = np.dtype([("ro", "<f4", 3), ("rd", "<f4", 3), ("max_t", "<f4"), ("flags", "<u4")])
+
+_lib = None
+
+
+class OWorld(C.Structure):
+    _fields_ = [("dx", C.c_int32), ("dy", C.c_int32), ("dz", C.c_int32), ("grid", C.c_void_p),
+                ("pool", C.c_void_p), ("super_grid", C.c_void_p), ("mats", C.c_void_p)]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        from .build_oracle import build
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    W = C.POINTER(OWorld)
+    lib.o_march.argtypes = [W, P, P, C.c_float, C.c_int, P, P]
+    lib.o_trace_solid.argtypes = [W, P, P, C.c_float, P]
+    lib.o_segment.argtypes = [W, P, P, P]
+    lib.o_trace_batch.argtypes = [W, C.c_int, P, C.c_uint32, P, C.c_int]
+    lib.o_frame.argtypes = [W, P, P, C.c_uint32, P, C.c_uint32, P, P, P, C.c_int]
+    lib.o_island_pass.restype = C.c_int
+    lib.o_island_pass.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, P, C.c_uint32, P, P, C.c_uint32, P, P]
+    lib.o_islands.restype = C.c_int
+    lib.o_islands.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, P, C.c_uint32, P]
+    lib.o_hashu.restype = C.c_uint32
+    lib.o_hashu.argtypes = [C.c_uint32]
+    lib.o_hash3i.restype = C.c_uint32
+    lib.o_hash3i.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+    lib.o_rnd.restype = C.c_float
+    lib.o_rnd.argtypes = [C.c_uint32]
+    lib.o_removes.restype = C.c_int
+    lib.o_removes.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+    _lib = lib
+    return lib
+
+
+class World:
+    """Oracle view of a world given as grid / pool / super grid / mats arrays."""
+
+    def __init__(self, dims, grid, pool, super_grid, mats):
+        self.grid = np.ascontiguousarray(grid, np.uint32)
+        pool = np.ascontiguousarray(pool, np.uint16)
+        if pool.size == 0:
+            pool = np.zeros(512, np.uint16)
+        self.pool = pool
+        self.super_grid = np.ascontiguousarray(super_grid, np.uint32)
+        self.mats = np.ascontiguousarray(mats, np.float32)
+        self.dims = tuple(int(d) for d in dims)
+        self.w = OWorld(self.dims[0], self.dims[1], self.dims[2], self.grid.ctypes.data, self.pool.ctypes.data,
+                        self.super_grid.ctypes.data, self.mats.ctypes.data)
+
+    @classmethod
+    def from_scene(cls, s):
+        return cls(s.dims, s.grid, s.pool, s.super_grid, s.mats)
+
+    def march(self, ro, rd, max_t=1e9, shadow=False):
+        ro = np.asarray(ro, np.float32)
+        rd = np.asarray(rd, np.float32)
+        out = np.zeros(1, HIT_DTYPE)
+        trans = np.zeros(3, np.float32)
+        load().o_march(C.byref(self.w), ro.ctypes.data, rd.ctypes.data, float(np.float32(max_t)), int(shadow),
+                       out.ctypes.data, trans.ctypes.data)
+        return out[0], trans
+
+    def trace(self, kind: int, rays: np.ndarray, threads: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        out = np.zeros(rays.shape[0], HIT_DTYPE)
+        load().o_trace_batch(C.byref(self.w), kind, rays.ctypes.data, rays.shape[0], out.ctypes.data,
+                             threads or os.cpu_count() or 1)
+        return out
+
+    def frame(self, u52, lights, rays, threads: int = 0):
+        u52 = np.ascontiguousarray(u52, np.float32)
+        lights = np.ascontiguousarray(lights, np.float32)
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        n = rays.shape[0]
+        count = int(lights[0]) if lights.size else 0
+        count = max(0, min(count, (lights.size // 4 - 1) // 2 if lights.size >= 4 else 0))
+        slots = 1 + count
+        prim = np.zeros(n, HIT_DTYPE)
+        sh = np.zeros(n * slots, HIT_DTYPE)
+        nc = C.c_uint32(0)
+        load().o_frame(C.byref(self.w), u52.ctypes.data, lights.ctypes.data, lights.size // 4, rays.ctypes.data, n,
+                       prim.ctypes.data, sh.ctypes.data, C.byref(nc), threads or os.cpu_count() or 1)
+        return prim, sh, int(nc.value)
+
+
+def island_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, cap: int = 1 << 20):
+    """Runs the edit chain on copies; returns (vox_after, kind_after, keys, counts, n, stats)."""
+    vox = np.array(vox, np.uint16, copy=True, order="C")
+    kind = np.array(kind, np.uint8, copy=True, order="C")
+    mats = np.ascontiguousarray(mats, np.float32)
+    imp = np.ascontiguousarray(impacts, np.float32).reshape(-1, 16)
+    keys = np.zeros(cap, np.uint32)
+    cnt = np.zeros(cap, np.uint32)
+    n = C.c_uint32(0)
+    stats = np.zeros(5, np.uint64)
+    rc = load().o_island_pass(dims[0], dims[1], dims[2], vox.ctypes.data, kind.ctypes.data, mats.ctypes.data,
+                              imp.ctypes.data, imp.shape[0], keys.ctypes.data, cnt.ctypes.data, cap, C.byref(n),
+                              stats.ctypes.data)
+    if rc != 0:
+        raise MemoryError("oracle island pass failed")
+    m = min(cap, n.value)
+    return vox, kind, keys[:m].copy(), cnt[:m].copy(), int(n.value), stats
+
+
+def islands(dims, vox: np.ndarray, mats, cap: int = 1 << 20):
+    vox = np.ascontiguousarray(vox, np.uint16)
+    mats = np.ascontiguousarray(mats, np.float32)
+    keys = np.zeros(cap, np.uint32)
+    cnt = np.zeros(cap, np.uint32)
+    n = C.c_uint32(0)
+    rc = load().o_islands(dims[0], dims[1], dims[2], vox.ctypes.data, mats.ctypes.data, keys.ctypes.data,
+                          cnt.ctypes.data, cap, C.byref(n))
+    if rc != 0:
+        raise MemoryError("oracle islands failed")
+    m = min(cap, n.value)
+    return keys[:m].copy(), cnt[:m].copy(), int(n.value)
+
+
+def hashu(x: int) -> int:
+    return int(load().o_hashu(x & 0xFFFFFFFF))
+
+
+def hash3i(x: int, y: int, z: int) -> int:
+    return int(load().o_hash3i(x, y, z))
+
+
+def rnd(seed: int) -> float:
+    return float(load().o_rnd(seed & 0xFFFFFFFF))
+
+
+def removes(impact16, mats, x, y, z, v) -> bool:
+    imp = np.ascontiguousarray(impact16, np.float32)
+    mats = np.ascontiguousarray(mats, np.float32)
+    return bool(load().o_removes(imp.ctypes.data, mats.ctypes.data, x, y, z, v))

@@ -1,0 +1,879 @@
+/*
+ * px_oracle.c - CPU oracle (TEST INFRASTRUCTURE ONLY, see px_oracle.h).
+ *
+ * A literal restatement, in plain C with FP32 arithmetic rounded once per
+ * operation in source order (-ffp-contract=off), of:
+ *   march / safeDir / glassSurface      src/render/shaders/raymarch.wgsl:22-162
+ *   brickVoxel                          src/render/shaders/render_common.wgsl:25-31
+ *   shade() shadow-ray setup            src/render/shaders/raymarch.wgsl:304-356
+ *   hashu / hash3i / rnd                src/render/shaders/shared.wgsl:65-81
+ *   readVoxel, traceSolid               src/render/shaders/sim.wgsl:74-81, 296-357
+ *   segment DDA of particleSim          src/render/shaders/sim.wgsl:1110-1145
+ *   removes / reach / impactJobs        src/render/shaders/sim.wgsl:229-248, 414-443
+ *   allocBricks, carve, carveEffects    src/render/shaders/sim.wgsl:459-579
+ *   pane flood (floodStep)              src/render/shaders/sim.wgsl:599-618
+ *   maint compaction                    src/render/shaders/sim.wgsl:1219-1274
+ * and a textbook breadth-first 6-connected labelling for the islands.
+ *
+ * Choices the WGSL leaves to the implementation, fixed here (the HIP library
+ * makes the same choices):
+ *   pow(x, 5.0)  = ((x*x)*(x*x))*x
+ *   mix(a, b, t) = a*(1-t) + b*t        (WGSL spec formula)
+ *   dot(a, b)    = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   length(a)    = sqrt(dot(a, a))
+ *   f32 -> i32   = truncation toward zero, saturating
+ *   a zero t is reported as +0.
+ */
+#include "px_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NONE 0xFFFFFFFFu
+#define UNIFORM_BIT 0x80000000u
+
+/* ---------------------------------------------------------------------- */
+/* scalar helpers */
+
+static inline float fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+/* WGSL step(edge, x) = 1 if x >= edge else 0 */
+static inline float fstep(float edge, float x) { return x >= edge ? 1.0f : 0.0f; }
+static inline int32_t f2i(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483520.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)x;
+}
+static inline int32_t iclamp(int32_t v, int32_t lo, int32_t hi) {
+  int32_t m = v > lo ? v : lo;
+  return m < hi ? m : hi;
+}
+static inline int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+static inline float pow5(float x) {
+  float x2 = x * x;
+  float x4 = x2 * x2;
+  return x4 * x;
+}
+static inline float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline float canon0(float t) { return t == 0.0f ? 0.0f : t; }
+
+static inline const float* matA(const float* m, uint32_t id) { return m + id * 20u; }
+static inline const float* matC(const float* m, uint32_t id) { return m + id * 20u + 8u; }
+static inline const float* matD(const float* m, uint32_t id) { return m + id * 20u + 12u; }
+static inline const float* matE(const float* m, uint32_t id) { return m + id * 20u + 16u; }
+static inline int isGlass(const float* m, uint32_t id) { return matD(m, id)[1] > 0.0f; }
+
+uint32_t o_hashu(uint32_t x) {
+  uint32_t h = x;
+  h ^= h >> 16u;
+  h *= 0x7feb352du;
+  h ^= h >> 15u;
+  h *= 0x846ca68bu;
+  h ^= h >> 16u;
+  return h;
+}
+
+uint32_t o_hash3i(int32_t x, int32_t y, int32_t z) {
+  return o_hashu(((uint32_t)x * 0x9E3779B1u) ^
+                 o_hashu(((uint32_t)y * 0x85EBCA77u) ^ o_hashu((uint32_t)z * 0xC2B2AE3Du)));
+}
+
+float o_rnd(uint32_t seed) { return (float)(o_hashu(seed) & 0xffffffu) / 16777216.0f; }
+
+/* face id of an axis-aligned normal vector (dirOf order), 6 for zero */
+static uint8_t face_of(const float* n) {
+  if (n[0] > 0.0f) return 0;
+  if (n[0] < 0.0f) return 1;
+  if (n[1] > 0.0f) return 2;
+  if (n[1] < 0.0f) return 3;
+  if (n[2] > 0.0f) return 4;
+  if (n[2] < 0.0f) return 5;
+  return 6;
+}
+
+static inline uint32_t key_of(const o_world* w, const int32_t* p) {
+  return (uint32_t)p[0] + (uint32_t)w->dx * ((uint32_t)p[1] + (uint32_t)w->dy * (uint32_t)p[2]);
+}
+
+/* render_common.wgsl:25-31 */
+static inline uint32_t brickVoxel(const o_world* w, uint32_t g, int32_t lx, int32_t ly, int32_t lz) {
+  if ((g & UNIFORM_BIT) != 0u) return g & 0xffffu;
+  uint32_t li = (uint32_t)(lx | (ly << 3) | (lz << 6));
+  uint32_t idx = (g - 1u) * 512u + li;
+  uint32_t word = w->pool[idx >> 1u];
+  return (word >> ((idx & 1u) * 16u)) & 0xffffu;
+}
+
+/* sim.wgsl:74-81 */
+static inline uint32_t readVoxel(const o_world* w, const int32_t* p) {
+  if (p[0] < 0 || p[1] < 0 || p[2] < 0 || p[0] >= w->dx || p[1] >= w->dy || p[2] >= w->dz) return 0u;
+  int32_t bx = w->dx >> 3, by = w->dy >> 3;
+  uint32_t g = w->grid[(uint32_t)(p[0] >> 3) + (uint32_t)bx * ((uint32_t)(p[1] >> 3) + (uint32_t)by * (uint32_t)(p[2] >> 3))];
+  if (g == 0u) return 0u;
+  if ((g & UNIFORM_BIT) != 0u) return g & 0xffffu;
+  uint32_t li = (uint32_t)((p[0] & 7) | ((p[1] & 7) << 3) | ((p[2] & 7) << 6));
+  uint32_t idx = (g - 1u) * 512u + li;
+  return (w->pool[idx >> 1u] >> ((idx & 1u) * 16u)) & 0xffffu;
+}
+
+static void set_hit(const o_world* w, o_hit* out, float t, const int32_t* vc, const float* n, uint32_t v,
+                    uint32_t steps) {
+  out->key = key_of(w, vc);
+  out->t = canon0(t);
+  out->steps = steps;
+  out->face = face_of(n);
+  out->mat = (uint8_t)(v & 0xffu);
+  out->value_hi = (uint16_t)((v >> 8) & 0xffu);
+}
+
+static void set_miss(o_hit* out, uint32_t steps) {
+  out->key = NONE;
+  out->t = 0.0f;
+  out->steps = steps;
+  out->face = 0xFF;
+  out->mat = 0;
+  out->value_hi = 0;
+}
+
+/* raymarch.wgsl:26-38 (glow is shading output and is not kept) */
+static void glassSurface(const float* mats, float* trans, uint32_t id, const float* n, const float* rd, int shadow) {
+  const float* tint = matA(mats, id);
+  float g = matD(mats, id)[1];
+  if (!shadow) {
+    float cosT = fabsf(dot3(n, rd));
+    float F = 0.04f + 0.96f * pow5(1.0f - cosT);
+    for (int a = 0; a < 3; a++) trans[a] = trans[a] * (1.0f - F);
+  }
+  for (int a = 0; a < 3; a++) {
+    float m = 1.0f * (1.0f - 0.6f) + tint[a] * 0.6f;
+    trans[a] = trans[a] * (m * g);
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* march: raymarch.wgsl:44-162 */
+
+void o_march(const o_world* w, const float ro[3], const float rdIn[3], float maxT, int shadow, o_hit* out,
+             float* trans_out) {
+  const float* mats = w->mats;
+  float trans[3] = {1.0f, 1.0f, 1.0f};
+  uint32_t outer = 0, inner = 0;
+  float rd[3], inv[3], t0[3], t1[3], tmn[3], tmx[3];
+  float dims[3] = {(float)w->dx, (float)w->dy, (float)w->dz};
+  for (int a = 0; a < 3; a++) {
+    rd[a] = fabsf(rdIn[a]) < 1e-6f ? fsign(rdIn[a] + 1e-30f) * 1e-6f : rdIn[a];
+    inv[a] = 1.0f / rd[a];
+    t0[a] = (0.0f - ro[a]) * inv[a];
+    t1[a] = (dims[a] - ro[a]) * inv[a];
+    tmn[a] = fminf(t0[a], t1[a]);
+    tmx[a] = fmaxf(t0[a], t1[a]);
+  }
+  float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
+  float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
+#define RETURN_MISS()                                     \
+  do {                                                    \
+    set_miss(out, outer + inner);                         \
+    if (trans_out) memcpy(trans_out, trans, sizeof trans); \
+    return;                                               \
+  } while (0)
+#define RETURN_HIT(T, VC, N, V)                           \
+  do {                                                    \
+    set_hit(w, out, (T), (VC), (N), (V), outer + inner);  \
+    if (trans_out) memcpy(trans_out, trans, sizeof trans); \
+    return;                                               \
+  } while (0)
+  if (t >= tExit) RETURN_MISS();
+
+  int32_t stp[3];
+  float stpF[3], pos[3];
+  for (int a = 0; a < 3; a++) {
+    stp[a] = f2i(fsign(rd[a]));
+    stpF[a] = (float)stp[a];
+    pos[a] = fstep(0.0f, rd[a]);
+  }
+  float normal[3] = {0.0f, 0.0f, 0.0f};
+  if (t > 0.0f) {
+    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) normal[0] = -stpF[0];
+    else if (tmn[1] >= tmn[2]) normal[1] = -stpF[1];
+    else normal[2] = -stpF[2];
+  }
+
+  int32_t bd[3] = {w->dx >> 3, w->dy >> 3, w->dz >> 3};
+  int32_t sd[3] = {(bd[0] + 3) >> 2, (bd[1] + 3) >> 2, (bd[2] + 3) >> 2};
+  int32_t sc[3];
+  float tDeltaS[3], tMaxS[3], tDeltaB[3], tDeltaV[3];
+  for (int a = 0; a < 3; a++) {
+    float p0 = ro[a] + rd[a] * (t + 1e-3f);
+    sc[a] = iclamp(f2i(floorf(p0 / 32.0f)), 0, sd[a] - 1);
+    tDeltaS[a] = fabsf(inv[a]) * 32.0f;
+    tMaxS[a] = (((float)sc[a] + pos[a]) * 32.0f - ro[a]) * inv[a];
+    tDeltaB[a] = fabsf(inv[a]) * 8.0f;
+    tDeltaV[a] = fabsf(inv[a]);
+  }
+  int inGlass = 0;
+  uint32_t glassId = 0u;
+
+  for (int i = 0; i < 256; i++) {
+    outer += 1u;
+    if (w->super_grid[(uint32_t)(sc[0] + sd[0] * (sc[1] + sd[1] * sc[2]))] != 0u) {
+      int32_t sLo[3], sHi[3], bc[3];
+      float tMaxB[3];
+      for (int a = 0; a < 3; a++) {
+        sLo[a] = sc[a] * 4;
+        sHi[a] = imin(sLo[a] + 3, bd[a] - 1);
+        bc[a] = iclamp(f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo[a], sHi[a]);
+        tMaxB[a] = (((float)bc[a] + pos[a]) * 8.0f - ro[a]) * inv[a];
+      }
+      float tb = t;
+      float nb[3] = {normal[0], normal[1], normal[2]};
+      for (int j = 0; j < 14; j++) {
+        outer += 1u;
+        uint32_t g = w->grid[(uint32_t)(bc[0] + bd[0] * (bc[1] + bd[1] * bc[2]))];
+        if (g != 0u) {
+          int32_t lo[3], vc[3];
+          for (int a = 0; a < 3; a++) {
+            lo[a] = bc[a] * 8;
+            vc[a] = iclamp(f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo[a], lo[a] + 7);
+          }
+          if ((g & UNIFORM_BIT) != 0u && !isGlass(mats, g & 0xffu)) RETURN_HIT(tb, vc, nb, g & 0xffffu);
+          float tMaxV[3];
+          for (int a = 0; a < 3; a++) tMaxV[a] = ((float)vc[a] + pos[a] - ro[a]) * inv[a];
+          float tv = tb;
+          float n[3] = {nb[0], nb[1], nb[2]};
+          for (int k = 0; k < 25; k++) {
+            inner += 1u;
+            uint32_t v = brickVoxel(w, g, vc[0] - lo[0], vc[1] - lo[1], vc[2] - lo[2]);
+            if (v != 0u) {
+              uint32_t id = v & 0xffu;
+              if (isGlass(mats, id)) {
+                if (!inGlass || id != glassId) glassSurface(mats, trans, id, n, rd, shadow);
+                inGlass = 1;
+                glassId = id;
+                if (fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f) RETURN_HIT(tv, vc, n, v);
+              } else {
+                RETURN_HIT(tv, vc, n, v);
+              }
+            } else {
+              inGlass = 0;
+            }
+            if (tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2]) {
+              tv = tMaxV[0]; vc[0] += stp[0]; tMaxV[0] += tDeltaV[0];
+              n[0] = -stpF[0]; n[1] = 0.0f; n[2] = 0.0f;
+            } else if (tMaxV[1] < tMaxV[2]) {
+              tv = tMaxV[1]; vc[1] += stp[1]; tMaxV[1] += tDeltaV[1];
+              n[0] = 0.0f; n[1] = -stpF[1]; n[2] = 0.0f;
+            } else {
+              tv = tMaxV[2]; vc[2] += stp[2]; tMaxV[2] += tDeltaV[2];
+              n[0] = 0.0f; n[1] = 0.0f; n[2] = -stpF[2];
+            }
+            if (tv > tExit) RETURN_MISS();
+            if (vc[0] < lo[0] || vc[1] < lo[1] || vc[2] < lo[2] || vc[0] > lo[0] + 7 || vc[1] > lo[1] + 7 ||
+                vc[2] > lo[2] + 7)
+              break;
+          }
+        } else {
+          inGlass = 0;
+        }
+        if (tMaxB[0] < tMaxB[1] && tMaxB[0] < tMaxB[2]) {
+          tb = tMaxB[0]; bc[0] += stp[0]; tMaxB[0] += tDeltaB[0];
+          nb[0] = -stpF[0]; nb[1] = 0.0f; nb[2] = 0.0f;
+        } else if (tMaxB[1] < tMaxB[2]) {
+          tb = tMaxB[1]; bc[1] += stp[1]; tMaxB[1] += tDeltaB[1];
+          nb[0] = 0.0f; nb[1] = -stpF[1]; nb[2] = 0.0f;
+        } else {
+          tb = tMaxB[2]; bc[2] += stp[2]; tMaxB[2] += tDeltaB[2];
+          nb[0] = 0.0f; nb[1] = 0.0f; nb[2] = -stpF[2];
+        }
+        if (tb > tExit) RETURN_MISS();
+        if (bc[0] < sLo[0] || bc[1] < sLo[1] || bc[2] < sLo[2] || bc[0] > sHi[0] || bc[1] > sHi[1] || bc[2] > sHi[2])
+          break;
+      }
+    } else {
+      inGlass = 0;
+    }
+    if (tMaxS[0] < tMaxS[1] && tMaxS[0] < tMaxS[2]) {
+      t = tMaxS[0]; sc[0] += stp[0]; tMaxS[0] += tDeltaS[0];
+      normal[0] = -stpF[0]; normal[1] = 0.0f; normal[2] = 0.0f;
+    } else if (tMaxS[1] < tMaxS[2]) {
+      t = tMaxS[1]; sc[1] += stp[1]; tMaxS[1] += tDeltaS[1];
+      normal[0] = 0.0f; normal[1] = -stpF[1]; normal[2] = 0.0f;
+    } else {
+      t = tMaxS[2]; sc[2] += stp[2]; tMaxS[2] += tDeltaS[2];
+      normal[0] = 0.0f; normal[1] = 0.0f; normal[2] = -stpF[2];
+    }
+    if (t > tExit || sc[0] < 0 || sc[1] < 0 || sc[2] < 0 || sc[0] >= sd[0] || sc[1] >= sd[1] || sc[2] >= sd[2]) break;
+  }
+  RETURN_MISS();
+#undef RETURN_MISS
+#undef RETURN_HIT
+}
+
+/* march() that also returns the hit normal as a vector (for the frame). */
+static void march_n(const o_world* w, const float ro[3], const float rd[3], float maxT, int shadow, o_hit* h,
+                    float* n_out) {
+  o_march(w, ro, rd, maxT, shadow, h, NULL);
+  n_out[0] = n_out[1] = n_out[2] = 0.0f;
+  if (h->face < 6) n_out[h->face >> 1] = (h->face & 1) ? -1.0f : 1.0f;
+}
+
+/* ---------------------------------------------------------------------- */
+/* traceSolid: sim.wgsl:296-357.  steps = brick iterations + voxel
+ * iterations; t = the voxel-level t of the hit (traceSolid itself returns no t). */
+
+void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], float maxT, o_hit* out) {
+  uint32_t steps = 0;
+  float rd[3], inv[3], t0[3], t1[3], tmn[3], tmx[3];
+  float dims[3] = {(float)w->dx, (float)w->dy, (float)w->dz};
+  for (int a = 0; a < 3; a++) {
+    rd[a] = fabsf(rdIn[a]) < 1e-6f ? fsign(rdIn[a] + 1e-30f) * 1e-6f : rdIn[a];
+    inv[a] = 1.0f / rd[a];
+    t0[a] = (0.0f - ro[a]) * inv[a];
+    t1[a] = (dims[a] - ro[a]) * inv[a];
+    tmn[a] = fminf(t0[a], t1[a]);
+    tmx[a] = fmaxf(t0[a], t1[a]);
+  }
+  float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
+  float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
+  if (t >= tExit) { set_miss(out, 0); return; }
+  int32_t stp[3];
+  float stpF[3], pos[3];
+  for (int a = 0; a < 3; a++) {
+    stp[a] = f2i(fsign(rd[a]));
+    stpF[a] = (float)stp[a];
+    pos[a] = fstep(0.0f, rd[a]);
+  }
+  float normal[3] = {0.0f, 0.0f, 0.0f};
+  if (t > 0.0f) {
+    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) normal[0] = -stpF[0];
+    else if (tmn[1] >= tmn[2]) normal[1] = -stpF[1];
+    else normal[2] = -stpF[2];
+  }
+  int32_t bd[3] = {w->dx >> 3, w->dy >> 3, w->dz >> 3};
+  int32_t bc[3];
+  float tDeltaB[3], tMaxB[3], tDeltaV[3];
+  for (int a = 0; a < 3; a++) {
+    bc[a] = iclamp(f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), 0, bd[a] - 1);
+    tDeltaB[a] = fabsf(inv[a]) * 8.0f;
+    tMaxB[a] = (((float)bc[a] + pos[a]) * 8.0f - ro[a]) * inv[a];
+    tDeltaV[a] = fabsf(inv[a]);
+  }
+  for (int i = 0; i < 1024; i++) {
+    steps++;
+    uint32_t g = w->grid[(uint32_t)(bc[0] + bd[0] * (bc[1] + bd[1] * bc[2]))];
+    if (g != 0u) {
+      int32_t lo[3], vc[3];
+      float tMaxV[3];
+      for (int a = 0; a < 3; a++) {
+        lo[a] = bc[a] * 8;
+        vc[a] = iclamp(f2i(floorf(ro[a] + rd[a] * (t + 1e-3f))), lo[a], lo[a] + 7);
+        tMaxV[a] = ((float)vc[a] + pos[a] - ro[a]) * inv[a];
+      }
+      float tv = t;
+      float n[3] = {normal[0], normal[1], normal[2]};
+      for (int k = 0; k < 25; k++) {
+        steps++;
+        uint32_t v = readVoxel(w, vc);
+        if (v != 0u) { set_hit(w, out, tv, vc, n, v, steps); return; }
+        if (tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2]) {
+          tv = tMaxV[0]; vc[0] += stp[0]; tMaxV[0] += tDeltaV[0];
+          n[0] = -stpF[0]; n[1] = 0.0f; n[2] = 0.0f;
+        } else if (tMaxV[1] < tMaxV[2]) {
+          tv = tMaxV[1]; vc[1] += stp[1]; tMaxV[1] += tDeltaV[1];
+          n[0] = 0.0f; n[1] = -stpF[1]; n[2] = 0.0f;
+        } else {
+          tv = tMaxV[2]; vc[2] += stp[2]; tMaxV[2] += tDeltaV[2];
+          n[0] = 0.0f; n[1] = 0.0f; n[2] = -stpF[2];
+        }
+        if (tv > tExit) { set_miss(out, steps); return; }
+        if (vc[0] < lo[0] || vc[1] < lo[1] || vc[2] < lo[2] || vc[0] > lo[0] + 7 || vc[1] > lo[1] + 7 ||
+            vc[2] > lo[2] + 7)
+          break;
+      }
+    }
+    if (tMaxB[0] < tMaxB[1] && tMaxB[0] < tMaxB[2]) {
+      t = tMaxB[0]; bc[0] += stp[0]; tMaxB[0] += tDeltaB[0];
+      normal[0] = -stpF[0]; normal[1] = 0.0f; normal[2] = 0.0f;
+    } else if (tMaxB[1] < tMaxB[2]) {
+      t = tMaxB[1]; bc[1] += stp[1]; tMaxB[1] += tDeltaB[1];
+      normal[0] = 0.0f; normal[1] = -stpF[1]; normal[2] = 0.0f;
+    } else {
+      t = tMaxB[2]; bc[2] += stp[2]; tMaxB[2] += tDeltaB[2];
+      normal[0] = 0.0f; normal[1] = 0.0f; normal[2] = -stpF[2];
+    }
+    if (t > tExit || bc[0] < 0 || bc[1] < 0 || bc[2] < 0 || bc[0] >= bd[0] || bc[1] >= bd[1] || bc[2] >= bd[2]) break;
+  }
+  set_miss(out, steps);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Segment DDA: sim.wgsl:1110-1145.  Inside case -> face 6, t 0, steps 0.
+ * Hit -> key of the solid cell entered, face = -step along the hit axis,
+ * t = tn, steps = loop iterations begun. */
+
+void o_segment(const o_world* w, const float pos[3], const float mv[3], o_hit* out) {
+  int32_t cell[3];
+  for (int a = 0; a < 3; a++) cell[a] = f2i(floorf(pos[a]));
+  uint32_t v0 = readVoxel(w, cell);
+  if (v0 != 0u) {
+    float zero[3] = {0.0f, 0.0f, 0.0f};
+    set_hit(w, out, 0.0f, cell, zero, v0, 0);
+    return;
+  }
+  float dist = sqrtf(dot3(mv, mv));
+  if (!(dist > 1e-6f)) { set_miss(out, 0); return; }
+  float dir[3], inv[3], tMax[3], tDelta[3];
+  int32_t stp[3];
+  for (int a = 0; a < 3; a++) {
+    float d = mv[a] / dist;
+    dir[a] = fabsf(d) < 1e-6f ? 1e-6f : d;
+  }
+  for (int a = 0; a < 3; a++) {
+    inv[a] = 1.0f / dir[a];
+    stp[a] = f2i(fsign(dir[a]));
+    tMax[a] = ((float)cell[a] + fstep(0.0f, dir[a]) - pos[a]) * inv[a];
+    tDelta[a] = fabsf(inv[a]);
+  }
+  uint32_t steps = 0;
+  for (int k = 0; k < 48; k++) {
+    steps++;
+    int axis;
+    if (tMax[0] < tMax[1] && tMax[0] < tMax[2]) axis = 0;
+    else if (tMax[1] < tMax[2]) axis = 1;
+    else axis = 2;
+    float tn = tMax[axis];
+    if (tn > dist) break;
+    int32_t next[3] = {cell[0], cell[1], cell[2]};
+    next[axis] += stp[axis];
+    uint32_t v = readVoxel(w, next);
+    if (v != 0u) {
+      float n[3] = {0.0f, 0.0f, 0.0f};
+      n[axis] = -(float)stp[axis];
+      set_hit(w, out, tn, next, n, v, steps);
+      return;
+    }
+    cell[0] = next[0]; cell[1] = next[1]; cell[2] = next[2];
+    tMax[axis] += tDelta[axis];
+  }
+  set_miss(out, steps);
+}
+
+/* ---------------------------------------------------------------------- */
+/* threaded batches */
+
+typedef struct batch_job {
+  const o_world* w;
+  int kind;
+  const float* rays;
+  uint32_t n;
+  o_hit* out;
+  /* frame */
+  const float* u52;
+  const float* lights;
+  uint32_t n_vec4;
+  o_hit* shadow_out;
+  uint32_t slots;
+  volatile uint32_t next;
+  uint32_t cast;
+  pthread_mutex_t mu;
+  int frame;
+} batch_job;
+
+static void trace_one(const o_world* w, int kind, const float* r, o_hit* out) {
+  int k = kind >= 0 ? kind : (int)(((const uint32_t*)r)[7] & 3u);
+  if (k == 0 || k == 1) o_march(w, r, r + 3, r[6], k == 1, out, NULL);
+  else if (k == 2) o_trace_solid(w, r, r + 3, r[6], out);
+  else o_segment(w, r, r + 3, out);
+}
+
+static void not_cast(o_hit* h) {
+  h->key = NONE;
+  h->t = 0.0f;
+  h->steps = 0;
+  h->face = 0xFE;
+  h->mat = 0;
+  h->value_hi = 0;
+}
+
+/* fsMain primary march + shade() shadow rays (quality 1). Returns rays cast. */
+static uint32_t frame_one(const batch_job* J, uint32_t i) {
+  const o_world* w = J->w;
+  const float* u = J->u52;
+  const float* r = J->rays + (size_t)i * 8u;
+  o_hit* h = J->out + i;
+  o_hit* sh = J->shadow_out + (size_t)i * J->slots;
+  float N[3];
+  march_n(w, r, r + 3, r[6], 0, h, N);
+  for (uint32_t s = 0; s < J->slots; s++) not_cast(&sh[s]);
+  if (h->face == 0xFF) return 0;
+  uint32_t id = h->mat;
+  if (matA(w->mats, id)[3] > 0.0f) return 0; /* emissive: shade() returns early (:309-311) */
+  uint32_t cast = 0;
+  float P[3], O[3];
+  for (int a = 0; a < 3; a++) P[a] = r[a] + r[3 + a] * h->t;
+  for (int a = 0; a < 3; a++) O[a] = P[a] + N[a] * 0.5f;
+  const float* sunL = u + 4;
+  float ndl = dot3(N, sunL);
+  if (ndl > 0.0f && u[7] > 0.0f) {
+    o_march(w, O, sunL, 1e9f, 1, &sh[0], NULL);
+    cast++;
+  }
+  int count = f2i(J->lights[0]);
+  float vs = u[3];
+  for (int li = 0; li < count && (uint32_t)li + 1u < J->slots; li++) {
+    const float* lp = J->lights + (1 + 2 * li) * 4;
+    const float* lc = J->lights + (2 + 2 * li) * 4;
+    float Lp[3] = {lp[0] / vs, lp[1] / vs, lp[2] / vs};
+    float radius = lp[3] / vs;
+    float L[3] = {Lp[0] - P[0], Lp[1] - P[1], Lp[2] - P[2]};
+    float d = sqrtf(dot3(L, L));
+    if (d > radius) continue;
+    L[0] = L[0] / d; L[1] = L[1] / d; L[2] = L[2] / d;
+    float nl = dot3(N, L);
+    if (nl <= 0.0f) continue;
+    if (lc[3] > 0.5f && u[21] > 0.5f) {
+      o_march(w, O, L, d - 2.0f, 1, &sh[1 + li], NULL);
+      cast++;
+    }
+  }
+  return cast;
+}
+
+static void* batch_worker(void* arg) {
+  batch_job* J = (batch_job*)arg;
+  const uint32_t chunk = 256;
+  uint32_t cast = 0;
+  for (;;) {
+    uint32_t b = __atomic_fetch_add(&J->next, chunk, __ATOMIC_RELAXED);
+    if (b >= J->n) break;
+    uint32_t e = b + chunk < J->n ? b + chunk : J->n;
+    for (uint32_t i = b; i < e; i++) {
+      if (J->frame) cast += frame_one(J, i);
+      else trace_one(J->w, J->kind, J->rays + (size_t)i * 8u, J->out + i);
+    }
+  }
+  __atomic_fetch_add(&J->cast, cast, __ATOMIC_RELAXED);
+  return NULL;
+}
+
+static void run_batch(batch_job* J, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  J->next = 0;
+  J->cast = 0;
+  if (threads == 1) {
+    batch_worker(J);
+    return;
+  }
+  pthread_t th[256];
+  int started = 0;
+  for (int i = 0; i < threads; i++) {
+    if (pthread_create(&th[i], NULL, batch_worker, J) != 0) break;
+    started++;
+  }
+  if (started == 0) batch_worker(J);
+  for (int i = 0; i < started; i++) pthread_join(th[i], NULL);
+}
+
+void o_trace_batch(const o_world* w, int kind, const float* rays, uint32_t n, o_hit* out, int threads) {
+  batch_job J;
+  memset(&J, 0, sizeof J);
+  J.w = w;
+  J.kind = kind;
+  J.rays = rays;
+  J.n = n;
+  J.out = out;
+  run_batch(&J, threads);
+}
+
+void o_frame(const o_world* w, const float* u52, const float* lights_vec4, uint32_t n_vec4, const float* rays,
+             uint32_t n, o_hit* prim_out, o_hit* shadow_out, uint32_t* n_cast, int threads) {
+  batch_job J;
+  memset(&J, 0, sizeof J);
+  J.w = w;
+  J.rays = rays;
+  J.n = n;
+  J.out = prim_out;
+  J.u52 = u52;
+  J.lights = lights_vec4;
+  J.n_vec4 = n_vec4;
+  J.shadow_out = shadow_out;
+  int count = n_vec4 > 0 ? f2i(lights_vec4[0]) : 0;
+  if (count < 0) count = 0;
+  if ((uint32_t)(1 + 2 * count) > n_vec4) count = n_vec4 > 0 ? (int)((n_vec4 - 1) / 2) : 0;
+  J.slots = 1u + (uint32_t)count;
+  J.frame = 1;
+  run_batch(&J, threads);
+  if (n_cast) *n_cast = J.cast;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Edit chain */
+
+int o_removes(const float* im, const float* mats, int32_t x, int32_t y, int32_t z, uint32_t v) {
+  uint32_t id = v & 0xffu;
+  float d[3] = {((float)x + 0.5f) - im[0], ((float)y + 0.5f) - im[1], ((float)z + 0.5f) - im[2]};
+  float d2 = dot3(d, d);
+  float jit = 0.8f + 0.4f * o_rnd(o_hash3i(x, y, z));
+  const float* E = matE(mats, id);
+  if (E[1] > 0.5f && d2 < im[4] * im[4] * jit * jit) return 1;
+  if (E[0] <= 0.0f) return 0;
+  float rr = fmaxf(im[3] * fminf(1.4f, E[0]), fminf(im[3], 2.5f)) * jit;
+  return d2 < rr * rr;
+}
+
+/* sim.wgsl:242-248 */
+static float reach(const float* im, const float* mats, uint32_t id) {
+  const float* E = matE(mats, id);
+  float r = 0.0f;
+  if (E[1] > 0.5f) r = im[4] * 1.2f;
+  if (E[0] > 0.0f) r = fmaxf(r, fmaxf(im[3] * fminf(1.4f, E[0]), fminf(im[3], 2.5f)) * 1.2f);
+  return r;
+}
+
+static inline uint32_t price_dimes(const float* mats, uint32_t v) {
+  float p = matD(mats, v & 0xffu)[3];
+  return (uint32_t)f2i(p * 10.0f + 0.5f);
+}
+
+typedef struct ed_world {
+  int32_t dx, dy, dz, bx, by, bz;
+  uint16_t* vox;
+  uint8_t* kind;
+  uint8_t* dirty;
+  const float* mats;
+} ed_world;
+
+static inline size_t vkey(const ed_world* e, int32_t x, int32_t y, int32_t z) {
+  return (size_t)x + (size_t)e->dx * ((size_t)y + (size_t)e->dy * (size_t)z);
+}
+static inline int in_world(const ed_world* e, int32_t x, int32_t y, int32_t z) {
+  return x >= 0 && y >= 0 && z >= 0 && x < e->dx && y < e->dy && z < e->dz;
+}
+static inline size_t cell_of(const ed_world* e, int32_t x, int32_t y, int32_t z) {
+  return (size_t)(x >> 3) + (size_t)e->bx * ((size_t)(y >> 3) + (size_t)e->by * (size_t)(z >> 3));
+}
+
+static int structural(const float* mats, uint32_t v) { return matE(mats, v & 0xffu)[2] > 0.5f; }
+static int is_rubble(uint32_t v) { return (v & 0x80u) != 0u; }
+
+static const int32_t DIRS[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+                  const float* impacts16, uint32_t n_impacts, uint32_t* lab_key, uint32_t* lab_count,
+                  uint32_t cap, uint32_t* n_labels, uint64_t* stats) {
+  ed_world E;
+  E.dx = dx; E.dy = dy; E.dz = dz;
+  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
+  E.vox = vox;
+  E.kind = kind;
+  E.mats = mats;
+  size_t ncell = (size_t)E.bx * E.by * E.bz;
+  E.dirty = (uint8_t*)calloc(ncell, 1);
+  uint8_t* req = (uint8_t*)calloc(ncell, 1);
+  if (!E.dirty || !req) { free(E.dirty); free(req); return -1; }
+  uint64_t destroyed = 0, bill = 0, allocs = 0, njobs = 0, flooded = 0;
+
+  /* impactJobs for every impact on the grid as it is before any alloc
+   * (sim.wgsl:414-443); jobs are collected per impact. */
+  size_t jcap = 1024, jn = 0;
+  uint32_t* jobs = (uint32_t*)malloc(jcap * 2 * sizeof(uint32_t));
+  int32_t bd[3] = {E.bx, E.by, E.bz};
+  for (uint32_t k = 0; k < n_impacts; k++) {
+    const float* im = impacts16 + (size_t)k * 16u;
+    float R = ceilf(fmaxf(im[3], im[4]) * 1.2f) + 1.0f;
+    int32_t bmin[3], bmax[3];
+    for (int a = 0; a < 3; a++) {
+      bmin[a] = iclamp(f2i(floorf((im[a] - R) / 8.0f)), 0, bd[a] - 1);
+      bmax[a] = iclamp(f2i(floorf((im[a] + R) / 8.0f)), 0, bd[a] - 1);
+    }
+    for (int32_t bz = bmin[2]; bz <= bmax[2]; bz++)
+      for (int32_t by = bmin[1]; by <= bmax[1]; by++)
+        for (int32_t bx = bmin[0]; bx <= bmax[0]; bx++) {
+          size_t ci = (size_t)bx + (size_t)E.bx * ((size_t)by + (size_t)E.by * (size_t)bz);
+          if (kind[ci] == 0) continue;
+          float lo[3] = {(float)((uint32_t)bx * 8u), (float)((uint32_t)by * 8u), (float)((uint32_t)bz * 8u)};
+          float e[3];
+          for (int a = 0; a < 3; a++) e[a] = fmaxf(fmaxf(lo[a] - im[a], 0.0f), im[a] - (lo[a] + 8.0f));
+          float near2 = dot3(e, e);
+          if (near2 > R * R) continue;
+          if (kind[ci] == 1) {
+            uint32_t uval = vox[vkey(&E, bx * 8, by * 8, bz * 8)];
+            float rch = reach(im, mats, uval & 0xffu);
+            if (near2 > rch * rch) continue;
+            req[ci] = 1;
+          }
+          if (jn == jcap) {
+            jcap *= 2;
+            jobs = (uint32_t*)realloc(jobs, jcap * 2 * sizeof(uint32_t));
+          }
+          jobs[jn * 2] = k;
+          jobs[jn * 2 + 1] = (uint32_t)ci;
+          jn++;
+        }
+  }
+  njobs = jn;
+  /* allocBricks (sim.wgsl:459-483): uniform cells become pool bricks */
+  for (size_t ci = 0; ci < ncell; ci++)
+    if (req[ci] && kind[ci] == 1) {
+      kind[ci] = 2;
+      E.dirty[ci] = 1;
+      allocs++;
+    }
+  /* carve (sim.wgsl:529-579): pool bricks only; removed pane voxels are kept
+   * as flood triggers (carveEffects :511-523). */
+  size_t tcap = 1024, tn = 0;
+  uint32_t* trig = (uint32_t*)malloc(tcap * 4 * sizeof(uint32_t));
+  for (size_t j = 0; j < jn; j++) {
+    const float* im = impacts16 + (size_t)jobs[j * 2] * 16u;
+    size_t ci = jobs[j * 2 + 1];
+    if (kind[ci] != 2) continue;
+    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8;
+    int32_t oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
+    int any = 0;
+    for (uint32_t l = 0; l < 512u; l++) {
+      int32_t x = ox + (int32_t)(l & 7u), y = oy + (int32_t)((l >> 3) & 7u), z = oz + (int32_t)(l >> 6);
+      size_t kk = vkey(&E, x, y, z);
+      uint32_t v = vox[kk];
+      if (v == 0u) continue;
+      if (!o_removes(im, mats, x, y, z, v)) continue;
+      vox[kk] = 0;
+      any = 1;
+      destroyed++;
+      bill += price_dimes(mats, v);
+      if (matE(mats, v & 0xffu)[1] > 1.5f) {
+        if (tn == tcap) {
+          tcap *= 2;
+          trig = (uint32_t*)realloc(trig, tcap * 4 * sizeof(uint32_t));
+        }
+        trig[tn * 4] = (uint32_t)x;
+        trig[tn * 4 + 1] = (uint32_t)y;
+        trig[tn * 4 + 2] = (uint32_t)z;
+        trig[tn * 4 + 3] = v & 0xffu;
+        tn++;
+      }
+    }
+    if (any) E.dirty[ci] = 1;
+  }
+  free(jobs);
+  /* Pane flood to completion (carveEffects :511-523 + floodStep :599-618):
+   * breadth-first over same-id voxels that sit in pool bricks (takeVoxel,
+   * :105-124, only removes voxels of pool bricks). */
+  size_t qcap = 1024, qh = 0, qt = 0;
+  uint32_t* q = (uint32_t*)malloc(qcap * 4 * sizeof(uint32_t));
+#define TAKE_PUSH(X, Y, Z, M)                                                        \
+  do {                                                                               \
+    int32_t X_ = (X), Y_ = (Y), Z_ = (Z);                                            \
+    if (in_world(&E, X_, Y_, Z_) && kind[cell_of(&E, X_, Y_, Z_)] == 2) {             \
+      size_t kk_ = vkey(&E, X_, Y_, Z_);                                             \
+      uint32_t v_ = vox[kk_];                                                        \
+      if (v_ != 0u && (v_ & 0xffu) == (M)) {                                         \
+        vox[kk_] = 0;                                                                \
+        E.dirty[cell_of(&E, X_, Y_, Z_)] = 1;                                        \
+        destroyed++;                                                                 \
+        flooded++;                                                                   \
+        bill += price_dimes(mats, v_);                                               \
+        if (qt == qcap) {                                                            \
+          qcap *= 2;                                                                 \
+          q = (uint32_t*)realloc(q, qcap * 4 * sizeof(uint32_t));                    \
+        }                                                                            \
+        q[qt * 4] = (uint32_t)X_;                                                    \
+        q[qt * 4 + 1] = (uint32_t)Y_;                                                \
+        q[qt * 4 + 2] = (uint32_t)Z_;                                                \
+        q[qt * 4 + 3] = (M);                                                         \
+        qt++;                                                                        \
+      }                                                                              \
+    }                                                                                \
+  } while (0)
+  for (size_t i = 0; i < tn; i++) {
+    int32_t x = (int32_t)trig[i * 4], y = (int32_t)trig[i * 4 + 1], z = (int32_t)trig[i * 4 + 2];
+    uint32_t m = trig[i * 4 + 3];
+    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
+  }
+  while (qh < qt) {
+    int32_t x = (int32_t)q[qh * 4], y = (int32_t)q[qh * 4 + 1], z = (int32_t)q[qh * 4 + 2];
+    uint32_t m = q[qh * 4 + 3];
+    qh++;
+    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
+  }
+#undef TAKE_PUSH
+  free(q);
+  free(trig);
+  /* maint (sim.wgsl:1219-1274) on every dirty cell */
+  for (size_t ci = 0; ci < ncell; ci++) {
+    if (!E.dirty[ci] || kind[ci] != 2) continue;
+    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8;
+    int32_t oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
+    uint32_t first = vox[vkey(&E, ox, oy, oz)];
+    int nz = 0, diff = 0;
+    for (uint32_t l = 0; l < 512u; l++) {
+      uint32_t v = vox[vkey(&E, ox + (int32_t)(l & 7u), oy + (int32_t)((l >> 3) & 7u), oz + (int32_t)(l >> 6))];
+      if (v != 0u) nz = 1;
+      if (v != first) diff = 1;
+    }
+    if (!nz) kind[ci] = 0;
+    else if (!diff) kind[ci] = 1;
+  }
+  free(E.dirty);
+  free(req);
+  if (stats) {
+    stats[0] = destroyed;
+    stats[1] = bill;
+    stats[2] = allocs;
+    stats[3] = njobs;
+    stats[4] = flooded;
+  }
+  return o_islands(dx, dy, dz, vox, mats, lab_key, lab_count, cap, n_labels);
+}
+
+/* Breadth-first labelling of the components of non-air, non-rubble,
+ * non-structural voxels.  Scanning keys in increasing order makes the first
+ * voxel of every component its minimum key. */
+int o_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats, uint32_t* lab_key,
+              uint32_t* lab_count, uint32_t cap, uint32_t* n_labels) {
+  size_t N = (size_t)dx * dy * dz;
+  uint8_t* seen = (uint8_t*)calloc((N + 7) / 8, 1);
+  uint32_t* q = (uint32_t*)malloc(N * sizeof(uint32_t) + 4);
+  if (!seen || !q) { free(seen); free(q); return -1; }
+  uint32_t nl = 0;
+  size_t plane = (size_t)dx * dy;
+  for (size_t s = 0; s < N; s++) {
+    uint32_t v = vox[s];
+    if (v == 0u || is_rubble(v) || structural(mats, v)) continue;
+    if (seen[s >> 3] & (1u << (s & 7))) continue;
+    seen[s >> 3] |= (uint8_t)(1u << (s & 7));
+    size_t qh = 0, qt = 0;
+    q[qt++] = (uint32_t)s;
+    int anchored = 0;
+    while (qh < qt) {
+      size_t k = q[qh++];
+      int32_t x = (int32_t)(k % (size_t)dx), y = (int32_t)((k / (size_t)dx) % (size_t)dy), z = (int32_t)(k / plane);
+      if (y == 0) anchored = 1;
+      for (int d = 0; d < 6; d++) {
+        int32_t nx = x + DIRS[d][0], ny = y + DIRS[d][1], nz = z + DIRS[d][2];
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= dx || ny >= dy || nz >= dz) continue;
+        size_t nk = (size_t)nx + (size_t)dx * ((size_t)ny + (size_t)dy * (size_t)nz);
+        uint32_t w = vox[nk];
+        if (w == 0u || is_rubble(w)) continue;
+        if (structural(mats, w)) { anchored = 1; continue; }
+        if (seen[nk >> 3] & (1u << (nk & 7))) continue;
+        seen[nk >> 3] |= (uint8_t)(1u << (nk & 7));
+        q[qt++] = (uint32_t)nk;
+      }
+    }
+    if (!anchored) {
+      if (nl < cap) {
+        lab_key[nl] = (uint32_t)s;
+        lab_count[nl] = (uint32_t)qt;
+      }
+      nl++;
+    }
+  }
+  free(seen);
+  free(q);
+  *n_labels = nl;
+  return 0;
+}

@@ -1,0 +1,83 @@
+/*
+ * px_oracle.h - CPU oracle for the brick-map voxel hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference's
+ * WGSL (Crunchy000/pixels) used as the checker of the HIP library: only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark call it.
+ * The product path never links or calls it.
+ *
+ * Built with -O2 -ffp-contract=off and without fast-math so every FP32
+ * operation rounds once, in the order the WGSL source writes it.
+ */
+#ifndef PX_ORACLE_H
+#define PX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct o_world {
+  int32_t dx, dy, dz;          /* voxel dims */
+  const uint32_t* grid;        /* brick grid */
+  const uint32_t* pool;        /* pool words: 256 per brick, 2 voxels per word */
+  const uint32_t* super_grid;  /* non-empty cell count per 4^3-brick region */
+  const float* mats;           /* 256 * 20 */
+} o_world;
+
+/* Same 16-byte record as px_hit in include/pixels.h. */
+typedef struct o_hit {
+  uint32_t key;
+  float t;
+  uint32_t steps;
+  uint8_t face;
+  uint8_t mat;
+  uint16_t value_hi;
+} o_hit;
+
+/* march() of raymarch.wgsl:44-162 on one ray. trans_out (3 floats) may be NULL. */
+void o_march(const o_world* w, const float ro[3], const float rd[3], float max_t, int shadow,
+             o_hit* out, float* trans_out);
+/* traceSolid() of sim.wgsl:296-357. */
+void o_trace_solid(const o_world* w, const float ro[3], const float rd[3], float max_t, o_hit* out);
+/* Segment DDA of particleSim, sim.wgsl:1110-1145 (pos = ro, displacement = mv). */
+void o_segment(const o_world* w, const float pos[3], const float mv[3], o_hit* out);
+
+/* Batch of 32-byte rays (ro[3], rd[3], max_t, flags) of one kind (0 primary,
+ * 1 shadow, 2 shoot, 3 segment; kind < 0 = per-ray flags & 3), threads >= 1. */
+void o_trace_batch(const o_world* w, int kind, const float* rays, uint32_t n, o_hit* out, int threads);
+
+/* One frame of fsMain + shade() shadow rays (raymarch.wgsl:304-356,388-398). */
+void o_frame(const o_world* w, const float* u52, const float* lights_vec4, uint32_t n_vec4,
+             const float* rays, uint32_t n, o_hit* prim_out, o_hit* shadow_out, uint32_t* n_cast,
+             int threads);
+
+/* Edit chain: carve + pane flood + maint + island labels (sim.wgsl:194-944,1205-1274).
+ * The world is given as a dense voxel array (x fastest) and a per-cell kind
+ * (0 empty, 1 uniform, 2 pool brick) taken from the grid before the pass.
+ * On return `vox` holds the decoded voxels after the pass and `kind` the
+ * canonical kinds after maint.  Labels of unsupported components are written
+ * (sorted by key) up to cap; *n_labels gets the total.  stats[0] = voxels
+ * destroyed, stats[1] = bill in dimes, stats[2] = cells allocated,
+ * stats[3] = carve jobs, stats[4] = voxels removed by the flood. */
+int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+                  const float* impacts16, uint32_t n_impacts, uint32_t* lab_key, uint32_t* lab_count,
+                  uint32_t cap, uint32_t* n_labels, uint64_t* stats);
+
+/* Unsupported-component labelling alone (the last stage of o_island_pass). */
+int o_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats,
+              uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels);
+
+/* Integer hashes of shared.wgsl:65-81 (exported for tests). */
+uint32_t o_hashu(uint32_t x);
+uint32_t o_hash3i(int32_t x, int32_t y, int32_t z);
+float o_rnd(uint32_t seed);
+/* removes() of sim.wgsl:229-239 for one voxel (impact = 16 floats). */
+int o_removes(const float* impact16, const float* mats, int32_t x, int32_t y, int32_t z, uint32_t v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,148 @@
+"""ctypes binding of libpixels_hip.so (the C ABI of include/pixels.h).
+
+The library is loaded from this package directory only; a missing library is
+an error (there is no CPU fallback for the hot path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpixels_hip.so")
+
+PX_OK = 0
+PX_NONE = 0xFFFFFFFF
+PX_UNIFORM_BIT = 0x80000000
+PX_UNIFORM_FLOATS = 52
+PX_MAT_FLOATS = 20
+PX_CTRL_WORDS = 36
+PX_ARGS_WORDS = 44
+PX_CAMERA_FLOATS = 64
+
+RAY_PRIMARY, RAY_SHADOW, RAY_SHOOT, RAY_SEGMENT, RAY_MIXED = 0, 1, 2, 3, 4
+FACE_ZERO, FACE_NOT_CAST, FACE_MISS = 6, 0xFE, 0xFF
+
+# CTRL_NAMES order, src/gpu/layout.ts:48-58
+CTRL_NAMES = [
+    "C_FREE_TOP", "C_IMPACTS", "C_JOBS", "C_ALLOCS", "C_ALLOC_N",
+    "C_FLOOD_CUR", "C_FLOOD_NEXT", "C_FLOOD_PAR",
+    "C_S_STATE", "C_S_CUR", "C_S_NEXT", "C_S_PAR", "C_NODES", "C_PENDING",
+    "C_S_ITERS", "C_SEED_N", "C_CHUNKS", "C_ARENA", "C_WRITES", "C_WRITE_N",
+    "C_DIRTY", "C_PCURSOR", "C_FLASH", "C_SPAWNED",
+    "S_BILL", "S_DESTROYED", "S_ALLOC_FAIL", "S_FALLING", "S_SEARCHES", "S_ISLANDS",
+    "S_SEED_DROP", "S_BUDGET", "S_HASH_FULL",
+]
+CTRL = {n: i for i, n in enumerate(CTRL_NAMES)}
+
+# numpy views of the 32-byte ray and 16-byte hit records
+RAY_DTYPE = np.dtype([("ro", "<f4", 3), ("rd", "<f4", 3), ("max_t", "<f4"), ("flags", "<u4")])
+HIT_DTYPE = np.dtype([("key", "<u4"), ("t", "<f4"), ("steps", "<u4"), ("face", "u1"), ("mat", "u1"),
+                      ("value_hi", "<u2")])
+
+
+class PxLimits(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "max_dirty", "max_alloc", "max_writes", "max_impacts", "max_jobs", "max_flood", "max_seeds",
+        "hash_size", "max_chunks", "max_arena", "max_lights", "max_anchors", "max_flashes",
+        "spawn_budget", "component_budget", "search_iters_per_frame", "flood_iters_per_frame")]
+
+
+class PxWorldDesc(C.Structure):
+    _fields_ = [
+        ("dims", C.c_uint32 * 3),
+        ("grid", C.POINTER(C.c_uint32)),
+        ("pool", C.POINTER(C.c_uint16)),
+        ("pool_used", C.c_uint32),
+        ("holes", C.POINTER(C.c_uint32)),
+        ("n_holes", C.c_uint32),
+        ("mats", C.POINTER(C.c_float)),
+        ("pool_capacity", C.c_uint32),
+        ("limits", C.POINTER(PxLimits)),
+        ("voxel_size", C.c_float),
+        ("device", C.c_int32),
+    ]
+
+
+# every exported symbol with (restype, argtypes)
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I32P = C.POINTER(C.c_int32)
+_U32P = C.POINTER(C.c_uint32)
+_FP = C.POINTER(C.c_float)
+SIGNATURES = {
+    "px_default_limits": (None, [C.POINTER(PxLimits)]),
+    "px_create": (C.c_int, [C.POINTER(PxWorldDesc), C.POINTER(_P)]),
+    "px_destroy": (C.c_int, [_P]),
+    "px_last_error": (C.c_char_p, [_P]),
+    "px_set_uniforms": (C.c_int, [_P, _FP]),
+    "px_set_lights": (C.c_int, [_P, _FP, _U32]),
+    "px_sync": (C.c_int, [_P]),
+    "px_stream": (_P, [_P]),
+    "px_trace": (C.c_int, [_P, _U32, _P, _U32, _P]),
+    "px_trace_device": (C.c_int, [_P, _U32, _P, _U32, _P]),
+    "px_trace_frame_device": (C.c_int, [_P, _P, _U32, _P, _P, _U32P]),
+    "px_frame_slots": (_U32, [_P]),
+    "px_probe": (C.c_int, [_P, _I32P, _I32P, _U32P]),
+    "px_probe_device": (C.c_int, [_P, _I32P, _I32P, _P]),
+    "px_read_ctrl": (C.c_int, [_P, _U32P, _U32P, _FP]),
+    "px_read_grid": (C.c_int, [_P, _U32P, _U32P]),
+    "px_read_pool": (C.c_int, [_P, _U32, _U32, _U32P]),
+    "px_world_info": (C.c_int, [_P, _U32P, _U32P, _U32P, _U32P]),
+    "px_push_impacts": (C.c_int, [_P, _FP, _U32]),
+    "px_island_pass": (C.c_int, [_P]),
+    "px_island_labels": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
+    "px_profile_begin": (C.c_int, [_P]),
+    "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
+}
+PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_SLOTS = 0, 1, 2, 3, 8
+
+_lib = None
+
+
+def load():
+    """Load libpixels_hip.so; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libpixels_hip.so is missing: build it with `python -m pixels_amd.build` "
+                           "(the hot path has no CPU fallback)")
+    # torch ships its own libamdhip64.so.7; loading it first makes this
+    # library bind to the same HIP runtime (one runtime per process), so torch
+    # tensors and this library's kernels can share device memory and streams.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("pixels error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != PX_OK:
+        lib = load()
+        raise PxError(rc, lib.px_last_error(ctx).decode(errors="replace"))
+
+
+def ptr(a: np.ndarray, ctype=C.c_uint32):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def default_limits() -> PxLimits:
+    lim = PxLimits()
+    load().px_default_limits(C.byref(lim))
+    return lim

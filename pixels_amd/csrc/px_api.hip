@@ -1,0 +1,420 @@
+// px_api.hip - the C ABI of libpixels_hip.so (declared in include/pixels.h).
+//
+// Context creation mirrors `new GpuWorld(device, world, scene)`
+// (src/gpu/GpuWorld.ts:46-137): pool capacity rule (:60), super-grid counts
+// (:90-100), free list = holes then the top of the pool in descending order
+// (:103-113), C_FREE_TOP (:111-113), indirect draw args (:114-117).
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "px_internal.h"
+
+static thread_local std::string g_create_error;
+
+int px_fail(px_ctx* ctx, int code, const char* what, const char* detail) {
+  std::string msg = std::string(what) + ": " + (detail ? detail : "");
+  if (ctx) ctx->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+static hipEvent_t take_event(px_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+PxProfScope::PxProfScope(px_ctx* ctx, int s) : c(ctx), slot(s) {
+  if (!c->prof) return;
+  a = take_event(c);
+  if (a) hipEventRecord(a, c->stream);
+}
+
+PxProfScope::~PxProfScope() {
+  if (!a) return;
+  hipEvent_t b = take_event(c);
+  if (!b) {
+    c->ev_pool.push_back(a);
+    return;
+  }
+  hipEventRecord(b, c->stream);
+  c->prof_rec.push_back({slot, a, b});
+}
+
+extern "C" {
+
+void px_default_limits(px_limits* l) {
+  if (!l) return;
+  // src/gpu/layout.ts:11-33
+  l->max_dirty = 65535;
+  l->max_alloc = 65535;
+  l->max_writes = 131072;
+  l->max_impacts = 64;
+  l->max_jobs = 65535;
+  l->max_flood = 131072;
+  l->max_seeds = 262144;
+  l->hash_size = 1u << 20;
+  l->max_chunks = 256;
+  l->max_arena = 1u << 20;
+  l->max_lights = 32;
+  l->max_anchors = 256;
+  l->max_flashes = 8;
+  l->spawn_budget = 60000;
+  l->component_budget = 400000;
+  l->search_iters_per_frame = 16;
+  l->flood_iters_per_frame = 12;
+}
+
+const char* px_last_error(const px_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_create_error.c_str();
+}
+
+static void release(px_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  for (auto& r : c->prof_rec) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
+                  c->d_super_mask, c->d_brick_mask, c->d_shadow_rays, c->d_counters};
+  for (void* p : bufs)
+    if (p) hipFree(p);
+  px_edit_free(c);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int px_create(const px_world_desc* d, px_ctx** out) {
+  if (!d || !out) return px_fail(nullptr, PX_ERR_ARG, "px_create", "null argument");
+  *out = nullptr;
+  for (int a = 0; a < 3; a++)
+    if (d->dims[a] == 0 || (d->dims[a] & 7u)) return px_fail(nullptr, PX_ERR_ARG, "px_create", "dims must be positive multiples of 8");
+  if (!d->grid || !d->mats || (d->pool_used && !d->pool) || (d->n_holes && !d->holes))
+    return px_fail(nullptr, PX_ERR_ARG, "px_create", "missing grid/pool/mats/holes");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return px_fail(nullptr, PX_ERR_NODEVICE, "px_create", "no HIP device");
+  if (d->device < 0 || d->device >= ndev) return px_fail(nullptr, PX_ERR_ARG, "px_create", "bad device ordinal");
+  uint64_t cells64 = (uint64_t)(d->dims[0] / 8) * (d->dims[1] / 8) * (d->dims[2] / 8);
+  uint64_t vox64 = (uint64_t)d->dims[0] * d->dims[1] * d->dims[2];
+  if (vox64 > 0xFFFFFFFFull) return px_fail(nullptr, PX_ERR_CAPACITY, "px_create", "voxel keys must fit in 32 bits");
+
+  px_ctx* c = new (std::nothrow) px_ctx();
+  if (!c) return px_fail(nullptr, PX_ERR_ARG, "px_create", "out of host memory");
+  c->device = d->device;
+  if (hipSetDevice(c->device) != hipSuccess) { release(c); return px_fail(nullptr, PX_ERR_HIP, "px_create", "hipSetDevice"); }
+  if (d->limits) c->lim = *d->limits;
+  else px_default_limits(&c->lim);
+  c->vs = d->voxel_size > 0.0f ? d->voxel_size : 1.0f;
+  for (int a = 0; a < 3; a++) {
+    c->dims[a] = d->dims[a];
+    c->bd[a] = d->dims[a] >> 3;
+    c->sd[a] = (c->bd[a] + 3u) >> 2;
+  }
+  c->ncell = (uint32_t)cells64;
+  c->nsuper = c->sd[0] * c->sd[1] * c->sd[2];
+  c->vox_total = (size_t)vox64;
+  uint32_t used = d->pool_used;
+  // GpuWorld.ts:57-60 (the storage-binding limit becomes the u32 word-index limit)
+  uint64_t max_bricks = (1ull << 32) / 512ull;
+  uint64_t cap = d->pool_capacity ? d->pool_capacity : std::max<uint64_t>((uint64_t)used + 98304ull, 2ull * used);
+  if (cap > max_bricks) cap = max_bricks;
+  if (used > cap) { release(c); return px_fail(nullptr, PX_ERR_CAPACITY, "px_create", "scene needs more bricks than the pool holds"); }
+  c->pool_cap = (uint32_t)cap;
+  c->pool_used = used;
+
+  std::string step;
+#define TRY(expr)                                                    \
+  do {                                                               \
+    hipError_t e_ = (expr);                                          \
+    if (e_ != hipSuccess) {                                          \
+      std::string m_ = std::string(#expr) + ": " + hipGetErrorString(e_); \
+      release(c);                                                    \
+      return px_fail(nullptr, PX_ERR_HIP, "px_create", m_.c_str());  \
+    }                                                                \
+  } while (0)
+  TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  TRY(hipMalloc(&c->d_grid, (size_t)c->ncell * 4));
+  TRY(hipMalloc(&c->d_pool, (size_t)c->pool_cap * 1024));
+  TRY(hipMalloc(&c->d_super, (size_t)c->nsuper * 4));
+  TRY(hipMalloc(&c->d_mats, PX_MAT_COUNT * PX_MAT_FLOATS * 4));
+  TRY(hipMalloc(&c->d_ctrl, PX_CTRL_WORDS * 4));
+  TRY(hipMalloc(&c->d_args, PX_ARGS_WORDS * 4));
+  TRY(hipMalloc(&c->d_camera, PX_CAMERA_FLOATS * 4));
+  TRY(hipMalloc(&c->d_free, (size_t)c->pool_cap * 4));
+  TRY(hipMalloc(&c->d_brick_mask, (size_t)((c->ncell + 31u) / 32u) * 4 + 256));
+  TRY(hipMalloc(&c->d_super_mask, (size_t)((c->nsuper + 31u) / 32u) * 4 + 256));
+  TRY(hipMalloc(&c->d_counters, 256));
+
+  // ---- one-time upload (GpuWorld.ts:85-117) ----
+  TRY(hipMemcpy(c->d_mats, d->mats, PX_MAT_COUNT * PX_MAT_FLOATS * 4, hipMemcpyHostToDevice));
+  TRY(hipMemcpy(c->d_grid, d->grid, (size_t)c->ncell * 4, hipMemcpyHostToDevice));
+  TRY(hipMemset(c->d_pool, 0, (size_t)c->pool_cap * 1024));
+  if (used) TRY(hipMemcpy(c->d_pool, d->pool, (size_t)used * 1024, hipMemcpyHostToDevice));
+  {
+    std::vector<uint32_t> sc(c->nsuper, 0u);
+    for (uint32_t z = 0; z < c->bd[2]; z++)
+      for (uint32_t y = 0; y < c->bd[1]; y++)
+        for (uint32_t x = 0; x < c->bd[0]; x++)
+          if (d->grid[x + c->bd[0] * (y + c->bd[1] * z)] != 0u)
+            sc[(x >> 2) + c->sd[0] * ((y >> 2) + c->sd[1] * (z >> 2))]++;
+    TRY(hipMemcpy(c->d_super, sc.data(), (size_t)c->nsuper * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> fl((size_t)d->n_holes + (c->pool_cap - used));
+    for (uint32_t i = 0; i < d->n_holes; i++) fl[i] = d->holes[i];
+    for (uint32_t i = 0; i < c->pool_cap - used; i++) fl[d->n_holes + i] = c->pool_cap - 1u - i;
+    if (!fl.empty()) TRY(hipMemcpy(c->d_free, fl.data(), fl.size() * 4, hipMemcpyHostToDevice));
+    uint32_t ctrl[PX_CTRL_WORDS] = {0};
+    ctrl[PX_C_FREE_TOP] = (uint32_t)fl.size();
+    TRY(hipMemcpy(c->d_ctrl, ctrl, sizeof ctrl, hipMemcpyHostToDevice));
+    uint32_t args[PX_ARGS_WORDS] = {0};
+    args[36] = 36;  // A_DRAWP
+    args[40] = 36;  // A_DRAWC
+    TRY(hipMemcpy(c->d_args, args, sizeof args, hipMemcpyHostToDevice));
+    float cam[PX_CAMERA_FLOATS] = {0};
+    TRY(hipMemcpy(c->d_camera, cam, sizeof cam, hipMemcpyHostToDevice));
+  }
+#undef TRY
+  c->h_uniforms[0] = (float)c->dims[0];
+  c->h_uniforms[1] = (float)c->dims[1];
+  c->h_uniforms[2] = (float)c->dims[2];
+  c->h_uniforms[3] = c->vs;
+  c->h_lights.assign(4, 0.0f);
+  c->shadow.slots = 1;
+  c->shadow.vs = c->vs;
+  c->masks_dirty = true;
+  int rc = px_launch_masks(c);
+  if (rc == PX_OK) rc = px_edit_alloc(c);
+  if (rc != PX_OK) {
+    std::string m = c->err;
+    release(c);
+    return px_fail(nullptr, rc, "px_create", m.c_str());
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    release(c);
+    return px_fail(nullptr, PX_ERR_HIP, "px_create", "initial upload failed");
+  }
+  *out = c;
+  return PX_OK;
+}
+
+int px_destroy(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  release(c);
+  return PX_OK;
+}
+
+static void decode_shadow(px_ctx* c) {
+  PxShadowDesc& s = c->shadow;
+  const float* u = c->h_uniforms;
+  s.sun[0] = u[4]; s.sun[1] = u[5]; s.sun[2] = u[6]; s.sun_w = u[7];
+  s.vs = u[3];
+  s.lamp_shadows = u[21];
+  float cf = c->h_lights.empty() ? 0.0f : c->h_lights[0];
+  int count = (int)cf;
+  if (count < 0) count = 0;
+  int avail = c->h_lights.size() >= 4 ? (int)((c->h_lights.size() / 4 - 1) / 2) : 0;
+  if (count > avail) count = avail;
+  if (count > 32) count = 32;
+  s.n_lights = (uint32_t)count;
+  s.slots = 1u + (uint32_t)count;
+  for (int i = 0; i < count; i++) {
+    for (int a = 0; a < 4; a++) s.lp[i][a] = c->h_lights[(1 + 2 * i) * 4 + a];
+    s.lc_w[i] = c->h_lights[(2 + 2 * i) * 4 + 3];
+  }
+}
+
+int px_set_uniforms(px_ctx* c, const float u[PX_UNIFORM_FLOATS]) {
+  if (!c || !u) return PX_ERR_ARG;
+  std::memcpy(c->h_uniforms, u, sizeof c->h_uniforms);
+  decode_shadow(c);
+  return PX_OK;
+}
+
+int px_set_lights(px_ctx* c, const float* lights, uint32_t n_vec4) {
+  if (!c || (!lights && n_vec4)) return PX_ERR_ARG;
+  if (n_vec4 == 0) c->h_lights.assign(4, 0.0f);
+  else c->h_lights.assign(lights, lights + (size_t)n_vec4 * 4);
+  if (c->h_lights[0] > 32.0f) return px_fail(c, PX_ERR_CAPACITY, "px_set_lights", "at most 32 lights");
+  decode_shadow(c);
+  return PX_OK;
+}
+
+uint32_t px_frame_slots(const px_ctx* c) { return c ? c->shadow.slots : 0u; }
+
+int px_sync(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+void* px_stream(px_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int px_trace_device(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, px_hit* d_out) {
+  if (!c || (n && (!d_rays || !d_out))) return PX_ERR_ARG;
+  if (kind > 4u) return px_fail(c, PX_ERR_ARG, "px_trace_device", "unknown ray kind");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_trace(c, kind, d_rays, n, d_out);
+}
+
+int px_trace(px_ctx* c, uint32_t kind, const px_ray* rays, uint32_t n, px_hit* out) {
+  if (!c || (n && (!rays || !out))) return PX_ERR_ARG;
+  if (kind > 4u) return px_fail(c, PX_ERR_ARG, "px_trace", "unknown ray kind");
+  if (n == 0) return PX_OK;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  px_ray* dr = nullptr;
+  px_hit* dh = nullptr;
+  PX_HIP_CHECK(c, hipMalloc(&dr, (size_t)n * sizeof(px_ray)));
+  if (hipMalloc(&dh, (size_t)n * sizeof(px_hit)) != hipSuccess) {
+    hipFree(dr);
+    return px_fail(c, PX_ERR_HIP, "px_trace", "hipMalloc");
+  }
+  int rc = PX_OK;
+  if (hipMemcpyAsync(dr, rays, (size_t)n * sizeof(px_ray), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = px_fail(c, PX_ERR_HIP, "px_trace", "upload");
+  if (rc == PX_OK) rc = px_launch_trace(c, kind, dr, n, dh);
+  if (rc == PX_OK && hipMemcpyAsync(out, dh, (size_t)n * sizeof(px_hit), hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = px_fail(c, PX_ERR_HIP, "px_trace", "download");
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == PX_OK) rc = px_fail(c, PX_ERR_HIP, "px_trace", "sync");
+  hipFree(dr);
+  hipFree(dh);
+  return rc;
+}
+
+int px_trace_frame_device(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
+                          uint32_t* n_cast) {
+  if (!c || (n && (!d_primary || !d_prim || !d_shadow))) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_frame(c, d_primary, n, d_prim, d_shadow, n_cast);
+}
+
+static bool box_ok(const px_ctx* c, const int32_t s[3]) {
+  (void)c;
+  return s[0] >= 0 && s[1] >= 0 && s[2] >= 0 && (uint64_t)s[0] * s[1] * s[2] <= (1ull << 34);
+}
+
+int px_probe_device(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out) {
+  if (!c || !o || !s) return PX_ERR_ARG;
+  if (!box_ok(c, s)) return px_fail(c, PX_ERR_ARG, "px_probe_device", "bad box size");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_probe(c, o, s, d_out);
+}
+
+int px_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* out) {
+  if (!c || !o || !s || !out) return PX_ERR_ARG;
+  if (!box_ok(c, s)) return px_fail(c, PX_ERR_ARG, "px_probe", "bad box size");
+  size_t n = (size_t)s[0] * s[1] * s[2];
+  if (n == 0) return PX_OK;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  uint32_t* d = nullptr;
+  PX_HIP_CHECK(c, hipMalloc(&d, n * 4));
+  int rc = px_launch_probe(c, o, s, d);
+  if (rc == PX_OK && hipMemcpyAsync(out, d, n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = px_fail(c, PX_ERR_HIP, "px_probe", "download");
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == PX_OK) rc = px_fail(c, PX_ERR_HIP, "px_probe", "sync");
+  hipFree(d);
+  return rc;
+}
+
+int px_read_ctrl(px_ctx* c, uint32_t* ctrl, uint32_t* args, float* cam) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (ctrl) PX_HIP_CHECK(c, hipMemcpyAsync(ctrl, c->d_ctrl, PX_CTRL_WORDS * 4, hipMemcpyDeviceToHost, c->stream));
+  if (args) PX_HIP_CHECK(c, hipMemcpyAsync(args, c->d_args, PX_ARGS_WORDS * 4, hipMemcpyDeviceToHost, c->stream));
+  if (cam) PX_HIP_CHECK(c, hipMemcpyAsync(cam, c->d_camera, PX_CAMERA_FLOATS * 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+int px_read_grid(px_ctx* c, uint32_t* grid, uint32_t* sgrid) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (grid) PX_HIP_CHECK(c, hipMemcpyAsync(grid, c->d_grid, (size_t)c->ncell * 4, hipMemcpyDeviceToHost, c->stream));
+  if (sgrid) PX_HIP_CHECK(c, hipMemcpyAsync(sgrid, c->d_super, (size_t)c->nsuper * 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+int px_read_pool(px_ctx* c, uint32_t first, uint32_t count, uint32_t* words) {
+  if (!c || (count && !words)) return PX_ERR_ARG;
+  if ((uint64_t)first + count > c->pool_cap) return px_fail(c, PX_ERR_ARG, "px_read_pool", "range outside the pool");
+  if (!count) return PX_OK;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipMemcpyAsync(words, c->d_pool + (size_t)first * 256, (size_t)count * 1024, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+int px_world_info(const px_ctx* c, uint32_t dims[3], uint32_t bricks[3], uint32_t supers[3], uint32_t* pool_capacity) {
+  if (!c) return PX_ERR_ARG;
+  for (int a = 0; a < 3; a++) {
+    if (dims) dims[a] = c->dims[a];
+    if (bricks) bricks[a] = c->bd[a];
+    if (supers) supers[a] = c->sd[a];
+  }
+  if (pool_capacity) *pool_capacity = c->pool_cap;
+  return PX_OK;
+}
+
+int px_push_impacts(px_ctx* c, const float* impacts16, uint32_t n) {
+  if (!c || (n && !impacts16)) return PX_ERR_ARG;
+  uint32_t maxi = c->lim.max_impacts < 64u ? c->lim.max_impacts : 64u;
+  if ((uint64_t)c->n_impacts + n > maxi)
+    return px_fail(c, PX_ERR_CAPACITY, "px_push_impacts", "more impacts than max_impacts (at most 64 per pass)");
+  if (!n) return PX_OK;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_impacts + (size_t)c->n_impacts * 16, impacts16, (size_t)n * 64,
+                                 hipMemcpyHostToDevice, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->n_impacts += n;
+  return PX_OK;
+}
+
+int px_profile_begin(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  c->prof = true;
+  return PX_OK;
+}
+
+int px_profile_end(px_ctx* c, double out[2 * PX_PROF_SLOTS]) {
+  if (!c || !out) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 2 * PX_PROF_SLOTS; k++) out[k] = 0.0;
+  for (auto& r : c->prof_rec) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.slot >= 0 && r.slot < PX_PROF_SLOTS) {
+      out[2 * r.slot] += ms;
+      out[2 * r.slot + 1] += 1.0;
+    }
+    c->ev_pool.push_back(r.a);
+    c->ev_pool.push_back(r.b);
+  }
+  c->prof_rec.clear();
+  c->prof = false;
+  return PX_OK;
+}
+
+int px_island_pass(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_island_pass(c);
+}
+
+int px_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n) {
+  if (!c || !n || (cap && (!min_key || !count))) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_read_island_labels(c, min_key, count, cap, n);
+}
+
+}  // extern "C"

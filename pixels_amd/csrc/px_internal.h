@@ -1,0 +1,175 @@
+// px_internal.h - context layout and device helpers shared by the HIP sources
+// of libpixels_hip.so.  Everything here is gfx950 (CDNA4) code: wave64,
+// 160 KB LDS per CU, FP32 rounded once per operation (built with
+// -ffp-contract=off so the arithmetic order of the reference's WGSL is kept).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pixels.h"
+
+#pragma clang fp contract(off)
+
+#define PX_HIP_CHECK(ctx, expr)                                                   \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) return px_fail((ctx), PX_ERR_HIP, #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// Read-only view of the world as the traversal kernels see it.
+struct PxWorldView {
+  const uint32_t* grid;
+  const uint32_t* pool;
+  const uint32_t* super_grid;
+  const float* mats;
+  const uint32_t* super_mask;  // 1 bit per super cell: count != 0
+  const uint32_t* brick_mask;  // 1 bit per brick cell: grid != 0
+  int32_t dims[3];
+  int32_t bd[3];
+  int32_t sd[3];
+  float fdims[3];
+};
+
+// Per-frame shadow-ray description (decoded from the uniforms + light list).
+struct PxShadowDesc {
+  float sun[3];
+  float sun_w;
+  float vs;
+  float lamp_shadows;  // U.flags.y
+  uint32_t n_lights;
+  uint32_t slots;
+  float lp[32][4];  // light pos (m), radius (m)
+  float lc_w[32];   // light shadow flag
+};
+
+struct px_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t dims[3] = {0, 0, 0}, bd[3] = {0, 0, 0}, sd[3] = {0, 0, 0};
+  uint32_t ncell = 0, nsuper = 0, pool_cap = 0, pool_used = 0;
+  px_limits lim{};
+  float vs = 1.0f;
+  // world
+  uint32_t* d_grid = nullptr;
+  uint32_t* d_pool = nullptr;
+  uint32_t* d_super = nullptr;
+  float* d_mats = nullptr;
+  uint32_t* d_ctrl = nullptr;
+  uint32_t* d_args = nullptr;
+  float* d_camera = nullptr;
+  uint32_t* d_free = nullptr;
+  // traversal acceleration (derived from grid / super grid)
+  uint32_t* d_super_mask = nullptr;
+  uint32_t* d_brick_mask = nullptr;
+  bool masks_dirty = true;
+  // frame
+  px_ray* d_shadow_rays = nullptr;
+  size_t shadow_cap = 0;
+  uint32_t* d_counters = nullptr;  // small scratch counters
+  // edit chain
+  float* d_impacts = nullptr;
+  uint32_t n_impacts = 0;
+  uint64_t* d_cell_imask = nullptr;  // per cell: bit k = impact k has a job here
+  uint32_t* d_alloc_flags = nullptr; // per cell bitset: alloc requested
+  uint32_t* d_dirty_flags = nullptr; // per cell bitset: dirty
+  uint32_t* d_job_list = nullptr;    // cells with at least one job
+  uint32_t* d_alloc_list = nullptr;
+  uint32_t* d_dirty_list = nullptr;
+  uint32_t* d_trig = nullptr;        // carved pane voxels (key, id)
+  uint32_t trig_cap = 0;
+  uint32_t* d_edit_ctr = nullptr;    // edit-pass counters
+  // component labelling
+  uint32_t* d_par = nullptr;     // union-find parent per voxel key (node ids are keys)
+  uint32_t* d_ncnt = nullptr;    // per node: voxel count, then per-root totals
+  uint32_t* d_nflag = nullptr;   // per node flags
+  uint16_t* d_lab = nullptr;     // per pool brick: local root of each voxel (0xFFFF = none)
+  uint32_t* d_nodes = nullptr;   // node list
+  uint32_t node_cap = 0;
+  uint32_t* d_out_labels = nullptr;  // (key, count) pairs
+  uint32_t label_cap = 0;
+  size_t vox_total = 0;
+  // profiling: event pairs recorded around kernels while `prof` is set
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct ProfRec {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec> prof_rec;
+  // host mirrors
+  float h_uniforms[PX_UNIFORM_FLOATS] = {0};
+  std::vector<float> h_lights;
+  PxShadowDesc shadow{};
+  std::string err;
+};
+
+int px_fail(px_ctx* ctx, int code, const char* what, const char* detail);
+
+// Records an event pair around the kernels launched in its scope when
+// profiling is on (px_profile_begin).
+struct PxProfScope {
+  px_ctx* c;
+  int slot;
+  hipEvent_t a = nullptr;
+  PxProfScope(px_ctx* ctx, int s);
+  ~PxProfScope();
+};
+PxWorldView px_view(const px_ctx* c);
+int px_refresh_masks(px_ctx* c);
+
+// launchers (px_trace.hip)
+int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, px_hit* d_out);
+int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
+                    uint32_t* n_cast);
+int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
+int px_launch_masks(px_ctx* c);
+// edit chain (px_edit.hip)
+int px_edit_alloc(px_ctx* c);
+void px_edit_free(px_ctx* c);
+int px_launch_island_pass(px_ctx* c);
+int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+
+// ---------------------------------------------------------------------------
+// device helpers (shared.wgsl:44-81, render_common.wgsl:25-31)
+
+#define PX_DEV __device__ __forceinline__
+
+PX_DEV uint32_t px_hashu(uint32_t x) {
+  uint32_t h = x;
+  h ^= h >> 16u;
+  h *= 0x7feb352du;
+  h ^= h >> 15u;
+  h *= 0x846ca68bu;
+  h ^= h >> 16u;
+  return h;
+}
+PX_DEV uint32_t px_hash3i(int32_t x, int32_t y, int32_t z) {
+  return px_hashu(((uint32_t)x * 0x9E3779B1u) ^
+                  px_hashu(((uint32_t)y * 0x85EBCA77u) ^ px_hashu((uint32_t)z * 0xC2B2AE3Du)));
+}
+PX_DEV float px_rnd(uint32_t seed) { return (float)(px_hashu(seed) & 0xffffffu) / 16777216.0f; }
+
+// WGSL i32(f32): truncation toward zero, saturating (v_cvt_i32_f32 saturates;
+// NaN gives 0).
+PX_DEV int32_t px_f2i(float x) { return (int32_t)__builtin_amdgcn_fmed3f(x, -2147483648.0f, 2147483520.0f); }
+PX_DEV int32_t px_clampi(int32_t v, int32_t lo, int32_t hi) { return min(max(v, lo), hi); }
+PX_DEV float px_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+PX_DEV float px_canon0(float t) { return t == 0.0f ? 0.0f : t; }
+PX_DEV float px_pow5(float x) {
+  float x2 = x * x;
+  float x4 = x2 * x2;
+  return x4 * x;
+}
+PX_DEV bool px_is_glass(const float* mats, uint32_t id) { return mats[id * 20u + 13u] > 0.0f; }
+
+PX_DEV uint32_t px_brick_voxel(const uint32_t* pool, uint32_t g, int32_t lx, int32_t ly, int32_t lz) {
+  if ((g & PX_UNIFORM_BIT) != 0u) return g & 0xffffu;
+  uint32_t li = (uint32_t)(lx | (ly << 3) | (lz << 6));
+  uint32_t idx = (g - 1u) * 512u + li;
+  uint32_t w = pool[idx >> 1u];
+  return (w >> ((idx & 1u) * 16u)) & 0xffffu;
+}

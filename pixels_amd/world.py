@@ -1,0 +1,198 @@
+"""Host-side mirror of the reference's GpuWorld / Simulation API over the C ABI.
+
+  GpuWorld    src/gpu/GpuWorld.ts:17-137   (one-time upload, pool sizing,
+                                            super counts, free list)
+  Simulation  src/gpu/Simulation.ts:61-323 (probe, per-frame stats readback)
+              plus the ray queries and the island pass of the hot path.
+
+Every call goes to libpixels_hip.so; device buffers may be passed as torch
+tensors (their data_ptr() is used) or host numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .scene import Scene
+
+
+def _devptr(t) -> int:
+    return int(t.data_ptr())
+
+
+class GpuWorld:
+    """Every device resource of one world (GpuWorld.ts:17-137)."""
+
+    def __init__(self, scene: Scene, device: int = 0, limits: N.PxLimits | None = None, pool_capacity: int = 0,
+                 voxel_size: float = 1.0 / 80.0):
+        lib = N.load()
+        self.lib = lib
+        self.scene = scene
+        d = N.PxWorldDesc()
+        d.dims[:] = scene.dims
+        self._grid = np.ascontiguousarray(scene.grid, np.uint32)
+        self._pool = np.ascontiguousarray(scene.pool, np.uint16)
+        self._mats = np.ascontiguousarray(scene.mats, np.float32)
+        d.grid = N.ptr(self._grid)
+        d.pool = N.ptr(self._pool, C.c_uint16) if self._pool.size else None
+        d.pool_used = scene.pool_used
+        d.holes = None
+        d.n_holes = 0
+        d.mats = N.ptr(self._mats, C.c_float)
+        d.pool_capacity = pool_capacity
+        self.limits = limits if limits is not None else N.default_limits()
+        d.limits = C.pointer(self.limits)
+        d.voxel_size = voxel_size
+        d.device = device
+        h = C.c_void_p()
+        N.check(lib.px_create(C.byref(d), C.byref(h)), None)
+        self.ctx = h
+        dims = (C.c_uint32 * 3)()
+        bricks = (C.c_uint32 * 3)()
+        supers = (C.c_uint32 * 3)()
+        cap = C.c_uint32()
+        N.check(lib.px_world_info(h, dims, bricks, supers, C.byref(cap)), h)
+        self.dims = tuple(dims)
+        self.bricks = tuple(bricks)
+        self.supers = tuple(supers)
+        self.numCells = self.bricks[0] * self.bricks[1] * self.bricks[2]
+        self.poolCapacity = int(cap.value)
+        self.initialBricks = scene.pool_used
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.px_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc: int) -> None:
+        N.check(rc, self.ctx)
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.px_stream(self.ctx) or 0)
+
+    def sync(self) -> None:
+        self.check(self.lib.px_sync(self.ctx))
+
+    def set_uniforms(self, u: np.ndarray) -> None:
+        u = np.ascontiguousarray(u, np.float32)
+        assert u.size == N.PX_UNIFORM_FLOATS
+        self.check(self.lib.px_set_uniforms(self.ctx, N.ptr(u, C.c_float)))
+
+    def set_lights(self, lights: np.ndarray) -> None:
+        lights = np.ascontiguousarray(lights, np.float32)
+        self.check(self.lib.px_set_lights(self.ctx, N.ptr(lights, C.c_float), lights.size // 4))
+
+    def profile_begin(self) -> None:
+        self.check(self.lib.px_profile_begin(self.ctx))
+
+    def profile_end(self) -> dict:
+        """{slot: (summed ms, launches)} of the kernels launched since profile_begin."""
+        out = (C.c_double * (2 * N.PROF_SLOTS))()
+        self.check(self.lib.px_profile_end(self.ctx, out))
+        return {k: (out[2 * k], int(out[2 * k + 1])) for k in range(N.PROF_SLOTS) if out[2 * k + 1] > 0}
+
+    def read_grid(self):
+        g = np.empty(self.numCells, np.uint32)
+        s = np.empty(self.supers[0] * self.supers[1] * self.supers[2], np.uint32)
+        self.check(self.lib.px_read_grid(self.ctx, N.ptr(g), N.ptr(s)))
+        return g, s
+
+    def read_pool(self, first: int, count: int) -> np.ndarray:
+        w = np.empty(count * 256, np.uint32)
+        self.check(self.lib.px_read_pool(self.ctx, first, count, N.ptr(w)))
+        return w
+
+
+class SimStats(dict):
+    """Decoded counters, as SimStats of Simulation.ts:24-39."""
+
+
+class Simulation:
+    """Dispatch / queue API of Simulation.ts over the library."""
+
+    def __init__(self, gw: GpuWorld):
+        self.gw = gw
+        self.lib = gw.lib
+        self.stats = SimStats()
+
+    # -- Simulation.probe (Simulation.ts:161-196) --------------------------
+    def probe(self, origin, size) -> np.ndarray:
+        o = (C.c_int32 * 3)(*origin)
+        s = (C.c_int32 * 3)(*size)
+        out = np.empty(int(size[0]) * int(size[1]) * int(size[2]), np.uint32)
+        self.gw.check(self.lib.px_probe(self.gw.ctx, o, s, N.ptr(out)))
+        return out
+
+    def probe_device(self, origin, size, out_tensor) -> None:
+        o = (C.c_int32 * 3)(*origin)
+        s = (C.c_int32 * 3)(*size)
+        self.gw.check(self.lib.px_probe_device(self.gw.ctx, o, s, C.c_void_p(_devptr(out_tensor))))
+
+    # -- afterSubmit readback (Simulation.ts:289-322) ----------------------
+    def read_ctrl(self):
+        ctrl = np.zeros(N.PX_CTRL_WORDS, np.uint32)
+        args = np.zeros(N.PX_ARGS_WORDS, np.uint32)
+        cam = np.zeros(N.PX_CAMERA_FLOATS, np.float32)
+        self.gw.check(self.lib.px_read_ctrl(self.gw.ctx, N.ptr(ctrl), N.ptr(args), N.ptr(cam, C.c_float)))
+        return ctrl, args, cam
+
+    def after_submit(self) -> SimStats:
+        ctrl, args, cam = self.read_ctrl()
+        c = N.CTRL
+        self.stats = SimStats(
+            bill=ctrl[c["S_BILL"]] / 10.0,
+            destroyed=int(ctrl[c["S_DESTROYED"]]),
+            falling=int(ctrl[c["S_FALLING"]]),
+            liveBricks=self.gw.poolCapacity - int(ctrl[c["C_FREE_TOP"]]),
+            allocFailures=int(ctrl[c["S_ALLOC_FAIL"]]),
+            searches=int(ctrl[c["S_SEARCHES"]]),
+            islands=int(ctrl[c["S_ISLANDS"]]),
+            searching=int(ctrl[c["C_S_STATE"]]) != 0,
+        )
+        return self.stats
+
+    # -- ray queries --------------------------------------------------------
+    def trace(self, kind: int, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, N.RAY_DTYPE)
+        out = np.zeros(rays.shape[0], N.HIT_DTYPE)
+        self.gw.check(self.lib.px_trace(self.gw.ctx, kind, rays.ctypes.data, rays.shape[0], out.ctypes.data))
+        return out
+
+    def trace_device(self, kind: int, rays_t, n: int, out_t) -> None:
+        self.gw.check(self.lib.px_trace_device(self.gw.ctx, kind, C.c_void_p(_devptr(rays_t)), n,
+                                               C.c_void_p(_devptr(out_t))))
+
+    def frame_slots(self) -> int:
+        return int(self.lib.px_frame_slots(self.gw.ctx))
+
+    def trace_frame_device(self, rays_t, n: int, prim_t, shadow_t, count: bool = False):
+        nc = C.c_uint32(0)
+        self.gw.check(self.lib.px_trace_frame_device(self.gw.ctx, C.c_void_p(_devptr(rays_t)), n,
+                                                     C.c_void_p(_devptr(prim_t)), C.c_void_p(_devptr(shadow_t)),
+                                                     C.byref(nc) if count else None))
+        return int(nc.value) if count else None
+
+    # -- edit chain -----------------------------------------------------------
+    def push_impacts(self, impacts: np.ndarray) -> None:
+        imp = np.ascontiguousarray(impacts, np.float32).reshape(-1, 16)
+        self.gw.check(self.lib.px_push_impacts(self.gw.ctx, N.ptr(imp, C.c_float), imp.shape[0]))
+
+    def island_pass(self) -> None:
+        self.gw.check(self.lib.px_island_pass(self.gw.ctx))
+
+    def island_labels(self, cap: int = 1 << 20):
+        keys = np.zeros(cap, np.uint32)
+        cnt = np.zeros(cap, np.uint32)
+        n = C.c_uint32(0)
+        self.gw.check(self.lib.px_island_labels(self.gw.ctx, N.ptr(keys), N.ptr(cnt), cap, C.byref(n)))
+        m = min(cap, int(n.value))
+        return keys[:m].copy(), cnt[:m].copy(), int(n.value)
