@@ -207,6 +207,14 @@ int px_island_pass(px_ctx* ctx);
  * *n receives the total count even when it exceeds cap.  Blocking. */
 int px_island_labels(px_ctx* ctx, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
 
+/* ---- checkpoint of the device world (no counterpart in the reference, which
+ * rebuilds its world from the seed on every load and never saves GPU state).
+ * px_snapshot copies grid, pool, super grid, ctrl words and free list to a
+ * device-side backup; px_restore copies them back and drops queued impacts
+ * and labels.  Both are asynchronous on the context's stream. */
+int px_snapshot(px_ctx* ctx);
+int px_restore(px_ctx* ctx);
+
 /* ---- profiling (no counterpart in the reference, which has no GPU timers;
  * for callers that report per-kernel time and the roofline) -------------
  * px_profile_begin starts recording HIP events around every kernel the

@@ -94,6 +94,8 @@ SIGNATURES = {
     "px_push_impacts": (C.c_int, [_P, _FP, _U32]),
     "px_island_pass": (C.c_int, [_P]),
     "px_island_labels": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
+    "px_snapshot": (C.c_int, [_P]),
+    "px_restore": (C.c_int, [_P]),
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }

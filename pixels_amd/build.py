@@ -69,11 +69,32 @@ def build_scene(verbose: bool = False) -> str:
     return out
 
 
+NODE_INCLUDE = "/usr/include/node"
+
+
+def build_napi(verbose: bool = False) -> str | None:
+    """N-API addon for the Node host (needs node_api.h); links libpixels_hip.so."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print("node_api.h not found: N-API addon skipped")
+        return None
+    out = os.path.join(HERE, "node", "pixels_napi.node")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + NODE_INCLUDE, "-DNAPI_VERSION=8",
+          os.path.join(HERE, "node", "px_napi.cc"), "-o", tmp, "-L" + HERE, "-lpixels_hip",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_all(verbose: bool = False) -> None:
     with ThreadPoolExecutor(max_workers=2) as ex:
         fs = [ex.submit(build_hip, verbose), ex.submit(build_scene, verbose)]
         for f in fs:
             f.result()
+    build_napi(verbose)
 
 
 if __name__ == "__main__":

@@ -85,7 +85,8 @@ static void release(px_ctx* c) {
   }
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
-                  c->d_super_mask, c->d_brick_mask, c->d_shadow_rays, c->d_counters};
+                  c->d_super_mask, c->d_brick_mask, c->d_shadow_rays, c->d_counters,
+                  c->b_grid, c->b_pool, c->b_super, c->b_ctrl, c->b_free};
   for (void* p : bufs)
     if (p) hipFree(p);
   px_edit_free(c);
@@ -181,6 +182,8 @@ int px_create(const px_world_desc* d, px_ctx** out) {
     float cam[PX_CAMERA_FLOATS] = {0};
     TRY(hipMemcpy(c->d_camera, cam, sizeof cam, hipMemcpyHostToDevice));
   }
+  // the uploads ran on the null stream; the context's stream does not wait for it
+  TRY(hipDeviceSynchronize());
 #undef TRY
   c->h_uniforms[0] = (float)c->dims[0];
   c->h_uniforms[1] = (float)c->dims[1];
@@ -376,8 +379,44 @@ int px_push_impacts(px_ctx* c, const float* impacts16, uint32_t n) {
   PX_HIP_CHECK(c, hipMemcpyAsync(c->d_impacts + (size_t)c->n_impacts * 16, impacts16, (size_t)n * 64,
                                  hipMemcpyHostToDevice, c->stream));
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  std::memcpy(c->h_impacts + (size_t)c->n_impacts * 16, impacts16, (size_t)n * 64);
   c->n_impacts += n;
   return PX_OK;
+}
+
+int px_snapshot(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (!c->b_grid) {
+    PX_HIP_CHECK(c, hipMalloc(&c->b_grid, (size_t)c->ncell * 4));
+    PX_HIP_CHECK(c, hipMalloc(&c->b_pool, (size_t)c->pool_cap * 1024));
+    PX_HIP_CHECK(c, hipMalloc(&c->b_super, (size_t)c->nsuper * 4));
+    PX_HIP_CHECK(c, hipMalloc(&c->b_ctrl, PX_CTRL_WORDS * 4));
+    PX_HIP_CHECK(c, hipMalloc(&c->b_free, (size_t)c->pool_cap * 4));
+  }
+  hipStream_t s = c->stream;
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->b_grid, c->d_grid, (size_t)c->ncell * 4, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->b_pool, c->d_pool, (size_t)c->pool_cap * 1024, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->b_super, c->d_super, (size_t)c->nsuper * 4, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->b_ctrl, c->d_ctrl, PX_CTRL_WORDS * 4, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->b_free, c->d_free, (size_t)c->pool_cap * 4, hipMemcpyDeviceToDevice, s));
+  return PX_OK;
+}
+
+int px_restore(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  if (!c->b_grid) return px_fail(c, PX_ERR_STATE, "px_restore", "no snapshot taken");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_grid, c->b_grid, (size_t)c->ncell * 4, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_pool, c->b_pool, (size_t)c->pool_cap * 1024, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_super, c->b_super, (size_t)c->nsuper * 4, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_ctrl, c->b_ctrl, PX_CTRL_WORDS * 4, hipMemcpyDeviceToDevice, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_free, c->b_free, (size_t)c->pool_cap * 4, hipMemcpyDeviceToDevice, s));
+  c->n_impacts = 0;
+  c->labels_valid = false;
+  c->masks_dirty = true;
+  return px_refresh_masks(c);
 }
 
 int px_profile_begin(px_ctx* c) {

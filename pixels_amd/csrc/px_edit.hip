@@ -1,20 +1,1059 @@
-// px_edit.hip - edit chain on the heap brick pool (placeholder: filled in by
-// the carve / flood / maint / island kernels).
+// px_edit.hip - the destruction edit chain on the heap brick pool, gfx950.
+//
+// One island pass = the reference's chain run to quiescence:
+//   impactJobs  sim.wgsl:414-443   cells in each impact's AABB -> jobs, uniform
+//                                  cells within reach() -> alloc requests
+//   allocBricks sim.wgsl:459-483   pop the free list, fill, publish (popFree :148-156)
+//   carve       sim.wgsl:529-579   removes() (:229-239) per voxel of every job brick
+//   pane flood  sim.wgsl:250-271,511-523,599-618
+//   maint       sim.wgsl:1219-1274 collapse all-air / all-same pool bricks
+//   islands     sim.wgsl:620-882   6-connected components of non-air,
+//                                  non-rubble, non-structural voxels; anchored by
+//                                  a voxel with y == 0 or a structural neighbour
+//
+// MI355X layout of the work:
+//  * impactJobs ORs impact k into a 64-bit mask per cell (MAX_IMPACTS = 64), so
+//    carve runs one wavefront per cell over all its impacts: each cell has a
+//    single writer and needs no CAS loop (the reference CASes words shared by
+//    several (impact, brick) workgroups).
+//  * The pane flood is a connected-component pass instead of a ring-by-ring
+//    BFS: over the voxels of pane materials (fragile 2) in pool bricks, same
+//    material id connected; every component holding a voxel that carve removed
+//    is removed whole.  That is the closure the reference's flood reaches
+//    (each carved pane voxel takes its same-id neighbours, each taken voxel
+//    takes its own), in a fixed number of launches.
+//  * Components are labelled in two levels: a union-find in LDS over the 512
+//    voxels of each brick (one row of 8 voxels per lane), then a lock-free
+//    union-find over brick-local components whose node id is the voxel key of
+//    the component's minimum voxel.  Links always point to the smaller key, so
+//    every root is the minimum voxel key of its component: the canonical label.
+//  * No counter is shared by the whole grid on the hot paths: the 8 XCDs have
+//    separate L2 caches, and device-wide atomics on one address serialise
+//    beyond them.  Node and pane-cell lists live in static per-block ranges,
+//    destroyed/bill/dirty counts are per-block partials summed by the last
+//    kernel, the free list is popped in one batch by request index, and maint
+//    walks the job and pane lists with test-and-clear of the dirty bits.
+//  * Every kernel is persistent over a device-side list or counter; the host
+//    never waits inside the pass.
+#include <algorithm>
+
 #include "px_internal.h"
 
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr uint32_t kLabNone = 0xFFFFu;
+constexpr uint32_t kFlagAnchor = 1u;
+constexpr uint32_t kFlagHit = 2u;
+constexpr uint32_t kEditBlocks = 8192;  // persistent grid: 32 single-wave blocks per CU (256 CUs)
+constexpr uint32_t kPartWords = 8;      // per-block partials
+constexpr uint32_t kCountWords = 4;     // per-block list lengths
+
+// edit-pass counters (ec[])
+enum : uint32_t {
+  EC_JOBCELLS = 0, EC_ALLOCS, EC_UNUSED2, EC_UNUSED3, EC_LABELS, EC_UNUSED5, EC_JOBS, EC_CHUNKS,
+  EC_UNUSED8, EC_OVERFLOW, EC_WORDS = 16
+};
+// per-block partials (part[b * kPartWords + k])
+enum : uint32_t { PT_CARVE_D = 0, PT_CARVE_B, PT_FLOOD_D, PT_FLOOD_B, PT_DIRTY, PT_FREED };
+// per-block list lengths (bcount[b * kCountWords + k])
+enum : uint32_t { BC_NODES = 0, BC_PANENODES, BC_PANECELLS };
+
+struct EditView {
+  uint32_t* grid;
+  uint32_t* pool;
+  uint32_t* sgrid;
+  const float* mats;
+  uint32_t* ctrl;
+  uint32_t* free_list;
+  uint32_t pool_cap;
+  int32_t dims[3], bd[3], sd[3];
+  uint32_t ncell;
+  uint64_t* imask;
+  uint32_t* alloc_flags;
+  uint32_t* dirty_flags;
+  uint32_t* job_list;
+  uint32_t* alloc_list;
+  uint32_t* pane_cells;  // kEditBlocks ranges of slots_cell
+  uint32_t* par;
+  uint32_t* cnt;
+  uint32_t* flag;
+  uint16_t* lab;
+  uint32_t* nodes;  // kEditBlocks ranges of slots_node
+  uint32_t slots_cell, slots_node;
+  uint32_t* part;
+  uint32_t* bcount;
+  uint32_t* out;
+  uint32_t out_cap;
+  uint32_t* ec;
+  const float* impacts;
+  uint32_t n_impacts;
+};
+
+// Union-find over voxel keys.  par[x] <= x always (links go to smaller keys),
+// so chains strictly decrease and every walk ends.  Plain loads are enough:
+// a stale parent is still an ancestor, and a stale root only makes the CAS in
+// uf_union fail and return the current parent.
+PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x) {
+  for (;;) {
+    uint32_t p = par[x];
+    if (p == x) return x;
+    uint32_t gp = par[p];
+    if (gp != p) par[x] = gp;  // path halving; gp < p < x
+    x = gp;
+  }
+}
+
+PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (a < b) {
+      uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t old = atomicCAS(par + a, a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// LDS union-find over the 512 voxels of one brick (links to smaller indices,
+// path halving; parents only decrease, so a racing halving store still
+// writes an ancestor)
+PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x) {
+  for (;;) {
+    uint32_t p = lab[x];
+    if (p == x) return x;
+    uint32_t gp = lab[p];
+    if (gp != p) lab[x] = gp;
+    x = gp;
+  }
+}
+
+PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = lds_find(lab, a);
+    b = lds_find(lab, b);
+    if (a == b) return;
+    if (a < b) {
+      uint32_t t = a;
+      a = b;
+      b = t;
+    }
+    uint32_t old = atomicCAS(lab + a, a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// inclusive prefix sum over the wavefront
+PX_DEV uint32_t wave_scan(uint32_t v) {
+  uint32_t lane = __lane_id();
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+PX_DEV uint32_t wave_sum(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+PX_DEV void cell_coord(const EditView& e, uint32_t ci, int32_t& bx, int32_t& by, int32_t& bz) {
+  bx = (int32_t)(ci % (uint32_t)e.bd[0]);
+  by = (int32_t)((ci / (uint32_t)e.bd[0]) % (uint32_t)e.bd[1]);
+  bz = (int32_t)(ci / ((uint32_t)e.bd[0] * (uint32_t)e.bd[1]));
+}
+
+PX_DEV uint32_t vkey(const EditView& e, int32_t x, int32_t y, int32_t z) {
+  return (uint32_t)x + (uint32_t)e.dims[0] * ((uint32_t)y + (uint32_t)e.dims[1] * (uint32_t)z);
+}
+
+// key offset of local index li inside a brick
+PX_DEV uint32_t loff(const EditView& e, uint32_t li) {
+  return (li & 7u) + (uint32_t)e.dims[0] * (((li >> 3) & 7u) + (uint32_t)e.dims[1] * (li >> 6));
+}
+
+PX_DEV uint32_t super_of(const EditView& e, uint32_t ci) {
+  int32_t bx, by, bz;
+  cell_coord(e, ci, bx, by, bz);
+  return (uint32_t)((bx >> 2) + e.sd[0] * ((by >> 2) + e.sd[1] * (bz >> 2)));
+}
+
+PX_DEV void set_dirty(const EditView& e, uint32_t ci) { atomicOr(&e.dirty_flags[ci >> 5], 1u << (ci & 31u)); }
+
+PX_DEV uint32_t price_dimes(const float* mats, uint32_t v) {
+  float p = mats[(v & 0xffu) * 20u + 15u];
+  return (uint32_t)px_f2i(p * 10.0f + 0.5f);
+}
+
+// reach() of sim.wgsl:242-248
+PX_DEV float reach(const float* im, const float* mats, uint32_t id) {
+  const float* E = mats + id * 20u + 16u;
+  float r = 0.0f;
+  if (E[1] > 0.5f) r = im[4] * 1.2f;
+  if (E[0] > 0.0f) r = fmaxf(r, fmaxf(im[3] * fminf(1.4f, E[0]), fminf(im[3], 2.5f)) * 1.2f);
+  return r;
+}
+
+// class tables: MODE 0 islands (1 = non-air, non-rubble, non-structural),
+// MODE 1 panes (material id of fragile-2 materials); str = structural, non-rubble
+template <int MODE>
+PX_DEV void load_tables(const float* mats, uint8_t* s_cls, uint8_t* s_str) {
+  for (uint32_t id = threadIdx.x; id < 256u; id += blockDim.x) {
+    const float* E = mats + id * 20u + 16u;
+    bool rubble = (id & 0x80u) != 0u;
+    bool structural = E[2] > 0.5f;
+    if (MODE == 0) s_cls[id] = (id != 0u && !rubble && !structural) ? 1u : 0u;
+    else s_cls[id] = (id != 0u && E[1] > 1.5f) ? (uint8_t)id : 0u;
+    s_str[id] = (id != 0u && !rubble && structural) ? 1u : 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_edit_reset(uint32_t* ec) {
+  if (threadIdx.x < EC_WORDS) ec[threadIdx.x] = 0u;
+}
+
+// impactJobs, sim.wgsl:414-443: one block row per impact; list appends are
+// aggregated per wavefront
+__global__ __launch_bounds__(256) void k_impact_jobs(EditView e) {
+  const uint32_t k = blockIdx.y;
+  const uint32_t lane = __lane_id();
+  const float* im = e.impacts + (size_t)k * 16u;
+  float R = ceilf(fmaxf(im[3], im[4]) * 1.2f) + 1.0f;
+  int32_t bmin[3], bmax[3];
+  for (int a = 0; a < 3; a++) {
+    bmin[a] = px_clampi(px_f2i(floorf((im[a] - R) / 8.0f)), 0, e.bd[a] - 1);
+    bmax[a] = px_clampi(px_f2i(floorf((im[a] + R) / 8.0f)), 0, e.bd[a] - 1);
+  }
+  uint32_t ex = (uint32_t)(bmax[0] - bmin[0] + 1), ey = (uint32_t)(bmax[1] - bmin[1] + 1);
+  uint32_t ez = (uint32_t)(bmax[2] - bmin[2] + 1);
+  uint32_t total = ex * ey * ez;
+  uint32_t jobs = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    uint32_t j = base + threadIdx.x;
+    bool job = false, alloc_new = false, cell_new = false;
+    uint32_t ci = 0;
+    if (j < total) {
+      uint32_t b0 = (uint32_t)bmin[0] + j % ex, b1 = (uint32_t)bmin[1] + (j / ex) % ey;
+      uint32_t b2 = (uint32_t)bmin[2] + j / (ex * ey);
+      ci = b0 + (uint32_t)e.bd[0] * (b1 + (uint32_t)e.bd[1] * b2);
+      uint32_t g = e.grid[ci];
+      if (g != 0u) {
+        float lo[3] = {(float)(b0 * 8u), (float)(b1 * 8u), (float)(b2 * 8u)};
+        float ev[3];
+        for (int a = 0; a < 3; a++) ev[a] = fmaxf(fmaxf(lo[a] - im[a], 0.0f), im[a] - (lo[a] + 8.0f));
+        float near2 = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
+        job = !(near2 > R * R);
+        if (job && (g & PX_UNIFORM_BIT) != 0u) {
+          float rch = reach(im, e.mats, g & 0xffu);
+          if (near2 > rch * rch) {
+            job = false;
+          } else {
+            uint32_t bit = 1u << (ci & 31u);
+            alloc_new = (atomicOr(&e.alloc_flags[ci >> 5], bit) & bit) == 0u;
+          }
+        }
+        if (job) cell_new = atomicOr((unsigned long long*)&e.imask[ci], 1ull << k) == 0ull;
+      }
+    }
+    jobs += job ? 1u : 0u;
+    uint64_t am = __ballot(alloc_new), cm = __ballot(cell_new);
+    uint64_t below = (1ull << lane) - 1ull;
+    uint32_t abase = 0, cbase = 0;
+    if (lane == 0 && am) abase = atomicAdd(&e.ec[EC_ALLOCS], (uint32_t)__popcll(am));
+    if (lane == 0 && cm) cbase = atomicAdd(&e.ec[EC_JOBCELLS], (uint32_t)__popcll(cm));
+    abase = __shfl(abase, 0);
+    cbase = __shfl(cbase, 0);
+    if (alloc_new) {
+      uint32_t q = abase + (uint32_t)__popcll(am & below);
+      if (q < e.ncell) e.alloc_list[q] = ci;
+    }
+    if (cell_new) {
+      uint32_t q = cbase + (uint32_t)__popcll(cm & below);
+      if (q < e.ncell) e.job_list[q] = ci;
+    }
+  }
+  jobs = wave_sum(jobs);
+  if (lane == 0 && jobs) atomicAdd(&e.ec[EC_JOBS], jobs);
+}
+
+// allocBricks, sim.wgsl:459-483, one wave per request.  Every request is a
+// uniform cell, so request i takes free_list[top - 1 - i] while i < top and
+// fails otherwise (popFree :148-156 pops one entry per request too); the top
+// moves in k_alloc_commit.
+__global__ __launch_bounds__(64) void k_alloc(EditView e) {
+  const uint32_t n = min(e.ec[EC_ALLOCS], e.ncell);
+  const uint32_t top = e.ctrl[PX_C_FREE_TOP];
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    uint32_t ci = e.alloc_list[i];
+    uint32_t g = e.grid[ci];
+    if (lane == 0) atomicAnd(&e.alloc_flags[ci >> 5], ~(1u << (ci & 31u)));
+    if (!(g == 0u || (g & PX_UNIFORM_BIT) != 0u)) continue;
+    if (i >= top || top > e.pool_cap) continue;
+    uint32_t p = e.free_list[top - 1u - i];
+    uint32_t fill = g != 0u ? (g & 0xffffu) : 0u;
+    fill |= fill << 16;
+    reinterpret_cast<uint4*>(e.pool + (size_t)p * 256u)[lane] = make_uint4(fill, fill, fill, fill);
+    __threadfence();
+    if (lane == 0) {
+      e.grid[ci] = p + 1u;
+      if (g == 0u) atomicAdd(&e.sgrid[super_of(e, ci)], 1u);
+      set_dirty(e, ci);
+    }
+  }
+}
+
+__global__ void k_alloc_commit(EditView e) {
+  if (threadIdx.x != 0) return;
+  uint32_t n = min(e.ec[EC_ALLOCS], e.ncell);
+  uint32_t top = e.ctrl[PX_C_FREE_TOP];
+  uint32_t used = top > e.pool_cap ? 0u : min(n, top);
+  e.ctrl[PX_C_FREE_TOP] = top - used;
+  e.ctrl[PX_S_ALLOC_FAIL] += n - used;
+}
+
+// Per-brick labelling.  MODE 0: islands (uniform bricks of a node class are
+// one node).  MODE 1: pane voxels (pool bricks only).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
+  // One wavefront per brick; lane l owns the row of 8 voxels (x = 0..7) at
+  // y = l & 7, z = l >> 3 (local indices 8l .. 8l+7).  Same-class runs along
+  // x get their label from bit arithmetic; unions happen only between runs of
+  // neighbouring rows (y+1: lane l+1, z+1: lane l+8) at the start of every
+  // overlap, so a full brick needs ~100 LDS unions instead of ~1300.
+  __shared__ uint32_t s_lab[512];
+  __shared__ uint32_t s_cnt[512];
+  __shared__ uint32_t s_flg[512];
+  __shared__ uint8_t s_cls[256];
+  __shared__ uint8_t s_str[256];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t ry = lane & 7u, rz = lane >> 3;
+  load_tables<MODE>(e.mats, s_cls, s_str);
+  __syncthreads();
+  uint32_t* my_nodes = e.nodes + (size_t)blockIdx.x * e.slots_node;
+  uint32_t* my_cells = e.pane_cells + (size_t)blockIdx.x * e.slots_cell;
+  uint32_t nn = 0u, nc = 0u;  // wave-uniform list lengths of this block
+  for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) {
+    uint32_t g = e.grid[ci];
+    if (g == 0u) continue;
+    int32_t bx, by, bz;
+    cell_coord(e, ci, bx, by, bz);
+    uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+    if ((g & PX_UNIFORM_BIT) != 0u) {
+      if (MODE == 1) continue;
+      if (!s_cls[g & 0xffu]) continue;
+      if (lane == 0) {
+        e.par[okey] = okey;
+        e.cnt[okey] = 512u;
+        e.flag[okey] = by == 0 ? kFlagAnchor : 0u;
+        if (nn < e.slots_node) my_nodes[nn] = okey;
+      }
+      nn++;
+      continue;
+    }
+    uint32_t p = g - 1u;
+    uint4 w = reinterpret_cast<const uint4*>(e.pool + (size_t)p * 256u)[lane];
+    uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+    uint32_t cl[8];
+    uint32_t m = 0u, sm = 0u;
+    for (int x = 0; x < 8; x++) {
+      uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+      cl[x] = v ? s_cls[v & 0xffu] : 0u;
+      if (cl[x]) m |= 1u << x;
+      if (v && s_str[v & 0xffu]) sm |= 1u << x;
+    }
+    if (__ballot(m != 0u) == 0ull) continue;  // no node voxel in this brick
+    uint32_t c_lo = cl[0] | (cl[1] << 8) | (cl[2] << 16) | (cl[3] << 24);
+    uint32_t c_hi = cl[4] | (cl[5] << 8) | (cl[6] << 16) | (cl[7] << 24);
+    uint32_t lx = 0u;  // x-links: voxel x continues the run of x-1
+    for (int x = 1; x < 8; x++)
+      if (cl[x] && cl[x] == cl[x - 1]) lx |= 1u << x;
+    uint32_t rs = m & ~lx;  // run starts
+    uint32_t ylo = __shfl_down(c_lo, 1), yhi = __shfl_down(c_hi, 1), ylx = __shfl_down(lx, 1);
+    uint32_t zlo = __shfl_down(c_lo, 8), zhi = __shfl_down(c_hi, 8), zlx = __shfl_down(lx, 8);
+    uint32_t sm_yp = __shfl_down(sm, 1), sm_zp = __shfl_down(sm, 8);
+    uint32_t sm_ym = __shfl_up(sm, 1), sm_zm = __shfl_up(sm, 8);
+    bool has_y = ry < 7u, has_z = rz < 7u;
+    uint32_t ey = 0u, ez = 0u;
+    for (int x = 0; x < 8; x++) {
+      uint32_t cy = ((x < 4 ? ylo : yhi) >> ((x & 3) * 8)) & 0xffu;
+      uint32_t cz = ((x < 4 ? zlo : zhi) >> ((x & 3) * 8)) & 0xffu;
+      if (has_y && cl[x] && cl[x] == cy) ey |= 1u << x;
+      if (has_z && cl[x] && cl[x] == cz) ez |= 1u << x;
+    }
+    for (int x = 0; x < 8; x++) {
+      uint32_t li = lane * 8u + (uint32_t)x;
+      uint32_t start = 31u - (uint32_t)__clz(rs & ((2u << x) - 1u));
+      s_lab[li] = ((m >> x) & 1u) ? lane * 8u + start : PX_NONE;
+      s_cnt[li] = 0u;
+      s_flg[li] = 0u;
+    }
+    __syncthreads();
+    for (int x = 0; x < 8; x++) {
+      uint32_t b = 1u << x, pb = b >> 1;
+      if ((ey & b) && !(x > 0 && (ey & pb) && (lx & b) && (ylx & b)))
+        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x);
+      if ((ez & b) && !(x > 0 && (ez & pb) && (lx & b) && (zlx & b)))
+        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x);
+    }
+    __syncthreads();
+    // anchors inside the brick (islands): y == 0 or a structural neighbour
+    uint32_t abits = 0u;
+    if (MODE == 0) {
+      uint32_t adj = ((sm << 1) | (sm >> 1)) & 0xffu;
+      if (has_y) adj |= sm_yp;
+      if (ry > 0u) adj |= sm_ym;
+      if (has_z) adj |= sm_zp;
+      if (rz > 0u) adj |= sm_zm;
+      if (by == 0 && ry == 0u) adj = 0xffu;
+      abits = m & adj;
+    }
+    uint32_t rootx[8];
+    uint32_t nroots = 0u;
+    for (int x = 0; x < 8; x++) rootx[x] = kLabNone;
+    for (int x = 0; x < 8; x++) {
+      if (!((rs >> x) & 1u)) continue;
+      uint32_t run = 1u << x;
+      for (int y = x + 1; y < 8 && ((lx >> y) & 1u); y++) run |= 1u << y;
+      uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x);
+      atomicAdd(&s_cnt[r], (uint32_t)__popc(run));
+      if (run & abits) atomicOr(&s_flg[r], kFlagAnchor);
+      for (int y = x; y < 8; y++)
+        if ((run >> y) & 1u) rootx[y] = r;
+      if (r == lane * 8u + (uint32_t)x) nroots++;
+    }
+    __syncthreads();
+    // roots become union-find nodes in this block's range
+    uint32_t incl = wave_scan(nroots);
+    uint32_t total = __shfl(incl, 63);
+    uint32_t slot = nn + incl - nroots;
+    for (int x = 0; x < 8; x++) {
+      if (!((rs >> x) & 1u)) continue;
+      uint32_t li = lane * 8u + (uint32_t)x;
+      if (rootx[x] != li) continue;
+      uint32_t node = okey + loff(e, li);
+      e.par[node] = node;
+      e.cnt[node] = s_cnt[li];
+      e.flag[node] = s_flg[li];
+      if (slot < e.slots_node) my_nodes[slot] = node;
+      else atomicOr(&e.ec[EC_OVERFLOW], 1u);
+      slot++;
+    }
+    nn += total;
+    uint4 lw;
+    lw.x = rootx[0] | (rootx[1] << 16);
+    lw.y = rootx[2] | (rootx[3] << 16);
+    lw.z = rootx[4] | (rootx[5] << 16);
+    lw.w = rootx[6] | (rootx[7] << 16);
+    reinterpret_cast<uint4*>(e.lab + (size_t)p * 512u)[lane] = lw;
+    if (MODE == 1 && total) {
+      if (lane == 0u) my_cells[nc] = ci;
+      nc++;
+    }
+    __syncthreads();
+  }
+  if (lane == 0u) {
+    e.bcount[blockIdx.x * kCountWords + (MODE == 0 ? BC_NODES : BC_PANENODES)] = min(nn, e.slots_node);
+    if (MODE == 1) e.bcount[blockIdx.x * kCountWords + BC_PANECELLS] = nc;
+  }
+}
+
+PX_DEV uint32_t cell_voxel(const EditView& e, uint32_t g, uint32_t li) {
+  if (g == 0u) return 0u;
+  if ((g & PX_UNIFORM_BIT) != 0u) return g & 0xffffu;
+  uint32_t idx = (g - 1u) * 512u + li;
+  return (e.pool[idx >> 1u] >> ((idx & 1u) * 16u)) & 0xffffu;
+}
+
+// Node of a voxel known to belong to a component; PX_NONE (and the internal
+// error bit in ec[EC_OVERFLOW]) if its brick label is not a local index.
+PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t li) {
+  if ((g & PX_UNIFORM_BIT) != 0u) return okey;
+  uint32_t r = e.lab[(size_t)(g - 1u) * 512u + li];
+  if (r >= 512u) {
+    atomicOr(&e.ec[EC_OVERFLOW], 2u);
+    return PX_NONE;
+  }
+  return okey + loff(e, r);
+}
+
+// Face merges between a cell and its +x / +y / +z neighbours (64 voxel pairs
+// per face, one per lane), plus anchors across faces (islands).
+template <int MODE>
+PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, const uint8_t* s_str) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t u = lane & 7u, v = lane >> 3;
+  uint32_t g = e.grid[ci];
+  if (g == 0u) return;
+  if (MODE == 1 && (g & PX_UNIFORM_BIT) != 0u) return;
+  int32_t bc[3];
+  cell_coord(e, ci, bc[0], bc[1], bc[2]);
+  uint32_t okey = vkey(e, bc[0] * 8, bc[1] * 8, bc[2] * 8);
+  for (int d = 0; d < 3; d++) {
+    if (bc[d] + 1 >= e.bd[d]) continue;
+    int32_t nc[3] = {bc[0], bc[1], bc[2]};
+    nc[d] += 1;
+    uint32_t cj = (uint32_t)(nc[0] + e.bd[0] * (nc[1] + e.bd[1] * nc[2]));
+    uint32_t gn = e.grid[cj];
+    if (gn == 0u) continue;
+    if (MODE == 1 && (gn & PX_UNIFORM_BIT) != 0u) continue;
+    uint32_t la, lb;
+    if (d == 0) { la = 7u | (u << 3) | (v << 6); lb = 0u | (u << 3) | (v << 6); }
+    else if (d == 1) { la = u | (7u << 3) | (v << 6); lb = u | (0u << 3) | (v << 6); }
+    else { la = u | (v << 3) | (7u << 6); lb = u | (v << 3) | (0u << 6); }
+    uint32_t va = cell_voxel(e, g, la), vb = cell_voxel(e, gn, lb);
+    uint32_t ca = va ? s_cls[va & 0xffu] : 0u, cb = vb ? s_cls[vb & 0xffu] : 0u;
+    uint32_t okn = vkey(e, nc[0] * 8, nc[1] * 8, nc[2] * 8);
+    uint32_t na = ca ? node_of(e, okey, g, la) : PX_NONE;
+    uint32_t nb = cb ? node_of(e, okn, gn, lb) : PX_NONE;
+    if (na == PX_NONE) ca = 0u;
+    if (nb == PX_NONE) cb = 0u;
+    bool join = ca != 0u && ca == cb;
+    // skip a pair that the lane to the left or above on the face also has:
+    // in raster order the first lane of every such region does the work
+    uint32_t jb = join ? 1u : 0u;
+    uint32_t pa = __shfl_up(na, 1), pb = __shfl_up(nb, 1), pj = __shfl_up(jb, 1);
+    uint32_t qa = __shfl_up(na, 8), qb = __shfl_up(nb, 8), qj = __shfl_up(jb, 8);
+    bool dup = (u > 0u && pj && pa == na && pb == nb) || (v > 0u && qj && qa == na && qb == nb);
+    if (join && !dup) uf_union(e.par, na, nb);
+    if (MODE == 0) {
+      uint32_t anc_a = (ca && vb && s_str[vb & 0xffu]) ? 1u : 0u;
+      uint32_t anc_b = (cb && va && s_str[va & 0xffu]) ? 1u : 0u;
+      uint32_t pna = __shfl_up(anc_a, 1), qna = __shfl_up(anc_a, 8);
+      uint32_t pnb = __shfl_up(anc_b, 1), qnb = __shfl_up(anc_b, 8);
+      if (anc_a && !((u > 0u && pna && pa == na) || (v > 0u && qna && qa == na))) atomicOr(&e.flag[na], kFlagAnchor);
+      if (anc_b && !((u > 0u && pnb && pb == nb) || (v > 0u && qnb && qb == nb))) atomicOr(&e.flag[nb], kFlagAnchor);
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_face_merge(EditView e) {
+  __shared__ uint8_t s_cls[256];
+  __shared__ uint8_t s_str[256];
+  load_tables<MODE>(e.mats, s_cls, s_str);
+  __syncthreads();
+  if (MODE == 0) {
+    for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) merge_cell<0>(e, ci, s_cls, s_str);
+  } else {
+    for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+      uint32_t n = e.bcount[b * kCountWords + BC_PANECELLS];
+      const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
+      for (uint32_t i = 0; i < n; i++) merge_cell<1>(e, cells[i], s_cls, s_str);
+    }
+  }
+}
+
+// After the unions: par[x] = root(x) for every node of block ranges `which`.
+// The walk only reads (no path halving), so each par[x] is written by x's
+// own thread alone and ends as the root.
+__global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    uint32_t n = e.bcount[b * kCountWords + which];
+    const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      uint32_t x = nodes[i];
+      uint32_t r = x;
+      for (;;) {
+        uint32_t p = __hip_atomic_load(e.par + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == r) break;
+        r = p;
+      }
+      __hip_atomic_store(e.par + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// carve, sim.wgsl:529-579, one wave per job cell over every impact that has a
+// job there.  Impacts are the outer (wave-uniform) loop; a voxel is tested
+// exactly (removes(), :229-239) only if its distance passes a conservative
+// bound for that impact: removes() needs d2 < (fr*jit)^2 or d2 < (rr*jit)^2
+// with jit <= 1.2 and rr <= max(1.4 r, 2.5), so d2 >= (1.2 max(1.4 r, 2.5, fr)
+// (1 + 1e-5))^2 can never remove.  The hash behind jit is computed once per
+// voxel, and only for voxels that pass some bound.  Removed pane voxels mark
+// their pane component for the flood.
+__global__ __launch_bounds__(64) void k_carve(EditView e) {
+  __shared__ float s_im[64 * 16];
+  __shared__ float s_bound2[64];
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t i = lane; i < e.n_impacts * 16u; i += 64u) s_im[i] = e.impacts[i];
+  __syncthreads();
+  if (lane < e.n_impacts) {
+    const float* im = s_im + lane * 16u;
+    float b = fmaxf(fmaxf(im[3] * 1.4f, 2.5f), im[4]) * 1.2f * 1.00001f;
+    s_bound2[lane] = b * b;
+  }
+  __syncthreads();
+  uint32_t acc_d = 0u, acc_b = 0u;
+  const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    uint32_t ci = e.job_list[i];
+    uint64_t mask = e.imask[ci];
+    uint32_t g = e.grid[ci];
+    __syncthreads();
+    if (lane == 0) e.imask[ci] = 0ull;
+    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
+    int32_t bx, by, bz;
+    cell_coord(e, ci, bx, by, bz);
+    uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+    uint32_t p = g - 1u;
+    uint4* wp = reinterpret_cast<uint4*>(e.pool + (size_t)p * 256u) + lane;
+    uint4 w4 = *wp;
+    uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+    // lane l owns the row y = l & 7, z = l >> 3 of the brick (x = 0..7)
+    const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
+    const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
+    // per-voxel terms of removes() that do not depend on the impact, loaded
+    // before the impact loop so that loop is arithmetic only
+    uint32_t alive = 0u, frag = 0u, soft = 0u;
+    float jit[8], m14[8];
+    for (int x = 0; x < 8; x++) {
+      uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+      jit[x] = 0.0f;
+      m14[x] = 0.0f;
+      if (!v) continue;
+      alive |= 1u << x;
+      const float* E = e.mats + (v & 0xffu) * 20u + 16u;
+      float sx = E[0], fy0 = E[1];
+      if (fy0 > 0.5f) frag |= 1u << x;
+      if (sx > 0.0f) soft |= 1u << x;
+      m14[x] = fminf(1.4f, sx);
+      jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
+    }
+    uint32_t rem = 0u;
+    uint64_t m = mask;
+    while (m) {
+      uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+      m &= m - 1ull;
+      const float* im = s_im + k * 16u;
+      const float b2 = s_bound2[k];
+      const float r = im[3], fr = im[4];
+      float dy = fy - im[1], dz = fz - im[2];
+      for (int x = 0; x < 8; x++) {
+        uint32_t bit = 1u << x;
+        if (!(alive & bit) || (rem & bit)) continue;
+        float dx = ((float)(bx * 8 + x) + 0.5f) - im[0];
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < b2)) continue;
+        float j = jit[x];
+        bool hit = false;
+        if ((frag & bit) && d2 < fr * fr * j * j) hit = true;
+        else if (soft & bit) {
+          float rr = fmaxf(r * m14[x], fminf(r, 2.5f)) * j;
+          hit = d2 < rr * rr;
+        }
+        if (hit) rem |= bit;
+      }
+    }
+    uint32_t destroyed = 0, bill = 0;
+    for (int x = 0; x < 8; x++) {
+      if (!((rem >> x) & 1u)) continue;
+      uint32_t sh = (uint32_t)(x & 1) * 16u;
+      uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
+      ww[x >> 1] &= ~(0xffffu << sh);
+      destroyed++;
+      bill += price_dimes(e.mats, v);
+      if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
+        uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
+        if (r < 512u) atomicOr(&e.flag[e.par[okey + loff(e, r)]], kFlagHit);
+        else atomicOr(&e.ec[EC_OVERFLOW], 2u);
+      }
+    }
+    if (rem) *wp = make_uint4(ww[0], ww[1], ww[2], ww[3]);
+    destroyed = wave_sum(destroyed);
+    bill = wave_sum(bill);
+    acc_d += destroyed;
+    acc_b += bill;
+    if (lane == 0 && destroyed) set_dirty(e, ci);
+  }
+  if (lane == 0) {
+    e.part[blockIdx.x * kPartWords + PT_CARVE_D] = acc_d;
+    e.part[blockIdx.x * kPartWords + PT_CARVE_B] = acc_b;
+  }
+}
+
+// The pane flood's closure: every pane voxel whose component was hit by carve.
+__global__ __launch_bounds__(64) void k_flood_remove(EditView e) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t acc_d = 0u, acc_b = 0u;
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    uint32_t nc = e.bcount[b * kCountWords + BC_PANECELLS];
+    const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
+    for (uint32_t i = 0; i < nc; i++) {
+      uint32_t ci = cells[i];
+      uint32_t g = e.grid[ci];
+      if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
+      int32_t bx, by, bz;
+      cell_coord(e, ci, bx, by, bz);
+      uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+      uint32_t p = g - 1u;
+      uint4* wp = reinterpret_cast<uint4*>(e.pool + (size_t)p * 256u) + lane;
+      uint4 w4 = *wp;
+      uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
+      uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+      uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
+      uint32_t destroyed = 0, bill = 0;
+      bool changed = false;
+      for (int j = 0; j < 8; j++) {
+        uint32_t sh = (uint32_t)(j & 1) * 16u;
+        uint32_t v = (ww[j >> 1] >> sh) & 0xffffu;
+        uint32_t r = (ll[j >> 1] >> sh) & 0xffffu;
+        if (v == 0u || r == kLabNone) continue;
+        if (r >= 512u) {
+          atomicOr(&e.ec[EC_OVERFLOW], 2u);
+          continue;
+        }
+        if ((e.flag[e.par[okey + loff(e, r)]] & kFlagHit) == 0u) continue;
+        ww[j >> 1] &= ~(0xffffu << sh);
+        changed = true;
+        destroyed++;
+        bill += price_dimes(e.mats, v);
+      }
+      if (changed) *wp = make_uint4(ww[0], ww[1], ww[2], ww[3]);
+      destroyed = wave_sum(destroyed);
+      bill = wave_sum(bill);
+      acc_d += destroyed;
+      acc_b += bill;
+      if (lane == 0 && destroyed) set_dirty(e, ci);
+    }
+  }
+  if (lane == 0) {
+    e.part[blockIdx.x * kPartWords + PT_FLOOD_D] = acc_d;
+    e.part[blockIdx.x * kPartWords + PT_FLOOD_B] = acc_b;
+  }
+}
+
+// maint of one cell (sim.wgsl:1219-1274; the density output is shading only)
+// if its dirty bit was set; returns 1 if it was dirty.
+PX_DEV uint32_t maint_cell(const EditView& e, uint32_t ci) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t bit = 1u << (ci & 31u);
+  uint32_t was = 0u;
+  if (lane == 0) was = (atomicAnd(&e.dirty_flags[ci >> 5], ~bit) & bit) ? 1u : 0u;
+  was = __shfl(was, 0);
+  if (!was) return 0u;
+  uint32_t g = e.grid[ci];
+  if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) return 1u;
+  uint32_t p = g - 1u;
+  uint4 w4 = reinterpret_cast<const uint4*>(e.pool + (size_t)p * 256u)[lane];
+  uint32_t first = __shfl(w4.x & 0xffffu, 0);
+  uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+  bool nz = false, diff = false;
+  for (int j = 0; j < 4; j++) {
+    uint32_t lo = ww[j] & 0xffffu, hi = ww[j] >> 16;
+    nz |= (lo | hi) != 0u;
+    diff |= lo != first || hi != first;
+  }
+  bool anynz = __ballot(nz) != 0ull, anydiff = __ballot(diff) != 0ull;
+  if (lane == 0) {
+    if (!anynz) {
+      e.grid[ci] = 0u;
+      uint32_t k = atomicAdd(&e.ctrl[PX_C_FREE_TOP], 1u);
+      e.free_list[k] = p;
+      atomicSub(&e.sgrid[super_of(e, ci)], 1u);
+    } else if (!anydiff) {
+      e.grid[ci] = PX_UNIFORM_BIT | first;
+      uint32_t k = atomicAdd(&e.ctrl[PX_C_FREE_TOP], 1u);
+      e.free_list[k] = p;
+    }
+  }
+  return 1u;
+}
+
+// Every dirty cell is a job cell (alloc, carve) or a pane cell (flood).
+__global__ __launch_bounds__(64) void k_maint(EditView e) {
+  uint32_t dirty = 0u;
+  const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) dirty += maint_cell(e, e.job_list[i]);
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    uint32_t nc = e.bcount[b * kCountWords + BC_PANECELLS];
+    const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
+    for (uint32_t i = 0; i < nc; i++) dirty += maint_cell(e, cells[i]);
+  }
+  if (threadIdx.x == 0) e.part[blockIdx.x * kPartWords + PT_DIRTY] = dirty;
+}
+
+// Per-root totals.  Lanes of a wavefront that share a root are reduced first
+// (one loop turn per distinct root), so a component spanning many bricks
+// takes one atomic per wavefront instead of one per brick-local node.
+__global__ __launch_bounds__(64) void k_aggregate(EditView e) {
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    uint32_t n = e.bcount[b * kCountWords + BC_NODES];
+    const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
+    for (uint32_t base = 0; base < n; base += 64u) {
+      uint32_t i = base + lane;
+      uint32_t x = i < n ? nodes[i] : 0u;
+      uint32_t r = i < n ? e.par[x] : 0u;
+      bool pending = i < n && r != x;
+      uint32_t c = pending ? e.cnt[x] : 0u;
+      uint32_t f = pending ? e.flag[x] : 0u;
+      uint64_t act = __ballot(pending);
+      while (act) {
+        uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+        uint32_t r0 = __shfl(r, (int)leader);
+        bool mine = pending && r == r0;
+        uint64_t peers = __ballot(mine);
+        uint32_t cs = wave_sum(mine ? c : 0u);
+        uint32_t fs = mine ? f : 0u;
+        for (int o = 32; o > 0; o >>= 1) fs |= __shfl_xor(fs, o);
+        if (lane == leader) {
+          atomicAdd(&e.cnt[r0], cs);
+          if (fs) atomicOr(&e.flag[r0], fs);
+        }
+        if (mine) pending = false;
+        act &= ~peers;
+      }
+    }
+  }
+}
+
+// Unsupported roots -> (label, count), appended with one atomic per wavefront.
+__global__ __launch_bounds__(64) void k_emit(EditView e) {
+  const uint32_t lane = threadIdx.x;
+  uint32_t chunks = 0u;
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    uint32_t n = e.bcount[b * kCountWords + BC_NODES];
+    const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
+    for (uint32_t base = 0; base < n; base += 64u) {
+      uint32_t i = base + lane;
+      uint32_t x = i < n ? nodes[i] : 0u;
+      bool lab = i < n && e.par[x] == x && (e.flag[x] & kFlagAnchor) == 0u;
+      uint32_t c = lab ? e.cnt[x] : 0u;
+      uint64_t m = __ballot(lab);
+      if (!m) continue;
+      uint32_t q0 = 0u;
+      if (lane == 0) q0 = atomicAdd(&e.ec[EC_LABELS], (uint32_t)__popcll(m));
+      q0 = __shfl(q0, 0);
+      if (lab) {
+        uint32_t q = q0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (q < e.out_cap) {
+          e.out[2u * q] = x;
+          e.out[2u * q + 1u] = c;
+        }
+        if (c >= 48u) chunks++;
+      }
+    }
+  }
+  chunks = wave_sum(chunks);
+  if (lane == 0 && chunks) atomicAdd(&e.ec[EC_CHUNKS], chunks);
+}
+
+// counters of the pass into the ctrl words the reference exposes
+__global__ __launch_bounds__(1024) void k_edit_finish(EditView e, uint32_t carved) {
+  __shared__ uint32_t s[6][1024];
+  uint32_t t = threadIdx.x;
+  uint32_t a[6] = {0, 0, 0, 0, 0, 0};
+  for (uint32_t b = t; b < kEditBlocks; b += blockDim.x) {
+    if (carved) {
+      a[0] += e.part[b * kPartWords + PT_CARVE_D] + e.part[b * kPartWords + PT_FLOOD_D];
+      a[1] += e.part[b * kPartWords + PT_CARVE_B] + e.part[b * kPartWords + PT_FLOOD_B];
+      a[2] += e.part[b * kPartWords + PT_DIRTY];
+    }
+    a[3] += e.bcount[b * kCountWords + BC_NODES];
+  }
+  for (int k = 0; k < 4; k++) s[k][t] = a[k];
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (t < o)
+      for (int k = 0; k < 4; k++) s[k][t] += s[k][t + o];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  e.ctrl[PX_S_DESTROYED] += s[0][0];
+  e.ctrl[PX_S_BILL] += s[1][0];
+  e.ctrl[PX_C_DIRTY] = s[2][0];
+  e.ctrl[PX_C_NODES] = s[3][0];
+  e.ctrl[PX_C_JOBS] = e.ec[EC_JOBS];
+  e.ctrl[PX_C_ALLOC_N] = e.ec[EC_ALLOCS];
+  e.ctrl[PX_C_ALLOCS] = 0u;
+  e.ctrl[PX_C_IMPACTS] = 0u;
+  e.ctrl[PX_S_ISLANDS] += e.ec[EC_CHUNKS];
+  e.ctrl[PX_S_SEARCHES] += 1u;
+}
+
+__global__ void k_zero64(uint64_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+__global__ void k_zero32(uint32_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+EditView make_view(px_ctx* c) {
+  EditView e;
+  e.grid = c->d_grid;
+  e.pool = c->d_pool;
+  e.sgrid = c->d_super;
+  e.mats = c->d_mats;
+  e.ctrl = c->d_ctrl;
+  e.free_list = c->d_free;
+  e.pool_cap = c->pool_cap;
+  for (int a = 0; a < 3; a++) {
+    e.dims[a] = (int32_t)c->dims[a];
+    e.bd[a] = (int32_t)c->bd[a];
+    e.sd[a] = (int32_t)c->sd[a];
+  }
+  e.ncell = c->ncell;
+  e.imask = c->d_cell_imask;
+  e.alloc_flags = c->d_alloc_flags;
+  e.dirty_flags = c->d_dirty_flags;
+  e.job_list = c->d_job_list;
+  e.alloc_list = c->d_alloc_list;
+  e.pane_cells = c->d_pane_cells;
+  e.par = c->d_par;
+  e.cnt = c->d_ncnt;
+  e.flag = c->d_nflag;
+  e.lab = c->d_lab;
+  e.nodes = c->d_nodes;
+  e.slots_cell = (c->ncell + kEditBlocks - 1u) / kEditBlocks;
+  e.slots_node = e.slots_cell * 256u;
+  e.part = c->d_part;
+  e.bcount = c->d_bcount;
+  e.out = c->d_out_labels;
+  e.out_cap = c->label_cap;
+  e.ec = c->d_edit_ctr;
+  e.impacts = c->d_impacts;
+  e.n_impacts = c->n_impacts;
+  return e;
+}
+
+}  // namespace
+
 int px_edit_alloc(px_ctx* c) {
+  size_t words = (c->ncell + 31u) / 32u;
+  size_t slots_cell = (c->ncell + kEditBlocks - 1u) / kEditBlocks;
   PX_HIP_CHECK(c, hipMalloc(&c->d_impacts, 64 * 16 * sizeof(float)));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_cell_imask, (size_t)c->ncell * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_alloc_flags, words * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_dirty_flags, words * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_job_list, (size_t)c->ncell * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_alloc_list, (size_t)c->ncell * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_pane_cells, slots_cell * kEditBlocks * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_part, (size_t)kEditBlocks * kPartWords * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_bcount, (size_t)kEditBlocks * kCountWords * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_edit_ctr, 64 * 4));
+  hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, c->stream, c->d_cell_imask, (size_t)c->ncell);
+  hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, c->stream, c->d_alloc_flags, words);
+  hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, c->stream, c->d_dirty_flags, words);
+  hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, c->d_edit_ctr, (size_t)64);
+  hipLaunchKernelGGL(k_zero32, dim3(64), dim3(256), 0, c->stream, c->d_bcount, (size_t)kEditBlocks * kCountWords);
+  PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
 }
 
 void px_edit_free(px_ctx* c) {
-  if (c->d_impacts) hipFree(c->d_impacts);
+  void* bufs[] = {c->d_impacts, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags, c->d_job_list, c->d_alloc_list,
+                  c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par, c->d_ncnt, c->d_nflag, c->d_lab,
+                  c->d_nodes, c->d_out_labels};
+  for (void* p : bufs)
+    if (p) hipFree(p);
   c->d_impacts = nullptr;
+  c->d_cell_imask = nullptr;
+  c->d_par = c->d_ncnt = c->d_nflag = nullptr;
+  c->d_lab = nullptr;
+  c->d_nodes = c->d_out_labels = nullptr;
 }
 
-int px_launch_island_pass(px_ctx* c) { return px_fail(c, PX_ERR_STATE, "px_island_pass", "not available yet"); }
+// the labelling arrays are sized by the voxel count and allocated on first use
+static int ensure_ccl(px_ctx* c) {
+  if (c->d_par) return PX_OK;
+  size_t nv = c->vox_total;
+  size_t slots_node = (size_t)((c->ncell + kEditBlocks - 1u) / kEditBlocks) * 256u;
+  size_t node_cap = slots_node * kEditBlocks;
+  size_t label_cap = std::min<size_t>(node_cap, nv);
+  PX_HIP_CHECK(c, hipMalloc(&c->d_par, nv * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_ncnt, nv * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_nflag, nv * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_lab, (size_t)c->pool_cap * 1024));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_nodes, node_cap * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_out_labels, label_cap * 8));
+  c->node_cap = (uint32_t)std::min<size_t>(node_cap, 0xFFFFFFF0u);
+  c->label_cap = (uint32_t)std::min<size_t>(label_cap, 0xFFFFFFF0u);
+  return PX_OK;
+}
 
-int px_read_island_labels(px_ctx* c, uint32_t*, uint32_t*, uint32_t, uint32_t* n) {
+int px_launch_island_pass(px_ctx* c) {
+  int rc = ensure_ccl(c);
+  if (rc) return rc;
+  PxProfScope prof(c, PX_PROF_EDIT);
+  EditView e = make_view(c);
+  hipStream_t s = c->stream;
+  const dim3 G(kEditBlocks), W(64);
+  hipLaunchKernelGGL(k_edit_reset, dim3(1), dim3(64), 0, s, c->d_edit_ctr);
+  const uint32_t carved = c->n_impacts ? 1u : 0u;
+  if (c->n_impacts) {
+    // blocks per impact row from the largest impact AABB (host copy of the impacts)
+    uint32_t maxcells = 1;
+    for (uint32_t k = 0; k < c->n_impacts; k++) {
+      const float* im = &c->h_impacts[k * 16];
+      float R = std::ceil(std::max(im[3], im[4]) * 1.2f) + 1.0f;
+      uint32_t cells = 1;
+      for (int a = 0; a < 3; a++) {
+        int32_t lo = (int32_t)std::floor((im[a] - R) / 8.0f), hi = (int32_t)std::floor((im[a] + R) / 8.0f);
+        lo = std::min(std::max(lo, 0), (int32_t)c->bd[a] - 1);
+        hi = std::min(std::max(hi, 0), (int32_t)c->bd[a] - 1);
+        cells *= (uint32_t)(hi - lo + 1);
+      }
+      maxcells = std::max(maxcells, cells);
+    }
+    uint32_t bx = std::min<uint32_t>((maxcells + 255u) / 256u, 256u);
+    hipLaunchKernelGGL(k_impact_jobs, dim3(bx, c->n_impacts), dim3(256), 0, s, e);
+    hipLaunchKernelGGL(k_alloc, G, W, 0, s, e);
+    hipLaunchKernelGGL(k_alloc_commit, dim3(1), dim3(64), 0, s, e);
+    // pane components on the world as it is after alloc and before carve
+    hipLaunchKernelGGL(k_local_ccl<1>, G, W, 0, s, e);
+    hipLaunchKernelGGL(k_face_merge<1>, G, W, 0, s, e);
+    hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_PANENODES);
+    hipLaunchKernelGGL(k_carve, G, W, 0, s, e);
+    hipLaunchKernelGGL(k_flood_remove, G, W, 0, s, e);
+    hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
+    c->masks_dirty = true;
+  }
+  hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
+  hipLaunchKernelGGL(k_aggregate, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_emit, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_edit_finish, dim3(1), dim3(1024), 0, s, e, carved);
+  PX_HIP_CHECK(c, hipGetLastError());
+  c->n_impacts = 0;
+  c->labels_valid = true;
+  return px_refresh_masks(c);
+}
+
+int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n) {
   *n = 0;
-  return px_fail(c, PX_ERR_STATE, "px_island_labels", "not available yet");
+  if (!c->labels_valid) return px_fail(c, PX_ERR_STATE, "px_island_labels", "no island pass has run");
+  uint32_t ec[EC_WORDS];
+  PX_HIP_CHECK(c, hipMemcpyAsync(ec, c->d_edit_ctr, sizeof ec, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  if (ec[EC_OVERFLOW] & 2u)
+    return px_fail(c, PX_ERR_STATE, "px_island_labels", "internal error: a brick label was not a local index");
+  if (ec[EC_OVERFLOW] || ec[EC_LABELS] > c->label_cap)
+    return px_fail(c, PX_ERR_CAPACITY, "px_island_labels", "component node list overflowed");
+  uint32_t m = ec[EC_LABELS];
+  std::vector<uint32_t> pairs((size_t)m * 2);
+  if (m) {
+    PX_HIP_CHECK(c, hipMemcpyAsync(pairs.data(), c->d_out_labels, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
+    PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
+  std::vector<uint64_t> packed(m);
+  for (uint32_t i = 0; i < m; i++) packed[i] = ((uint64_t)pairs[2 * i] << 32) | pairs[2 * i + 1];
+  std::sort(packed.begin(), packed.end());
+  for (uint32_t i = 0; i < m && i < cap; i++) {
+    min_key[i] = (uint32_t)(packed[i] >> 32);
+    count[i] = (uint32_t)packed[i];
+  }
+  *n = m;
+  return PX_OK;
 }

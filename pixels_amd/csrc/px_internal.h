@@ -78,9 +78,11 @@ struct px_ctx {
   uint32_t* d_dirty_flags = nullptr; // per cell bitset: dirty
   uint32_t* d_job_list = nullptr;    // cells with at least one job
   uint32_t* d_alloc_list = nullptr;
-  uint32_t* d_dirty_list = nullptr;
-  uint32_t* d_trig = nullptr;        // carved pane voxels (key, id)
-  uint32_t trig_cap = 0;
+  uint32_t* d_part = nullptr;        // per-block partial sums of the edit pass
+  uint32_t* d_bcount = nullptr;      // per-block list lengths of the edit pass
+  uint32_t* d_pane_cells = nullptr;  // pool cells holding pane voxels
+  float h_impacts[64 * 16] = {0};     // host copy of the queued impacts
+  bool labels_valid = false;
   uint32_t* d_edit_ctr = nullptr;    // edit-pass counters
   // component labelling
   uint32_t* d_par = nullptr;     // union-find parent per voxel key (node ids are keys)
@@ -92,6 +94,12 @@ struct px_ctx {
   uint32_t* d_out_labels = nullptr;  // (key, count) pairs
   uint32_t label_cap = 0;
   size_t vox_total = 0;
+  // device-side checkpoint (px_snapshot / px_restore)
+  uint32_t* b_grid = nullptr;
+  uint32_t* b_pool = nullptr;
+  uint32_t* b_super = nullptr;
+  uint32_t* b_ctrl = nullptr;
+  uint32_t* b_free = nullptr;
   // profiling: event pairs recorded around kernels while `prof` is set
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;

@@ -1,0 +1,317 @@
+// px_napi.cc - N-API addon: a thin binding of the C ABI (include/pixels.h)
+// for the reference's TypeScript host (src/gpu/GpuWorld.ts, Simulation.ts).
+// Every function maps 1:1 onto a px_* entry point; typed arrays are passed as
+// host buffers.  Errors become thrown JS Errors carrying px_last_error().
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pixels.h"
+
+namespace {
+
+#define NAPI_CALL(env, call)                                   \
+  do {                                                         \
+    if ((call) != napi_ok) {                                   \
+      napi_throw_error((env), nullptr, "N-API call failed: " #call); \
+      return nullptr;                                          \
+    }                                                          \
+  } while (0)
+
+napi_value throw_px(napi_env env, px_ctx* ctx, int rc) {
+  std::string m = std::string("pixels error ") + std::to_string(rc) + ": " + px_last_error(ctx);
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+bool get_args(napi_env env, napi_callback_info info, size_t n, napi_value* argv) {
+  size_t argc = n;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok) return false;
+  for (size_t i = argc; i < n; i++) napi_get_undefined(env, &argv[i]);
+  return true;
+}
+
+px_ctx* get_ctx(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a pixels context");
+    return nullptr;
+  }
+  return static_cast<px_ctx*>(p);
+}
+
+// pointer + byte length of a TypedArray / ArrayBuffer / Buffer
+bool get_bytes(napi_env env, napi_value v, void** data, size_t* bytes) {
+  bool is_ta = false, is_ab = false;
+  napi_is_typedarray(env, v, &is_ta);
+  if (is_ta) {
+    napi_typedarray_type t;
+    size_t len = 0, off = 0;
+    napi_value ab;
+    if (napi_get_typedarray_info(env, v, &t, &len, data, &ab, &off) != napi_ok) return false;
+    size_t el = 1;
+    switch (t) {
+      case napi_int16_array: case napi_uint16_array: el = 2; break;
+      case napi_int32_array: case napi_uint32_array: case napi_float32_array: el = 4; break;
+      case napi_float64_array: el = 8; break;
+      default: el = 1;
+    }
+    *bytes = len * el;
+    return true;
+  }
+  napi_is_arraybuffer(env, v, &is_ab);
+  if (is_ab) return napi_get_arraybuffer_info(env, v, data, bytes) == napi_ok;
+  return false;
+}
+
+uint32_t get_u32(napi_env env, napi_value v, uint32_t dflt) {
+  uint32_t x = dflt;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_number) napi_get_value_uint32(env, v, &x);
+  return x;
+}
+
+double get_f64(napi_env env, napi_value v, double dflt) {
+  double x = dflt;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_number) napi_get_value_double(env, v, &x);
+  return x;
+}
+
+napi_value prop(napi_env env, napi_value obj, const char* name) {
+  napi_value v;
+  if (napi_get_named_property(env, obj, name, &v) != napi_ok) napi_get_undefined(env, &v);
+  return v;
+}
+
+napi_value new_typed(napi_env env, napi_typedarray_type t, size_t count, size_t el, void** data) {
+  napi_value ab, out;
+  if (napi_create_arraybuffer(env, count * el, data, &ab) != napi_ok) return nullptr;
+  if (napi_create_typedarray(env, t, count, ab, 0, &out) != napi_ok) return nullptr;
+  return out;
+}
+
+// create({dims, grid, pool, poolUsed, holes, mats, poolCapacity, voxelSize, device}) -> context
+napi_value Create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  napi_value d = argv[0];
+  px_world_desc desc;
+  std::memset(&desc, 0, sizeof desc);
+  napi_value dims = prop(env, d, "dims");
+  for (uint32_t a = 0; a < 3; a++) {
+    napi_value e;
+    napi_get_element(env, dims, a, &e);
+    desc.dims[a] = get_u32(env, e, 0);
+  }
+  void* p = nullptr;
+  size_t nb = 0;
+  if (!get_bytes(env, prop(env, d, "grid"), &p, &nb)) { napi_throw_type_error(env, nullptr, "grid: Uint32Array"); return nullptr; }
+  desc.grid = static_cast<const uint32_t*>(p);
+  if (get_bytes(env, prop(env, d, "pool"), &p, &nb)) desc.pool = static_cast<const uint16_t*>(p);
+  desc.pool_used = get_u32(env, prop(env, d, "poolUsed"), 0);
+  if (get_bytes(env, prop(env, d, "holes"), &p, &nb)) {
+    desc.holes = static_cast<const uint32_t*>(p);
+    desc.n_holes = (uint32_t)(nb / 4);
+  }
+  if (!get_bytes(env, prop(env, d, "mats"), &p, &nb) || nb < PX_MAT_COUNT * PX_MAT_FLOATS * 4) {
+    napi_throw_type_error(env, nullptr, "mats: Float32Array(256*20)");
+    return nullptr;
+  }
+  desc.mats = static_cast<const float*>(p);
+  desc.pool_capacity = get_u32(env, prop(env, d, "poolCapacity"), 0);
+  desc.voxel_size = (float)get_f64(env, prop(env, d, "voxelSize"), 1.0 / 80.0);
+  desc.device = (int32_t)get_u32(env, prop(env, d, "device"), 0);
+  desc.limits = nullptr;
+  px_ctx* ctx = nullptr;
+  int rc = px_create(&desc, &ctx);
+  if (rc != PX_OK) return throw_px(env, nullptr, rc);
+  napi_value out;
+  NAPI_CALL(env, napi_create_external(env, ctx, nullptr, nullptr, &out));
+  return out;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  px_destroy(c);
+  return nullptr;
+}
+
+napi_value SetUniforms(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[1], &p, &nb) || nb < PX_UNIFORM_FLOATS * 4) {
+    napi_throw_type_error(env, nullptr, "uniforms: Float32Array(52)");
+    return nullptr;
+  }
+  int rc = px_set_uniforms(c, static_cast<const float*>(p));
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+napi_value SetLights(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[1], &p, &nb)) { napi_throw_type_error(env, nullptr, "lights: Float32Array"); return nullptr; }
+  int rc = px_set_lights(c, static_cast<const float*>(p), (uint32_t)(nb / 16));
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+// trace(ctx, kind, rays: ArrayBuffer of 32-byte records) -> Uint8Array of 16-byte hits
+napi_value Trace(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t kind = get_u32(env, argv[1], 0);
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[2], &p, &nb) || nb % sizeof(px_ray)) {
+    napi_throw_type_error(env, nullptr, "rays: 32-byte records");
+    return nullptr;
+  }
+  uint32_t n = (uint32_t)(nb / sizeof(px_ray));
+  void* outp = nullptr;
+  napi_value out = new_typed(env, napi_uint8_array, (size_t)n * sizeof(px_hit), 1, &outp);
+  if (!out) return nullptr;
+  int rc = px_trace(c, kind, static_cast<const px_ray*>(p), n, static_cast<px_hit*>(outp));
+  return rc ? throw_px(env, c, rc) : out;
+}
+
+// probe(ctx, [ox,oy,oz], [sx,sy,sz]) -> Uint32Array (Simulation.probe)
+napi_value Probe(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  int32_t o[3], s[3];
+  for (uint32_t a = 0; a < 3; a++) {
+    napi_value e;
+    napi_get_element(env, argv[1], a, &e);
+    napi_get_value_int32(env, e, &o[a]);
+    napi_get_element(env, argv[2], a, &e);
+    napi_get_value_int32(env, e, &s[a]);
+  }
+  if (s[0] < 0 || s[1] < 0 || s[2] < 0) { napi_throw_range_error(env, nullptr, "negative size"); return nullptr; }
+  size_t n = (size_t)s[0] * s[1] * s[2];
+  void* outp = nullptr;
+  napi_value out = new_typed(env, napi_uint32_array, n, 4, &outp);
+  if (!out) return nullptr;
+  int rc = px_probe(c, o, s, static_cast<uint32_t*>(outp));
+  return rc ? throw_px(env, c, rc) : out;
+}
+
+// readCtrl(ctx) -> {ctrl: Uint32Array(36), args: Uint32Array(44), cam: Float32Array(64)}
+napi_value ReadCtrl(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void *pc, *pa, *pm;
+  napi_value ctrl = new_typed(env, napi_uint32_array, PX_CTRL_WORDS, 4, &pc);
+  napi_value args = new_typed(env, napi_uint32_array, PX_ARGS_WORDS, 4, &pa);
+  napi_value cam = new_typed(env, napi_float32_array, PX_CAMERA_FLOATS, 4, &pm);
+  if (!ctrl || !args || !cam) return nullptr;
+  int rc = px_read_ctrl(c, static_cast<uint32_t*>(pc), static_cast<uint32_t*>(pa), static_cast<float*>(pm));
+  if (rc) return throw_px(env, c, rc);
+  napi_value obj;
+  NAPI_CALL(env, napi_create_object(env, &obj));
+  napi_set_named_property(env, obj, "ctrl", ctrl);
+  napi_set_named_property(env, obj, "args", args);
+  napi_set_named_property(env, obj, "cam", cam);
+  return obj;
+}
+
+napi_value PushImpacts(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[1], &p, &nb) || nb % 64) {
+    napi_throw_type_error(env, nullptr, "impacts: Float32Array of 16-float records");
+    return nullptr;
+  }
+  int rc = px_push_impacts(c, static_cast<const float*>(p), (uint32_t)(nb / 64));
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+napi_value IslandPass(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = px_island_pass(c);
+  if (rc == PX_OK) rc = px_sync(c);
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+// islandLabels(ctx) -> {keys: Uint32Array, counts: Uint32Array}
+napi_value IslandLabels(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  uint32_t n = 0;
+  int rc = px_island_labels(c, nullptr, nullptr, 0, &n);
+  if (rc) return throw_px(env, c, rc);
+  void *pk, *pc;
+  napi_value keys = new_typed(env, napi_uint32_array, n, 4, &pk);
+  napi_value counts = new_typed(env, napi_uint32_array, n, 4, &pc);
+  if (!keys || !counts) return nullptr;
+  uint32_t m = 0;
+  rc = px_island_labels(c, static_cast<uint32_t*>(pk), static_cast<uint32_t*>(pc), n, &m);
+  if (rc) return throw_px(env, c, rc);
+  napi_value obj;
+  NAPI_CALL(env, napi_create_object(env, &obj));
+  napi_set_named_property(env, obj, "keys", keys);
+  napi_set_named_property(env, obj, "counts", counts);
+  return obj;
+}
+
+napi_value LastError(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  void* p = nullptr;
+  napi_get_value_external(env, argv[0], &p);
+  napi_value s;
+  NAPI_CALL(env, napi_create_string_utf8(env, px_last_error(static_cast<px_ctx*>(p)), NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct {
+    const char* name;
+    napi_callback fn;
+  } fns[] = {
+      {"create", Create}, {"destroy", Destroy}, {"setUniforms", SetUniforms}, {"setLights", SetLights},
+      {"trace", Trace}, {"probe", Probe}, {"readCtrl", ReadCtrl}, {"pushImpacts", PushImpacts},
+      {"islandPass", IslandPass}, {"islandLabels", IslandLabels}, {"lastError", LastError},
+  };
+  for (auto& f : fns) {
+    napi_value v;
+    napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.fn, nullptr, &v);
+    napi_set_named_property(env, exports, f.name, v);
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

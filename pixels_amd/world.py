@@ -91,6 +91,12 @@ class GpuWorld:
         lights = np.ascontiguousarray(lights, np.float32)
         self.check(self.lib.px_set_lights(self.ctx, N.ptr(lights, C.c_float), lights.size // 4))
 
+    def snapshot(self) -> None:
+        self.check(self.lib.px_snapshot(self.ctx))
+
+    def restore(self) -> None:
+        self.check(self.lib.px_restore(self.ctx))
+
     def profile_begin(self) -> None:
         self.check(self.lib.px_profile_begin(self.ctx))
 
@@ -110,6 +116,33 @@ class GpuWorld:
         w = np.empty(count * 256, np.uint32)
         self.check(self.lib.px_read_pool(self.ctx, first, count, N.ptr(w)))
         return w
+
+
+def impacts_from_hits(dims, rays: np.ndarray, hits: np.ndarray, radius, fragile_radius=None, impulse=0.0):
+    """Impact records (16 floats, pushImpact layout sim.wgsl:203-211) for ray
+    hits, centred as shoot() does (sim.wgsl:382-385):
+    c = voxel + 0.5 - n * min(r * 0.3, 2)."""
+    dx, dy, _ = dims
+    ok = hits["face"] != N.FACE_MISS
+    h = hits[ok]
+    r = np.broadcast_to(np.asarray(radius, np.float32), hits.shape)[ok]
+    fr = r if fragile_radius is None else np.broadcast_to(np.asarray(fragile_radius, np.float32), hits.shape)[ok]
+    key = h["key"].astype(np.int64)
+    vox = np.stack([key % dx, (key // dx) % dy, key // (dx * dy)], 1).astype(np.float32)
+    n = np.zeros((h.shape[0], 3), np.float32)
+    face = h["face"].astype(np.int64)
+    sel = face < 6
+    n[np.nonzero(sel)[0], face[sel] >> 1] = np.where(face[sel] & 1, -1.0, 1.0).astype(np.float32)
+    off = np.minimum(r * np.float32(0.3), np.float32(2.0)).astype(np.float32)
+    c = (vox + np.float32(0.5)) - n * off[:, None]
+    out = np.zeros((h.shape[0], 16), np.float32)
+    out[:, 0:3] = c
+    out[:, 3] = r
+    out[:, 4] = fr
+    out[:, 5] = impulse
+    out[:, 6:9] = n
+    out[:, 9:12] = rays["rd"][ok]
+    return out
 
 
 class SimStats(dict):
