@@ -28,6 +28,7 @@ struct PxWorldView {
   const float* mats;
   const uint32_t* super_mask;  // 1 bit per super cell: count != 0
   const uint32_t* brick_mask;  // 1 bit per brick cell: grid != 0
+  const uint64_t* super_bm;    // per super cell: 64-bit brick occupancy
   int32_t dims[3];
   int32_t bd[3];
   int32_t sd[3];
@@ -65,6 +66,11 @@ struct px_ctx {
   // traversal acceleration (derived from grid / super grid)
   uint32_t* d_super_mask = nullptr;
   uint32_t* d_brick_mask = nullptr;
+  uint64_t* d_super_bm = nullptr;
+  // traversal launch shape (variant 0 = nested-loop kernels, 1 = persistent)
+  struct Tune {
+    uint32_t variant = 1, blocks = 2048, block = 256, chunk = 256, refill_min = 16;
+  } tune;
   bool masks_dirty = true;
   // frame
   px_ray* d_shadow_rays = nullptr;
@@ -135,6 +141,11 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
                     uint32_t* n_cast);
 int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
 int px_launch_masks(px_ctx* c);
+// px_march.hip
+int px_launch_super_bm(px_ctx* c);
+bool px_march_fits(const px_ctx* c);
+int px_launch_march(px_ctx* c, bool shadow, const px_ray* d_rays, uint32_t n, px_hit* d_out);
+int px_launch_frame_persistent(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow);
 // edit chain (px_edit.hip)
 int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);

@@ -542,6 +542,7 @@ PxWorldView px_view(const px_ctx* c) {
   w.mats = c->d_mats;
   w.super_mask = c->d_super_mask;
   w.brick_mask = c->d_brick_mask;
+  w.super_bm = c->d_super_bm;
   for (int a = 0; a < 3; a++) {
     w.dims[a] = (int32_t)c->dims[a];
     w.bd[a] = (int32_t)c->bd[a];
@@ -557,6 +558,8 @@ int px_launch_masks(px_ctx* c) {
   hipLaunchKernelGGL(k_masks, dim3(blocks), dim3(256), 0, c->stream, c->d_grid, c->ncell, c->d_super, c->nsuper,
                      c->d_brick_mask, c->d_super_mask);
   PX_HIP_CHECK(c, hipGetLastError());
+  int rc = px_launch_super_bm(c);
+  if (rc) return rc;
   c->masks_dirty = false;
   return PX_OK;
 }
@@ -567,6 +570,8 @@ int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, 
   if (n == 0) return PX_OK;
   int rc = px_refresh_masks(c);
   if (rc) return rc;
+  if (c->tune.variant == 1u && (kind == PX_RAY_PRIMARY || kind == PX_RAY_SHADOW) && px_march_fits(c))
+    return px_launch_march(c, kind == PX_RAY_SHADOW, d_rays, n, d_out);
   PxWorldView w = px_view(c);
   dim3 grid((n + 255u) / 256u), block(256);
   PxProfScope prof(c, PX_PROF_TRACE);
@@ -597,6 +602,15 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
     c->shadow_cap = 0;
     PX_HIP_CHECK(c, hipMalloc(&c->d_shadow_rays, need * sizeof(px_ray)));
     c->shadow_cap = need;
+  }
+  if (c->tune.variant == 1u && px_march_fits(c)) {
+    rc = px_launch_frame_persistent(c, d_primary, n, d_prim, d_shadow);
+    if (rc) return rc;
+    if (n_cast) {
+      PX_HIP_CHECK(c, hipMemcpyAsync(n_cast, c->d_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    }
+    return PX_OK;
   }
   // a fill kernel, not hipMemsetAsync, so the launch sequence can be captured in a graph
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_counters, 1u, 0u);
