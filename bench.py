@@ -80,14 +80,6 @@ def main() -> int:
     gw.sync()
     log("warmup done")
 
-    # launch shape: measure the nested-loop kernels, then let the autotuner
-    # pick the persistent kernel's shape (outputs are identical for all)
-    from pixels_amd.tune import tune_frame
-    shape, tuned_ms, tune_results = tune_frame(gw, sim, lambda: sim.trace_frame_device(d_rays, n, d_prim, d_sh),
-                                               log=log if args.verbose else None)
-    plain_ms = tune_results[0][1]
-    log("autotune: nested-loop kernels %.3f ms/frame, best %s %.3f ms/frame" % (plain_ms, shape, tuned_ms))
-
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -118,10 +110,7 @@ def main() -> int:
     k_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
     ms_p = k_ms.get(N.PROF_PRIMARY, 0.0)
     ms_s = k_ms.get(N.PROF_SHADOW, 0.0)
-    ms_g = k_ms.get(N.PROF_SHADOW_GEN, 0.0)
-    persistent = shape.get("variant", 0) == 1
-    names = (("k_march_persistent<1,true>", "k_march_persistent<0,false>") if persistent else
-             ("k_trace_shadow_queue", "k_frame_primary"))
+    names = ("k_trace_shadow_queue", "k_frame_primary")
     if ms_s >= ms_p:
         dom, dom_bytes, dom_ms = names[0], bytes_sh, ms_s
     else:
@@ -130,7 +119,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes,
-                "kernels_ms": {"primary": round(ms_p, 4), "shadow_gen": round(ms_g, 4), "shadow": round(ms_s, 4)}}
+                "kernels_ms": {"primary": round(ms_p, 4), "shadow": round(ms_s, 4)}}
 
     cpu = None
     parity = None
@@ -170,8 +159,7 @@ def main() -> int:
         "config": {"workload": "frame: 1920x1080 primary + sun + 8 point-light shadow rays, 1280x352x960 grid",
                    "grid": list(scene.dims), "primitives": scene.n_prims, "pool_bricks": scene.pool_used,
                    "fill": round(scene.fill, 4), "primary_rays": n, "shadow_rays": n_cast,
-                   "bit_exact_vs_oracle": parity, "launch_shape": shape,
-                   "nested_loop_ms_per_frame": round(plain_ms, 4)},
+                   "bit_exact_vs_oracle": parity},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "island_pass": island,

@@ -225,23 +225,9 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_SHADOW 1   /* frame: shadow-ray march */
 #define PX_PROF_TRACE 2    /* px_trace / px_trace_device */
 #define PX_PROF_EDIT 3     /* island pass, all kernels */
-#define PX_PROF_SHADOW_GEN 4 /* frame: shadow-ray generation (persistent variant) */
 #define PX_PROF_SLOTS 8
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);
-
-/* ---- traversal launch shape (no counterpart in the reference, whose WebGPU
- * implementation schedules fragments itself).  variant 0 = one ray per
- * thread with the nested loops of march(); variant 1 = persistent wavefronts
- * whose lanes take a new ray from a per-wavefront chunk when theirs ends.
- * blocks = workgroups launched, block = threads per workgroup (multiple of
- * 64, at most 256), chunk = rays claimed per atomic, refill_min = idle lanes
- * that trigger a refill.  A 0 argument keeps the current value.  Outputs are
- * bit-identical for every setting. */
-int px_set_tuning(px_ctx* ctx, uint32_t variant, uint32_t blocks, uint32_t block, uint32_t chunk,
-                  uint32_t refill_min);
-/* out = {variant, blocks, block, chunk, refill_min} */
-int px_get_tuning(const px_ctx* ctx, uint32_t out[5]);
 
 #ifdef __cplusplus
 }

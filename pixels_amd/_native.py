@@ -96,12 +96,10 @@ SIGNATURES = {
     "px_island_labels": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
     "px_snapshot": (C.c_int, [_P]),
     "px_restore": (C.c_int, [_P]),
-    "px_set_tuning": (C.c_int, [_P, _U32, _U32, _U32, _U32, _U32]),
-    "px_get_tuning": (C.c_int, [_P, _U32P]),
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
-PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_SHADOW_GEN, PROF_SLOTS = 0, 1, 2, 3, 4, 8
+PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_SLOTS = 0, 1, 2, 3, 8
 
 _lib = None
 

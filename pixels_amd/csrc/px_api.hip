@@ -85,7 +85,7 @@ static void release(px_ctx* c) {
   }
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
-                  c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_shadow_rays, c->d_counters,
+                  c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_shadow_rays, c->d_wave_count, c->d_counters,
                   c->b_grid, c->b_pool, c->b_super, c->b_ctrl, c->b_free};
   for (void* p : bufs)
     if (p) hipFree(p);
@@ -418,30 +418,6 @@ int px_restore(px_ctx* c) {
   c->labels_valid = false;
   c->masks_dirty = true;
   return px_refresh_masks(c);
-}
-
-int px_set_tuning(px_ctx* c, uint32_t variant, uint32_t blocks, uint32_t block, uint32_t chunk, uint32_t refill_min) {
-  if (!c) return PX_ERR_ARG;
-  if (variant > 1u) return px_fail(c, PX_ERR_ARG, "px_set_tuning", "variant must be 0 or 1");
-  if (block && (block % 64u || block > 256u)) return px_fail(c, PX_ERR_ARG, "px_set_tuning", "block: 64, 128, 192 or 256");
-  if (refill_min > 64u) return px_fail(c, PX_ERR_ARG, "px_set_tuning", "refill_min: at most 64");
-  if (blocks > 65536u) return px_fail(c, PX_ERR_ARG, "px_set_tuning", "blocks: at most 65536");
-  c->tune.variant = variant;
-  if (blocks) c->tune.blocks = blocks;
-  if (block) c->tune.block = block;
-  if (chunk) c->tune.chunk = chunk;
-  if (refill_min) c->tune.refill_min = refill_min;
-  return PX_OK;
-}
-
-int px_get_tuning(const px_ctx* c, uint32_t out[5]) {
-  if (!c || !out) return PX_ERR_ARG;
-  out[0] = c->tune.variant;
-  out[1] = c->tune.blocks;
-  out[2] = c->tune.block;
-  out[3] = c->tune.chunk;
-  out[4] = c->tune.refill_min;
-  return PX_OK;
 }
 
 int px_profile_begin(px_ctx* c) {

@@ -67,14 +67,12 @@ struct px_ctx {
   uint32_t* d_super_mask = nullptr;
   uint32_t* d_brick_mask = nullptr;
   uint64_t* d_super_bm = nullptr;
-  // traversal launch shape (variant 0 = nested-loop kernels, 1 = persistent)
-  struct Tune {
-    uint32_t variant = 1, blocks = 2048, block = 256, chunk = 256, refill_min = 16;
-  } tune;
   bool masks_dirty = true;
   // frame
   px_ray* d_shadow_rays = nullptr;
   size_t shadow_cap = 0;
+  uint32_t* d_wave_count = nullptr;  // shadow rays queued per wavefront region
+  uint32_t wave_count_cap = 0;
   uint32_t* d_counters = nullptr;  // small scratch counters
   // edit chain
   float* d_impacts = nullptr;
@@ -141,11 +139,6 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
                     uint32_t* n_cast);
 int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
 int px_launch_masks(px_ctx* c);
-// px_march.hip
-int px_launch_super_bm(px_ctx* c);
-bool px_march_fits(const px_ctx* c);
-int px_launch_march(px_ctx* c, bool shadow, const px_ray* d_rays, uint32_t n, px_hit* d_out);
-int px_launch_frame_persistent(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow);
 // edit chain (px_edit.hip)
 int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);

@@ -51,14 +51,58 @@ PX_DEV void glass_surface(const float* __restrict__ mats, float trans[3], uint32
   }
 }
 
-// march() of raymarch.wgsl:44-162, one ray per lane.
-PX_DEV MarchOut march(const PxWorldView& w, const float ro[3], const float rdIn[3], float maxT, bool shadow) {
+// Per-block tables of march(): super-cell occupancy bits (count != 0) and the
+// 256-bit glass table (material D.y > 0).  `sup` is LDS when the world's bits
+// fit, else the global copy.
+struct MarchTables {
+  const uint32_t* sup;
+  const uint32_t* glass;
+};
+
+PX_DEV bool bit_of(const uint32_t* bits, uint32_t i) { return ((bits[i >> 5] >> (i & 31u)) & 1u) != 0u; }
+
+// Block-cooperative: every thread of the block calls it before any early exit.
+template <bool SUP_LDS>
+PX_DEV MarchTables load_march_tables(const PxWorldView& w, uint32_t* s_mem) {
+  uint32_t* s_glass = s_mem;
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < 256u; base += blockDim.x) {
+    uint32_t m = base + threadIdx.x;
+    uint64_t b = __ballot(m < 256u && w.mats[m * 20u + 13u] > 0.0f);
+    uint32_t word = (base + wave * 64u) >> 5;
+    if (lane == 0u && word < 8u) {
+      s_glass[word] = (uint32_t)b;
+      s_glass[word + 1u] = (uint32_t)(b >> 32);
+    }
+  }
+  MarchTables T;
+  T.glass = s_glass;
+  if (SUP_LDS) {
+    uint32_t* s_sup = s_mem + 8;
+    const uint32_t sw = ((uint32_t)(w.sd[0] * w.sd[1] * w.sd[2]) + 31u) / 32u;
+    for (uint32_t i = threadIdx.x; i < sw; i += blockDim.x) s_sup[i] = w.super_mask[i];
+    T.sup = s_sup;
+  } else {
+    T.sup = w.super_mask;
+  }
+  __syncthreads();
+  return T;
+}
+
+// march() of raymarch.wgsl:44-162, one ray per lane.  Same loops, caps and
+// FP32 operations as the WGSL; what differs is where occupancy comes from
+// (super bit in LDS, brick bit from the super cell's 64-bit mask, the grid
+// word only for non-empty bricks) and that each DDA step is branch-free.
+PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro[3], const float rdIn[3],
+                      float maxT, bool shadow) {
   const float* __restrict__ mats = w.mats;
   MarchOut o;
   o.key = PX_NONE;
   o.t = 0.0f;
   o.face = 0xFFu;
   o.value = 0u;
+  o.steps = 0u;
   uint32_t steps = 0;
   float trans[3] = {1.0f, 1.0f, 1.0f};
   float rd[3], inv[3], tmn[3], tmx[3];
@@ -71,83 +115,94 @@ PX_DEV MarchOut march(const PxWorldView& w, const float ro[3], const float rdIn[
     tmx[a] = fmaxf(t0, t1);
   }
   float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
-  float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
-  o.steps = 0;
+  const float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
   if (t >= tExit) return o;
 
   int32_t stp[3];
-  float pos[3];
+  float pos[3], tDeltaS[3], tDeltaB[3], tDeltaV[3];
+  uint32_t fc[3];
   for (int a = 0; a < 3; a++) {
     stp[a] = rd[a] > 0.0f ? 1 : (rd[a] < 0.0f ? -1 : 0);
     pos[a] = rd[a] >= 0.0f ? 1.0f : 0.0f;
+    fc[a] = step_face(a, stp[a]);
+    tDeltaS[a] = fabsf(inv[a]) * 32.0f;
+    tDeltaB[a] = fabsf(inv[a]) * 8.0f;
+    tDeltaV[a] = fabsf(inv[a]);
   }
   uint32_t normal = 6u;
   if (t > 0.0f) {
-    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) normal = step_face(0, stp[0]);
-    else if (tmn[1] >= tmn[2]) normal = step_face(1, stp[1]);
-    else normal = step_face(2, stp[2]);
+    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) normal = fc[0];
+    else if (tmn[1] >= tmn[2]) normal = fc[1];
+    else normal = fc[2];
   }
   int32_t sc[3];
-  float tMaxS[3], tDeltaS[3], tDeltaB[3], tDeltaV[3];
+  float tMaxS[3];
   for (int a = 0; a < 3; a++) {
     float p0 = ro[a] + rd[a] * (t + 1e-3f);
     sc[a] = px_clampi(px_f2i(floorf(p0 / 32.0f)), 0, w.sd[a] - 1);
-    tDeltaS[a] = fabsf(inv[a]) * 32.0f;
     tMaxS[a] = (((float)sc[a] + pos[a]) * 32.0f - ro[a]) * inv[a];
-    tDeltaB[a] = fabsf(inv[a]) * 8.0f;
-    tDeltaV[a] = fabsf(inv[a]);
   }
   bool inGlass = false;
   uint32_t glassId = 0u;
 
   for (int i = 0; i < 256; i++) {
     steps++;
-    if (w.super_grid[(uint32_t)(sc[0] + w.sd[0] * (sc[1] + w.sd[1] * sc[2]))] != 0u) {
-      int32_t sLo[3], sHi[3], bc[3];
+    const uint32_t sidx = (uint32_t)(sc[0] + w.sd[0] * (sc[1] + w.sd[1] * sc[2]));
+    if (bit_of(T.sup, sidx)) {
+      const uint64_t bm = w.super_bm[sidx];
+      int32_t sLo[3], hiL[3], lb[3];
       float tMaxB[3];
       for (int a = 0; a < 3; a++) {
         sLo[a] = sc[a] * 4;
-        sHi[a] = min(sLo[a] + 3, w.bd[a] - 1);
-        bc[a] = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo[a], sHi[a]);
-        tMaxB[a] = (((float)bc[a] + pos[a]) * 8.0f - ro[a]) * inv[a];
+        const int32_t sHi = min(sLo[a] + 3, w.bd[a] - 1);
+        hiL[a] = sHi - sLo[a];
+        const int32_t bc = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo[a], sHi);
+        lb[a] = bc - sLo[a];
+        tMaxB[a] = (((float)bc + pos[a]) * 8.0f - ro[a]) * inv[a];
       }
       float tb = t;
       uint32_t nb = normal;
       for (int j = 0; j < 14; j++) {
         steps++;
-        uint32_t g = w.grid[(uint32_t)(bc[0] + w.bd[0] * (bc[1] + w.bd[1] * bc[2]))];
-        if (g != 0u) {
-          int32_t lo[3], vc[3];
+        if ((bm >> (uint32_t)(lb[0] + 4 * (lb[1] + 4 * lb[2]))) & 1ull) {
+          int32_t lo[3], lv[3];
+          float tMaxV[3];
           for (int a = 0; a < 3; a++) {
-            lo[a] = bc[a] * 8;
-            vc[a] = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo[a], lo[a] + 7);
+            lo[a] = (sLo[a] + lb[a]) * 8;
+            const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo[a], lo[a] + 7);
+            lv[a] = vc - lo[a];
+            tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
           }
-          if ((g & PX_UNIFORM_BIT) != 0u && !px_is_glass(mats, g & 0xffu)) {
-            o.key = key_of(w, vc[0], vc[1], vc[2]);
+          const uint32_t g = w.grid[(uint32_t)((lo[0] >> 3) + w.bd[0] * ((lo[1] >> 3) + w.bd[1] * (lo[2] >> 3)))];
+          const bool uni = (g & PX_UNIFORM_BIT) != 0u;
+          if (uni && !bit_of(T.glass, g & 0xffu)) {
+            o.key = key_of(w, lo[0] + lv[0], lo[1] + lv[1], lo[2] + lv[2]);
             o.t = px_canon0(tb);
             o.face = nb;
             o.value = g & 0xffffu;
             o.steps = steps;
             return o;
           }
-          float tMaxV[3];
-          for (int a = 0; a < 3; a++) tMaxV[a] = ((float)vc[a] + pos[a] - ro[a]) * inv[a];
+          const uint32_t* __restrict__ pb = w.pool + (size_t)(g - 1u) * 256u;
           float tv = tb;
           uint32_t n = nb;
           for (int k = 0; k < 25; k++) {
             steps++;
-            uint32_t v = px_brick_voxel(w.pool, g, vc[0] - lo[0], vc[1] - lo[1], vc[2] - lo[2]);
+            const uint32_t li = (uint32_t)(lv[0] | (lv[1] << 3) | (lv[2] << 6));
+            uint32_t v;
+            if (uni) v = g & 0xffffu;
+            else v = (pb[li >> 1] >> ((li & 1u) * 16u)) & 0xffffu;
             if (v != 0u) {
-              uint32_t id = v & 0xffu;
+              const uint32_t id = v & 0xffu;
               bool hit = true;
-              if (px_is_glass(mats, id)) {
+              if (bit_of(T.glass, id)) {
                 if (!inGlass || id != glassId) glass_surface(mats, trans, id, n, rd, shadow);
                 inGlass = true;
                 glassId = id;
                 hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
               }
               if (hit) {
-                o.key = key_of(w, vc[0], vc[1], vc[2]);
+                o.key = key_of(w, lo[0] + lv[0], lo[1] + lv[1], lo[2] + lv[2]);
                 o.t = px_canon0(tv);
                 o.face = n;
                 o.value = v;
@@ -157,43 +212,61 @@ PX_DEV MarchOut march(const PxWorldView& w, const float ro[3], const float rdIn[
             } else {
               inGlass = false;
             }
-            if (tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2]) {
-              tv = tMaxV[0]; vc[0] += stp[0]; tMaxV[0] += tDeltaV[0]; n = step_face(0, stp[0]);
-            } else if (tMaxV[1] < tMaxV[2]) {
-              tv = tMaxV[1]; vc[1] += stp[1]; tMaxV[1] += tDeltaV[1]; n = step_face(1, stp[1]);
-            } else {
-              tv = tMaxV[2]; vc[2] += stp[2]; tMaxV[2] += tDeltaV[2]; n = step_face(2, stp[2]);
+            const bool a0 = tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2];
+            const bool a1 = !a0 && tMaxV[1] < tMaxV[2];
+            const bool a2 = !a0 && !a1;
+            tv = a0 ? tMaxV[0] : (a1 ? tMaxV[1] : tMaxV[2]);
+            n = a0 ? fc[0] : (a1 ? fc[1] : fc[2]);
+            tMaxV[0] = a0 ? tMaxV[0] + tDeltaV[0] : tMaxV[0];
+            tMaxV[1] = a1 ? tMaxV[1] + tDeltaV[1] : tMaxV[1];
+            tMaxV[2] = a2 ? tMaxV[2] + tDeltaV[2] : tMaxV[2];
+            lv[0] += a0 ? stp[0] : 0;
+            lv[1] += a1 ? stp[1] : 0;
+            lv[2] += a2 ? stp[2] : 0;
+            if (tv > tExit) {
+              o.steps = steps;
+              return o;
             }
-            if (tv > tExit) { o.steps = steps; return o; }
-            if (vc[0] < lo[0] || vc[1] < lo[1] || vc[2] < lo[2] || vc[0] > lo[0] + 7 || vc[1] > lo[1] + 7 ||
-                vc[2] > lo[2] + 7)
-              break;
+            if ((uint32_t)(lv[0] | lv[1] | lv[2]) > 7u) break;
           }
         } else {
           inGlass = false;
         }
-        if (tMaxB[0] < tMaxB[1] && tMaxB[0] < tMaxB[2]) {
-          tb = tMaxB[0]; bc[0] += stp[0]; tMaxB[0] += tDeltaB[0]; nb = step_face(0, stp[0]);
-        } else if (tMaxB[1] < tMaxB[2]) {
-          tb = tMaxB[1]; bc[1] += stp[1]; tMaxB[1] += tDeltaB[1]; nb = step_face(1, stp[1]);
-        } else {
-          tb = tMaxB[2]; bc[2] += stp[2]; tMaxB[2] += tDeltaB[2]; nb = step_face(2, stp[2]);
+        const bool a0 = tMaxB[0] < tMaxB[1] && tMaxB[0] < tMaxB[2];
+        const bool a1 = !a0 && tMaxB[1] < tMaxB[2];
+        const bool a2 = !a0 && !a1;
+        tb = a0 ? tMaxB[0] : (a1 ? tMaxB[1] : tMaxB[2]);
+        nb = a0 ? fc[0] : (a1 ? fc[1] : fc[2]);
+        tMaxB[0] = a0 ? tMaxB[0] + tDeltaB[0] : tMaxB[0];
+        tMaxB[1] = a1 ? tMaxB[1] + tDeltaB[1] : tMaxB[1];
+        tMaxB[2] = a2 ? tMaxB[2] + tDeltaB[2] : tMaxB[2];
+        lb[0] += a0 ? stp[0] : 0;
+        lb[1] += a1 ? stp[1] : 0;
+        lb[2] += a2 ? stp[2] : 0;
+        if (tb > tExit) {
+          o.steps = steps;
+          return o;
         }
-        if (tb > tExit) { o.steps = steps; return o; }
-        if (bc[0] < sLo[0] || bc[1] < sLo[1] || bc[2] < sLo[2] || bc[0] > sHi[0] || bc[1] > sHi[1] || bc[2] > sHi[2])
+        if ((uint32_t)lb[0] > (uint32_t)hiL[0] || (uint32_t)lb[1] > (uint32_t)hiL[1] ||
+            (uint32_t)lb[2] > (uint32_t)hiL[2])
           break;
       }
     } else {
       inGlass = false;
     }
-    if (tMaxS[0] < tMaxS[1] && tMaxS[0] < tMaxS[2]) {
-      t = tMaxS[0]; sc[0] += stp[0]; tMaxS[0] += tDeltaS[0]; normal = step_face(0, stp[0]);
-    } else if (tMaxS[1] < tMaxS[2]) {
-      t = tMaxS[1]; sc[1] += stp[1]; tMaxS[1] += tDeltaS[1]; normal = step_face(1, stp[1]);
-    } else {
-      t = tMaxS[2]; sc[2] += stp[2]; tMaxS[2] += tDeltaS[2]; normal = step_face(2, stp[2]);
-    }
-    if (t > tExit || sc[0] < 0 || sc[1] < 0 || sc[2] < 0 || sc[0] >= w.sd[0] || sc[1] >= w.sd[1] || sc[2] >= w.sd[2])
+    const bool a0 = tMaxS[0] < tMaxS[1] && tMaxS[0] < tMaxS[2];
+    const bool a1 = !a0 && tMaxS[1] < tMaxS[2];
+    const bool a2 = !a0 && !a1;
+    t = a0 ? tMaxS[0] : (a1 ? tMaxS[1] : tMaxS[2]);
+    normal = a0 ? fc[0] : (a1 ? fc[1] : fc[2]);
+    tMaxS[0] = a0 ? tMaxS[0] + tDeltaS[0] : tMaxS[0];
+    tMaxS[1] = a1 ? tMaxS[1] + tDeltaS[1] : tMaxS[1];
+    tMaxS[2] = a2 ? tMaxS[2] + tDeltaS[2] : tMaxS[2];
+    sc[0] += a0 ? stp[0] : 0;
+    sc[1] += a1 ? stp[1] : 0;
+    sc[2] += a2 ? stp[2] : 0;
+    if (t > tExit || (uint32_t)sc[0] >= (uint32_t)w.sd[0] || (uint32_t)sc[1] >= (uint32_t)w.sd[1] ||
+        (uint32_t)sc[2] >= (uint32_t)w.sd[2])
       break;
   }
   o.steps = steps;
@@ -388,9 +461,12 @@ PX_DEV void load_ray(const px_ray* r, float ro[3], float rd[3], float& maxT, uin
   flags = __float_as_uint(b.w);
 }
 
-template <int KIND>
+template <int KIND, bool SUP_LDS>
 __global__ __launch_bounds__(256) void k_trace(PxWorldView w, const px_ray* __restrict__ rays, uint32_t n,
                                                px_hit* __restrict__ out) {
+  extern __shared__ uint32_t s_mem[];
+  MarchTables T;
+  if (KIND == 0 || KIND == 1 || KIND < 0) T = load_march_tables<SUP_LDS>(w, s_mem);
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float ro[3], rd[3], maxT;
@@ -398,20 +474,24 @@ __global__ __launch_bounds__(256) void k_trace(PxWorldView w, const px_ray* __re
   load_ray(rays + i, ro, rd, maxT, flags);
   int kind = KIND >= 0 ? KIND : (int)(flags & 3u);
   MarchOut m;
-  if (kind == 0) m = march(w, ro, rd, maxT, false);
-  else if (kind == 1) m = march(w, ro, rd, maxT, true);
+  if (kind == 0) m = march(w, T, ro, rd, maxT, false);
+  else if (kind == 1) m = march(w, T, ro, rd, maxT, true);
   else if (kind == 2) m = trace_solid(w, ro, rd, maxT);
   else m = segment(w, ro, rd);
   store_hit(out + i, m);
 }
 
-// Primary march + shadow-ray generation.  Shadow rays are appended to a
-// compacted queue (one atomic per wavefront, lanes placed by ballot prefix),
-// with the destination slot in `flags`.
+// Primary march + shadow-ray generation.  Wavefront `gw` owns the queue
+// region [gw * 64 * slots, (gw + 1) * 64 * slots): its shadow rays are packed
+// there slot by slot (lanes placed by ballot prefix, no atomics) with the
+// destination slot in `flags`, and their number goes to wave_count[gw].
+template <bool SUP_LDS>
 __global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDesc sd,
                                                        const px_ray* __restrict__ rays, uint32_t n,
                                                        px_hit* __restrict__ prim_out, px_hit* __restrict__ sh_out,
-                                                       px_ray* __restrict__ queue, uint32_t* __restrict__ counter) {
+                                                       px_ray* __restrict__ queue, uint32_t* __restrict__ wave_count) {
+  extern __shared__ uint32_t s_mem[];
+  const MarchTables T = load_march_tables<SUP_LDS>(w, s_mem);
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = i < n;
   float ro[3] = {0, 0, 0}, rd[3] = {0, 0, 1}, maxT = 0.0f;
@@ -421,7 +501,7 @@ __global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDe
   h.value = 0u;
   if (live) {
     load_ray(rays + i, ro, rd, maxT, flags);
-    h = march(w, ro, rd, maxT, false);
+    h = march(w, T, ro, rd, maxT, false);
     store_hit(prim_out + i, h);
     for (uint32_t s = 0; s < sd.slots; s++) store_not_cast(sh_out + (size_t)i * sd.slots + s);
   }
@@ -434,6 +514,10 @@ __global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDe
     O[a] = P[a] + N[a] * 0.5f;
   }
   // up to `slots` rays per lane; each slot is compacted across the wave
+  const uint32_t gwave = i >> 6;
+  px_ray* __restrict__ wq = queue + (size_t)gwave * 64u * sd.slots;
+  const uint32_t lane = __lane_id();
+  uint32_t wcount = 0u;
   for (uint32_t s = 0; s < sd.slots; s++) {
     bool cast = false;
     float dir[3] = {0, 0, 0}, mt = 0.0f;
@@ -462,32 +546,52 @@ __global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDe
         }
       }
     }
-    uint64_t mask = __ballot(cast);
-    if (mask == 0) continue;
-    uint32_t lane = __lane_id();
-    uint32_t base = 0;
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
+    const uint64_t mask = __ballot(cast);
     if (cast) {
-      uint32_t off = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      float4* q = reinterpret_cast<float4*>(queue + off);
+      uint32_t off = wcount + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+      float4* q = reinterpret_cast<float4*>(wq + off);
       q[0] = make_float4(O[0], O[1], O[2], dir[0]);
       q[1] = make_float4(dir[1], dir[2], mt, __uint_as_float(i * sd.slots + s));
     }
+    wcount += (uint32_t)__popcll(mask);
+  }
+  if (lane == 0u) wave_count[gwave] = wcount;
+}
+
+// One wavefront per queue region of k_frame_primary.
+template <bool SUP_LDS>
+__global__ __launch_bounds__(256) void k_trace_shadow_queue(PxWorldView w, const px_ray* __restrict__ queue,
+                                                            const uint32_t* __restrict__ wave_count,
+                                                            uint32_t n_regions, uint32_t slots,
+                                                            px_hit* __restrict__ sh_out) {
+  extern __shared__ uint32_t s_mem[];
+  const MarchTables T = load_march_tables<SUP_LDS>(w, s_mem);
+  const uint32_t region = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (region >= n_regions) return;
+  const uint32_t cnt = wave_count[region];
+  const px_ray* __restrict__ rq = queue + (size_t)region * 64u * slots;
+  for (uint32_t k = __lane_id(); k < cnt; k += 64u) {
+    float ro[3], rd[3], maxT;
+    uint32_t slot;
+    load_ray(rq + k, ro, rd, maxT, slot);
+    MarchOut m = march(w, T, ro, rd, maxT, true);
+    store_hit(sh_out + slot, m);
   }
 }
 
-__global__ __launch_bounds__(256) void k_trace_shadow_queue(PxWorldView w, const px_ray* __restrict__ queue,
-                                                            const uint32_t* __restrict__ counter,
-                                                            px_hit* __restrict__ sh_out) {
-  uint32_t n = *counter;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float ro[3], rd[3], maxT;
-    uint32_t slot;
-    load_ray(queue + i, ro, rd, maxT, slot);
-    MarchOut m = march(w, ro, rd, maxT, true);
-    store_hit(sh_out + slot, m);
+// Sum of the per-wavefront shadow-ray counts (one block).
+__global__ __launch_bounds__(1024) void k_sum_counts(const uint32_t* __restrict__ cnt, uint32_t n,
+                                                     uint32_t* __restrict__ out) {
+  __shared__ uint32_t part[16];
+  uint32_t s = 0u;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += cnt[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+  if (__lane_id() == 0u) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0u) {
+    uint32_t t = 0u;
+    for (uint32_t k = 0; k < blockDim.x / 64u; k++) t += part[k];
+    *out = t;
   }
 }
 
@@ -532,6 +636,25 @@ __global__ void k_masks(const uint32_t* __restrict__ grid, uint32_t ncell, const
   (void)lane;
 }
 
+// Per-super-cell 64-bit brick occupancy (bit = lx + 4 (ly + 4 lz)).
+__global__ void k_super_bm(const uint32_t* __restrict__ grid, int32_t bx, int32_t by, int32_t bz, int32_t sx,
+                           int32_t sy, int32_t sz, uint64_t* __restrict__ out) {
+  uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t ns = (uint32_t)(sx * sy * sz);
+  if (si >= ns) return;
+  int32_t s0 = (int32_t)(si % (uint32_t)sx), s1 = (int32_t)((si / (uint32_t)sx) % (uint32_t)sy);
+  int32_t s2 = (int32_t)(si / ((uint32_t)sx * (uint32_t)sy));
+  uint64_t m = 0ull;
+  for (int32_t lz = 0; lz < 4; lz++)
+    for (int32_t ly = 0; ly < 4; ly++)
+      for (int32_t lx = 0; lx < 4; lx++) {
+        int32_t x = s0 * 4 + lx, y = s1 * 4 + ly, z = s2 * 4 + lz;
+        if (x >= bx || y >= by || z >= bz) continue;
+        if (grid[(uint32_t)(x + bx * (y + by * z))] != 0u) m |= 1ull << (lx + 4 * (ly + 4 * lz));
+      }
+  out[si] = m;
+}
+
 }  // namespace
 
 PxWorldView px_view(const px_ctx* c) {
@@ -558,29 +681,46 @@ int px_launch_masks(px_ctx* c) {
   hipLaunchKernelGGL(k_masks, dim3(blocks), dim3(256), 0, c->stream, c->d_grid, c->ncell, c->d_super, c->nsuper,
                      c->d_brick_mask, c->d_super_mask);
   PX_HIP_CHECK(c, hipGetLastError());
-  int rc = px_launch_super_bm(c);
-  if (rc) return rc;
+  const uint32_t ns = c->nsuper;
+  hipLaunchKernelGGL(k_super_bm, dim3((ns + 255u) / 256u), dim3(256), 0, c->stream, c->d_grid, (int32_t)c->bd[0],
+                     (int32_t)c->bd[1], (int32_t)c->bd[2], (int32_t)c->sd[0], (int32_t)c->sd[1], (int32_t)c->sd[2],
+                     c->d_super_bm);
+  PX_HIP_CHECK(c, hipGetLastError());
   c->masks_dirty = false;
   return PX_OK;
 }
 
 int px_refresh_masks(px_ctx* c) { return c->masks_dirty ? px_launch_masks(c) : PX_OK; }
 
+// LDS of march()'s tables: 8 glass words, then the super-cell bits when they
+// fit in 64 KB (else the kernels read the global copy).
+static size_t table_lds(const px_ctx* c, bool* sup_lds) {
+  size_t full = (8u + (c->nsuper + 31u) / 32u) * 4u;
+  *sup_lds = full <= 64u * 1024u;
+  return *sup_lds ? full : 32u;
+}
+
+template <int KIND>
+static void launch_trace_kind(px_ctx* c, const PxWorldView& w, const px_ray* d_rays, uint32_t n, px_hit* d_out) {
+  bool sup_lds;
+  size_t lds = table_lds(c, &sup_lds);
+  dim3 grid((n + 255u) / 256u), block(256);
+  if (sup_lds) hipLaunchKernelGGL((k_trace<KIND, true>), grid, block, lds, c->stream, w, d_rays, n, d_out);
+  else hipLaunchKernelGGL((k_trace<KIND, false>), grid, block, lds, c->stream, w, d_rays, n, d_out);
+}
+
 int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, px_hit* d_out) {
   if (n == 0) return PX_OK;
   int rc = px_refresh_masks(c);
   if (rc) return rc;
-  if (c->tune.variant == 1u && (kind == PX_RAY_PRIMARY || kind == PX_RAY_SHADOW) && px_march_fits(c))
-    return px_launch_march(c, kind == PX_RAY_SHADOW, d_rays, n, d_out);
   PxWorldView w = px_view(c);
-  dim3 grid((n + 255u) / 256u), block(256);
   PxProfScope prof(c, PX_PROF_TRACE);
   switch (kind) {
-    case PX_RAY_PRIMARY: hipLaunchKernelGGL(k_trace<0>, grid, block, 0, c->stream, w, d_rays, n, d_out); break;
-    case PX_RAY_SHADOW: hipLaunchKernelGGL(k_trace<1>, grid, block, 0, c->stream, w, d_rays, n, d_out); break;
-    case PX_RAY_SHOOT: hipLaunchKernelGGL(k_trace<2>, grid, block, 0, c->stream, w, d_rays, n, d_out); break;
-    case PX_RAY_SEGMENT: hipLaunchKernelGGL(k_trace<3>, grid, block, 0, c->stream, w, d_rays, n, d_out); break;
-    default: hipLaunchKernelGGL(k_trace<-1>, grid, block, 0, c->stream, w, d_rays, n, d_out); break;
+    case PX_RAY_PRIMARY: launch_trace_kind<0>(c, w, d_rays, n, d_out); break;
+    case PX_RAY_SHADOW: launch_trace_kind<1>(c, w, d_rays, n, d_out); break;
+    case PX_RAY_SHOOT: launch_trace_kind<2>(c, w, d_rays, n, d_out); break;
+    case PX_RAY_SEGMENT: launch_trace_kind<3>(c, w, d_rays, n, d_out); break;
+    default: launch_trace_kind<-1>(c, w, d_rays, n, d_out); break;
   }
   PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
@@ -594,7 +734,9 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
   }
   int rc = px_refresh_masks(c);
   if (rc) return rc;
-  size_t need = (size_t)n * c->shadow.slots;
+  // shadow-ray queue: one region of 64 * slots rays per wavefront of primaries
+  const uint32_t n_regions = (n + 63u) / 64u;
+  const size_t need = (size_t)n_regions * 64u * c->shadow.slots;
   if (need > 0xFFFFFFFFull) return px_fail(c, PX_ERR_CAPACITY, "px_trace_frame_device", "too many shadow slots");
   if (need > c->shadow_cap) {
     if (c->d_shadow_rays) hipFree(c->d_shadow_rays);
@@ -603,33 +745,41 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
     PX_HIP_CHECK(c, hipMalloc(&c->d_shadow_rays, need * sizeof(px_ray)));
     c->shadow_cap = need;
   }
-  if (c->tune.variant == 1u && px_march_fits(c)) {
-    rc = px_launch_frame_persistent(c, d_primary, n, d_prim, d_shadow);
-    if (rc) return rc;
-    if (n_cast) {
-      PX_HIP_CHECK(c, hipMemcpyAsync(n_cast, c->d_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-      PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    }
-    return PX_OK;
+  if (n_regions > c->wave_count_cap) {
+    if (c->d_wave_count) hipFree(c->d_wave_count);
+    c->d_wave_count = nullptr;
+    c->wave_count_cap = 0;
+    PX_HIP_CHECK(c, hipMalloc(&c->d_wave_count, (size_t)n_regions * sizeof(uint32_t)));
+    c->wave_count_cap = n_regions;
   }
-  // a fill kernel, not hipMemsetAsync, so the launch sequence can be captured in a graph
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, c->d_counters, 1u, 0u);
   PxWorldView w = px_view(c);
+  bool sup_lds;
+  const size_t lds = table_lds(c, &sup_lds);
   {
     PxProfScope prof(c, PX_PROF_PRIMARY);
-    hipLaunchKernelGGL(k_frame_primary, dim3((n + 255u) / 256u), dim3(256), 0, c->stream, w, c->shadow, d_primary, n,
-                       d_prim, d_shadow, c->d_shadow_rays, c->d_counters);
+    dim3 grid((n + 255u) / 256u), block(256);
+    if (sup_lds)
+      hipLaunchKernelGGL(k_frame_primary<true>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
+                         d_shadow, c->d_shadow_rays, c->d_wave_count);
+    else
+      hipLaunchKernelGGL(k_frame_primary<false>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
+                         d_shadow, c->d_shadow_rays, c->d_wave_count);
   }
   PX_HIP_CHECK(c, hipGetLastError());
-  uint32_t blocks = (uint32_t)((need + 255u) / 256u);
-  if (blocks > 8192u) blocks = 8192u;
+  const uint32_t blocks = (n_regions + 3u) / 4u;
   {
     PxProfScope prof(c, PX_PROF_SHADOW);
-    hipLaunchKernelGGL(k_trace_shadow_queue, dim3(blocks), dim3(256), 0, c->stream, w, c->d_shadow_rays,
-                       c->d_counters, d_shadow);
+    if (sup_lds)
+      hipLaunchKernelGGL(k_trace_shadow_queue<true>, dim3(blocks), dim3(256), lds, c->stream, w, c->d_shadow_rays,
+                         c->d_wave_count, n_regions, c->shadow.slots, d_shadow);
+    else
+      hipLaunchKernelGGL(k_trace_shadow_queue<false>, dim3(blocks), dim3(256), lds, c->stream, w, c->d_shadow_rays,
+                         c->d_wave_count, n_regions, c->shadow.slots, d_shadow);
   }
   PX_HIP_CHECK(c, hipGetLastError());
   if (n_cast) {
+    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, c->stream, c->d_wave_count, n_regions, c->d_counters);
+    PX_HIP_CHECK(c, hipGetLastError());
     PX_HIP_CHECK(c, hipMemcpyAsync(n_cast, c->d_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   }

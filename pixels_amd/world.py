@@ -97,15 +97,6 @@ class GpuWorld:
     def restore(self) -> None:
         self.check(self.lib.px_restore(self.ctx))
 
-    def set_tuning(self, variant: int, blocks: int = 0, block: int = 0, chunk: int = 0, refill_min: int = 0) -> None:
-        """Traversal launch shape: variant 0 nested loops, 1 persistent wavefronts."""
-        self.check(self.lib.px_set_tuning(self.ctx, variant, blocks, block, chunk, refill_min))
-
-    def get_tuning(self) -> dict:
-        out = np.zeros(5, np.uint32)
-        self.check(self.lib.px_get_tuning(self.ctx, N.ptr(out)))
-        return dict(zip(("variant", "blocks", "block", "chunk", "refill_min"), (int(v) for v in out)))
-
     def profile_begin(self) -> None:
         self.check(self.lib.px_profile_begin(self.ctx))
 
