@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Grays/s primary+shadow @1280x352x960 bit-exact; island-pass ms; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 BYTES_IN, BYTES_OUT, BYTES_STEP = 28, 13, 4
+COPY_BYTES = 256 << 20
 
 
 def log(*a):
@@ -95,6 +96,13 @@ def main() -> int:
     prof = gw.profile_end()
     frame_ms = ev0.elapsed_time(ev1) / args.steps
     log("timed %d frames: %.3f ms/frame" % (args.steps, frame_ms))
+    # a plain device copy of COPY_BYTES: under rocprofv3 --pmc its FETCH_SIZE /
+    # WRITE_SIZE give the scale of those counters against known bytes
+    cal_src = torch.zeros(COPY_BYTES, dtype=torch.uint8, device=dev)
+    cal_dst = torch.empty_like(cal_src)
+    cal_dst.copy_(cal_src)
+    torch.cuda.synchronize()
+    del cal_src, cal_dst
 
     prim = d_prim.cpu().numpy().view(N.HIT_DTYPE)
     sh = d_sh.cpu().numpy().view(N.HIT_DTYPE)

@@ -116,7 +116,8 @@ typedef struct px_world_desc {
   const uint32_t* holes;   /* free bricks inside [0, pool_used) (VoxelWorld.freeBricks) */
   uint32_t n_holes;
   const float* mats;       /* 256 * 20 floats, packMaterials() layout */
-  uint32_t pool_capacity;  /* 0 = min(limit, max(used + 98304, 2 * used)), GpuWorld.ts:60 */
+  uint32_t pool_capacity;  /* 0 = min(limit, max(used + 98304, 2 * used)), GpuWorld.ts:60;
+                              limit = 4M bricks (4 GiB of pool) */
   const px_limits* limits; /* NULL = the reference LIMITS */
   float voxel_size;        /* metres per voxel, U.dims.w */
   int32_t device;          /* HIP device ordinal */

@@ -124,8 +124,10 @@ int px_create(const px_world_desc* d, px_ctx** out) {
   c->nsuper = c->sd[0] * c->sd[1] * c->sd[2];
   c->vox_total = (size_t)vox64;
   uint32_t used = d->pool_used;
-  // GpuWorld.ts:57-60 (the storage-binding limit becomes the u32 word-index limit)
-  uint64_t max_bricks = (1ull << 32) / 512ull;
+  // GpuWorld.ts:57-60; the storage-binding limit becomes 4 GiB of pool (4M
+  // bricks of 1 KiB), so every pool byte offset fits the 32-bit offsets the
+  // traversal kernels address the pool with.
+  uint64_t max_bricks = (1ull << 32) / 1024ull;
   uint64_t cap = d->pool_capacity ? d->pool_capacity : std::max<uint64_t>((uint64_t)used + 98304ull, 2ull * used);
   if (cap > max_bricks) cap = max_bricks;
   if (used > cap) { release(c); return px_fail(nullptr, PX_ERR_CAPACITY, "px_create", "scene needs more bricks than the pool holds"); }

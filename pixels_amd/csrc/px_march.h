@@ -1,0 +1,291 @@
+// px_march.h - march() of src/render/shaders/raymarch.wgsl:44-162 (+ safeDir
+// :22-24, glassSurface :26-38) for gfx950, one ray per lane.
+//
+// The loops, iteration caps and FP32 operations are the WGSL's: three nested
+// DDAs (32^3 super cell, 8^3 brick, voxel), at most 256 / 14 / 25 turns, a
+// uniform opaque brick ends the ray at the brick's entry voxel.  What this
+// version changes is the cost of a turn on CDNA4:
+//
+//  * super-cell occupancy is a bit in LDS, brick occupancy a bit of the super
+//    cell's 64-bit mask (one 8-byte load on entering the super cell), so only
+//    non-empty bricks read the grid and only their voxels read the pool;
+//  * every DDA step is branch-free (compare, select), with
+//    tMax += |inv| * s written as one fma: |inv| * s is exact for the
+//    power-of-two scales s = 1, 8, 32, so the sum rounds once either way;
+//  * faces are carried as an axis code (0..2, 3 = zero normal) and turned
+//    into the reference's face id only at the hit, brick and voxel
+//    coordinates are local to their parent cell, and the glass state is one
+//    word - all to keep the live state small enough for more wavefronts per
+//    SIMD.
+#pragma once
+
+#include "px_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace px {
+
+struct MarchOut {
+  uint32_t key;
+  float t;
+  uint32_t steps;
+  uint32_t face;   // 0..6, 0xFF miss
+  uint32_t value;  // 16-bit voxel value of the hit
+};
+
+PX_DEV uint32_t key_of(const PxWorldView& w, int32_t x, int32_t y, int32_t z) {
+  return (uint32_t)x + (uint32_t)w.dims[0] * ((uint32_t)y + (uint32_t)w.dims[1] * (uint32_t)z);
+}
+
+// face of the normal vec3f(-stpF[axis]) set when stepping along `axis`
+PX_DEV uint32_t step_face(int axis, int32_t stp) { return (uint32_t)(axis * 2) + (stp > 0 ? 1u : 0u); }
+
+// Per-block tables of march(): super-cell occupancy bits (count != 0) and the
+// 256-bit glass table (material D.y > 0).  `sup` is LDS when the world's bits
+// fit, else the global copy.
+struct MarchTables {
+  const uint32_t* sup;
+  const uint32_t* glass;
+};
+
+PX_DEV bool bit_of(const uint32_t* bits, uint32_t i) { return ((bits[i >> 5] >> (i & 31u)) & 1u) != 0u; }
+
+// Block-cooperative: every thread of the block calls it before any early exit.
+template <bool SUP_LDS>
+PX_DEV MarchTables load_march_tables(const PxWorldView& w, uint32_t* s_mem) {
+  uint32_t* s_glass = s_mem;
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < 256u; base += blockDim.x) {
+    uint32_t m = base + threadIdx.x;
+    uint64_t b = __ballot(m < 256u && w.mats[m * 20u + 13u] > 0.0f);
+    uint32_t word = (base + wave * 64u) >> 5;
+    if (lane == 0u && word < 8u) {
+      s_glass[word] = (uint32_t)b;
+      s_glass[word + 1u] = (uint32_t)(b >> 32);
+    }
+  }
+  MarchTables T;
+  T.glass = s_glass;
+  if (SUP_LDS) {
+    uint32_t* s_sup = s_mem + 8;
+    const uint32_t sw = ((uint32_t)(w.sd[0] * w.sd[1] * w.sd[2]) + 31u) / 32u;
+    for (uint32_t i = threadIdx.x; i < sw; i += blockDim.x) s_sup[i] = w.super_mask[i];
+    T.sup = s_sup;
+  } else {
+    T.sup = w.super_mask;
+  }
+  __syncthreads();
+  return T;
+}
+
+// glassSurface (raymarch.wgsl:26-38): Fresnel term for camera rays, then the
+// tinted absorption.  dot(n, rd) with an axis-aligned unit normal is
+// +-rd[axis] (0 for the zero normal).
+PX_DEV void glass_surface(const float* __restrict__ mats, float trans[3], uint32_t id, uint32_t axis,
+                          const float rd[3], bool shadow) {
+  const float* A = mats + id * 20u;
+  float g = mats[id * 20u + 13u];
+  if (!shadow) {
+    float cosT = axis < 3u ? fabsf(axis == 0u ? rd[0] : (axis == 1u ? rd[1] : rd[2])) : 0.0f;
+    float F = 0.04f + 0.96f * px_pow5(1.0f - cosT);
+    for (int a = 0; a < 3; a++) trans[a] = trans[a] * (1.0f - F);
+  }
+  for (int a = 0; a < 3; a++) {
+    float m = 1.0f * (1.0f - 0.6f) + A[a] * 0.6f;
+    trans[a] = trans[a] * (m * g);
+  }
+}
+
+// face id of an axis code (3 = zero normal)
+PX_DEV uint32_t face_of_axis(uint32_t axis, const float rd[3]) {
+  if (axis >= 3u) return 6u;
+  const float r = axis == 0u ? rd[0] : (axis == 1u ? rd[1] : rd[2]);
+  return axis * 2u + (r > 0.0f ? 1u : 0u);
+}
+
+// Every DDA step below is the WGSL's if / else-if chain written as selects:
+// axis 0 when m0 < m1 && m0 < m2, else axis 1 when m1 < m2, else axis 2
+// (ties go to the later axis); that axis's tMax advances by |inv| * scale
+// and its cell coordinate by stp.
+template <bool SHADOW>
+PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro[3], const float rdIn[3],
+                      float maxT) {
+  MarchOut o;
+  o.key = PX_NONE;
+  o.t = 0.0f;
+  o.face = 0xFFu;
+  o.value = 0u;
+  o.steps = 0u;
+  float rd[3], inv[3], tmn[3], tmx[3];
+  for (int a = 0; a < 3; a++) {
+    rd[a] = fabsf(rdIn[a]) < 1e-6f ? px_sign(rdIn[a] + 1e-30f) * 1e-6f : rdIn[a];
+    inv[a] = 1.0f / rd[a];
+    float t0 = (0.0f - ro[a]) * inv[a];
+    float t1 = (w.fdims[a] - ro[a]) * inv[a];
+    tmn[a] = fminf(t0, t1);
+    tmx[a] = fmaxf(t0, t1);
+  }
+  float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
+  const float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
+  if (t >= tExit) return o;
+
+  int32_t stp[3];
+  float pos[3];
+  for (int a = 0; a < 3; a++) {
+    stp[a] = rd[a] > 0.0f ? 1 : (rd[a] < 0.0f ? -1 : 0);
+    pos[a] = rd[a] >= 0.0f ? 1.0f : 0.0f;
+  }
+  uint32_t axisS = 3u;  // normal as an axis code
+  if (t > 0.0f) {
+    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) axisS = 0u;
+    else if (tmn[1] >= tmn[2]) axisS = 1u;
+    else axisS = 2u;
+  }
+  int32_t sc[3];
+  float tMaxS[3];
+  for (int a = 0; a < 3; a++) {
+    float p0 = ro[a] + rd[a] * (t + 1e-3f);
+    sc[a] = px_clampi(px_f2i(floorf(p0 / 32.0f)), 0, w.sd[a] - 1);
+    tMaxS[a] = (((float)sc[a] + pos[a]) * 32.0f - ro[a]) * inv[a];
+  }
+  float trans[3] = {1.0f, 1.0f, 1.0f};
+  uint32_t glassState = 0u;  // 0x100 | id while inside glass `id`, else 0
+  uint32_t steps = 0u;
+
+  for (int i = 0; i < 256; i++) {
+    steps++;
+    const uint32_t sidx = (uint32_t)(sc[0] + w.sd[0] * (sc[1] + w.sd[1] * sc[2]));
+    if (bit_of(T.sup, sidx)) {
+      const uint64_t bm = w.super_bm[sidx];
+      int32_t hiL[3], lb[3];
+      float tMaxB[3];
+      for (int a = 0; a < 3; a++) {
+        const int32_t sLo = sc[a] * 4;
+        const int32_t sHi = min(sLo + 3, w.bd[a] - 1);
+        hiL[a] = sHi - sLo;
+        const int32_t bc = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo, sHi);
+        lb[a] = bc - sLo;
+        tMaxB[a] = (((float)bc + pos[a]) * 8.0f - ro[a]) * inv[a];
+      }
+      float tb = t;
+      uint32_t axisB = axisS;
+      for (int j = 0; j < 14; j++) {
+        steps++;
+        if ((bm >> (uint32_t)(lb[0] + 4 * (lb[1] + 4 * lb[2]))) & 1ull) {
+          int32_t lv[3];
+          float tMaxV[3];
+          for (int a = 0; a < 3; a++) {
+            const int32_t lo = (sc[a] * 4 + lb[a]) * 8;
+            const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo, lo + 7);
+            lv[a] = vc - lo;
+            tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
+          }
+          const uint32_t bcx = (uint32_t)(sc[0] * 4 + lb[0]), bcy = (uint32_t)(sc[1] * 4 + lb[1]),
+                         bcz = (uint32_t)(sc[2] * 4 + lb[2]);
+          const uint32_t g = w.grid[bcx + (uint32_t)w.bd[0] * (bcy + (uint32_t)w.bd[1] * bcz)];
+          const bool uni = (g & PX_UNIFORM_BIT) != 0u;
+          if (uni && !bit_of(T.glass, g & 0xffu)) {
+            o.key = key_of(w, (int32_t)bcx * 8 + lv[0], (int32_t)bcy * 8 + lv[1], (int32_t)bcz * 8 + lv[2]);
+            o.t = px_canon0(tb);
+            o.face = face_of_axis(axisB, rd);
+            o.value = g & 0xffffu;
+            o.steps = steps;
+            return o;
+          }
+          // byte offset of the brick in the pool (pool bricks only; the pool is < 4 GiB)
+          const uint32_t bbase = (g - 1u) * 1024u;
+          float tv = tb;
+          uint32_t axisV = axisB;
+          const float d0 = fabsf(inv[0]), d1 = fabsf(inv[1]), d2 = fabsf(inv[2]);
+          for (int k = 0; k < 25; k++) {
+            steps++;
+            const uint32_t li = (uint32_t)lv[0] | ((uint32_t)lv[1] << 3) | ((uint32_t)lv[2] << 6);
+            uint32_t v;
+            if (uni) v = g & 0xffffu;
+            else v = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(w.pool) + (uint64_t)(bbase + li * 2u));
+            if (v != 0u) {
+              const uint32_t id = v & 0xffu;
+              bool hit = true;
+              if (bit_of(T.glass, id)) {
+                if (glassState != (0x100u | id)) glass_surface(w.mats, trans, id, axisV, rd, SHADOW);
+                glassState = 0x100u | id;
+                hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
+              }
+              if (hit) {
+                o.key = key_of(w, (int32_t)bcx * 8 + lv[0], (int32_t)bcy * 8 + lv[1], (int32_t)bcz * 8 + lv[2]);
+                o.t = px_canon0(tv);
+                o.face = face_of_axis(axisV, rd);
+                o.value = v;
+                o.steps = steps;
+                return o;
+              }
+            } else {
+              glassState = 0u;
+            }
+            const float m0 = tMaxV[0], m1 = tMaxV[1], m2 = tMaxV[2];
+            const bool a0 = m0 < m1 && m0 < m2;
+            const bool b12 = m1 < m2;
+            tv = a0 ? m0 : (b12 ? m1 : m2);
+            axisV = a0 ? 0u : (b12 ? 1u : 2u);
+            tMaxV[0] = a0 ? m0 + d0 : m0;
+            tMaxV[1] = (!a0 && b12) ? m1 + d1 : m1;
+            tMaxV[2] = (!a0 && !b12) ? m2 + d2 : m2;
+            lv[0] = a0 ? lv[0] + stp[0] : lv[0];
+            lv[1] = (!a0 && b12) ? lv[1] + stp[1] : lv[1];
+            lv[2] = (!a0 && !b12) ? lv[2] + stp[2] : lv[2];
+            if (tv > tExit) {
+              o.steps = steps;
+              return o;
+            }
+            if ((uint32_t)(lv[0] | lv[1] | lv[2]) > 7u) break;
+          }
+        } else {
+          glassState = 0u;
+        }
+        {
+          const float m0 = tMaxB[0], m1 = tMaxB[1], m2 = tMaxB[2];
+          const bool a0 = m0 < m1 && m0 < m2;
+          const bool b12 = m1 < m2;
+          tb = a0 ? m0 : (b12 ? m1 : m2);
+          axisB = a0 ? 0u : (b12 ? 1u : 2u);
+          tMaxB[0] = a0 ? __builtin_fmaf(fabsf(inv[0]), 8.0f, m0) : m0;
+          tMaxB[1] = (!a0 && b12) ? __builtin_fmaf(fabsf(inv[1]), 8.0f, m1) : m1;
+          tMaxB[2] = (!a0 && !b12) ? __builtin_fmaf(fabsf(inv[2]), 8.0f, m2) : m2;
+          lb[0] = a0 ? lb[0] + stp[0] : lb[0];
+          lb[1] = (!a0 && b12) ? lb[1] + stp[1] : lb[1];
+          lb[2] = (!a0 && !b12) ? lb[2] + stp[2] : lb[2];
+        }
+        if (tb > tExit) {
+          o.steps = steps;
+          return o;
+        }
+        if ((uint32_t)lb[0] > (uint32_t)hiL[0] || (uint32_t)lb[1] > (uint32_t)hiL[1] ||
+            (uint32_t)lb[2] > (uint32_t)hiL[2])
+          break;
+      }
+    } else {
+      glassState = 0u;
+    }
+    {
+      const float m0 = tMaxS[0], m1 = tMaxS[1], m2 = tMaxS[2];
+      const bool a0 = m0 < m1 && m0 < m2;
+      const bool b12 = m1 < m2;
+      t = a0 ? m0 : (b12 ? m1 : m2);
+      axisS = a0 ? 0u : (b12 ? 1u : 2u);
+      tMaxS[0] = a0 ? __builtin_fmaf(fabsf(inv[0]), 32.0f, m0) : m0;
+      tMaxS[1] = (!a0 && b12) ? __builtin_fmaf(fabsf(inv[1]), 32.0f, m1) : m1;
+      tMaxS[2] = (!a0 && !b12) ? __builtin_fmaf(fabsf(inv[2]), 32.0f, m2) : m2;
+      sc[0] = a0 ? sc[0] + stp[0] : sc[0];
+      sc[1] = (!a0 && b12) ? sc[1] + stp[1] : sc[1];
+      sc[2] = (!a0 && !b12) ? sc[2] + stp[2] : sc[2];
+    }
+    if (t > tExit || (uint32_t)sc[0] >= (uint32_t)w.sd[0] || (uint32_t)sc[1] >= (uint32_t)w.sd[1] ||
+        (uint32_t)sc[2] >= (uint32_t)w.sd[2])
+      break;
+  }
+  o.steps = steps;
+  return o;
+}
+
+}  // namespace px

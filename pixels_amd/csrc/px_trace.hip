@@ -15,263 +15,20 @@
 // fixed as: pow(x,5) = ((x*x)*(x*x))*x, mix(a,b,t) = a*(1-t) + b*t,
 // dot = (x*x' + y*y') + z*z', length = sqrt(dot), zero t reported as +0.
 #include "px_internal.h"
+#include "px_march.h"
 
 #pragma clang fp contract(off)
 
 namespace {
 
-struct MarchOut {
-  uint32_t key;
-  float t;
-  uint32_t steps;
-  uint32_t face;   // 0..6, 0xFF miss
-  uint32_t value;  // 16-bit voxel value of the hit
-};
+// Register budget of the march kernels: 6 wavefronts per SIMD (<= 80 VGPRs)
+// fits the live state of the three nested DDAs without spilling.
+#define PX_MARCH_WAVES __attribute__((amdgpu_waves_per_eu(6)))
 
-PX_DEV uint32_t key_of(const PxWorldView& w, int32_t x, int32_t y, int32_t z) {
-  return (uint32_t)x + (uint32_t)w.dims[0] * ((uint32_t)y + (uint32_t)w.dims[1] * (uint32_t)z);
-}
-
-// face of the normal vec3f(-stpF[axis]) set when stepping along `axis`
-PX_DEV uint32_t step_face(int axis, int32_t stp) { return (uint32_t)(axis * 2) + (stp > 0 ? 1u : 0u); }
-
-PX_DEV void glass_surface(const float* __restrict__ mats, float trans[3], uint32_t id, uint32_t face,
-                          const float rd[3], bool shadow) {
-  const float* A = mats + id * 20u;
-  float g = mats[id * 20u + 13u];
-  if (!shadow) {
-    // dot(n, rd) with an axis-aligned unit normal (or zero) is +-rd[axis] (or 0)
-    float cosT = face < 6u ? fabsf(rd[face >> 1]) : 0.0f;
-    float F = 0.04f + 0.96f * px_pow5(1.0f - cosT);
-    for (int a = 0; a < 3; a++) trans[a] = trans[a] * (1.0f - F);
-  }
-  for (int a = 0; a < 3; a++) {
-    float m = 1.0f * (1.0f - 0.6f) + A[a] * 0.6f;
-    trans[a] = trans[a] * (m * g);
-  }
-}
-
-// Per-block tables of march(): super-cell occupancy bits (count != 0) and the
-// 256-bit glass table (material D.y > 0).  `sup` is LDS when the world's bits
-// fit, else the global copy.
-struct MarchTables {
-  const uint32_t* sup;
-  const uint32_t* glass;
-};
-
-PX_DEV bool bit_of(const uint32_t* bits, uint32_t i) { return ((bits[i >> 5] >> (i & 31u)) & 1u) != 0u; }
-
-// Block-cooperative: every thread of the block calls it before any early exit.
-template <bool SUP_LDS>
-PX_DEV MarchTables load_march_tables(const PxWorldView& w, uint32_t* s_mem) {
-  uint32_t* s_glass = s_mem;
-  const uint32_t lane = __lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  for (uint32_t base = 0; base < 256u; base += blockDim.x) {
-    uint32_t m = base + threadIdx.x;
-    uint64_t b = __ballot(m < 256u && w.mats[m * 20u + 13u] > 0.0f);
-    uint32_t word = (base + wave * 64u) >> 5;
-    if (lane == 0u && word < 8u) {
-      s_glass[word] = (uint32_t)b;
-      s_glass[word + 1u] = (uint32_t)(b >> 32);
-    }
-  }
-  MarchTables T;
-  T.glass = s_glass;
-  if (SUP_LDS) {
-    uint32_t* s_sup = s_mem + 8;
-    const uint32_t sw = ((uint32_t)(w.sd[0] * w.sd[1] * w.sd[2]) + 31u) / 32u;
-    for (uint32_t i = threadIdx.x; i < sw; i += blockDim.x) s_sup[i] = w.super_mask[i];
-    T.sup = s_sup;
-  } else {
-    T.sup = w.super_mask;
-  }
-  __syncthreads();
-  return T;
-}
-
-// march() of raymarch.wgsl:44-162, one ray per lane.  Same loops, caps and
-// FP32 operations as the WGSL; what differs is where occupancy comes from
-// (super bit in LDS, brick bit from the super cell's 64-bit mask, the grid
-// word only for non-empty bricks) and that each DDA step is branch-free.
-PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro[3], const float rdIn[3],
-                      float maxT, bool shadow) {
-  const float* __restrict__ mats = w.mats;
-  MarchOut o;
-  o.key = PX_NONE;
-  o.t = 0.0f;
-  o.face = 0xFFu;
-  o.value = 0u;
-  o.steps = 0u;
-  uint32_t steps = 0;
-  float trans[3] = {1.0f, 1.0f, 1.0f};
-  float rd[3], inv[3], tmn[3], tmx[3];
-  for (int a = 0; a < 3; a++) {
-    rd[a] = fabsf(rdIn[a]) < 1e-6f ? px_sign(rdIn[a] + 1e-30f) * 1e-6f : rdIn[a];
-    inv[a] = 1.0f / rd[a];
-    float t0 = (0.0f - ro[a]) * inv[a];
-    float t1 = (w.fdims[a] - ro[a]) * inv[a];
-    tmn[a] = fminf(t0, t1);
-    tmx[a] = fmaxf(t0, t1);
-  }
-  float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
-  const float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
-  if (t >= tExit) return o;
-
-  int32_t stp[3];
-  float pos[3], tDeltaS[3], tDeltaB[3], tDeltaV[3];
-  uint32_t fc[3];
-  for (int a = 0; a < 3; a++) {
-    stp[a] = rd[a] > 0.0f ? 1 : (rd[a] < 0.0f ? -1 : 0);
-    pos[a] = rd[a] >= 0.0f ? 1.0f : 0.0f;
-    fc[a] = step_face(a, stp[a]);
-    tDeltaS[a] = fabsf(inv[a]) * 32.0f;
-    tDeltaB[a] = fabsf(inv[a]) * 8.0f;
-    tDeltaV[a] = fabsf(inv[a]);
-  }
-  uint32_t normal = 6u;
-  if (t > 0.0f) {
-    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) normal = fc[0];
-    else if (tmn[1] >= tmn[2]) normal = fc[1];
-    else normal = fc[2];
-  }
-  int32_t sc[3];
-  float tMaxS[3];
-  for (int a = 0; a < 3; a++) {
-    float p0 = ro[a] + rd[a] * (t + 1e-3f);
-    sc[a] = px_clampi(px_f2i(floorf(p0 / 32.0f)), 0, w.sd[a] - 1);
-    tMaxS[a] = (((float)sc[a] + pos[a]) * 32.0f - ro[a]) * inv[a];
-  }
-  bool inGlass = false;
-  uint32_t glassId = 0u;
-
-  for (int i = 0; i < 256; i++) {
-    steps++;
-    const uint32_t sidx = (uint32_t)(sc[0] + w.sd[0] * (sc[1] + w.sd[1] * sc[2]));
-    if (bit_of(T.sup, sidx)) {
-      const uint64_t bm = w.super_bm[sidx];
-      int32_t sLo[3], hiL[3], lb[3];
-      float tMaxB[3];
-      for (int a = 0; a < 3; a++) {
-        sLo[a] = sc[a] * 4;
-        const int32_t sHi = min(sLo[a] + 3, w.bd[a] - 1);
-        hiL[a] = sHi - sLo[a];
-        const int32_t bc = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo[a], sHi);
-        lb[a] = bc - sLo[a];
-        tMaxB[a] = (((float)bc + pos[a]) * 8.0f - ro[a]) * inv[a];
-      }
-      float tb = t;
-      uint32_t nb = normal;
-      for (int j = 0; j < 14; j++) {
-        steps++;
-        if ((bm >> (uint32_t)(lb[0] + 4 * (lb[1] + 4 * lb[2]))) & 1ull) {
-          int32_t lo[3], lv[3];
-          float tMaxV[3];
-          for (int a = 0; a < 3; a++) {
-            lo[a] = (sLo[a] + lb[a]) * 8;
-            const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo[a], lo[a] + 7);
-            lv[a] = vc - lo[a];
-            tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
-          }
-          const uint32_t g = w.grid[(uint32_t)((lo[0] >> 3) + w.bd[0] * ((lo[1] >> 3) + w.bd[1] * (lo[2] >> 3)))];
-          const bool uni = (g & PX_UNIFORM_BIT) != 0u;
-          if (uni && !bit_of(T.glass, g & 0xffu)) {
-            o.key = key_of(w, lo[0] + lv[0], lo[1] + lv[1], lo[2] + lv[2]);
-            o.t = px_canon0(tb);
-            o.face = nb;
-            o.value = g & 0xffffu;
-            o.steps = steps;
-            return o;
-          }
-          const uint32_t* __restrict__ pb = w.pool + (size_t)(g - 1u) * 256u;
-          float tv = tb;
-          uint32_t n = nb;
-          for (int k = 0; k < 25; k++) {
-            steps++;
-            const uint32_t li = (uint32_t)(lv[0] | (lv[1] << 3) | (lv[2] << 6));
-            uint32_t v;
-            if (uni) v = g & 0xffffu;
-            else v = (pb[li >> 1] >> ((li & 1u) * 16u)) & 0xffffu;
-            if (v != 0u) {
-              const uint32_t id = v & 0xffu;
-              bool hit = true;
-              if (bit_of(T.glass, id)) {
-                if (!inGlass || id != glassId) glass_surface(mats, trans, id, n, rd, shadow);
-                inGlass = true;
-                glassId = id;
-                hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
-              }
-              if (hit) {
-                o.key = key_of(w, lo[0] + lv[0], lo[1] + lv[1], lo[2] + lv[2]);
-                o.t = px_canon0(tv);
-                o.face = n;
-                o.value = v;
-                o.steps = steps;
-                return o;
-              }
-            } else {
-              inGlass = false;
-            }
-            const bool a0 = tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2];
-            const bool a1 = !a0 && tMaxV[1] < tMaxV[2];
-            const bool a2 = !a0 && !a1;
-            tv = a0 ? tMaxV[0] : (a1 ? tMaxV[1] : tMaxV[2]);
-            n = a0 ? fc[0] : (a1 ? fc[1] : fc[2]);
-            tMaxV[0] = a0 ? tMaxV[0] + tDeltaV[0] : tMaxV[0];
-            tMaxV[1] = a1 ? tMaxV[1] + tDeltaV[1] : tMaxV[1];
-            tMaxV[2] = a2 ? tMaxV[2] + tDeltaV[2] : tMaxV[2];
-            lv[0] += a0 ? stp[0] : 0;
-            lv[1] += a1 ? stp[1] : 0;
-            lv[2] += a2 ? stp[2] : 0;
-            if (tv > tExit) {
-              o.steps = steps;
-              return o;
-            }
-            if ((uint32_t)(lv[0] | lv[1] | lv[2]) > 7u) break;
-          }
-        } else {
-          inGlass = false;
-        }
-        const bool a0 = tMaxB[0] < tMaxB[1] && tMaxB[0] < tMaxB[2];
-        const bool a1 = !a0 && tMaxB[1] < tMaxB[2];
-        const bool a2 = !a0 && !a1;
-        tb = a0 ? tMaxB[0] : (a1 ? tMaxB[1] : tMaxB[2]);
-        nb = a0 ? fc[0] : (a1 ? fc[1] : fc[2]);
-        tMaxB[0] = a0 ? tMaxB[0] + tDeltaB[0] : tMaxB[0];
-        tMaxB[1] = a1 ? tMaxB[1] + tDeltaB[1] : tMaxB[1];
-        tMaxB[2] = a2 ? tMaxB[2] + tDeltaB[2] : tMaxB[2];
-        lb[0] += a0 ? stp[0] : 0;
-        lb[1] += a1 ? stp[1] : 0;
-        lb[2] += a2 ? stp[2] : 0;
-        if (tb > tExit) {
-          o.steps = steps;
-          return o;
-        }
-        if ((uint32_t)lb[0] > (uint32_t)hiL[0] || (uint32_t)lb[1] > (uint32_t)hiL[1] ||
-            (uint32_t)lb[2] > (uint32_t)hiL[2])
-          break;
-      }
-    } else {
-      inGlass = false;
-    }
-    const bool a0 = tMaxS[0] < tMaxS[1] && tMaxS[0] < tMaxS[2];
-    const bool a1 = !a0 && tMaxS[1] < tMaxS[2];
-    const bool a2 = !a0 && !a1;
-    t = a0 ? tMaxS[0] : (a1 ? tMaxS[1] : tMaxS[2]);
-    normal = a0 ? fc[0] : (a1 ? fc[1] : fc[2]);
-    tMaxS[0] = a0 ? tMaxS[0] + tDeltaS[0] : tMaxS[0];
-    tMaxS[1] = a1 ? tMaxS[1] + tDeltaS[1] : tMaxS[1];
-    tMaxS[2] = a2 ? tMaxS[2] + tDeltaS[2] : tMaxS[2];
-    sc[0] += a0 ? stp[0] : 0;
-    sc[1] += a1 ? stp[1] : 0;
-    sc[2] += a2 ? stp[2] : 0;
-    if (t > tExit || (uint32_t)sc[0] >= (uint32_t)w.sd[0] || (uint32_t)sc[1] >= (uint32_t)w.sd[1] ||
-        (uint32_t)sc[2] >= (uint32_t)w.sd[2])
-      break;
-  }
-  o.steps = steps;
-  return o;
-}
+using px::MarchOut;
+using px::MarchTables;
+using px::key_of;
+using px::step_face;
 
 PX_DEV uint32_t read_voxel(const PxWorldView& w, int32_t x, int32_t y, int32_t z) {
   if (x < 0 || y < 0 || z < 0 || x >= w.dims[0] || y >= w.dims[1] || z >= w.dims[2]) return 0u;
@@ -462,11 +219,11 @@ PX_DEV void load_ray(const px_ray* r, float ro[3], float rd[3], float& maxT, uin
 }
 
 template <int KIND, bool SUP_LDS>
-__global__ __launch_bounds__(256) void k_trace(PxWorldView w, const px_ray* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace(PxWorldView w, const px_ray* __restrict__ rays, uint32_t n,
                                                px_hit* __restrict__ out) {
   extern __shared__ uint32_t s_mem[];
   MarchTables T;
-  if (KIND == 0 || KIND == 1 || KIND < 0) T = load_march_tables<SUP_LDS>(w, s_mem);
+  if (KIND == 0 || KIND == 1 || KIND < 0) T = px::load_march_tables<SUP_LDS>(w, s_mem);
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float ro[3], rd[3], maxT;
@@ -474,8 +231,8 @@ __global__ __launch_bounds__(256) void k_trace(PxWorldView w, const px_ray* __re
   load_ray(rays + i, ro, rd, maxT, flags);
   int kind = KIND >= 0 ? KIND : (int)(flags & 3u);
   MarchOut m;
-  if (kind == 0) m = march(w, T, ro, rd, maxT, false);
-  else if (kind == 1) m = march(w, T, ro, rd, maxT, true);
+  if (kind == 0) m = px::march<false>(w, T, ro, rd, maxT);
+  else if (kind == 1) m = px::march<true>(w, T, ro, rd, maxT);
   else if (kind == 2) m = trace_solid(w, ro, rd, maxT);
   else m = segment(w, ro, rd);
   store_hit(out + i, m);
@@ -486,12 +243,12 @@ __global__ __launch_bounds__(256) void k_trace(PxWorldView w, const px_ray* __re
 // there slot by slot (lanes placed by ballot prefix, no atomics) with the
 // destination slot in `flags`, and their number goes to wave_count[gw].
 template <bool SUP_LDS>
-__global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDesc sd,
+__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_frame_primary(PxWorldView w, PxShadowDesc sd,
                                                        const px_ray* __restrict__ rays, uint32_t n,
                                                        px_hit* __restrict__ prim_out, px_hit* __restrict__ sh_out,
                                                        px_ray* __restrict__ queue, uint32_t* __restrict__ wave_count) {
   extern __shared__ uint32_t s_mem[];
-  const MarchTables T = load_march_tables<SUP_LDS>(w, s_mem);
+  const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = i < n;
   float ro[3] = {0, 0, 0}, rd[3] = {0, 0, 1}, maxT = 0.0f;
@@ -501,7 +258,7 @@ __global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDe
   h.value = 0u;
   if (live) {
     load_ray(rays + i, ro, rd, maxT, flags);
-    h = march(w, T, ro, rd, maxT, false);
+    h = px::march<false>(w, T, ro, rd, maxT);
     store_hit(prim_out + i, h);
     for (uint32_t s = 0; s < sd.slots; s++) store_not_cast(sh_out + (size_t)i * sd.slots + s);
   }
@@ -560,12 +317,12 @@ __global__ __launch_bounds__(256) void k_frame_primary(PxWorldView w, PxShadowDe
 
 // One wavefront per queue region of k_frame_primary.
 template <bool SUP_LDS>
-__global__ __launch_bounds__(256) void k_trace_shadow_queue(PxWorldView w, const px_ray* __restrict__ queue,
+__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWorldView w, const px_ray* __restrict__ queue,
                                                             const uint32_t* __restrict__ wave_count,
                                                             uint32_t n_regions, uint32_t slots,
                                                             px_hit* __restrict__ sh_out) {
   extern __shared__ uint32_t s_mem[];
-  const MarchTables T = load_march_tables<SUP_LDS>(w, s_mem);
+  const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   const uint32_t region = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (region >= n_regions) return;
   const uint32_t cnt = wave_count[region];
@@ -574,7 +331,7 @@ __global__ __launch_bounds__(256) void k_trace_shadow_queue(PxWorldView w, const
     float ro[3], rd[3], maxT;
     uint32_t slot;
     load_ray(rq + k, ro, rd, maxT, slot);
-    MarchOut m = march(w, T, ro, rd, maxT, true);
+    MarchOut m = px::march<true>(w, T, ro, rd, maxT);
     store_hit(sh_out + slot, m);
   }
 }
