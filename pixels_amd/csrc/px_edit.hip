@@ -188,6 +188,25 @@ PX_DEV uint32_t super_of(const EditView& e, uint32_t ci) {
 
 PX_DEV void set_dirty(const EditView& e, uint32_t ci) { atomicOr(&e.dirty_flags[ci >> 5], 1u << (ci & 31u)); }
 
+// flag[node] |= bits for every lane with node != PX_NONE, one atomic per
+// distinct node of the wavefront, and none when a coherent read shows the
+// bits already set: the lanes of a wavefront mostly name one component root,
+// and same-address atomics from every lane would serialise.
+PX_DEV void wave_flag_unique(uint32_t* flag, uint32_t node, uint32_t bits) {
+  const uint32_t lane = __lane_id();
+  for (;;) {
+    const uint64_t act = __ballot(node != PX_NONE);
+    if (act == 0ull) return;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+    const uint32_t n0 = __shfl(node, (int)leader);
+    if (lane == leader) {
+      const uint32_t f = __hip_atomic_load(flag + n0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((f & bits) != bits) atomicOr(flag + n0, bits);
+    }
+    if (node == n0) node = PX_NONE;
+  }
+}
+
 PX_DEV uint32_t price_dimes(const float* mats, uint32_t v) {
   float p = mats[(v & 0xffu) * 20u + 15u];
   return (uint32_t)px_f2i(p * 10.0f + 0.5f);
@@ -657,17 +676,21 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
     }
     uint32_t destroyed = 0, bill = 0;
     for (int x = 0; x < 8; x++) {
-      if (!((rem >> x) & 1u)) continue;
-      uint32_t sh = (uint32_t)(x & 1) * 16u;
-      uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
-      ww[x >> 1] &= ~(0xffffu << sh);
-      destroyed++;
-      bill += price_dimes(e.mats, v);
-      if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
-        uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
-        if (r < 512u) atomicOr(&e.flag[e.par[okey + loff(e, r)]], kFlagHit);
-        else atomicOr(&e.ec[EC_OVERFLOW], 2u);
+      uint32_t hit_root = PX_NONE;
+      if ((rem >> x) & 1u) {
+        uint32_t sh = (uint32_t)(x & 1) * 16u;
+        uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
+        ww[x >> 1] &= ~(0xffffu << sh);
+        destroyed++;
+        bill += price_dimes(e.mats, v);
+        if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
+          uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
+          if (r < 512u) hit_root = e.par[okey + loff(e, r)];
+          else atomicOr(&e.ec[EC_OVERFLOW], 2u);
+        }
       }
+      // the pane component this voxel belongs to is hit (flooded later)
+      wave_flag_unique(e.flag, hit_root, kFlagHit);
     }
     if (rem) *wp = make_uint4(ww[0], ww[1], ww[2], ww[3]);
     destroyed = wave_sum(destroyed);
@@ -786,7 +809,11 @@ __global__ __launch_bounds__(64) void k_maint(EditView e) {
 // Per-root totals.  Lanes of a wavefront that share a root are reduced first
 // (one loop turn per distinct root), so a component spanning many bricks
 // takes one atomic per wavefront instead of one per brick-local node.
+template <int PHASE>
 __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
+  // PHASE 0: node flags -> root flags; PHASE 1: node counts -> root counts,
+  // for roots left unanchored only (anchored components need no count, and
+  // the largest of them, the floor, would take an atomic from every block)
   const uint32_t lane = threadIdx.x;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t n = e.bcount[b * kCountWords + BC_NODES];
@@ -796,20 +823,32 @@ __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
       uint32_t x = i < n ? nodes[i] : 0u;
       uint32_t r = i < n ? e.par[x] : 0u;
       bool pending = i < n && r != x;
-      uint32_t c = pending ? e.cnt[x] : 0u;
-      uint32_t f = pending ? e.flag[x] : 0u;
+      uint32_t v = 0u;
+      if (pending) {
+        if (PHASE == 0) {
+          v = e.flag[x];
+          pending = v != 0u;
+        } else {
+          pending = (__hip_atomic_load(e.flag + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFlagAnchor) == 0u;
+          v = pending ? e.cnt[x] : 0u;
+        }
+      }
       uint64_t act = __ballot(pending);
       while (act) {
         uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
         uint32_t r0 = __shfl(r, (int)leader);
         bool mine = pending && r == r0;
         uint64_t peers = __ballot(mine);
-        uint32_t cs = wave_sum(mine ? c : 0u);
-        uint32_t fs = mine ? f : 0u;
-        for (int o = 32; o > 0; o >>= 1) fs |= __shfl_xor(fs, o);
-        if (lane == leader) {
-          atomicAdd(&e.cnt[r0], cs);
-          if (fs) atomicOr(&e.flag[r0], fs);
+        if (PHASE == 0) {
+          uint32_t fs = mine ? v : 0u;
+          for (int o = 32; o > 0; o >>= 1) fs |= __shfl_xor(fs, o);
+          if (lane == leader) {
+            uint32_t have = __hip_atomic_load(e.flag + r0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((have & fs) != fs) atomicOr(&e.flag[r0], fs);
+          }
+        } else {
+          uint32_t cs = wave_sum(mine ? v : 0u);
+          if (lane == leader) atomicAdd(&e.cnt[r0], cs);
         }
         if (mine) pending = false;
         act &= ~peers;
@@ -1022,7 +1061,8 @@ int px_launch_island_pass(px_ctx* c) {
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
-  hipLaunchKernelGGL(k_aggregate, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_aggregate<0>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_aggregate<1>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_emit, G, W, 0, s, e);
   hipLaunchKernelGGL(k_edit_finish, dim3(1), dim3(1024), 0, s, e, carved);
   PX_HIP_CHECK(c, hipGetLastError());
