@@ -44,6 +44,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity check")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-query leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is per-GPU work (rays shard with the brick map "
@@ -150,6 +151,7 @@ def main() -> int:
 
     gw.close()
     island = island_leg(args)
+    batched = None if args.no_batch else batch_leg(args)
 
     out = {
         "metric": METRIC,
@@ -171,6 +173,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "island_pass": island,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -226,6 +229,141 @@ def island_leg(args):
         res["bit_exact_vs_oracle"] = bool(n == on and np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt)
                                           and np.array_equal(vox, ovox.astype(np.uint32)))
         log("island pass bit-exact vs oracle: %s" % res["bit_exact_vs_oracle"])
+    gw.close()
+    return res
+
+
+def shadow_queries(scene, prim_rays, prim_hits, u, lights, n_max):
+    """Shadow rays of shade() (raymarch.wgsl:323-356) for primary hits, in
+    pixel-then-slot order, repeated or cut to exactly n_max (host arithmetic;
+    they are inputs)."""
+    ok = (prim_hits["face"] != 0xFF) & ~(scene.mats.reshape(-1, 20)[prim_hits["mat"], 3] > 0)
+    r, h = prim_rays[ok], prim_hits[ok]
+    n = np.zeros((r.shape[0], 3), np.float32)
+    f = h["face"].astype(np.int64)
+    sel = f < 6
+    n[np.nonzero(sel)[0], f[sel] >> 1] = np.where(f[sel] & 1, -1.0, 1.0)
+    p = r["ro"] + r["rd"] * h["t"][:, None]
+    o = (p + n * np.float32(0.5)).astype(np.float32)
+    vs, sun = u[3], u[4:7]
+    cand = []
+    c = (n @ sun) > 0
+    cand.append((c, np.broadcast_to(sun, n.shape), np.full(n.shape[0], 1e9, np.float32)))
+    l4 = lights.reshape(-1, 4)
+    for i in range(int(l4[0, 0])):
+        lp = l4[1 + 2 * i]
+        d = lp[:3] / vs - p
+        dist = np.sqrt((d * d).sum(1))
+        dn = d / np.maximum(dist, 1e-12)[:, None]
+        cand.append(((dist <= lp[3] / vs) & ((n * dn).sum(1) > 0), dn, dist - 2.0))
+    sel = np.stack([cc[0] for cc in cand], 1)  # (hits, slots)
+    rows, slots = np.nonzero(sel)
+    rows, slots = np.resize(rows, n_max), np.resize(slots, n_max)
+    from pixels_amd import _native as N
+    out = np.zeros(rows.shape[0], N.RAY_DTYPE)
+    out["ro"] = o[rows]
+    dirs = np.stack([cc[1] for cc in cand], 1)
+    mts = np.stack([cc[2] for cc in cand], 1)
+    out["rd"] = dirs[rows, slots]
+    out["max_t"] = mts[rows, slots]
+    out["flags"] = 1
+    return out
+
+
+def batch_leg(args):
+    """Queries of very uneven counts on the 2560x704x1920 scene (the
+    full-scale generator scaled x2, 160 voxels per metre): 2M primary, 3M
+    shadow, 50k shoot rays from 64 origins, 262,144 debris segments of
+    length U[1,200] voxels.  One autotuned persistent launch replayed as a HIP
+    graph, against the four per-kind launches; outputs compared bit for bit,
+    and a sample of each kind against the oracle."""
+    import torch
+    from pixels_amd import GpuWorld, Scene, Simulation
+    from pixels_amd import _native as N
+    from pixels_amd.scene import LARGE_SCALED
+
+    t0 = time.time()
+    scene = Scene.build(LARGE_SCALED, seed=args.seed, dims=(2560, 704, 1920), scale=2.0)
+    vs = 1.0 / 160.0
+    log("batch scene %s fill %.3f pool bricks %d (%.1f%% of cells) built in %.1fs" %
+        (scene.dims, scene.fill, scene.pool_used, 100.0 * scene.pool_used / scene.grid.size, time.time() - t0))
+    gw = GpuWorld(scene, voxel_size=vs)
+    sim = Simulation(gw)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed + 100)
+    W, H = 2048, 1024
+    prim = scene.camera_rays(W, H, 90.0, seed=args.seed + 1, tile=8)
+    u, lights = scene.frame_inputs(W, H, seed=args.seed + 1, voxel_size=vs, light_radius_vox=8192.0)
+    d_prim = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
+    d_out0 = torch.empty(prim.shape[0] * 16, dtype=torch.uint8, device=dev)
+    sim.trace_device(0, d_prim, prim.shape[0], d_out0)
+    gw.sync()
+    ph = d_out0.cpu().numpy().view(N.HIT_DTYPE)
+    shadow = shadow_queries(scene, prim, ph, u, lights, 3 * (1 << 20))
+    n_shoot, n_seg = 50000, 262144
+    shoot = np.zeros(n_shoot, N.RAY_DTYPE)
+    origins = np.zeros((64, 3), np.float32)
+    for i in range(64):
+        origins[i] = scene.camera(seed=args.seed + 200 + i)[:3]
+    shoot["ro"] = origins[rng.integers(0, 64, n_shoot)]
+    d = rng.normal(size=(n_shoot, 3))
+    shoot["rd"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    shoot["max_t"] = 80.0 / vs
+    shoot["flags"] = 2
+    seg = np.zeros(n_seg, N.RAY_DTYPE)
+    seg["ro"] = rng.uniform([0, 0, 0], scene.dims, size=(n_seg, 3))
+    d = rng.normal(size=(n_seg, 3))
+    seg["rd"] = d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(1, 200, size=(n_seg, 1))
+    seg["flags"] = 3
+    host = {0: prim, 1: shadow, 2: shoot, 3: seg}
+    rays = {k: torch.from_numpy(np.ascontiguousarray(v).view(np.uint8).copy()).to(dev) for k, v in host.items()}
+    outb = {k: torch.empty(v.shape[0] * 16, dtype=torch.uint8, device=dev) for k, v in host.items()}
+    outp = {k: torch.empty(v.shape[0] * 16, dtype=torch.uint8, device=dev) for k, v in host.items()}
+    counts = {k: int(v.shape[0]) for k, v in host.items()}
+    total = sum(counts.values())
+    q = sim.query_set({k: (rays[k], counts[k]) for k in host}, outb)
+    torch.cuda.synchronize()
+
+    def per_kind():
+        for k in host:
+            sim.trace_device(k, rays[k], counts[k], outp[k])
+
+    reps = max(3, args.steps // 2)
+    per_kind()
+    gw.sync()
+    gw.profile_begin()
+    for _ in range(reps):
+        per_kind()
+    plain_ms = gw.profile_end()[N.PROF_TRACE][0] / reps
+    shape, tuned_ms = sim.autotune_batch(q, repeats=2)
+    sim.trace_batch(q)
+    gw.sync()
+    gw.profile_begin()
+    for _ in range(reps):
+        sim.trace_batch(q)
+    batch_ms = gw.profile_end()[N.PROF_BATCH][0] / reps
+    same = all(np.array_equal(outb[k].cpu().numpy(), outp[k].cpu().numpy()) for k in host)
+    res = {"counts": {"primary": counts[0], "shadow": counts[1], "shoot": counts[2], "segment": counts[3]},
+           "grid": list(scene.dims), "pool_bricks": scene.pool_used,
+           "nonempty_brick_frac": round(float((scene.grid != 0).mean()), 4),
+           "batch_ms": round(batch_ms, 4), "per_kind_ms": round(plain_ms, 4),
+           "batch_grays_s": round(total / (batch_ms * 1e-3) / 1e9, 4),
+           "per_kind_grays_s": round(total / (plain_ms * 1e-3) / 1e9, 4),
+           "speedup": round(plain_ms / batch_ms, 3), "launch_shape": shape,
+           "autotune_best_ms": round(tuned_ms, 4), "bit_exact_vs_per_kind": bool(same)}
+    log("batched %d queries: graph %.3f ms vs per-kind launches %.3f ms (x%.2f), shape %s, same bits %s" %
+        (total, batch_ms, plain_ms, plain_ms / batch_ms, shape, same))
+    if not args.no_cpu:
+        import oracle
+        ow = oracle.World.from_scene(scene)
+        ok = True
+        for k in host:
+            idx = np.linspace(0, counts[k] - 1, min(4000, counts[k])).astype(np.int64)
+            got = outb[k].cpu().numpy().view(N.HIT_DTYPE)[idx]
+            exp = ow.trace(k, host[k][idx])
+            ok = ok and bool(np.array_equal(got.view(np.uint8), exp.view(np.uint8)))
+        res["bit_exact_vs_oracle_sample"] = ok
+        log("batched sample bit-exact vs oracle: %s" % ok)
     gw.close()
     return res
 

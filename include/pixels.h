@@ -172,6 +172,33 @@ int px_trace_frame_device(px_ctx* ctx, const px_ray* d_primary, uint32_t n, px_h
 /* Slots per primary ray that px_trace_frame_device writes (1 + light count). */
 uint32_t px_frame_slots(const px_ctx* ctx);
 
+/* ---- batched queries ------------------------------------------------------
+ * New entry the hot path needs (the reference runs each query kind inside its
+ * own shader: fsMain raymarch.wgsl:388-398, shoot sim.wgsl:371-403, the
+ * segment DDA of particleSim sim.wgsl:1110-1145).  Queries of all four kinds,
+ * of any (uneven) counts, run in ONE persistent launch: wavefronts claim
+ * chunks of `chunk` rays of one kind from per-XCD cursors and trace them with
+ * the same code and results as px_trace_device (outputs are bit-identical for
+ * every shape).  The launch is captured once into a HIP graph and replayed
+ * while the query set and shape stay the same.  Device buffers; asynchronous. */
+typedef struct px_query_set {
+  const px_ray* rays[4]; /* indexed by kind PX_RAY_PRIMARY..PX_RAY_SEGMENT */
+  px_hit* out[4];
+  uint32_t n[4];
+} px_query_set;
+int px_trace_batch(px_ctx* ctx, const px_query_set* q);
+/* Launch shape of px_trace_batch: block = threads per workgroup (64, 128 or
+ * 256), blocks = workgroups (0 = six wavefronts per SIMD of the device),
+ * chunk = rays per claim (>= 1). */
+int px_set_batch_shape(px_ctx* ctx, uint32_t block, uint32_t blocks, uint32_t chunk);
+/* out = {block, blocks, chunk} */
+int px_get_batch_shape(const px_ctx* ctx, uint32_t out[3]);
+/* Autotuner: times px_trace_batch on `q` over the shapes block {64, 128, 256}
+ * x wavefronts per CU {8, 16, 24} x chunk {64, 256, 1024} (`repeats` timed
+ * launches each, after one warm-up), leaves the fastest set and writes it to
+ * out[3] and its mean time to *best_ms.  Blocking. */
+int px_autotune_batch(px_ctx* ctx, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms);
+
 /* Replaces Simulation.probe (Simulation.ts:161-196, probe.wgsl:25-32):
  * decoded voxel values of the box [o, o+s), x fastest.  Host buffer. */
 int px_probe(px_ctx* ctx, const int32_t o[3], const int32_t s[3], uint32_t* out);
@@ -226,6 +253,7 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_SHADOW 1   /* frame: shadow-ray march */
 #define PX_PROF_TRACE 2    /* px_trace / px_trace_device */
 #define PX_PROF_EDIT 3     /* island pass, all kernels */
+#define PX_PROF_BATCH 4    /* px_trace_batch (the graph launch) */
 #define PX_PROF_SLOTS 8
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);

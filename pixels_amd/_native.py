@@ -66,6 +66,10 @@ class PxWorldDesc(C.Structure):
     ]
 
 
+class PxQuerySet(C.Structure):
+    _fields_ = [("rays", C.c_void_p * 4), ("out", C.c_void_p * 4), ("n", C.c_uint32 * 4)]
+
+
 # every exported symbol with (restype, argtypes)
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -85,6 +89,10 @@ SIGNATURES = {
     "px_trace_device": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_frame_device": (C.c_int, [_P, _P, _U32, _P, _P, _U32P]),
     "px_frame_slots": (_U32, [_P]),
+    "px_trace_batch": (C.c_int, [_P, C.POINTER(PxQuerySet)]),
+    "px_set_batch_shape": (C.c_int, [_P, _U32, _U32, _U32]),
+    "px_get_batch_shape": (C.c_int, [_P, _U32P]),
+    "px_autotune_batch": (C.c_int, [_P, C.POINTER(PxQuerySet), _U32, _U32P, C.POINTER(C.c_float)]),
     "px_probe": (C.c_int, [_P, _I32P, _I32P, _U32P]),
     "px_probe_device": (C.c_int, [_P, _I32P, _I32P, _P]),
     "px_read_ctrl": (C.c_int, [_P, _U32P, _U32P, _FP]),
@@ -99,7 +107,7 @@ SIGNATURES = {
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
-PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_SLOTS = 0, 1, 2, 3, 8
+PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_SLOTS = 0, 1, 2, 3, 4, 8
 
 _lib = None
 

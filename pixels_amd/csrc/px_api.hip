@@ -84,8 +84,10 @@ static void release(px_ctx* c) {
     hipEventDestroy(r.b);
   }
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+  if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
   void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
                   c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_shadow_rays, c->d_wave_count, c->d_counters,
+                  c->d_batch_cursors,
                   c->b_grid, c->b_pool, c->b_super, c->b_ctrl, c->b_free};
   for (void* p : bufs)
     if (p) hipFree(p);
@@ -157,6 +159,12 @@ int px_create(const px_world_desc* d, px_ctx** out) {
   TRY(hipMalloc(&c->d_super_mask, (size_t)((c->nsuper + 31u) / 32u) * 4 + 256));
   TRY(hipMalloc(&c->d_super_bm, (size_t)c->nsuper * 8 + 256));
   TRY(hipMalloc(&c->d_counters, 256));
+  TRY(hipMalloc(&c->d_batch_cursors, 8 * 256));
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
+      c->n_cu = (uint32_t)ncu;
+  }
 
   // ---- one-time upload (GpuWorld.ts:85-117) ----
   TRY(hipMemcpy(c->d_mats, d->mats, PX_MAT_COUNT * PX_MAT_FLOATS * 4, hipMemcpyHostToDevice));
@@ -302,6 +310,53 @@ int px_trace_frame_device(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit
   if (!c || (n && (!d_primary || !d_prim || !d_shadow))) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
   return px_launch_frame(c, d_primary, n, d_prim, d_shadow, n_cast);
+}
+
+static int check_query_set(px_ctx* c, const px_query_set* q, const char* what) {
+  if (!q) return px_fail(c, PX_ERR_ARG, what, "query set is NULL");
+  uint64_t total = 0;
+  for (int k = 0; k < 4; k++) {
+    if (q->n[k] && (!q->rays[k] || !q->out[k])) return px_fail(c, PX_ERR_ARG, what, "NULL buffer for a non-empty kind");
+    total += q->n[k];
+  }
+  if (total > 0xFFFFFFFFull) return px_fail(c, PX_ERR_CAPACITY, what, "more than 2^32-1 queries");
+  return PX_OK;
+}
+
+int px_trace_batch(px_ctx* c, const px_query_set* q) {
+  if (!c) return PX_ERR_ARG;
+  int rc = check_query_set(c, q, "px_trace_batch");
+  if (rc) return rc;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_batch(c, q);
+}
+
+int px_set_batch_shape(px_ctx* c, uint32_t block, uint32_t blocks, uint32_t chunk) {
+  if (!c) return PX_ERR_ARG;
+  if (block != 64u && block != 128u && block != 256u)
+    return px_fail(c, PX_ERR_ARG, "px_set_batch_shape", "block must be 64, 128 or 256");
+  if (chunk == 0u) return px_fail(c, PX_ERR_ARG, "px_set_batch_shape", "chunk must be at least 1");
+  if (blocks > (1u << 20)) return px_fail(c, PX_ERR_ARG, "px_set_batch_shape", "blocks: at most 2^20");
+  c->batch_block = block;
+  c->batch_blocks = blocks;
+  c->batch_chunk = chunk;
+  return PX_OK;
+}
+
+int px_get_batch_shape(const px_ctx* c, uint32_t out[3]) {
+  if (!c || !out) return PX_ERR_ARG;
+  out[0] = c->batch_block;
+  out[1] = c->batch_blocks ? c->batch_blocks : c->n_cu * 6u * 4u * 64u / c->batch_block;
+  out[2] = c->batch_chunk;
+  return PX_OK;
+}
+
+int px_autotune_batch(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms) {
+  if (!c || !out) return PX_ERR_ARG;
+  int rc = check_query_set(c, q, "px_autotune_batch");
+  if (rc) return rc;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_autotune_batch_impl(c, q, repeats ? repeats : 3u, out, best_ms);
 }
 
 static bool box_ok(const px_ctx* c, const int32_t s[3]) {

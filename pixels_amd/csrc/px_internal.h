@@ -73,6 +73,12 @@ struct px_ctx {
   size_t shadow_cap = 0;
   uint32_t* d_wave_count = nullptr;  // shadow rays queued per wavefront region
   uint32_t wave_count_cap = 0;
+  // batched queries: launch shape, per-XCD claim cursors, the captured graph
+  uint32_t n_cu = 256;
+  uint32_t batch_block = 256, batch_blocks = 0, batch_chunk = 256;
+  uint32_t* d_batch_cursors = nullptr;
+  hipGraphExec_t batch_exec = nullptr;
+  unsigned char batch_key[160] = {0};
   uint32_t* d_counters = nullptr;  // small scratch counters
   // edit chain
   float* d_impacts = nullptr;
@@ -138,6 +144,8 @@ int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, 
 int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
                     uint32_t* n_cast);
 int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
+int px_launch_batch(px_ctx* c, const px_query_set* q);
+int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms);
 int px_launch_masks(px_ctx* c);
 // edit chain (px_edit.hip)
 int px_edit_alloc(px_ctx* c);

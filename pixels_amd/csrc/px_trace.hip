@@ -14,6 +14,8 @@
 // ones the reference's arithmetic defines.  Choices WGSL leaves open are
 // fixed as: pow(x,5) = ((x*x)*(x*x))*x, mix(a,b,t) = a*(1-t) + b*t,
 // dot = (x*x' + y*y') + z*z', length = sqrt(dot), zero t reported as +0.
+#include <cstring>
+
 #include "px_internal.h"
 #include "px_march.h"
 
@@ -336,6 +338,64 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
   }
 }
 
+// ---- batched queries ---------------------------------------------------
+// One persistent launch over queries of all four kinds.  The chunk space is
+// the kinds laid end to end in `order` (longest-running kinds first, so the
+// short ones fill the tail), cut into chunks of `chunk` rays that never
+// straddle two kinds.  Chunk c belongs to claim group c % kBatchGroups; a
+// workgroup claims from the group of its XCD (workgroups are dealt to the 8
+// XCDs round-robin), then from the others once that group is drained, so no
+// single counter takes every claim.
+constexpr uint32_t kBatchGroups = 8;
+constexpr uint32_t kCursorStride = 64;  // words between cursors (256 bytes)
+
+struct BatchArgs {
+  const px_ray* rays[4];
+  px_hit* out[4];
+  uint32_t n[4];
+  uint32_t first_chunk[5];  // chunk index where each kind of `order` starts
+  uint32_t order[4];        // processing order -> kind
+  uint32_t chunk;
+  uint32_t total_chunks;
+};
+
+template <bool SUP_LDS>
+__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_batch(PxWorldView w, BatchArgs a,
+                                                                     uint32_t* __restrict__ cursors) {
+  extern __shared__ uint32_t s_mem[];
+  const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
+  const uint32_t lane = __lane_id();
+  const uint32_t home = blockIdx.x % kBatchGroups;
+  for (uint32_t gi = 0; gi < kBatchGroups; gi++) {
+    const uint32_t g = (home + gi) % kBatchGroups;
+    for (;;) {
+      uint32_t j = 0u;
+      if (lane == 0u) j = atomicAdd(cursors + g * kCursorStride, 1u);
+      j = __shfl(j, 0);
+      const uint32_t c = j * kBatchGroups + g;
+      if (c >= a.total_chunks) break;
+      uint32_t pos = 0u;
+      while (pos < 3u && c >= a.first_chunk[pos + 1u]) pos++;
+      const uint32_t kind = a.order[pos];
+      const uint32_t start = (c - a.first_chunk[pos]) * a.chunk;
+      const uint32_t end = min(start + a.chunk, a.n[kind]);
+      const px_ray* __restrict__ rays = a.rays[kind];
+      px_hit* __restrict__ out = a.out[kind];
+      for (uint32_t i = start + lane; i < end; i += 64u) {
+        float ro[3], rd[3], maxT;
+        uint32_t flags;
+        load_ray(rays + i, ro, rd, maxT, flags);
+        MarchOut m;
+        if (kind == PX_RAY_PRIMARY) m = px::march<false>(w, T, ro, rd, maxT);
+        else if (kind == PX_RAY_SHADOW) m = px::march<true>(w, T, ro, rd, maxT);
+        else if (kind == PX_RAY_SHOOT) m = trace_solid(w, ro, rd, maxT);
+        else m = segment(w, ro, rd);
+        store_hit(out + i, m);
+      }
+    }
+  }
+}
+
 // Sum of the per-wavefront shadow-ray counts (one block).
 __global__ __launch_bounds__(1024) void k_sum_counts(const uint32_t* __restrict__ cnt, uint32_t n,
                                                      uint32_t* __restrict__ out) {
@@ -540,6 +600,144 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
     PX_HIP_CHECK(c, hipMemcpyAsync(n_cast, c->d_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   }
+  return PX_OK;
+}
+
+// Kinds in processing order: shoot rays (long, few), primary, shadow, then
+// the short debris segments.
+static const uint32_t kBatchOrder[4] = {PX_RAY_SHOOT, PX_RAY_PRIMARY, PX_RAY_SHADOW, PX_RAY_SEGMENT};
+
+static BatchArgs batch_args(const px_ctx* c, const px_query_set* q) {
+  BatchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.chunk = c->batch_chunk;
+  uint32_t acc = 0u;
+  for (int p = 0; p < 4; p++) {
+    const uint32_t k = kBatchOrder[p];
+    a.order[p] = k;
+    a.first_chunk[p] = acc;
+    acc += (q->n[k] + a.chunk - 1u) / a.chunk;
+  }
+  a.first_chunk[4] = acc;
+  a.total_chunks = acc;
+  for (int k = 0; k < 4; k++) {
+    a.rays[k] = q->rays[k];
+    a.out[k] = q->out[k];
+    a.n[k] = q->n[k];
+  }
+  return a;
+}
+
+static uint32_t batch_blocks(const px_ctx* c) {
+  // default: six wavefronts per SIMD (the march kernels' register budget)
+  return c->batch_blocks ? c->batch_blocks : c->n_cu * 6u * 4u * 64u / c->batch_block;
+}
+
+// Captures [cursor reset, k_trace_batch] into a graph for this query set and
+// shape (kept while they stay the same) and launches it.
+int px_launch_batch(px_ctx* c, const px_query_set* q) {
+  int rc = px_refresh_masks(c);
+  if (rc) return rc;
+  const BatchArgs a = batch_args(c, q);
+  if (a.total_chunks == 0u) return PX_OK;
+  unsigned char key[sizeof(c->batch_key)];
+  static_assert(sizeof(BatchArgs) + 3 * sizeof(uint32_t) <= sizeof(key), "batch key size");
+  memset(key, 0, sizeof(key));
+  memcpy(key, &a, sizeof(a));
+  const uint32_t shape[3] = {c->batch_block, batch_blocks(c), c->batch_chunk};
+  memcpy(key + sizeof(a), shape, sizeof(shape));
+  if (!c->batch_exec || memcmp(key, c->batch_key, sizeof(key)) != 0) {
+    if (c->batch_exec) {
+      hipGraphExecDestroy(c->batch_exec);
+      c->batch_exec = nullptr;
+    }
+    PxWorldView w = px_view(c);
+    bool sup_lds;
+    const size_t lds = table_lds(c, &sup_lds);
+    hipGraph_t graph = nullptr;
+    PX_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    // a fill kernel, not hipMemsetAsync, inside the graph
+    hipLaunchKernelGGL(k_fill_u32, dim3(8), dim3(64), 0, c->stream, c->d_batch_cursors,
+                       kBatchGroups * kCursorStride, 0u);
+    if (sup_lds)
+      hipLaunchKernelGGL(k_trace_batch<true>, dim3(shape[1]), dim3(shape[0]), lds, c->stream, w, a,
+                         c->d_batch_cursors);
+    else
+      hipLaunchKernelGGL(k_trace_batch<false>, dim3(shape[1]), dim3(shape[0]), lds, c->stream, w, a,
+                         c->d_batch_cursors);
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      return px_fail(c, PX_ERR_HIP, "px_trace_batch", hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipError_t ie = hipGraphInstantiate(&c->batch_exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      c->batch_exec = nullptr;
+      return px_fail(c, PX_ERR_HIP, "px_trace_batch", hipGetErrorString(ie));
+    }
+    memcpy(c->batch_key, key, sizeof(key));
+  }
+  PxProfScope prof(c, PX_PROF_BATCH);
+  PX_HIP_CHECK(c, hipGraphLaunch(c->batch_exec, c->stream));
+  return PX_OK;
+}
+
+int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms) {
+  static const uint32_t blocks_opt[3] = {64u, 128u, 256u};
+  static const uint32_t waves_opt[3] = {8u, 16u, 24u};  // wavefronts per CU
+  static const uint32_t chunk_opt[3] = {64u, 256u, 1024u};
+  const uint32_t keep[3] = {c->batch_block, c->batch_blocks, c->batch_chunk};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PX_HIP_CHECK(c, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    hipEventDestroy(e0);
+    return px_fail(c, PX_ERR_HIP, "px_autotune_batch", "hipEventCreate");
+  }
+  float best = 3.0e38f;
+  uint32_t bs[3] = {keep[0], keep[1], keep[2]};
+  int rc = PX_OK;
+  for (uint32_t bi = 0; bi < 3 && rc == PX_OK; bi++)
+    for (uint32_t wi = 0; wi < 3 && rc == PX_OK; wi++)
+      for (uint32_t ci = 0; ci < 3 && rc == PX_OK; ci++) {
+        c->batch_block = blocks_opt[bi];
+        c->batch_blocks = c->n_cu * waves_opt[wi] * 64u / blocks_opt[bi];
+        c->batch_chunk = chunk_opt[ci];
+        rc = px_launch_batch(c, q);  // capture + warm-up
+        if (rc) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) rc = px_fail(c, PX_ERR_HIP, "px_autotune_batch", "event");
+        for (uint32_t r = 0; r < repeats && rc == PX_OK; r++) rc = px_launch_batch(c, q);
+        if (rc) break;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          rc = px_fail(c, PX_ERR_HIP, "px_autotune_batch", "event");
+          break;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)repeats;
+        if (ms < best) {
+          best = ms;
+          bs[0] = c->batch_block;
+          bs[1] = c->batch_blocks;
+          bs[2] = c->batch_chunk;
+        }
+      }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) {
+    c->batch_block = keep[0];
+    c->batch_blocks = keep[1];
+    c->batch_chunk = keep[2];
+    return rc;
+  }
+  c->batch_block = bs[0];
+  c->batch_blocks = bs[1];
+  c->batch_chunk = bs[2];
+  out[0] = bs[0];
+  out[1] = bs[1];
+  out[2] = bs[2];
+  if (best_ms) *best_ms = best;
   return PX_OK;
 }
 

@@ -214,6 +214,34 @@ class Simulation:
                                                      C.byref(nc) if count else None))
         return int(nc.value) if count else None
 
+    # -- batched queries (one persistent launch, replayed as a HIP graph) ----
+    @staticmethod
+    def query_set(rays_by_kind: dict, out_by_kind: dict) -> N.PxQuerySet:
+        """{kind: (tensor of px_ray records, count)}, {kind: tensor of px_hit records}."""
+        q = N.PxQuerySet()
+        for k, (t, n) in rays_by_kind.items():
+            q.rays[k] = _devptr(t) if n else None
+            q.out[k] = _devptr(out_by_kind[k]) if n else None
+            q.n[k] = n
+        return q
+
+    def trace_batch(self, q: N.PxQuerySet) -> None:
+        self.gw.check(self.lib.px_trace_batch(self.gw.ctx, C.byref(q)))
+
+    def set_batch_shape(self, block: int, blocks: int = 0, chunk: int = 256) -> None:
+        self.gw.check(self.lib.px_set_batch_shape(self.gw.ctx, block, blocks, chunk))
+
+    def get_batch_shape(self) -> dict:
+        out = np.zeros(3, np.uint32)
+        self.gw.check(self.lib.px_get_batch_shape(self.gw.ctx, N.ptr(out)))
+        return dict(block=int(out[0]), blocks=int(out[1]), chunk=int(out[2]))
+
+    def autotune_batch(self, q: N.PxQuerySet, repeats: int = 3):
+        out = np.zeros(3, np.uint32)
+        ms = C.c_float(0.0)
+        self.gw.check(self.lib.px_autotune_batch(self.gw.ctx, C.byref(q), repeats, N.ptr(out), C.byref(ms)))
+        return dict(block=int(out[0]), blocks=int(out[1]), chunk=int(out[2])), float(ms.value)
+
     # -- edit chain -----------------------------------------------------------
     def push_impacts(self, impacts: np.ndarray) -> None:
         imp = np.ascontiguousarray(impacts, np.float32).reshape(-1, 16)
