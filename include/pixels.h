@@ -254,9 +254,46 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_TRACE 2    /* px_trace / px_trace_device */
 #define PX_PROF_EDIT 3     /* island pass, all kernels */
 #define PX_PROF_BATCH 4    /* px_trace_batch (the graph launch) */
+#define PX_PROF_MLP 5      /* ray-cost predictor: features, predict, train, bin */
 #define PX_PROF_SLOTS 8
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);
+
+/* ---- learned ray-cost binning (no counterpart in the reference, whose
+ * traversal is the plain DDA; for callers that reorder rays into
+ * cost-homogeneous wavefronts).  A 16-64-64-1 MLP (ReLU) predicts log2 of a
+ * ray's step count from 16 features: origin / dims, unit direction,
+ * super-cell occupancy at 8 points along the clipped ray, log2 extent, kind.
+ * BF16 weights and activations, FP32 accumulation on MFMA, FP32 Adam state.
+ * Binning only reorders rays: px_trace_perm_device traces every ray with the
+ * same kernel as px_trace_device, so outputs are bit-identical. */
+typedef struct px_mlp px_mlp;
+/* Number of FP32 parameters: W1[16][64], b1[64], W2[64][64], b2[64], w3[64], b3. */
+uint32_t px_mlp_param_count(void);
+/* Weights He-uniform from a splitmix64 stream of `seed`, biases 0. */
+int px_mlp_create(px_ctx* ctx, uint64_t seed, px_mlp** out);
+int px_mlp_destroy(px_mlp* m);
+int px_mlp_get_params(px_mlp* m, float* h_params);
+/* Sets the FP32 parameters (and their BF16 copies) and resets the Adam state. */
+int px_mlp_set_params(px_mlp* m, const float* h_params);
+/* d_feat: n * 16 BF16 features of rays of one kind (PX_RAY_*). */
+int px_mlp_features(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, uint32_t n, uint16_t* d_feat);
+/* d_pred: n FP32 predictions of log2(steps). */
+int px_mlp_predict(px_mlp* m, const uint16_t* d_feat, uint32_t n, float* d_pred);
+/* Gradient of (1/n) sum (pred - target)^2 over a batch (blocking): h_grad
+ * gets px_mlp_param_count() floats in the parameter order, *h_loss the loss. */
+int px_mlp_grad(px_mlp* m, const uint16_t* d_feat, const float* d_target, uint32_t n, float* h_grad, float* h_loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss; *h_loss (may be
+ * NULL: then asynchronous) gets the batch loss before the step. */
+int px_mlp_train_step(px_mlp* m, const uint16_t* d_feat, const float* d_target, uint32_t n, float lr, float* h_loss);
+/* Stable counting sort of each window of `window` rays into `buckets` cost
+ * buckets over [lo, hi) of the prediction; d_perm[k] = source index. */
+int px_bin_rays(px_ctx* ctx, const float* d_pred, uint32_t n, uint32_t window, uint32_t buckets, float lo, float hi,
+                uint32_t* d_perm);
+/* Traces d_rays[d_perm[k]] for k = 0..n-1 in that order and writes the hit to
+ * d_out[d_perm[k]] (kinds as px_trace_device). */
+int px_trace_perm_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
+                         px_hit* d_out);
 
 #ifdef __cplusplus
 }

@@ -104,10 +104,21 @@ SIGNATURES = {
     "px_island_labels": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
     "px_snapshot": (C.c_int, [_P]),
     "px_restore": (C.c_int, [_P]),
+    "px_trace_perm_device": (C.c_int, [_P, _U32, _P, _P, _U32, _P]),
+    "px_mlp_param_count": (_U32, []),
+    "px_mlp_create": (C.c_int, [_P, C.c_uint64, C.POINTER(_P)]),
+    "px_mlp_destroy": (C.c_int, [_P]),
+    "px_mlp_get_params": (C.c_int, [_P, _FP]),
+    "px_mlp_set_params": (C.c_int, [_P, _FP]),
+    "px_mlp_features": (C.c_int, [_P, _U32, _P, _U32, _P]),
+    "px_mlp_predict": (C.c_int, [_P, _P, _U32, _P]),
+    "px_mlp_grad": (C.c_int, [_P, _P, _P, _U32, _FP, _FP]),
+    "px_mlp_train_step": (C.c_int, [_P, _P, _P, _U32, C.c_float, _FP]),
+    "px_bin_rays": (C.c_int, [_P, _P, _U32, _U32, _U32, C.c_float, C.c_float, _P]),
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
-PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_SLOTS = 0, 1, 2, 3, 4, 8
+PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_SLOTS = 0, 1, 2, 3, 4, 5, 8
 
 _lib = None
 

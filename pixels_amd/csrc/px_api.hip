@@ -281,6 +281,14 @@ int px_trace_device(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, 
   return px_launch_trace(c, kind, d_rays, n, d_out);
 }
 
+int px_trace_perm_device(px_ctx* c, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
+                         px_hit* d_out) {
+  if (!c || (n && (!d_rays || !d_perm || !d_out))) return PX_ERR_ARG;
+  if (kind > 3u) return px_fail(c, PX_ERR_ARG, "px_trace_perm_device", "unknown ray kind");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_trace_perm(c, kind, d_rays, d_perm, n, d_out);
+}
+
 int px_trace(px_ctx* c, uint32_t kind, const px_ray* rays, uint32_t n, px_hit* out) {
   if (!c || (n && (!rays || !out))) return PX_ERR_ARG;
   if (kind > 4u) return px_fail(c, PX_ERR_ARG, "px_trace", "unknown ray kind");

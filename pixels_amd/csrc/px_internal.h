@@ -145,6 +145,8 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
                     uint32_t* n_cast);
 int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
 int px_launch_batch(px_ctx* c, const px_query_set* q);
+int px_launch_trace_perm(px_ctx* c, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
+                         px_hit* d_out);
 int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms);
 int px_launch_masks(px_ctx* c);
 // edit chain (px_edit.hip)

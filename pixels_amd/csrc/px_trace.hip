@@ -240,6 +240,28 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace(PxWorldView w, con
   store_hit(out + i, m);
 }
 
+// k_trace over rays taken in the order of perm (results to their own slots)
+template <int KIND, bool SUP_LDS>
+__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_perm(PxWorldView w, const px_ray* __restrict__ rays,
+                                                                    const uint32_t* __restrict__ perm, uint32_t n,
+                                                                    px_hit* __restrict__ out) {
+  extern __shared__ uint32_t s_mem[];
+  MarchTables T;
+  if (KIND == 0 || KIND == 1) T = px::load_march_tables<SUP_LDS>(w, s_mem);
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t i = perm[k];
+  float ro[3], rd[3], maxT;
+  uint32_t flags;
+  load_ray(rays + i, ro, rd, maxT, flags);
+  MarchOut m;
+  if (KIND == 0) m = px::march<false>(w, T, ro, rd, maxT);
+  else if (KIND == 1) m = px::march<true>(w, T, ro, rd, maxT);
+  else if (KIND == 2) m = trace_solid(w, ro, rd, maxT);
+  else m = segment(w, ro, rd);
+  store_hit(out + i, m);
+}
+
 // Primary march + shadow-ray generation.  Wavefront `gw` owns the queue
 // region [gw * 64 * slots, (gw + 1) * 64 * slots): its shadow rays are packed
 // there slot by slot (lanes placed by ballot prefix, no atomics) with the
@@ -538,6 +560,33 @@ int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, 
     case PX_RAY_SHOOT: launch_trace_kind<2>(c, w, d_rays, n, d_out); break;
     case PX_RAY_SEGMENT: launch_trace_kind<3>(c, w, d_rays, n, d_out); break;
     default: launch_trace_kind<-1>(c, w, d_rays, n, d_out); break;
+  }
+  PX_HIP_CHECK(c, hipGetLastError());
+  return PX_OK;
+}
+
+template <int KIND>
+static void launch_perm_kind(px_ctx* c, const PxWorldView& w, const px_ray* d_rays, const uint32_t* d_perm,
+                             uint32_t n, px_hit* d_out) {
+  bool sup_lds;
+  size_t lds = table_lds(c, &sup_lds);
+  dim3 grid((n + 255u) / 256u), block(256);
+  if (sup_lds) hipLaunchKernelGGL((k_trace_perm<KIND, true>), grid, block, lds, c->stream, w, d_rays, d_perm, n, d_out);
+  else hipLaunchKernelGGL((k_trace_perm<KIND, false>), grid, block, lds, c->stream, w, d_rays, d_perm, n, d_out);
+}
+
+int px_launch_trace_perm(px_ctx* c, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
+                         px_hit* d_out) {
+  if (n == 0) return PX_OK;
+  int rc = px_refresh_masks(c);
+  if (rc) return rc;
+  PxWorldView w = px_view(c);
+  PxProfScope prof(c, PX_PROF_TRACE);
+  switch (kind) {
+    case PX_RAY_PRIMARY: launch_perm_kind<0>(c, w, d_rays, d_perm, n, d_out); break;
+    case PX_RAY_SHADOW: launch_perm_kind<1>(c, w, d_rays, d_perm, n, d_out); break;
+    case PX_RAY_SHOOT: launch_perm_kind<2>(c, w, d_rays, d_perm, n, d_out); break;
+    default: launch_perm_kind<3>(c, w, d_rays, d_perm, n, d_out); break;
   }
   PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;

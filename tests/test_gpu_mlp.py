@@ -1,0 +1,221 @@
+"""GPU tests of the learned ray-cost binning (px_mlp.hip) against the CPU
+oracle (oracle/mlp.py):
+
+* forward predictions and batch gradients of the BF16/MFMA kernels equal
+  the oracle's BF16-rounding restatement to FP32-accumulation-order noise;
+* one Adam step on the GPU equals the oracle's Adam on the GPU gradient;
+* features are in [-1, 1] with the occupancy samples at exactly +-1;
+* binning is a stable permutation sorted by bucket inside every window, and
+  the permuted launch gives the plain launch's hits bit for bit;
+* training on the DDA's own step counts cuts the predictor's held-out error
+  to under a quarter of the untrained network's (the gain the training
+  buys); the wavefront cost of its bins is printed beside the plain order
+  and the bins of the true step counts.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def world(small_scene):
+    from pixels_amd import GpuWorld, Simulation
+    gw = GpuWorld(small_scene)
+    yield small_scene, gw, Simulation(gw)
+    gw.close()
+
+
+def _rand_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(mlp.N_PARAMS, np.float32)
+    p[mlp.OFF_W1:mlp.OFF_B1] = rng.uniform(-0.6, 0.6, mlp.N_IN * mlp.N_H)
+    p[mlp.OFF_B1:mlp.OFF_W2] = rng.uniform(-0.1, 0.3, mlp.N_H)
+    p[mlp.OFF_W2:mlp.OFF_B2] = rng.uniform(-0.25, 0.25, mlp.N_H * mlp.N_H)
+    p[mlp.OFF_B2:mlp.OFF_W3] = rng.uniform(-0.1, 0.3, mlp.N_H)
+    p[mlp.OFF_W3:mlp.OFF_B3] = rng.uniform(-0.3, 0.3, mlp.N_H)
+    p[mlp.OFF_B3] = 1.5
+    return p
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_forward_and_gradient_match_oracle(world):
+    import torch
+    from pixels_amd.learned import CostModel
+    s, gw, sim = world
+    m = CostModel(gw, seed=7)
+    try:
+        p = _rand_params(11)
+        m.set_params(p)
+        assert np.array_equal(m.params(), p)
+        rng = np.random.default_rng(12)
+        n = 1000  # not a multiple of 64
+        x = mlp.bf16(rng.uniform(-1, 1, (n, 16)).astype(np.float32))
+        xb = (x.view(np.uint32) >> 16).astype(np.uint16)
+        t = rng.uniform(0, 8, n).astype(np.float32)
+        dx, dt = _dev(xb), _dev(t)
+        pred = torch.zeros(n, dtype=torch.float32, device="cuda")
+        print("case: mlp forward n=1000", flush=True)
+        m.predict(dx, n, pred)
+        gw.sync()
+        got = pred.cpu().numpy()
+        ref, _, _ = mlp.forward_bf16(p, x)
+        err = np.abs(got - ref)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 5e-3, err.max()
+        assert np.quantile(err, 0.99) < 0.05
+        print("case: mlp gradient n=1000", flush=True)
+        loss, g = m.grad(dx, dt, n)
+        rloss, rg = mlp.grad_bf16(p, x, t)
+        assert abs(loss - rloss) <= 2e-3 * rloss
+        assert np.linalg.norm(g - rg) / np.linalg.norm(rg) < 2e-2
+        for lo, hi in ((mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_W2, mlp.OFF_B2), (mlp.OFF_W3, mlp.OFF_B3)):
+            gr, rr = g[lo:hi], rg[lo:hi]
+            cos = float(np.dot(gr, rr) / (np.linalg.norm(gr) * np.linalg.norm(rr)))
+            assert cos > 0.999, (lo, cos)
+        # one Adam step equals the oracle's Adam on the kernel's own gradient
+        print("case: adam step", flush=True)
+        m.set_params(p)
+        _, g0 = m.grad(dx, dt, n)
+        m.train_step(dx, dt, n, lr=1e-3)
+        p1 = m.params()
+        z = np.zeros_like(p)
+        rp1, _, _ = mlp.adam(p, g0, z, z, 1, lr=1e-3)
+        assert np.allclose(p1, rp1, rtol=1e-6, atol=1e-7)
+    finally:
+        m.close()
+
+
+def test_features_range(world):
+    import torch
+    from pixels_amd.learned import CostModel
+    s, gw, sim = world
+    m = CostModel(gw)
+    try:
+        rays = s.camera_rays(320, 180, 90.0, seed=4)
+        n = rays.shape[0]
+        feat = torch.zeros(n * 16, dtype=torch.int16, device="cuda")
+        m.features(0, _dev(rays.view(np.uint8)), n, feat)
+        gw.sync()
+        f = mlp.bf16_bits_to_f32(feat.cpu().numpy().view(np.uint16)).reshape(n, 16)
+        assert np.all(f >= -1.0) and np.all(f <= 1.0)
+        assert np.all(np.isin(f[:, 6:14], (-1.0, 1.0)))
+        assert np.all(f[:, 15] == -1.0)
+        d = rays["rd"] / np.linalg.norm(rays["rd"], axis=1, keepdims=True)
+        assert np.allclose(f[:, 3:6], d, atol=1e-2)
+        # rays inside occupied space see some occupied super cells
+        assert (f[:, 6:14] > 0).mean() > 0.2
+    finally:
+        m.close()
+
+
+def _trace_steps(sim, gw, kind, rays):
+    import torch
+    n = rays.shape[0]
+    out = torch.zeros(n * 16, dtype=torch.uint8, device="cuda")
+    sim.trace_device(kind, _dev(rays.view(np.uint8)), n, out)
+    gw.sync()
+    return out.cpu().numpy().view(mlp_hit_dtype())
+
+
+def mlp_hit_dtype():
+    import oracle
+    return oracle.HIT_DTYPE
+
+
+def test_binning_is_stable_permutation_and_bit_exact(world):
+    import torch
+    from pixels_amd.learned import bin_rays, trace_perm
+    s, gw, sim = world
+    rays = s.camera_rays(480, 270, 90.0, seed=8)
+    n = rays.shape[0]
+    rng = np.random.default_rng(3)
+    pred = rng.uniform(-1, 10, n).astype(np.float32)
+    perm = torch.zeros(n, dtype=torch.int32, device="cuda")
+    win, nb, lo, hi = 1000, 32, 0.0, 9.0
+    print("case: bin n=%d" % n, flush=True)
+    bin_rays(gw, _dev(pred), n, perm, window=win, buckets=nb, lo=lo, hi=hi)
+    gw.sync()
+    pm = perm.cpu().numpy().astype(np.int64)
+    b = np.clip(np.floor((pred - lo) * (nb / (hi - lo))), 0, nb - 1).astype(np.int64)
+    for w0 in range(0, n, win):
+        seg = pm[w0:w0 + win]
+        assert np.array_equal(np.sort(seg), np.arange(w0, min(w0 + win, n)))
+        exp = w0 + np.lexsort((np.arange(seg.size), b[w0:w0 + win]))
+        assert np.array_equal(seg, exp)
+    for kind in (0, 1):
+        ref = _trace_steps(sim, gw, kind, rays)
+        out = torch.zeros(n * 16, dtype=torch.uint8, device="cuda")
+        trace_perm(gw, kind, _dev(rays.view(np.uint8)), perm, n, out)
+        gw.sync()
+        assert np.array_equal(out.cpu().numpy(), ref.view(np.uint8))
+
+
+def test_training_gain_over_untrained(world):
+    """Training on the DDA's own step counts makes the predictor far better on
+    a held-out camera than the network it started from; the wavefront cost
+    of the resulting bins is reported beside the plain order and the bins of
+    the true step counts."""
+    import torch
+    from pixels_amd.learned import CostModel, bin_rays, log2_steps, wave_cost
+    s, gw, sim = world
+
+    def data(seed, w=640, h=360):
+        rays = s.camera_rays(w, h, 90.0, seed=seed)
+        hits = _trace_steps(sim, gw, 0, rays)
+        n = rays.shape[0]
+        feat = torch.zeros(n * 16, dtype=torch.int16, device="cuda")
+        m.features(0, _dev(rays.view(np.uint8)), n, feat)
+        return rays, hits["steps"].astype(np.int64), feat
+
+    m = CostModel(gw, seed=3)
+    try:
+        untrained = m.params()
+        feats, targets = [], []
+        for cam in range(4):
+            _, st, f = data(31 + cam)
+            feats.append(f)
+            targets.append(_dev(log2_steps(st)))
+        tf, tt = torch.cat(feats), torch.cat(targets)
+        n = tt.numel()
+        order = torch.randperm(n, device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
+        tf = tf.view(n, 16)[order].reshape(-1).contiguous()
+        tt = tt[order].contiguous()
+        bs = 65536
+        print("case: training on %d rays" % n, flush=True)
+        for ep in range(10):
+            for off in range(0, n - bs + 1, bs):
+                m.train_step(tf[off * 16:], tt[off:], bs, lr=1e-3)
+        trained = m.params()
+        # held-out camera
+        rays, steps, f = data(47)
+        nh = rays.shape[0]
+        target = _dev(log2_steps(steps))
+        nb = nh // 64 * 64
+        losses = {}
+        for name, params in (("untrained", untrained), ("trained", trained)):
+            m.set_params(params)
+            losses[name], _ = m.grad(f, target, nb)
+        assert losses["trained"] < 0.25 * losses["untrained"], losses
+        pred = torch.zeros(nh, dtype=torch.float32, device="cuda")
+        perm = torch.zeros(nh, dtype=torch.int32, device="cuda")
+        m.set_params(trained)
+        m.predict(f, nh, pred)
+        bin_rays(gw, pred, nh, perm, window=256, buckets=32, lo=0.0, hi=9.0)
+        gw.sync()
+        p_trained = pred.cpu().numpy()
+        c_trained = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+        bin_rays(gw, target, nh, perm, window=256, buckets=32, lo=0.0, hi=9.0)
+        gw.sync()
+        c_ideal = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+        mae = float(np.mean(np.abs(np.exp2(p_trained) - np.maximum(steps, 1))))
+        print("held-out log2 MSE: untrained %.3f trained %.3f; MAE %.2f steps (mean %.1f); wave cost: plain %d, "
+              "trained bins %d, true-cost bins %d" % (losses["untrained"], losses["trained"], mae, steps.mean(),
+                                                      wave_cost(steps), c_trained, c_ideal), flush=True)
+    finally:
+        m.close()
