@@ -45,6 +45,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-query leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned ray-cost binning leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is per-GPU work (rays shard with the brick map "
@@ -152,6 +153,7 @@ def main() -> int:
     gw.close()
     island = island_leg(args)
     batched = None if args.no_batch else batch_leg(args)
+    learned = None if args.no_learned else learned_leg(args)
 
     out = {
         "metric": METRIC,
@@ -174,6 +176,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "island_pass": island,
         "batched": batched,
+        "learned_binning": learned,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -366,6 +369,127 @@ def batch_leg(args):
         log("batched sample bit-exact vs oracle: %s" % ok)
     gw.close()
     return res
+
+
+def learned_leg(args):
+    """Learned ray-cost binning on the 512x256x512 scenes (plain, and with
+    the structural floor): a 16-64-64-1 BF16 MLP trained on the GPU with
+    Adam (lr 1e-3, batch 65,536) on ~16M (ray, step count) pairs that the
+    plain DDA produces for primary and shadow rays of seeded cameras, then
+    used on a held-out camera: features + prediction + 32-bucket binning in
+    windows + the permuted launch, against the unsorted launch."""
+    import torch
+    from pixels_amd import GpuWorld, Scene, Simulation
+    from pixels_amd import _native as N
+    from pixels_amd.learned import CostModel, bin_rays, log2_steps, trace_perm, wave_cost
+    from pixels_amd.scene import SMALL, SMALL_FLOOR
+
+    dev = torch.device("cuda:0")
+    W, H = 1920, 1080
+    out = {"scenes": [], "train_pairs": 0}
+
+    def tr(sim, gw, kind, rays_np, d_rays):
+        n = rays_np.shape[0]
+        o = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+        sim.trace_device(kind, d_rays, n, o)
+        gw.sync()
+        return o
+
+    t_all = time.time()
+    for kind_scene, seed in ((SMALL, 1), (SMALL_FLOOR, 3)):
+        scene = Scene.build(kind_scene, seed=seed)
+        gw = GpuWorld(scene)
+        sim = Simulation(gw)
+        model = CostModel(gw, seed=args.seed)
+
+        def rays_of(cam):
+            prim = scene.camera_rays(W, H, 90.0, seed=cam, tile=8)
+            dp = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
+            ph = tr(sim, gw, 0, prim, dp).cpu().numpy().view(N.HIT_DTYPE)
+            u, lights = scene.frame_inputs(W, H, seed=cam)
+            sh = shadow_queries(scene, prim, ph, u, lights, W * H)
+            ds = torch.from_numpy(sh.view(np.uint8).copy()).to(dev)
+            return [(0, prim, dp), (1, sh, ds)]
+
+        feats, targets = [], []
+        for cam in (seed * 100 + 1, seed * 100 + 2):
+            for kind, r, d in rays_of(cam):
+                n = r.shape[0]
+                f = torch.empty(n * 16, dtype=torch.int16, device=dev)
+                model.features(kind, d, n, f)
+                steps = tr(sim, gw, kind, r, d).cpu().numpy().view(N.HIT_DTYPE)["steps"]
+                feats.append(f)
+                targets.append(torch.from_numpy(log2_steps(steps)).to(dev))
+        tf, tt = torch.cat(feats), torch.cat(targets)
+        n = tt.numel()
+        out["train_pairs"] += n
+        order = torch.randperm(n, device=dev, generator=torch.Generator(device=dev).manual_seed(args.seed))
+        tf = tf.view(n, 16)[order].reshape(-1).contiguous()
+        tt = tt[order].contiguous()
+        bs, epochs = 65536, 4
+        gw.sync()
+        gw.profile_begin()
+        t0 = time.time()
+        first = model.train_step(tf, tt, bs, lr=1e-3, want_loss=True)
+        for ep in range(epochs):
+            for off in range(0, n - bs + 1, bs):
+                model.train_step(tf[off * 16:], tt[off:], bs, lr=1e-3)
+        last = model.train_step(tf, tt, bs, lr=1e-3, want_loss=True)
+        train_s = time.time() - t0
+        train_gpu_ms = gw.profile_end().get(N.PROF_MLP, (0.0, 0))[0]
+        del tf, tt, feats, targets
+        res = {"scene": list(scene.dims), "train_pairs": n, "epochs": epochs, "batch": bs,
+               "loss_first": round(first, 4), "loss_last": round(last, 4), "train_wall_s": round(train_s, 3),
+               "train_gpu_ms": round(train_gpu_ms, 3)}
+        for kind, r, d in rays_of(seed * 100 + 9):  # held-out camera
+            n = r.shape[0]
+            plain = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+            binned = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+            f = torch.empty(n * 16, dtype=torch.int16, device=dev)
+            pred = torch.empty(n, dtype=torch.float32, device=dev)
+            perm = torch.empty(n, dtype=torch.int32, device=dev)
+
+            def run_plain():
+                sim.trace_device(kind, d, n, plain)
+
+            def run_binned():
+                model.features(kind, d, n, f)
+                model.predict(f, n, pred)
+                bin_rays(gw, pred, n, perm, window=256, buckets=32, lo=0.0, hi=9.0)
+                trace_perm(gw, kind, d, perm, n, binned)
+            run_plain()
+            run_binned()
+            gw.sync()
+            reps = 5
+            gw.profile_begin()
+            for _ in range(reps):
+                run_plain()
+            p_ms = gw.profile_end()[N.PROF_TRACE][0] / reps
+            gw.profile_begin()
+            for _ in range(reps):
+                run_binned()
+            prof = gw.profile_end()
+            b_trace_ms = prof[N.PROF_TRACE][0] / reps
+            b_ms = b_trace_ms + prof.get(N.PROF_MLP, (0.0, 0))[0] / reps
+            hits = plain.cpu().numpy().view(N.HIT_DTYPE)
+            steps = hits["steps"].astype(np.int64)
+            pr = pred.cpu().numpy()
+            pm = perm.cpu().numpy().astype(np.int64)
+            res["primary" if kind == 0 else "shadow"] = {
+                "rays": n, "plain_grays_s": round(n / (p_ms * 1e-3) / 1e9, 4),
+                "binned_grays_s": round(n / (b_ms * 1e-3) / 1e9, 4),
+                "binned_trace_only_grays_s": round(n / (b_trace_ms * 1e-3) / 1e9, 4),
+                "speedup": round(p_ms / b_ms, 3), "speedup_trace_only": round(p_ms / b_trace_ms, 3),
+                "mae_steps": round(float(np.mean(np.abs(np.exp2(pr) - np.maximum(steps, 1)))), 2),
+                "mean_steps": round(float(steps.mean()), 2),
+                "wave_cost_plain": wave_cost(steps), "wave_cost_binned": wave_cost(steps[pm]),
+                "bit_identical": bool(torch.equal(plain, binned))}
+        out["scenes"].append(res)
+        log("learned binning %s: %s" % (scene.dims, json.dumps(res)))
+        model.close()
+        gw.close()
+    out["wall_s"] = round(time.time() - t_all, 2)
+    return out
 
 
 if __name__ == "__main__":

@@ -51,15 +51,10 @@ PX_DEV f32x4 mfma(s16x4 a, s16x4 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_
 // ld): lane reads S[r0 + lane % 16][k0 + 4 (lane / 16) .. +3].  This is the A
 // operand of an [r][k] matrix and the B operand of a [k][c] matrix stored as
 // S[c][k].
+// (S, ld and k0 are multiples of 4 elements, so each lane reads 8 aligned bytes.)
 PX_DEV s16x4 frag(const uint16_t* S, int ld, int r0, int k0) {
   const uint32_t l = __lane_id();
-  const uint16_t* p = S + (r0 + (int)(l & 15u)) * ld + k0 + 4 * (int)(l >> 4);
-  s16x4 v;
-  v.x = (short)p[0];
-  v.y = (short)p[1];
-  v.z = (short)p[2];
-  v.w = (short)p[3];
-  return v;
+  return *reinterpret_cast<const s16x4*>(S + (r0 + (int)(l & 15u)) * ld + k0 + 4 * (int)(l >> 4));
 }
 
 PX_DEV s16x4 pack4(float a, float b, float c, float d) {
@@ -72,7 +67,7 @@ PX_DEV s16x4 pack4(float a, float b, float c, float d) {
 }
 
 // bf16 copies of the weights in the layouts the fragments read
-struct MlpLds {
+struct alignas(16) MlpLds {
   uint16_t w1t[kH * kIn];  // W1T[o][f]
   uint16_t w2t[kH * kH];   // W2T[o][i]
   uint16_t w2[kH * kH];    // W2[i][o]
@@ -117,11 +112,7 @@ PX_DEV void forward(const MlpLds& L, const uint16_t* __restrict__ feat, uint32_t
   for (int c = 0; c < 4; c++) {
     const uint32_t s = s0 + 16u * c + (l & 15u);
     if (s < n) {
-      const uint16_t* p = feat + (size_t)s * kIn + 4 * g;
-      F.x[c].x = (short)p[0];
-      F.x[c].y = (short)p[1];
-      F.x[c].z = (short)p[2];
-      F.x[c].w = (short)p[3];
+      F.x[c] = *reinterpret_cast<const s16x4*>(feat + (size_t)s * kIn + 4 * g);
     } else {
       F.x[c] = s16x4{0, 0, 0, 0};
     }
@@ -160,13 +151,15 @@ PX_DEV void forward(const MlpLds& L, const uint16_t* __restrict__ feat, uint32_t
   }
 }
 
-__global__ __launch_bounds__(64) void k_mlp_predict(const uint16_t* __restrict__ wbf, const float* __restrict__ par,
-                                                    const uint16_t* __restrict__ feat, uint32_t n,
-                                                    float* __restrict__ pred) {
+// 4 wavefronts per block share the LDS weights; each takes 64 samples a turn.
+__global__ __launch_bounds__(256) void k_mlp_predict(const uint16_t* __restrict__ wbf, const float* __restrict__ par,
+                                                     const uint16_t* __restrict__ feat, uint32_t n,
+                                                     float* __restrict__ pred) {
   __shared__ MlpLds L;
   load_weights(L, wbf, par);
   const uint32_t l = __lane_id();
-  for (uint32_t s0 = blockIdx.x * 64u; s0 < n; s0 += gridDim.x * 64u) {
+  const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  for (uint32_t s0 = (blockIdx.x * waves + wave) * 64u; s0 < n; s0 += gridDim.x * waves * 64u) {
     Fwd F;
     forward(L, feat, n, s0, F);
     if (l < 16u)
@@ -184,11 +177,12 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wb
                                                  const uint16_t* __restrict__ feat, const float* __restrict__ target,
                                                  uint32_t n, float inv_batch, float* __restrict__ part) {
   __shared__ MlpLds L;
-  __shared__ uint16_t sA[kH * 64];   // feature-major [f][s]: H1, then dH1
-  __shared__ uint16_t sB[kH * 64];   // feature-major [f][s]: H2, then dH2
-  __shared__ uint16_t sX[kIn * 64];  // X^T feature-major [f][s]
-  __shared__ uint16_t sD[16 * 64];   // row 0 = dy (bf16), rows 1..15 zero; also ones for bias sums
-  __shared__ uint16_t sOne[16 * 64]; // column-vector of ones as a [c][s] B fragment source (row 0 = 1)
+  // fragments read 8 bytes per lane: every array is 16-byte aligned
+  __shared__ alignas(16) uint16_t sA[kH * 64];    // feature-major [f][s]: H1, then dH1
+  __shared__ alignas(16) uint16_t sB[kH * 64];    // feature-major [f][s]: H2, then dH2
+  __shared__ alignas(16) uint16_t sX[kIn * 64];   // X^T feature-major [f][s]
+  __shared__ alignas(16) uint16_t sD[16 * 64];    // row 0 = dy (bf16), rows 1..15 zero
+  __shared__ alignas(16) uint16_t sOne[16 * 64];  // row 0 = 1.0: the B fragment of a column of ones
   load_weights(L, wbf, par);
   const uint32_t l = __lane_id();
   const int g = (int)(l >> 4), col = (int)(l & 15u);
@@ -325,14 +319,16 @@ __global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wb
   if (l == 0u) out[kParams] = loss;
 }
 
-// Sums the partials of nb blocks in a fixed order (deterministic), one
-// parameter (or the loss) per thread.
-__global__ void k_mlp_reduce(const float* __restrict__ part, uint32_t nb, float* __restrict__ grad) {
+// Sums the partials of nb records in a fixed order (deterministic), one
+// parameter (or the loss) per thread; blockIdx.y takes records
+// [y * per, (y + 1) * per) and writes record y of `out`.
+__global__ void k_mlp_reduce(const float* __restrict__ part, uint32_t nb, uint32_t per, float* __restrict__ out) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= (uint32_t)kPartFloats) return;
+  const uint32_t b0 = blockIdx.y * per, b1 = min(b0 + per, nb);
   float s = 0.0f;
-  for (uint32_t b = 0; b < nb; b++) s += part[(size_t)b * kPartFloats + p];
-  grad[p] = s;
+  for (uint32_t b = b0; b < b1; b++) s += part[(size_t)b * kPartFloats + p];
+  out[(size_t)blockIdx.y * kPartFloats + p] = s;
 }
 
 // Adam (Kingma & Ba 2015, bias-corrected), then the bf16 copy of every weight.
@@ -584,6 +580,7 @@ int px_mlp_features(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, 
   PxWorldView w = px_view(c);
   const size_t full = (8u + (c->nsuper + 31u) / 32u) * 4u;
   const float horizon = (float)std::max(c->dims[0], std::max(c->dims[1], c->dims[2]));
+  PxProfScope prof(c, PX_PROF_MLP);
   if (full <= 64u * 1024u)
     hipLaunchKernelGGL(k_mlp_features<true>, dim3((n + 255u) / 256u), dim3(256), full, c->stream, w, d_rays, n, kind,
                        horizon, d_feat);
@@ -599,11 +596,11 @@ int px_mlp_predict(px_mlp* m, const uint16_t* d_feat, uint32_t n, float* d_pred)
   if (n == 0) return PX_OK;
   px_ctx* c = m->c;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  uint32_t blocks = (n + 63u) / 64u;
-  const uint32_t cap = c->n_cu * 16u;
+  uint32_t blocks = (n + 255u) / 256u;
+  const uint32_t cap = c->n_cu * 2u;
   if (blocks > cap) blocks = cap;
   PxProfScope prof(c, PX_PROF_MLP);
-  hipLaunchKernelGGL(k_mlp_predict, dim3(blocks), dim3(64), 0, c->stream, m->d_wbf, m->d_par, d_feat, n, d_pred);
+  hipLaunchKernelGGL(k_mlp_predict, dim3(blocks), dim3(256), 0, c->stream, m->d_wbf, m->d_par, d_feat, n, d_pred);
   PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
 }
@@ -615,14 +612,21 @@ static int mlp_grad(px_mlp* m, const uint16_t* d_feat, const float* d_target, ui
     (void)hipFree(m->d_part);
     m->d_part = nullptr;
     m->part_cap = 0;
-    MLP_HIP(m, hipMalloc(&m->d_part, (size_t)nb * kPartFloats * 4));
+    // per-block records, then the first-pass sums (one per 32 records)
+    MLP_HIP(m, hipMalloc(&m->d_part, ((size_t)nb + (nb + 31u) / 32u) * kPartFloats * 4));
     m->part_cap = nb;
   }
   hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, c->stream, m->d_wbf, m->d_par, d_feat, d_target, n,
                      1.0f / (float)n, m->d_part);
   MLP_HIP(m, hipGetLastError());
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartFloats + 255) / 256), dim3(256), 0, c->stream, m->d_part, nb,
-                     m->d_grad);
+  // two fixed-order passes: nb records -> splits records -> 1
+  const uint32_t per = 32u, splits = (nb + per - 1u) / per;
+  float* second = m->d_part + (size_t)nb * kPartFloats;
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartFloats + 255) / 256, splits), dim3(256), 0, c->stream, m->d_part, nb,
+                     per, second);
+  MLP_HIP(m, hipGetLastError());
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartFloats + 255) / 256, 1), dim3(256), 0, c->stream, second, splits,
+                     splits, m->d_grad);
   MLP_HIP(m, hipGetLastError());
   return PX_OK;
 }

@@ -54,12 +54,15 @@ class CostModel:
 
     def features(self, kind: int, rays_t, n: int, feat_t) -> None:
         """feat_t: n * 16 bf16 (a torch tensor of >= 32 n bytes)."""
+        self.gw.after_torch()
         self.gw.check(self.lib.px_mlp_features(self.gw.ctx, kind, _dp(rays_t), n, _dp(feat_t)))
 
     def predict(self, feat_t, n: int, pred_t) -> None:
+        self.gw.after_torch()
         self.gw.check(self.lib.px_mlp_predict(self.h, _dp(feat_t), n, _dp(pred_t)))
 
     def grad(self, feat_t, target_t, n: int):
+        self.gw.after_torch()
         g = np.zeros(self.n_params, np.float32)
         loss = C.c_float(0.0)
         self.gw.check(self.lib.px_mlp_grad(self.h, _dp(feat_t), _dp(target_t), n, N.ptr(g, C.c_float),
@@ -67,6 +70,7 @@ class CostModel:
         return float(loss.value), g
 
     def train_step(self, feat_t, target_t, n: int, lr: float = 1e-3, want_loss: bool = False):
+        self.gw.after_torch()
         loss = C.c_float(0.0)
         self.gw.check(self.lib.px_mlp_train_step(self.h, _dp(feat_t), _dp(target_t), n, lr,
                                                  C.byref(loss) if want_loss else None))
@@ -74,10 +78,12 @@ class CostModel:
 
 
 def bin_rays(gw, pred_t, n: int, perm_t, window: int = 256, buckets: int = 32, lo: float = 0.0, hi: float = 9.0):
+    gw.after_torch()
     gw.check(gw.lib.px_bin_rays(gw.ctx, _dp(pred_t), n, window, buckets, lo, hi, _dp(perm_t)))
 
 
 def trace_perm(gw, kind: int, rays_t, perm_t, n: int, out_t) -> None:
+    gw.after_torch()
     gw.check(gw.lib.px_trace_perm_device(gw.ctx, kind, _dp(rays_t), _dp(perm_t), n, _dp(out_t)))
 
 

@@ -79,6 +79,19 @@ class GpuWorld:
     def stream(self) -> int:
         return int(self.lib.px_stream(self.ctx) or 0)
 
+    def after_torch(self) -> None:
+        """Orders the library's stream after the work torch has queued on its
+        current stream (an event wait, no host stall): device buffers that
+        torch just wrote are complete before a kernel of the library reads
+        them.  The library's results are read back after px_sync()."""
+        import torch
+        if not torch.cuda.is_available():
+            return
+        ext = getattr(self, "_ext_stream", None)
+        if ext is None:
+            ext = self._ext_stream = torch.cuda.ExternalStream(self.stream)
+        ext.wait_stream(torch.cuda.current_stream())
+
     def sync(self) -> None:
         self.check(self.lib.px_sync(self.ctx))
 
@@ -166,6 +179,7 @@ class Simulation:
         return out
 
     def probe_device(self, origin, size, out_tensor) -> None:
+        self.gw.after_torch()
         o = (C.c_int32 * 3)(*origin)
         s = (C.c_int32 * 3)(*size)
         self.gw.check(self.lib.px_probe_device(self.gw.ctx, o, s, C.c_void_p(_devptr(out_tensor))))
@@ -201,6 +215,7 @@ class Simulation:
         return out
 
     def trace_device(self, kind: int, rays_t, n: int, out_t) -> None:
+        self.gw.after_torch()
         self.gw.check(self.lib.px_trace_device(self.gw.ctx, kind, C.c_void_p(_devptr(rays_t)), n,
                                                C.c_void_p(_devptr(out_t))))
 
@@ -208,6 +223,7 @@ class Simulation:
         return int(self.lib.px_frame_slots(self.gw.ctx))
 
     def trace_frame_device(self, rays_t, n: int, prim_t, shadow_t, count: bool = False):
+        self.gw.after_torch()
         nc = C.c_uint32(0)
         self.gw.check(self.lib.px_trace_frame_device(self.gw.ctx, C.c_void_p(_devptr(rays_t)), n,
                                                      C.c_void_p(_devptr(prim_t)), C.c_void_p(_devptr(shadow_t)),
@@ -226,6 +242,7 @@ class Simulation:
         return q
 
     def trace_batch(self, q: N.PxQuerySet) -> None:
+        self.gw.after_torch()
         self.gw.check(self.lib.px_trace_batch(self.gw.ctx, C.byref(q)))
 
     def set_batch_shape(self, block: int, blocks: int = 0, chunk: int = 256) -> None:
