@@ -69,7 +69,7 @@ struct px_ctx {
   uint64_t* d_super_bm = nullptr;
   bool masks_dirty = true;
   // frame
-  px_ray* d_shadow_rays = nullptr;
+  uint32_t* d_shadow_queue = nullptr;  // output slot of each queued shadow ray
   size_t shadow_cap = 0;
   uint32_t* d_wave_count = nullptr;  // shadow rays queued per wavefront region
   uint32_t wave_count_cap = 0;

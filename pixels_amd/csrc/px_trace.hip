@@ -262,101 +262,118 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_perm(PxWorldView w
   store_hit(out + i, m);
 }
 
-// Primary march + shadow-ray generation.  Wavefront `gw` owns the queue
-// region [gw * 64 * slots, (gw + 1) * 64 * slots): its shadow rays are packed
-// there slot by slot (lanes placed by ballot prefix, no atomics) with the
-// destination slot in `flags`, and their number goes to wave_count[gw].
+// Shadow ray `s` of shade() (raymarch.wgsl:323-356) for a primary hit at
+// t / face: slot 0 the sun, slot 1 + k light k.  Returns whether it is cast;
+// origin P + N 0.5, direction and max t are written either way.
+PX_DEV bool shadow_ray(const PxWorldView& w, const PxShadowDesc& sd, const float ro[3], const float rd[3], float t,
+                       uint32_t face, uint32_t value, uint32_t s, float O[3], float dir[3], float& mt) {
+  float N[3] = {0.0f, 0.0f, 0.0f};
+  if (face < 6u) N[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
+  float P[3];
+  for (int a = 0; a < 3; a++) {
+    P[a] = ro[a] + rd[a] * t;
+    O[a] = P[a] + N[a] * 0.5f;
+  }
+  dir[0] = dir[1] = dir[2] = 0.0f;
+  mt = 0.0f;
+  if (face == 0xFFu || w.mats[(value & 0xffu) * 20u + 3u] > 0.0f) return false;  // miss or emissive
+  if (s == 0u) {
+    const float ndl = N[0] * sd.sun[0] + N[1] * sd.sun[1] + N[2] * sd.sun[2];
+    dir[0] = sd.sun[0];
+    dir[1] = sd.sun[1];
+    dir[2] = sd.sun[2];
+    mt = 1e9f;
+    return ndl > 0.0f && sd.sun_w > 0.0f;
+  }
+  const float* lp = sd.lp[s - 1u];
+  float L[3];
+  for (int a = 0; a < 3; a++) L[a] = lp[a] / sd.vs - P[a];
+  const float radius = lp[3] / sd.vs;
+  const float d = sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  if (d > radius) return false;
+  for (int a = 0; a < 3; a++) L[a] = L[a] / d;
+  const float nl = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
+  dir[0] = L[0];
+  dir[1] = L[1];
+  dir[2] = L[2];
+  mt = d - 2.0f;
+  return nl > 0.0f && sd.lc_w[s - 1u] > 0.5f && sd.lamp_shadows > 0.5f;
+}
+
+// Primary march + the decision of which shadow rays to cast.  Wavefront `gw`
+// owns the queue region [gw * 64 * slots, (gw + 1) * 64 * slots): the output
+// slot (pixel * slots + s) of each cast ray, packed there slot by slot (lanes
+// placed by ballot prefix, no atomics); its count goes to wave_count[gw].
+// The shadow kernel rebuilds each ray from the primary ray and hit, so a
+// queue entry is 4 bytes instead of a 32-byte ray.
 template <bool SUP_LDS>
 __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_frame_primary(PxWorldView w, PxShadowDesc sd,
                                                        const px_ray* __restrict__ rays, uint32_t n,
-                                                       px_hit* __restrict__ prim_out, px_hit* __restrict__ sh_out,
-                                                       px_ray* __restrict__ queue, uint32_t* __restrict__ wave_count) {
+                                                       px_hit* __restrict__ prim_out,
+                                                       uint32_t* __restrict__ queue, uint32_t* __restrict__ wave_count) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = i < n;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
   float ro[3] = {0, 0, 0}, rd[3] = {0, 0, 1}, maxT = 0.0f;
   uint32_t flags = 0;
   MarchOut h;
   h.face = 0xFFu;
   h.value = 0u;
+  h.t = 0.0f;
   if (live) {
     load_ray(rays + i, ro, rd, maxT, flags);
     h = px::march<false>(w, T, ro, rd, maxT);
     store_hit(prim_out + i, h);
-    for (uint32_t s = 0; s < sd.slots; s++) store_not_cast(sh_out + (size_t)i * sd.slots + s);
   }
-  bool shade = live && h.face != 0xFFu && !(w.mats[(h.value & 0xffu) * 20u + 3u] > 0.0f);
-  float N[3] = {0.0f, 0.0f, 0.0f};
-  if (h.face < 6u) N[h.face >> 1] = (h.face & 1u) ? -1.0f : 1.0f;
-  float P[3], O[3];
-  for (int a = 0; a < 3; a++) {
-    P[a] = ro[a] + rd[a] * h.t;
-    O[a] = P[a] + N[a] * 0.5f;
-  }
-  // up to `slots` rays per lane; each slot is compacted across the wave
   const uint32_t gwave = i >> 6;
-  px_ray* __restrict__ wq = queue + (size_t)gwave * 64u * sd.slots;
+  uint32_t* __restrict__ wq = queue + (size_t)gwave * 64u * sd.slots;
   const uint32_t lane = __lane_id();
   uint32_t wcount = 0u;
   for (uint32_t s = 0; s < sd.slots; s++) {
-    bool cast = false;
-    float dir[3] = {0, 0, 0}, mt = 0.0f;
-    if (shade) {
-      if (s == 0) {
-        float ndl = N[0] * sd.sun[0] + N[1] * sd.sun[1] + N[2] * sd.sun[2];
-        if (ndl > 0.0f && sd.sun_w > 0.0f) {
-          cast = true;
-          dir[0] = sd.sun[0]; dir[1] = sd.sun[1]; dir[2] = sd.sun[2];
-          mt = 1e9f;
-        }
-      } else {
-        const float* lp = sd.lp[s - 1];
-        float L[3];
-        for (int a = 0; a < 3; a++) L[a] = lp[a] / sd.vs - P[a];
-        float radius = lp[3] / sd.vs;
-        float d = sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
-        if (!(d > radius)) {
-          for (int a = 0; a < 3; a++) L[a] = L[a] / d;
-          float nl = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
-          if (nl > 0.0f && sd.lc_w[s - 1] > 0.5f && sd.lamp_shadows > 0.5f) {
-            cast = true;
-            dir[0] = L[0]; dir[1] = L[1]; dir[2] = L[2];
-            mt = d - 2.0f;
-          }
-        }
-      }
-    }
+    float O[3], dir[3], mt;
+    const bool cast = live && shadow_ray(w, sd, ro, rd, h.t, h.face, h.value, s, O, dir, mt);
     const uint64_t mask = __ballot(cast);
-    if (cast) {
-      uint32_t off = wcount + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-      float4* q = reinterpret_cast<float4*>(wq + off);
-      q[0] = make_float4(O[0], O[1], O[2], dir[0]);
-      q[1] = make_float4(dir[1], dir[2], mt, __uint_as_float(i * sd.slots + s));
-    }
+    if (cast) wq[wcount + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = i * sd.slots + s;
     wcount += (uint32_t)__popcll(mask);
   }
   if (lane == 0u) wave_count[gwave] = wcount;
 }
 
-// One wavefront per queue region of k_frame_primary.
+// One wavefront per queue region of k_frame_primary.  It first writes the
+// "not cast" record to all 64 x slots outputs of its pixels (one contiguous
+// run), then traces the queued rays, whose hits overwrite their slots in
+// lines the first pass left in L2.
 template <bool SUP_LDS>
-__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWorldView w, const px_ray* __restrict__ queue,
+__global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWorldView w, PxShadowDesc sd,
+                                                            const px_ray* __restrict__ rays,
+                                                            const px_hit* __restrict__ prim,
+                                                            const uint32_t* __restrict__ queue,
                                                             const uint32_t* __restrict__ wave_count,
-                                                            uint32_t n_regions, uint32_t slots,
-                                                            px_hit* __restrict__ sh_out) {
+                                                            uint32_t n, px_hit* __restrict__ sh_out) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   const uint32_t region = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t n_regions = (n + 63u) / 64u;
   if (region >= n_regions) return;
+  const uint32_t slots = sd.slots;
+  const uint32_t first = region * 64u * slots, last = min(region * 64u + 64u, n) * slots;
+  for (uint32_t k = first + __lane_id(); k < last; k += 64u) store_not_cast(sh_out + k);
+  __threadfence_block();  // the hits below land after these records
   const uint32_t cnt = wave_count[region];
-  const px_ray* __restrict__ rq = queue + (size_t)region * 64u * slots;
+  const uint32_t* __restrict__ rq = queue + (size_t)region * 64u * slots;
   for (uint32_t k = __lane_id(); k < cnt; k += 64u) {
+    const uint32_t code = rq[k];
+    const uint32_t i = code / slots, s = code - i * slots;
     float ro[3], rd[3], maxT;
-    uint32_t slot;
-    load_ray(rq + k, ro, rd, maxT, slot);
-    MarchOut m = px::march<true>(w, T, ro, rd, maxT);
-    store_hit(sh_out + slot, m);
+    uint32_t flags;
+    load_ray(rays + i, ro, rd, maxT, flags);
+    const uint4 hv = *reinterpret_cast<const uint4*>(prim + i);
+    const uint32_t face = hv.w & 0xffu, value = (hv.w >> 8) & 0xffffu;
+    float O[3], dir[3], mt;
+    shadow_ray(w, sd, ro, rd, __uint_as_float(hv.y), face, value, s, O, dir, mt);
+    MarchOut m = px::march<true>(w, T, O, dir, mt);
+    store_hit(sh_out + code, m);
   }
 }
 
@@ -600,15 +617,15 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
   }
   int rc = px_refresh_masks(c);
   if (rc) return rc;
-  // shadow-ray queue: one region of 64 * slots rays per wavefront of primaries
+  // shadow-ray queue: one region of 64 * slots entries per wavefront of primaries
   const uint32_t n_regions = (n + 63u) / 64u;
   const size_t need = (size_t)n_regions * 64u * c->shadow.slots;
   if (need > 0xFFFFFFFFull) return px_fail(c, PX_ERR_CAPACITY, "px_trace_frame_device", "too many shadow slots");
   if (need > c->shadow_cap) {
-    if (c->d_shadow_rays) hipFree(c->d_shadow_rays);
-    c->d_shadow_rays = nullptr;
+    if (c->d_shadow_queue) (void)hipFree(c->d_shadow_queue);
+    c->d_shadow_queue = nullptr;
     c->shadow_cap = 0;
-    PX_HIP_CHECK(c, hipMalloc(&c->d_shadow_rays, need * sizeof(px_ray)));
+    PX_HIP_CHECK(c, hipMalloc(&c->d_shadow_queue, need * sizeof(uint32_t)));
     c->shadow_cap = need;
   }
   if (n_regions > c->wave_count_cap) {
@@ -626,21 +643,21 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
     dim3 grid((n + 255u) / 256u), block(256);
     if (sup_lds)
       hipLaunchKernelGGL(k_frame_primary<true>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
-                         d_shadow, c->d_shadow_rays, c->d_wave_count);
+                         c->d_shadow_queue, c->d_wave_count);
     else
       hipLaunchKernelGGL(k_frame_primary<false>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
-                         d_shadow, c->d_shadow_rays, c->d_wave_count);
+                         c->d_shadow_queue, c->d_wave_count);
   }
   PX_HIP_CHECK(c, hipGetLastError());
   const uint32_t blocks = (n_regions + 3u) / 4u;
   {
     PxProfScope prof(c, PX_PROF_SHADOW);
     if (sup_lds)
-      hipLaunchKernelGGL(k_trace_shadow_queue<true>, dim3(blocks), dim3(256), lds, c->stream, w, c->d_shadow_rays,
-                         c->d_wave_count, n_regions, c->shadow.slots, d_shadow);
+      hipLaunchKernelGGL(k_trace_shadow_queue<true>, dim3(blocks), dim3(256), lds, c->stream, w, c->shadow,
+                         d_primary, d_prim, c->d_shadow_queue, c->d_wave_count, n, d_shadow);
     else
-      hipLaunchKernelGGL(k_trace_shadow_queue<false>, dim3(blocks), dim3(256), lds, c->stream, w, c->d_shadow_rays,
-                         c->d_wave_count, n_regions, c->shadow.slots, d_shadow);
+      hipLaunchKernelGGL(k_trace_shadow_queue<false>, dim3(blocks), dim3(256), lds, c->stream, w, c->shadow,
+                         d_primary, d_prim, c->d_shadow_queue, c->d_wave_count, n, d_shadow);
   }
   PX_HIP_CHECK(c, hipGetLastError());
   if (n_cast) {
