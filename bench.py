@@ -29,6 +29,26 @@ BYTES_IN, BYTES_OUT, BYTES_STEP = 28, 13, 4
 COPY_BYTES = 256 << 20
 
 
+TRAFFIC_FILE = "profiles/hbm_traffic_frame_1920x1080.md"
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch and L2 hit % of `kernel` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy);
+    (None, None) if the summary has no line for it."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    traffic = hit = None
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[1] == kernel:
+                if f[0] == "traffic_bytes":
+                    traffic = int(f[2])
+                elif f[0] == "l2_hit_pct":
+                    hit = float(f[2])
+    return traffic, hit
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -126,8 +146,10 @@ def main() -> int:
     else:
         dom, dom_bytes, dom_ms = names[1], bytes_prim, ms_p
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    traffic, l2_hit = profiled_traffic(dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "l2_hit_pct": l2_hit,
+                "traffic_source": TRAFFIC_FILE if traffic is not None else None, "kernel": dom,
                 "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes,
                 "kernels_ms": {"primary": round(ms_p, 4), "shadow": round(ms_s, 4)}}
 

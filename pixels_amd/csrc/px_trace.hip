@@ -342,8 +342,7 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_frame_primary(PxWorldVie
 
 // One wavefront per queue region of k_frame_primary.  It first writes the
 // "not cast" record to all 64 x slots outputs of its pixels (one contiguous
-// run), then traces the queued rays, whose hits overwrite their slots in
-// lines the first pass left in L2.
+// run), then traces the queued rays and overwrites their slots with the hits.
 template <bool SUP_LDS>
 __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWorldView w, PxShadowDesc sd,
                                                             const px_ray* __restrict__ rays,
