@@ -230,6 +230,18 @@ int px_push_impacts(px_ctx* ctx, const float* impacts16, uint32_t n);
  * (the anchor rules of visit/searchStep, :767-827).  Asynchronous; labels are
  * read with px_island_labels. */
 int px_island_pass(px_ctx* ctx);
+/* Replaces the per-frame `Simulation.encode(enc)` + submit of the hot-path
+ * stages (Simulation.ts:200-273): runs the selected stages in the
+ * reference's order on the context's stream, non-blocking.  PX_STAGE_EDIT =
+ * the island pass above (a no-op without queued impacts).  Unknown bits are
+ * an error. */
+#define PX_STAGE_EDIT 1u
+int px_sim_step(px_ctx* ctx, uint32_t stages_mask);
+/* The edit stages run until no work is pending (the flood and the island
+ * search of the reference iterate over frames until their queues empty and
+ * C_S_STATE is 0; here one island pass reaches that state), then waits for
+ * the stream.  Blocking. */
+int px_run_to_quiescence(px_ctx* ctx);
 /* Unsupported components of the last island pass, sorted by label: the
  * canonical label (minimum voxel key of the component) and its voxel count.
  * *n receives the total count even when it exceeds cap.  Blocking. */

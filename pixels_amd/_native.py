@@ -23,6 +23,7 @@ PX_ARGS_WORDS = 44
 PX_CAMERA_FLOATS = 64
 
 RAY_PRIMARY, RAY_SHADOW, RAY_SHOOT, RAY_SEGMENT, RAY_MIXED = 0, 1, 2, 3, 4
+STAGE_EDIT = 1
 FACE_ZERO, FACE_NOT_CAST, FACE_MISS = 6, 0xFE, 0xFF
 
 # CTRL_NAMES order, src/gpu/layout.ts:48-58
@@ -101,6 +102,8 @@ SIGNATURES = {
     "px_world_info": (C.c_int, [_P, _U32P, _U32P, _U32P, _U32P]),
     "px_push_impacts": (C.c_int, [_P, _FP, _U32]),
     "px_island_pass": (C.c_int, [_P]),
+    "px_sim_step": (C.c_int, [_P, _U32]),
+    "px_run_to_quiescence": (C.c_int, [_P]),
     "px_island_labels": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
     "px_snapshot": (C.c_int, [_P]),
     "px_restore": (C.c_int, [_P]),

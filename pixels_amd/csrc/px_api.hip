@@ -516,6 +516,25 @@ int px_island_pass(px_ctx* c) {
   return px_launch_island_pass(c);
 }
 
+int px_sim_step(px_ctx* c, uint32_t stages_mask) {
+  if (!c) return PX_ERR_ARG;
+  if (stages_mask & ~PX_STAGE_EDIT) return px_fail(c, PX_ERR_ARG, "px_sim_step", "unknown stage bits");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if ((stages_mask & PX_STAGE_EDIT) && c->n_impacts) return px_launch_island_pass(c);
+  return PX_OK;
+}
+
+int px_run_to_quiescence(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (c->n_impacts) {
+    int rc = px_launch_island_pass(c);
+    if (rc) return rc;
+  }
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
 int px_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n) {
   if (!c || !n || (cap && (!min_key || !count))) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));

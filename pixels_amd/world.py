@@ -267,6 +267,13 @@ class Simulation:
     def island_pass(self) -> None:
         self.gw.check(self.lib.px_island_pass(self.gw.ctx))
 
+    def encode(self, stages: int = N.STAGE_EDIT) -> None:
+        """Simulation.encode (Simulation.ts:200-273): the hot-path stages, non-blocking."""
+        self.gw.check(self.lib.px_sim_step(self.gw.ctx, stages))
+
+    def run_to_quiescence(self) -> None:
+        self.gw.check(self.lib.px_run_to_quiescence(self.gw.ctx))
+
     def island_labels(self, cap: int = 1 << 20):
         keys = np.zeros(cap, np.uint32)
         cnt = np.zeros(cap, np.uint32)

@@ -192,3 +192,34 @@ def test_alloc_failure_is_counted():
     cells = np.unique(np.stack(np.nonzero(removed), 1) // 8, axis=0)
     assert 0 < cells.shape[0] <= 4
     gw.close()
+
+
+def test_sim_step_and_quiescence(floor_scene, floor_impacts):
+    """px_sim_step(PX_STAGE_EDIT) runs the same pass as px_island_pass (and is
+    a no-op without queued impacts); px_run_to_quiescence leaves no pending
+    impacts; unknown stage bits are an error."""
+    from pixels_amd import GpuWorld, PxError, Simulation
+    s = floor_scene
+    gw = GpuWorld(s)
+    sim = Simulation(gw)
+    try:
+        print("case: sim_step", flush=True)
+        sim.encode()  # nothing queued: no-op
+        gw.sync()
+        grid0, _ = gw.read_grid()
+        assert np.array_equal(grid0, s.grid)
+        sim.push_impacts(floor_impacts[:32])
+        sim.encode()
+        keys, cnt, n = sim.island_labels()
+        vox = sim.probe((0, 0, 0), s.dims).reshape(s.dims[2], s.dims[1], s.dims[0])
+        grid, sgrid = gw.read_grid()
+        ctrl, _, _ = sim.read_ctrl()
+        check_against_oracle(s, floor_impacts[:32], dict(keys=keys, cnt=cnt, n=n, vox=vox, grid=grid, sgrid=sgrid,
+                                                          ctrl=ctrl))
+        sim.run_to_quiescence()  # nothing pending any more
+        grid2, _ = gw.read_grid()
+        assert np.array_equal(grid2, grid)
+        with pytest.raises(PxError):
+            sim.encode(2)
+    finally:
+        gw.close()
