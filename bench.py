@@ -66,6 +66,7 @@ def main() -> int:
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-query leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned ray-cost binning leg")
+    ap.add_argument("--no-debris", action="store_true", help="skip the debris-particle leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is per-GPU work (rays shard with the brick map "
@@ -172,6 +173,7 @@ def main() -> int:
         parity = same(prim, rp) and same(sh, rs) and rnc == n_cast
         log("bit-exact vs oracle: %s" % parity)
 
+    debris_res = None if args.no_debris else debris_leg(args, scene, gw, u, rays, prim)
     gw.close()
     island = island_leg(args)
     batched = None if args.no_batch else batch_leg(args)
@@ -199,6 +201,7 @@ def main() -> int:
         "island_pass": island,
         "batched": batched,
         "learned_binning": learned,
+        "debris": debris_res,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -255,6 +258,136 @@ def island_leg(args):
                                           and np.array_equal(vox, ovox.astype(np.uint32)))
         log("island pass bit-exact vs oracle: %s" % res["bit_exact_vs_oracle"])
     gw.close()
+    return res
+
+
+DEBRIS_RING = 1 << 18   # MAX_PARTICLES default, config.ts:38
+DEBRIS_BIG = 1 << 21    # its maximum
+# algorithmic bytes per live slot of k_particle_sim: the 32-byte record read
+# and written back, the status byte; per block of 256 slots 8 bytes of counts
+DEBRIS_SLOT_BYTES = 65
+
+
+def surface_debris(dims, prim_rays, prim_hits, n, seed):
+    """n debris records at the air voxels in front of primary-ray hits (where
+    carveEffects spawns them, sim.wgsl:495-509), thrown off the surface at up
+    to 60 voxels/s; flags and life as pixels_amd.scene.debris."""
+    from pixels_amd import _native as N
+    from pixels_amd.scene import debris
+    rng = np.random.default_rng(seed)
+    ok = np.nonzero(prim_hits["face"] < 6)[0]
+    pick = ok[rng.integers(0, ok.size, n)]
+    h = prim_hits[pick]
+    dx, dy = dims[0], dims[1]
+    key = h["key"].astype(np.int64)
+    cell = np.stack([key % dx, (key // dx) % dy, key // (dx * dy)], 1)
+    normal = np.zeros((n, 3), np.int64)
+    ax = h["face"].astype(np.int64) // 2
+    sgn = np.where(h["face"] % 2 == 0, 1, -1)  # dirOf order: +x, -x, +y, -y, +z, -z
+    normal[np.arange(n), ax] = sgn
+    rec = debris(dims, n, seed=seed, speed=20.0)
+    rec["pos"] = (cell + normal + 0.5).astype(np.float32)
+    rec["vel"] = (rec["vel"] + normal * rng.uniform(5.0, 40.0, (n, 1))).astype(np.float32)
+    return rec.astype(N.PARTICLE_DTYPE)
+
+
+def debris_leg(args, scene, gw, u, prim_rays, prim_hits):
+    """Debris step (frameBegin + particleSim + alive / write lists) over a ring
+    whose every slot holds a live particle thrown off the frame's visible
+    surfaces, on the same grid as the frame; 60 fps, gravity 9.81 m/s^2 at 80
+    voxels per metre."""
+    import torch
+
+    import oracle
+    from pixels_amd import Simulation
+    from pixels_amd import _native as N
+
+    sim = Simulation(gw)
+    uu = np.array(u, np.float32).copy()
+    dt, vs = np.float32(1.0 / 60.0), np.float32(uu[3])
+    g = np.float32(9.81) / vs
+    uu[24], uu[25] = dt, g
+    gw.set_uniforms(uu)
+    res = {}
+    for ring in (DEBRIS_RING, DEBRIS_BIG):
+        lim = gw.limits
+        recs = surface_debris(scene.dims, prim_rays, prim_hits, ring, seed=17)
+        sim.particles_init(ring)
+        d_recs = torch.from_numpy(recs.view(np.uint8).copy()).cuda()
+        torch.cuda.synchronize()
+        # fill the ring in budget-sized spawns, one frame each
+        b = int(lim.spawn_budget)
+        for off in range(0, ring, b):
+            sim.frame_begin()
+            sim.spawn_device(d_recs[off * 32:], min(b, ring - off))
+        gw.sync()
+        start = sim.read_particles()
+        steps = max(args.steps, 10)
+        for _ in range(args.warmup):
+            sim.frame_begin()
+            sim.particle_step()
+        gw.sync()
+        # timed: restart from the filled ring so every run sees the same states
+        sim.particles_init(ring)
+        for off in range(0, ring, b):
+            sim.frame_begin()
+            sim.spawn_device(d_recs[off * 32:], min(b, ring - off))
+        gw.sync()
+        gw.profile_begin()
+        for _ in range(steps):
+            sim.frame_begin()
+            sim.particle_step()
+        prof = gw.profile_end()
+        alive = len(sim.read_alive())
+        ms = prof[N.PROF_PARTICLES][0] / prof[N.PROF_PARTICLES][1]
+        sim_ms = prof[N.PROF_PARTICLE_SIM][0] / prof[N.PROF_PARTICLE_SIM][1]
+        algo = DEBRIS_SLOT_BYTES * ring + 8 * (ring // 256)
+        r = {"ring": ring, "steps": steps, "ms_per_step": round(ms, 5), "sim_kernel_ms": round(sim_ms, 5),
+             "particles_per_s": round(ring / (ms * 1e-3), 1), "alive_after": alive,
+             "roofline": {"bound": "hbm", "kernel": "k_particle_sim", "algorithmic_bytes": algo,
+                          "achieved": round(algo / (sim_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(algo / (sim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        log("debris ring %d: %.4f ms/step (sim kernel %.4f ms), %.3g particles/s, %d alive after %d steps" %
+            (ring, ms, sim_ms, r["particles_per_s"], alive, steps))
+        if ring == DEBRIS_RING and not args.no_cpu:
+            # parity of one step from the filled ring, and the CPU baseline
+            w = oracle.World.from_scene(scene)
+            P = oracle.Particles(ring, spawn_budget=int(lim.spawn_budget), max_writes=int(lim.max_writes))
+            sim.particles_init(ring)
+            for off in range(0, ring, b):
+                sim.frame_begin()
+                P.frame_begin()
+                sim.spawn_device(d_recs[off * 32:], min(b, ring - off))
+                P.spawn(recs[off:off + b])
+            sim.frame_begin()
+            sim.particle_step()
+            # CPU baseline: the same `steps`-step sequence from the filled
+            # ring, repeated from a saved copy for about 10 s
+            saved = (P.parts.copy(), P.ctr.copy())
+            t1 = time.time()
+            k = 0
+            while k == 0 or (time.time() - t1 < 10.0 and k < 50 * steps):
+                P.parts[:], P.ctr[:] = saved
+                for _ in range(steps):
+                    P.frame_begin()
+                    P.step(w, dt, g, vs)
+                    k += 1
+            cpu_s = (time.time() - t1) / k
+            got = sim.read_particles()
+            P1 = oracle.Particles(ring, spawn_budget=int(lim.spawn_budget), max_writes=int(lim.max_writes))
+            for off in range(0, ring, b):
+                P1.frame_begin()
+                P1.spawn(recs[off:off + b])
+            P1.frame_begin()
+            P1.step(w, dt, g, vs)
+            r["bit_exact_vs_oracle"] = bool(got.tobytes() == P1.parts.tobytes() and
+                                            np.array_equal(sim.read_alive(), P1.alive_list()) and
+                                            np.array_equal(sim.read_writes(), P1.write_list()))
+            r["cpu_baseline"] = {"value": round(ring / cpu_s, 1), "unit": "particles/s", "cores": 1, "kind": "port",
+                                 "sample": "%d oracle steps of the same %d-slot ring, %.3f s each" % (k, ring, cpu_s)}
+            log("debris bit-exact vs oracle: %s; oracle %.3f s/step" % (r["bit_exact_vs_oracle"], cpu_s))
+        res["ring_%d" % ring] = r
+        del d_recs
     return res
 
 

@@ -236,6 +236,11 @@ int px_island_pass(px_ctx* ctx);
  * the island pass above (a no-op without queued impacts).  Unknown bits are
  * an error. */
 #define PX_STAGE_EDIT 1u
+/* PX_STAGE_FRAME_BEGIN = px_frame_begin, run first; PX_STAGE_PARTICLES =
+ * px_particle_step, run after the edit stage as particleSim runs after the
+ * edit passes (Simulation.ts:223-262). */
+#define PX_STAGE_PARTICLES 2u
+#define PX_STAGE_FRAME_BEGIN 4u
 int px_sim_step(px_ctx* ctx, uint32_t stages_mask);
 /* The edit stages run until no work is pending (the flood and the island
  * search of the reference iterate over frames until their queues empty and
@@ -246,6 +251,59 @@ int px_run_to_quiescence(px_ctx* ctx);
  * canonical label (minimum voxel key of the component) and its voxel count.
  * *n receives the total count even when it exceeds cap.  Blocking. */
 int px_island_labels(px_ctx* ctx, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+
+/* ---- debris particles ------------------------------------------------------
+ * Particle record (shared.wgsl:33-42): position and velocity in voxel units,
+ * the 16-bit voxel value in the low bits of `value` with the flags above it,
+ * seconds of life left (<= 0 = free slot). */
+typedef struct px_particle {
+  float pos[3];
+  uint32_t value;
+  float vel[3];
+  float life;
+} px_particle;
+#define PX_P_SETTLED 0x10000u
+#define PX_P_DEPOSIT 0x20000u
+#define PX_P_DUST 0x40000u
+
+/* Allocates the particle ring of max_particles slots (a power of two in
+ * [2^12, 2^21], MAX_PARTICLES of config.ts:38; 0 = 2^18), the alive list
+ * (H_ALIVE, layout.ts:105) and the write list (H_WRITE_LIST, max_writes
+ * pairs, layout.ts:91), all zeroed, and zeroes C_PCURSOR.  Again to resize
+ * (the ring is cleared).  Blocking. */
+int px_particles_init(px_ctx* ctx, uint32_t max_particles);
+/* frameBegin (sim.wgsl:363-369): C_SPAWNED, C_DIRTY, C_WRITES and C_ALLOCS
+ * to 0, the particle draw args (A_DRAWP) to (36, 0, 0, 0).  Asynchronous. */
+int px_frame_begin(px_ctx* ctx);
+/* spawn() (sim.wgsl:166-170) for every record of src in order: a record past
+ * the frame's spawn budget (C_SPAWNED >= spawn_budget) is dropped, the others
+ * go to ring slot C_PCURSOR++ & (max_particles - 1); C_SPAWNED counts every
+ * call.  px_spawn takes a host buffer, px_spawn_device a device buffer (both
+ * asynchronous; the host buffer may be reused on return). */
+int px_spawn(px_ctx* ctx, const px_particle* src, uint32_t n);
+int px_spawn_device(px_ctx* ctx, const px_particle* d_src, uint32_t n);
+/* particleSim (sim.wgsl:1088-1173) over every slot with dt = U.sim.x,
+ * gravity = U.sim.y and the voxel size U.dims.w of the last px_set_uniforms:
+ * life, gravity and drag, the pop out of solids, the 48-cell segment DDA with
+ * its bounce, the rest check that settles a particle and, for a DEPOSIT
+ * particle inside the world, requests a rubble voxel write (pushWrite
+ * :186-192).  The reference appends alive slots (pushAlive :1082-1085, count
+ * in A_DRAWP + 1) and write requests (C_WRITES) with atomics in schedule
+ * order; here both lists are in slot order, and when requests exceed
+ * max_writes the lowest slots get the writes (the rest settle with the
+ * failed-write life, as the reference's losers do).  Asynchronous. */
+int px_particle_step(px_ctx* ctx);
+/* Copies ring slots [first, first + count) to the host.  Blocking. */
+int px_read_particles(px_ctx* ctx, uint32_t first, uint32_t count, px_particle* out);
+/* The alive list of the frame (slot indices, args[A_DRAWP + 1] of them) and
+ * the write list (key, value pairs, min(C_WRITES, max_writes) of them, the
+ * count writeArgs :1179-1184 takes).  *n receives the full count; at most cap
+ * entries are copied.  Blocking. */
+int px_read_alive(px_ctx* ctx, uint32_t* out, uint32_t cap, uint32_t* n);
+int px_read_writes(px_ctx* ctx, uint32_t* pairs, uint32_t cap, uint32_t* n);
+/* Device addresses of the ring (px_particle[max_particles]), the alive list
+ * and the write list, for callers that draw or consume them on the GPU. */
+int px_particle_buffers(px_ctx* ctx, void** particles, void** alive, void** writes);
 
 /* ---- checkpoint of the device world (no counterpart in the reference, which
  * rebuilds its world from the seed on every load and never saves GPU state).
@@ -267,6 +325,8 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_EDIT 3     /* island pass, all kernels */
 #define PX_PROF_BATCH 4    /* px_trace_batch (the graph launch) */
 #define PX_PROF_MLP 5      /* ray-cost predictor: features, predict, train, bin */
+#define PX_PROF_PARTICLES 6 /* spawn + particle step, all kernels */
+#define PX_PROF_PARTICLE_SIM 7 /* the integration kernel of the step alone */
 #define PX_PROF_SLOTS 8
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);

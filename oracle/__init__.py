@@ -19,6 +19,9 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 HIT_DTYPE = np.dtype([("key", "<u4"), ("t", "<f4"), ("steps", "<u4"), ("face", "u1"), ("mat", "u1"),
                       ("value_hi", "<u2")])
 RAY_DTYPE = np.dtype([("ro", "<f4", 3), ("rd", "<f4", 3), ("max_t", "<f4"), ("flags", "<u4")])
+PARTICLE_DTYPE = np.dtype([("pos", "<f4", 3), ("value", "<u4"), ("vel", "<f4", 3), ("life", "<f4")])
+# counter words of the particle oracle: C_SPAWNED, C_PCURSOR, C_WRITES, alive count
+P_SPAWNED, P_PCURSOR, P_WRITES, P_ALIVE = range(4)
 
 _lib = None
 
@@ -55,6 +58,9 @@ def load():
     lib.o_rnd.argtypes = [C.c_uint32]
     lib.o_removes.restype = C.c_int
     lib.o_removes.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+    lib.o_frame_begin.argtypes = [P]
+    lib.o_spawn.argtypes = [P, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
+    lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
     _lib = lib
     return lib
 
@@ -108,6 +114,42 @@ class World:
         load().o_frame(C.byref(self.w), u52.ctypes.data, lights.ctypes.data, lights.size // 4, rays.ctypes.data, n,
                        prim.ctypes.data, sh.ctypes.data, C.byref(nc), threads or os.cpu_count() or 1)
         return prim, sh, int(nc.value)
+
+
+class Particles:
+    """Serial restatement of the debris particle state: the ring of
+    max_particles records, C_SPAWNED / C_PCURSOR / C_WRITES, the alive list
+    and the write list of the last step."""
+
+    def __init__(self, max_particles: int, spawn_budget: int = 60000, max_writes: int = 131072):
+        assert max_particles & (max_particles - 1) == 0
+        self.n = int(max_particles)
+        self.budget = int(spawn_budget)
+        self.max_writes = int(max_writes)
+        self.parts = np.zeros(self.n, PARTICLE_DTYPE)
+        self.ctr = np.zeros(4, np.uint32)
+        self.alive = np.zeros(self.n, np.uint32)
+        self.writes = np.zeros(2 * self.max_writes, np.uint32)
+
+    def frame_begin(self) -> None:
+        load().o_frame_begin(self.ctr.ctypes.data)
+
+    def spawn(self, src: np.ndarray) -> None:
+        src = np.ascontiguousarray(src, PARTICLE_DTYPE)
+        load().o_spawn(self.parts.ctypes.data, self.n, self.budget, self.ctr.ctypes.data, src.ctypes.data,
+                       src.shape[0])
+
+    def step(self, world: "World", dt: float, gravity: float, vs: float) -> None:
+        load().o_particle_step(C.byref(world.w), self.parts.ctypes.data, self.n, float(np.float32(dt)),
+                               float(np.float32(gravity)), float(np.float32(vs)), self.max_writes,
+                               self.ctr.ctypes.data, self.writes.ctypes.data, self.alive.ctypes.data)
+
+    def alive_list(self) -> np.ndarray:
+        return self.alive[:int(self.ctr[P_ALIVE])].copy()
+
+    def write_list(self, base: int = 0) -> np.ndarray:
+        n = min(int(self.ctr[P_WRITES]), self.max_writes)
+        return self.writes[2 * base:2 * n].reshape(-1, 2).copy()
 
 
 def island_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, cap: int = 1 << 20):

@@ -13,6 +13,7 @@
  *   allocBricks, carve, carveEffects    src/render/shaders/sim.wgsl:459-579
  *   pane flood (floodStep)              src/render/shaders/sim.wgsl:599-618
  *   maint compaction                    src/render/shaders/sim.wgsl:1219-1274
+ *   spawn / frameBegin / particleSim    src/render/shaders/sim.wgsl:166-170, 363-369, 1088-1173
  * and a textbook breadth-first 6-connected labelling for the islands.
  *
  * Choices the WGSL leaves to the implementation, fixed here (the HIP library
@@ -876,4 +877,145 @@ int o_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const flo
   free(q);
   *n_labels = nl;
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Debris particles: spawn() sim.wgsl:166-170, frameBegin :363-369 (its
+ * particle words), particleSim :1088-1173.  The reference appends to the
+ * alive and write lists with atomics, so their order is whatever the GPU
+ * schedule gives; this restatement is the serial schedule in particle-index
+ * order, which the HIP library reproduces exactly: list entries in index
+ * order and, when write requests exceed max_writes, the lowest-indexed
+ * requests win. */
+
+#define P_SETTLED 0x10000u
+#define P_DEPOSIT 0x20000u
+#define P_DUST 0x40000u
+
+void o_frame_begin(uint32_t* ctr) {
+  ctr[O_P_SPAWNED] = 0u;
+  ctr[O_P_WRITES] = 0u;
+  ctr[O_P_ALIVE] = 0u;
+}
+
+void o_spawn(o_particle* parts, uint32_t max_particles, uint32_t budget, uint32_t* ctr, const o_particle* src,
+             uint32_t n) {
+  for (uint32_t j = 0; j < n; j++) {
+    uint32_t s = ctr[O_P_SPAWNED]++;
+    if (s >= budget) continue;
+    uint32_t i = (ctr[O_P_PCURSOR]++) & (max_particles - 1u);
+    parts[i] = src[j];
+  }
+}
+
+static inline int solid_at(const o_world* w, const int32_t* p) { return readVoxel(w, p) != 0u; }
+
+/* pushAlive (sim.wgsl:1082-1085); entries past the list's capacity are dropped */
+#define PUSH_ALIVE(i)                                        \
+  do {                                                       \
+    uint32_t k_ = ctr[O_P_ALIVE]++;                          \
+    if (k_ < max_particles) alive[k_] = (i);                 \
+  } while (0)
+
+void o_particle_step(const o_world* w, o_particle* parts, uint32_t max_particles, float dt, float gravity, float vs,
+                     uint32_t max_writes, uint32_t* ctr, uint32_t* write_list, uint32_t* alive) {
+  for (uint32_t i = 0; i < max_particles; i++) {
+    o_particle p = parts[i];
+    if (p.life <= 0.0f) continue;
+    p.life -= dt;
+    if (p.life <= 0.0f) {
+      parts[i].life = 0.0f;
+      continue;
+    }
+    if ((p.value & P_SETTLED) != 0u) {
+      parts[i].life = p.life;
+      PUSH_ALIVE(i);
+      continue;
+    }
+    int dust = (p.value & P_DUST) != 0u;
+    float g = dust ? gravity * 0.15f : gravity;
+    p.vel[1] -= g * dt;
+    float drag = 1.0f - (dust ? 2.5f : 0.2f) * dt;
+    for (int a = 0; a < 3; a++) p.vel[a] *= drag;
+    int32_t cell[3];
+    for (int a = 0; a < 3; a++) cell[a] = f2i(floorf(p.pos[a]));
+    if (solid_at(w, cell)) {
+      p.pos[1] += 1.0f;
+      parts[i] = p;
+      PUSH_ALIVE(i);
+      continue;
+    }
+    float mv[3] = {p.vel[0] * dt, p.vel[1] * dt, p.vel[2] * dt};
+    float dist = sqrtf(dot3(mv, mv));
+    if (dist > 1e-6f) {
+      float dir[3], inv[3], tMax[3], tDelta[3];
+      int32_t stp[3];
+      for (int a = 0; a < 3; a++) {
+        float d = mv[a] / dist;
+        dir[a] = fabsf(d) < 1e-6f ? 1e-6f : d;
+      }
+      for (int a = 0; a < 3; a++) {
+        inv[a] = 1.0f / dir[a];
+        stp[a] = f2i(fsign(dir[a]));
+        tMax[a] = ((float)cell[a] + fstep(0.0f, dir[a]) - p.pos[a]) * inv[a];
+        tDelta[a] = fabsf(inv[a]);
+      }
+      float t = 0.0f;
+      int hit_axis = -1;
+      for (int k = 0; k < 48; k++) {
+        int axis;
+        if (tMax[0] < tMax[1] && tMax[0] < tMax[2]) axis = 0;
+        else if (tMax[1] < tMax[2]) axis = 1;
+        else axis = 2;
+        float tn = tMax[axis];
+        if (tn > dist) break;
+        int32_t next[3] = {cell[0], cell[1], cell[2]};
+        next[axis] += stp[axis];
+        if (solid_at(w, next)) {
+          t = tn;
+          hit_axis = axis;
+          break;
+        }
+        cell[0] = next[0]; cell[1] = next[1]; cell[2] = next[2];
+        tMax[axis] += tDelta[axis];
+      }
+      if (hit_axis < 0) {
+        for (int a = 0; a < 3; a++) p.pos[a] += mv[a];
+      } else {
+        float s = fmaxf(t - 1e-3f, 0.0f);
+        for (int a = 0; a < 3; a++) p.pos[a] += dir[a] * s;
+        float vin = p.vel[hit_axis];
+        p.vel[hit_axis] = -vin * 0.25f;
+        float fr = hit_axis == 1 ? 0.55f : 0.75f;
+        for (int a = 0; a < 3; a++)
+          if (a != hit_axis) p.vel[a] *= fr;
+      }
+    }
+    /* rest check: slow and supported from below */
+    int32_t c[3];
+    for (int a = 0; a < 3; a++) c[a] = f2i(floorf(p.pos[a]));
+    int32_t below[3] = {c[0], c[1] - 1, c[2]};
+    if (sqrtf(dot3(p.vel, p.vel)) < 0.6f / vs && solid_at(w, below)) {
+      for (int a = 0; a < 3; a++) {
+        p.pos[a] = (float)c[a] + 0.5f;
+        p.vel[a] = 0.0f;
+      }
+      p.value |= P_SETTLED;
+      int wrote = 0;
+      if ((p.value & P_DEPOSIT) != 0u && c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < w->dx && c[1] < w->dy &&
+          c[2] < w->dz) {
+        uint32_t k = ctr[O_P_WRITES]++;
+        if (k < max_writes) {
+          write_list[2 * k] = key_of(w, c);
+          write_list[2 * k + 1] = (p.value & 0xffffu) | 0x80u;
+          wrote = 1;
+        }
+      }
+      if (wrote) p.life = fminf(p.life, 0.5f);
+      else p.life = fminf(p.life, dust ? 0.8f : 4.0f);
+    }
+    if (p.pos[1] < -200.0f) p.life = 0.0f;
+    parts[i] = p;
+    if (p.life > 0.0f) PUSH_ALIVE(i);
+  }
 }

@@ -76,6 +76,32 @@ float o_rnd(uint32_t seed);
 /* removes() of sim.wgsl:229-239 for one voxel (impact = 16 floats). */
 int o_removes(const float* impact16, const float* mats, int32_t x, int32_t y, int32_t z, uint32_t v);
 
+/* Debris particle (shared.wgsl:33-38), 32 bytes. */
+typedef struct o_particle {
+  float pos[3];
+  uint32_t value;
+  float vel[3];
+  float life;
+} o_particle;
+
+/* Particle counters: C_SPAWNED, C_PCURSOR, C_WRITES (sim.wgsl:167-168,187)
+ * and the alive count args[A_DRAWP + 1] (:1083). */
+#define O_P_SPAWNED 0
+#define O_P_PCURSOR 1
+#define O_P_WRITES 2
+#define O_P_ALIVE 3
+
+/* frameBegin (sim.wgsl:363-369): zeroes C_SPAWNED, C_WRITES and the alive count. */
+void o_frame_begin(uint32_t* ctr);
+/* spawn() (sim.wgsl:166-170) for src[0..n) in order. */
+void o_spawn(o_particle* parts, uint32_t max_particles, uint32_t budget, uint32_t* ctr, const o_particle* src,
+             uint32_t n);
+/* particleSim (sim.wgsl:1088-1173) over every slot, serially in index order:
+ * alive[] gets the alive slots, write_list (key, value) pairs the deposit
+ * writes that fit max_writes. */
+void o_particle_step(const o_world* w, o_particle* parts, uint32_t max_particles, float dt, float gravity, float vs,
+                     uint32_t max_writes, uint32_t* ctr, uint32_t* write_list, uint32_t* alive);
+
 #ifdef __cplusplus
 }
 #endif

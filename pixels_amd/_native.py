@@ -23,7 +23,8 @@ PX_ARGS_WORDS = 44
 PX_CAMERA_FLOATS = 64
 
 RAY_PRIMARY, RAY_SHADOW, RAY_SHOOT, RAY_SEGMENT, RAY_MIXED = 0, 1, 2, 3, 4
-STAGE_EDIT = 1
+STAGE_EDIT, STAGE_PARTICLES, STAGE_FRAME_BEGIN = 1, 2, 4
+P_SETTLED, P_DEPOSIT, P_DUST = 0x10000, 0x20000, 0x40000
 FACE_ZERO, FACE_NOT_CAST, FACE_MISS = 6, 0xFE, 0xFF
 
 # CTRL_NAMES order, src/gpu/layout.ts:48-58
@@ -42,6 +43,8 @@ CTRL = {n: i for i, n in enumerate(CTRL_NAMES)}
 RAY_DTYPE = np.dtype([("ro", "<f4", 3), ("rd", "<f4", 3), ("max_t", "<f4"), ("flags", "<u4")])
 HIT_DTYPE = np.dtype([("key", "<u4"), ("t", "<f4"), ("steps", "<u4"), ("face", "u1"), ("mat", "u1"),
                       ("value_hi", "<u2")])
+# 32-byte px_particle record
+PARTICLE_DTYPE = np.dtype([("pos", "<f4", 3), ("value", "<u4"), ("vel", "<f4", 3), ("life", "<f4")])
 
 
 class PxLimits(C.Structure):
@@ -118,10 +121,21 @@ SIGNATURES = {
     "px_mlp_grad": (C.c_int, [_P, _P, _P, _U32, _FP, _FP]),
     "px_mlp_train_step": (C.c_int, [_P, _P, _P, _U32, C.c_float, _FP]),
     "px_bin_rays": (C.c_int, [_P, _P, _U32, _U32, _U32, C.c_float, C.c_float, _P]),
+    "px_particles_init": (C.c_int, [_P, _U32]),
+    "px_frame_begin": (C.c_int, [_P]),
+    "px_spawn": (C.c_int, [_P, _P, _U32]),
+    "px_spawn_device": (C.c_int, [_P, _P, _U32]),
+    "px_particle_step": (C.c_int, [_P]),
+    "px_read_particles": (C.c_int, [_P, _U32, _U32, _P]),
+    "px_read_alive": (C.c_int, [_P, _U32P, _U32, _U32P]),
+    "px_read_writes": (C.c_int, [_P, _U32P, _U32, _U32P]),
+    "px_particle_buffers": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
-PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_SLOTS = 0, 1, 2, 3, 4, 5, 8
+PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_PARTICLES = 0, 1, 2, 3, 4, 5, 6
+PROF_PARTICLE_SIM = 7
+PROF_SLOTS = 8
 
 _lib = None
 

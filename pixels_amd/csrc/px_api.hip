@@ -92,6 +92,7 @@ static void release(px_ctx* c) {
   for (void* p : bufs)
     if (p) hipFree(p);
   px_edit_free(c);
+  px_particles_free(c);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -518,9 +519,18 @@ int px_island_pass(px_ctx* c) {
 
 int px_sim_step(px_ctx* c, uint32_t stages_mask) {
   if (!c) return PX_ERR_ARG;
-  if (stages_mask & ~PX_STAGE_EDIT) return px_fail(c, PX_ERR_ARG, "px_sim_step", "unknown stage bits");
+  const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN;
+  if (stages_mask & ~known) return px_fail(c, PX_ERR_ARG, "px_sim_step", "unknown stage bits");
+  if ((stages_mask & PX_STAGE_PARTICLES) && !c->d_particles)
+    return px_fail(c, PX_ERR_STATE, "px_sim_step", "particle stage without px_particles_init");
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  if ((stages_mask & PX_STAGE_EDIT) && c->n_impacts) return px_launch_island_pass(c);
+  if (stages_mask & PX_STAGE_FRAME_BEGIN) {
+    if (int rc = px_launch_frame_begin(c)) return rc;
+  }
+  if ((stages_mask & PX_STAGE_EDIT) && c->n_impacts) {
+    if (int rc = px_launch_island_pass(c)) return rc;
+  }
+  if (stages_mask & PX_STAGE_PARTICLES) return px_launch_particle_step(c);
   return PX_OK;
 }
 

@@ -104,6 +104,15 @@ struct px_ctx {
   uint32_t* d_out_labels = nullptr;  // (key, count) pairs
   uint32_t label_cap = 0;
   size_t vox_total = 0;
+  // debris particles
+  uint32_t max_particles = 0;
+  float4* d_particles = nullptr;    // ring: 2 float4 per particle (pos, value) (vel, life)
+  uint32_t* d_alive = nullptr;      // alive list (slot indices)
+  uint32_t* d_write_list = nullptr; // (key, value) pairs, lim.max_writes of them
+  uint8_t* d_pstatus = nullptr;     // per slot: bit 0 alive, bit 1 write request
+  uint32_t* d_pblock = nullptr;     // per block: alive / write counts, then offsets
+  float4* d_spawn_stage = nullptr;  // host spawn records staged on the device
+  uint32_t spawn_cap = 0;
   // device-side checkpoint (px_snapshot / px_restore)
   uint32_t* b_grid = nullptr;
   uint32_t* b_pool = nullptr;
@@ -154,6 +163,10 @@ int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);
 int px_launch_island_pass(px_ctx* c);
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+// debris particles (px_particles.hip)
+void px_particles_free(px_ctx* c);
+int px_launch_frame_begin(px_ctx* c);
+int px_launch_particle_step(px_ctx* c);
 
 // ---------------------------------------------------------------------------
 // device helpers (shared.wgsl:44-81, render_common.wgsl:25-31)

@@ -288,4 +288,61 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
   return o;
 }
 
+// readVoxel (sim.wgsl:74-81): decoded voxel at p, 0 outside the world.
+PX_DEV uint32_t read_voxel(const PxWorldView& w, int32_t x, int32_t y, int32_t z) {
+  if (x < 0 || y < 0 || z < 0 || x >= w.dims[0] || y >= w.dims[1] || z >= w.dims[2]) return 0u;
+  uint32_t g = w.grid[(uint32_t)(x >> 3) + (uint32_t)w.bd[0] * ((uint32_t)(y >> 3) + (uint32_t)w.bd[1] * (uint32_t)(z >> 3))];
+  if (g == 0u) return 0u;
+  return px_brick_voxel(w.pool, g, x & 7, y & 7, z & 7);
+}
+
+// The debris DDA of particleSim (sim.wgsl:1110-1145) after its inside test:
+// from `cell` (the start cell, advanced in place) along the displacement mv
+// of length dist > 1e-6, at most 48 cells, stopping at the first solid cell
+// entered with t <= dist.  Returns the hit axis (-1 = none); t, the solid
+// voxel's value and the loop turns begun go to the out parameters, and dir
+// gets the direction the loop used (mv / dist with components under 1e-6 in
+// magnitude replaced by 1e-6).
+PX_DEV int segment_walk(const PxWorldView& w, int32_t cell[3], const float pos[3], const float mv[3], float dist,
+                        float dir[3], float* t_hit, uint32_t* v_hit, uint32_t* steps_out) {
+  float inv[3], tMax[3], tDelta[3];
+  int32_t stp[3];
+  for (int a = 0; a < 3; a++) {
+    float d = mv[a] / dist;
+    dir[a] = fabsf(d) < 1e-6f ? 1e-6f : d;
+  }
+  for (int a = 0; a < 3; a++) {
+    inv[a] = 1.0f / dir[a];
+    stp[a] = dir[a] > 0.0f ? 1 : (dir[a] < 0.0f ? -1 : 0);
+    tMax[a] = ((float)cell[a] + (dir[a] >= 0.0f ? 1.0f : 0.0f) - pos[a]) * inv[a];
+    tDelta[a] = fabsf(inv[a]);
+  }
+  uint32_t steps = 0;
+  for (int k = 0; k < 48; k++) {
+    steps++;
+    int axis;
+    if (tMax[0] < tMax[1] && tMax[0] < tMax[2]) axis = 0;
+    else if (tMax[1] < tMax[2]) axis = 1;
+    else axis = 2;
+    float tn = axis == 0 ? tMax[0] : (axis == 1 ? tMax[1] : tMax[2]);
+    if (tn > dist) break;
+    int32_t nx = cell[0] + (axis == 0 ? stp[0] : 0);
+    int32_t ny = cell[1] + (axis == 1 ? stp[1] : 0);
+    int32_t nz = cell[2] + (axis == 2 ? stp[2] : 0);
+    uint32_t v = read_voxel(w, nx, ny, nz);
+    cell[0] = nx; cell[1] = ny; cell[2] = nz;
+    if (v != 0u) {
+      *t_hit = tn;
+      *v_hit = v;
+      *steps_out = steps;
+      return axis;
+    }
+    if (axis == 0) tMax[0] += tDelta[0];
+    else if (axis == 1) tMax[1] += tDelta[1];
+    else tMax[2] += tDelta[2];
+  }
+  *steps_out = steps;
+  return -1;
+}
+
 }  // namespace px

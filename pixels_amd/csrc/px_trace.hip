@@ -31,13 +31,7 @@ using px::MarchOut;
 using px::MarchTables;
 using px::key_of;
 using px::step_face;
-
-PX_DEV uint32_t read_voxel(const PxWorldView& w, int32_t x, int32_t y, int32_t z) {
-  if (x < 0 || y < 0 || z < 0 || x >= w.dims[0] || y >= w.dims[1] || z >= w.dims[2]) return 0u;
-  uint32_t g = w.grid[(uint32_t)(x >> 3) + (uint32_t)w.bd[0] * ((uint32_t)(y >> 3) + (uint32_t)w.bd[1] * (uint32_t)(z >> 3))];
-  if (g == 0u) return 0u;
-  return px_brick_voxel(w.pool, g, x & 7, y & 7, z & 7);
-}
+using px::read_voxel;
 
 // traceSolid() of sim.wgsl:296-357.  steps = brick + voxel iterations, t = the
 // voxel-level t of the hit.
@@ -132,7 +126,9 @@ PX_DEV MarchOut trace_solid(const PxWorldView& w, const float ro[3], const float
   return o;
 }
 
-// Segment DDA of particleSim, sim.wgsl:1110-1145.
+// Segment DDA of particleSim, sim.wgsl:1110-1145.  A start inside a solid
+// reports face 6; a hit reports the solid cell entered, t of the crossing and
+// the face of the step.
 PX_DEV MarchOut segment(const PxWorldView& w, const float pos[3], const float mv[3]) {
   MarchOut o;
   o.key = PX_NONE;
@@ -151,45 +147,17 @@ PX_DEV MarchOut segment(const PxWorldView& w, const float pos[3], const float mv
   }
   float dist = sqrtf(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
   if (!(dist > 1e-6f)) return o;
-  float dir[3], inv[3], tMax[3], tDelta[3];
-  int32_t stp[3];
-  for (int a = 0; a < 3; a++) {
-    float d = mv[a] / dist;
-    dir[a] = fabsf(d) < 1e-6f ? 1e-6f : d;
-  }
-  for (int a = 0; a < 3; a++) {
-    inv[a] = 1.0f / dir[a];
-    stp[a] = dir[a] > 0.0f ? 1 : (dir[a] < 0.0f ? -1 : 0);
-    tMax[a] = ((float)cell[a] + (dir[a] >= 0.0f ? 1.0f : 0.0f) - pos[a]) * inv[a];
-    tDelta[a] = fabsf(inv[a]);
-  }
-  uint32_t steps = 0;
-  for (int k = 0; k < 48; k++) {
-    steps++;
-    int axis;
-    if (tMax[0] < tMax[1] && tMax[0] < tMax[2]) axis = 0;
-    else if (tMax[1] < tMax[2]) axis = 1;
-    else axis = 2;
-    float tn = axis == 0 ? tMax[0] : (axis == 1 ? tMax[1] : tMax[2]);
-    if (tn > dist) break;
-    int32_t nx = cell[0] + (axis == 0 ? stp[0] : 0);
-    int32_t ny = cell[1] + (axis == 1 ? stp[1] : 0);
-    int32_t nz = cell[2] + (axis == 2 ? stp[2] : 0);
-    uint32_t v = read_voxel(w, nx, ny, nz);
-    if (v != 0u) {
-      o.key = key_of(w, nx, ny, nz);
-      o.t = px_canon0(tn);
-      o.face = step_face(axis, axis == 0 ? stp[0] : (axis == 1 ? stp[1] : stp[2]));
-      o.value = v;
-      o.steps = steps;
-      return o;
-    }
-    cell[0] = nx; cell[1] = ny; cell[2] = nz;
-    if (axis == 0) tMax[0] += tDelta[0];
-    else if (axis == 1) tMax[1] += tDelta[1];
-    else tMax[2] += tDelta[2];
-  }
+  float dir[3], tn = 0.0f;
+  uint32_t v = 0u, steps = 0u;
+  const int axis = px::segment_walk(w, cell, pos, mv, dist, dir, &tn, &v, &steps);
   o.steps = steps;
+  if (axis >= 0) {
+    o.key = key_of(w, cell[0], cell[1], cell[2]);
+    o.t = px_canon0(tn);
+    const float da = axis == 0 ? dir[0] : (axis == 1 ? dir[1] : dir[2]);
+    o.face = step_face(axis, da > 0.0f ? 1 : -1);
+    o.value = v;
+  }
   return o;
 }
 

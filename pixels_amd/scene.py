@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from ._native import RAY_DTYPE
+from ._native import P_DEPOSIT, P_DUST, PARTICLE_DTYPE, RAY_DTYPE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpxscene.so")
@@ -137,3 +137,29 @@ class Scene:
         _load().pxs_frame_inputs(self._h, seed, width, height, voxel_size, n_lights, light_radius_vox,
                                  u.ctypes.data, lights.ctypes.data)
         return u, lights
+
+
+def debris(dims, n: int, seed: int = 1, speed: float = 60.0, life=(2.0, 8.0), deposit: float = 0.6,
+           dust: float = 0.2, margin: int = 0) -> np.ndarray:
+    """Synthetic debris records (px_particle) of the kind carveEffects spawns
+    (sim.wgsl:495-509): voxel-centre positions uniform in the world box grown
+    by `margin` voxels on x and z, velocities of magnitude up to `speed`
+    voxels/s with an upward bias, a material id U{1..16} with a random shade
+    byte, DEPOSIT / DUST flags with the given probabilities, and life in
+    seconds uniform in `life`."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros(n, PARTICLE_DTYPE)
+    lo = np.array([-margin, 0, -margin])
+    hi = np.array([dims[0] + margin, dims[1], dims[2] + margin])
+    cell = rng.integers(lo, hi, size=(n, 3))
+    out["pos"] = (cell + 0.5).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d[:, 1] = np.abs(d[:, 1]) * 0.7 + 0.3
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out["vel"] = (d * rng.uniform(0.0, speed, (n, 1))).astype(np.float32)
+    mat = rng.integers(1, 17, n).astype(np.uint32)
+    shade = rng.integers(0, 256, n).astype(np.uint32)
+    flags = np.where(rng.random(n) < deposit, P_DEPOSIT, 0) | np.where(rng.random(n) < dust, P_DUST, 0)
+    out["value"] = mat | (shade << 8) | flags.astype(np.uint32)
+    out["life"] = rng.uniform(life[0], life[1], n).astype(np.float32)
+    return out

@@ -281,3 +281,49 @@ class Simulation:
         self.gw.check(self.lib.px_island_labels(self.gw.ctx, N.ptr(keys), N.ptr(cnt), cap, C.byref(n)))
         m = min(cap, int(n.value))
         return keys[:m].copy(), cnt[:m].copy(), int(n.value)
+
+    # -- debris particles (sim.wgsl:166-170, 363-369, 1082-1173) ----------------
+    def particles_init(self, max_particles: int = 1 << 18) -> None:
+        self.gw.check(self.lib.px_particles_init(self.gw.ctx, max_particles))
+        self.max_particles = int(max_particles) or (1 << 18)
+
+    def frame_begin(self) -> None:
+        self.gw.check(self.lib.px_frame_begin(self.gw.ctx))
+
+    def spawn(self, records: np.ndarray) -> None:
+        rec = np.ascontiguousarray(records, N.PARTICLE_DTYPE)
+        self.gw.check(self.lib.px_spawn(self.gw.ctx, rec.ctypes.data, rec.shape[0]))
+
+    def spawn_device(self, records_t, n: int) -> None:
+        self.gw.after_torch()
+        self.gw.check(self.lib.px_spawn_device(self.gw.ctx, C.c_void_p(_devptr(records_t)), n))
+
+    def particle_step(self) -> None:
+        """particleSim with dt, gravity and voxel size from the uniforms."""
+        self.gw.check(self.lib.px_particle_step(self.gw.ctx))
+
+    def read_particles(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.max_particles - first if count is None else count
+        out = np.zeros(count, N.PARTICLE_DTYPE)
+        self.gw.check(self.lib.px_read_particles(self.gw.ctx, first, count, out.ctypes.data))
+        return out
+
+    def read_alive(self) -> np.ndarray:
+        n = C.c_uint32(0)
+        self.gw.check(self.lib.px_read_alive(self.gw.ctx, None, 0, C.byref(n)))
+        out = np.zeros(max(1, int(n.value)), np.uint32)
+        self.gw.check(self.lib.px_read_alive(self.gw.ctx, N.ptr(out), out.size, C.byref(n)))
+        return out[:min(int(n.value), self.max_particles)].copy()
+
+    def read_writes(self) -> np.ndarray:
+        """(key, value) pairs of the frame's deposit writes, shape (n, 2)."""
+        n = C.c_uint32(0)
+        self.gw.check(self.lib.px_read_writes(self.gw.ctx, None, 0, C.byref(n)))
+        out = np.zeros(2 * max(1, int(n.value)), np.uint32)
+        self.gw.check(self.lib.px_read_writes(self.gw.ctx, N.ptr(out), int(n.value), C.byref(n)))
+        return out[:2 * int(n.value)].reshape(-1, 2).copy()
+
+    def particle_buffers(self):
+        p, a, w = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self.gw.check(self.lib.px_particle_buffers(self.gw.ctx, C.byref(p), C.byref(a), C.byref(w)))
+        return int(p.value or 0), int(a.value or 0), int(w.value or 0)
