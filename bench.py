@@ -30,13 +30,14 @@ COPY_BYTES = 256 << 20
 
 
 TRAFFIC_FILE = "profiles/hbm_traffic_frame_1920x1080.md"
+DEBRIS_TRAFFIC_FILE = "profiles/debris_step.md"
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch and L2 hit % of `kernel` from the committed
+def profiled_traffic(kernel, source=TRAFFIC_FILE):
+    """HBM bytes per launch and L2 hit % of `kernel` from a committed
     rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by a 256 MiB copy);
     (None, None) if the summary has no line for it."""
-    path = os.path.join(ROOT, TRAFFIC_FILE)
+    path = os.path.join(ROOT, source)
     traffic = hit = None
     if os.path.exists(path):
         for line in open(path):
@@ -264,7 +265,7 @@ def island_leg(args):
 DEBRIS_RING = 1 << 18   # MAX_PARTICLES default, config.ts:38
 DEBRIS_BIG = 1 << 21    # its maximum
 # algorithmic bytes per live slot of k_particle_sim: the 32-byte record read
-# and written back, the status byte; per block of 256 slots 8 bytes of counts
+# and written back, the status byte; per tile of 1024 slots 8 bytes of counts
 DEBRIS_SLOT_BYTES = 65
 
 
@@ -341,12 +342,14 @@ def debris_leg(args, scene, gw, u, prim_rays, prim_hits):
         alive = len(sim.read_alive())
         ms = prof[N.PROF_PARTICLES][0] / prof[N.PROF_PARTICLES][1]
         sim_ms = prof[N.PROF_PARTICLE_SIM][0] / prof[N.PROF_PARTICLE_SIM][1]
-        algo = DEBRIS_SLOT_BYTES * ring + 8 * (ring // 256)
+        algo = DEBRIS_SLOT_BYTES * ring + 8 * (ring // 1024)
+        traffic, _ = profiled_traffic("k_particle_sim@%d" % ring, DEBRIS_TRAFFIC_FILE)
         r = {"ring": ring, "steps": steps, "ms_per_step": round(ms, 5), "sim_kernel_ms": round(sim_ms, 5),
              "particles_per_s": round(ring / (ms * 1e-3), 1), "alive_after": alive,
              "roofline": {"bound": "hbm", "kernel": "k_particle_sim", "algorithmic_bytes": algo,
                           "achieved": round(algo / (sim_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(algo / (sim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+                          "frac": round(algo / (sim_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic,
+                          "traffic_source": DEBRIS_TRAFFIC_FILE if traffic is not None else None}}
         log("debris ring %d: %.4f ms/step (sim kernel %.4f ms), %.3g particles/s, %d alive after %d steps" %
             (ring, ms, sim_ms, r["particles_per_s"], alive, steps))
         if ring == DEBRIS_RING and not args.no_cpu:

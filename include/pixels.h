@@ -238,7 +238,8 @@ int px_island_pass(px_ctx* ctx);
 #define PX_STAGE_EDIT 1u
 /* PX_STAGE_FRAME_BEGIN = px_frame_begin, run first; PX_STAGE_PARTICLES =
  * px_particle_step, run after the edit stage as particleSim runs after the
- * edit passes (Simulation.ts:223-262). */
+ * edit passes; PX_STAGE_WRITES = px_apply_writes, run last as the write
+ * passes follow particleSim (Simulation.ts:223-268). */
 #define PX_STAGE_PARTICLES 2u
 #define PX_STAGE_FRAME_BEGIN 4u
 int px_sim_step(px_ctx* ctx, uint32_t stages_mask);
@@ -305,6 +306,24 @@ int px_read_writes(px_ctx* ctx, uint32_t* pairs, uint32_t cap, uint32_t* n);
  * and the write list, for callers that draw or consume them on the GPU. */
 int px_particle_buffers(px_ctx* ctx, void** particles, void** alive, void** writes);
 
+/* ---- voxel writes ------------------------------------------------------------
+ * writeArgs + writeRequest + allocBricks + writeApply (sim.wgsl:1178-1203,
+ * requestBrick :94-102, putVoxel :127-146, allocBricks :459-483), then maint
+ * (:1219-1274) of the cells touched: takes n = min(C_WRITES, max_writes)
+ * entries of the write list (C_WRITE_N = n, C_WRITES = 0), gives every cell
+ * that a write targets and that holds no brick a pool brick, and puts each
+ * write's value into the first empty voxel of p, p + y, p + 2y, stopping at
+ * a cell without a pool brick.  The reference serves requests and puts in
+ * atomic order; here they follow the list order: the first max_alloc
+ * distinct cells are served, and a later write to a taken voxel stacks above
+ * the earlier one.  Asynchronous. */
+#define PX_STAGE_WRITES 8u
+int px_apply_writes(px_ctx* ctx);
+/* Appends (key, value) pairs to the write list as pushWrite (sim.wgsl:186-192)
+ * does: C_WRITES grows by n, entries past max_writes are dropped.  For callers
+ * that deposit voxels from the host.  Blocking on the copy. */
+int px_push_writes(px_ctx* ctx, const uint32_t* pairs, uint32_t n);
+
 /* ---- checkpoint of the device world (no counterpart in the reference, which
  * rebuilds its world from the seed on every load and never saves GPU state).
  * px_snapshot copies grid, pool, super grid, ctrl words and free list to a
@@ -327,7 +346,8 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_MLP 5      /* ray-cost predictor: features, predict, train, bin */
 #define PX_PROF_PARTICLES 6 /* spawn + particle step, all kernels */
 #define PX_PROF_PARTICLE_SIM 7 /* the integration kernel of the step alone */
-#define PX_PROF_SLOTS 8
+#define PX_PROF_WRITES 8   /* voxel-write stage, all kernels */
+#define PX_PROF_SLOTS 16
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);
 

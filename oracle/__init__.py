@@ -58,6 +58,8 @@ def load():
     lib.o_rnd.argtypes = [C.c_uint32]
     lib.o_removes.restype = C.c_int
     lib.o_removes.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+    lib.o_apply_writes.restype = C.c_int
+    lib.o_apply_writes.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]
     lib.o_frame_begin.argtypes = [P]
     lib.o_spawn.argtypes = [P, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
     lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
@@ -169,6 +171,20 @@ def island_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, cap: int
         raise MemoryError("oracle island pass failed")
     m = min(cap, n.value)
     return vox, kind, keys[:m].copy(), cnt[:m].copy(), int(n.value), stats
+
+
+def apply_writes(dims, vox: np.ndarray, kind: np.ndarray, writes: np.ndarray, max_alloc: int = 65535,
+                 free_top: int = 1 << 30):
+    """Voxel writes + maint on copies; returns (vox_after, kind_after, stats)."""
+    vox = np.array(vox, np.uint16, copy=True, order="C")
+    kind = np.array(kind, np.uint8, copy=True, order="C")
+    w = np.ascontiguousarray(writes, np.uint32).reshape(-1, 2)
+    stats = np.zeros(4, np.uint64)
+    rc = load().o_apply_writes(dims[0], dims[1], dims[2], vox.ctypes.data, kind.ctypes.data, w.ctypes.data,
+                               w.shape[0], max_alloc, free_top, stats.ctypes.data)
+    if rc != 0:
+        raise MemoryError("oracle writes failed")
+    return vox, kind, stats
 
 
 def islands(dims, vox: np.ndarray, mats, cap: int = 1 << 20):

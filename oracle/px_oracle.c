@@ -14,6 +14,7 @@
  *   pane flood (floodStep)              src/render/shaders/sim.wgsl:599-618
  *   maint compaction                    src/render/shaders/sim.wgsl:1219-1274
  *   spawn / frameBegin / particleSim    src/render/shaders/sim.wgsl:166-170, 363-369, 1088-1173
+ *   voxel writes (request, apply)       src/render/shaders/sim.wgsl:94-102, 127-146, 1178-1203
  * and a textbook breadth-first 6-connected labelling for the islands.
  *
  * Choices the WGSL leaves to the implementation, fixed here (the HIP library
@@ -1018,4 +1019,105 @@ void o_particle_step(const o_world* w, o_particle* parts, uint32_t max_particles
     parts[i] = p;
     if (p.life > 0.0f) PUSH_ALIVE(i);
   }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Voxel writes: writeArgs / writeRequest / allocBricks / writeApply
+ * (sim.wgsl:1178-1203, requestBrick :94-102, putVoxel :127-146, allocBricks
+ * :459-483) and maint on the cells they touched (:1219-1274), on the dense
+ * world of o_island_pass.  Serial in list order: the reference's requests
+ * and puts race through atomics; the HIP library reproduces this order. */
+
+/* putVoxel: 0 = written, 1 = cell occupied, 2 = no pool brick there */
+static uint32_t put_voxel(ed_world* e, int32_t x, int32_t y, int32_t z, uint32_t v) {
+  if (!in_world(e, x, y, z)) return 2u;
+  size_t ci = cell_of(e, x, y, z);
+  if (e->kind[ci] == 0) return 2u;
+  if (e->kind[ci] == 1) return 1u;
+  size_t k = vkey(e, x, y, z);
+  if (e->vox[k] != 0u) return 1u;
+  e->vox[k] = (uint16_t)v;
+  e->dirty[ci] = 1;
+  return 0u;
+}
+
+static void maint_dirty(ed_world* e, size_t ncell) {
+  for (size_t ci = 0; ci < ncell; ci++) {
+    if (!e->dirty[ci]) continue;
+    e->dirty[ci] = 0;
+    if (e->kind[ci] != 2) continue;
+    int32_t ox = (int32_t)(ci % (size_t)e->bx) * 8;
+    int32_t oy = (int32_t)((ci / (size_t)e->bx) % (size_t)e->by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)e->bx * e->by)) * 8;
+    uint32_t first = e->vox[vkey(e, ox, oy, oz)];
+    int nz = 0, diff = 0;
+    for (uint32_t l = 0; l < 512u; l++) {
+      uint32_t v = e->vox[vkey(e, ox + (int32_t)(l & 7u), oy + (int32_t)((l >> 3) & 7u), oz + (int32_t)(l >> 6))];
+      if (v != 0u) nz = 1;
+      if (v != first) diff = 1;
+    }
+    if (!nz) e->kind[ci] = 0;
+    else if (!diff) e->kind[ci] = 1;
+  }
+}
+
+int o_apply_writes(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const uint32_t* writes,
+                   uint32_t n, uint32_t max_alloc, uint32_t free_top, uint64_t* stats) {
+  ed_world E;
+  E.dx = dx; E.dy = dy; E.dz = dz;
+  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
+  E.vox = vox;
+  E.kind = kind;
+  E.mats = NULL;
+  size_t ncell = (size_t)E.bx * E.by * E.bz;
+  E.dirty = (uint8_t*)calloc(ncell, 1);
+  uint8_t* req = (uint8_t*)calloc(ncell, 1);
+  if (!E.dirty || !req) { free(E.dirty); free(req); return -1; }
+  uint64_t written = 0, allocated = 0, dropped = 0, dirty = 0;
+  /* writeRequest: the first cell of each write, if it has no brick; the
+   * alloc list keeps the first max_alloc distinct cells in list order */
+  uint32_t nreq = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t key = writes[2 * i];
+    int32_t x = (int32_t)(key % (uint32_t)dx), y = (int32_t)((key / (uint32_t)dx) % (uint32_t)dy);
+    int32_t z = (int32_t)(key / ((uint32_t)dx * (uint32_t)dy));
+    size_t ci = cell_of(&E, x, y, z);
+    if (kind[ci] != 0 || req[ci]) continue;
+    req[ci] = 1;
+    if (nreq < max_alloc) {
+      /* allocBricks: request r gets a brick while free ones remain */
+      if (nreq < free_top) {
+        kind[ci] = 2;
+        E.dirty[ci] = 1;
+        allocated++;
+      } else {
+        dropped++;
+      }
+      nreq++;
+    } else {
+      dropped++;
+    }
+  }
+  /* writeApply: stack up to 2 voxels above a taken cell */
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t key = writes[2 * i], v = writes[2 * i + 1];
+    int32_t x = (int32_t)(key % (uint32_t)dx), y = (int32_t)((key / (uint32_t)dx) % (uint32_t)dy);
+    int32_t z = (int32_t)(key / ((uint32_t)dx * (uint32_t)dy));
+    for (int up = 0; up < 3; up++) {
+      uint32_t r = put_voxel(&E, x, y + up, z, v);
+      if (r == 0u) written++;
+      if (r != 1u) break;
+    }
+  }
+  for (size_t ci = 0; ci < ncell; ci++) dirty += E.dirty[ci];
+  maint_dirty(&E, ncell);
+  free(E.dirty);
+  free(req);
+  if (stats) {
+    stats[0] = written;
+    stats[1] = allocated;
+    stats[2] = dropped;
+    stats[3] = dirty;
+  }
+  return 0;
 }

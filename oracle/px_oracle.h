@@ -102,6 +102,17 @@ void o_spawn(o_particle* parts, uint32_t max_particles, uint32_t budget, uint32_
 void o_particle_step(const o_world* w, o_particle* parts, uint32_t max_particles, float dt, float gravity, float vs,
                      uint32_t max_writes, uint32_t* ctr, uint32_t* write_list, uint32_t* alive);
 
+/* Voxel writes (writeArgs / writeRequest / allocBricks / writeApply,
+ * sim.wgsl:1178-1203, 459-483) and maint of the touched cells, on the dense
+ * world of o_island_pass: writes = (key, value) pairs in list order.  Cells
+ * of a write's first target without a brick get one (the first max_alloc
+ * distinct cells, at most free_top of them); each write then puts its value
+ * into the first empty voxel of p, p+y, p+2y, stopping at a cell without a
+ * pool brick.  stats: [0] voxels written, [1] cells allocated, [2] requests
+ * dropped, [3] cells dirtied. */
+int o_apply_writes(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const uint32_t* writes,
+                   uint32_t n, uint32_t max_alloc, uint32_t free_top, uint64_t* stats);
+
 #ifdef __cplusplus
 }
 #endif

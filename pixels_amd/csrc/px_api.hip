@@ -93,6 +93,7 @@ static void release(px_ctx* c) {
     if (p) hipFree(p);
   px_edit_free(c);
   px_particles_free(c);
+  px_writes_free(c);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -519,7 +520,7 @@ int px_island_pass(px_ctx* c) {
 
 int px_sim_step(px_ctx* c, uint32_t stages_mask) {
   if (!c) return PX_ERR_ARG;
-  const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN;
+  const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN | PX_STAGE_WRITES;
   if (stages_mask & ~known) return px_fail(c, PX_ERR_ARG, "px_sim_step", "unknown stage bits");
   if ((stages_mask & PX_STAGE_PARTICLES) && !c->d_particles)
     return px_fail(c, PX_ERR_STATE, "px_sim_step", "particle stage without px_particles_init");
@@ -530,8 +531,17 @@ int px_sim_step(px_ctx* c, uint32_t stages_mask) {
   if ((stages_mask & PX_STAGE_EDIT) && c->n_impacts) {
     if (int rc = px_launch_island_pass(c)) return rc;
   }
-  if (stages_mask & PX_STAGE_PARTICLES) return px_launch_particle_step(c);
+  if (stages_mask & PX_STAGE_PARTICLES) {
+    if (int rc = px_launch_particle_step(c)) return rc;
+  }
+  if (stages_mask & PX_STAGE_WRITES) return px_launch_writes(c);
   return PX_OK;
+}
+
+int px_apply_writes(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_writes(c);
 }
 
 int px_run_to_quiescence(px_ctx* c) {

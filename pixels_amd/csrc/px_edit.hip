@@ -53,7 +53,9 @@ constexpr uint32_t kCountWords = 4;     // per-block list lengths
 // edit-pass counters (ec[])
 enum : uint32_t {
   EC_JOBCELLS = 0, EC_ALLOCS, EC_UNUSED2, EC_UNUSED3, EC_LABELS, EC_UNUSED5, EC_JOBS, EC_CHUNKS,
-  EC_UNUSED8, EC_OVERFLOW, EC_WORDS = 16
+  EC_UNUSED8, EC_OVERFLOW, EC_WORDS = 16,
+  // voxel-write stage (after the island pass's words)
+  EC_W_N = 16, EC_W_DIRTY, EC_W_WRITTEN, EC_W_DROP
 };
 // per-block partials (part[b * kPartWords + k])
 enum : uint32_t { PT_CARVE_D = 0, PT_CARVE_B, PT_FLOOD_D, PT_FLOOD_B, PT_DIRTY, PT_FREED };
@@ -89,6 +91,13 @@ struct EditView {
   uint32_t* ec;
   const float* impacts;
   uint32_t n_impacts;
+  // voxel writes
+  uint32_t* wlist;
+  uint32_t max_writes, max_alloc;
+  uint32_t* cfirst;
+  uint64_t* wkeys;
+  const uint64_t* wsorted;
+  uint32_t* wdirty;
 };
 
 // Union-find over voxel keys.  par[x] <= x always (links go to smaller keys),
@@ -921,6 +930,180 @@ __global__ __launch_bounds__(1024) void k_edit_finish(EditView e, uint32_t carve
   e.ctrl[PX_S_SEARCHES] += 1u;
 }
 
+// ---------------------------------------------------------------------------
+// Voxel writes: writeArgs / writeRequest / allocBricks / writeApply
+// (sim.wgsl:1178-1203, 94-102, 127-146, 459-483) and maint of the cells they
+// touch.  The reference requests bricks and puts voxels through atomics in
+// schedule order; here the order is the write list's:
+//  * a cell without a brick is requested by its first write in list order
+//    (atomicMin of the write index per cell), and the requests are ranked in
+//    list order, so the first max_alloc distinct cells are the ones served;
+//  * writes only interact inside one world column (a put goes to p, p+y or
+//    p+2y), so the list is radix-sorted by (column, index) and one lane per
+//    column applies that column's writes in list order: the serial result,
+//    with every column in parallel.
+
+PX_DEV void key_pos(const EditView& e, uint32_t key, int32_t& x, int32_t& y, int32_t& z) {
+  x = (int32_t)(key % (uint32_t)e.dims[0]);
+  y = (int32_t)((key / (uint32_t)e.dims[0]) % (uint32_t)e.dims[1]);
+  z = (int32_t)(key / ((uint32_t)e.dims[0] * (uint32_t)e.dims[1]));
+}
+
+PX_DEV uint32_t cell_of_pos(const EditView& e, int32_t x, int32_t y, int32_t z) {
+  return (uint32_t)(x >> 3) + (uint32_t)e.bd[0] * ((uint32_t)(y >> 3) + (uint32_t)e.bd[1] * (uint32_t)(z >> 3));
+}
+
+__global__ void k_write_args(EditView e) {
+  if (threadIdx.x != 0) return;
+  const uint32_t n = min(e.ctrl[PX_C_WRITES], e.max_writes);
+  e.ctrl[PX_C_WRITE_N] = n;
+  e.ctrl[PX_C_WRITES] = 0u;
+  e.ec[EC_W_N] = n;
+  e.ec[EC_W_DIRTY] = 0u;
+  e.ec[EC_W_WRITTEN] = 0u;
+  e.ec[EC_W_DROP] = 0u;
+  e.ec[EC_ALLOCS] = 0u;
+}
+
+// First write per brick-less cell, and the sort keys (column << 32 | index;
+// all ones past the list).
+__global__ __launch_bounds__(256) void k_write_first(EditView e) {
+  const uint32_t n = e.ec[EC_W_N];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < e.max_writes; i += gridDim.x * blockDim.x) {
+    uint64_t k = ~0ull;
+    if (i < n) {
+      int32_t x, y, z;
+      key_pos(e, e.wlist[2u * i], x, y, z);
+      const uint32_t ci = cell_of_pos(e, x, y, z);
+      if (e.grid[ci] == 0u) atomicMin(&e.cfirst[ci], i);
+      const uint64_t col = (uint64_t)x + (uint64_t)e.dims[0] * (uint64_t)z;
+      k = (col << 32) | i;
+    }
+    e.wkeys[i] = k;
+  }
+}
+
+// writeRequest: the alloc list in list order (one workgroup, tiles of
+// blockDim.x writes, a block prefix sum per tile); resets cfirst.
+__global__ __launch_bounds__(1024) void k_write_request(EditView e) {
+  __shared__ uint32_t s_wave[16];
+  const uint32_t n = e.ec[EC_W_N];
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t base = 0u;
+  for (uint32_t t0 = 0; t0 < n; t0 += blockDim.x) {
+    const uint32_t i = t0 + threadIdx.x;
+    uint32_t ci = 0u;
+    bool req = false;
+    if (i < n) {
+      int32_t x, y, z;
+      key_pos(e, e.wlist[2u * i], x, y, z);
+      ci = cell_of_pos(e, x, y, z);
+      req = e.grid[ci] == 0u && e.cfirst[ci] == i;
+    }
+    const uint64_t m = __ballot(req);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0u, total = 0u;
+    for (uint32_t k = 0; k < nw; k++) {
+      const uint32_t c = s_wave[k];
+      before += k < wave ? c : 0u;
+      total += c;
+    }
+    if (req) {
+      const uint32_t r = base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (r < e.max_alloc && r < e.ncell) e.alloc_list[r] = ci;
+      e.cfirst[ci] = 0xFFFFFFFFu;
+    }
+    base += total;
+    __syncthreads();  // s_wave is reused by the next tile
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t served = min(min(base, e.max_alloc), e.ncell);
+    e.ec[EC_ALLOCS] = served;
+    e.ec[EC_W_DROP] = base - served;
+  }
+}
+
+// putVoxel (sim.wgsl:127-146): 0 = written, 1 = occupied, 2 = no pool brick.
+// The lane owns voxel p for this launch (one lane per column), so the empty
+// check and the OR of its half-word cannot race with another write to it; the
+// other half of the word may belong to the neighbouring column, hence the
+// atomic OR.
+PX_DEV uint32_t put_voxel(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t v) {
+  if (x < 0 || y < 0 || z < 0 || x >= e.dims[0] || y >= e.dims[1] || z >= e.dims[2]) return 2u;
+  const uint32_t ci = cell_of_pos(e, x, y, z);
+  const uint32_t g = e.grid[ci];
+  if (g == 0u) return 2u;
+  if ((g & PX_UNIFORM_BIT) != 0u) return 1u;
+  const uint32_t idx = (g - 1u) * 512u + (uint32_t)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6));
+  const uint32_t sh = (idx & 1u) * 16u;
+  const uint32_t old = __hip_atomic_load(e.pool + (idx >> 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (((old >> sh) & 0xffffu) != 0u) return 1u;
+  atomicOr(e.pool + (idx >> 1), (v & 0xffffu) << sh);
+  const uint32_t bit = 1u << (ci & 31u);
+  if ((atomicOr(&e.dirty_flags[ci >> 5], bit) & bit) == 0u) {
+    const uint32_t k = atomicAdd(&e.ec[EC_W_DIRTY], 1u);
+    if (k < e.max_writes) e.wdirty[k] = ci;
+  }
+  return 0u;
+}
+
+// writeApply: one lane per column segment of the sorted keys.
+__global__ __launch_bounds__(256) void k_write_apply(EditView e) {
+  uint32_t written = 0u;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < e.max_writes; t += gridDim.x * blockDim.x) {
+    const uint64_t k = e.wsorted[t];
+    if (k == ~0ull) continue;
+    const uint32_t col = (uint32_t)(k >> 32);
+    if (t > 0u && (uint32_t)(e.wsorted[t - 1u] >> 32) == col) continue;  // not the segment head
+    for (uint32_t j = t; j < e.max_writes; j++) {
+      const uint64_t kj = e.wsorted[j];
+      if (kj == ~0ull || (uint32_t)(kj >> 32) != col) break;
+      const uint32_t i = (uint32_t)kj;
+      int32_t x, y, z;
+      key_pos(e, e.wlist[2u * i], x, y, z);
+      const uint32_t v = e.wlist[2u * i + 1u];
+      // a taken cell stacks the voxel on top, up to two voxels higher
+      for (int up = 0; up < 3; up++) {
+        const uint32_t r = put_voxel(e, x, y + up, z, v);
+        if (r == 0u) written++;
+        if (r != 1u) break;
+      }
+    }
+  }
+  written = wave_sum(written);
+  if (__lane_id() == 0 && written) atomicAdd(&e.ec[EC_W_WRITTEN], written);
+}
+
+// maint of the cells the puts dirtied and of the cells alloc gave a brick
+__global__ __launch_bounds__(64) void k_write_maint(EditView e) {
+  const uint32_t nd = min(e.ec[EC_W_DIRTY], e.max_writes);
+  const uint32_t na = min(e.ec[EC_ALLOCS], e.ncell);
+  for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) maint_cell(e, e.wdirty[i]);
+  for (uint32_t i = blockIdx.x; i < na; i += gridDim.x) maint_cell(e, e.alloc_list[i]);
+}
+
+__global__ void k_write_finish(EditView e) {
+  if (threadIdx.x != 0) return;
+  e.ctrl[PX_C_DIRTY] += min(e.ec[EC_W_DIRTY], e.max_writes);
+  e.ctrl[PX_C_ALLOC_N] = e.ec[EC_ALLOCS];
+}
+
+// pushWrite (sim.wgsl:186-192) for n staged pairs: entry C_WRITES + j while
+// it is below max_writes; C_WRITES grows by n in k_push_commit.
+__global__ __launch_bounds__(256) void k_push_writes(EditView e, const uint32_t* __restrict__ pairs, uint32_t n) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t k0 = e.ctrl[PX_C_WRITES];
+  if (k0 >= e.max_writes || j >= e.max_writes - k0) return;
+  e.wlist[2u * (k0 + j)] = pairs[2u * j];
+  e.wlist[2u * (k0 + j) + 1u] = pairs[2u * j + 1u];
+}
+
+__global__ void k_push_commit(EditView e, uint32_t n) {
+  if (threadIdx.x == 0) e.ctrl[PX_C_WRITES] += n;
+}
+
 __global__ void k_zero64(uint64_t* p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
@@ -963,6 +1146,13 @@ EditView make_view(px_ctx* c) {
   e.ec = c->d_edit_ctr;
   e.impacts = c->d_impacts;
   e.n_impacts = c->n_impacts;
+  e.wlist = c->d_write_list;
+  e.max_writes = c->lim.max_writes;
+  e.max_alloc = c->lim.max_alloc;
+  e.cfirst = c->d_cfirst;
+  e.wkeys = c->d_wkeys;
+  e.wsorted = c->d_wsorted;
+  e.wdirty = c->d_wdirty;
   return e;
 }
 
@@ -1095,5 +1285,77 @@ int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_
     count[i] = (uint32_t)packed[i];
   }
   *n = m;
+  return PX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// voxel writes
+
+int px_writes_alloc(px_ctx* c) {
+  if (c->d_write_list) return PX_OK;
+  const size_t nw = c->lim.max_writes ? c->lim.max_writes : 1u;
+  PX_HIP_CHECK(c, hipMalloc(&c->d_write_list, nw * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_cfirst, (size_t)c->ncell * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_wkeys, nw * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_wsorted, nw * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_wdirty, nw * 4));
+  PX_HIP_CHECK(c, hipMemsetAsync(c->d_write_list, 0, nw * 8, c->stream));
+  PX_HIP_CHECK(c, hipMemsetAsync(c->d_cfirst, 0xFF, (size_t)c->ncell * 4, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+void px_writes_free(px_ctx* c) {
+  void* bufs[] = {c->d_write_list, c->d_cfirst, c->d_wkeys, c->d_wsorted, c->d_wdirty, c->d_sort_tmp};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  c->d_write_list = c->d_cfirst = c->d_wdirty = nullptr;
+  c->d_wkeys = c->d_wsorted = nullptr;
+  c->d_sort_tmp = nullptr;
+  c->sort_tmp_bytes = 0;
+}
+
+int px_launch_writes(px_ctx* c) {
+  if (int rc = px_writes_alloc(c)) return rc;
+  if (c->lim.max_writes == 0) return PX_OK;
+  PxProfScope prof(c, PX_PROF_WRITES);
+  EditView e = make_view(c);
+  hipStream_t s = c->stream;
+  const uint32_t nw = c->lim.max_writes;
+  const dim3 G(std::min<uint32_t>((nw + 255u) / 256u, 2048u)), W(256);
+  hipLaunchKernelGGL(k_write_args, dim3(1), dim3(64), 0, s, e);
+  hipLaunchKernelGGL(k_write_first, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_write_request, dim3(1), dim3(1024), 0, s, e);
+  hipLaunchKernelGGL(k_alloc, dim3(kEditBlocks), dim3(64), 0, s, e);
+  hipLaunchKernelGGL(k_alloc_commit, dim3(1), dim3(64), 0, s, e);
+  PX_HIP_CHECK(c, hipGetLastError());
+  // column bits: x + dx * z < dx * dz
+  const uint64_t ncol = (uint64_t)c->dims[0] * c->dims[2];
+  uint32_t bits = 1;
+  while (bits < 32 && (1ull << bits) < ncol) bits++;
+  if (int rc = px_sort_u64(c, c->d_wkeys, c->d_wsorted, nw, 32u + bits)) return rc;
+  hipLaunchKernelGGL(k_write_apply, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_write_maint, dim3(kEditBlocks), dim3(64), 0, s, e);
+  hipLaunchKernelGGL(k_write_finish, dim3(1), dim3(64), 0, s, e);
+  PX_HIP_CHECK(c, hipGetLastError());
+  c->masks_dirty = true;
+  return px_refresh_masks(c);
+}
+
+extern "C" int px_push_writes(px_ctx* c, const uint32_t* pairs, uint32_t n) {
+  if (!c || (n && !pairs)) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int rc = px_writes_alloc(c)) return rc;
+  if (n == 0) return PX_OK;
+  // staged in the sorted-key buffer (max_writes pairs fit; later ones are dropped anyway)
+  const uint32_t m = std::min(n, c->lim.max_writes);
+  if (m) PX_HIP_CHECK(c, hipMemcpyAsync(c->d_wsorted, pairs, (size_t)m * 8, hipMemcpyHostToDevice, c->stream));
+  EditView e = make_view(c);
+  if (m)
+    hipLaunchKernelGGL(k_push_writes, dim3((m + 255u) / 256u), dim3(256), 0, c->stream, e,
+                       reinterpret_cast<const uint32_t*>(c->d_wsorted), m);
+  hipLaunchKernelGGL(k_push_commit, dim3(1), dim3(64), 0, c->stream, e, n);
+  PX_HIP_CHECK(c, hipGetLastError());
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return PX_OK;
 }

@@ -104,13 +104,20 @@ struct px_ctx {
   uint32_t* d_out_labels = nullptr;  // (key, count) pairs
   uint32_t label_cap = 0;
   size_t vox_total = 0;
+  // voxel writes (write list shared by the debris and chunk stages)
+  uint32_t* d_write_list = nullptr; // (key, value) pairs, lim.max_writes of them
+  uint32_t* d_cfirst = nullptr;     // per cell: first write index needing a brick (0xFFFFFFFF = none)
+  uint64_t* d_wkeys = nullptr;      // per write: column << 32 | index, then sorted
+  uint64_t* d_wsorted = nullptr;
+  uint32_t* d_wdirty = nullptr;     // cells the puts dirtied
+  void* d_sort_tmp = nullptr;       // radix-sort temporary storage
+  size_t sort_tmp_bytes = 0;
   // debris particles
   uint32_t max_particles = 0;
   float4* d_particles = nullptr;    // ring: 2 float4 per particle (pos, value) (vel, life)
   uint32_t* d_alive = nullptr;      // alive list (slot indices)
-  uint32_t* d_write_list = nullptr; // (key, value) pairs, lim.max_writes of them
   uint8_t* d_pstatus = nullptr;     // per slot: bit 0 alive, bit 1 write request
-  uint32_t* d_pblock = nullptr;     // per block: alive / write counts, then offsets
+  uint32_t* d_pblock = nullptr;     // per tile: alive / write counts, then offsets
   float4* d_spawn_stage = nullptr;  // host spawn records staged on the device
   uint32_t spawn_cap = 0;
   // device-side checkpoint (px_snapshot / px_restore)
@@ -163,6 +170,11 @@ int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);
 int px_launch_island_pass(px_ctx* c);
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+// voxel writes (px_edit.hip) and the radix sort they use (px_sort.hip)
+int px_writes_alloc(px_ctx* c);
+void px_writes_free(px_ctx* c);
+int px_launch_writes(px_ctx* c);
+int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, uint32_t end_bit);
 // debris particles (px_particles.hip)
 void px_particles_free(px_ctx* c);
 int px_launch_frame_begin(px_ctx* c);

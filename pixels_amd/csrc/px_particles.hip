@@ -10,12 +10,12 @@
 // alive slots and the deposit writes to global lists with one atomicAdd per
 // particle, so the list order is the GPU's schedule and, past MAX_WRITES, the
 // winning requests are whichever came first.  Here a step is three launches
-// with no global atomics:
+// with no global atomics, over tiles of kTile slots (one workgroup each):
 //
 //   k_particle_sim    one lane per slot: the physics, the new record, a
-//                     status byte (alive, write request) and per-block
-//                     counts of both (wave ballots, no atomics);
-//   k_particle_scan   one workgroup: exclusive offsets of the block counts,
+//                     status byte (alive, write request) and per-tile counts
+//                     of both (wave ballots);
+//   k_particle_scan   one workgroup: exclusive offsets of the tile counts,
 //                     added to the list lengths already in args / ctrl;
 //   k_particle_emit   one lane per slot: alive and write entries at their
 //                     slot-order rank; a write request past max_writes gets
@@ -32,7 +32,9 @@
 
 namespace {
 
-constexpr uint32_t kBlock = 256;
+constexpr uint32_t kBlock = 1024;          // 16 wavefronts
+constexpr uint32_t kSub = 1;                // sub-tiles of kBlock slots per workgroup
+constexpr uint32_t kTile = kBlock * kSub;  // slots per workgroup
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kA_DRAWP = 36;  // layout.ts:41-45: 12 dispatch triples, then A_DRAWP
 constexpr uint8_t kAlive = 1;
@@ -46,8 +48,8 @@ PX_DEV bool solid_at(const PxWorldView& w, int32_t x, int32_t y, int32_t z) {
   return px::read_voxel(w, x, y, z) != 0u;
 }
 
-// Rank of this lane among the block's lanes with `flag`, and the block total
-// (4 wavefronts of 64).  Block-cooperative.
+// Rank of this lane among the block's lanes with `flag`, and the block total.
+// Block-cooperative.
 PX_DEV uint32_t block_rank(bool flag, uint32_t* s_wave, uint32_t* total) {
   const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
   const uint64_t m = __ballot(flag);
@@ -60,23 +62,33 @@ PX_DEV uint32_t block_rank(bool flag, uint32_t* s_wave, uint32_t* total) {
     sum += c;
   }
   *total = sum;
+  __syncthreads();  // s_wave may be rewritten by the next call
   return before + lanes_below(m);
 }
 
-__global__ __launch_bounds__(kBlock) void k_particle_sim(PxWorldView w, float4* __restrict__ P, float dt, float grav,
-                                                         float rest_speed, uint8_t* __restrict__ status,
-                                                         uint32_t* __restrict__ bcount) {
-  __shared__ uint32_t s_a[4], s_w[4];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+// Inclusive prefix sum over the wavefront.
+PX_DEV uint32_t wave_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o);
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+// particleSim (sim.wgsl:1088-1173) for slot i; returns its status byte.
+PX_DEV uint8_t sim_slot(const PxWorldView& w, float4* __restrict__ P, uint32_t i, float dt, float grav,
+                        float rest_speed) {
   uint8_t st = 0;
-  float4 b = P[2u * i + 1u];  // vel.xyz, life
+  // both halves at once: one memory round trip for live slots
+  const float4 a = P[2u * i];       // pos.xyz, value
+  const float4 b = P[2u * i + 1u];  // vel.xyz, life
   float life = b.w;
   if (!(life <= 0.0f)) {
     life = life - dt;
     if (life <= 0.0f) {
       P[2u * i + 1u].w = 0.0f;
     } else {
-      float4 a = P[2u * i];  // pos.xyz, value
       uint32_t value = __float_as_uint(a.w);
       if ((value & PX_P_SETTLED) != 0u) {
         P[2u * i + 1u].w = life;
@@ -140,30 +152,33 @@ __global__ __launch_bounds__(kBlock) void k_particle_sim(PxWorldView w, float4* 
       }
     }
   }
-  status[i] = st;
-  uint32_t ta, tw;
-  block_rank((st & kAlive) != 0, s_a, &ta);
-  block_rank((st & kWrite) != 0, s_w, &tw);
+  return st;
+}
+
+__global__ __launch_bounds__(kBlock) void k_particle_sim(PxWorldView w, float4* __restrict__ P, float dt, float grav,
+                                                         float rest_speed, uint8_t* __restrict__ status,
+                                                         uint32_t* __restrict__ bcount) {
+  __shared__ uint32_t s_a[kBlock / 64], s_w[kBlock / 64];
+  uint32_t ca = 0u, cw = 0u;
+  for (uint32_t s = 0; s < kSub; s++) {
+    const uint32_t i = blockIdx.x * kTile + s * kBlock + threadIdx.x;
+    const uint8_t st = sim_slot(w, P, i, dt, grav, rest_speed);
+    status[i] = st;
+    uint32_t ta, tw;
+    block_rank((st & kAlive) != 0, s_a, &ta);
+    block_rank((st & kWrite) != 0, s_w, &tw);
+    ca += ta;
+    cw += tw;
+  }
   if (threadIdx.x == 0) {
-    bcount[2u * blockIdx.x] = ta;
-    bcount[2u * blockIdx.x + 1u] = tw;
+    bcount[2u * blockIdx.x] = ca;
+    bcount[2u * blockIdx.x + 1u] = cw;
   }
 }
 
-// Inclusive prefix sum over the wavefront.
-PX_DEV uint32_t wave_scan(uint32_t v) {
-  const uint32_t lane = __lane_id();
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t t = __shfl_up(v, o);
-    if (lane >= (uint32_t)o) v += t;
-  }
-  return v;
-}
-
-// Exclusive offsets of the per-block counts (in place: bcount[2b], [2b+1]
-// become the first alive slot and the first write index of block b) and the
-// new list lengths.  One workgroup of kScanThreads, each thread owning `per`
-// consecutive blocks.
+// Exclusive offsets of the per-tile counts (in place: bcount[2b], [2b+1]
+// become the first alive slot and the first write index of tile b) and the
+// new list lengths.  One workgroup, each thread owning `per` consecutive tiles.
 __global__ __launch_bounds__(kScanThreads) void k_particle_scan(uint32_t* __restrict__ bcount, uint32_t nb,
                                                                 uint32_t* __restrict__ ctrl,
                                                                 uint32_t* __restrict__ args) {
@@ -186,7 +201,7 @@ __global__ __launch_bounds__(kScanThreads) void k_particle_scan(uint32_t* __rest
     s_a[wave] = ia;
     s_w[wave] = iw;
   }
-  __syncthreads();
+  __syncthreads();  // also orders every thread's read of the bases before the update below
   uint32_t pa = 0u, pw = 0u, ta = 0u, tw = 0u;
   for (uint32_t k = 0; k < kScanThreads / 64u; k++) {
     pa += k < wave ? s_a[k] : 0u;
@@ -205,7 +220,6 @@ __global__ __launch_bounds__(kScanThreads) void k_particle_scan(uint32_t* __rest
       ow += cw;
     }
   }
-  // every thread read the bases before the barrier above
   if (threadIdx.x == 0) {
     args[kA_DRAWP + 1u] = base_a + ta;
     ctrl[PX_C_WRITES] = base_w + tw;  // requests, granted or not (pushWrite counts every call)
@@ -217,32 +231,37 @@ __global__ __launch_bounds__(kBlock) void k_particle_emit(PxWorldView w, float4*
                                                           const uint32_t* __restrict__ boff,
                                                           uint32_t* __restrict__ alive, uint32_t alive_cap,
                                                           uint32_t* __restrict__ wlist, uint32_t max_writes) {
-  __shared__ uint32_t s_a[4], s_w[4];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const uint8_t st = status[i];
-  uint32_t ta, tw;
-  const uint32_t ra = block_rank((st & kAlive) != 0, s_a, &ta);
-  const uint32_t rw = block_rank((st & kWrite) != 0, s_w, &tw);
-  if (st & kAlive) {
-    const uint32_t slot = boff[2u * blockIdx.x] + ra;
-    if (slot < alive_cap) alive[slot] = i;
-  }
-  if (st & kWrite) {
-    const uint32_t k = boff[2u * blockIdx.x + 1u] + rw;
-    const float4 a = P[2u * i];
-    const float life = P[2u * i + 1u].w;
-    const uint32_t value = __float_as_uint(a.w);
-    float nl;
-    if (k < max_writes) {
-      // settled at the cell centre: floor(pos) is the cell
-      const int32_t cx = px_f2i(floorf(a.x)), cy = px_f2i(floorf(a.y)), cz = px_f2i(floorf(a.z));
-      wlist[2u * k] = px::key_of(w, cx, cy, cz);
-      wlist[2u * k + 1u] = (value & 0xffffu) | 0x80u;  // rubbleOf (sim.wgsl:65)
-      nl = fminf(life, 0.5f);
-    } else {
-      nl = fminf(life, (value & PX_P_DUST) != 0u ? 0.8f : 4.0f);
+  __shared__ uint32_t s_a[kBlock / 64], s_w[kBlock / 64];
+  uint32_t oa = boff[2u * blockIdx.x], ow = boff[2u * blockIdx.x + 1u];
+  for (uint32_t s = 0; s < kSub; s++) {
+    const uint32_t i = blockIdx.x * kTile + s * kBlock + threadIdx.x;
+    const uint8_t st = status[i];
+    uint32_t ta, tw;
+    const uint32_t ra = block_rank((st & kAlive) != 0, s_a, &ta);
+    const uint32_t rw = block_rank((st & kWrite) != 0, s_w, &tw);
+    if (st & kAlive) {
+      const uint32_t slot = oa + ra;
+      if (slot < alive_cap) alive[slot] = i;
     }
-    P[2u * i + 1u].w = nl;
+    if (st & kWrite) {
+      const uint32_t k = ow + rw;
+      const float4 a = P[2u * i];
+      const float life = P[2u * i + 1u].w;
+      const uint32_t value = __float_as_uint(a.w);
+      float nl;
+      if (k < max_writes) {
+        // settled at the cell centre: floor(pos) is the cell
+        const int32_t cx = px_f2i(floorf(a.x)), cy = px_f2i(floorf(a.y)), cz = px_f2i(floorf(a.z));
+        wlist[2u * k] = px::key_of(w, cx, cy, cz);
+        wlist[2u * k + 1u] = (value & 0xffffu) | 0x80u;  // rubbleOf (sim.wgsl:65)
+        nl = fminf(life, 0.5f);
+      } else {
+        nl = fminf(life, (value & PX_P_DUST) != 0u ? 0.8f : 4.0f);
+      }
+      P[2u * i + 1u].w = nl;
+    }
+    oa += ta;
+    ow += tw;
   }
 }
 
@@ -304,11 +323,11 @@ int launch_spawn(px_ctx* c, const float4* d_src, uint32_t n) {
 }  // namespace
 
 void px_particles_free(px_ctx* c) {
-  void* bufs[] = {c->d_particles, c->d_alive, c->d_write_list, c->d_pstatus, c->d_pblock, c->d_spawn_stage};
+  void* bufs[] = {c->d_particles, c->d_alive, c->d_pstatus, c->d_pblock, c->d_spawn_stage};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   c->d_particles = nullptr;
-  c->d_alive = c->d_write_list = c->d_pblock = nullptr;
+  c->d_alive = c->d_pblock = nullptr;
   c->d_pstatus = nullptr;
   c->d_spawn_stage = nullptr;
   c->spawn_cap = 0;
@@ -325,7 +344,7 @@ int px_launch_particle_step(px_ctx* c) {
   if (int rc = need_particles(c, "px_particle_step")) return rc;
   const float dt = c->h_uniforms[24], grav = c->h_uniforms[25], vs = c->h_uniforms[3];
   const float rest = 0.6f / vs;
-  const uint32_t nb = c->max_particles / kBlock;
+  const uint32_t nb = c->max_particles / kTile;
   PxProfScope prof(c, PX_PROF_PARTICLES);
   PxWorldView w = px_view(c);
   {
@@ -351,7 +370,8 @@ int px_particles_init(px_ctx* c, uint32_t max_particles) {
   PX_HIP_CHECK(c, hipSetDevice(c->device));
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   px_particles_free(c);
-  const size_t np = max_particles, nb = np / kBlock, nw = c->lim.max_writes;
+  if (int rc = px_writes_alloc(c)) return rc;
+  const size_t np = max_particles, nb = np / kTile;
   int rc = PX_OK;
 #define TRY(x)                                                     \
   do {                                                             \
@@ -364,12 +384,10 @@ int px_particles_init(px_ctx* c, uint32_t max_particles) {
   } while (0)
   TRY(hipMalloc(&c->d_particles, np * sizeof(px_particle)));
   TRY(hipMalloc(&c->d_alive, np * 4));
-  TRY(hipMalloc(&c->d_write_list, (nw ? nw : 1) * 8));
   TRY(hipMalloc(&c->d_pstatus, np));
   TRY(hipMalloc(&c->d_pblock, nb * 8));
   TRY(hipMemsetAsync(c->d_particles, 0, np * sizeof(px_particle), c->stream));
   TRY(hipMemsetAsync(c->d_alive, 0, np * 4, c->stream));
-  TRY(hipMemsetAsync(c->d_write_list, 0, (nw ? nw : 1) * 8, c->stream));
   TRY(hipMemsetAsync(c->d_ctrl + PX_C_PCURSOR, 0, 4, c->stream));
   TRY(hipStreamSynchronize(c->stream));
 #undef TRY
@@ -451,8 +469,8 @@ int px_read_alive(px_ctx* c, uint32_t* out, uint32_t cap, uint32_t* n) {
 
 int px_read_writes(px_ctx* c, uint32_t* pairs, uint32_t cap, uint32_t* n) {
   if (!c || !n || (cap && !pairs)) return PX_ERR_ARG;
-  if (int rc = need_particles(c, "px_read_writes")) return rc;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int rc = px_writes_alloc(c)) return rc;
   uint32_t cnt = 0;
   PX_HIP_CHECK(c, hipMemcpyAsync(&cnt, c->d_ctrl + PX_C_WRITES, 4, hipMemcpyDeviceToHost, c->stream));
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));

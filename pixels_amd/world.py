@@ -327,3 +327,13 @@ class Simulation:
         p, a, w = C.c_void_p(), C.c_void_p(), C.c_void_p()
         self.gw.check(self.lib.px_particle_buffers(self.gw.ctx, C.byref(p), C.byref(a), C.byref(w)))
         return int(p.value or 0), int(a.value or 0), int(w.value or 0)
+
+    # -- voxel writes (sim.wgsl:1178-1203) ------------------------------------------
+    def push_writes(self, pairs: np.ndarray) -> None:
+        """Append (key, value) pairs to the write list (pushWrite from the host)."""
+        w = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        self.gw.check(self.lib.px_push_writes(self.gw.ctx, w.ctypes.data, w.shape[0]))
+
+    def apply_writes(self) -> None:
+        """writeArgs + writeRequest + alloc + writeApply + maint of the touched cells."""
+        self.gw.check(self.lib.px_apply_writes(self.gw.ctx))
