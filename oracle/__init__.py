@@ -60,6 +60,12 @@ def load():
     lib.o_removes.argtypes = [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
     lib.o_apply_writes.restype = C.c_int
     lib.o_apply_writes.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_uint32, C.c_uint32, C.c_uint32, P]
+    lib.o_resolve.restype = C.c_int
+    lib.o_resolve.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, C.c_uint32, P, P, P, P, P, C.c_uint32,
+                              P]
+    lib.o_chunk_frame.restype = C.c_int
+    lib.o_chunk_frame.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, P, P, C.c_float, C.c_float,
+                                  C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, P]
     lib.o_frame_begin.argtypes = [P]
     lib.o_spawn.argtypes = [P, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
     lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
@@ -185,6 +191,89 @@ def apply_writes(dims, vox: np.ndarray, kind: np.ndarray, writes: np.ndarray, ma
     if rc != 0:
         raise MemoryError("oracle writes failed")
     return vox, kind, stats
+
+
+class ResolveParams(C.Structure):
+    _fields_ = [("max_chunks", C.c_uint32), ("max_arena", C.c_uint32), ("component_budget", C.c_uint32),
+                ("max_alloc", C.c_uint32), ("free_top", C.c_uint32), ("vs", C.c_float), ("frame_seed", C.c_uint32)]
+
+
+# counter words of the chunk oracle
+(R_C_CHUNKS, R_C_ARENA, R_S_ISLANDS, R_S_BUDGET, R_S_BILL, R_S_DESTROYED, R_ALLOCS, R_ALLOC_FAIL, R_S_FALLING,
+ R_C_WRITES, R_C_IMPACTS) = range(11)
+R_WORDS = 12
+
+
+def frame_seed(frame: float) -> int:
+    """frameSeed() of sim.wgsl:69: u32(U.time.y) * 0x9E3779B9."""
+    f = float(np.float32(frame))
+    u = 0 if not f > 0 else min(int(f), 0xFFFFFFFF)
+    return (u * 0x9E3779B9) & 0xFFFFFFFF
+
+
+class Chunks:
+    """Serial restatement of island extraction and the falling chunks: the
+    chunk table, the arena, the counters, and the write / impact lists the
+    chunk stages push to."""
+
+    def __init__(self, dims, mats, max_chunks=256, max_arena=1 << 20, component_budget=400000, max_alloc=65535,
+                 max_writes=131072, max_impacts=64):
+        self.dims = tuple(int(d) for d in dims)
+        self.mats = np.ascontiguousarray(mats, np.float32)
+        self.max_chunks, self.max_arena = max_chunks, max_arena
+        self.component_budget, self.max_alloc = component_budget, max_alloc
+        self.max_writes, self.max_impacts = max_writes, max_impacts
+        self.chunks = np.zeros(max_chunks * 16, np.uint32)
+        self.arena = np.zeros(2 * max_arena, np.uint32)
+        self.ctr = np.zeros(R_WORDS, np.uint32)
+        self.writes = np.zeros(2 * max_writes, np.uint32)
+        self.impacts = np.zeros(max_impacts * 16, np.float32)
+
+    def params(self, vs, frame, free_top):
+        return ResolveParams(self.max_chunks, self.max_arena, self.component_budget, self.max_alloc,
+                             min(int(free_top), 0xFFFFFFFF), float(np.float32(vs)), frame_seed(frame))
+
+    def resolve(self, vox, kind, keys, counts, vs, frame, free_top=1 << 31, spawn_cap=1 << 22):
+        """Returns (vox_after, kind_after, spawn records in order)."""
+        vox = np.array(vox, np.uint16, copy=True, order="C")
+        kind = np.array(kind, np.uint8, copy=True, order="C")
+        keys = np.ascontiguousarray(keys, np.uint32)
+        counts = np.ascontiguousarray(counts, np.uint32)
+        sp = np.zeros(spawn_cap, PARTICLE_DTYPE)
+        ns = C.c_uint32(0)
+        prm = self.params(vs, frame, free_top)
+        d = self.dims
+        rc = load().o_resolve(d[0], d[1], d[2], vox.ctypes.data, kind.ctypes.data, self.mats.ctypes.data,
+                              keys.ctypes.data, counts.ctypes.data, keys.size, C.byref(prm), self.chunks.ctypes.data,
+                              self.arena.ctypes.data, self.ctr.ctypes.data, sp.ctypes.data, spawn_cap, C.byref(ns))
+        if rc != 0:
+            raise MemoryError("oracle resolve failed")
+        return vox, kind, sp[:min(spawn_cap, ns.value)].copy()
+
+    def frame(self, vox, kind, dt, gravity, vs, frame, spawn_cap=1 << 22):
+        """One frame of the chunk stages; returns spawn records in order."""
+        vox = np.ascontiguousarray(vox, np.uint16)
+        kind = np.ascontiguousarray(kind, np.uint8)
+        sp = np.zeros(spawn_cap, PARTICLE_DTYPE)
+        ns = C.c_uint32(0)
+        prm = self.params(vs, frame, 0)
+        d = self.dims
+        rc = load().o_chunk_frame(d[0], d[1], d[2], vox.ctypes.data, kind.ctypes.data, self.mats.ctypes.data,
+                                  self.chunks.ctypes.data, self.arena.ctypes.data, self.ctr.ctypes.data,
+                                  C.byref(prm), float(np.float32(dt)), float(np.float32(gravity)), self.max_writes,
+                                  self.writes.ctypes.data, self.max_impacts, self.impacts.ctypes.data,
+                                  sp.ctypes.data, spawn_cap, C.byref(ns))
+        if rc != 0:
+            raise MemoryError("oracle chunk frame failed")
+        return sp[:min(spawn_cap, ns.value)].copy()
+
+    def write_list(self):
+        n = min(int(self.ctr[R_C_WRITES]), self.max_writes)
+        return self.writes[:2 * n].reshape(-1, 2).copy()
+
+    def impact_list(self):
+        n = min(int(self.ctr[R_C_IMPACTS]), self.max_impacts)
+        return self.impacts[:16 * n].reshape(-1, 16).copy()
 
 
 def islands(dims, vox: np.ndarray, mats, cap: int = 1 << 20):

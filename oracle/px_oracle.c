@@ -15,6 +15,7 @@
  *   maint compaction                    src/render/shaders/sim.wgsl:1219-1274
  *   spawn / frameBegin / particleSim    src/render/shaders/sim.wgsl:166-170, 363-369, 1088-1173
  *   voxel writes (request, apply)       src/render/shaders/sim.wgsl:94-102, 127-146, 1178-1203
+ *   extraction, falling chunks          src/render/shaders/sim.wgsl:172-178, 274-284, 862-923, 951-1075
  * and a textbook breadth-first 6-connected labelling for the islands.
  *
  * Choices the WGSL leaves to the implementation, fixed here (the HIP library
@@ -1118,6 +1119,351 @@ int o_apply_writes(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* k
     stats[1] = allocated;
     stats[2] = dropped;
     stats[3] = dirty;
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Island extraction and falling chunks: resolveChunks / resolveExpand /
+ * resolveExtract (sim.wgsl:862-923) with dropRubbleAbove (:274-284) and the
+ * crumb debris; chunkArgs / chunkBegin / chunkCollide / chunkStep / chunkLand
+ * (:951-1075).  Serial restatement in a fixed order, which the HIP library
+ * reproduces: unsupported components in increasing label order, voxels in
+ * cell order then local index order (x | y<<3 | z<<6), arena entries in
+ * arena order.  The reference takes these orders from its atomics. */
+
+#define ANY_NONE 0xFFFFFFFFu
+
+static inline float rnd01(uint32_t seed) { return (float)(o_hashu(seed) & 0xffffffu) / 16777216.0f; }
+
+static uint32_t debris_flags(const float* mats, uint32_t v, uint32_t seed) {
+  uint32_t id = v & 0xffu;
+  if (matD(mats, id)[2] > 0.0f) return 0u; /* liquids splash and vanish */
+  int fragile = matE(mats, id)[1] > 0.5f;
+  if (rnd01(seed ^ 0x51ed27u) < (fragile ? 0.55f : 0.75f)) return P_DEPOSIT;
+  if (rnd01(seed ^ 0x9e3779u) < 0.5f) return P_DUST;
+  return 0u;
+}
+
+static void push_spawn(o_particle* out, uint32_t cap, uint32_t* n, const float pos[3], const float vel[3],
+                       uint32_t value, float life) {
+  if (*n < cap) {
+    o_particle* p = &out[*n];
+    for (int a = 0; a < 3; a++) {
+      p->pos[a] = pos[a];
+      p->vel[a] = vel[a];
+    }
+    p->value = value;
+    p->life = life;
+  }
+  (*n)++;
+}
+
+/* takeVoxel (sim.wgsl:105-124): pool bricks only; mat = material id or 256 (any) */
+static uint32_t take_voxel(ed_world* e, int32_t x, int32_t y, int32_t z, uint32_t mat) {
+  if (!in_world(e, x, y, z)) return 0u;
+  size_t ci = cell_of(e, x, y, z);
+  if (e->kind[ci] != 2) return 0u;
+  size_t k = vkey(e, x, y, z);
+  uint32_t v = e->vox[k];
+  if (v == 0u || (mat < 256u && (v & 0xffu) != mat)) return 0u;
+  e->vox[k] = 0;
+  e->dirty[ci] = 1;
+  return v;
+}
+
+static uint32_t read_dense(const ed_world* e, int32_t x, int32_t y, int32_t z) {
+  if (!in_world(e, x, y, z)) return 0u;
+  return e->vox[vkey(e, x, y, z)];
+}
+
+/* dropRubbleAbove (sim.wgsl:274-284) */
+static void drop_rubble_above(ed_world* e, int32_t x, int32_t y, int32_t z, float vs, uint32_t fs, o_particle* sp,
+                              uint32_t cap, uint32_t* ns) {
+  for (int32_t d = 1; d <= 12; d++) {
+    int32_t qy = y + d;
+    uint32_t v = read_dense(e, x, qy, z);
+    if (v == 0u || !is_rubble(v)) return;
+    uint32_t t = take_voxel(e, x, qy, z, v & 0xffu);
+    if (t == 0u) return;
+    uint32_t seed = (uint32_t)vkey(e, x, qy, z) ^ fs;
+    float pos[3] = {(float)x + 0.5f, (float)qy + 0.5f, (float)z + 0.5f};
+    float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
+    float m[3] = {0.4f, 0.0f, 0.4f};
+    float vel[3];
+    for (int a = 0; a < 3; a++) vel[a] = ((r[a] - 0.5f) * m[a]) / vs;
+    push_spawn(sp, cap, ns, pos, vel, t | P_DEPOSIT, 6.0f + 2.0f * rnd01(seed + 5u));
+  }
+}
+
+int o_resolve(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+              const uint32_t* lab_key, const uint32_t* lab_count, uint32_t n_labels, const o_resolve_params* prm,
+              uint32_t* chunks, uint32_t* arena, uint32_t* ctr, o_particle* spawns, uint32_t spawn_cap,
+              uint32_t* n_spawns) {
+  ed_world E;
+  E.dx = dx; E.dy = dy; E.dz = dz;
+  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
+  E.vox = vox;
+  E.kind = kind;
+  E.mats = mats;
+  size_t N = (size_t)dx * dy * dz, ncell = (size_t)E.bx * E.by * E.bz;
+  E.dirty = (uint8_t*)calloc(ncell, 1);
+  uint32_t* comp = (uint32_t*)calloc(N, sizeof(uint32_t)); /* 0 = not extracted, else tag */
+  uint32_t* q = (uint32_t*)malloc(N * sizeof(uint32_t) + 4);
+  uint8_t* req = (uint8_t*)calloc(ncell, 1);
+  if (!E.dirty || !comp || !q || !req) {
+    free(E.dirty); free(comp); free(q); free(req);
+    return -1;
+  }
+  *n_spawns = 0;
+  size_t plane = (size_t)dx * dy;
+  /* resolveChunks: tag = (cid + 1) for chunk voxels, 0x8000 | anything for crumbs */
+  for (uint32_t i = 0; i < n_labels; i++) {
+    uint32_t cnt = lab_count[i];
+    if (cnt > prm->component_budget) {
+      ctr[O_R_S_BUDGET]++;
+      continue; /* assumed supported */
+    }
+    uint32_t tag = 0x10000u; /* crumble */
+    if (cnt >= 48u) {
+      uint32_t id = ctr[O_R_C_CHUNKS]++;
+      if (id < prm->max_chunks) {
+        uint32_t* b = chunks + (size_t)id * 16u;
+        for (int k = 0; k < 16; k++) b[k] = 0u;
+        b[6] = ANY_NONE;
+        b[0] = 1u;
+        ctr[O_R_S_ISLANDS]++;
+        tag = id + 1u;
+      }
+    }
+    /* flood the component from its minimum voxel */
+    size_t qh = 0, qt = 0;
+    size_t s0 = lab_key[i];
+    comp[s0] = tag;
+    q[qt++] = (uint32_t)s0;
+    while (qh < qt) {
+      size_t k = q[qh++];
+      int32_t x = (int32_t)(k % (size_t)dx), y = (int32_t)((k / (size_t)dx) % (size_t)dy), z = (int32_t)(k / plane);
+      for (int d = 0; d < 6; d++) {
+        int32_t nx = x + DIRS[d][0], ny = y + DIRS[d][1], nz = z + DIRS[d][2];
+        if (!in_world(&E, nx, ny, nz)) continue;
+        size_t nk = vkey(&E, nx, ny, nz);
+        uint32_t w = vox[nk];
+        if (w == 0u || is_rubble(w) || structural(mats, w) || comp[nk]) continue;
+        comp[nk] = tag;
+        q[qt++] = (uint32_t)nk;
+      }
+    }
+  }
+  /* resolveExpand + allocBricks: uniform cells holding extracted voxels, in cell order */
+  uint32_t nreq = 0;
+  for (size_t ci = 0; ci < ncell; ci++) {
+    if (kind[ci] != 1) continue;
+    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8, oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
+    if (!comp[vkey(&E, ox, oy, oz)]) continue; /* a uniform brick is one component */
+    if (nreq < prm->max_alloc) {
+      if (nreq < prm->free_top) {
+        kind[ci] = 2;
+        E.dirty[ci] = 1;
+        ctr[O_R_ALLOCS]++;
+      } else {
+        ctr[O_R_ALLOC_FAIL]++;
+      }
+      nreq++;
+    }
+  }
+  /* resolveExtract in cell order, local index order */
+  for (size_t ci = 0; ci < ncell; ci++) {
+    if (kind[ci] != 2) continue;
+    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8, oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
+    for (uint32_t l = 0; l < 512u; l++) {
+      int32_t x = ox + (int32_t)(l & 7u), y = oy + (int32_t)((l >> 3) & 7u), z = oz + (int32_t)(l >> 6);
+      size_t k = vkey(&E, x, y, z);
+      uint32_t tag = comp[k];
+      if (!tag) continue;
+      comp[k] = 0;
+      uint32_t v = take_voxel(&E, x, y, z, 256u);
+      if (v == 0u) continue;
+      drop_rubble_above(&E, x, y, z, prm->vs, prm->frame_seed, spawns, spawn_cap, n_spawns);
+      if (tag < 0x10000u) {
+        uint32_t a = ctr[O_R_C_ARENA]++;
+        if (a < prm->max_arena) {
+          arena[2u * a] = (uint32_t)k;
+          arena[2u * a + 1u] = v | ((tag - 1u) << 16);
+          uint32_t* b = chunks + (size_t)(tag - 1u) * 16u;
+          if (matE(mats, v & 0xffu)[1] > 0.5f) b[4] = 1u;
+          b[5]++;
+          continue;
+        }
+      }
+      /* crumbs: straight to debris */
+      ctr[O_R_S_BILL] += price_dimes(mats, v);
+      ctr[O_R_S_DESTROYED]++;
+      uint32_t seed = (uint32_t)k ^ prm->frame_seed;
+      float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+      float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
+      float off[3] = {0.5f, 0.0f, 0.5f}, m[3] = {0.6f, 0.3f, 0.6f};
+      float vel[3];
+      for (int a = 0; a < 3; a++) vel[a] = ((r[a] - off[a]) * m[a]) / prm->vs;
+      push_spawn(spawns, spawn_cap, n_spawns, pos, vel, v | debris_flags(mats, v, seed), 4.0f + 3.0f * rnd01(seed + 9u));
+    }
+  }
+  maint_dirty(&E, ncell);
+  free(E.dirty);
+  free(comp);
+  free(q);
+  free(req);
+  return 0;
+}
+
+static inline float u2f(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static inline uint32_t f2u_bits(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+/* WGSL u32(f32): truncation toward zero, saturating, NaN -> 0 */
+static inline uint32_t f2u(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967040.0f) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+                  uint32_t* chunks, uint32_t* arena, uint32_t* ctr, const o_resolve_params* prm, float dt,
+                  float gravity, uint32_t max_writes, uint32_t* writes, uint32_t max_impacts, float* impacts,
+                  o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns) {
+  ed_world E;
+  E.dx = dx; E.dy = dy; E.dz = dz;
+  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
+  E.vox = vox;
+  E.kind = kind;
+  E.mats = mats;
+  E.dirty = NULL;
+  *n_spawns = 0;
+  const float vs = prm->vs;
+  /* chunkArgs (:951-975) */
+  uint32_t n = ctr[O_R_C_CHUNKS] < prm->max_chunks ? ctr[O_R_C_CHUNKS] : prm->max_chunks;
+  uint32_t active = 0;
+  for (uint32_t c = 0; c < n; c++) {
+    uint32_t* b = chunks + (size_t)c * 16u;
+    uint32_t st = b[0];
+    if (st == 2u) b[0] = 0u;
+    if (st == 1u && (b[5] == 0u || b[2] > (uint32_t)dy)) {
+      b[0] = 0u;
+      st = 0u;
+    }
+    if (st == 1u) active++;
+  }
+  if (active == 0u) {
+    ctr[O_R_C_CHUNKS] = 0u;
+    ctr[O_R_C_ARENA] = 0u;
+  }
+  ctr[O_R_S_FALLING] = active;
+  n = ctr[O_R_C_CHUNKS] < prm->max_chunks ? ctr[O_R_C_CHUNKS] : prm->max_chunks;
+  uint32_t na = ctr[O_R_C_ARENA] < prm->max_arena ? ctr[O_R_C_ARENA] : prm->max_arena;
+  /* chunkBegin (:977-991) */
+  for (uint32_t c = 0; c < n; c++) {
+    uint32_t* b = chunks + (size_t)c * 16u;
+    if (b[0] != 1u) continue;
+    float vy = fminf(u2f(b[1]) + gravity * dt, 25.0f / vs);
+    float travel = vy * dt + u2f(b[3]);
+    uint32_t steps = f2u(floorf(travel));
+    if (steps > 48u) steps = 48u;
+    b[1] = f2u_bits(vy);
+    b[3] = f2u_bits(travel - floorf(travel));
+    b[8] = steps;
+    b[6] = ANY_NONE;
+    b[7] = 0u;
+  }
+  /* chunkCollide (:993-1021), arena order */
+  for (uint32_t j = 0; j < na; j++) {
+    uint32_t vv = arena[2u * j + 1u], v = vv & 0xffffu;
+    if (v == 0u) continue;
+    uint32_t* b = chunks + (size_t)(vv >> 16) * 16u;
+    if (b[0] != 1u) continue;
+    uint32_t steps = b[8];
+    if (steps == 0u) continue;
+    uint32_t key = arena[2u * j];
+    int32_t px = (int32_t)(key % (uint32_t)dx), py = (int32_t)((key / (uint32_t)dx) % (uint32_t)dy) - (int32_t)b[2];
+    int32_t pz = (int32_t)(key / ((uint32_t)dx * (uint32_t)dy));
+    float speed = u2f(b[1]) * vs;
+    for (uint32_t s = 1; s <= steps; s++) {
+      int32_t qy = py - (int32_t)s;
+      if (qy < 0) {
+        if (s < b[6]) b[6] = s;
+        break;
+      }
+      uint32_t w = read_dense(&E, px, qy, pz);
+      if (w != 0u) {
+        if (matE(mats, w & 0xffu)[1] > 0.5f && speed > 1.5f && (b[7]++) < 6u) {
+          uint32_t k = ctr[O_R_C_IMPACTS]++;
+          if (k < max_impacts) {
+            float* im = impacts + (size_t)k * 16u;
+            for (int t = 0; t < 16; t++) im[t] = 0.0f;
+            im[0] = (float)px + 0.5f; im[1] = (float)qy + 0.5f; im[2] = (float)pz + 0.5f;
+            im[3] = 0.02f / vs; im[4] = 0.12f / vs; im[5] = speed * 0.4f / vs;
+            im[6] = 0.0f; im[7] = 1.0f; im[8] = 0.0f;
+            im[9] = 0.0f; im[10] = -1.0f; im[11] = 0.0f;
+          }
+        }
+        if (s < b[6]) b[6] = s;
+        break;
+      }
+    }
+  }
+  /* chunkStep (:1023-1042) */
+  for (uint32_t c = 0; c < n; c++) {
+    uint32_t* b = chunks + (size_t)c * 16u;
+    if (b[0] != 1u) continue;
+    uint32_t steps = b[8];
+    if (steps == 0u) continue;
+    uint32_t blocked = b[6], drop = steps;
+    if (blocked <= steps) {
+      drop = blocked - 1u;
+      if (b[7] > 0u) {
+        b[1] = f2u_bits(u2f(b[1]) * 0.6f);
+      } else {
+        b[9] = f2u_bits(u2f(b[1]) * vs);
+        b[0] = 2u;
+      }
+    }
+    b[2] += drop;
+  }
+  /* chunkLand (:1044-1075), arena order */
+  for (uint32_t j = 0; j < na; j++) {
+    uint32_t vv = arena[2u * j + 1u], v = vv & 0xffffu;
+    if (v == 0u) continue;
+    uint32_t* b = chunks + (size_t)(vv >> 16) * 16u;
+    if (b[0] != 2u) continue;
+    arena[2u * j + 1u] = 0u;
+    uint32_t key = arena[2u * j];
+    int32_t px = (int32_t)(key % (uint32_t)dx), py = (int32_t)((key / (uint32_t)dx) % (uint32_t)dy) - (int32_t)b[2];
+    int32_t pz = (int32_t)(key / ((uint32_t)dx * (uint32_t)dy));
+    float speed = u2f(b[9]);
+    if (b[4] != 0u && speed > 2.0f && matE(mats, v & 0xffu)[1] > 0.5f) {
+      ctr[O_R_S_BILL] += price_dimes(mats, v);
+      ctr[O_R_S_DESTROYED]++;
+      uint32_t seed = key ^ prm->frame_seed;
+      float r[3] = {(rnd01(seed) - 0.5f) * 0.9f, rnd01(seed + 1u) * 0.5f, (rnd01(seed + 2u) - 0.5f) * 0.9f};
+      float vel[3];
+      for (int a = 0; a < 3; a++) vel[a] = (r[a] * speed) / vs;
+      float pos[3] = {(float)px + 0.5f, (float)py + 0.5f, (float)pz + 0.5f};
+      push_spawn(spawns, spawn_cap, n_spawns, pos, vel, v | debris_flags(mats, v, seed), 5.0f + 3.0f * rnd01(seed + 3u));
+      continue;
+    }
+    uint32_t k = ctr[O_R_C_WRITES]++;
+    if (k < max_writes) {
+      int32_t p[3] = {px, py, pz};
+      writes[2u * k] = (uint32_t)vkey(&E, p[0], p[1], p[2]);
+      writes[2u * k + 1u] = v;
+    }
   }
   return 0;
 }

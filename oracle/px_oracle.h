@@ -113,6 +113,43 @@ void o_particle_step(const o_world* w, o_particle* parts, uint32_t max_particles
 int o_apply_writes(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const uint32_t* writes,
                    uint32_t n, uint32_t max_alloc, uint32_t free_top, uint64_t* stats);
 
+/* Island extraction and falling chunks (sim.wgsl:862-923, 951-1075): chunk
+ * table words as the reference (0 state, 1 vy f32, 2 dy, 3 frac f32,
+ * 4 fragile, 5 voxel count, 6 blockedAt, 7 broke count, 8 steps, 9 landing
+ * speed f32); arena entries (key, value | chunk << 16). */
+typedef struct o_resolve_params {
+  uint32_t max_chunks, max_arena, component_budget, max_alloc, free_top;
+  float vs;            /* U.dims.w */
+  uint32_t frame_seed; /* u32(U.time.y) * 0x9E3779B9 */
+} o_resolve_params;
+/* counters of the chunk stages */
+#define O_R_C_CHUNKS 0
+#define O_R_C_ARENA 1
+#define O_R_S_ISLANDS 2
+#define O_R_S_BUDGET 3
+#define O_R_S_BILL 4
+#define O_R_S_DESTROYED 5
+#define O_R_ALLOCS 6
+#define O_R_ALLOC_FAIL 7
+#define O_R_S_FALLING 8
+#define O_R_C_WRITES 9
+#define O_R_C_IMPACTS 10
+#define O_R_WORDS 12
+/* resolveChunks + resolveExpand + allocBricks + resolveExtract + maint on the
+ * dense world after an island pass; labels sorted by key (o_islands).
+ * Spawned records (dropRubbleAbove, crumbs) go to spawns in order. */
+int o_resolve(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+              const uint32_t* lab_key, const uint32_t* lab_count, uint32_t n_labels, const o_resolve_params* prm,
+              uint32_t* chunks, uint32_t* arena, uint32_t* ctr, o_particle* spawns, uint32_t spawn_cap,
+              uint32_t* n_spawns);
+/* One frame of chunkArgs / chunkBegin / chunkCollide / chunkStep / chunkLand:
+ * writes (key, value) pushed at C_WRITES, impacts (16 floats) at C_IMPACTS,
+ * shattered voxels spawned in arena order.  The world is read, not written. */
+int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+                  uint32_t* chunks, uint32_t* arena, uint32_t* ctr, const o_resolve_params* prm, float dt,
+                  float gravity, uint32_t max_writes, uint32_t* writes, uint32_t max_impacts, float* impacts,
+                  o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns);
+
 #ifdef __cplusplus
 }
 #endif
