@@ -236,10 +236,12 @@ int px_island_pass(px_ctx* ctx);
  * the island pass above (a no-op without queued impacts).  Unknown bits are
  * an error. */
 #define PX_STAGE_EDIT 1u
-/* PX_STAGE_FRAME_BEGIN = px_frame_begin, run first; PX_STAGE_PARTICLES =
- * px_particle_step, run after the edit stage as particleSim runs after the
- * edit passes; PX_STAGE_WRITES = px_apply_writes, run last as the write
- * passes follow particleSim (Simulation.ts:223-268). */
+/* Stage bits, run in this order (Simulation.ts:223-268): PX_STAGE_FRAME_BEGIN
+ * (px_frame_begin), PX_STAGE_EDIT (the island pass, when impacts are queued,
+ * including those falling chunks queued), PX_STAGE_RESOLVE
+ * (px_resolve_islands, when an island pass has run since the last one),
+ * PX_STAGE_CHUNKS (px_chunk_step), PX_STAGE_PARTICLES (px_particle_step),
+ * PX_STAGE_WRITES (px_apply_writes). */
 #define PX_STAGE_PARTICLES 2u
 #define PX_STAGE_FRAME_BEGIN 4u
 int px_sim_step(px_ctx* ctx, uint32_t stages_mask);
@@ -324,6 +326,42 @@ int px_apply_writes(px_ctx* ctx);
  * that deposit voxels from the host.  Blocking on the copy. */
 int px_push_writes(px_ctx* ctx, const uint32_t* pairs, uint32_t n);
 
+/* ---- island extraction and falling chunks -------------------------------------
+ * px_resolve_islands: resolveChunks + resolveExpand + allocBricks +
+ * resolveExtract (sim.wgsl:862-923) on the labels of the last island pass:
+ * a component over component_budget voxels stays (S_BUDGET); of the others,
+ * those with >= 48 voxels take the free chunk ids in increasing label order
+ * (C_CHUNKS counts them all, S_ISLANDS the ones that got an id), the rest
+ * crumble.  Uniform bricks holding such voxels get pool bricks first; then
+ * every such voxel, in cell order and local index order, leaves the world:
+ * the rubble stacked on it tumbles as debris (dropRubbleAbove :274-284), and
+ * the voxel goes to its chunk's arena (C_ARENA) or, crumbling or past
+ * max_arena, becomes debris (S_BILL, S_DESTROYED).  Debris is spawned into
+ * the particle ring (allocated with the default size if px_particles_init was
+ * not called), under the frame's spawn budget; maint runs on the touched
+ * cells.  The reference orders these by its atomics.  Asynchronous; fails
+ * with PX_ERR_STATE when no island pass has run since the last call. */
+int px_resolve_islands(px_ctx* ctx);
+/* One frame of the falling chunks: chunkArgs, chunkBegin, chunkCollide,
+ * chunkStep, chunkLand (sim.wgsl:951-1075) with dt, gravity and the voxel size
+ * of the uniforms.  A chunk crossing a fragile voxel at > 1.5 m/s breaks
+ * through and queues impacts (at most 6 per chunk, in arena order; they join
+ * the next island pass, which then reads them back); a landed chunk's voxels
+ * are pushed to the write list in arena order, or shatter into debris when
+ * the chunk is fragile and lands faster than 2 m/s.  Asynchronous. */
+int px_chunk_step(px_ctx* ctx);
+/* Chunk table (max_chunks x 16 words, the reference's layout: state, vy,
+ * fallen, frac, fragile, voxel count, blockedAt, broke count, steps, landing
+ * speed) and the arena (key, value | chunk << 16 pairs, min(C_ARENA,
+ * max_arena) of them; *n_arena gets the count, at most arena_cap copied).
+ * Either pointer may be NULL.  Blocking. */
+int px_read_chunks(px_ctx* ctx, uint32_t* table, uint32_t* arena, uint32_t arena_cap, uint32_t* n_arena);
+/* Impacts queued by falling chunks and not yet taken by an island pass
+ * (16 floats each, pushImpact's layout); *n gets C_IMPACTS.  Blocking. */
+int px_read_device_impacts(px_ctx* ctx, float* out16, uint32_t cap, uint32_t* n);
+#define PX_STAGE_RESOLVE 16u
+#define PX_STAGE_CHUNKS 32u
+
 /* ---- checkpoint of the device world (no counterpart in the reference, which
  * rebuilds its world from the seed on every load and never saves GPU state).
  * px_snapshot copies grid, pool, super grid, ctrl words and free list to a
@@ -347,6 +385,7 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_PARTICLES 6 /* spawn + particle step, all kernels */
 #define PX_PROF_PARTICLE_SIM 7 /* the integration kernel of the step alone */
 #define PX_PROF_WRITES 8   /* voxel-write stage, all kernels */
+#define PX_PROF_CHUNKS 9   /* island extraction and chunk frames, all kernels */
 #define PX_PROF_SLOTS 16
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);

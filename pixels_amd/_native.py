@@ -23,7 +23,7 @@ PX_ARGS_WORDS = 44
 PX_CAMERA_FLOATS = 64
 
 RAY_PRIMARY, RAY_SHADOW, RAY_SHOOT, RAY_SEGMENT, RAY_MIXED = 0, 1, 2, 3, 4
-STAGE_EDIT, STAGE_PARTICLES, STAGE_FRAME_BEGIN, STAGE_WRITES = 1, 2, 4, 8
+STAGE_EDIT, STAGE_PARTICLES, STAGE_FRAME_BEGIN, STAGE_WRITES, STAGE_RESOLVE, STAGE_CHUNKS = 1, 2, 4, 8, 16, 32
 P_SETTLED, P_DEPOSIT, P_DUST = 0x10000, 0x20000, 0x40000
 FACE_ZERO, FACE_NOT_CAST, FACE_MISS = 6, 0xFE, 0xFF
 
@@ -131,12 +131,16 @@ SIGNATURES = {
     "px_read_writes": (C.c_int, [_P, _U32P, _U32, _U32P]),
     "px_particle_buffers": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "px_apply_writes": (C.c_int, [_P]),
+    "px_resolve_islands": (C.c_int, [_P]),
+    "px_chunk_step": (C.c_int, [_P]),
+    "px_read_chunks": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
+    "px_read_device_impacts": (C.c_int, [_P, _FP, _U32, _U32P]),
     "px_push_writes": (C.c_int, [_P, _P, _U32]),
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_PARTICLES = 0, 1, 2, 3, 4, 5, 6
-PROF_PARTICLE_SIM, PROF_WRITES = 7, 8
+PROF_PARTICLE_SIM, PROF_WRITES, PROF_CHUNKS = 7, 8, 9
 PROF_SLOTS = 16
 
 _lib = None

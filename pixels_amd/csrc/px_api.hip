@@ -5,6 +5,7 @@
 // (:90-100), free list = holes then the top of the pool in descending order
 // (:103-113), C_FREE_TOP (:111-113), indirect draw args (:114-117).
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -94,6 +95,7 @@ static void release(px_ctx* c) {
   px_edit_free(c);
   px_particles_free(c);
   px_writes_free(c);
+  px_chunks_free(c);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -515,12 +517,14 @@ int px_profile_end(px_ctx* c, double out[2 * PX_PROF_SLOTS]) {
 int px_island_pass(px_ctx* c) {
   if (!c) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int rc = px_merge_device_impacts(c)) return rc;
   return px_launch_island_pass(c);
 }
 
 int px_sim_step(px_ctx* c, uint32_t stages_mask) {
   if (!c) return PX_ERR_ARG;
-  const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN | PX_STAGE_WRITES;
+  const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN | PX_STAGE_WRITES |
+                         PX_STAGE_RESOLVE | PX_STAGE_CHUNKS;
   if (stages_mask & ~known) return px_fail(c, PX_ERR_ARG, "px_sim_step", "unknown stage bits");
   if ((stages_mask & PX_STAGE_PARTICLES) && !c->d_particles)
     return px_fail(c, PX_ERR_STATE, "px_sim_step", "particle stage without px_particles_init");
@@ -528,13 +532,68 @@ int px_sim_step(px_ctx* c, uint32_t stages_mask) {
   if (stages_mask & PX_STAGE_FRAME_BEGIN) {
     if (int rc = px_launch_frame_begin(c)) return rc;
   }
-  if ((stages_mask & PX_STAGE_EDIT) && c->n_impacts) {
-    if (int rc = px_launch_island_pass(c)) return rc;
+  if (stages_mask & PX_STAGE_EDIT) {
+    if (int rc = px_merge_device_impacts(c)) return rc;
+    if (c->n_impacts) {
+      if (int rc = px_launch_island_pass(c)) return rc;
+    }
+  }
+  if ((stages_mask & PX_STAGE_RESOLVE) && c->labels_valid) {
+    if (int rc = px_launch_resolve(c)) return rc;
+  }
+  if (stages_mask & PX_STAGE_CHUNKS) {
+    if (int rc = px_launch_chunk_frame(c)) return rc;
   }
   if (stages_mask & PX_STAGE_PARTICLES) {
     if (int rc = px_launch_particle_step(c)) return rc;
   }
   if (stages_mask & PX_STAGE_WRITES) return px_launch_writes(c);
+  return PX_OK;
+}
+
+int px_resolve_islands(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_resolve(c);
+}
+
+int px_chunk_step(px_ctx* c) {
+  if (!c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_chunk_frame(c);
+}
+
+int px_read_chunks(px_ctx* c, uint32_t* table, uint32_t* arena, uint32_t arena_cap, uint32_t* n_arena) {
+  if (!c || (arena && arena_cap && !n_arena)) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int rc = px_chunks_alloc(c)) return rc;
+  uint32_t na = 0;
+  PX_HIP_CHECK(c, hipMemcpyAsync(&na, c->d_ctrl + PX_C_ARENA, 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  na = std::min(na, c->lim.max_arena);
+  if (n_arena) *n_arena = na;
+  if (table)
+    PX_HIP_CHECK(c, hipMemcpyAsync(table, c->d_chunks, (size_t)std::min<uint32_t>(c->lim.max_chunks, 511u) * 64,
+                                   hipMemcpyDeviceToHost, c->stream));
+  const uint32_t m = std::min(na, arena_cap);
+  if (arena && m) PX_HIP_CHECK(c, hipMemcpyAsync(arena, c->d_arena, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+int px_read_device_impacts(px_ctx* c, float* out16, uint32_t cap, uint32_t* n) {
+  if (!c || !n || (cap && !out16)) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int rc = px_chunks_alloc(c)) return rc;
+  uint32_t k = 0;
+  PX_HIP_CHECK(c, hipMemcpyAsync(&k, c->d_ctrl + PX_C_IMPACTS, 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  *n = k;
+  const uint32_t m = std::min(std::min(k, cap), c->lim.max_impacts);
+  if (m) {
+    PX_HIP_CHECK(c, hipMemcpyAsync(out16, c->d_dev_impacts, (size_t)m * 64, hipMemcpyDeviceToHost, c->stream));
+    PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
   return PX_OK;
 }
 
@@ -547,6 +606,7 @@ int px_apply_writes(px_ctx* c) {
 int px_run_to_quiescence(px_ctx* c) {
   if (!c) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
+  if (int rc = px_merge_device_impacts(c)) return rc;
   if (c->n_impacts) {
     int rc = px_launch_island_pass(c);
     if (rc) return rc;

@@ -55,8 +55,17 @@ enum : uint32_t {
   EC_JOBCELLS = 0, EC_ALLOCS, EC_UNUSED2, EC_UNUSED3, EC_LABELS, EC_UNUSED5, EC_JOBS, EC_CHUNKS,
   EC_UNUSED8, EC_OVERFLOW, EC_WORDS = 16,
   // voxel-write stage (after the island pass's words)
-  EC_W_N = 16, EC_W_DIRTY, EC_W_WRITTEN, EC_W_DROP
+  EC_W_N = 16, EC_W_DIRTY, EC_W_WRITTEN, EC_W_DROP,
+  // extraction and chunk stages
+  EC_R_CELLS = 24, EC_R_DIRTY, EC_R_BILL, EC_R_DESTROYED, EC_R_SPAWN_ACC, EC_C_LAND, EC_C_CAND, EC_R_ARENA0
 };
+// component flags written by the extraction (beside kFlagAnchor / kFlagHit):
+// kept in the world (over the component budget), crumbles into debris, and
+// the chunk id + 1 in bits 8..16 (resolveChunks, sim.wgsl:862-880)
+constexpr uint32_t kFlagKeep = 4u;
+constexpr uint32_t kFlagCrumb = 8u;
+constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
+constexpr uint32_t kMaxChunkIds = 511;  // chunk id + 1 fits bits 8..16
 // per-block partials (part[b * kPartWords + k])
 enum : uint32_t { PT_CARVE_D = 0, PT_CARVE_B, PT_FLOOD_D, PT_FLOOD_B, PT_DIRTY, PT_FREED };
 // per-block list lengths (bcount[b * kCountWords + k])
@@ -98,6 +107,20 @@ struct EditView {
   uint64_t* wkeys;
   const uint64_t* wsorted;
   uint32_t* wdirty;
+  // extraction and chunks
+  uint32_t* chunks;  // max_chunks x 16 words (sim.wgsl:941-946 layout)
+  uint32_t* arena;   // (key, value | chunk << 16) pairs
+  uint32_t max_chunks, max_arena, comp_budget, max_impacts;
+  uint2* cpair;      // per cell, then per extract cell / arena entry: counts
+  uint2* coff;       // exclusive offsets of cpair
+  uint2* tiles;      // per 1024-element tile: sums, then offsets
+  uint32_t* rlist;   // extract cells in cell order
+  uint32_t* rdirty;  // cells dirtied by the extraction
+  float4* parts;     // particle ring
+  uint32_t pmask, spawn_budget;
+  uint32_t fseed;    // frameSeed(), sim.wgsl:69
+  float vs, dt, grav;
+  float* dev_impacts;  // impacts pushed by falling chunks
 };
 
 // Union-find over voxel keys.  par[x] <= x always (links go to smaller keys),
@@ -926,7 +949,6 @@ __global__ __launch_bounds__(1024) void k_edit_finish(EditView e, uint32_t carve
   e.ctrl[PX_C_ALLOC_N] = e.ec[EC_ALLOCS];
   e.ctrl[PX_C_ALLOCS] = 0u;
   e.ctrl[PX_C_IMPACTS] = 0u;
-  e.ctrl[PX_S_ISLANDS] += e.ec[EC_CHUNKS];
   e.ctrl[PX_S_SEARCHES] += 1u;
 }
 
@@ -1111,6 +1133,737 @@ __global__ void k_zero32(uint32_t* p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
+// ---------------------------------------------------------------------------
+// Ordered scans.  The extraction and chunk stages need list positions in a
+// fixed order (cells, then voxels, then arena entries) without global
+// atomics: per-element (x, y) counts are scanned in tiles of 1024 (one
+// workgroup each), the tile sums in one workgroup, then applied.  Lengths
+// may live on the device (n_dev), so grids are sized by capacity and tiles
+// past the length exit.
+
+constexpr uint32_t kScanTile = 1024;
+
+// Exclusive prefix of (x, y) over the workgroup (1024 threads); returns the
+// block total in *tot.  Block-cooperative.
+PX_DEV uint2 block_excl2(uint2 v, uint2* tot) {
+  __shared__ uint32_t s_x[16], s_y[16];
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t ix = wave_scan(v.x), iy = wave_scan(v.y);
+  if (lane == 63u) {
+    s_x[wave] = ix;
+    s_y[wave] = iy;
+  }
+  __syncthreads();
+  uint32_t px = 0u, py = 0u, tx = 0u, ty = 0u;
+  const uint32_t nw = blockDim.x >> 6;
+  for (uint32_t k = 0; k < nw; k++) {
+    px += k < wave ? s_x[k] : 0u;
+    py += k < wave ? s_y[k] : 0u;
+    tx += s_x[k];
+    ty += s_y[k];
+  }
+  __syncthreads();
+  *tot = make_uint2(tx, ty);
+  return make_uint2(px + ix - v.x, py + iy - v.y);
+}
+
+PX_DEV uint32_t scan_len(const uint32_t* n_dev, uint32_t n_fixed) { return n_dev ? *n_dev : n_fixed; }
+
+__global__ __launch_bounds__(1024) void k_scan2_tiles(const uint2* __restrict__ in, const uint32_t* n_dev,
+                                                      uint32_t n_fixed, uint2* __restrict__ tiles) {
+  const uint32_t n = scan_len(n_dev, n_fixed);
+  const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
+  if (blockIdx.x * kScanTile >= n) return;
+  const uint2 v = i < n ? in[i] : make_uint2(0u, 0u);
+  uint2 tot;
+  block_excl2(v, &tot);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+// tiles -> exclusive offsets (+ base_x / base_y), totals to tot[0..1]
+__global__ __launch_bounds__(1024) void k_scan2_top(uint2* __restrict__ tiles, const uint32_t* n_dev, uint32_t n_fixed,
+                                                    const uint32_t* base_x, const uint32_t* base_y,
+                                                    uint32_t* __restrict__ tot) {
+  const uint32_t n = scan_len(n_dev, n_fixed);
+  const uint32_t nt = (n + kScanTile - 1u) / kScanTile;
+  uint2 carry = make_uint2(base_x ? *base_x : 0u, base_y ? *base_y : 0u);
+  const uint2 base = carry;
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < nt; t0 += blockDim.x) {
+    const uint32_t t = t0 + threadIdx.x;
+    const uint2 v = t < nt ? tiles[t] : make_uint2(0u, 0u);
+    uint2 bt;
+    const uint2 ex = block_excl2(v, &bt);
+    if (t < nt) tiles[t] = make_uint2(carry.x + ex.x, carry.y + ex.y);
+    carry.x += bt.x;
+    carry.y += bt.y;
+  }
+  if (threadIdx.x == 0) {
+    tot[0] = carry.x - base.x;
+    tot[1] = carry.y - base.y;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan2_apply(const uint2* __restrict__ in, const uint32_t* n_dev,
+                                                      uint32_t n_fixed, const uint2* __restrict__ tiles,
+                                                      uint2* __restrict__ out) {
+  const uint32_t n = scan_len(n_dev, n_fixed);
+  const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
+  if (blockIdx.x * kScanTile >= n) return;
+  const uint2 v = i < n ? in[i] : make_uint2(0u, 0u);
+  uint2 tot;
+  const uint2 ex = block_excl2(v, &tot);
+  const uint2 t = tiles[blockIdx.x];
+  if (i < n) out[i] = make_uint2(t.x + ex.x, t.y + ex.y);
+}
+
+// ---------------------------------------------------------------------------
+// Island extraction: resolveChunks / resolveExpand / resolveExtract
+// (sim.wgsl:862-923) with dropRubbleAbove (:274-284) and the crumb debris.
+// Order: unsupported components by label, voxels by cell then local index.
+
+PX_DEV float rnd01(uint32_t seed) { return px_rnd(seed); }
+
+PX_DEV bool mat_fragile(const float* mats, uint32_t v) { return mats[(v & 0xffu) * 20u + 17u] > 0.5f; }
+
+// debrisFlags, sim.wgsl:172-178
+PX_DEV uint32_t debris_flags(const float* mats, uint32_t v, uint32_t seed) {
+  const uint32_t id = v & 0xffu;
+  if (mats[id * 20u + 14u] > 0.0f) return 0u;  // liquids splash and vanish
+  if (rnd01(seed ^ 0x51ed27u) < (mats[id * 20u + 17u] > 0.5f ? 0.55f : 0.75f)) return PX_P_DEPOSIT;
+  if (rnd01(seed ^ 0x9e3779u) < 0.5f) return PX_P_DUST;
+  return 0u;
+}
+
+// spawn() into the ring for record j of a batch of `total`: stored when j is
+// within the budget left at the start of the batch and no later accepted
+// record of the batch maps to the same slot.
+PX_DEV void spawn_at(const EditView& e, uint32_t j, uint32_t accepted, uint32_t cursor, const float pos[3],
+                     const float vel[3], uint32_t value, float life) {
+  if (j >= accepted || accepted - j > e.pmask + 1u) return;
+  const uint32_t slot = (cursor + j) & e.pmask;
+  e.parts[2u * slot] = make_float4(pos[0], pos[1], pos[2], __uint_as_float(value));
+  e.parts[2u * slot + 1u] = make_float4(vel[0], vel[1], vel[2], life);
+}
+
+PX_DEV uint32_t spawn_accepted(const EditView& e, uint32_t total) {
+  const uint32_t spawned = e.ctrl[PX_C_SPAWNED];
+  const uint32_t room = spawned >= e.spawn_budget ? 0u : e.spawn_budget - spawned;
+  return min(total, room);
+}
+
+// decoded voxel (0 outside the world) and whether its cell holds a pool brick
+PX_DEV uint32_t voxel_at(const EditView& e, int32_t x, int32_t y, int32_t z, bool* pool) {
+  *pool = false;
+  if (x < 0 || y < 0 || z < 0 || x >= e.dims[0] || y >= e.dims[1] || z >= e.dims[2]) return 0u;
+  const uint32_t g = e.grid[cell_of_pos(e, x, y, z)];
+  if (g == 0u) return 0u;
+  if ((g & PX_UNIFORM_BIT) != 0u) return g & 0xffffu;
+  *pool = true;
+  const uint32_t idx = (g - 1u) * 512u + (uint32_t)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6));
+  return (__hip_atomic_load(e.pool + (idx >> 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> ((idx & 1u) * 16u)) &
+         0xffffu;
+}
+
+// rubble voxels dropRubbleAbove takes above p: consecutive rubble voxels in
+// pool bricks, at most 12
+PX_DEV uint32_t rubble_above(const EditView& e, int32_t x, int32_t y, int32_t z) {
+  uint32_t n = 0u;
+  for (int32_t d = 1; d <= 12; d++) {
+    bool pool;
+    const uint32_t v = voxel_at(e, x, y + d, z, &pool);
+    if (v == 0u || (v & 0x80u) == 0u || !pool) break;
+    n++;
+  }
+  return n;
+}
+
+// clear a voxel known to be in a pool brick; the first clear of a cell
+// lists it for maint
+PX_DEV void clear_voxel(const EditView& e, int32_t x, int32_t y, int32_t z) {
+  const uint32_t ci = cell_of_pos(e, x, y, z);
+  const uint32_t g = e.grid[ci];
+  const uint32_t idx = (g - 1u) * 512u + (uint32_t)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6));
+  atomicAnd(e.pool + (idx >> 1), ~(0xffffu << ((idx & 1u) * 16u)));
+  const uint32_t bit = 1u << (ci & 31u);
+  if ((atomicOr(&e.dirty_flags[ci >> 5], bit) & bit) == 0u) {
+    const uint32_t k = atomicAdd(&e.ec[EC_R_DIRTY], 1u);
+    if (k < e.ncell) e.rdirty[k] = ci;
+  }
+}
+
+// R1: chunk ids for the big unsupported components in label order (one
+// workgroup).  Labels over the component budget stay (kFlagKeep, S_BUDGET);
+// of the big ones (>= 48 voxels) the smallest labels get the free chunk ids,
+// found by a binary search for the label threshold; all others crumble.
+__global__ __launch_bounds__(1024) void k_resolve_chunks(EditView e) {
+  __shared__ uint32_t s_red[3][16];
+  __shared__ uint32_t s_roots[512];
+  __shared__ uint32_t s_nroots;
+  const uint32_t n = min(e.ec[EC_LABELS], e.out_cap);
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  auto block_sum = [&](uint32_t v, int slot) -> uint32_t {
+    v = wave_sum(v);
+    if (lane == 0) s_red[slot][wave] = v;
+    __syncthreads();
+    uint32_t t = 0u;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); k++) t += s_red[slot][k];
+    __syncthreads();
+    return t;
+  };
+  uint32_t keep = 0u, big = 0u;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t key = e.out[2u * i], cnt = e.out[2u * i + 1u];
+    if (cnt > e.comp_budget) {
+      e.flag[key] |= kFlagKeep;
+      keep++;
+    } else if (cnt >= 48u) {
+      big++;
+    }
+  }
+  keep = block_sum(keep, 0);
+  big = block_sum(big, 1);
+  const uint32_t c0 = e.ctrl[PX_C_CHUNKS];
+  const uint32_t room = c0 < e.max_chunks ? e.max_chunks - c0 : 0u;
+  const uint32_t K = min(room, big);
+  // smallest T with #(big labels <= T) >= K (labels are distinct)
+  uint32_t T = 0u;
+  bool any = K > 0u;
+  if (any && K < big) {
+    uint32_t lo = 0u, hi = 0xFFFFFFFFu;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      uint32_t c = 0u;
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t key = e.out[2u * i], cnt = e.out[2u * i + 1u];
+        if (cnt >= 48u && cnt <= e.comp_budget && key <= mid) c++;
+      }
+      c = block_sum(c, 2);
+      if (c >= K) hi = mid;
+      else lo = mid + 1u;
+    }
+    T = lo;
+  } else if (any) {
+    T = 0xFFFFFFFFu;
+  }
+  if (threadIdx.x == 0) s_nroots = 0u;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t key = e.out[2u * i], cnt = e.out[2u * i + 1u];
+    if (cnt > e.comp_budget) continue;
+    if (any && cnt >= 48u && key <= T) {
+      const uint32_t k = atomicAdd(&s_nroots, 1u);
+      if (k < 512u) s_roots[k] = key;
+    } else {
+      e.flag[key] |= kFlagCrumb;
+    }
+  }
+  __syncthreads();
+  const uint32_t nr = min(s_nroots, 512u);
+  for (uint32_t k = threadIdx.x; k < nr; k += blockDim.x) {
+    const uint32_t key = s_roots[k];
+    uint32_t rank = 0u;
+    for (uint32_t j = 0; j < nr; j++) rank += s_roots[j] < key ? 1u : 0u;
+    const uint32_t id = c0 + rank;
+    uint32_t* b = e.chunks + (size_t)id * 16u;
+    for (int w = 0; w < 16; w++) b[w] = 0u;
+    b[6] = kNoBlock;
+    b[0] = 1u;  // falling
+    e.flag[key] |= (id + 1u) << 8;
+  }
+  if (threadIdx.x == 0) {
+    e.ctrl[PX_C_CHUNKS] = c0 + big;
+    e.ctrl[PX_S_ISLANDS] += K;
+    e.ctrl[PX_S_BUDGET] += keep;
+  }
+}
+
+// R2: cells holding voxels of extracted components; uniform ones need a pool
+// brick first (resolveExpand)
+__global__ __launch_bounds__(64) void k_resolve_mark(EditView e) {
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    const uint32_t n = e.bcount[b * kCountWords + BC_NODES];
+    const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const uint32_t x = nodes[i];
+      const uint32_t f = e.flag[e.par[x]];
+      if ((f & (kFlagAnchor | kFlagKeep)) != 0u) continue;
+      int32_t px, py, pz;
+      key_pos(e, x, px, py, pz);
+      const uint32_t ci = cell_of_pos(e, px, py, pz);
+      e.cpair[ci] = make_uint2(1u, (e.grid[ci] & PX_UNIFORM_BIT) != 0u ? 1u : 0u);
+    }
+  }
+}
+
+// R3: extract list and alloc list in cell order
+__global__ __launch_bounds__(256) void k_resolve_lists(EditView e, const uint32_t* tot) {
+  for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < e.ncell; ci += gridDim.x * blockDim.x) {
+    const uint2 f = e.cpair[ci];
+    if (f.x == 0u) continue;
+    const uint2 o = e.coff[ci];
+    e.rlist[o.x] = ci | (f.y << 31);  // top bit: the cell was uniform (one node, no brick labels)
+    if (f.y && o.y < e.max_alloc && o.y < e.ncell) e.alloc_list[o.y] = ci;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    e.ec[EC_R_CELLS] = tot[0];
+    e.ec[EC_ALLOCS] = min(min(tot[1], e.max_alloc), e.ncell);
+  }
+}
+
+// Class of voxel li of extract cell ci: 0 stays, 1 + chunk id = to the
+// chunk, 0xFFFFFFFF = crumbles.  The cell must hold a pool brick now (alloc
+// may have failed for a uniform cell: its voxels cannot be taken).
+PX_DEV uint32_t extract_class(const EditView& e, uint32_t g, bool was_uniform, uint32_t okey, uint32_t li,
+                              uint32_t v) {
+  if (v == 0u) return 0u;
+  uint32_t node;
+  if (was_uniform) {
+    node = okey;
+  } else {
+    const uint32_t r = e.lab[(size_t)(g - 1u) * 512u + li];
+    if (r >= 512u) return 0u;
+    node = okey + loff(e, r);
+  }
+  const uint32_t f = e.flag[e.par[node]];
+  if ((f & (kFlagAnchor | kFlagKeep)) != 0u) return 0u;
+  const uint32_t cid = (f >> 8) & 0x1ffu;
+  if (cid != 0u && (f & kFlagCrumb) == 0u) return cid;
+  return 0xFFFFFFFFu;
+}
+
+// Per extract cell (one wavefront; lane = row y|z<<3, 8 voxels in x):
+// PASS 0 counts the chunk voxels; PASS 1 counts the debris records (rubble
+// above every taken voxel, plus the crumbs, including chunk voxels past the
+// arena); PASS 2 takes the voxels, writes the arena and the records.
+template <int PASS>
+__global__ __launch_bounds__(64) void k_extract(EditView e, const uint32_t* spawn_tot) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t n = e.ec[EC_R_CELLS];
+  uint32_t bill = 0u, destroyed = 0u;
+  uint32_t accepted = 0u, cursor = 0u;
+  if (PASS == 2) {
+    accepted = spawn_accepted(e, spawn_tot[1]);
+    cursor = e.ctrl[PX_C_PCURSOR];
+  }
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t ent = e.rlist[i];
+    const uint32_t ci = ent & 0x7FFFFFFFu;
+    const bool was_uniform = (ent >> 31) != 0u;
+    const uint32_t g = e.grid[ci];
+    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) {  // no brick could be given: nothing is taken
+      if (PASS < 2 && lane == 0) {
+        uint2 c = e.cpair[i];
+        if (PASS == 0) c = make_uint2(0u, 0u);
+        else c.y = 0u;
+        e.cpair[i] = c;
+      }
+      continue;
+    }
+    int32_t bx, by, bz;
+    cell_coord(e, ci, bx, by, bz);
+    const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+    const uint4 w4 = reinterpret_cast<const uint4*>(e.pool + (size_t)(g - 1u) * 256u)[lane];
+    const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+    uint32_t cls[8], val[8];
+    uint32_t na = 0u;
+    for (int x = 0; x < 8; x++) {
+      val[x] = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+      cls[x] = extract_class(e, g, was_uniform, okey, lane * 8u + (uint32_t)x, val[x]);
+      if (cls[x] != 0u && cls[x] != 0xFFFFFFFFu) na++;
+    }
+    if (PASS == 0) {
+      const uint32_t t = wave_sum(na);
+      if (lane == 0) e.cpair[i] = make_uint2(t, 0u);
+      continue;
+    }
+    // arena slots of this lane's chunk voxels: cell offset + lanes before + in-lane order
+    const uint32_t abase = e.coff[i].x + wave_scan(na) - na;
+    const int32_t ry = (int32_t)(lane & 7u), rz = (int32_t)(lane >> 3);
+    uint32_t ns = 0u, rub[8];
+    uint32_t a = abase;
+    for (int x = 0; x < 8; x++) {
+      rub[x] = 0u;
+      if (cls[x] == 0u) continue;
+      if (cls[x] != 0xFFFFFFFFu) {
+        if (a >= e.max_arena) cls[x] = 0xFFFFFFFFu;  // the arena is full: crumbles
+        a++;
+      }
+      rub[x] = rubble_above(e, bx * 8 + x, by * 8 + ry, bz * 8 + rz);
+      ns += rub[x] + (cls[x] == 0xFFFFFFFFu ? 1u : 0u);
+    }
+    if (PASS == 1) {
+      const uint32_t t = wave_sum(ns);
+      if (lane == 0) e.cpair[i].y = t;
+      continue;
+    }
+    // PASS 2
+    uint32_t j = e.coff[i].y + wave_scan(ns) - ns;
+    a = abase;
+    for (int x = 0; x < 8; x++) {
+      if (cls[x] == 0u) continue;
+      const int32_t px = bx * 8 + x, py = by * 8 + ry, pz = bz * 8 + rz;
+      const uint32_t v = val[x];
+      clear_voxel(e, px, py, pz);
+      // dropRubbleAbove: bottom to top, before the voxel's own record
+      for (uint32_t d = 1; d <= rub[x]; d++) {
+        const int32_t qy = py + (int32_t)d;
+        bool pool;
+        const uint32_t t = voxel_at(e, px, qy, pz, &pool);
+        clear_voxel(e, px, qy, pz);
+        const uint32_t seed = vkey(e, px, qy, pz) ^ e.fseed;
+        const float pos[3] = {(float)px + 0.5f, (float)qy + 0.5f, (float)pz + 0.5f};
+        const float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
+        const float m[3] = {0.4f, 0.0f, 0.4f};
+        float vel[3];
+        for (int k = 0; k < 3; k++) vel[k] = ((r[k] - 0.5f) * m[k]) / e.vs;
+        spawn_at(e, j, accepted, cursor, pos, vel, t | PX_P_DEPOSIT, 6.0f + 2.0f * rnd01(seed + 5u));
+        j++;
+      }
+      if (cls[x] != 0xFFFFFFFFu) {
+        e.arena[2u * a] = vkey(e, px, py, pz);
+        e.arena[2u * a + 1u] = v | ((cls[x] - 1u) << 16);
+        a++;
+        continue;
+      }
+      // crumbs: straight to debris
+      bill += price_dimes(e.mats, v);
+      destroyed++;
+      const uint32_t seed = vkey(e, px, py, pz) ^ e.fseed;
+      const float pos[3] = {(float)px + 0.5f, (float)py + 0.5f, (float)pz + 0.5f};
+      const float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
+      const float off[3] = {0.5f, 0.0f, 0.5f}, m[3] = {0.6f, 0.3f, 0.6f};
+      float vel[3];
+      for (int k = 0; k < 3; k++) vel[k] = ((r[k] - off[k]) * m[k]) / e.vs;
+      spawn_at(e, j, accepted, cursor, pos, vel, v | debris_flags(e.mats, v, seed), 4.0f + 3.0f * rnd01(seed + 9u));
+      j++;
+    }
+  }
+  if (PASS == 2) {
+    bill = wave_sum(bill);
+    destroyed = wave_sum(destroyed);
+    if (lane == 0 && (bill | destroyed)) {
+      atomicAdd(&e.ec[EC_R_BILL], bill);
+      atomicAdd(&e.ec[EC_R_DESTROYED], destroyed);
+    }
+  }
+}
+
+// chunk voxel counts and fragile flags from the arena entries this pass added
+// (per-workgroup tallies in LDS, one global add per chunk per workgroup)
+__global__ __launch_bounds__(256) void k_arena_tally(EditView e, const uint32_t* tot) {
+  __shared__ uint32_t s_cnt[512], s_frag[512];
+  for (uint32_t k = threadIdx.x; k < 512u; k += blockDim.x) s_cnt[k] = s_frag[k] = 0u;
+  __syncthreads();
+  const uint32_t a0 = e.ec[EC_R_ARENA0], a1 = min(a0 + tot[0], e.max_arena);
+  for (uint32_t a = a0 + blockIdx.x * blockDim.x + threadIdx.x; a < a1; a += gridDim.x * blockDim.x) {
+    const uint32_t vv = e.arena[2u * a + 1u];
+    const uint32_t c = vv >> 16;
+    atomicAdd(&s_cnt[c], 1u);
+    if (mat_fragile(e.mats, vv & 0xffffu)) s_frag[c] = 1u;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < 512u; k += blockDim.x) {
+    if (s_cnt[k]) atomicAdd(e.chunks + (size_t)k * 16u + 5u, s_cnt[k]);
+    if (s_frag[k]) e.chunks[(size_t)k * 16u + 4u] = 1u;
+  }
+}
+
+__global__ void k_resolve_begin(EditView e) {
+  if (threadIdx.x != 0) return;
+  e.ec[EC_R_DIRTY] = 0u;
+  e.ec[EC_R_BILL] = 0u;
+  e.ec[EC_R_DESTROYED] = 0u;
+  e.ec[EC_R_ARENA0] = e.ctrl[PX_C_ARENA];  // first arena entry of this pass
+}
+
+__global__ void k_resolve_finish(EditView e, const uint32_t* arena_tot, const uint32_t* spawn_tot) {
+  if (threadIdx.x != 0) return;
+  const uint32_t total = spawn_tot[1];
+  const uint32_t acc = spawn_accepted(e, total);
+  e.ctrl[PX_C_SPAWNED] += total;
+  e.ctrl[PX_C_PCURSOR] += acc;
+  e.ctrl[PX_C_ARENA] += arena_tot[0];
+  e.ctrl[PX_S_BILL] += e.ec[EC_R_BILL];
+  e.ctrl[PX_S_DESTROYED] += e.ec[EC_R_DESTROYED];
+  e.ctrl[PX_C_DIRTY] += min(e.ec[EC_R_DIRTY], e.ncell);
+}
+
+// maint of the cells the extraction dirtied and of the cells alloc expanded
+__global__ __launch_bounds__(64) void k_resolve_maint(EditView e) {
+  const uint32_t nd = min(e.ec[EC_R_DIRTY], e.ncell);
+  const uint32_t na = min(e.ec[EC_ALLOCS], e.ncell);
+  for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) maint_cell(e, e.rdirty[i]);
+  for (uint32_t i = blockIdx.x; i < na; i += gridDim.x) maint_cell(e, e.alloc_list[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Falling chunks: chunkArgs / chunkBegin / chunkCollide / chunkStep /
+// chunkLand (sim.wgsl:951-1075).  Arena order decides which voxels of a chunk
+// push the (at most 6) impacts of a break-through, the order of the writes
+// and of the shatter debris.
+
+PX_DEV float u2f(uint32_t u) { return __uint_as_float(u); }
+PX_DEV uint32_t f2bits(float f) { return __float_as_uint(f); }
+// WGSL u32(f32): truncation toward zero, saturating, NaN -> 0
+PX_DEV uint32_t px_f2u(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967040.0f) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+// chunkArgs + chunkBegin, one thread per chunk (one workgroup)
+__global__ __launch_bounds__(512) void k_chunk_begin(EditView e) {
+  __shared__ uint32_t s_act[8];
+  const uint32_t c = threadIdx.x;
+  const uint32_t n0 = min(e.ctrl[PX_C_CHUNKS], e.max_chunks);
+  uint32_t* b = e.chunks + (size_t)c * 16u;
+  uint32_t act = 0u;
+  if (c < n0) {
+    uint32_t st = b[0];
+    if (st == 2u) b[0] = 0u;
+    if (st == 1u && (b[5] == 0u || b[2] > (uint32_t)e.dims[1])) {
+      b[0] = 0u;
+      st = 0u;
+    }
+    act = st == 1u ? 1u : 0u;
+  }
+  act = wave_sum(act);
+  if (__lane_id() == 0) s_act[threadIdx.x >> 6] = act;
+  __syncthreads();
+  uint32_t active = 0u;
+  for (uint32_t k = 0; k < (blockDim.x >> 6); k++) active += s_act[k];
+  if (c == 0) {
+    if (active == 0u) {
+      e.ctrl[PX_C_CHUNKS] = 0u;
+      e.ctrl[PX_C_ARENA] = 0u;
+    }
+    e.ctrl[PX_S_FALLING] = active;
+  }
+  const uint32_t n = active == 0u ? 0u : n0;
+  if (c == 0) e.ec[EC_C_LAND] = active == 0u ? 0u : min(e.ctrl[PX_C_ARENA], e.max_arena);
+  if (c < n && b[0] == 1u) {
+    const float vy = fminf(u2f(b[1]) + e.grav * e.dt, 25.0f / e.vs);
+    const float travel = vy * e.dt + u2f(b[3]);
+    const uint32_t steps = min(px_f2u(floorf(travel)), 48u);
+    b[1] = f2bits(vy);
+    b[3] = f2bits(travel - floorf(travel));
+    b[8] = steps;
+    b[6] = kNoBlock;
+    b[7] = 0u;
+  }
+}
+
+PX_DEV uint32_t arena_len(const EditView& e) { return min(e.ctrl[PX_C_ARENA], e.max_arena); }
+
+// chunkCollide: the first solid voxel below each falling voxel within the
+// frame's steps.  blockedAt is a min (any order); a break-through candidate
+// (fragile voxel hit at speed > 1.5) is counted in word 7 and flagged
+// (cpair[j].x = 1) for the impact selection.
+__global__ __launch_bounds__(256) void k_chunk_collide(EditView e) {
+  const uint32_t na = arena_len(e);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < na; j += gridDim.x * blockDim.x) {
+    uint32_t cand = 0u;
+    const uint32_t vv = e.arena[2u * j + 1u], v = vv & 0xffffu;
+    if (v != 0u) {
+      uint32_t* b = e.chunks + (size_t)(vv >> 16) * 16u;
+      const uint32_t steps = b[0] == 1u ? b[8] : 0u;
+      if (steps != 0u) {
+        int32_t px, py, pz;
+        key_pos(e, e.arena[2u * j], px, py, pz);
+        py -= (int32_t)b[2];
+        const float speed = u2f(b[1]) * e.vs;
+        for (uint32_t s = 1; s <= steps; s++) {
+          const int32_t qy = py - (int32_t)s;
+          if (qy < 0) {
+            atomicMin(b + 6, s);
+            break;
+          }
+          bool pool;
+          const uint32_t w = voxel_at(e, px, qy, pz, &pool);
+          if (w != 0u) {
+            if (mat_fragile(e.mats, w) && speed > 1.5f) {
+              atomicAdd(b + 7, 1u);
+              cand = 1u;
+            }
+            atomicMin(b + 6, s);
+            break;
+          }
+        }
+      }
+    }
+    e.cpair[j] = make_uint2(cand, 0u);
+  }
+}
+
+// One round of the impact selection: the smallest candidate index per chunk
+// (a wavefront reduces its lanes per chunk first), marked chosen (2).
+__global__ __launch_bounds__(256) void k_cand_min(EditView e, uint32_t* best) {
+  const uint32_t na = arena_len(e);
+  const uint32_t lane = __lane_id();
+  for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < na; j0 += gridDim.x * blockDim.x) {
+    const uint32_t j = j0 + threadIdx.x;
+    uint32_t c = PX_NONE;
+    if (j < na && e.cpair[j].x == 1u) c = e.arena[2u * j + 1u] >> 16;
+    for (;;) {
+      const uint64_t act = __ballot(c != PX_NONE);
+      if (act == 0ull) break;
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+      const uint32_t c0 = __shfl(c, (int)leader);
+      // lanes are in increasing j: the first lane of the chunk holds its smallest index
+      if (lane == leader) atomicMin(best + c0, j);
+      if (c == c0) c = PX_NONE;
+    }
+  }
+}
+
+__global__ void k_cand_pick(EditView e, uint32_t* best) {
+  const uint32_t c = threadIdx.x;
+  if (c >= e.max_chunks) return;
+  const uint32_t j = best[c];
+  if (j != PX_NONE) {
+    e.cpair[j].x = 2u;
+    best[c] = PX_NONE;
+  }
+}
+
+// chosen candidates -> (1, 0) for the ordered compaction of the impacts
+__global__ __launch_bounds__(256) void k_cand_flags(EditView e) {
+  const uint32_t na = arena_len(e);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < na; j += gridDim.x * blockDim.x)
+    e.cpair[j] = make_uint2(e.cpair[j].x == 2u ? 1u : 0u, 0u);
+}
+
+// pushImpact (sim.wgsl:203-211) for the chosen candidates in arena order
+__global__ __launch_bounds__(256) void k_cand_impacts(EditView e, const uint32_t* tot) {
+  const uint32_t na = arena_len(e);
+  const uint32_t k0 = e.ctrl[PX_C_IMPACTS];
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < na; j += gridDim.x * blockDim.x) {
+    if (e.cpair[j].x == 0u) continue;
+    const uint32_t k = k0 + e.coff[j].x;
+    if (k >= e.max_impacts) continue;
+    const uint32_t vv = e.arena[2u * j + 1u];
+    const uint32_t* b = e.chunks + (size_t)(vv >> 16) * 16u;
+    int32_t px, py, pz;
+    key_pos(e, e.arena[2u * j], px, py, pz);
+    py -= (int32_t)b[2];
+    // the candidate's hit is at blockedAt of its own walk: find it again
+    int32_t qy = py;
+    for (uint32_t s = 1; s <= b[8]; s++) {
+      bool pool;
+      qy = py - (int32_t)s;
+      if (qy >= 0 && voxel_at(e, px, qy, pz, &pool) != 0u) break;
+    }
+    const float speed = u2f(b[1]) * e.vs;
+    float* im = e.dev_impacts + (size_t)k * 16u;
+    im[0] = (float)px + 0.5f;
+    im[1] = (float)qy + 0.5f;
+    im[2] = (float)pz + 0.5f;
+    im[3] = 0.02f / e.vs;
+    im[4] = 0.12f / e.vs;
+    im[5] = speed * 0.4f / e.vs;
+    im[6] = 0.0f; im[7] = 1.0f; im[8] = 0.0f;
+    im[9] = 0.0f; im[10] = -1.0f; im[11] = 0.0f;
+    im[12] = 0.0f; im[13] = 0.0f; im[14] = 0.0f; im[15] = 0.0f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) e.ec[EC_C_CAND] = tot[0];
+}
+
+// chunkStep, one thread per chunk
+__global__ __launch_bounds__(512) void k_chunk_step(EditView e) {
+  const uint32_t c = threadIdx.x;
+  if (threadIdx.x == 0) e.ctrl[PX_C_IMPACTS] += e.ec[EC_C_CAND];
+  if (c >= min(e.ctrl[PX_C_CHUNKS], e.max_chunks)) return;
+  uint32_t* b = e.chunks + (size_t)c * 16u;
+  if (b[0] != 1u) return;
+  const uint32_t steps = b[8];
+  if (steps == 0u) return;
+  const uint32_t blocked = b[6];
+  uint32_t drop = steps;
+  if (blocked <= steps) {
+    drop = blocked - 1u;
+    if (b[7] > 0u) {
+      b[1] = f2bits(u2f(b[1]) * 0.6f);  // broke through something: keep falling
+    } else {
+      b[9] = f2bits(u2f(b[1]) * e.vs);
+      b[0] = 2u;
+    }
+  }
+  b[2] += drop;
+}
+
+// chunkLand: per arena entry of a landed chunk, (write, shatter) counts
+__global__ __launch_bounds__(256) void k_land_count(EditView e) {
+  const uint32_t na = arena_len(e);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < na; j += gridDim.x * blockDim.x) {
+    uint2 f = make_uint2(0u, 0u);
+    const uint32_t vv = e.arena[2u * j + 1u], v = vv & 0xffffu;
+    if (v != 0u) {
+      const uint32_t* b = e.chunks + (size_t)(vv >> 16) * 16u;
+      if (b[0] == 2u) {
+        const bool shatter = b[4] != 0u && u2f(b[9]) > 2.0f && mat_fragile(e.mats, v);
+        f = shatter ? make_uint2(0u, 1u) : make_uint2(1u, 0u);
+      }
+    }
+    e.cpair[j] = f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_land_emit(EditView e, const uint32_t* tot) {
+  const uint32_t na = arena_len(e);
+  const uint32_t w0 = e.ctrl[PX_C_WRITES];
+  const uint32_t accepted = spawn_accepted(e, tot[1]);
+  const uint32_t cursor = e.ctrl[PX_C_PCURSOR];
+  uint32_t bill = 0u, destroyed = 0u;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < na; j += gridDim.x * blockDim.x) {
+    const uint2 f = e.cpair[j];
+    if ((f.x | f.y) == 0u) continue;
+    const uint32_t vv = e.arena[2u * j + 1u], v = vv & 0xffffu;
+    const uint32_t key = e.arena[2u * j];
+    const uint32_t* b = e.chunks + (size_t)(vv >> 16) * 16u;
+    e.arena[2u * j + 1u] = 0u;
+    int32_t px, py, pz;
+    key_pos(e, key, px, py, pz);
+    py -= (int32_t)b[2];
+    const uint2 o = e.coff[j];
+    if (f.y) {
+      bill += price_dimes(e.mats, v);
+      destroyed++;
+      const float speed = u2f(b[9]);
+      const uint32_t seed = key ^ e.fseed;
+      const float r[3] = {(rnd01(seed) - 0.5f) * 0.9f, rnd01(seed + 1u) * 0.5f, (rnd01(seed + 2u) - 0.5f) * 0.9f};
+      float vel[3];
+      for (int k = 0; k < 3; k++) vel[k] = (r[k] * speed) / e.vs;
+      const float pos[3] = {(float)px + 0.5f, (float)py + 0.5f, (float)pz + 0.5f};
+      spawn_at(e, o.y, accepted, cursor, pos, vel, v | debris_flags(e.mats, v, seed), 5.0f + 3.0f * rnd01(seed + 3u));
+      continue;
+    }
+    const uint32_t k = w0 + o.x;  // pushWrite, sim.wgsl:186-192
+    if (k < e.max_writes) {
+      e.wlist[2u * k] = vkey(e, px, py, pz);
+      e.wlist[2u * k + 1u] = v;
+    }
+  }
+  bill = wave_sum(bill);
+  destroyed = wave_sum(destroyed);
+  if (__lane_id() == 0 && (bill | destroyed)) {
+    atomicAdd(&e.ec[EC_R_BILL], bill);
+    atomicAdd(&e.ec[EC_R_DESTROYED], destroyed);
+  }
+}
+
+__global__ void k_land_finish(EditView e, const uint32_t* tot) {
+  if (threadIdx.x != 0) return;
+  const uint32_t acc = spawn_accepted(e, tot[1]);
+  e.ctrl[PX_C_WRITES] += tot[0];
+  e.ctrl[PX_C_SPAWNED] += tot[1];
+  e.ctrl[PX_C_PCURSOR] += acc;
+  e.ctrl[PX_S_BILL] += e.ec[EC_R_BILL];
+  e.ctrl[PX_S_DESTROYED] += e.ec[EC_R_DESTROYED];
+  e.ec[EC_R_BILL] = 0u;
+  e.ec[EC_R_DESTROYED] = 0u;
+}
+
 EditView make_view(px_ctx* c) {
   EditView e;
   e.grid = c->d_grid;
@@ -1153,6 +1906,31 @@ EditView make_view(px_ctx* c) {
   e.wkeys = c->d_wkeys;
   e.wsorted = c->d_wsorted;
   e.wdirty = c->d_wdirty;
+  e.chunks = c->d_chunks;
+  e.arena = c->d_arena;
+  e.max_chunks = std::min<uint32_t>(c->lim.max_chunks, kMaxChunkIds);
+  e.max_arena = c->lim.max_arena;
+  e.comp_budget = c->lim.component_budget;
+  e.max_impacts = c->lim.max_impacts;
+  e.cpair = c->d_cpair;
+  e.coff = c->d_coff;
+  e.tiles = c->d_tiles;
+  e.rlist = c->d_rlist;
+  e.rdirty = c->d_rdirty;
+  e.parts = c->d_particles;
+  e.pmask = c->max_particles ? c->max_particles - 1u : 0u;
+  e.spawn_budget = c->lim.spawn_budget;
+  e.vs = c->h_uniforms[3];
+  e.dt = c->h_uniforms[24];
+  e.grav = c->h_uniforms[25];
+  {
+    // frameSeed() = u32(U.time.y) * 0x9E3779B9 (sim.wgsl:69)
+    const float fy = c->h_uniforms[29];
+    uint32_t f = 0u;
+    if (fy > 0.0f) f = fy >= 4294967040.0f ? 0xFFFFFFFFu : (uint32_t)fy;
+    e.fseed = f * 0x9E3779B9u;
+  }
+  e.dev_impacts = c->d_dev_impacts;
   return e;
 }
 
@@ -1356,6 +2134,145 @@ extern "C" int px_push_writes(px_ctx* c, const uint32_t* pairs, uint32_t n) {
                        reinterpret_cast<const uint32_t*>(c->d_wsorted), m);
   hipLaunchKernelGGL(k_push_commit, dim3(1), dim3(64), 0, c->stream, e, n);
   PX_HIP_CHECK(c, hipGetLastError());
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// island extraction and falling chunks
+
+int px_chunks_alloc(px_ctx* c) {
+  if (c->d_chunks) return PX_OK;
+  if (int rc = px_writes_alloc(c)) return rc;
+  if (!c->d_particles) {
+    // the debris the extraction and the landings spawn needs the ring
+    if (int rc = px_particles_init(c, 0)) return rc;
+  }
+  const size_t nch = kMaxChunkIds + 1u, na = c->lim.max_arena ? c->lim.max_arena : 1u;
+  const size_t cap = std::max<size_t>(c->ncell, na);
+  const size_t tiles = (cap + kScanTile - 1u) / kScanTile;
+  PX_HIP_CHECK(c, hipMalloc(&c->d_chunks, nch * 16 * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_arena, na * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_cpair, cap * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_coff, cap * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_tiles, tiles * 8));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_rlist, (size_t)c->ncell * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_rdirty, (size_t)c->ncell * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_rtot, 64));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_best, nch * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_dev_impacts, (size_t)std::max<uint32_t>(c->lim.max_impacts, 1u) * 64));
+  PX_HIP_CHECK(c, hipMemsetAsync(c->d_chunks, 0, nch * 16 * 4, c->stream));
+  PX_HIP_CHECK(c, hipMemsetAsync(c->d_arena, 0, na * 8, c->stream));
+  PX_HIP_CHECK(c, hipMemsetAsync(c->d_best, 0xFF, nch * 4, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  c->scan_cap = cap;
+  return PX_OK;
+}
+
+void px_chunks_free(px_ctx* c) {
+  void* bufs[] = {c->d_chunks, c->d_arena, c->d_cpair, c->d_coff, c->d_tiles, c->d_rlist,
+                  c->d_rdirty, c->d_rtot, c->d_best, c->d_dev_impacts};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  c->d_chunks = c->d_arena = c->d_rlist = c->d_rdirty = c->d_rtot = c->d_best = nullptr;
+  c->d_cpair = c->d_coff = c->d_tiles = nullptr;
+  c->d_dev_impacts = nullptr;
+}
+
+// exclusive (x, y) offsets of e.cpair[0..n) into e.coff, totals into tot[0..1]
+static void scan2(px_ctx* c, const EditView& e, const uint32_t* n_dev, uint32_t n_fixed, const uint32_t* base_x,
+                  const uint32_t* base_y, uint32_t* tot) {
+  const uint32_t ntiles = (uint32_t)((c->scan_cap + kScanTile - 1u) / kScanTile);
+  hipLaunchKernelGGL(k_scan2_tiles, dim3(ntiles), dim3(kScanTile), 0, c->stream, e.cpair, n_dev, n_fixed, e.tiles);
+  hipLaunchKernelGGL(k_scan2_top, dim3(1), dim3(1024), 0, c->stream, e.tiles, n_dev, n_fixed, base_x, base_y, tot);
+  hipLaunchKernelGGL(k_scan2_apply, dim3(ntiles), dim3(kScanTile), 0, c->stream, e.cpair, n_dev, n_fixed, e.tiles,
+                     e.coff);
+}
+
+int px_launch_resolve(px_ctx* c) {
+  if (!c->labels_valid) return px_fail(c, PX_ERR_STATE, "px_resolve_islands", "no island pass to resolve");
+  if (int rc = px_chunks_alloc(c)) return rc;
+  PxProfScope prof(c, PX_PROF_CHUNKS);
+  EditView e = make_view(c);
+  hipStream_t s = c->stream;
+  const dim3 G(kEditBlocks), W(64);
+  uint32_t* t1 = c->d_rtot;       // cells: (extract, expand) totals
+  uint32_t* t2 = c->d_rtot + 4;   // (arena, -) totals
+  uint32_t* t3 = c->d_rtot + 8;   // (arena, spawn) totals
+  hipLaunchKernelGGL(k_resolve_begin, dim3(1), dim3(64), 0, s, e);
+  hipLaunchKernelGGL(k_zero32, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint32_t*>(c->d_cpair),
+                     (size_t)c->ncell * 2);
+  hipLaunchKernelGGL(k_resolve_chunks, dim3(1), dim3(1024), 0, s, e);
+  hipLaunchKernelGGL(k_resolve_mark, G, W, 0, s, e);
+  scan2(c, e, nullptr, c->ncell, nullptr, nullptr, t1);
+  hipLaunchKernelGGL(k_resolve_lists, dim3(1024), dim3(256), 0, s, e, t1);
+  hipLaunchKernelGGL(k_alloc, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_alloc_commit, dim3(1), dim3(64), 0, s, e);
+  const uint32_t* ncells = c->d_edit_ctr + EC_R_CELLS;
+  const uint32_t* arena0 = c->d_ctrl + PX_C_ARENA;
+  hipLaunchKernelGGL(k_extract<0>, G, W, 0, s, e, t3);
+  scan2(c, e, ncells, 0, arena0, nullptr, t2);
+  hipLaunchKernelGGL(k_extract<1>, G, W, 0, s, e, t3);
+  scan2(c, e, ncells, 0, arena0, nullptr, t3);
+  hipLaunchKernelGGL(k_extract<2>, G, W, 0, s, e, t3);
+  hipLaunchKernelGGL(k_arena_tally, dim3(256), dim3(256), 0, s, e, t2);
+  hipLaunchKernelGGL(k_resolve_finish, dim3(1), dim3(64), 0, s, e, t2, t3);
+  hipLaunchKernelGGL(k_resolve_maint, G, W, 0, s, e);
+  PX_HIP_CHECK(c, hipGetLastError());
+  c->labels_valid = false;
+  c->masks_dirty = true;
+  return px_refresh_masks(c);
+}
+
+int px_launch_chunk_frame(px_ctx* c) {
+  if (int rc = px_chunks_alloc(c)) return rc;
+  PxProfScope prof(c, PX_PROF_CHUNKS);
+  EditView e = make_view(c);
+  hipStream_t s = c->stream;
+  const uint32_t ga = (uint32_t)std::min<size_t>((std::max<uint32_t>(c->lim.max_arena, 1u) + 255u) / 256u, 2048u);
+  const dim3 GA(ga), WA(256);
+  uint32_t* t = c->d_rtot + 12;
+  const uint32_t* na = c->d_edit_ctr + EC_C_LAND;
+  hipLaunchKernelGGL(k_chunk_begin, dim3(1), dim3(512), 0, s, e);
+  hipLaunchKernelGGL(k_chunk_collide, GA, WA, 0, s, e);
+  for (int r = 0; r < 6; r++) {
+    hipLaunchKernelGGL(k_cand_min, GA, WA, 0, s, e, c->d_best);
+    hipLaunchKernelGGL(k_cand_pick, dim3(1), dim3(512), 0, s, e, c->d_best);
+  }
+  hipLaunchKernelGGL(k_cand_flags, GA, WA, 0, s, e);
+  scan2(c, e, na, 0, nullptr, nullptr, t);
+  hipLaunchKernelGGL(k_cand_impacts, GA, WA, 0, s, e, t);
+  hipLaunchKernelGGL(k_chunk_step, dim3(1), dim3(512), 0, s, e);
+  hipLaunchKernelGGL(k_land_count, GA, WA, 0, s, e);
+  scan2(c, e, na, 0, nullptr, nullptr, t);
+  hipLaunchKernelGGL(k_land_emit, GA, WA, 0, s, e, t);
+  hipLaunchKernelGGL(k_land_finish, dim3(1), dim3(64), 0, s, e, t);
+  PX_HIP_CHECK(c, hipGetLastError());
+  c->dev_impacts_pending = true;
+  return PX_OK;
+}
+
+// Impacts pushed on the device by falling chunks join the host-side queue
+// of the next island pass (a short blocking read).
+int px_merge_device_impacts(px_ctx* c) {
+  if (!c->dev_impacts_pending || !c->d_dev_impacts) return PX_OK;
+  c->dev_impacts_pending = false;
+  uint32_t n = 0;
+  PX_HIP_CHECK(c, hipMemcpyAsync(&n, c->d_ctrl + PX_C_IMPACTS, 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  const uint32_t maxi = std::min<uint32_t>(c->lim.max_impacts, 64u);
+  n = std::min(n, maxi);
+  const uint32_t take = std::min<uint32_t>(n, maxi - std::min(maxi, c->n_impacts));
+  if (take) {
+    PX_HIP_CHECK(c, hipMemcpyAsync(c->h_impacts + (size_t)c->n_impacts * 16, c->d_dev_impacts, (size_t)take * 64,
+                                   hipMemcpyDeviceToHost, c->stream));
+    PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+    PX_HIP_CHECK(c, hipMemcpyAsync(c->d_impacts + (size_t)c->n_impacts * 16, c->h_impacts + (size_t)c->n_impacts * 16,
+                                   (size_t)take * 64, hipMemcpyHostToDevice, c->stream));
+    c->n_impacts += take;
+  }
+  const uint32_t zero = 0u;
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_ctrl + PX_C_IMPACTS, &zero, 4, hipMemcpyHostToDevice, c->stream));
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   return PX_OK;
 }

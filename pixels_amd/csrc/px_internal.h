@@ -112,6 +112,19 @@ struct px_ctx {
   uint32_t* d_wdirty = nullptr;     // cells the puts dirtied
   void* d_sort_tmp = nullptr;       // radix-sort temporary storage
   size_t sort_tmp_bytes = 0;
+  // island extraction and falling chunks
+  uint32_t* d_chunks = nullptr;       // max_chunks x 16 words
+  uint32_t* d_arena = nullptr;        // (key, value | chunk << 16) pairs, max_arena of them
+  uint2* d_cpair = nullptr;           // per cell / extract cell / arena entry (x, y) counts
+  uint2* d_coff = nullptr;            // their exclusive offsets
+  uint2* d_tiles = nullptr;           // per 1024-element tile sums / offsets
+  uint32_t* d_rlist = nullptr;        // extract cells
+  uint32_t* d_rdirty = nullptr;       // cells the extraction dirtied
+  uint32_t* d_rtot = nullptr;         // scan totals
+  uint32_t* d_best = nullptr;         // per chunk: smallest candidate index of a selection round
+  float* d_dev_impacts = nullptr;     // impacts pushed by falling chunks (MAX_IMPACTS x 16)
+  bool dev_impacts_pending = false;   // a chunk frame ran since the last island pass
+  size_t scan_cap = 0;                // elements d_cpair / d_coff hold
   // debris particles
   uint32_t max_particles = 0;
   float4* d_particles = nullptr;    // ring: 2 float4 per particle (pos, value) (vel, life)
@@ -175,6 +188,12 @@ int px_writes_alloc(px_ctx* c);
 void px_writes_free(px_ctx* c);
 int px_launch_writes(px_ctx* c);
 int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, uint32_t end_bit);
+// island extraction and falling chunks (px_edit.hip)
+int px_chunks_alloc(px_ctx* c);
+void px_chunks_free(px_ctx* c);
+int px_launch_resolve(px_ctx* c);
+int px_launch_chunk_frame(px_ctx* c);
+int px_merge_device_impacts(px_ctx* c);
 // debris particles (px_particles.hip)
 void px_particles_free(px_ctx* c);
 int px_launch_frame_begin(px_ctx* c);
