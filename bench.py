@@ -258,7 +258,59 @@ def island_leg(args):
         res["bit_exact_vs_oracle"] = bool(n == on and np.array_equal(keys, okeys) and np.array_equal(cnt, ocnt)
                                           and np.array_equal(vox, ovox.astype(np.uint32)))
         log("island pass bit-exact vs oracle: %s" % res["bit_exact_vs_oracle"])
+    # extraction and falling chunks continue from the snapshot (after the comparison above)
+    res["extraction"] = extraction_leg(args, scene, gw, sim, impacts)
     gw.close()
+    return res
+
+
+def extraction_leg(args, scene, gw, sim, impacts):
+    """After the island pass: the extraction of every unsupported piece
+    (chunks into the arena, crumbs and tumbling rubble into a 2^18 ring),
+    then frames of falling chunks + debris + voxel writes, 60 fps."""
+    from pixels_amd import CTRL
+    from pixels_amd import _native as N
+    vs = np.float32(1.0 / 80.0)
+    u = np.zeros(N.PX_UNIFORM_FLOATS, np.float32)
+    u[0:3] = scene.dims
+    u[3], u[24], u[25], u[29] = vs, np.float32(1.0 / 60.0), np.float32(9.81) / vs, 5.0
+    gw.set_uniforms(u)
+    sim.particles_init(1 << 18)
+    reps = []
+    for _ in range(max(3, args.steps // 4)):
+        gw.restore()
+        sim.particles_init(1 << 18)
+        sim.push_impacts(impacts)
+        sim.encode(N.STAGE_FRAME_BEGIN | N.STAGE_EDIT)
+        gw.sync()
+        gw.profile_begin()
+        sim.encode(N.STAGE_RESOLVE)
+        prof = gw.profile_end()
+        reps.append(prof[N.PROF_CHUNKS][0])
+    ctrl, _, _ = sim.read_ctrl()
+    res = {"resolve_ms": round(float(np.median(reps)), 4), "reps": len(reps),
+           "chunks": int(ctrl[CTRL["S_ISLANDS"]]), "arena_voxels": int(ctrl[CTRL["C_ARENA"]]),
+           "debris_spawned": int(ctrl[CTRL["C_SPAWNED"]])}
+    frames = {"chunks": [], "particles": [], "writes": [], "frame": []}
+    for f in range(6, 6 + max(20, args.steps)):
+        u[29] = f
+        gw.set_uniforms(u)
+        gw.profile_begin()
+        sim.encode(N.STAGE_FRAME_BEGIN | N.STAGE_CHUNKS | N.STAGE_PARTICLES | N.STAGE_WRITES)
+        prof = gw.profile_end()
+        c = prof.get(N.PROF_CHUNKS, (0.0, 0))[0]
+        p = prof.get(N.PROF_PARTICLES, (0.0, 0))[0]
+        w = prof.get(N.PROF_WRITES, (0.0, 0))[0]
+        frames["chunks"].append(c)
+        frames["particles"].append(p)
+        frames["writes"].append(w)
+        frames["frame"].append(c + p + w)
+    ctrl, _, _ = sim.read_ctrl()
+    res.update({k + "_ms": round(float(np.median(v)), 4) for k, v in frames.items()})
+    res["frames"] = len(frames["frame"])
+    res["falling_after"] = int(ctrl[CTRL["S_FALLING"]])
+    log("extraction: %.3f ms (%d chunks, %d arena voxels, %d debris); frame of chunks + debris + writes %.3f ms"
+        % (res["resolve_ms"], res["chunks"], res["arena_voxels"], res["debris_spawned"], res["frame_ms"]))
     return res
 
 

@@ -1497,14 +1497,28 @@ __global__ __launch_bounds__(64) void k_extract(EditView e, const uint32_t* spaw
       if (lane == 0) e.cpair[i].y = t;
       continue;
     }
-    // PASS 2
+    // PASS 2: the cell's own voxels are cleared word by word (one atomic AND
+    // per pool word: rubble in the same word may be taken by another
+    // wavefront) and the cell is listed for maint once
     uint32_t j = e.coff[i].y + wave_scan(ns) - ns;
     a = abase;
+    uint32_t wmask[4] = {0u, 0u, 0u, 0u};
+    for (int x = 0; x < 8; x++)
+      if (cls[x] != 0u) wmask[x >> 1] |= 0xffffu << ((x & 1) * 16);
+    uint32_t* row = e.pool + (size_t)(g - 1u) * 256u + lane * 4u;
+    for (int q = 0; q < 4; q++)
+      if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
+    if (__ballot((wmask[0] | wmask[1] | wmask[2] | wmask[3]) != 0u) != 0ull && lane == 0) {
+      const uint32_t bit = 1u << (ci & 31u);
+      if ((atomicOr(&e.dirty_flags[ci >> 5], bit) & bit) == 0u) {
+        const uint32_t k = atomicAdd(&e.ec[EC_R_DIRTY], 1u);
+        if (k < e.ncell) e.rdirty[k] = ci;
+      }
+    }
     for (int x = 0; x < 8; x++) {
       if (cls[x] == 0u) continue;
       const int32_t px = bx * 8 + x, py = by * 8 + ry, pz = bz * 8 + rz;
       const uint32_t v = val[x];
-      clear_voxel(e, px, py, pz);
       // dropRubbleAbove: bottom to top, before the voxel's own record
       for (uint32_t d = 1; d <= rub[x]; d++) {
         const int32_t qy = py + (int32_t)d;
