@@ -66,6 +66,9 @@ def load():
     lib.o_chunk_frame.restype = C.c_int
     lib.o_chunk_frame.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, P, P, C.c_float, C.c_float,
                                   C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, P]
+    lib.o_raster.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, C.c_uint32, P, P]
+    lib.o_prim_sdf.restype = C.c_double
+    lib.o_prim_sdf.argtypes = [P, C.c_double, C.c_double, C.c_double]
     lib.o_frame_begin.argtypes = [P]
     lib.o_spawn.argtypes = [P, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
     lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
@@ -274,6 +277,22 @@ class Chunks:
     def impact_list(self):
         n = min(int(self.ctr[R_C_IMPACTS]), self.max_impacts)
         return self.impacts[:16 * n].reshape(-1, 16).copy()
+
+
+def raster(dims, prims: np.ndarray):
+    """apply() of every primitive in order on an empty world; returns
+    (dense voxels shaped (dz, dy, dx), per-cell kinds)."""
+    dx, dy, dz = (int(d) for d in dims)
+    vox = np.zeros((dz, dy, dx), np.uint16)
+    kind = np.zeros((dx // 8) * (dy // 8) * (dz // 8), np.uint8)
+    pr = np.ascontiguousarray(prims)
+    load().o_raster(dx, dy, dz, pr.ctypes.data, pr.shape[0], vox.ctypes.data, kind.ctypes.data)
+    return vox, kind
+
+
+def prim_sdf(prim, x, y, z) -> float:
+    pr = np.ascontiguousarray(np.atleast_1d(prim))
+    return float(load().o_prim_sdf(pr.ctypes.data, x, y, z))
 
 
 def islands(dims, vox: np.ndarray, mats, cap: int = 1 << 20):

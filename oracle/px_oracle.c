@@ -16,6 +16,7 @@
  *   spawn / frameBegin / particleSim    src/render/shaders/sim.wgsl:166-170, 363-369, 1088-1173
  *   voxel writes (request, apply)       src/render/shaders/sim.wgsl:94-102, 127-146, 1178-1203
  *   extraction, falling chunks          src/render/shaders/sim.wgsl:172-178, 274-284, 862-923, 951-1075
+ *   scene rasteriser apply()            src/world/shapes.ts:24-143, 334-414
  * and a textbook breadth-first 6-connected labelling for the islands.
  *
  * Choices the WGSL leaves to the implementation, fixed here (the HIP library
@@ -1466,4 +1467,162 @@ int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* ki
     }
   }
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Scene rasteriser: apply() of src/world/shapes.ts:334-414 (Op.Add with a
+ * constant paint), one primitive after another over the whole world as the
+ * reference runs it, on a dense voxel array with a per-cell kind (0 empty,
+ * 1 uniform, 2 pool brick) that follows setBrickUniform / compactCell
+ * (src/world/VoxelWorld.ts:147-156).  SDFs (shapes.ts:24-143): box, rounded
+ * box, sphere, capped cylinder, capsule, torus, axis-aligned box, all in
+ * voxel units; a yaw turns the primitive about the vertical axis through its
+ * centre (local = (c·dx + s·dz, dy, −s·dx + c·dz)).  Double precision as the
+ * reference's JavaScript numbers. */
+
+enum { OP_BOX = 0, OP_ROUNDBOX, OP_SPHERE, OP_CYLINDER, OP_CAPSULE, OP_TORUS, OP_AABB };
+
+static inline double dmax(double a, double b) { return a < b ? b : a; }
+static inline double dmin(double a, double b) { return b < a ? b : a; }
+static inline double dlen2(double a, double b) { return sqrt(a * a + b * b); }
+static inline double dlen3(double a, double b, double c) { return sqrt(a * a + b * b + c * c); }
+
+static void prim_local(const o_prim* P, double x, double y, double z, double* lx, double* ly, double* lz) {
+  double dx = x - P->p[0], dy = y - P->p[1], dz = z - P->p[2];
+  *lx = P->cy * dx + P->sy * dz;
+  *ly = dy;
+  *lz = -P->sy * dx + P->cy * dz;
+}
+
+static void prim_axis(int axis, double lx, double ly, double lz, double* a, double* b, double* h) {
+  if (axis == 1) { *a = lx; *b = lz; *h = ly; }
+  else if (axis == 0) { *a = ly; *b = lz; *h = lx; }
+  else { *a = lx; *b = ly; *h = lz; }
+}
+
+double o_prim_sdf(const o_prim* P, double x, double y, double z) {
+  double lx, ly, lz, a, b, h;
+  switch (P->type) {
+    case OP_BOX: {
+      prim_local(P, x, y, z, &lx, &ly, &lz);
+      double qx = fabs(lx) - P->p[3], qy = fabs(ly) - P->p[4], qz = fabs(lz) - P->p[5];
+      double out = dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0));
+      return out + dmin(dmax(qx, dmax(qy, qz)), 0.0);
+    }
+    case OP_ROUNDBOX: {
+      prim_local(P, x, y, z, &lx, &ly, &lz);
+      double r = P->p[6];
+      double qx = fabs(lx) - (P->p[3] - r), qy = fabs(ly) - (P->p[4] - r), qz = fabs(lz) - (P->p[5] - r);
+      return dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0)) + dmin(dmax(qx, dmax(qy, qz)), 0.0) - r;
+    }
+    case OP_SPHERE:
+      return dlen3(x - P->p[0], y - P->p[1], z - P->p[2]) - P->p[3];
+    case OP_CYLINDER: {
+      prim_local(P, x, y, z, &lx, &ly, &lz);
+      prim_axis((int)P->p[5], lx, ly, lz, &a, &b, &h);
+      double dr = dlen2(a, b) - P->p[3], dh = fabs(h) - P->p[4];
+      return dmin(dmax(dr, dh), 0.0) + dlen2(dmax(dr, 0.0), dmax(dh, 0.0));
+    }
+    case OP_CAPSULE: {
+      double ax = P->p[0], ay = P->p[1], az = P->p[2];
+      double bax = P->p[3] - ax, bay = P->p[4] - ay, baz = P->p[5] - az;
+      double bb = bax * bax + bay * bay + baz * baz;
+      if (bb == 0.0) bb = 1e-12;
+      double px = x - ax, py = y - ay, pz = z - az;
+      double t = dmin(1.0, dmax(0.0, (px * bax + py * bay + pz * baz) / bb));
+      return dlen3(px - bax * t, py - bay * t, pz - baz * t) - P->p[6];
+    }
+    case OP_TORUS: {
+      prim_local(P, x, y, z, &lx, &ly, &lz);
+      prim_axis((int)P->p[5], lx, ly, lz, &a, &b, &h);
+      return dlen2(dlen2(a, b) - P->p[3], h) - P->p[4];
+    }
+    case OP_AABB: {
+      double cx = 0.5 * (P->p[0] + P->p[3]), cy = 0.5 * (P->p[1] + P->p[4]), cz = 0.5 * (P->p[2] + P->p[5]);
+      double hx = 0.5 * (P->p[3] - P->p[0]), hy = 0.5 * (P->p[4] - P->p[1]), hz = 0.5 * (P->p[5] - P->p[2]);
+      double qx = fabs(x - cx) - hx, qy = fabs(y - cy) - hy, qz = fabs(z - cz) - hz;
+      double out = dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0));
+      return out + dmin(dmax(qx, dmax(qy, qz)), 0.0);
+    }
+  }
+  return 1e30;
+}
+
+/* shapes.ts:47-48: the exact containment test of an axis-aligned box */
+static int prim_contains(const o_prim* P, int* has, double x0, double y0, double z0, double x1, double y1, double z1) {
+  *has = 0;
+  if (P->type == OP_AABB) {
+    *has = 1;
+    return x0 >= P->p[0] && x1 <= P->p[3] && y0 >= P->p[1] && y1 <= P->p[4] && z0 >= P->p[2] && z1 <= P->p[5];
+  }
+  if (P->type == OP_BOX && P->sy == 0.0 && P->cy == 1.0) {
+    *has = 1;
+    return x0 >= P->p[0] - P->p[3] && x1 <= P->p[0] + P->p[3] && y0 >= P->p[1] - P->p[4] &&
+           y1 <= P->p[1] + P->p[4] && z0 >= P->p[2] - P->p[5] && z1 <= P->p[2] + P->p[5];
+  }
+  return 0;
+}
+
+void o_raster(int32_t dx, int32_t dy, int32_t dz, const o_prim* prims, uint32_t n, uint16_t* vox, uint8_t* kind) {
+  const double brickHalfDiag = 4.0 * sqrt(3.0) + 0.01;
+  const double subHalfDiag = 2.0 * sqrt(3.0) + 0.01;
+  const int32_t bx_n = dx >> 3, by_n = dy >> 3;
+  const int32_t dims[3] = {dx, dy, dz};
+  uint16_t scratch[512];
+  for (uint32_t k = 0; k < n; k++) {
+    const o_prim* P = &prims[k];
+    const uint16_t pv = (uint16_t)P->mat;
+    int32_t v0[3], v1[3];
+    for (int a = 0; a < 3; a++) {
+      double lo = floor(P->lo[a] - 0.5), hi = ceil(P->hi[a] + 0.5);
+      v0[a] = lo < 0.0 ? 0 : (int32_t)lo;
+      v1[a] = hi > (double)(dims[a] - 1) ? dims[a] - 1 : (int32_t)hi;
+    }
+    if (v0[0] > v1[0] || v0[1] > v1[1] || v0[2] > v1[2]) continue;
+    for (int32_t bz = v0[2] >> 3; bz <= v1[2] >> 3; bz++)
+      for (int32_t by = v0[1] >> 3; by <= v1[1] >> 3; by++)
+        for (int32_t bx = v0[0] >> 3; bx <= v1[0] >> 3; bx++) {
+          size_t ci = (size_t)bx + (size_t)bx_n * ((size_t)by + (size_t)by_n * (size_t)bz);
+          int32_t ox = bx * 8, oy = by * 8, oz = bz * 8;
+          int has = 0;
+          int full = prim_contains(P, &has, ox + 0.5, oy + 0.5, oz + 0.5, ox + 7.5, oy + 7.5, oz + 7.5);
+          if (!full) {
+            double d = o_prim_sdf(P, ox + 4.0, oy + 4.0, oz + 4.0);
+            if (d > brickHalfDiag) continue;
+            full = d < -brickHalfDiag;
+          }
+          if (full) { /* setBrickUniform */
+            for (uint32_t l = 0; l < 512u; l++)
+              vox[(size_t)(ox + (l & 7u)) + (size_t)dx * ((size_t)(oy + ((l >> 3) & 7u)) + (size_t)dy * (size_t)(oz + (l >> 6)))] = pv;
+            kind[ci] = pv ? 1 : 0;
+            continue;
+          }
+          for (uint32_t l = 0; l < 512u; l++)
+            scratch[l] = vox[(size_t)(ox + (l & 7u)) + (size_t)dx * ((size_t)(oy + ((l >> 3) & 7u)) + (size_t)dy * (size_t)(oz + (l >> 6)))];
+          int changed = 0;
+          for (int s = 0; s < 8; s++) {
+            int32_t sx = ox + (s & 1) * 4, sy = oy + ((s >> 1) & 1) * 4, sz = oz + ((s >> 2) & 1) * 4;
+            double d = o_prim_sdf(P, sx + 2.0, sy + 2.0, sz + 2.0);
+            if (d > subHalfDiag) continue;
+            int subFull = d < -subHalfDiag;
+            for (int32_t z = sz; z < sz + 4; z++)
+              for (int32_t y = sy; y < sy + 4; y++)
+                for (int32_t x = sx; x < sx + 4; x++) {
+                  if (!subFull && o_prim_sdf(P, x + 0.5, y + 0.5, z + 0.5) > 0) continue;
+                  int li = (x & 7) | ((y & 7) << 3) | ((z & 7) << 6);
+                  if (scratch[li] != pv) {
+                    scratch[li] = pv;
+                    changed = 1;
+                  }
+                }
+          }
+          if (!changed) continue;
+          int same = 1;
+          for (uint32_t l = 0; l < 512u; l++) {
+            vox[(size_t)(ox + (l & 7u)) + (size_t)dx * ((size_t)(oy + ((l >> 3) & 7u)) + (size_t)dy * (size_t)(oz + (l >> 6)))] = scratch[l];
+            if (scratch[l] != scratch[0]) same = 0;
+          }
+          kind[ci] = same ? (scratch[0] ? 1 : 0) : 2; /* compactCell */
+        }
+  }
 }

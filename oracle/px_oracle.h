@@ -150,6 +150,23 @@ int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* ki
                   float gravity, uint32_t max_writes, uint32_t* writes, uint32_t max_impacts, float* impacts,
                   o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns);
 
+/* SDF primitive in voxel units, the px_prim layout of include/pixels.h:
+ * type 0 box (c, h), 1 rounded box (c, h, r), 2 sphere (c, r), 3 capped
+ * cylinder (c, r, half height, axis), 4 capsule (a, b, r), 5 torus (c, R, r,
+ * axis), 6 axis-aligned box (lo, hi); cy, sy: cos / sin of the yaw; lo, hi:
+ * world bounds. */
+typedef struct o_prim {
+  uint32_t type;
+  uint32_t mat;
+  double p[12];
+  double cy, sy;
+  double lo[3], hi[3];
+} o_prim;
+double o_prim_sdf(const o_prim* p, double x, double y, double z);
+/* apply() of shapes.ts:334-414 for prims in order (Op.Add, constant paint)
+ * on a dense world (vox, x fastest; kind per cell, compactCell semantics). */
+void o_raster(int32_t dx, int32_t dy, int32_t dz, const o_prim* prims, uint32_t n, uint16_t* vox, uint8_t* kind);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,10 @@ CTRL = {n: i for i, n in enumerate(CTRL_NAMES)}
 RAY_DTYPE = np.dtype([("ro", "<f4", 3), ("rd", "<f4", 3), ("max_t", "<f4"), ("flags", "<u4")])
 HIT_DTYPE = np.dtype([("key", "<u4"), ("t", "<f4"), ("steps", "<u4"), ("face", "u1"), ("mat", "u1"),
                       ("value_hi", "<u2")])
+# 168-byte px_prim record (SDF primitive of the scene rasteriser)
+PRIM_DTYPE = np.dtype([("type", "<u4"), ("mat", "<u4"), ("p", "<f8", 12), ("cy", "<f8"), ("sy", "<f8"),
+                       ("lo", "<f8", 3), ("hi", "<f8", 3)])
+PRIM_BOX, PRIM_ROUNDBOX, PRIM_SPHERE, PRIM_CYLINDER, PRIM_CAPSULE, PRIM_TORUS, PRIM_AABB = range(7)
 # 32-byte px_particle record
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", 3), ("value", "<u4"), ("vel", "<f4", 3), ("life", "<f4")])
 

@@ -38,6 +38,7 @@
 #include <algorithm>
 
 #include "px_internal.h"
+#include "px_scan.h"
 
 #pragma clang fp contract(off)
 
@@ -1133,89 +1134,6 @@ __global__ void k_zero32(uint32_t* p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// ---------------------------------------------------------------------------
-// Ordered scans.  The extraction and chunk stages need list positions in a
-// fixed order (cells, then voxels, then arena entries) without global
-// atomics: per-element (x, y) counts are scanned in tiles of 1024 (one
-// workgroup each), the tile sums in one workgroup, then applied.  Lengths
-// may live on the device (n_dev), so grids are sized by capacity and tiles
-// past the length exit.
-
-constexpr uint32_t kScanTile = 1024;
-
-// Exclusive prefix of (x, y) over the workgroup (1024 threads); returns the
-// block total in *tot.  Block-cooperative.
-PX_DEV uint2 block_excl2(uint2 v, uint2* tot) {
-  __shared__ uint32_t s_x[16], s_y[16];
-  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-  const uint32_t ix = wave_scan(v.x), iy = wave_scan(v.y);
-  if (lane == 63u) {
-    s_x[wave] = ix;
-    s_y[wave] = iy;
-  }
-  __syncthreads();
-  uint32_t px = 0u, py = 0u, tx = 0u, ty = 0u;
-  const uint32_t nw = blockDim.x >> 6;
-  for (uint32_t k = 0; k < nw; k++) {
-    px += k < wave ? s_x[k] : 0u;
-    py += k < wave ? s_y[k] : 0u;
-    tx += s_x[k];
-    ty += s_y[k];
-  }
-  __syncthreads();
-  *tot = make_uint2(tx, ty);
-  return make_uint2(px + ix - v.x, py + iy - v.y);
-}
-
-PX_DEV uint32_t scan_len(const uint32_t* n_dev, uint32_t n_fixed) { return n_dev ? *n_dev : n_fixed; }
-
-__global__ __launch_bounds__(1024) void k_scan2_tiles(const uint2* __restrict__ in, const uint32_t* n_dev,
-                                                      uint32_t n_fixed, uint2* __restrict__ tiles) {
-  const uint32_t n = scan_len(n_dev, n_fixed);
-  const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
-  if (blockIdx.x * kScanTile >= n) return;
-  const uint2 v = i < n ? in[i] : make_uint2(0u, 0u);
-  uint2 tot;
-  block_excl2(v, &tot);
-  if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
-}
-
-// tiles -> exclusive offsets (+ base_x / base_y), totals to tot[0..1]
-__global__ __launch_bounds__(1024) void k_scan2_top(uint2* __restrict__ tiles, const uint32_t* n_dev, uint32_t n_fixed,
-                                                    const uint32_t* base_x, const uint32_t* base_y,
-                                                    uint32_t* __restrict__ tot) {
-  const uint32_t n = scan_len(n_dev, n_fixed);
-  const uint32_t nt = (n + kScanTile - 1u) / kScanTile;
-  uint2 carry = make_uint2(base_x ? *base_x : 0u, base_y ? *base_y : 0u);
-  const uint2 base = carry;
-  __syncthreads();
-  for (uint32_t t0 = 0; t0 < nt; t0 += blockDim.x) {
-    const uint32_t t = t0 + threadIdx.x;
-    const uint2 v = t < nt ? tiles[t] : make_uint2(0u, 0u);
-    uint2 bt;
-    const uint2 ex = block_excl2(v, &bt);
-    if (t < nt) tiles[t] = make_uint2(carry.x + ex.x, carry.y + ex.y);
-    carry.x += bt.x;
-    carry.y += bt.y;
-  }
-  if (threadIdx.x == 0) {
-    tot[0] = carry.x - base.x;
-    tot[1] = carry.y - base.y;
-  }
-}
-
-__global__ __launch_bounds__(1024) void k_scan2_apply(const uint2* __restrict__ in, const uint32_t* n_dev,
-                                                      uint32_t n_fixed, const uint2* __restrict__ tiles,
-                                                      uint2* __restrict__ out) {
-  const uint32_t n = scan_len(n_dev, n_fixed);
-  const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
-  if (blockIdx.x * kScanTile >= n) return;
-  const uint2 v = i < n ? in[i] : make_uint2(0u, 0u);
-  uint2 tot;
-  const uint2 ex = block_excl2(v, &tot);
-  const uint2 t = tiles[blockIdx.x];
-  if (i < n) out[i] = make_uint2(t.x + ex.x, t.y + ex.y);
-}
 
 // ---------------------------------------------------------------------------
 // Island extraction: resolveChunks / resolveExpand / resolveExtract
@@ -2196,11 +2114,7 @@ void px_chunks_free(px_ctx* c) {
 // exclusive (x, y) offsets of e.cpair[0..n) into e.coff, totals into tot[0..1]
 static void scan2(px_ctx* c, const EditView& e, const uint32_t* n_dev, uint32_t n_fixed, const uint32_t* base_x,
                   const uint32_t* base_y, uint32_t* tot) {
-  const uint32_t ntiles = (uint32_t)((c->scan_cap + kScanTile - 1u) / kScanTile);
-  hipLaunchKernelGGL(k_scan2_tiles, dim3(ntiles), dim3(kScanTile), 0, c->stream, e.cpair, n_dev, n_fixed, e.tiles);
-  hipLaunchKernelGGL(k_scan2_top, dim3(1), dim3(1024), 0, c->stream, e.tiles, n_dev, n_fixed, base_x, base_y, tot);
-  hipLaunchKernelGGL(k_scan2_apply, dim3(ntiles), dim3(kScanTile), 0, c->stream, e.cpair, n_dev, n_fixed, e.tiles,
-                     e.coff);
+  scan2_launch(c->stream, e.cpair, e.coff, e.tiles, c->scan_cap, n_dev, n_fixed, base_x, base_y, tot);
 }
 
 int px_launch_resolve(px_ctx* c) {

@@ -555,6 +555,27 @@ pxs_scene* pxs_build(int kind, uint64_t seed, int32_t dx, int32_t dy, int32_t dz
   return reinterpret_cast<pxs_scene*>(S);
 }
 
+// Primitives of the scene in the px_prim layout of include/pixels.h (type,
+// mat, p[12], cy, sy, lo[3], hi[3]); returns the count.  out may be NULL.
+uint32_t pxs_export_prims(const pxs_scene* s, void* out) {
+  const Scene* S = reinterpret_cast<const Scene*>(s);
+  if (out) {
+    unsigned char* o = static_cast<unsigned char*>(out);
+    for (const Prim& P : S->prims) {
+      uint32_t t = (uint32_t)P.type, m = P.mat;
+      std::memcpy(o, &t, 4);
+      std::memcpy(o + 4, &m, 4);
+      std::memcpy(o + 8, P.p, 96);
+      std::memcpy(o + 104, &P.cy, 8);
+      std::memcpy(o + 112, &P.sy, 8);
+      std::memcpy(o + 120, P.lo, 24);
+      std::memcpy(o + 144, P.hi, 24);
+      o += 168;
+    }
+  }
+  return (uint32_t)S->prims.size();
+}
+
 // Encode a dense voxel array (x fastest) into the brick map with canonical
 // compaction; mats may be NULL for the built-in table.
 pxs_scene* pxs_encode_dense(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats) {

@@ -15,7 +15,7 @@ import os
 
 import numpy as np
 
-from ._native import P_DEPOSIT, P_DUST, PARTICLE_DTYPE, RAY_DTYPE
+from ._native import P_DEPOSIT, P_DUST, PARTICLE_DTYPE, PRIM_DTYPE, RAY_DTYPE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpxscene.so")
@@ -47,6 +47,8 @@ def _load():
     lib.pxs_pick_camera.argtypes = [P, C.c_uint64, C.POINTER(C.c_double)]
     lib.pxs_camera_rays.argtypes = [C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_double, C.c_int32, P]
     lib.pxs_frame_inputs.argtypes = [P, C.c_uint64, C.c_int32, C.c_int32, C.c_float, C.c_int, C.c_double, P, P]
+    lib.pxs_export_prims.restype = C.c_uint32
+    lib.pxs_export_prims.argtypes = [P, P]
     _lib = lib
     return lib
 
@@ -129,6 +131,14 @@ class Scene:
         rays = np.zeros(width * height, RAY_DTYPE)
         _load().pxs_camera_rays(arr, width, height, fov_deg, tile, rays.ctypes.data)
         return rays
+
+    def prims(self) -> np.ndarray:
+        """The scene's SDF primitives in the px_prim layout (PRIM_DTYPE)."""
+        n = _load().pxs_export_prims(self._h, None)
+        out = np.zeros(n, PRIM_DTYPE)
+        if n:
+            _load().pxs_export_prims(self._h, out.ctypes.data)
+        return out
 
     def frame_inputs(self, width: int, height: int, seed: int = 1, voxel_size: float = 1.0 / 80.0,
                      n_lights: int = 8, light_radius_vox: float = 4096.0):
