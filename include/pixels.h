@@ -125,6 +125,35 @@ typedef struct px_world_desc {
 
 typedef struct px_ctx px_ctx;
 
+/* ---- SDF primitives of the scene rasteriser (src/world/shapes.ts:24-143),
+ * in voxel units.  p[] by type:
+ *   PX_PRIM_BOX       centre[3], half extents[3]
+ *   PX_PRIM_ROUNDBOX  centre[3], half extents[3], corner radius
+ *   PX_PRIM_SPHERE    centre[3], radius
+ *   PX_PRIM_CYLINDER  centre[3], radius, half height, axis (0 x, 1 y, 2 z)
+ *   PX_PRIM_CAPSULE   a[3], b[3], radius
+ *   PX_PRIM_TORUS     centre[3], major radius, minor radius, axis
+ *   PX_PRIM_AABB      lo[3], hi[3] (exact containment test, shapes.ts:47-48)
+ * cy, sy: cosine and sine of a yaw about the vertical axis through the
+ * centre (1, 0 for none; the unyawed box also has the exact containment
+ * test).  lo / hi: world bounds of the shape (Shape.min / max); the
+ * rasteriser visits the bricks of [lo - 0.5, hi + 0.5].  mat: the voxel
+ * value painted. */
+#define PX_PRIM_BOX 0u
+#define PX_PRIM_ROUNDBOX 1u
+#define PX_PRIM_SPHERE 2u
+#define PX_PRIM_CYLINDER 3u
+#define PX_PRIM_CAPSULE 4u
+#define PX_PRIM_TORUS 5u
+#define PX_PRIM_AABB 6u
+typedef struct px_prim {
+  uint32_t type;
+  uint32_t mat;
+  double p[12];
+  double cy, sy;
+  double lo[3], hi[3];
+} px_prim;
+
 /* Fills *l with the reference LIMITS (layout.ts:11-33). */
 void px_default_limits(px_limits* l);
 
@@ -132,6 +161,14 @@ void px_default_limits(px_limits* l);
  * the pool, builds the super-grid counts (:90-100) and the free list
  * (:103-113), writes C_FREE_TOP (:111-113) and uploads everything once. */
 int px_create(const px_world_desc* desc, px_ctx** out);
+/* Builds the world on the GPU: apply() of shapes.ts:334-414 (Op.Add,
+ * constant paint) for prims[0..n) in order over an empty world of desc->dims,
+ * straight into the context's brick map (no host grid or pool: desc->grid,
+ * pool, holes and pool_used are ignored; pool_capacity 0 = GpuWorld.ts:60 on
+ * the bricks the scene needs).  Pool bricks are numbered in cell order.
+ * Blocking.  New entry: the reference rasterises on the CPU at load time
+ * (README.md:192 names the GPU version as its next step). */
+int px_create_from_prims(const px_world_desc* desc, const px_prim* prims, uint32_t n, px_ctx** out);
 /* Frees every device buffer of the context. */
 int px_destroy(px_ctx* ctx);
 /* Message of the last failed call on ctx (or of the last failed px_create when

@@ -87,6 +87,7 @@ _FP = C.POINTER(C.c_float)
 SIGNATURES = {
     "px_default_limits": (None, [C.POINTER(PxLimits)]),
     "px_create": (C.c_int, [C.POINTER(PxWorldDesc), C.POINTER(_P)]),
+    "px_create_from_prims": (C.c_int, [C.POINTER(PxWorldDesc), _P, _U32, C.POINTER(_P)]),
     "px_destroy": (C.c_int, [_P]),
     "px_last_error": (C.c_char_p, [_P]),
     "px_set_uniforms": (C.c_int, [_P, _FP]),

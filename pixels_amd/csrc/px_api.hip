@@ -100,24 +100,22 @@ static void release(px_ctx* c) {
   delete c;
 }
 
-int px_create(const px_world_desc* d, px_ctx** out) {
-  if (!d || !out) return px_fail(nullptr, PX_ERR_ARG, "px_create", "null argument");
+// Validation and the context shell (device, limits, dims, stream).
+static int ctx_begin(const px_world_desc* d, const char* what, px_ctx** out) {
   *out = nullptr;
   for (int a = 0; a < 3; a++)
-    if (d->dims[a] == 0 || (d->dims[a] & 7u)) return px_fail(nullptr, PX_ERR_ARG, "px_create", "dims must be positive multiples of 8");
-  if (!d->grid || !d->mats || (d->pool_used && !d->pool) || (d->n_holes && !d->holes))
-    return px_fail(nullptr, PX_ERR_ARG, "px_create", "missing grid/pool/mats/holes");
+    if (d->dims[a] == 0 || (d->dims[a] & 7u)) return px_fail(nullptr, PX_ERR_ARG, what, "dims must be positive multiples of 8");
+  if (!d->mats) return px_fail(nullptr, PX_ERR_ARG, what, "missing mats");
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return px_fail(nullptr, PX_ERR_NODEVICE, "px_create", "no HIP device");
-  if (d->device < 0 || d->device >= ndev) return px_fail(nullptr, PX_ERR_ARG, "px_create", "bad device ordinal");
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return px_fail(nullptr, PX_ERR_NODEVICE, what, "no HIP device");
+  if (d->device < 0 || d->device >= ndev) return px_fail(nullptr, PX_ERR_ARG, what, "bad device ordinal");
   uint64_t cells64 = (uint64_t)(d->dims[0] / 8) * (d->dims[1] / 8) * (d->dims[2] / 8);
   uint64_t vox64 = (uint64_t)d->dims[0] * d->dims[1] * d->dims[2];
-  if (vox64 > 0xFFFFFFFFull) return px_fail(nullptr, PX_ERR_CAPACITY, "px_create", "voxel keys must fit in 32 bits");
-
+  if (vox64 > 0xFFFFFFFFull) return px_fail(nullptr, PX_ERR_CAPACITY, what, "voxel keys must fit in 32 bits");
   px_ctx* c = new (std::nothrow) px_ctx();
-  if (!c) return px_fail(nullptr, PX_ERR_ARG, "px_create", "out of host memory");
+  if (!c) return px_fail(nullptr, PX_ERR_ARG, what, "out of host memory");
   c->device = d->device;
-  if (hipSetDevice(c->device) != hipSuccess) { release(c); return px_fail(nullptr, PX_ERR_HIP, "px_create", "hipSetDevice"); }
+  if (hipSetDevice(c->device) != hipSuccess) { release(c); return px_fail(nullptr, PX_ERR_HIP, what, "hipSetDevice"); }
   if (d->limits) c->lim = *d->limits;
   else px_default_limits(&c->lim);
   c->vs = d->voxel_size > 0.0f ? d->voxel_size : 1.0f;
@@ -129,28 +127,36 @@ int px_create(const px_world_desc* d, px_ctx** out) {
   c->ncell = (uint32_t)cells64;
   c->nsuper = c->sd[0] * c->sd[1] * c->sd[2];
   c->vox_total = (size_t)vox64;
-  uint32_t used = d->pool_used;
-  // GpuWorld.ts:57-60; the storage-binding limit becomes 4 GiB of pool (4M
-  // bricks of 1 KiB), so every pool byte offset fits the 32-bit offsets the
-  // traversal kernels address the pool with.
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    release(c);
+    return px_fail(nullptr, PX_ERR_HIP, what, "hipStreamCreateWithFlags");
+  }
+  *out = c;
+  return PX_OK;
+}
+
+// World buffers for `used` pool bricks (capacity GpuWorld.ts:57-60), the
+// material table, the counters and the free list (holes first, then the top
+// of the pool descending, GpuWorld.ts:103-113).  Grid, pool contents and the
+// super grid are the caller's.
+static int ctx_world(px_ctx* c, const px_world_desc* d, uint32_t used, const uint32_t* holes, uint32_t n_holes,
+                     const char* what) {
+  // the storage-binding limit becomes 4 GiB of pool (4M bricks of 1 KiB), so
+  // every pool byte offset fits the 32-bit offsets the traversal kernels use
   uint64_t max_bricks = (1ull << 32) / 1024ull;
   uint64_t cap = d->pool_capacity ? d->pool_capacity : std::max<uint64_t>((uint64_t)used + 98304ull, 2ull * used);
   if (cap > max_bricks) cap = max_bricks;
-  if (used > cap) { release(c); return px_fail(nullptr, PX_ERR_CAPACITY, "px_create", "scene needs more bricks than the pool holds"); }
+  if (used > cap) return px_fail(nullptr, PX_ERR_CAPACITY, what, "scene needs more bricks than the pool holds");
   c->pool_cap = (uint32_t)cap;
   c->pool_used = used;
-
-  std::string step;
 #define TRY(expr)                                                    \
   do {                                                               \
     hipError_t e_ = (expr);                                          \
     if (e_ != hipSuccess) {                                          \
       std::string m_ = std::string(#expr) + ": " + hipGetErrorString(e_); \
-      release(c);                                                    \
-      return px_fail(nullptr, PX_ERR_HIP, "px_create", m_.c_str());  \
+      return px_fail(nullptr, PX_ERR_HIP, what, m_.c_str());         \
     }                                                                \
   } while (0)
-  TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   TRY(hipMalloc(&c->d_grid, (size_t)c->ncell * 4));
   TRY(hipMalloc(&c->d_pool, (size_t)c->pool_cap * 1024));
   TRY(hipMalloc(&c->d_super, (size_t)c->nsuper * 4));
@@ -169,37 +175,30 @@ int px_create(const px_world_desc* d, px_ctx** out) {
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0)
       c->n_cu = (uint32_t)ncu;
   }
-
   // ---- one-time upload (GpuWorld.ts:85-117) ----
   TRY(hipMemcpy(c->d_mats, d->mats, PX_MAT_COUNT * PX_MAT_FLOATS * 4, hipMemcpyHostToDevice));
-  TRY(hipMemcpy(c->d_grid, d->grid, (size_t)c->ncell * 4, hipMemcpyHostToDevice));
   TRY(hipMemset(c->d_pool, 0, (size_t)c->pool_cap * 1024));
-  if (used) TRY(hipMemcpy(c->d_pool, d->pool, (size_t)used * 1024, hipMemcpyHostToDevice));
-  {
-    std::vector<uint32_t> sc(c->nsuper, 0u);
-    for (uint32_t z = 0; z < c->bd[2]; z++)
-      for (uint32_t y = 0; y < c->bd[1]; y++)
-        for (uint32_t x = 0; x < c->bd[0]; x++)
-          if (d->grid[x + c->bd[0] * (y + c->bd[1] * z)] != 0u)
-            sc[(x >> 2) + c->sd[0] * ((y >> 2) + c->sd[1] * (z >> 2))]++;
-    TRY(hipMemcpy(c->d_super, sc.data(), (size_t)c->nsuper * 4, hipMemcpyHostToDevice));
-    std::vector<uint32_t> fl((size_t)d->n_holes + (c->pool_cap - used));
-    for (uint32_t i = 0; i < d->n_holes; i++) fl[i] = d->holes[i];
-    for (uint32_t i = 0; i < c->pool_cap - used; i++) fl[d->n_holes + i] = c->pool_cap - 1u - i;
-    if (!fl.empty()) TRY(hipMemcpy(c->d_free, fl.data(), fl.size() * 4, hipMemcpyHostToDevice));
-    uint32_t ctrl[PX_CTRL_WORDS] = {0};
-    ctrl[PX_C_FREE_TOP] = (uint32_t)fl.size();
-    TRY(hipMemcpy(c->d_ctrl, ctrl, sizeof ctrl, hipMemcpyHostToDevice));
-    uint32_t args[PX_ARGS_WORDS] = {0};
-    args[36] = 36;  // A_DRAWP
-    args[40] = 36;  // A_DRAWC
-    TRY(hipMemcpy(c->d_args, args, sizeof args, hipMemcpyHostToDevice));
-    float cam[PX_CAMERA_FLOATS] = {0};
-    TRY(hipMemcpy(c->d_camera, cam, sizeof cam, hipMemcpyHostToDevice));
-  }
-  // the uploads ran on the null stream; the context's stream does not wait for it
-  TRY(hipDeviceSynchronize());
+  std::vector<uint32_t> fl((size_t)n_holes + (c->pool_cap - used));
+  for (uint32_t i = 0; i < n_holes; i++) fl[i] = holes[i];
+  for (uint32_t i = 0; i < c->pool_cap - used; i++) fl[n_holes + i] = c->pool_cap - 1u - i;
+  if (!fl.empty()) TRY(hipMemcpy(c->d_free, fl.data(), fl.size() * 4, hipMemcpyHostToDevice));
+  uint32_t ctrl[PX_CTRL_WORDS] = {0};
+  ctrl[PX_C_FREE_TOP] = (uint32_t)fl.size();
+  TRY(hipMemcpy(c->d_ctrl, ctrl, sizeof ctrl, hipMemcpyHostToDevice));
+  uint32_t args[PX_ARGS_WORDS] = {0};
+  args[36] = 36;  // A_DRAWP
+  args[40] = 36;  // A_DRAWC
+  TRY(hipMemcpy(c->d_args, args, sizeof args, hipMemcpyHostToDevice));
+  float cam[PX_CAMERA_FLOATS] = {0};
+  TRY(hipMemcpy(c->d_camera, cam, sizeof cam, hipMemcpyHostToDevice));
 #undef TRY
+  return PX_OK;
+}
+
+// Uniform defaults, traversal masks, edit-chain buffers; waits for the uploads.
+static int ctx_finish(px_ctx* c, const char* what) {
+  // the uploads ran on the null stream; the context's stream does not wait for it
+  if (hipDeviceSynchronize() != hipSuccess) return px_fail(nullptr, PX_ERR_HIP, what, "upload failed");
   c->h_uniforms[0] = (float)c->dims[0];
   c->h_uniforms[1] = (float)c->dims[1];
   c->h_uniforms[2] = (float)c->dims[2];
@@ -210,14 +209,65 @@ int px_create(const px_world_desc* d, px_ctx** out) {
   c->masks_dirty = true;
   int rc = px_launch_masks(c);
   if (rc == PX_OK) rc = px_edit_alloc(c);
-  if (rc != PX_OK) {
-    std::string m = c->err;
-    release(c);
-    return px_fail(nullptr, rc, "px_create", m.c_str());
+  if (rc != PX_OK) return px_fail(nullptr, rc, what, c->err.c_str());
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return px_fail(nullptr, PX_ERR_HIP, what, "initial upload failed");
+  return PX_OK;
+}
+
+int px_create(const px_world_desc* d, px_ctx** out) {
+  if (!d || !out) return px_fail(nullptr, PX_ERR_ARG, "px_create", "null argument");
+  *out = nullptr;
+  if (!d->grid || (d->pool_used && !d->pool) || (d->n_holes && !d->holes))
+    return px_fail(nullptr, PX_ERR_ARG, "px_create", "missing grid/pool/mats/holes");
+  px_ctx* c = nullptr;
+  if (int rc = ctx_begin(d, "px_create", &c)) return rc;
+  int rc = ctx_world(c, d, d->pool_used, d->holes, d->n_holes, "px_create");
+  if (rc == PX_OK) {
+    bool ok = hipMemcpy(c->d_grid, d->grid, (size_t)c->ncell * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && d->pool_used)
+      ok = hipMemcpy(c->d_pool, d->pool, (size_t)d->pool_used * 1024, hipMemcpyHostToDevice) == hipSuccess;
+    std::vector<uint32_t> sc(c->nsuper, 0u);
+    for (uint32_t z = 0; z < c->bd[2]; z++)
+      for (uint32_t y = 0; y < c->bd[1]; y++)
+        for (uint32_t x = 0; x < c->bd[0]; x++)
+          if (d->grid[x + c->bd[0] * (y + c->bd[1] * z)] != 0u)
+            sc[(x >> 2) + c->sd[0] * ((y >> 2) + c->sd[1] * (z >> 2))]++;
+    if (ok) ok = hipMemcpy(c->d_super, sc.data(), (size_t)c->nsuper * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) rc = px_fail(nullptr, PX_ERR_HIP, "px_create", "world upload failed");
   }
-  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+  if (rc == PX_OK) rc = ctx_finish(c, "px_create");
+  if (rc != PX_OK) {
     release(c);
-    return px_fail(nullptr, PX_ERR_HIP, "px_create", "initial upload failed");
+    return rc;
+  }
+  *out = c;
+  return PX_OK;
+}
+
+int px_create_from_prims(const px_world_desc* d, const px_prim* prims, uint32_t n, px_ctx** out) {
+  if (!d || !out || (n && !prims)) return px_fail(nullptr, PX_ERR_ARG, "px_create_from_prims", "null argument");
+  *out = nullptr;
+  for (uint32_t k = 0; k < n; k++)
+    if (prims[k].type > PX_PRIM_AABB)
+      return px_fail(nullptr, PX_ERR_ARG, "px_create_from_prims", "unknown primitive type");
+  px_ctx* c = nullptr;
+  if (int rc = ctx_begin(d, "px_create_from_prims", &c)) return rc;
+  int rc = PX_OK;
+  {
+    PxRaster R;
+    uint32_t used = 0;
+    rc = px_raster_begin(c, prims, n, &R, &used);
+    if (rc != PX_OK) rc = px_fail(nullptr, rc, "px_create_from_prims", c->err.c_str());
+    if (rc == PX_OK) rc = ctx_world(c, d, used, nullptr, 0, "px_create_from_prims");
+    if (rc == PX_OK) {
+      rc = px_raster_finish(c, &R);
+      if (rc != PX_OK) rc = px_fail(nullptr, rc, "px_create_from_prims", c->err.c_str());
+    }
+  }
+  if (rc == PX_OK) rc = ctx_finish(c, "px_create_from_prims");
+  if (rc != PX_OK) {
+    release(c);
+    return rc;
   }
   *out = c;
   return PX_OK;

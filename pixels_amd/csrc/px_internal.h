@@ -183,6 +183,37 @@ int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);
 int px_launch_island_pass(px_ctx* c);
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+// scene rasteriser (px_scene.hip): px_raster_begin rasterises prims into
+// these temporaries and reports the pool bricks the world needs;
+// px_raster_finish writes the grid, pool and super grid of the context,
+// whose buffers exist by then.
+struct PxRaster {
+  px_prim* d_prims = nullptr;
+  uint64_t* d_pairs = nullptr;
+  uint64_t* d_sorted = nullptr;
+  uint32_t* d_counts = nullptr;
+  uint64_t* d_offs = nullptr;
+  uint32_t* d_cfirst = nullptr;
+  uint8_t* d_state = nullptr;
+  uint16_t* d_val = nullptr;
+  uint32_t* d_slot = nullptr;
+  uint32_t* d_temp = nullptr;
+  uint32_t* d_ctr = nullptr;
+  uint2* d_flags = nullptr;
+  uint2* d_pidx = nullptr;
+  uint2* d_tiles = nullptr;
+  uint32_t* d_tot = nullptr;
+  uint32_t temp_cap = 0;
+  ~PxRaster() {
+    void* bufs[] = {d_prims, d_pairs, d_sorted, d_counts, d_offs, d_cfirst, d_state, d_val,
+                    d_slot, d_temp, d_ctr, d_flags, d_pidx, d_tiles, d_tot};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+  }
+};
+
+int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, uint32_t* used);
+int px_raster_finish(px_ctx* c, PxRaster* R);
 // voxel writes (px_edit.hip) and the radix sort they use (px_sort.hip)
 int px_writes_alloc(px_ctx* c);
 void px_writes_free(px_ctx* c);

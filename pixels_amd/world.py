@@ -25,19 +25,16 @@ def _devptr(t) -> int:
 class GpuWorld:
     """Every device resource of one world (GpuWorld.ts:17-137)."""
 
-    def __init__(self, scene: Scene, device: int = 0, limits: N.PxLimits | None = None, pool_capacity: int = 0,
-                 voxel_size: float = 1.0 / 80.0):
+    def __init__(self, scene: Scene | None, device: int = 0, limits: N.PxLimits | None = None, pool_capacity: int = 0,
+                 voxel_size: float = 1.0 / 80.0, prims: np.ndarray | None = None, dims=None, mats=None):
+        """From a host-built scene (px_create), or, with `prims` (PRIM_DTYPE),
+        `dims` and `mats`, rasterised on the GPU (px_create_from_prims)."""
         lib = N.load()
         self.lib = lib
         self.scene = scene
         d = N.PxWorldDesc()
-        d.dims[:] = scene.dims
-        self._grid = np.ascontiguousarray(scene.grid, np.uint32)
-        self._pool = np.ascontiguousarray(scene.pool, np.uint16)
-        self._mats = np.ascontiguousarray(scene.mats, np.float32)
-        d.grid = N.ptr(self._grid)
-        d.pool = N.ptr(self._pool, C.c_uint16) if self._pool.size else None
-        d.pool_used = scene.pool_used
+        d.dims[:] = scene.dims if scene is not None else dims
+        self._mats = np.ascontiguousarray(scene.mats if scene is not None else mats, np.float32)
         d.holes = None
         d.n_holes = 0
         d.mats = N.ptr(self._mats, C.c_float)
@@ -47,7 +44,20 @@ class GpuWorld:
         d.voxel_size = voxel_size
         d.device = device
         h = C.c_void_p()
-        N.check(lib.px_create(C.byref(d), C.byref(h)), None)
+        if prims is None:
+            self._grid = np.ascontiguousarray(scene.grid, np.uint32)
+            self._pool = np.ascontiguousarray(scene.pool, np.uint16)
+            d.grid = N.ptr(self._grid)
+            d.pool = N.ptr(self._pool, C.c_uint16) if self._pool.size else None
+            d.pool_used = scene.pool_used
+            N.check(lib.px_create(C.byref(d), C.byref(h)), None)
+        else:
+            pr = np.ascontiguousarray(prims, N.PRIM_DTYPE)
+            d.grid = None
+            d.pool = None
+            d.pool_used = 0
+            N.check(lib.px_create_from_prims(C.byref(d), pr.ctypes.data if pr.size else None, pr.shape[0],
+                                             C.byref(h)), None)
         self.ctx = h
         dims = (C.c_uint32 * 3)()
         bricks = (C.c_uint32 * 3)()
@@ -59,7 +69,7 @@ class GpuWorld:
         self.supers = tuple(supers)
         self.numCells = self.bricks[0] * self.bricks[1] * self.bricks[2]
         self.poolCapacity = int(cap.value)
-        self.initialBricks = scene.pool_used
+        self.initialBricks = scene.pool_used if scene is not None else None
 
     def close(self) -> None:
         if getattr(self, "ctx", None):
