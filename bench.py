@@ -68,6 +68,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-query leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned ray-cost binning leg")
     ap.add_argument("--no-debris", action="store_true", help="skip the debris-particle leg")
+    ap.add_argument("--no-scene", action="store_true", help="skip the GPU scene-build leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is per-GPU work (rays shard with the brick map "
@@ -176,6 +177,7 @@ def main() -> int:
 
     debris_res = None if args.no_debris else debris_leg(args, scene, gw, u, rays, prim)
     gw.close()
+    scene_build = None if args.no_scene else scene_leg(args, scene)
     island = island_leg(args)
     batched = None if args.no_batch else batch_leg(args)
     learned = None if args.no_learned else learned_leg(args)
@@ -203,6 +205,7 @@ def main() -> int:
         "batched": batched,
         "learned_binning": learned,
         "debris": debris_res,
+        "scene_build": scene_build,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -261,6 +264,44 @@ def island_leg(args):
     # extraction and falling chunks continue from the snapshot (after the comparison above)
     res["extraction"] = extraction_leg(args, scene, gw, sim, impacts)
     gw.close()
+    return res
+
+
+def scene_leg(args, scene):
+    """The scene build of the 1280x352x960 grid from its 2002 SDF primitives
+    on the GPU (px_create_from_prims: pairs, sort, per-brick replay, pool
+    fill), wall time of the whole call including context creation, against
+    the host scene builder (all host threads) and the oracle's dense
+    rasteriser (one thread)."""
+    from pixels_amd import GpuWorld
+    prims = scene.prims()
+    walls = []
+    for i in range(4):
+        t1 = time.time()
+        gw = GpuWorld(None, prims=prims, dims=scene.dims, mats=scene.mats)
+        gw.sync()
+        walls.append(time.time() - t1)
+        if i == 0:
+            grid, sup = gw.read_grid()
+            pool = gw.read_pool(0, scene.pool_used).view(np.uint16).reshape(-1)
+            exact = bool(np.array_equal(grid, scene.grid) and np.array_equal(sup, scene.super_grid)
+                         and np.array_equal(pool, scene.pool))
+        gw.close()
+    t1 = time.time()
+    from pixels_amd.scene import LARGE, Scene
+    Scene.build(LARGE, seed=args.seed)
+    host_s = time.time() - t1
+    res = {"gpu_ms": round(float(np.median(walls[1:])) * 1e3, 2), "gpu_ms_first": round(walls[0] * 1e3, 2),
+           "host_builder_ms": round(host_s * 1e3, 2), "host_builder_threads": os.cpu_count(),
+           "primitives": int(prims.shape[0]), "pool_bricks": scene.pool_used, "equals_host_builder": exact}
+    log("scene build on the GPU: %.1f ms (first %.1f ms), host builder %.1f ms, equal: %s" %
+        (res["gpu_ms"], res["gpu_ms_first"], res["host_builder_ms"], exact))
+    if not args.no_cpu:
+        import oracle
+        t1 = time.time()
+        vox, kind = oracle.raster(scene.dims, prims)
+        res["cpu_oracle_ms"] = round((time.time() - t1) * 1e3, 1)
+        del vox, kind
     return res
 
 

@@ -193,20 +193,17 @@ struct PxRaster {
   uint64_t* d_sorted = nullptr;
   uint32_t* d_counts = nullptr;
   uint64_t* d_offs = nullptr;
-  uint32_t* d_cfirst = nullptr;
   uint8_t* d_state = nullptr;
   uint16_t* d_val = nullptr;
   uint32_t* d_slot = nullptr;
   uint32_t* d_temp = nullptr;
-  uint32_t* d_ctr = nullptr;
   uint2* d_flags = nullptr;
   uint2* d_pidx = nullptr;
   uint2* d_tiles = nullptr;
   uint32_t* d_tot = nullptr;
-  uint32_t temp_cap = 0;
   ~PxRaster() {
-    void* bufs[] = {d_prims, d_pairs, d_sorted, d_counts, d_offs, d_cfirst, d_state, d_val,
-                    d_slot, d_temp, d_ctr, d_flags, d_pidx, d_tiles, d_tot};
+    void* bufs[] = {d_prims, d_pairs, d_sorted, d_counts, d_offs, d_state, d_val,
+                    d_slot, d_temp, d_flags, d_pidx, d_tiles, d_tot};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
   }

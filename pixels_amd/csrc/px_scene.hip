@@ -9,13 +9,14 @@
 //                      (brick, primitive) pairs its brick-centre test keeps;
 //   radix sort         pairs by (brick, primitive): each brick's primitives
 //                      in order (rocPRIM, px_sort.hip);
-//   k_cell_heads       first pair of every brick;
-//   k_raster           one wavefront per brick, the 512 voxels in registers
+//   k_raster           one wavefront per brick (launched per pair, the first
+//                      pair of each brick does the work), the 512 voxels in registers
 //                      (lane = row y | z<<3, 8 voxels in x): the brick test,
 //                      the 8 sub-block tests, the voxel tests, compactCell
 //                      (VoxelWorld.ts:147-156) after every change;
-//   k_pool_index +     pool bricks numbered in cell order (as the CPU scene
-//   k_fill_world       builder does), grid entries, brick copies, and the
+//   k_pool_flags +     pool bricks numbered in cell order by an ordered scan
+//   k_fill_world       (as the CPU scene builder numbers them), grid entries,
+//                      brick copies, and the
 //                      super-grid counts (GpuWorld.ts:90-100).
 // SDFs and tests are in double precision, as the reference's JavaScript
 // numbers, with the oracle's operation order (-ffp-contract=off): the world
@@ -29,8 +30,6 @@
 #pragma clang fp contract(off)
 
 namespace {
-
-constexpr uint32_t kNoPair = 0xFFFFFFFFu;
 
 PX_DEV double dmax(double a, double b) { return a < b ? b : a; }
 PX_DEV double dmin(double a, double b) { return b < a ? b : a; }
@@ -50,51 +49,58 @@ PX_DEV void prim_axis(int axis, double lx, double ly, double lz, double& a, doub
   else { a = lx; b = ly; h = lz; }
 }
 
-// SDFs of shapes.ts:24-143 in voxel units
-PX_DEV double prim_sdf(const px_prim& P, double x, double y, double z) {
+// SDFs of shapes.ts:24-143 in voxel units, one type at compile time
+template <uint32_t T>
+__device__ __forceinline__ double sdf_of(const px_prim& P, double x, double y, double z) {
   double lx, ly, lz, a, b, h;
+  if constexpr (T == PX_PRIM_BOX) {
+    prim_local(P, x, y, z, lx, ly, lz);
+    const double qx = fabs(lx) - P.p[3], qy = fabs(ly) - P.p[4], qz = fabs(lz) - P.p[5];
+    const double out = dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0));
+    return out + dmin(dmax(qx, dmax(qy, qz)), 0.0);
+  } else if constexpr (T == PX_PRIM_ROUNDBOX) {
+    prim_local(P, x, y, z, lx, ly, lz);
+    const double r = P.p[6];
+    const double qx = fabs(lx) - (P.p[3] - r), qy = fabs(ly) - (P.p[4] - r), qz = fabs(lz) - (P.p[5] - r);
+    return dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0)) + dmin(dmax(qx, dmax(qy, qz)), 0.0) - r;
+  } else if constexpr (T == PX_PRIM_SPHERE) {
+    return dlen3(x - P.p[0], y - P.p[1], z - P.p[2]) - P.p[3];
+  } else if constexpr (T == PX_PRIM_CYLINDER) {
+    prim_local(P, x, y, z, lx, ly, lz);
+    prim_axis((int)P.p[5], lx, ly, lz, a, b, h);
+    const double dr = dlen2(a, b) - P.p[3], dh = fabs(h) - P.p[4];
+    return dmin(dmax(dr, dh), 0.0) + dlen2(dmax(dr, 0.0), dmax(dh, 0.0));
+  } else if constexpr (T == PX_PRIM_CAPSULE) {
+    const double ax = P.p[0], ay = P.p[1], az = P.p[2];
+    const double bax = P.p[3] - ax, bay = P.p[4] - ay, baz = P.p[5] - az;
+    double bb = bax * bax + bay * bay + baz * baz;
+    if (bb == 0.0) bb = 1e-12;
+    const double px = x - ax, py = y - ay, pz = z - az;
+    const double t = dmin(1.0, dmax(0.0, (px * bax + py * bay + pz * baz) / bb));
+    return dlen3(px - bax * t, py - bay * t, pz - baz * t) - P.p[6];
+  } else if constexpr (T == PX_PRIM_TORUS) {
+    prim_local(P, x, y, z, lx, ly, lz);
+    prim_axis((int)P.p[5], lx, ly, lz, a, b, h);
+    return dlen2(dlen2(a, b) - P.p[3], h) - P.p[4];
+  } else {
+    static_assert(T == PX_PRIM_AABB, "primitive type");
+    const double cx = 0.5 * (P.p[0] + P.p[3]), cy = 0.5 * (P.p[1] + P.p[4]), cz = 0.5 * (P.p[2] + P.p[5]);
+    const double hx = 0.5 * (P.p[3] - P.p[0]), hy = 0.5 * (P.p[4] - P.p[1]), hz = 0.5 * (P.p[5] - P.p[2]);
+    const double qx = fabs(x - cx) - hx, qy = fabs(y - cy) - hy, qz = fabs(z - cz) - hz;
+    const double out = dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0));
+    return out + dmin(dmax(qx, dmax(qy, qz)), 0.0);
+  }
+}
+
+PX_DEV double prim_sdf(const px_prim& P, double x, double y, double z) {
   switch (P.type) {
-    case PX_PRIM_BOX: {
-      prim_local(P, x, y, z, lx, ly, lz);
-      const double qx = fabs(lx) - P.p[3], qy = fabs(ly) - P.p[4], qz = fabs(lz) - P.p[5];
-      const double out = dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0));
-      return out + dmin(dmax(qx, dmax(qy, qz)), 0.0);
-    }
-    case PX_PRIM_ROUNDBOX: {
-      prim_local(P, x, y, z, lx, ly, lz);
-      const double r = P.p[6];
-      const double qx = fabs(lx) - (P.p[3] - r), qy = fabs(ly) - (P.p[4] - r), qz = fabs(lz) - (P.p[5] - r);
-      return dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0)) + dmin(dmax(qx, dmax(qy, qz)), 0.0) - r;
-    }
-    case PX_PRIM_SPHERE:
-      return dlen3(x - P.p[0], y - P.p[1], z - P.p[2]) - P.p[3];
-    case PX_PRIM_CYLINDER: {
-      prim_local(P, x, y, z, lx, ly, lz);
-      prim_axis((int)P.p[5], lx, ly, lz, a, b, h);
-      const double dr = dlen2(a, b) - P.p[3], dh = fabs(h) - P.p[4];
-      return dmin(dmax(dr, dh), 0.0) + dlen2(dmax(dr, 0.0), dmax(dh, 0.0));
-    }
-    case PX_PRIM_CAPSULE: {
-      const double ax = P.p[0], ay = P.p[1], az = P.p[2];
-      const double bax = P.p[3] - ax, bay = P.p[4] - ay, baz = P.p[5] - az;
-      double bb = bax * bax + bay * bay + baz * baz;
-      if (bb == 0.0) bb = 1e-12;
-      const double px = x - ax, py = y - ay, pz = z - az;
-      const double t = dmin(1.0, dmax(0.0, (px * bax + py * bay + pz * baz) / bb));
-      return dlen3(px - bax * t, py - bay * t, pz - baz * t) - P.p[6];
-    }
-    case PX_PRIM_TORUS: {
-      prim_local(P, x, y, z, lx, ly, lz);
-      prim_axis((int)P.p[5], lx, ly, lz, a, b, h);
-      return dlen2(dlen2(a, b) - P.p[3], h) - P.p[4];
-    }
-    case PX_PRIM_AABB: {
-      const double cx = 0.5 * (P.p[0] + P.p[3]), cy = 0.5 * (P.p[1] + P.p[4]), cz = 0.5 * (P.p[2] + P.p[5]);
-      const double hx = 0.5 * (P.p[3] - P.p[0]), hy = 0.5 * (P.p[4] - P.p[1]), hz = 0.5 * (P.p[5] - P.p[2]);
-      const double qx = fabs(x - cx) - hx, qy = fabs(y - cy) - hy, qz = fabs(z - cz) - hz;
-      const double out = dlen3(dmax(qx, 0.0), dmax(qy, 0.0), dmax(qz, 0.0));
-      return out + dmin(dmax(qx, dmax(qy, qz)), 0.0);
-    }
+    case PX_PRIM_BOX: return sdf_of<PX_PRIM_BOX>(P, x, y, z);
+    case PX_PRIM_ROUNDBOX: return sdf_of<PX_PRIM_ROUNDBOX>(P, x, y, z);
+    case PX_PRIM_SPHERE: return sdf_of<PX_PRIM_SPHERE>(P, x, y, z);
+    case PX_PRIM_CYLINDER: return sdf_of<PX_PRIM_CYLINDER>(P, x, y, z);
+    case PX_PRIM_CAPSULE: return sdf_of<PX_PRIM_CAPSULE>(P, x, y, z);
+    case PX_PRIM_TORUS: return sdf_of<PX_PRIM_TORUS>(P, x, y, z);
+    case PX_PRIM_AABB: return sdf_of<PX_PRIM_AABB>(P, x, y, z);
   }
   return 1e30;
 }
@@ -170,37 +176,84 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-__global__ void k_cell_heads(const uint64_t* __restrict__ pairs, uint64_t n, uint32_t* __restrict__ cfirst) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t ci = (uint32_t)(pairs[i] >> 32);
-    if (i == 0 || (uint32_t)(pairs[i - 1] >> 32) != ci) cfirst[ci] = (uint32_t)i;
+// Sets v[base + k] to pv for the k whose bit is set in bits4; true if a voxel
+// changed.
+__device__ __forceinline__ bool put4(uint32_t (&v)[8], int base, uint32_t bits4, uint32_t pv) {
+  bool changed = false;
+  for (int k = 0; k < 4; k++) {
+    const bool set = ((bits4 >> k) & 1u) != 0u;
+    changed |= set && v[base + k] != pv;
+    v[base + k] = set ? pv : v[base + k];
   }
+  return changed;
 }
 
-// One wavefront per brick.  state: 0 empty, 1 uniform (val), 2 explicit.
+// The sub-block and voxel tests of one partial brick for one primitive of
+// type T (shapes.ts:380-405).  The voxels stay in the row layout (lane = row
+// ry | rz << 3, v = its 8 voxels in x); the SDF work is sub-block major:
+//   - one wave-wide pass for the 8 sub-block tests (lane & 7 = sub-block
+//     s = x half | y half << 1 | z half << 2), reduced to full / partial
+//     masks by ballots;
+//   - one wave-wide pass per partial sub-block, lane = its voxel
+//     (x & 3) | (y & 3) << 2 | (z & 3) << 4, reduced to a 64-bit inside mask
+//     that the rows of that sub-block read their 4 bits from.
+// Every voxel set writes the same pv, so the sub-block order of the
+// reference does not change the result.  No SDF sits under a lane-divergent
+// branch (a per-lane early exit around the f64 square roots left partial
+// bricks empty with hipcc -O3 for gfx950).
+template <uint32_t T>
+__device__ __forceinline__ bool raster_partial(const px_prim& P, int32_t ox, int32_t oy, int32_t oz, uint32_t lane,
+                                               uint32_t pv, uint32_t (&v)[8]) {
+  const double sd = sub_half_diag();
+  const uint32_t s8 = lane & 7u;
+  const double ds = sdf_of<T>(P, ox + (int32_t)(s8 & 1u) * 4 + 2.0, oy + (int32_t)((s8 >> 1) & 1u) * 4 + 2.0,
+                              oz + (int32_t)(s8 >> 2) * 4 + 2.0);
+  const uint32_t full_m = (uint32_t)__ballot(ds < -sd) & 0xFFu;
+  uint32_t part_m = (uint32_t)__ballot(!(ds > sd) && !(ds < -sd)) & 0xFFu;
+  const uint32_t ry = lane & 7u, rz = lane >> 3;
+  const uint32_t srow = ((ry >> 2) << 1) | ((rz >> 2) << 2);  // this row's sub-blocks: srow, srow | 1
+  const uint32_t rbit = ((ry & 3u) << 2) | ((rz & 3u) << 4);   // its voxel x = 0 in a sub-block's mask
+  bool changed = put4(v, 0, ((full_m >> srow) & 1u) ? 0xFu : 0u, pv);
+  changed |= put4(v, 4, ((full_m >> (srow | 1u)) & 1u) ? 0xFu : 0u, pv);
+  const int32_t lx = (int32_t)(lane & 3u), ly = (int32_t)((lane >> 2) & 3u), lz = (int32_t)(lane >> 4);
+  while (part_m != 0u) {
+    const uint32_t s = (uint32_t)__builtin_ctz(part_m);
+    part_m &= part_m - 1u;
+    const int32_t x = ox + (int32_t)(s & 1u) * 4 + lx, y = oy + (int32_t)((s >> 1) & 1u) * 4 + ly,
+                  z = oz + (int32_t)(s >> 2) * 4 + lz;
+    const double dv = sdf_of<T>(P, x + 0.5, y + 0.5, z + 0.5);
+    const uint64_t in = __ballot(!(dv > 0.0));
+    const uint32_t bits = (srow == (s & 6u)) ? (uint32_t)(in >> rbit) & 0xFu : 0u;
+    if (s & 1u)
+      changed |= put4(v, 4, bits, pv);
+    else
+      changed |= put4(v, 0, bits, pv);
+  }
+  return changed;
+}
+
+// One workgroup (one wavefront) per run of pairs of one brick: the workgroup
+// at the first pair j0 of a brick replays the brick's primitives in order; the
+// others return.  Cells without pairs keep the state 0 they were filled with.
+// state: 0 empty, 1 uniform (val), 2 explicit (voxels staged in temp at
+// brick slot j0, so no counter is shared between wavefronts).
 __global__ __launch_bounds__(64) void k_raster(const px_prim* __restrict__ prims, const uint64_t* __restrict__ pairs,
-                                               uint64_t npairs, const uint32_t* __restrict__ cfirst, RasterDims d,
-                                               uint8_t* __restrict__ out_state, uint16_t* __restrict__ out_val,
-                                               uint32_t* __restrict__ out_slot, uint32_t* __restrict__ temp,
-                                               uint32_t* __restrict__ slot_ctr, uint32_t temp_cap) {
+                                               uint64_t npairs, RasterDims d, uint8_t* __restrict__ out_state,
+                                               uint16_t* __restrict__ out_val, uint32_t* __restrict__ out_slot,
+                                               uint32_t* __restrict__ temp) {
   const uint32_t lane = threadIdx.x;
-  const int32_t ry = (int32_t)(lane & 7u), rz = (int32_t)(lane >> 3);
-  const double hd = brick_half_diag(), sd = sub_half_diag();
-  for (uint32_t ci = blockIdx.x; ci < d.ncell; ci += gridDim.x) {
-    const uint32_t i0 = cfirst[ci];
-    if (i0 == kNoPair) {
-      if (lane == 0) out_state[ci] = 0u;
-      continue;
-    }
+  const double hd = brick_half_diag();
+  for (uint64_t j0 = blockIdx.x; j0 < npairs; j0 += gridDim.x) {
+    const uint32_t ci = (uint32_t)(pairs[j0] >> 32);
+    if (j0 > 0 && (uint32_t)(pairs[j0 - 1] >> 32) == ci) continue;
     const int32_t bx = (int32_t)(ci % (uint32_t)d.bd[0]);
     const int32_t by = (int32_t)((ci / (uint32_t)d.bd[0]) % (uint32_t)d.bd[1]);
     const int32_t bz = (int32_t)(ci / ((uint32_t)d.bd[0] * (uint32_t)d.bd[1]));
     const int32_t ox = bx * 8, oy = by * 8, oz = bz * 8;
-    const int32_t y = oy + ry, z = oz + rz;
     uint32_t state = 0u, val = 0u;
     uint32_t v[8];
     for (int x = 0; x < 8; x++) v[x] = 0u;
-    for (uint64_t j = i0; j < npairs; j++) {
+    for (uint64_t j = j0; j < npairs; j++) {
       const uint64_t key = pairs[j];
       if ((uint32_t)(key >> 32) != ci) break;
       const px_prim& P = prims[(uint32_t)key];
@@ -217,24 +270,16 @@ __global__ __launch_bounds__(64) void k_raster(const px_prim* __restrict__ prims
         for (int x = 0; x < 8; x++) v[x] = pv;
         continue;
       }
-      // Every lane evaluates its two sub-block tests and its 8 voxel SDFs in
-      // straight-line code and masks the result: the same sets as skipping
-      // per sub-block (shapes.ts:375-405), without divergent early exits
-      // around the f64 square roots (that form left partial bricks empty
-      // with hipcc -O3 for gfx950).
       bool changed = false;
-      for (int xh = 0; xh < 2; xh++) {
-        const int32_t sx = ox + xh * 4, sy = oy + (ry & 4), sz = oz + (rz & 4);
-        const double ds = prim_sdf(P, sx + 2.0, sy + 2.0, sz + 2.0);
-        const bool sub_skip = ds > sd, sub_full = ds < -sd;
-        for (int k = 0; k < 4; k++) {
-          const int xi = xh * 4 + k;
-          const double dv = prim_sdf(P, ox + xi + 0.5, y + 0.5, z + 0.5);
-          const bool set = !sub_skip && (sub_full || !(dv > 0.0));
-          const bool diff = v[xi] != pv;
-          changed |= set && diff;
-          v[xi] = set ? pv : v[xi];
-        }
+      switch (P.type) {
+        case PX_PRIM_BOX: changed = raster_partial<PX_PRIM_BOX>(P, ox, oy, oz, lane, pv, v); break;
+        case PX_PRIM_ROUNDBOX: changed = raster_partial<PX_PRIM_ROUNDBOX>(P, ox, oy, oz, lane, pv, v); break;
+        case PX_PRIM_SPHERE: changed = raster_partial<PX_PRIM_SPHERE>(P, ox, oy, oz, lane, pv, v); break;
+        case PX_PRIM_CYLINDER: changed = raster_partial<PX_PRIM_CYLINDER>(P, ox, oy, oz, lane, pv, v); break;
+        case PX_PRIM_CAPSULE: changed = raster_partial<PX_PRIM_CAPSULE>(P, ox, oy, oz, lane, pv, v); break;
+        case PX_PRIM_TORUS: changed = raster_partial<PX_PRIM_TORUS>(P, ox, oy, oz, lane, pv, v); break;
+        case PX_PRIM_AABB: changed = raster_partial<PX_PRIM_AABB>(P, ox, oy, oz, lane, pv, v); break;
+        default: break;  // unknown types failed the brick test (SDF 1e30)
       }
       if (__ballot(changed) == 0ull) continue;
       // compactCell
@@ -253,18 +298,13 @@ __global__ __launch_bounds__(64) void k_raster(const px_prim* __restrict__ prims
       out_val[ci] = (uint16_t)val;
     }
     if (state == 2u) {
-      uint32_t slot = 0u;
-      if (lane == 0) slot = atomicAdd(slot_ctr, 1u);
-      slot = __shfl(slot, 0);
-      if (lane == 0) out_slot[ci] = slot;
-      if (slot < temp_cap) {
-        uint4 w;
-        w.x = v[0] | (v[1] << 16);
-        w.y = v[2] | (v[3] << 16);
-        w.z = v[4] | (v[5] << 16);
-        w.w = v[6] | (v[7] << 16);
-        reinterpret_cast<uint4*>(temp + (size_t)slot * 256u)[lane] = w;
-      }
+      if (lane == 0) out_slot[ci] = (uint32_t)j0;
+      uint4 w;
+      w.x = v[0] | (v[1] << 16);
+      w.y = v[2] | (v[3] << 16);
+      w.z = v[4] | (v[5] << 16);
+      w.w = v[6] | (v[7] << 16);
+      reinterpret_cast<uint4*>(temp + (size_t)j0 * 256u)[lane] = w;
     }
   }
 }
@@ -328,13 +368,12 @@ int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, ui
   const RasterDims d = raster_dims(c);
   const size_t ncell = c->ncell;
   *used = 0;
-  PX_HIP_CHECK(c, hipMalloc(&R->d_cfirst, ncell * 4));
-  PX_HIP_CHECK(c, hipMalloc(&R->d_state, ncell));
+  const size_t state_words = (ncell + 3) / 4;
+  PX_HIP_CHECK(c, hipMalloc(&R->d_state, state_words * 4));
   PX_HIP_CHECK(c, hipMalloc(&R->d_val, ncell * 2));
   PX_HIP_CHECK(c, hipMalloc(&R->d_slot, ncell * 4));
-  PX_HIP_CHECK(c, hipMalloc(&R->d_ctr, 64));
-  hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, R->d_cfirst, ncell, kNoPair);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, R->d_ctr, (size_t)16, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, s, reinterpret_cast<uint32_t*>(R->d_state), state_words,
+                     0u);
   uint64_t npairs = 0;
   if (n) {
     PX_HIP_CHECK(c, hipMalloc(&R->d_prims, (size_t)n * sizeof(px_prim)));
@@ -361,22 +400,28 @@ int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, ui
       hipLaunchKernelGGL(k_prim_pairs<1>, dim3(g), dim3(256), 0, s, R->d_prims, n, d, nullptr, R->d_offs,
                          R->d_pairs);
       PX_HIP_CHECK(c, hipGetLastError());
-      uint32_t cbits = 1, pbits = 1;
+      uint32_t cbits = 1;
       while (cbits < 32 && (1ull << cbits) < ncell) cbits++;
-      while (pbits < 32 && (1ull << pbits) < n) pbits++;
-      (void)pbits;
       if (int rc = px_sort_u64(c, R->d_pairs, R->d_sorted, (uint32_t)npairs, 32u + cbits)) return rc;
-      hipLaunchKernelGGL(k_cell_heads, dim3(1024), dim3(256), 0, s, R->d_sorted, npairs, R->d_cfirst);
     }
   }
-  // bricks with at least one pair: at most min(cells, pairs) explicit bricks
-  R->temp_cap = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(ncell, npairs));
-  PX_HIP_CHECK(c, hipMalloc(&R->d_temp, (size_t)R->temp_cap * 1024));
-  hipLaunchKernelGGL(k_raster, dim3(std::min<uint32_t>(c->ncell, 16384u)), dim3(64), 0, s, R->d_prims, R->d_sorted,
-                     npairs, R->d_cfirst, d, R->d_state, R->d_val, R->d_slot, R->d_temp, R->d_ctr, R->temp_cap);
+  // one staging brick per pair: the brick at its first pair's index
+  PX_HIP_CHECK(c, hipMalloc(&R->d_temp, (size_t)std::max<uint64_t>(1u, npairs) * 1024));
+  if (npairs) {
+    hipLaunchKernelGGL(k_raster, dim3((uint32_t)std::min<uint64_t>(npairs, 1u << 24)), dim3(64), 0, s, R->d_prims,
+                       R->d_sorted, npairs, d, R->d_state, R->d_val, R->d_slot, R->d_temp);
+    PX_HIP_CHECK(c, hipGetLastError());
+  }
+  // pool bricks numbered in cell order; the total sizes the pool
+  PX_HIP_CHECK(c, hipMalloc(&R->d_flags, ncell * 8));
+  PX_HIP_CHECK(c, hipMalloc(&R->d_pidx, ncell * 8));
+  PX_HIP_CHECK(c, hipMalloc(&R->d_tiles, ((ncell + kScanTile - 1) / kScanTile) * 8 + 8));
+  PX_HIP_CHECK(c, hipMalloc(&R->d_tot, 16));
+  hipLaunchKernelGGL(k_pool_flags, dim3(1024), dim3(256), 0, s, R->d_state, c->ncell, R->d_flags);
+  scan2_launch(s, R->d_flags, R->d_pidx, R->d_tiles, ncell, nullptr, c->ncell, nullptr, nullptr, R->d_tot);
   PX_HIP_CHECK(c, hipGetLastError());
   uint32_t nused = 0;
-  PX_HIP_CHECK(c, hipMemcpyAsync(&nused, R->d_ctr, 4, hipMemcpyDeviceToHost, s));
+  PX_HIP_CHECK(c, hipMemcpyAsync(&nused, R->d_tot, 4, hipMemcpyDeviceToHost, s));
   PX_HIP_CHECK(c, hipStreamSynchronize(s));
   *used = nused;
   return PX_OK;
@@ -385,13 +430,6 @@ int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, ui
 int px_raster_finish(px_ctx* c, PxRaster* R) {
   hipStream_t s = c->stream;
   const RasterDims d = raster_dims(c);
-  const size_t ncell = c->ncell;
-  PX_HIP_CHECK(c, hipMalloc(&R->d_flags, ncell * 8));
-  PX_HIP_CHECK(c, hipMalloc(&R->d_pidx, ncell * 8));
-  PX_HIP_CHECK(c, hipMalloc(&R->d_tiles, ((ncell + kScanTile - 1) / kScanTile) * 8 + 8));
-  PX_HIP_CHECK(c, hipMalloc(&R->d_tot, 16));
-  hipLaunchKernelGGL(k_pool_flags, dim3(1024), dim3(256), 0, s, R->d_state, c->ncell, R->d_flags);
-  scan2_launch(s, R->d_flags, R->d_pidx, R->d_tiles, ncell, nullptr, c->ncell, nullptr, nullptr, R->d_tot);
   hipLaunchKernelGGL(k_fill_world, dim3(std::min<uint32_t>(c->ncell, 16384u)), dim3(64), 0, s, R->d_state, R->d_val,
                      R->d_slot, R->d_pidx, R->d_temp, c->ncell, c->d_grid, c->d_pool);
   hipLaunchKernelGGL(k_super_counts, dim3(256), dim3(256), 0, s, c->d_grid, d, c->d_super);
