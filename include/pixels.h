@@ -206,6 +206,27 @@ int px_trace_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, uint32_t n
  * NULL) receives the number of shadow rays actually traced (blocking then). */
 int px_trace_frame_device(px_ctx* ctx, const px_ray* d_primary, uint32_t n, px_hit* d_primary_out,
                           px_hit* d_shadow_out, uint32_t* n_shadow_cast);
+
+/* fsMain (raymarch.wgsl:387-450) to an image: the colour of every primary
+ * ray as RGBA floats (alpha 1; no depth), with shade() (:304-366): material
+ * colour (render_common.wgsl:58-140), coarse AO from the density texture and
+ * fine AO, sun and point lights with the shadow rays of
+ * px_trace_frame_device, hemisphere ambient; glass transmittance and glow
+ * along the primary ray; the reflection ray of reflective materials when
+ * U.flags.x > 0.5 (shaded at quality 0); the sky (:212-264) for misses.
+ * U.flags.w > 0 selects the debug views 1-9 (:399-423).  Uniforms and lights
+ * come from px_set_uniforms / px_set_lights.  sin, cos, exp, pow and atan2
+ * are evaluated in double precision and rounded once to FP32.  Device
+ * buffers: d_rgba holds 4 floats per ray, 16-byte aligned; asynchronous on
+ * the context's stream.  New entry: the reference shades inside its
+ * fragment shader. */
+int px_render_frame_device(px_ctx* ctx, const px_ray* d_rays, uint32_t n, float* d_rgba);
+/* Same over host buffers, blocking. */
+int px_render(px_ctx* ctx, const px_ray* rays, uint32_t n, float* rgba);
+/* The AO density texture (GpuWorld.ts:77-101, maint sim.wgsl:1255-1271):
+ * one byte per brick cell, round(opaque voxels / 512 * 255), rebuilt from the
+ * brick map after edits; out holds bx*by*bz bytes. */
+int px_read_density(px_ctx* ctx, uint8_t* out);
 /* Slots per primary ray that px_trace_frame_device writes (1 + light count). */
 uint32_t px_frame_slots(const px_ctx* ctx);
 
@@ -423,6 +444,10 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_PARTICLE_SIM 7 /* the integration kernel of the step alone */
 #define PX_PROF_WRITES 8   /* voxel-write stage, all kernels */
 #define PX_PROF_CHUNKS 9   /* island extraction and chunk frames, all kernels */
+#define PX_PROF_DENSITY 10 /* AO density rebuild */
+#define PX_PROF_RENDER_PRIMARY 11 /* render: primary march with transmittance / glow + shadow queue */
+#define PX_PROF_RENDER_SHADOW 12  /* render: queued shadow rays with transmittance */
+#define PX_PROF_SHADE 13   /* render: shading (or the debug-view kernel) */
 #define PX_PROF_SLOTS 16
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);

@@ -69,6 +69,17 @@ def load():
     lib.o_raster.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, C.c_uint32, P, P]
     lib.o_prim_sdf.restype = C.c_double
     lib.o_prim_sdf.argtypes = [P, C.c_double, C.c_double, C.c_double]
+    for name in ("o_sinf", "o_cosf", "o_expf", "o_exp2f", "o_log2f"):
+        getattr(lib, name).restype = C.c_float
+        getattr(lib, name).argtypes = [C.c_float]
+    for name in ("o_powf", "o_atan2f"):
+        getattr(lib, name).restype = C.c_float
+        getattr(lib, name).argtypes = [C.c_float, C.c_float]
+    lib.o_march_ex.argtypes = [W, P, P, P, C.c_float, C.c_int, P, P, P, P, P]
+    lib.o_sky_color.argtypes = [P, P, P]
+    lib.o_material_color.argtypes = [P, P, C.c_uint32, P, P]
+    lib.o_density.argtypes = [W, P]
+    lib.o_render.argtypes = [W, P, P, C.c_uint32, P, P, C.c_uint32, P, C.c_int]
     lib.o_frame_begin.argtypes = [P]
     lib.o_spawn.argtypes = [P, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
     lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
@@ -125,6 +136,62 @@ class World:
         load().o_frame(C.byref(self.w), u52.ctypes.data, lights.ctypes.data, lights.size // 4, rays.ctypes.data, n,
                        prim.ctypes.data, sh.ctypes.data, C.byref(nc), threads or os.cpu_count() or 1)
         return prim, sh, int(nc.value)
+
+
+    # -- shading (px_shade.c) ------------------------------------------------
+    def density(self) -> np.ndarray:
+        """AO density texture: one byte per brick cell."""
+        bd = [d // 8 for d in self.dims]
+        out = np.zeros(bd[0] * bd[1] * bd[2], np.uint8)
+        load().o_density(C.byref(self.w), out.ctypes.data)
+        return out
+
+    def render(self, u52, lights, rays, threads: int = 0, density=None) -> np.ndarray:
+        """fsMain colour of every ray: (n, 4) float32 RGBA."""
+        u52 = np.ascontiguousarray(u52, np.float32)
+        lights = np.ascontiguousarray(lights, np.float32)
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        dens = self.density() if density is None else np.ascontiguousarray(density, np.uint8)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        load().o_render(C.byref(self.w), u52.ctypes.data, lights.ctypes.data, lights.size // 4, dens.ctypes.data,
+                        rays.ctypes.data, rays.shape[0], out.ctypes.data, threads or os.cpu_count() or 1)
+        return out
+
+    def march_ex(self, u52, ro, rd, max_t=1e9, shadow=False):
+        """march() with (hit, trans, glow, outer, inner)."""
+        u52 = np.ascontiguousarray(u52, np.float32)
+        ro = np.asarray(ro, np.float32)
+        rd = np.asarray(rd, np.float32)
+        out = np.zeros(1, HIT_DTYPE)
+        trans = np.zeros(3, np.float32)
+        glow = np.zeros(3, np.float32)
+        o, i = C.c_uint32(0), C.c_uint32(0)
+        load().o_march_ex(C.byref(self.w), u52.ctypes.data, ro.ctypes.data, rd.ctypes.data,
+                          float(np.float32(max_t)), int(shadow), out.ctypes.data, trans.ctypes.data,
+                          glow.ctypes.data, C.byref(o), C.byref(i))
+        return out[0], trans, glow, int(o.value), int(i.value)
+
+
+def sky_color(u52, rd) -> np.ndarray:
+    u52 = np.ascontiguousarray(u52, np.float32)
+    rd = np.ascontiguousarray(rd, np.float32)
+    out = np.zeros(3, np.float32)
+    load().o_sky_color(u52.ctypes.data, rd.ctypes.data, out.ctypes.data)
+    return out
+
+
+def material_color(mats, u52, value: int, vp) -> np.ndarray:
+    mats = np.ascontiguousarray(mats, np.float32)
+    u52 = np.ascontiguousarray(u52, np.float32)
+    vp = np.ascontiguousarray(vp, np.int32)
+    out = np.zeros(3, np.float32)
+    load().o_material_color(mats.ctypes.data, u52.ctypes.data, int(value), vp.ctypes.data, out.ctypes.data)
+    return out
+
+
+def fmath(name: str, *args) -> float:
+    """One of the oracle's elementary functions: sin cos exp exp2 log2 pow atan2."""
+    return float(getattr(load(), "o_%sf" % name)(*[float(np.float32(a)) for a in args]))
 
 
 class Particles:

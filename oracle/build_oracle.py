@@ -1,6 +1,6 @@
 """Build recipe of the CPU oracle (test infrastructure only).
 
-Compiles px_oracle.c into oracle/liboracle.so with IEEE single-precision
+Compiles px_oracle.c and px_shade.c into oracle/liboracle.so with IEEE single-precision
 arithmetic rounded once per operation (no FMA contraction, no fast-math).
 The reference (Crunchy000/pixels) is TypeScript + WGSL with no C/C++ sources,
 so there is no reference build to place under oracle/_ref.
@@ -17,7 +17,8 @@ def build(verbose: bool = False) -> str:
     out = os.path.join(HERE, "liboracle.so")
     tmp = out + ".tmp"
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           "-fexcess-precision=standard", "-pthread", os.path.join(HERE, "px_oracle.c"), "-lm", "-o", tmp]
+           "-fexcess-precision=standard", "-pthread", os.path.join(HERE, "px_oracle.c"), os.path.join(HERE, "px_shade.c"),
+           "-lm", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n%s\n%s" % (r.stdout, r.stderr))

@@ -17,7 +17,8 @@
  *   voxel writes (request, apply)       src/render/shaders/sim.wgsl:94-102, 127-146, 1178-1203
  *   extraction, falling chunks          src/render/shaders/sim.wgsl:172-178, 274-284, 862-923, 951-1075
  *   scene rasteriser apply()            src/world/shapes.ts:24-143, 334-414
- * and a textbook breadth-first 6-connected labelling for the islands.
+ * and (px_shade.c) the shading of fsMain: sky, material colour, AO, shade(),
+ * reflections and the debug views; plus a textbook breadth-first 6-connected labelling for the islands.
  *
  * Choices the WGSL leaves to the implementation, fixed here (the HIP library
  * makes the same choices):
@@ -141,13 +142,27 @@ static void set_miss(o_hit* out, uint32_t steps) {
   out->value_hi = 0;
 }
 
-/* raymarch.wgsl:26-38 (glow is shading output and is not kept) */
-static void glassSurface(const float* mats, float* trans, uint32_t id, const float* n, const float* rd, int shadow) {
+/* raymarch.wgsl:26-38.  glow may be NULL (no shading output); with glow the
+ * camera-ray branch adds the Fresnel-weighted sky and the sun glint. */
+static void glassSurface(const float* mats, const float* u52, float* trans, float* glow, uint32_t id, const float* n,
+                         const float* rd, int shadow) {
   const float* tint = matA(mats, id);
   float g = matD(mats, id)[1];
   if (!shadow) {
     float cosT = fabsf(dot3(n, rd));
     float F = 0.04f + 0.96f * pow5(1.0f - cosT);
+    if (glow) {
+      float r[3], sky[3];
+      float dn = dot3(n, rd);
+      for (int a = 0; a < 3; a++) r[a] = rd[a] - 2.0f * dn * n[a];
+      float sr = dot3(r, u52 + 4);
+      float p = o_powf(sr < 0.0f ? 0.0f : sr, matC(mats, id)[3]);
+      o_sky_color(u52, r, sky);
+      for (int a = 0; a < 3; a++) {
+        float glint = p * u52[8 + a] * u52[7] * 2.0f;
+        glow[a] = glow[a] + trans[a] * (F * sky[a] * 0.35f + glint * matC(mats, id)[2]);
+      }
+    }
     for (int a = 0; a < 3; a++) trans[a] = trans[a] * (1.0f - F);
   }
   for (int a = 0; a < 3; a++) {
@@ -159,10 +174,12 @@ static void glassSurface(const float* mats, float* trans, uint32_t id, const flo
 /* ---------------------------------------------------------------------- */
 /* march: raymarch.wgsl:44-162 */
 
-void o_march(const o_world* w, const float ro[3], const float rdIn[3], float maxT, int shadow, o_hit* out,
-             float* trans_out) {
+void o_march_ex(const o_world* w, const float* u52, const float ro[3], const float rdIn[3], float maxT, int shadow,
+                o_hit* out, float* trans_out, float* glow_out, uint32_t* outer_out, uint32_t* inner_out) {
   const float* mats = w->mats;
   float trans[3] = {1.0f, 1.0f, 1.0f};
+  float glow[3] = {0.0f, 0.0f, 0.0f};
+  float* gl = (glow_out && u52) ? glow : NULL;
   uint32_t outer = 0, inner = 0;
   float rd[3], inv[3], t0[3], t1[3], tmn[3], tmx[3];
   float dims[3] = {(float)w->dx, (float)w->dy, (float)w->dz};
@@ -176,16 +193,23 @@ void o_march(const o_world* w, const float ro[3], const float rdIn[3], float max
   }
   float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
   float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
+#define MARCH_OUTPUTS()                                   \
+  do {                                                    \
+    if (trans_out) memcpy(trans_out, trans, sizeof trans); \
+    if (glow_out) memcpy(glow_out, glow, sizeof glow);     \
+    if (outer_out) *outer_out = outer;                     \
+    if (inner_out) *inner_out = inner;                     \
+  } while (0)
 #define RETURN_MISS()                                     \
   do {                                                    \
     set_miss(out, outer + inner);                         \
-    if (trans_out) memcpy(trans_out, trans, sizeof trans); \
+    MARCH_OUTPUTS();                                      \
     return;                                               \
   } while (0)
 #define RETURN_HIT(T, VC, N, V)                           \
   do {                                                    \
     set_hit(w, out, (T), (VC), (N), (V), outer + inner);  \
-    if (trans_out) memcpy(trans_out, trans, sizeof trans); \
+    MARCH_OUTPUTS();                                      \
     return;                                               \
   } while (0)
   if (t >= tExit) RETURN_MISS();
@@ -252,7 +276,7 @@ void o_march(const o_world* w, const float ro[3], const float rdIn[3], float max
             if (v != 0u) {
               uint32_t id = v & 0xffu;
               if (isGlass(mats, id)) {
-                if (!inGlass || id != glassId) glassSurface(mats, trans, id, n, rd, shadow);
+                if (!inGlass || id != glassId) glassSurface(mats, u52, trans, gl, id, n, rd, shadow);
                 inGlass = 1;
                 glassId = id;
                 if (fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f) RETURN_HIT(tv, vc, n, v);
@@ -312,6 +336,12 @@ void o_march(const o_world* w, const float ro[3], const float rdIn[3], float max
   RETURN_MISS();
 #undef RETURN_MISS
 #undef RETURN_HIT
+#undef MARCH_OUTPUTS
+}
+
+void o_march(const o_world* w, const float ro[3], const float rdIn[3], float maxT, int shadow, o_hit* out,
+             float* trans_out) {
+  o_march_ex(w, NULL, ro, rdIn, maxT, shadow, out, trans_out, NULL, NULL, NULL);
 }
 
 /* march() that also returns the hit normal as a vector (for the frame). */

@@ -39,6 +39,12 @@ typedef struct o_hit {
 /* march() of raymarch.wgsl:44-162 on one ray. trans_out (3 floats) may be NULL. */
 void o_march(const o_world* w, const float ro[3], const float rd[3], float max_t, int shadow,
              o_hit* out, float* trans_out);
+/* march() with the shading outputs of raymarch.wgsl:6-14: transmittance,
+ * glow (camera rays: glassSurface adds the Fresnel-weighted sky and the sun
+ * glint, which needs the 52 uniform floats u52) and the outer / inner step
+ * counters (:18-20).  Any output pointer may be NULL. */
+void o_march_ex(const o_world* w, const float* u52, const float ro[3], const float rd[3], float max_t, int shadow,
+                o_hit* out, float* trans_out, float* glow_out, uint32_t* outer_out, uint32_t* inner_out);
 /* traceSolid() of sim.wgsl:296-357. */
 void o_trace_solid(const o_world* w, const float ro[3], const float rd[3], float max_t, o_hit* out);
 /* Segment DDA of particleSim, sim.wgsl:1110-1145 (pos = ro, displacement = mv). */
@@ -68,6 +74,26 @@ int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* ki
 /* Unsupported-component labelling alone (the last stage of o_island_pass). */
 int o_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats,
               uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels);
+
+/* Shading (px_shade.c).  Elementary functions in double precision rounded
+ * once to FP32 (the implementation's choice for WGSL's sin, cos, exp, exp2,
+ * log2, pow, atan2). */
+float o_sinf(float x);
+float o_cosf(float x);
+float o_expf(float x);
+float o_exp2f(float x);
+float o_log2f(float x);
+float o_powf(float x, float y);
+float o_atan2f(float y, float x);
+/* skyColor (raymarch.wgsl:212-264) and materialColor (render_common.wgsl:58-140) */
+void o_sky_color(const float* u52, const float rd[3], float out[3]);
+void o_material_color(const float* mats, const float* u52, uint32_t v, const int32_t vp[3], float out[3]);
+/* AO density texture, one byte per brick cell (VoxelWorld.ts:159-170). */
+void o_density(const o_world* w, uint8_t* out);
+/* fsMain colour (raymarch.wgsl:387-450) for each 32-byte ray: RGBA floats,
+ * the debug views of U.flags.w included.  density from o_density. */
+void o_render(const o_world* w, const float* u52, const float* lights_vec4, uint32_t n_vec4, const uint8_t* density,
+              const float* rays, uint32_t n, float* rgba, int threads);
 
 /* Integer hashes of shared.wgsl:65-81 (exported for tests). */
 uint32_t o_hashu(uint32_t x);

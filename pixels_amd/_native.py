@@ -97,6 +97,9 @@ SIGNATURES = {
     "px_trace": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_device": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_frame_device": (C.c_int, [_P, _P, _U32, _P, _P, _U32P]),
+    "px_render_frame_device": (C.c_int, [_P, _P, _U32, _P]),
+    "px_render": (C.c_int, [_P, _P, _U32, _P]),
+    "px_read_density": (C.c_int, [_P, _P]),
     "px_frame_slots": (_U32, [_P]),
     "px_trace_batch": (C.c_int, [_P, C.POINTER(PxQuerySet)]),
     "px_set_batch_shape": (C.c_int, [_P, _U32, _U32, _U32]),
@@ -146,6 +149,7 @@ SIGNATURES = {
 }
 PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_PARTICLES = 0, 1, 2, 3, 4, 5, 6
 PROF_PARTICLE_SIM, PROF_WRITES, PROF_CHUNKS = 7, 8, 9
+PROF_DENSITY, PROF_RENDER_PRIMARY, PROF_RENDER_SHADOW, PROF_SHADE = 10, 11, 12, 13
 PROF_SLOTS = 16
 
 _lib = None

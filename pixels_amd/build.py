@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["px_api.hip", "px_trace.hip", "px_edit.hip", "px_mlp.hip", "px_particles.hip", "px_sort.hip", "px_scene.hip"]
+HIP_SOURCES = ["px_api.hip", "px_trace.hip", "px_edit.hip", "px_mlp.hip", "px_particles.hip", "px_sort.hip", "px_scene.hip", "px_render.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950",

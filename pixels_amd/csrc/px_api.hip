@@ -88,7 +88,7 @@ static void release(px_ctx* c) {
   if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
   void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
                   c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_shadow_queue, c->d_wave_count, c->d_counters,
-                  c->d_batch_cursors,
+                  c->d_batch_cursors, c->d_density, c->d_rprim, c->d_rtg, c->d_rsh,
                   c->b_grid, c->b_pool, c->b_super, c->b_ctrl, c->b_free};
   for (void* p : bufs)
     if (p) hipFree(p);
@@ -206,7 +206,7 @@ static int ctx_finish(px_ctx* c, const char* what) {
   c->h_lights.assign(4, 0.0f);
   c->shadow.slots = 1;
   c->shadow.vs = c->vs;
-  c->masks_dirty = true;
+  c->masks_dirty = c->density_dirty = true;
   int rc = px_launch_masks(c);
   if (rc == PX_OK) rc = px_edit_alloc(c);
   if (rc != PX_OK) return px_fail(nullptr, rc, what, c->err.c_str());
@@ -374,6 +374,47 @@ int px_trace_frame_device(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit
   return px_launch_frame(c, d_primary, n, d_prim, d_shadow, n_cast);
 }
 
+int px_render_frame_device(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba) {
+  if (!c || (n && (!d_rays || !d_rgba))) return PX_ERR_ARG;
+  if (reinterpret_cast<uintptr_t>(d_rgba) & 15u)
+    return px_fail(c, PX_ERR_ARG, "px_render_frame_device", "d_rgba must be 16-byte aligned");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_render(c, d_rays, n, d_rgba);
+}
+
+int px_render(px_ctx* c, const px_ray* rays, uint32_t n, float* rgba) {
+  if (!c || (n && (!rays || !rgba))) return PX_ERR_ARG;
+  if (n == 0) return PX_OK;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  px_ray* dr = nullptr;
+  float* dc = nullptr;
+  PX_HIP_CHECK(c, hipMalloc(&dr, (size_t)n * sizeof(px_ray)));
+  if (hipMalloc(&dc, (size_t)n * 16u) != hipSuccess) {
+    hipFree(dr);
+    return px_fail(c, PX_ERR_HIP, "px_render", "hipMalloc");
+  }
+  int rc = PX_OK;
+  if (hipMemcpyAsync(dr, rays, (size_t)n * sizeof(px_ray), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    rc = px_fail(c, PX_ERR_HIP, "px_render", "upload");
+  if (rc == PX_OK) rc = px_launch_render(c, dr, n, dc);
+  if (rc == PX_OK && hipMemcpyAsync(rgba, dc, (size_t)n * 16u, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = px_fail(c, PX_ERR_HIP, "px_render", "download");
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == PX_OK) rc = px_fail(c, PX_ERR_HIP, "px_render", "sync");
+  hipFree(dr);
+  hipFree(dc);
+  return rc;
+}
+
+int px_read_density(px_ctx* c, uint8_t* out) {
+  if (!c || !out) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  int rc = px_refresh_density(c);
+  if (rc) return rc;
+  PX_HIP_CHECK(c, hipMemcpyAsync(out, c->d_density, c->ncell, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
 static int check_query_set(px_ctx* c, const px_query_set* q, const char* what) {
   if (!q) return px_fail(c, PX_ERR_ARG, what, "query set is NULL");
   uint64_t total = 0;
@@ -535,7 +576,7 @@ int px_restore(px_ctx* c) {
   PX_HIP_CHECK(c, hipMemcpyAsync(c->d_free, c->b_free, (size_t)c->pool_cap * 4, hipMemcpyDeviceToDevice, s));
   c->n_impacts = 0;
   c->labels_valid = false;
-  c->masks_dirty = true;
+  c->masks_dirty = c->density_dirty = true;
   return px_refresh_masks(c);
 }
 

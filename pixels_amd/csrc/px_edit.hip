@@ -1956,7 +1956,7 @@ int px_launch_island_pass(px_ctx* c) {
     hipLaunchKernelGGL(k_carve, G, W, 0, s, e);
     hipLaunchKernelGGL(k_flood_remove, G, W, 0, s, e);
     hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
-    c->masks_dirty = true;
+    c->masks_dirty = c->density_dirty = true;
   }
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
@@ -2048,7 +2048,7 @@ int px_launch_writes(px_ctx* c) {
   hipLaunchKernelGGL(k_write_maint, dim3(kEditBlocks), dim3(64), 0, s, e);
   hipLaunchKernelGGL(k_write_finish, dim3(1), dim3(64), 0, s, e);
   PX_HIP_CHECK(c, hipGetLastError());
-  c->masks_dirty = true;
+  c->masks_dirty = c->density_dirty = true;
   return px_refresh_masks(c);
 }
 
@@ -2148,7 +2148,7 @@ int px_launch_resolve(px_ctx* c) {
   hipLaunchKernelGGL(k_resolve_maint, G, W, 0, s, e);
   PX_HIP_CHECK(c, hipGetLastError());
   c->labels_valid = false;
-  c->masks_dirty = true;
+  c->masks_dirty = c->density_dirty = true;
   return px_refresh_masks(c);
 }
 

@@ -133,6 +133,16 @@ struct px_ctx {
   uint32_t* d_pblock = nullptr;     // per tile: alive / write counts, then offsets
   float4* d_spawn_stage = nullptr;  // host spawn records staged on the device
   uint32_t spawn_cap = 0;
+  // rendering: AO density per cell, per-pixel hits, transmittance / glow and
+  // shadow results of the last frame
+  uint8_t* d_density = nullptr;
+  bool density_dirty = true;
+  px_hit* d_rprim = nullptr;
+  size_t rprim_cap = 0;
+  float4* d_rtg = nullptr;
+  size_t rtg_cap = 0;
+  float4* d_rsh = nullptr;
+  size_t rsh_cap = 0;
   // device-side checkpoint (px_snapshot / px_restore)
   uint32_t* b_grid = nullptr;
   uint32_t* b_pool = nullptr;
@@ -174,6 +184,9 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
                     uint32_t* n_cast);
 int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
 int px_launch_batch(px_ctx* c, const px_query_set* q);
+// px_render.hip
+int px_refresh_density(px_ctx* c);
+int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba);
 int px_launch_trace_perm(px_ctx* c, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
                          px_hit* d_out);
 int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms);

@@ -20,10 +20,21 @@
 #pragma once
 
 #include "px_internal.h"
+#include "px_shade.h"
 
 #pragma clang fp contract(off)
 
 namespace px {
+
+// Shading outputs of a march (the Hit fields of raymarch.wgsl:6-14 the
+// traversal does not need): transmittance through glass, the light glass
+// surfaces add to camera rays, and the voxel-loop iterations (the debug
+// views' inner counter; outer = steps - inner).
+struct MarchShade {
+  float trans[3];
+  float glow[3];
+  uint32_t inner;
+};
 
 struct MarchOut {
   uint32_t key;
@@ -97,6 +108,35 @@ PX_DEV void glass_surface(const float* __restrict__ mats, float trans[3], uint32
   }
 }
 
+// glassSurface with the camera-ray glow (raymarch.wgsl:26-38): the normal is
+// -sign(rd) along `axis` (zero for axis 3); the sky and the sun glint seen in
+// the reflected direction add to glow before the Fresnel loss.
+PX_DEV void glass_surface_glow(const float* __restrict__ mats, const PxSky& sky, float trans[3], float glow[3],
+                               uint32_t id, uint32_t axis, const float rd[3], const int32_t stp[3]) {
+  const float* A = mats + id * 20u;
+  const float* Cm = A + 8;
+  const float g = mats[id * 20u + 13u];
+  float n[3] = {0.0f, 0.0f, 0.0f};
+  if (axis < 3u) n[axis] = -(float)stp[axis];
+  const float cosT = fabsf(dot3f(n, rd));
+  const float F = 0.04f + 0.96f * px_pow5(1.0f - cosT);
+  const float dn = dot3f(n, rd);
+  const float r[3] = {rd[0] - 2.0f * dn * n[0], rd[1] - 2.0f * dn * n[1], rd[2] - 2.0f * dn * n[2]};
+  const float sr = dot3f(r, sky.sun);
+  const float p = fm::powf_(sr < 0.0f ? 0.0f : sr, Cm[3]);
+  const float3 skc = sky_color(sky, make_float3(r[0], r[1], r[2]));
+  const float sk[3] = {skc.x, skc.y, skc.z};
+  for (int a = 0; a < 3; a++) {
+    const float glint = p * sky.sun_col[a] * sky.sun[3] * 2.0f;
+    glow[a] = glow[a] + trans[a] * (F * sk[a] * 0.35f + glint * Cm[2]);
+  }
+  for (int a = 0; a < 3; a++) trans[a] = trans[a] * (1.0f - F);
+  for (int a = 0; a < 3; a++) {
+    const float m = 1.0f * (1.0f - 0.6f) + A[a] * 0.6f;
+    trans[a] = trans[a] * (m * g);
+  }
+}
+
 // face id of an axis code (3 = zero normal)
 PX_DEV uint32_t face_of_axis(uint32_t axis, const float rd[3]) {
   if (axis >= 3u) return 6u;
@@ -108,10 +148,24 @@ PX_DEV uint32_t face_of_axis(uint32_t axis, const float rd[3]) {
 // axis 0 when m0 < m1 && m0 < m2, else axis 1 when m1 < m2, else axis 2
 // (ties go to the later axis); that axis's tMax advances by |inv| * scale
 // and its cell coordinate by stp.
-template <bool SHADOW>
+// With SHADE, `ms` receives the transmittance, glow (camera rays, with `sky`)
+// and inner step count of the march.
+template <bool SHADOW, bool SHADE = false>
 PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro[3], const float rdIn[3],
-                      float maxT) {
+                      float maxT, const PxSky* sky = nullptr, MarchShade* ms = nullptr) {
   MarchOut o;
+  float glow[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t inner = 0u;
+  float trans[3] = {1.0f, 1.0f, 1.0f};
+  auto done = [&]() {
+    if constexpr (SHADE) {
+      for (int a = 0; a < 3; a++) {
+        ms->trans[a] = trans[a];
+        ms->glow[a] = glow[a];
+      }
+      ms->inner = inner;
+    }
+  };
   o.key = PX_NONE;
   o.t = 0.0f;
   o.face = 0xFFu;
@@ -128,7 +182,10 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
   }
   float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
   const float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
-  if (t >= tExit) return o;
+  if (t >= tExit) {
+    done();
+    return o;
+  }
 
   int32_t stp[3];
   float pos[3];
@@ -149,7 +206,6 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
     sc[a] = px_clampi(px_f2i(floorf(p0 / 32.0f)), 0, w.sd[a] - 1);
     tMaxS[a] = (((float)sc[a] + pos[a]) * 32.0f - ro[a]) * inv[a];
   }
-  float trans[3] = {1.0f, 1.0f, 1.0f};
   uint32_t glassState = 0u;  // 0x100 | id while inside glass `id`, else 0
   uint32_t steps = 0u;
 
@@ -191,6 +247,7 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
             o.face = face_of_axis(axisB, rd);
             o.value = g & 0xffffu;
             o.steps = steps;
+            done();
             return o;
           }
           // byte offset of the brick in the pool (pool bricks only; the pool is < 4 GiB)
@@ -200,6 +257,7 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
           const float d0 = fabsf(inv[0]), d1 = fabsf(inv[1]), d2 = fabsf(inv[2]);
           for (int k = 0; k < 25; k++) {
             steps++;
+            if constexpr (SHADE) inner++;
             const uint32_t li = (uint32_t)lv[0] | ((uint32_t)lv[1] << 3) | ((uint32_t)lv[2] << 6);
             uint32_t v;
             if (uni) v = g & 0xffffu;
@@ -208,7 +266,10 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
               const uint32_t id = v & 0xffu;
               bool hit = true;
               if (bit_of(T.glass, id)) {
-                if (glassState != (0x100u | id)) glass_surface(w.mats, trans, id, axisV, rd, SHADOW);
+                if (glassState != (0x100u | id)) {
+                  if constexpr (SHADE && !SHADOW) glass_surface_glow(w.mats, *sky, trans, glow, id, axisV, rd, stp);
+                  else glass_surface(w.mats, trans, id, axisV, rd, SHADOW);
+                }
                 glassState = 0x100u | id;
                 hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
               }
@@ -218,6 +279,7 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
                 o.face = face_of_axis(axisV, rd);
                 o.value = v;
                 o.steps = steps;
+                done();
                 return o;
               }
             } else {
@@ -236,6 +298,7 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
             lv[2] = (!a0 && !b12) ? lv[2] + stp[2] : lv[2];
             if (tv > tExit) {
               o.steps = steps;
+              done();
               return o;
             }
             if ((uint32_t)(lv[0] | lv[1] | lv[2]) > 7u) break;
@@ -258,6 +321,7 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
         }
         if (tb > tExit) {
           o.steps = steps;
+          done();
           return o;
         }
         if ((uint32_t)lb[0] > (uint32_t)hiL[0] || (uint32_t)lb[1] > (uint32_t)hiL[1] ||
@@ -285,7 +349,73 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
       break;
   }
   o.steps = steps;
+  done();
   return o;
+}
+
+PX_DEV void store_hit(px_hit* out, const MarchOut& m) {
+  uint4 v;
+  v.x = m.key;
+  v.y = __float_as_uint(m.t);
+  v.z = m.steps;
+  v.w = (m.face & 0xffu) | ((m.value & 0xffu) << 8) | (((m.value >> 8) & 0xffu) << 16);
+  *reinterpret_cast<uint4*>(out) = v;
+}
+
+PX_DEV void store_not_cast(px_hit* out) {
+  uint4 v;
+  v.x = PX_NONE;
+  v.y = 0u;
+  v.z = 0u;
+  v.w = 0xFEu;
+  *reinterpret_cast<uint4*>(out) = v;
+}
+
+PX_DEV void load_ray(const px_ray* r, float ro[3], float rd[3], float& maxT, uint32_t& flags) {
+  const float4* p = reinterpret_cast<const float4*>(r);
+  float4 a = p[0], b = p[1];
+  ro[0] = a.x; ro[1] = a.y; ro[2] = a.z;
+  rd[0] = a.w; rd[1] = b.x; rd[2] = b.y;
+  maxT = b.z;
+  flags = __float_as_uint(b.w);
+}
+
+// Shadow ray `s` of shade() (raymarch.wgsl:323-356) for a primary hit at
+// t / face: slot 0 the sun, slot 1 + k light k.  Returns whether it is cast;
+// origin P + N 0.5, direction and max t are written either way.
+PX_DEV bool shadow_ray(const PxWorldView& w, const PxShadowDesc& sd, const float ro[3], const float rd[3], float t,
+                       uint32_t face, uint32_t value, uint32_t s, float O[3], float dir[3], float& mt) {
+  float N[3] = {0.0f, 0.0f, 0.0f};
+  if (face < 6u) N[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
+  float P[3];
+  for (int a = 0; a < 3; a++) {
+    P[a] = ro[a] + rd[a] * t;
+    O[a] = P[a] + N[a] * 0.5f;
+  }
+  dir[0] = dir[1] = dir[2] = 0.0f;
+  mt = 0.0f;
+  if (face == 0xFFu || w.mats[(value & 0xffu) * 20u + 3u] > 0.0f) return false;  // miss or emissive
+  if (s == 0u) {
+    const float ndl = N[0] * sd.sun[0] + N[1] * sd.sun[1] + N[2] * sd.sun[2];
+    dir[0] = sd.sun[0];
+    dir[1] = sd.sun[1];
+    dir[2] = sd.sun[2];
+    mt = 1e9f;
+    return ndl > 0.0f && sd.sun_w > 0.0f;
+  }
+  const float* lp = sd.lp[s - 1u];
+  float L[3];
+  for (int a = 0; a < 3; a++) L[a] = lp[a] / sd.vs - P[a];
+  const float radius = lp[3] / sd.vs;
+  const float d = sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+  if (d > radius) return false;
+  for (int a = 0; a < 3; a++) L[a] = L[a] / d;
+  const float nl = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
+  dir[0] = L[0];
+  dir[1] = L[1];
+  dir[2] = L[2];
+  mt = d - 2.0f;
+  return nl > 0.0f && sd.lc_w[s - 1u] > 0.5f && sd.lamp_shadows > 0.5f;
 }
 
 // readVoxel (sim.wgsl:74-81): decoded voxel at p, 0 outside the world.

@@ -30,8 +30,12 @@ namespace {
 using px::MarchOut;
 using px::MarchTables;
 using px::key_of;
-using px::step_face;
+using px::load_ray;
 using px::read_voxel;
+using px::shadow_ray;
+using px::step_face;
+using px::store_hit;
+using px::store_not_cast;
 
 // traceSolid() of sim.wgsl:296-357.  steps = brick + voxel iterations, t = the
 // voxel-level t of the hit.
@@ -161,33 +165,6 @@ PX_DEV MarchOut segment(const PxWorldView& w, const float pos[3], const float mv
   return o;
 }
 
-PX_DEV void store_hit(px_hit* out, const MarchOut& m) {
-  uint4 v;
-  v.x = m.key;
-  v.y = __float_as_uint(m.t);
-  v.z = m.steps;
-  v.w = (m.face & 0xffu) | ((m.value & 0xffu) << 8) | (((m.value >> 8) & 0xffu) << 16);
-  *reinterpret_cast<uint4*>(out) = v;
-}
-
-PX_DEV void store_not_cast(px_hit* out) {
-  uint4 v;
-  v.x = PX_NONE;
-  v.y = 0u;
-  v.z = 0u;
-  v.w = 0xFEu;
-  *reinterpret_cast<uint4*>(out) = v;
-}
-
-PX_DEV void load_ray(const px_ray* r, float ro[3], float rd[3], float& maxT, uint32_t& flags) {
-  const float4* p = reinterpret_cast<const float4*>(r);
-  float4 a = p[0], b = p[1];
-  ro[0] = a.x; ro[1] = a.y; ro[2] = a.z;
-  rd[0] = a.w; rd[1] = b.x; rd[2] = b.y;
-  maxT = b.z;
-  flags = __float_as_uint(b.w);
-}
-
 template <int KIND, bool SUP_LDS>
 __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace(PxWorldView w, const px_ray* __restrict__ rays, uint32_t n,
                                                px_hit* __restrict__ out) {
@@ -230,44 +207,6 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_perm(PxWorldView w
   store_hit(out + i, m);
 }
 
-// Shadow ray `s` of shade() (raymarch.wgsl:323-356) for a primary hit at
-// t / face: slot 0 the sun, slot 1 + k light k.  Returns whether it is cast;
-// origin P + N 0.5, direction and max t are written either way.
-PX_DEV bool shadow_ray(const PxWorldView& w, const PxShadowDesc& sd, const float ro[3], const float rd[3], float t,
-                       uint32_t face, uint32_t value, uint32_t s, float O[3], float dir[3], float& mt) {
-  float N[3] = {0.0f, 0.0f, 0.0f};
-  if (face < 6u) N[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
-  float P[3];
-  for (int a = 0; a < 3; a++) {
-    P[a] = ro[a] + rd[a] * t;
-    O[a] = P[a] + N[a] * 0.5f;
-  }
-  dir[0] = dir[1] = dir[2] = 0.0f;
-  mt = 0.0f;
-  if (face == 0xFFu || w.mats[(value & 0xffu) * 20u + 3u] > 0.0f) return false;  // miss or emissive
-  if (s == 0u) {
-    const float ndl = N[0] * sd.sun[0] + N[1] * sd.sun[1] + N[2] * sd.sun[2];
-    dir[0] = sd.sun[0];
-    dir[1] = sd.sun[1];
-    dir[2] = sd.sun[2];
-    mt = 1e9f;
-    return ndl > 0.0f && sd.sun_w > 0.0f;
-  }
-  const float* lp = sd.lp[s - 1u];
-  float L[3];
-  for (int a = 0; a < 3; a++) L[a] = lp[a] / sd.vs - P[a];
-  const float radius = lp[3] / sd.vs;
-  const float d = sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
-  if (d > radius) return false;
-  for (int a = 0; a < 3; a++) L[a] = L[a] / d;
-  const float nl = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
-  dir[0] = L[0];
-  dir[1] = L[1];
-  dir[2] = L[2];
-  mt = d - 2.0f;
-  return nl > 0.0f && sd.lc_w[s - 1u] > 0.5f && sd.lamp_shadows > 0.5f;
-}
-
 // Primary march + the decision of which shadow rays to cast.  Wavefront `gw`
 // owns the queue region [gw * 64 * slots, (gw + 1) * 64 * slots): the output
 // slot (pixel * slots + s) of each cast ray, packed there slot by slot (lanes
@@ -305,7 +244,8 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_frame_primary(PxWorldVie
     if (cast) wq[wcount + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = i * sd.slots + s;
     wcount += (uint32_t)__popcll(mask);
   }
-  if (lane == 0u) wave_count[gwave] = wcount;
+  // the last block may hold wavefronts past the last region (no live lanes)
+  if (lane == 0u && gwave < (n + 63u) / 64u) wave_count[gwave] = wcount;
 }
 
 // One wavefront per queue region of k_frame_primary.  It first writes the

@@ -240,6 +240,28 @@ class Simulation:
                                                      C.byref(nc) if count else None))
         return int(nc.value) if count else None
 
+    # -- shading to an image (fsMain, raymarch.wgsl:387-450) ------------------
+    def render(self, rays: np.ndarray) -> np.ndarray:
+        """RGBA float colour of every ray (host buffers, blocking)."""
+        rays = np.ascontiguousarray(rays, N.RAY_DTYPE)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        if rays.shape[0]:
+            self.gw.check(self.lib.px_render(self.gw.ctx, N.ptr(rays), rays.shape[0], N.ptr(out)))
+        return out
+
+    def render_device(self, rays_t, n: int, rgba_t) -> None:
+        """Device variant: rays_t holds n px_ray records, rgba_t 4 floats per ray."""
+        self.gw.after_torch()
+        self.gw.check(self.lib.px_render_frame_device(self.gw.ctx, C.c_void_p(_devptr(rays_t)), n,
+                                                      C.c_void_p(_devptr(rgba_t))))
+
+    def read_density(self) -> np.ndarray:
+        """The AO density texture, one byte per brick cell (x fastest)."""
+        bd = [d // 8 for d in self.gw.dims]
+        out = np.zeros(bd[0] * bd[1] * bd[2], np.uint8)
+        self.gw.check(self.lib.px_read_density(self.gw.ctx, N.ptr(out)))
+        return out
+
     # -- batched queries (one persistent launch, replayed as a HIP graph) ----
     @staticmethod
     def query_set(rays_by_kind: dict, out_by_kind: dict) -> N.PxQuerySet:
