@@ -69,6 +69,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned ray-cost binning leg")
     ap.add_argument("--no-debris", action="store_true", help="skip the debris-particle leg")
     ap.add_argument("--no-scene", action="store_true", help="skip the GPU scene-build leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the shading-to-an-image leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is per-GPU work (rays shard with the brick map "
@@ -176,6 +177,7 @@ def main() -> int:
         log("bit-exact vs oracle: %s" % parity)
 
     debris_res = None if args.no_debris else debris_leg(args, scene, gw, u, rays, prim)
+    render_res = None if args.no_render else render_leg(args, scene, gw, u, lights, d_rays, n)
     gw.close()
     scene_build = None if args.no_scene else scene_leg(args, scene)
     island = island_leg(args)
@@ -206,6 +208,7 @@ def main() -> int:
         "learned_binning": learned,
         "debris": debris_res,
         "scene_build": scene_build,
+        "render": render_res,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -264,6 +267,61 @@ def island_leg(args):
     # extraction and falling chunks continue from the snapshot (after the comparison above)
     res["extraction"] = extraction_leg(args, scene, gw, sim, impacts)
     gw.close()
+    return res
+
+
+def render_leg(args, scene, gw, u, lights, d_rays, n):
+    """fsMain to an image on the full-scale scene: the frame's rays with
+    shading, glass glow, reflections (U.flags.x = 1) and the sky, RGBA floats
+    into HBM; per-kernel times, parity of a pixel sample with the oracle."""
+    import torch
+
+    from pixels_amd import Simulation
+    from pixels_amd import _native as N
+    sim = Simulation(gw)
+    ur = u.copy()
+    ur[20] = 1.0
+    gw.set_uniforms(ur)
+    gw.set_lights(lights)
+    d_rgba = torch.empty(n * 4, dtype=torch.float32, device=torch.device("cuda:0"))
+    torch.cuda.synchronize()
+    for _ in range(max(1, args.warmup)):
+        sim.render_device(d_rays, n, d_rgba)
+    gw.sync()
+    steps = max(3, args.steps // 2)
+    gw.profile_begin()
+    t1 = time.time()
+    for _ in range(steps):
+        sim.render_device(d_rays, n, d_rgba)
+    gw.sync()
+    wall = (time.time() - t1) / steps
+    prof = gw.profile_end()
+    k_ms = {k: (v[0] / steps) for k, v in prof.items() if v[1]}
+    ms = sum(k_ms.get(k, 0.0) for k in (N.PROF_RENDER_PRIMARY, N.PROF_RENDER_SHADOW, N.PROF_SHADE))
+    res = {"ms": round(ms, 4), "wall_ms": round(wall * 1e3, 4), "mpix_s": round(n / (ms * 1e-3) / 1e6, 1),
+           "kernels_ms": {"primary": round(k_ms.get(N.PROF_RENDER_PRIMARY, 0.0), 4),
+                          "shadow": round(k_ms.get(N.PROF_RENDER_SHADOW, 0.0), 4),
+                          "shade": round(k_ms.get(N.PROF_SHADE, 0.0), 4)},
+           "pixels": n, "reflections": True}
+    log("render: %.3f ms/frame (primary %.3f, shadow %.3f, shade %.3f), %.0f Mpixel/s" %
+        (ms, res["kernels_ms"]["primary"], res["kernels_ms"]["shadow"], res["kernels_ms"]["shade"], res["mpix_s"]))
+    if not args.no_cpu:
+        import oracle
+        img = d_rgba.cpu().numpy().reshape(-1, 4)
+        rays = d_rays.cpu().numpy().view(N.RAY_DTYPE)
+        pick = np.arange(n)
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 64)
+        ow = oracle.World.from_scene(scene)
+        dens = ow.density()
+        t1 = time.time()
+        exp = ow.render(ur, lights, rays[pick], threads=threads, density=dens)
+        cpu_s = time.time() - t1
+        res["bit_exact_sample"] = bool(np.array_equal(img[pick].view(np.uint32), exp.view(np.uint32)))
+        res["cpu_oracle"] = {"mpix_s": round(pick.size / cpu_s / 1e6, 4), "threads": threads,
+                             "sample": "the whole frame (%d pixels), %.2f s" % (pick.size, cpu_s)}
+        log("render bit-exact vs oracle on %d pixels: %s; oracle %.3f Mpixel/s on %d threads" %
+            (pick.size, res["bit_exact_sample"], res["cpu_oracle"]["mpix_s"], threads))
+    gw.set_uniforms(u)
     return res
 
 
