@@ -297,14 +297,16 @@ def render_leg(args, scene, gw, u, lights, d_rays, n):
     wall = (time.time() - t1) / steps
     prof = gw.profile_end()
     k_ms = {k: (v[0] / steps) for k, v in prof.items() if v[1]}
-    ms = sum(k_ms.get(k, 0.0) for k in (N.PROF_RENDER_PRIMARY, N.PROF_RENDER_SHADOW, N.PROF_SHADE))
+    ms = sum(k_ms.get(k, 0.0) for k in (N.PROF_RENDER_PRIMARY, N.PROF_RENDER_SHADOW, N.PROF_REFLECT, N.PROF_SHADE))
     res = {"ms": round(ms, 4), "wall_ms": round(wall * 1e3, 4), "mpix_s": round(n / (ms * 1e-3) / 1e6, 1),
            "kernels_ms": {"primary": round(k_ms.get(N.PROF_RENDER_PRIMARY, 0.0), 4),
                           "shadow": round(k_ms.get(N.PROF_RENDER_SHADOW, 0.0), 4),
+                          "reflect": round(k_ms.get(N.PROF_REFLECT, 0.0), 4),
                           "shade": round(k_ms.get(N.PROF_SHADE, 0.0), 4)},
            "pixels": n, "reflections": True}
-    log("render: %.3f ms/frame (primary %.3f, shadow %.3f, shade %.3f), %.0f Mpixel/s" %
-        (ms, res["kernels_ms"]["primary"], res["kernels_ms"]["shadow"], res["kernels_ms"]["shade"], res["mpix_s"]))
+    log("render: %.3f ms/frame (primary %.3f, shadow %.3f, reflect %.3f, shade %.3f), %.0f Mpixel/s" %
+        (ms, res["kernels_ms"]["primary"], res["kernels_ms"]["shadow"], res["kernels_ms"]["reflect"],
+         res["kernels_ms"]["shade"], res["mpix_s"]))
     if not args.no_cpu:
         import oracle
         img = d_rgba.cpu().numpy().reshape(-1, 4)

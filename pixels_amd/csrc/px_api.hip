@@ -88,7 +88,7 @@ static void release(px_ctx* c) {
   if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
   void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
                   c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_shadow_queue, c->d_wave_count, c->d_counters,
-                  c->d_batch_cursors, c->d_density, c->d_rprim, c->d_rtg, c->d_rsh,
+                  c->d_batch_cursors, c->d_density, c->d_rprim, c->d_rtg, c->d_rsh, c->d_rrq, c->d_rrc,
                   c->b_grid, c->b_pool, c->b_super, c->b_ctrl, c->b_free};
   for (void* p : bufs)
     if (p) hipFree(p);

@@ -143,6 +143,10 @@ struct px_ctx {
   size_t rtg_cap = 0;
   float4* d_rsh = nullptr;
   size_t rsh_cap = 0;
+  uint32_t* d_rrq = nullptr;  // reflection queue: 64 entries per region, then one count per region
+  size_t rrq_cap = 0;
+  float4* d_rrc = nullptr;    // reflection colour per pixel
+  size_t rrc_cap = 0;
   // device-side checkpoint (px_snapshot / px_restore)
   uint32_t* b_grid = nullptr;
   uint32_t* b_pool = nullptr;

@@ -8,13 +8,18 @@
 // ray.  Here the frame is split so each march kernel runs coherent work:
 //   k_density         AO density per brick cell (VoxelWorld.ts:159-170, the
 //                     maint rule of sim.wgsl:1255-1271), rebuilt after edits;
-//   k_render_primary  primary march with transmittance and glow, and the
+//   k_render_primary  primary march with transmittance and glow, the
 //                     per-wavefront queue of the shadow rays shade() casts
-//                     (the same queue as the traversal frame);
+//                     (the same queue as the traversal frame) and of the
+//                     reflection rays;
 //   k_render_shadow   the queued shadow rays: transmittance and hit per slot;
+//   k_render_reflect  the reflection rays of reflective pixels, compacted in
+//                     tile order into dense wavefronts (ordered scan of the
+//                     per-region counts), marched with glow and shaded at
+//                     quality 0;
 //   k_shade           per pixel: material colour, coarse + fine AO, sun and
 //                     lights with the queued shadow results, hemisphere
-//                     ambient, the reflection march and its quality-0 shade,
+//                     ambient, the reflection mixed in by its Fresnel term,
 //                     the sky for misses;
 //   k_render_debug    debug views 1-9, each pixel marched in the reference's
 //                     order (their step counters count every march).
@@ -23,6 +28,7 @@
 
 #include "px_internal.h"
 #include "px_march.h"
+#include "px_scan.h"
 #include "px_shade.h"
 
 #pragma clang fp contract(off)
@@ -210,38 +216,18 @@ PX_DEV float3 shade_px(const PxWorldView& w, const MarchTables& T, const PxShade
   return make_float3(out[0], out[1], out[2]);
 }
 
-// The reflection of fsMain (raymarch.wgsl:429-440) mixed into `base`.
-PX_DEV void reflect_px(const PxWorldView& w, const MarchTables& T, const PxShadeDesc& S,
-                       const uint8_t* __restrict__ dens, const HitS& h, const float ro[3], const float rd[3],
-                       float base[3]) {
-  const float refl = w.mats[(h.value & 0xffu) * 20u + 12u];
-  if (!(refl > 0.0f && S.flags[0] > 0.5f)) return;
-  float P[3], O[3], R[3], nrd[3];
+// The reflection ray of fsMain (raymarch.wgsl:429-440) for a hit on a
+// reflective material: origin P + N 0.5 and direction reflect(rd, N).
+PX_DEV void reflection_ray(const HitS& h, const float ro[3], const float rd[3], float O[3], float R[3]) {
+  float P[3];
   for (int a = 0; a < 3; a++) P[a] = ro[a] + rd[a] * h.t;
   const float dn = px::dot3f(h.N, rd);
   for (int a = 0; a < 3; a++) R[a] = rd[a] - 2.0f * dn * h.N[a];
-  for (int a = 0; a < 3; a++) nrd[a] = -rd[a];
-  const float cosT = px::maxf(px::dot3f(nrd, h.N), 0.0f);
-  const float F = refl + (1.0f - refl) * px_pow5(1.0f - cosT) * refl * 2.0f;
   for (int a = 0; a < 3; a++) O[a] = P[a] + h.N[a] * 0.5f;
-  MarchShade ms;
-  const MarchOut m = px::march<false, true>(w, T, O, R, 1e9f, &S.sky, &ms);
-  float rc[3];
-  if (m.face != 0xFFu) {
-    const HitS rh = decode_march(w, m);
-    uint32_t o = 0, i = 0;
-    const float3 rs = shade_px<0>(w, T, S, dens, rh, O, R, nullptr, &o, &i);
-    rc[0] = ms.trans[0] * rs.x + ms.glow[0];
-    rc[1] = ms.trans[1] * rs.y + ms.glow[1];
-    rc[2] = ms.trans[2] * rs.z + ms.glow[2];
-  } else {
-    const float3 sk = px::sky_color(S.sky, make_float3(R[0], R[1], R[2]));
-    rc[0] = ms.trans[0] * sk.x + ms.glow[0];
-    rc[1] = ms.trans[1] * sk.y + ms.glow[1];
-    rc[2] = ms.trans[2] * sk.z + ms.glow[2];
-  }
-  const float fm = px::clampf(F, 0.0f, 0.9f);
-  for (int a = 0; a < 3; a++) base[a] = px::mixf(base[a], rc[a], fm);
+}
+
+PX_DEV bool reflects(const PxWorldView& w, const PxShadeDesc& S, const HitS& h) {
+  return h.hit && w.mats[(h.value & 0xffu) * 20u + 12u] > 0.0f && S.flags[0] > 0.5f;
 }
 
 PX_DEV void store_rgba(float* __restrict__ rgba, uint32_t i, const float c[3]) {
@@ -252,11 +238,13 @@ PX_DEV void store_rgba(float* __restrict__ rgba, uint32_t i, const float c[3]) {
 // Primary march with transmittance and glow (tg: 2 float4 per pixel) and the
 // shadow-ray queue of the traversal frame (per-wavefront regions).
 template <bool SUP_LDS>
-__global__ __launch_bounds__(256) void k_render_primary(PxWorldView w, PxShadowDesc sd, PxSky sky,
+__global__ __launch_bounds__(256) void k_render_primary(PxWorldView w, PxShadowDesc sd, PxSky sky, float refl_on,
                                                         const px_ray* __restrict__ rays, uint32_t n,
                                                         px_hit* __restrict__ prim_out, float4* __restrict__ tg,
                                                         uint32_t* __restrict__ queue,
-                                                        uint32_t* __restrict__ wave_count) {
+                                                        uint32_t* __restrict__ wave_count,
+                                                        uint32_t* __restrict__ rqueue,
+                                                        uint2* __restrict__ rcount) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,8 +274,62 @@ __global__ __launch_bounds__(256) void k_render_primary(PxWorldView w, PxShadowD
     if (cast) wq[wcount + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = i * sd.slots + s;
     wcount += (uint32_t)__popcll(mask);
   }
+  // reflection rays of the region's pixels, packed by ballot prefix
+  const bool refl = live && h.face != 0xFFu && w.mats[(h.value & 0xffu) * 20u + 12u] > 0.0f && refl_on > 0.5f;
+  const uint64_t rmask = __ballot(refl);
+  if (refl) rqueue[(size_t)gwave * 64u + (uint32_t)__popcll(rmask & ((1ull << lane) - 1ull))] = i;
   // the last block may hold wavefronts past the last region (no live lanes)
-  if (lane == 0u && gwave < (n + 63u) / 64u) wave_count[gwave] = wcount;
+  if (lane == 0u && gwave < (n + 63u) / 64u) {
+    wave_count[gwave] = wcount;
+    rcount[gwave] = make_uint2((uint32_t)__popcll(rmask), 0u);
+  }
+}
+
+// The reflection rays of all regions in region order: region r's entries go
+// to offset roff[r] (ordered scan of the counts).
+__global__ __launch_bounds__(256) void k_refl_compact(const uint32_t* __restrict__ rqueue,
+                                                      const uint2* __restrict__ rcount,
+                                                      const uint2* __restrict__ roff, uint32_t n_regions,
+                                                      uint32_t* __restrict__ dense) {
+  const uint32_t region = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (region >= n_regions) return;
+  const uint32_t lane = __lane_id();
+  if (lane < rcount[region].x) dense[roff[region].x + lane] = rqueue[(size_t)region * 64u + lane];
+}
+
+// One lane per queued reflection ray (dense list, *total of them): marched
+// with glow and shaded at quality 0; rc[pixel] = trans * colour + glow (or
+// the sky).
+template <bool SUP_LDS>
+__global__ __launch_bounds__(256) void k_render_reflect(PxWorldView w, PxShadeDesc S, const px_ray* __restrict__ rays,
+                                                        const px_hit* __restrict__ prim,
+                                                        const uint32_t* __restrict__ dense,
+                                                        const uint32_t* __restrict__ total,
+                                                        const uint8_t* __restrict__ dens, float4* __restrict__ rc) {
+  extern __shared__ uint32_t s_mem[];
+  const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= *total) return;
+  const uint32_t i = dense[j];
+  float ro[3], rd[3], maxT;
+  uint32_t flags;
+  px::load_ray(rays + i, ro, rd, maxT, flags);
+  const uint4 hv = *reinterpret_cast<const uint4*>(prim + i);
+  const HitS h = decode_hit(w, hv.x, __uint_as_float(hv.y), hv.w & 0xffu, (hv.w >> 8) & 0xffffu);
+  float O[3], R[3];
+  reflection_ray(h, ro, rd, O, R);
+  MarchShade ms;
+  const MarchOut m = px::march<false, true>(w, T, O, R, 1e9f, &S.sky, &ms);
+  float3 c;
+  if (m.face != 0xFFu) {
+    const HitS rh = decode_march(w, m);
+    uint32_t o = 0, in = 0;
+    c = shade_px<0>(w, T, S, dens, rh, O, R, nullptr, &o, &in);
+  } else {
+    c = px::sky_color(S.sky, make_float3(R[0], R[1], R[2]));
+  }
+  rc[i] = make_float4(ms.trans[0] * c.x + ms.glow[0], ms.trans[1] * c.y + ms.glow[1], ms.trans[2] * c.z + ms.glow[2],
+                      0.0f);
 }
 
 // One wavefront per queue region: every slot of its pixels first reads
@@ -331,8 +373,8 @@ template <bool SUP_LDS>
 __global__ __launch_bounds__(256) void k_shade(PxWorldView w, PxShadeDesc S, uint32_t slots,
                                                const px_ray* __restrict__ rays, uint32_t n,
                                                const px_hit* __restrict__ prim, const float4* __restrict__ tg,
-                                               const float4* __restrict__ sh, const uint8_t* __restrict__ dens,
-                                               float* __restrict__ rgba) {
+                                               const float4* __restrict__ sh, const float4* __restrict__ rc,
+                                               const uint8_t* __restrict__ dens, float* __restrict__ rgba) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,7 +396,16 @@ __global__ __launch_bounds__(256) void k_shade(PxWorldView w, PxShadeDesc S, uin
     uint32_t o = 0, in = 0;
     const float3 b = shade_px<1>(w, T, S, dens, h, ro, rd, sh + (size_t)i * slots, &o, &in);
     float base[3] = {b.x, b.y, b.z};
-    reflect_px(w, T, S, dens, h, ro, rd, base);
+    if (reflects(w, S, h)) {  // mix(base, rc, clamp(F, 0, 0.9)), raymarch.wgsl:431-440
+      const float refl = w.mats[(h.value & 0xffu) * 20u + 12u];
+      const float nrd[3] = {-rd[0], -rd[1], -rd[2]};
+      const float cosT = px::maxf(px::dot3f(nrd, h.N), 0.0f);
+      const float F = refl + (1.0f - refl) * px_pow5(1.0f - cosT) * refl * 2.0f;
+      const float fm = px::clampf(F, 0.0f, 0.9f);
+      const float4 r = rc[i];
+      const float rv[3] = {r.x, r.y, r.z};
+      for (int a = 0; a < 3; a++) base[a] = px::mixf(base[a], rv[a], fm);
+    }
     for (int a = 0; a < 3; a++) c[a] = trans[a] * base[a] + glow[a];
   }
   store_rgba(rgba, i, c);
@@ -520,14 +571,27 @@ int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba)
   if ((rc = grow(c, &c->d_rprim, &c->rprim_cap, (size_t)n, "render hits"))) return rc;
   if ((rc = grow(c, &c->d_rtg, &c->rtg_cap, (size_t)n * 2u, "render transmittance"))) return rc;
   if ((rc = grow(c, &c->d_rsh, &c->rsh_cap, qneed, "render shadow results"))) return rc;
+  // reflection queue (u32 words): 64 entries per region, per-region counts
+  // and their offsets (uint2), scan tiles, the total, the dense list
+  const size_t ntile = ((size_t)n_regions + kScanTile - 1u) / kScanTile + 1u;
+  const size_t o_cnt = (size_t)n_regions * 64u, o_off = o_cnt + 2u * n_regions, o_tiles = o_off + 2u * n_regions;
+  const size_t o_tot = o_tiles + 2u * ntile, o_dense = o_tot + 4u;
+  if ((rc = grow(c, &c->d_rrq, &c->rrq_cap, o_dense + (size_t)n, "reflection queue"))) return rc;
+  if ((rc = grow(c, &c->d_rrc, &c->rrc_cap, (size_t)n, "reflection colours"))) return rc;
+  uint32_t* rqueue = c->d_rrq;
+  uint2* rcount = reinterpret_cast<uint2*>(c->d_rrq + o_cnt);
+  uint2* roff = reinterpret_cast<uint2*>(c->d_rrq + o_off);
+  uint2* rtiles = reinterpret_cast<uint2*>(c->d_rrq + o_tiles);
+  uint32_t* rtot = c->d_rrq + o_tot;
+  uint32_t* rdense = c->d_rrq + o_dense;
   {
     PxProfScope prof(c, PX_PROF_RENDER_PRIMARY);
     if (sup_lds)
-      hipLaunchKernelGGL(k_render_primary<true>, grid, block, lds, c->stream, w, c->shadow, S.sky, d_rays, n,
-                         c->d_rprim, c->d_rtg, c->d_shadow_queue, c->d_wave_count);
+      hipLaunchKernelGGL(k_render_primary<true>, grid, block, lds, c->stream, w, c->shadow, S.sky, S.flags[0],
+                         d_rays, n, c->d_rprim, c->d_rtg, c->d_shadow_queue, c->d_wave_count, rqueue, rcount);
     else
-      hipLaunchKernelGGL(k_render_primary<false>, grid, block, lds, c->stream, w, c->shadow, S.sky, d_rays, n,
-                         c->d_rprim, c->d_rtg, c->d_shadow_queue, c->d_wave_count);
+      hipLaunchKernelGGL(k_render_primary<false>, grid, block, lds, c->stream, w, c->shadow, S.sky, S.flags[0],
+                         d_rays, n, c->d_rprim, c->d_rtg, c->d_shadow_queue, c->d_wave_count, rqueue, rcount);
   }
   PX_HIP_CHECK(c, hipGetLastError());
   {
@@ -542,13 +606,26 @@ int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba)
   }
   PX_HIP_CHECK(c, hipGetLastError());
   {
+    PxProfScope prof(c, PX_PROF_REFLECT);
+    scan2_launch(c->stream, rcount, roff, rtiles, n_regions, nullptr, n_regions, nullptr, nullptr, rtot);
+    hipLaunchKernelGGL(k_refl_compact, dim3((n_regions + 3u) / 4u), dim3(256), 0, c->stream, rqueue, rcount, roff,
+                       n_regions, rdense);
+    if (sup_lds)
+      hipLaunchKernelGGL(k_render_reflect<true>, grid, block, lds, c->stream, w, S, d_rays, c->d_rprim, rdense, rtot,
+                         c->d_density, c->d_rrc);
+    else
+      hipLaunchKernelGGL(k_render_reflect<false>, grid, block, lds, c->stream, w, S, d_rays, c->d_rprim, rdense, rtot,
+                         c->d_density, c->d_rrc);
+  }
+  PX_HIP_CHECK(c, hipGetLastError());
+  {
     PxProfScope prof(c, PX_PROF_SHADE);
     if (sup_lds)
       hipLaunchKernelGGL(k_shade<true>, grid, block, lds, c->stream, w, S, slots, d_rays, n, c->d_rprim, c->d_rtg,
-                         c->d_rsh, c->d_density, d_rgba);
+                         c->d_rsh, c->d_rrc, c->d_density, d_rgba);
     else
       hipLaunchKernelGGL(k_shade<false>, grid, block, lds, c->stream, w, S, slots, d_rays, n, c->d_rprim, c->d_rtg,
-                         c->d_rsh, c->d_density, d_rgba);
+                         c->d_rsh, c->d_rrc, c->d_density, d_rgba);
   }
   PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
