@@ -10,6 +10,10 @@ const path = require('path');
 const addon = require(path.join(__dirname, 'pixels_napi.node'));
 
 const RAY = { PRIMARY: 0, SHADOW: 1, SHOOT: 2, SEGMENT: 3, MIXED: 4 };
+// stage bits of Simulation.encode (include/pixels.h PX_STAGE_*), run in the
+// reference's order: frame begin, edit, resolve, chunks, particles, writes
+const STAGE = { EDIT: 1, PARTICLES: 2, FRAME_BEGIN: 4, WRITES: 8, RESOLVE: 16, CHUNKS: 32 };
+const STAGE_ALL = 63;
 const FACE = { ZERO: 6, NOT_CAST: 0xfe, MISS: 0xff };
 // CTRL_NAMES order, src/gpu/layout.ts:48-58
 const CTRL_NAMES = [
@@ -55,6 +59,25 @@ class GpuWorld {
     this.uniform = new Float32Array(52);
   }
 
+  /**
+   * The scene rasterised on the GPU (apply() of shapes.ts:334-414 for every
+   * primitive in order): prims is an ArrayBuffer of 168-byte px_prim records.
+   */
+  static fromPrims(device, dims, prims, mats, options) {
+    const opt = options || {};
+    const gw = Object.create(GpuWorld.prototype);
+    gw.ctx = addon.createFromPrims({ dims: dims, mats: mats, poolCapacity: opt.poolCapacity || 0,
+      voxelSize: opt.voxelSize || 1 / 80, device: device || 0 }, prims);
+    gw.dims = { x: dims[0], y: dims[1], z: dims[2] };
+    gw.bricks = { x: dims[0] >> 3, y: dims[1] >> 3, z: dims[2] >> 3 };
+    gw.supers = { x: Math.ceil(gw.bricks.x / 4), y: Math.ceil(gw.bricks.y / 4), z: Math.ceil(gw.bricks.z / 4) };
+    gw.numCells = gw.bricks.x * gw.bricks.y * gw.bricks.z;
+    gw.initialBricks = 0;
+    gw.poolCapacity = opt.poolCapacity || 0;  // 0: sized by the library from the bricks the scene needs
+    gw.uniform = new Float32Array(52);
+    return gw;
+  }
+
   /** Uniform block (shared.wgsl:5-19), written once per frame. */
   writeUniforms(u) { addon.setUniforms(this.ctx, u || this.uniform); }
 
@@ -88,11 +111,37 @@ class Simulation {
     this.pendingImpacts += impacts.length / 16;
   }
 
-  /** The hot-path stages of one frame: the edit chain run to quiescence. */
-  encode() {
-    addon.islandPass(this.gw.ctx);
+  /**
+   * The stages of one frame (Simulation.encode, Simulation.ts:200-273).
+   * Without a mask: the edit chain run to quiescence; with a mask of STAGE
+   * bits: those stages in the reference's order.
+   */
+  encode(stages) {
+    if (stages === undefined) addon.islandPass(this.gw.ctx);
+    else addon.simStep(this.gw.ctx, stages);
     this.pendingImpacts = 0;
   }
+
+  /** Debris particle ring of maxParticles slots (0 = MAX_PARTICLES, config.ts:38). */
+  particlesInit(maxParticles) {
+    addon.particlesInit(this.gw.ctx, maxParticles || 0);
+    this.maxParticles = maxParticles || 262144;
+  }
+
+  /** spawn() (sim.wgsl:166-170) of 32-byte particle records, in order. */
+  spawn(records) { addon.spawn(this.gw.ctx, records); }
+
+  /** Ring slots [first, first + count) as 32-byte records. */
+  readParticles(first, count) { return addon.readParticles(this.gw.ctx, first || 0, count || this.maxParticles); }
+
+  /** The frame's alive list (the particle draw list). */
+  readAlive() { return addon.readAlive(this.gw.ctx, this.maxParticles || 262144); }
+
+  /** pushWrite (sim.wgsl:186-192) of (key, value) pairs. */
+  pushWrites(pairs) { addon.pushWrites(this.gw.ctx, pairs); }
+
+  /** Chunk table (16 words per chunk) and arena (key, value | chunk << 16). */
+  readChunks(arenaCap) { return addon.readChunks(this.gw.ctx, arenaCap); }
 
   /** Unsupported components of the last pass: {keys, counts}, sorted by key. */
   islandLabels() { return addon.islandLabels(this.gw.ctx); }
@@ -120,4 +169,15 @@ class Simulation {
   }
 }
 
-module.exports = { GpuWorld, Simulation, RAY, FACE, CTRL, CTRL_NAMES, addon };
+/** fsMain (raymarch.wgsl:387-450) over explicit camera rays. */
+class Renderer {
+  constructor(gw) { this.gw = gw; }
+
+  /** RGBA float colour of every 32-byte ray record (debug views by U.flags.w). */
+  render(rays) { return addon.render(this.gw.ctx, rays); }
+
+  /** The AO density texture, one byte per brick cell. */
+  readDensity() { return addon.readDensity(this.gw.ctx, this.gw.numCells); }
+}
+
+module.exports = { GpuWorld, Simulation, Renderer, RAY, FACE, STAGE, STAGE_ALL, CTRL, CTRL_NAMES, addon };

@@ -285,6 +285,190 @@ napi_value IslandLabels(napi_env env, napi_callback_info info) {
   return obj;
 }
 
+// createFromPrims({dims, mats, poolCapacity, voxelSize, device}, prims: 168-byte
+// px_prim records) -> context: the scene rasterised on the GPU (shapes.ts:334-414)
+napi_value CreateFromPrims(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  napi_value d = argv[0];
+  px_world_desc desc;
+  std::memset(&desc, 0, sizeof desc);
+  napi_value dims = prop(env, d, "dims");
+  for (uint32_t a = 0; a < 3; a++) {
+    napi_value e;
+    napi_get_element(env, dims, a, &e);
+    desc.dims[a] = get_u32(env, e, 0);
+  }
+  void* p = nullptr;
+  size_t nb = 0;
+  if (!get_bytes(env, prop(env, d, "mats"), &p, &nb) || nb < PX_MAT_COUNT * PX_MAT_FLOATS * 4) {
+    napi_throw_type_error(env, nullptr, "mats: Float32Array(256*20)");
+    return nullptr;
+  }
+  desc.mats = static_cast<const float*>(p);
+  desc.pool_capacity = get_u32(env, prop(env, d, "poolCapacity"), 0);
+  desc.voxel_size = (float)get_f64(env, prop(env, d, "voxelSize"), 1.0 / 80.0);
+  desc.device = (int32_t)get_u32(env, prop(env, d, "device"), 0);
+  void* pp = nullptr;
+  size_t pb = 0;
+  if (!get_bytes(env, argv[1], &pp, &pb) || pb % sizeof(px_prim)) {
+    napi_throw_type_error(env, nullptr, "prims: 168-byte px_prim records");
+    return nullptr;
+  }
+  px_ctx* ctx = nullptr;
+  int rc = px_create_from_prims(&desc, static_cast<const px_prim*>(pp), (uint32_t)(pb / sizeof(px_prim)), &ctx);
+  if (rc != PX_OK) return throw_px(env, nullptr, rc);
+  napi_value out;
+  NAPI_CALL(env, napi_create_external(env, ctx, nullptr, nullptr, &out));
+  return out;
+}
+
+// simStep(ctx, stagesMask): Simulation.encode's stages in the reference's order
+napi_value SimStep(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = px_sim_step(c, get_u32(env, argv[1], 0));
+  if (rc == PX_OK) rc = px_sync(c);
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+napi_value ParticlesInit(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = px_particles_init(c, get_u32(env, argv[1], 0));
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+// spawn(ctx, records: 32-byte particle records)
+napi_value Spawn(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[1], &p, &nb) || nb % sizeof(px_particle)) {
+    napi_throw_type_error(env, nullptr, "particles: 32-byte records");
+    return nullptr;
+  }
+  int rc = px_spawn(c, static_cast<const px_particle*>(p), (uint32_t)(nb / sizeof(px_particle)));
+  if (rc == PX_OK) rc = px_sync(c);
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+// readParticles(ctx, first, count) -> Uint8Array of 32-byte records
+napi_value ReadParticles(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  const uint32_t first = get_u32(env, argv[1], 0), count = get_u32(env, argv[2], 0);
+  void* outp = nullptr;
+  napi_value out = new_typed(env, napi_uint8_array, (size_t)count * sizeof(px_particle), 1, &outp);
+  if (!out) return nullptr;
+  int rc = px_read_particles(c, first, count, static_cast<px_particle*>(outp));
+  return rc ? throw_px(env, c, rc) : out;
+}
+
+// readAlive(ctx, cap) -> Uint32Array of ring slots (the particle draw list)
+napi_value ReadAlive(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  const uint32_t cap = get_u32(env, argv[1], 0);
+  std::vector<uint32_t> buf(cap ? cap : 1);
+  uint32_t n = 0;
+  int rc = px_read_alive(c, buf.data(), cap, &n);
+  if (rc) return throw_px(env, c, rc);
+  const uint32_t m = n < cap ? n : cap;
+  void* outp = nullptr;
+  napi_value out = new_typed(env, napi_uint32_array, m, 4, &outp);
+  if (!out) return nullptr;
+  if (m) std::memcpy(outp, buf.data(), (size_t)m * 4);
+  return out;
+}
+
+// pushWrites(ctx, pairs: Uint32Array of (key, value))
+napi_value PushWrites(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[1], &p, &nb) || nb % 8) {
+    napi_throw_type_error(env, nullptr, "writes: Uint32Array of (key, value) pairs");
+    return nullptr;
+  }
+  int rc = px_push_writes(c, static_cast<const uint32_t*>(p), (uint32_t)(nb / 8));
+  return rc ? throw_px(env, c, rc) : nullptr;
+}
+
+// readChunks(ctx, arenaCap) -> {table: Uint32Array(maxChunks * 16), arena: Uint32Array}
+napi_value ReadChunks(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  px_limits lim;
+  px_default_limits(&lim);
+  const uint32_t cap = get_u32(env, argv[1], lim.max_arena);
+  void *pt, *pa;
+  napi_value table = new_typed(env, napi_uint32_array, (size_t)lim.max_chunks * 16u, 4, &pt);
+  std::vector<uint32_t> arena((size_t)cap * 2u + 2u);
+  uint32_t n = 0;
+  if (!table) return nullptr;
+  int rc = px_read_chunks(c, static_cast<uint32_t*>(pt), arena.data(), cap, &n);
+  if (rc) return throw_px(env, c, rc);
+  const uint32_t m = n < cap ? n : cap;
+  napi_value ar = new_typed(env, napi_uint32_array, (size_t)m * 2u, 4, &pa);
+  if (!ar) return nullptr;
+  if (m) std::memcpy(pa, arena.data(), (size_t)m * 8u);
+  napi_value obj;
+  NAPI_CALL(env, napi_create_object(env, &obj));
+  napi_set_named_property(env, obj, "table", table);
+  napi_set_named_property(env, obj, "arena", ar);
+  return obj;
+}
+
+// render(ctx, rays: 32-byte records) -> Float32Array of RGBA per ray (fsMain)
+napi_value Render(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* p;
+  size_t nb;
+  if (!get_bytes(env, argv[1], &p, &nb) || nb % sizeof(px_ray)) {
+    napi_throw_type_error(env, nullptr, "rays: 32-byte records");
+    return nullptr;
+  }
+  const uint32_t n = (uint32_t)(nb / sizeof(px_ray));
+  void* outp = nullptr;
+  napi_value out = new_typed(env, napi_float32_array, (size_t)n * 4u, 4, &outp);
+  if (!out) return nullptr;
+  int rc = px_render(c, static_cast<const px_ray*>(p), n, static_cast<float*>(outp));
+  return rc ? throw_px(env, c, rc) : out;
+}
+
+// readDensity(ctx, cells) -> Uint8Array (the AO density texture)
+napi_value ReadDensity(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  px_ctx* c = get_ctx(env, argv[0]);
+  if (!c) return nullptr;
+  void* outp = nullptr;
+  napi_value out = new_typed(env, napi_uint8_array, get_u32(env, argv[1], 0), 1, &outp);
+  if (!out) return nullptr;
+  int rc = px_read_density(c, static_cast<uint8_t*>(outp));
+  return rc ? throw_px(env, c, rc) : out;
+}
+
 napi_value LastError(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return nullptr;
@@ -303,6 +487,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"create", Create}, {"destroy", Destroy}, {"setUniforms", SetUniforms}, {"setLights", SetLights},
       {"trace", Trace}, {"probe", Probe}, {"readCtrl", ReadCtrl}, {"pushImpacts", PushImpacts},
       {"islandPass", IslandPass}, {"islandLabels", IslandLabels}, {"lastError", LastError},
+      {"createFromPrims", CreateFromPrims}, {"simStep", SimStep}, {"particlesInit", ParticlesInit},
+      {"spawn", Spawn}, {"readParticles", ReadParticles}, {"readAlive", ReadAlive}, {"pushWrites", PushWrites},
+      {"readChunks", ReadChunks}, {"render", Render}, {"readDensity", ReadDensity},
   };
   for (auto& f : fns) {
     napi_value v;

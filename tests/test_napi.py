@@ -19,14 +19,16 @@ needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reaso
 @needs_node
 def test_addon_loads_and_exports():
     js = ("const m = require(%r); console.log(JSON.stringify({fns: Object.keys(m.addon).sort(), "
-          "cls: [typeof m.GpuWorld, typeof m.Simulation], ctrl: m.CTRL.S_DESTROYED}));"
+          "cls: [typeof m.GpuWorld, typeof m.Simulation, typeof m.Renderer], ctrl: m.CTRL.S_DESTROYED}));"
           % os.path.join(ROOT, "pixels_amd", "node", "index.js"))
     r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["fns"] == sorted(["create", "destroy", "setUniforms", "setLights", "trace", "probe", "readCtrl",
-                                 "pushImpacts", "islandPass", "islandLabels", "lastError"])
-    assert out["cls"] == ["function", "function"]
+                                 "pushImpacts", "islandPass", "islandLabels", "lastError", "createFromPrims",
+                                 "simStep", "particlesInit", "spawn", "readParticles", "readAlive", "pushWrites",
+                                 "readChunks", "render", "readDensity"])
+    assert out["cls"] == ["function", "function", "function"]
     assert out["ctrl"] == 25
 
 
@@ -86,3 +88,63 @@ sim.probe([0, 0, 0], [128, 64, 128]).then((vox) => {
     assert np.array_equal(vox, ovox.astype(np.uint32))
     assert out["n"] == on and out["keys"] == okeys.tolist() and out["counts"] == ocnt.tolist()
     assert out["destroyed"] == int(stats[0])
+
+
+@needs_node
+@pytest.mark.gpu
+def test_node_host_scene_build_render_and_debris(tmp_path):
+    """The Node host builds a world from primitives on the GPU, renders it
+    (fsMain colour), and runs a debris frame through the stage mask; each
+    result equals the oracle's."""
+    import oracle
+    from pixels_amd import _native as N
+    from pixels_amd.scene import LARGE, Scene, debris
+    from tests.test_shade_oracle import lights, uniforms
+    s = Scene.build(LARGE, seed=4, dims=(128, 64, 96), n_prims=40)
+    prims = s.prims()
+    rays = s.camera_rays(64, 48, 90.0, seed=2)
+    u = uniforms(s.dims)
+    u[24] = np.float32(1.0 / 60.0)
+    li = lights([((40.0, 30.0, 40.0), 60.0, (1.0, 0.8, 0.6), 1.0)])
+    recs = debris(s.dims, 500, seed=3, speed=80.0)
+    for name, arr in (("prims", prims), ("mats", s.mats), ("rays", rays), ("u", u), ("li", li), ("recs", recs)):
+        np.ascontiguousarray(arr).tofile(str(tmp_path / name))
+    js = r"""
+const fs = require('fs');
+const m = require(%r);
+const d = %r;
+const rd = (n, T) => { const b = fs.readFileSync(d + '/' + n); return new T(b.buffer, b.byteOffset, b.byteLength / T.BYTES_PER_ELEMENT); };
+const gw = m.GpuWorld.fromPrims(0, [128, 64, 96], rd('prims', Uint8Array), rd('mats', Float32Array));
+gw.writeUniforms(rd('u', Float32Array));
+gw.writeLights(rd('li', Float32Array));
+const img = new m.Renderer(gw).render(rd('rays', Uint8Array));
+fs.writeFileSync(d + '/img', Buffer.from(img.buffer));
+const sim = new m.Simulation(gw);
+sim.particlesInit(4096);
+sim.encode(m.STAGE.FRAME_BEGIN);
+sim.spawn(rd('recs', Uint8Array));
+sim.encode(m.STAGE.PARTICLES);
+fs.writeFileSync(d + '/parts', Buffer.from(sim.readParticles(0, 4096).buffer));
+fs.writeFileSync(d + '/alive', Buffer.from(sim.readAlive().buffer));
+sim.probe([0, 0, 0], [128, 64, 96]).then((vox) => {
+  fs.writeFileSync(d + '/vox', Buffer.from(vox.buffer));
+  gw.destroy();
+  console.log('ok');
+});
+""" % (os.path.join(ROOT, "pixels_amd", "node", "index.js"), str(tmp_path))
+    print("case: node host scene build + render + debris", flush=True)
+    r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    vox, _ = oracle.raster(s.dims, prims)
+    got = np.fromfile(str(tmp_path / "vox"), np.uint32).reshape(vox.shape)
+    assert np.array_equal(got, vox.astype(np.uint32))
+    ow = oracle.World.from_scene(Scene.from_dense(vox, s.mats))
+    img = np.fromfile(str(tmp_path / "img"), np.float32).reshape(-1, 4)
+    assert np.array_equal(img.view(np.uint32), ow.render(u, li, rays).view(np.uint32))
+    P = oracle.Particles(4096)
+    P.frame_begin()
+    P.spawn(recs)
+    P.step(ow, np.float32(1.0 / 60.0), np.float32(u[25]), np.float32(u[3]))
+    parts = np.fromfile(str(tmp_path / "parts"), N.PARTICLE_DTYPE)
+    assert parts.tobytes() == P.parts.tobytes()
+    assert np.array_equal(np.fromfile(str(tmp_path / "alive"), np.uint32), P.alive_list())
