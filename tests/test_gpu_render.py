@@ -9,7 +9,9 @@ bit for bit on every RGBA float:
 * the seeded 512x256x512 scene with its frame inputs and reflections on;
 * the AO density texture against the oracle's, before and after an island
   pass, and the render of the edited world;
-* the device entry point against the host one."""
+* the device entry point against the host one;
+* ragged pixel counts (1, 63, 65, 257), no lights and the 32-light maximum,
+  an empty world (all sky) and zero rays."""
 import numpy as np
 import pytest
 
@@ -144,3 +146,51 @@ def test_device_entry_equals_host(gal):
     gw.sync()
     got = d_rgba.cpu().numpy().reshape(-1, 4)
     assert np.array_equal(f32bits(got), f32bits(host))
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 257])
+def test_ragged_counts_match_oracle(gal, n):
+    """Pixel counts that do not fill a wavefront or a workgroup."""
+    s, gw, sim, ow = gal
+    u = uniforms(s.dims)
+    li = lights()
+    rays = panorama(64, 32)[np.random.default_rng(n).permutation(64 * 32)[:n]]
+    print("case: ragged render n=%d" % n, flush=True)
+    gw.set_uniforms(u)
+    gw.set_lights(li)
+    _compare(sim.render(rays), ow.render(u, li, rays), "n=%d" % n)
+
+
+@pytest.mark.parametrize("n_lights", [0, 32])
+def test_light_counts_match_oracle(gal, n_lights):
+    s, gw, sim, ow = gal
+    rng = np.random.default_rng(n_lights)
+    entries = [(tuple(rng.uniform(4, 90, 3)), float(rng.uniform(10, 80)), tuple(rng.uniform(0.2, 1.0, 3)),
+                float(rng.integers(0, 2))) for _ in range(n_lights)]
+    u = uniforms(s.dims)
+    li = lights(entries)
+    rays = panorama(96, 48)
+    print("case: render with %d lights" % n_lights, flush=True)
+    gw.set_uniforms(u)
+    gw.set_lights(li)
+    _compare(sim.render(rays), ow.render(u, li, rays), "%d lights" % n_lights)
+
+
+def test_empty_world_is_sky_and_zero_rays():
+    from pixels_amd.scene import Scene
+    from tests.test_shade_oracle import palette
+    s = Scene.from_dense(np.zeros((32, 32, 32), np.uint16), palette())
+    gw, sim = _open(s)
+    try:
+        u = uniforms(s.dims, night=0.6)
+        gw.set_uniforms(u)
+        gw.set_lights(lights())
+        rays = panorama(64, 32, cam=(16.0, 16.0, 16.0))
+        print("case: empty world", flush=True)
+        got = sim.render(rays)
+        ow = oracle.World.from_scene(s)
+        _compare(got, ow.render(u, lights(), rays), "empty world")
+        assert not sim.read_density().any()
+        assert sim.render(rays[:0]).shape == (0, 4)
+    finally:
+        gw.close()
