@@ -332,8 +332,9 @@ __global__ __launch_bounds__(256) void k_render_reflect(PxWorldView w, PxShadeDe
                       0.0f);
 }
 
-// One wavefront per queue region: every slot of its pixels first reads
-// "not cast", then the queued rays write (transmittance, hit).
+// One wavefront per queue region: the slots no queued ray covers read "not
+// cast" (a bitmap of the queue in LDS), the queued rays write
+// (transmittance, hit).
 template <bool SUP_LDS>
 __global__ __launch_bounds__(256) void k_render_shadow(PxWorldView w, PxShadowDesc sd, const px_ray* __restrict__ rays,
                                                        const px_hit* __restrict__ prim,
@@ -347,11 +348,30 @@ __global__ __launch_bounds__(256) void k_render_shadow(PxWorldView w, PxShadowDe
   if (region >= n_regions) return;
   const uint32_t slots = sd.slots;
   const uint32_t first = region * 64u * slots, last = min(region * 64u + 64u, n) * slots;
-  for (uint32_t k = first + __lane_id(); k < last; k += 64u) sh_out[k] = make_float4(1.0f, 1.0f, 1.0f, kShNotCast);
-  __threadfence_block();
+  const uint32_t lane = __lane_id();
   const uint32_t cnt = wave_count[region];
   const uint32_t* __restrict__ rq = queue + (size_t)region * 64u * slots;
-  for (uint32_t k = __lane_id(); k < cnt; k += 64u) {
+  // bitmap of the region's cast slots (slots <= 33): only the others get the
+  // "not cast" record, so every output is written once
+  __shared__ uint32_t s_cast[4][(64u * 33u) / 32u];
+  uint32_t* bm = s_cast[threadIdx.x >> 6];
+  const uint32_t nwords = (64u * slots + 31u) / 32u;
+  for (uint32_t j = lane; j < nwords; j += 64u) bm[j] = 0u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t k = lane; k < cnt; k += 64u) {
+    const uint32_t local = rq[k] - first;
+    atomicOr(&bm[local >> 5], 1u << (local & 31u));
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t k = first + lane; k < last; k += 64u) {
+    const uint32_t local = k - first;
+    if (((bm[local >> 5] >> (local & 31u)) & 1u) == 0u) sh_out[k] = make_float4(1.0f, 1.0f, 1.0f, kShNotCast);
+  }
+  for (uint32_t k = lane; k < cnt; k += 64u) {
     const uint32_t code = rq[k];
     const uint32_t i = code / slots, s = code - i * slots;
     float ro[3], rd[3], maxT;
