@@ -80,6 +80,7 @@ const char* px_last_error(const px_ctx* ctx) {
 static void release(px_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);  // no queued kernel still uses what is freed
   for (auto& r : c->prof_rec) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);

@@ -181,6 +181,13 @@ struct PxProfScope {
 };
 PxWorldView px_view(const px_ctx* c);
 int px_refresh_masks(px_ctx* c);
+// Dynamic LDS of march()'s tables: 8 glass words, then the super-cell bits
+// when they fit in 64 KB (*sup_lds; else the kernels read the global copy).
+inline size_t px_table_lds(const px_ctx* c, bool* sup_lds) {
+  const size_t full = (8u + (c->nsuper + 31u) / 32u) * 4u;
+  *sup_lds = full <= 64u * 1024u;
+  return *sup_lds ? full : 32u;
+}
 
 // launchers (px_trace.hip)
 int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, px_hit* d_out);
@@ -205,6 +212,7 @@ int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_
 // px_raster_finish writes the grid, pool and super grid of the context,
 // whose buffers exist by then.
 struct PxRaster {
+  hipStream_t stream = nullptr;  // the stream its kernels run on: drained before the frees
   px_prim* d_prims = nullptr;
   uint64_t* d_pairs = nullptr;
   uint64_t* d_sorted = nullptr;
@@ -219,6 +227,7 @@ struct PxRaster {
   uint2* d_tiles = nullptr;
   uint32_t* d_tot = nullptr;
   ~PxRaster() {
+    if (stream) (void)hipStreamSynchronize(stream);
     void* bufs[] = {d_prims, d_pairs, d_sorted, d_counts, d_offs, d_state, d_val,
                     d_slot, d_temp, d_flags, d_pidx, d_tiles, d_tot};
     for (void* p : bufs)

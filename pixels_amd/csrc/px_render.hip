@@ -507,11 +507,6 @@ int32_t f2i_host(float x) {
   return (int32_t)x;
 }
 
-size_t table_lds(const px_ctx* c, bool* sup_lds) {
-  const size_t full = (8u + (c->nsuper + 31u) / 32u) * 4u;
-  *sup_lds = full <= 64u * 1024u;
-  return *sup_lds ? full : 32u;
-}
 
 PxShadeDesc shade_desc(const px_ctx* c) {
   PxShadeDesc S;
@@ -569,7 +564,7 @@ int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba)
   const PxShadeDesc S = shade_desc(c);
   const PxWorldView w = px_view(c);
   bool sup_lds;
-  const size_t lds = table_lds(c, &sup_lds);
+  const size_t lds = px_table_lds(c, &sup_lds);
   const dim3 grid((n + 255u) / 256u), block(256);
   if (f2i_host(S.flags[3]) > 0) {
     PxProfScope prof(c, PX_PROF_SHADE);

@@ -365,6 +365,7 @@ static RasterDims raster_dims(const px_ctx* c) {
 
 int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, uint32_t* used) {
   hipStream_t s = c->stream;
+  R->stream = s;
   const RasterDims d = raster_dims(c);
   const size_t ncell = c->ncell;
   *used = 0;

@@ -474,18 +474,11 @@ int px_launch_masks(px_ctx* c) {
 
 int px_refresh_masks(px_ctx* c) { return c->masks_dirty ? px_launch_masks(c) : PX_OK; }
 
-// LDS of march()'s tables: 8 glass words, then the super-cell bits when they
-// fit in 64 KB (else the kernels read the global copy).
-static size_t table_lds(const px_ctx* c, bool* sup_lds) {
-  size_t full = (8u + (c->nsuper + 31u) / 32u) * 4u;
-  *sup_lds = full <= 64u * 1024u;
-  return *sup_lds ? full : 32u;
-}
 
 template <int KIND>
 static void launch_trace_kind(px_ctx* c, const PxWorldView& w, const px_ray* d_rays, uint32_t n, px_hit* d_out) {
   bool sup_lds;
-  size_t lds = table_lds(c, &sup_lds);
+  size_t lds = px_table_lds(c, &sup_lds);
   dim3 grid((n + 255u) / 256u), block(256);
   if (sup_lds) hipLaunchKernelGGL((k_trace<KIND, true>), grid, block, lds, c->stream, w, d_rays, n, d_out);
   else hipLaunchKernelGGL((k_trace<KIND, false>), grid, block, lds, c->stream, w, d_rays, n, d_out);
@@ -512,7 +505,7 @@ template <int KIND>
 static void launch_perm_kind(px_ctx* c, const PxWorldView& w, const px_ray* d_rays, const uint32_t* d_perm,
                              uint32_t n, px_hit* d_out) {
   bool sup_lds;
-  size_t lds = table_lds(c, &sup_lds);
+  size_t lds = px_table_lds(c, &sup_lds);
   dim3 grid((n + 255u) / 256u), block(256);
   if (sup_lds) hipLaunchKernelGGL((k_trace_perm<KIND, true>), grid, block, lds, c->stream, w, d_rays, d_perm, n, d_out);
   else hipLaunchKernelGGL((k_trace_perm<KIND, false>), grid, block, lds, c->stream, w, d_rays, d_perm, n, d_out);
@@ -563,7 +556,7 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
   }
   PxWorldView w = px_view(c);
   bool sup_lds;
-  const size_t lds = table_lds(c, &sup_lds);
+  const size_t lds = px_table_lds(c, &sup_lds);
   {
     PxProfScope prof(c, PX_PROF_PRIMARY);
     dim3 grid((n + 255u) / 256u), block(256);
@@ -645,7 +638,7 @@ int px_launch_batch(px_ctx* c, const px_query_set* q) {
     }
     PxWorldView w = px_view(c);
     bool sup_lds;
-    const size_t lds = table_lds(c, &sup_lds);
+    const size_t lds = px_table_lds(c, &sup_lds);
     hipGraph_t graph = nullptr;
     PX_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     // a fill kernel, not hipMemsetAsync, inside the graph
