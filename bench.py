@@ -252,6 +252,14 @@ def island_leg(args):
     res = {"ms": round(float(np.median(reps)), 4), "ms_min": round(float(np.min(reps)), 4), "reps": len(reps),
            "impacts": int(impacts.shape[0]), "grid": list(scene.dims), "unsupported_components": n,
            "destroyed": int(ctrl[CTRL["S_DESTROYED"]]) - 0, "target_ms": 2.0}
+    # algorithmic bytes of the pass: one occupancy read (4 B per cell + 1 KiB
+    # per pool brick) + 8 B per solid voxel, on the world before the pass
+    solid = int(np.count_nonzero(scene.dense()))
+    alg = 4 * scene.grid.size + 1024 * scene.pool_used + 8 * solid
+    ach = alg / (res["ms"] * 1e-3) / 1e9
+    res["roofline"] = {"bound": "hbm", "algorithmic_bytes": alg, "solid_voxels": solid,
+                       "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(ach / HBM_PEAK_GBS, 4)}
     log("island pass: %.3f ms median over %d passes, %d unsupported components" % (res["ms"], len(reps), n))
     if not args.no_cpu:
         import oracle
