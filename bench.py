@@ -692,16 +692,18 @@ def batch_leg(args):
 
 def learned_leg(args):
     """Learned ray-cost binning on the 512x256x512 scenes (plain, and with
-    the structural floor): a 16-64-64-1 BF16 MLP trained on the GPU with
-    Adam (lr 1e-3, batch 65,536) on ~16M (ray, step count) pairs that the
-    plain DDA produces for primary and shadow rays of seeded cameras, then
-    used on a held-out camera: features + prediction + 32-bucket binning in
-    windows + the permuted launch, against the unsorted launch."""
+    the structural floor) and on the 1280x352x960 frame scene: a 16-64-64-1
+    BF16 MLP trained on the GPU with Adam (lr 1e-3, batch 65,536) on ~16M
+    (ray, step count) pairs per scene that the plain DDA produces for primary
+    and shadow rays of seeded cameras, then used on a held-out camera:
+    features + prediction + a stable global sort into 32 cost buckets (one
+    fused launch, an offset scan and a scatter) + the permuted launch, against the unsorted
+    launch of the same rays."""
     import torch
     from pixels_amd import GpuWorld, Scene, Simulation
     from pixels_amd import _native as N
     from pixels_amd.learned import CostModel, bin_rays, log2_steps, trace_perm, wave_cost
-    from pixels_amd.scene import SMALL, SMALL_FLOOR
+    from pixels_amd.scene import LARGE, SMALL, SMALL_FLOOR
 
     dev = torch.device("cuda:0")
     W, H = 1920, 1080
@@ -715,7 +717,7 @@ def learned_leg(args):
         return o
 
     t_all = time.time()
-    for kind_scene, seed in ((SMALL, 1), (SMALL_FLOOR, 3)):
+    for kind_scene, seed in ((SMALL, 1), (SMALL_FLOOR, 3), (LARGE, 2)):
         scene = Scene.build(kind_scene, seed=seed)
         gw = GpuWorld(scene)
         sim = Simulation(gw)
@@ -772,13 +774,18 @@ def learned_leg(args):
                 sim.trace_device(kind, d, n, plain)
 
             def run_binned():
-                model.features(kind, d, n, f)
-                model.predict(f, n, pred)
-                bin_rays(gw, pred, n, perm, window=256, buckets=32, lo=0.0, hi=9.0)
+                model.sort_rays(kind, d, n, perm, buckets=32, lo=0.0, hi=9.0)
                 trace_perm(gw, kind, d, perm, n, binned)
             run_plain()
             run_binned()
             gw.sync()
+            # the separate kernels give the same order (prediction read back for the MAE)
+            model.features(kind, d, n, f)
+            model.predict(f, n, pred)
+            ref_perm = torch.empty(n, dtype=torch.int32, device=dev)
+            bin_rays(gw, pred, n, ref_perm, window=n, buckets=32, lo=0.0, hi=9.0)
+            gw.sync()
+            same_order = bool(torch.equal(ref_perm, perm))
             reps = 5
             gw.profile_begin()
             for _ in range(reps):
@@ -789,7 +796,8 @@ def learned_leg(args):
                 run_binned()
             prof = gw.profile_end()
             b_trace_ms = prof[N.PROF_TRACE][0] / reps
-            b_ms = b_trace_ms + prof.get(N.PROF_MLP, (0.0, 0))[0] / reps
+            sort_ms = prof.get(N.PROF_MLP, (0.0, 0))[0] / reps
+            b_ms = b_trace_ms + sort_ms
             hits = plain.cpu().numpy().view(N.HIT_DTYPE)
             steps = hits["steps"].astype(np.int64)
             pr = pred.cpu().numpy()
@@ -798,11 +806,12 @@ def learned_leg(args):
                 "rays": n, "plain_grays_s": round(n / (p_ms * 1e-3) / 1e9, 4),
                 "binned_grays_s": round(n / (b_ms * 1e-3) / 1e9, 4),
                 "binned_trace_only_grays_s": round(n / (b_trace_ms * 1e-3) / 1e9, 4),
+                "sort_ms": round(sort_ms, 4), "plain_ms": round(p_ms, 4), "binned_trace_ms": round(b_trace_ms, 4),
                 "speedup": round(p_ms / b_ms, 3), "speedup_trace_only": round(p_ms / b_trace_ms, 3),
                 "mae_steps": round(float(np.mean(np.abs(np.exp2(pr) - np.maximum(steps, 1)))), 2),
                 "mean_steps": round(float(steps.mean()), 2),
                 "wave_cost_plain": wave_cost(steps), "wave_cost_binned": wave_cost(steps[pm]),
-                "bit_identical": bool(torch.equal(plain, binned))}
+                "bit_identical": bool(torch.equal(plain, binned)), "order_equals_separate_kernels": same_order}
         out["scenes"].append(res)
         log("learned binning %s: %s" % (scene.dims, json.dumps(res)))
         model.close()

@@ -484,6 +484,14 @@ int px_mlp_train_step(px_mlp* m, const uint16_t* d_feat, const float* d_target, 
  * buckets over [lo, hi) of the prediction; d_perm[k] = source index. */
 int px_bin_rays(px_ctx* ctx, const float* d_pred, uint32_t n, uint32_t window, uint32_t buckets, float lo, float hi,
                 uint32_t* d_perm);
+/* Features, prediction and a stable counting sort of ALL n rays into
+ * `buckets` (1..64) cost buckets over [lo, hi), in three launches: d_perm is
+ * the order px_bin_rays(window = n) gives for px_mlp_predict's output on
+ * px_mlp_features(kind, d_rays).  Device scratch (keys, per-block counts)
+ * grows with n and lives with the model.  Calls on one model are ordered on
+ * its context's stream. */
+int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo, float hi,
+                     uint32_t* d_perm);
 /* Traces d_rays[d_perm[k]] for k = 0..n-1 in that order and writes the hit to
  * d_out[d_perm[k]] (kinds as px_trace_device). */
 int px_trace_perm_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,

@@ -69,6 +69,14 @@ class CostModel:
                                            C.byref(loss)))
         return float(loss.value), g
 
+    def sort_rays(self, kind: int, rays_t, n: int, perm_t, buckets: int = 32, lo: float = 0.0,
+                  hi: float = 9.0) -> None:
+        """Features, prediction and a stable sort of all n rays by cost
+        bucket (three launches): perm_t gets the order bin_rays(window=n)
+        gives for predict(features(rays))."""
+        self.gw.after_torch()
+        self.gw.check(self.lib.px_mlp_sort_rays(self.h, kind, _dp(rays_t), n, buckets, lo, hi, _dp(perm_t)))
+
     def train_step(self, feat_t, target_t, n: int, lr: float = 1e-3, want_loss: bool = False):
         self.gw.after_torch()
         loss = C.c_float(0.0)

@@ -7,6 +7,7 @@ oracle (oracle/mlp.py):
 * features are in [-1, 1] with the occupancy samples at exactly +-1;
 * binning is a stable permutation sorted by bucket inside every window, and
   the permuted launch gives the plain launch's hits bit for bit;
+* the fused prediction + global sort equals the separate kernels' order;
 * training on the DDA's own step counts cuts the predictor's held-out error
   to under a quarter of the untrained network's (the gain the training
   buys); the wavefront cost of its bins is printed beside the plain order
@@ -154,6 +155,65 @@ def test_binning_is_stable_permutation_and_bit_exact(world):
         trace_perm(gw, kind, _dev(rays.view(np.uint8)), perm, n, out)
         gw.sync()
         assert np.array_equal(out.cpu().numpy(), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fused_global_sort_equals_separate_kernels(world, kind):
+    """px_mlp_sort_rays (features + network + global counting sort, three
+    launches) gives exactly the permutation of features -> predict ->
+    bin_rays(window = n), which is numpy's stable sort of the buckets; the
+    permuted launch then equals the plain launch.  Ragged counts (not whole
+    256-ray tiles, fewer rays than one tile, one ray) and a smaller second
+    call on the same scratch are covered."""
+    import torch
+    from pixels_amd.learned import CostModel, bin_rays, trace_perm
+    s, gw, sim = world
+    m = CostModel(gw, seed=5)
+    try:
+        m.set_params(_rand_params(21 + kind))
+        rays_all = s.camera_rays(480, 270, 90.0, seed=9 + kind)
+        if kind == 1:  # shadow-like rays: from the primary hits toward a fixed sun, bounded
+            hits = _trace_steps(sim, gw, 0, rays_all)
+            ok = hits["face"] != 0xFF
+            r = rays_all[ok].copy()
+            r["ro"] = r["ro"] + r["rd"] * (hits["t"][ok, None] - 0.5)
+            sun = np.array([0.3, 0.8, 0.2], np.float32)
+            r["rd"] = sun / np.linalg.norm(sun)
+            r["max_t"] = 1e9
+            rays_all = r
+        nb = 32
+        for n in (rays_all.shape[0], 1000, 255, 1):
+            rays = rays_all[:n]
+            d = _dev(rays.view(np.uint8))
+            feat = torch.zeros(n * 16, dtype=torch.int16, device="cuda")
+            pred = torch.zeros(n, dtype=torch.float32, device="cuda")
+            m.features(kind, d, n, feat)
+            m.predict(feat, n, pred)
+            gw.sync()
+            pr = pred.cpu().numpy()
+            lo, hi = float(np.quantile(pr, 0.02)), float(np.quantile(pr, 0.98)) + 1e-3
+            if n == 1:
+                lo, hi = float(pr[0]) - 1.0, float(pr[0]) + 1.0
+            ref = torch.zeros(n, dtype=torch.int32, device="cuda")
+            bin_rays(gw, pred, n, ref, window=n, buckets=nb, lo=lo, hi=hi)
+            perm = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+            print("case: fused sort kind=%d n=%d" % (kind, n), flush=True)
+            m.sort_rays(kind, d, n, perm, buckets=nb, lo=lo, hi=hi)
+            gw.sync()
+            pm, rf = perm.cpu().numpy().astype(np.int64), ref.cpu().numpy().astype(np.int64)
+            scale = np.float32(nb) / (np.float32(hi) - np.float32(lo))
+            b = np.clip(np.floor((pr - np.float32(lo)) * scale), 0, nb - 1).astype(np.int64)
+            assert np.array_equal(pm, rf)
+            assert np.array_equal(pm, np.lexsort((np.arange(n), b)))
+            if n > 1000:
+                assert len(np.unique(b)) > nb // 2  # the buckets are really used
+                plain = _trace_steps(sim, gw, kind, rays)
+                out = torch.zeros(n * 16, dtype=torch.uint8, device="cuda")
+                trace_perm(gw, kind, d, perm, n, out)
+                gw.sync()
+                assert np.array_equal(out.cpu().numpy(), plain.view(np.uint8))
+    finally:
+        m.close()
 
 
 def test_training_gain_over_untrained(world):
