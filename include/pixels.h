@@ -492,6 +492,11 @@ int px_bin_rays(px_ctx* ctx, const float* d_pred, uint32_t n, uint32_t window, u
  * its context's stream. */
 int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo, float hi,
                      uint32_t* d_perm);
+/* The learned-binned traversal in one call: px_mlp_sort_rays, then
+ * px_trace_perm_device in that order (d_perm is the scratch order).  The
+ * hits equal px_trace_device's for the same rays, bit for bit. */
+int px_mlp_trace_sorted(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo,
+                        float hi, uint32_t* d_perm, px_hit* d_out);
 /* Traces d_rays[d_perm[k]] for k = 0..n-1 in that order and writes the hit to
  * d_out[d_perm[k]] (kinds as px_trace_device). */
 int px_trace_perm_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,

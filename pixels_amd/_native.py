@@ -130,6 +130,7 @@ SIGNATURES = {
     "px_mlp_train_step": (C.c_int, [_P, _P, _P, _U32, C.c_float, _FP]),
     "px_bin_rays": (C.c_int, [_P, _P, _U32, _U32, _U32, C.c_float, C.c_float, _P]),
     "px_mlp_sort_rays": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_float, C.c_float, _P]),
+    "px_mlp_trace_sorted": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_float, C.c_float, _P, _P]),
     "px_particles_init": (C.c_int, [_P, _U32]),
     "px_frame_begin": (C.c_int, [_P]),
     "px_spawn": (C.c_int, [_P, _P, _U32]),

@@ -1012,4 +1012,12 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   return PX_OK;
 }
 
+int px_mlp_trace_sorted(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo,
+                        float hi, uint32_t* d_perm, px_hit* d_out) {
+  if (!m) return PX_ERR_ARG;
+  int rc = px_mlp_sort_rays(m, kind, d_rays, n, buckets, lo, hi, d_perm);
+  if (rc) return rc;
+  return px_trace_perm_device(m->c, kind, d_rays, d_perm, n, d_out);
+}
+
 }  // extern "C"

@@ -77,6 +77,14 @@ class CostModel:
         self.gw.after_torch()
         self.gw.check(self.lib.px_mlp_sort_rays(self.h, kind, _dp(rays_t), n, buckets, lo, hi, _dp(perm_t)))
 
+    def trace_sorted(self, kind: int, rays_t, n: int, perm_t, out_t, buckets: int = 32, lo: float = 0.0,
+                     hi: float = 9.0) -> None:
+        """sort_rays, then the permuted launch, in one call (one submission
+        of four kernels); out_t gets trace_device's hits bit for bit."""
+        self.gw.after_torch()
+        self.gw.check(self.lib.px_mlp_trace_sorted(self.h, kind, _dp(rays_t), n, buckets, lo, hi, _dp(perm_t),
+                                                   _dp(out_t)))
+
     def train_step(self, feat_t, target_t, n: int, lr: float = 1e-3, want_loss: bool = False):
         self.gw.after_torch()
         loss = C.c_float(0.0)

@@ -212,6 +212,12 @@ def test_fused_global_sort_equals_separate_kernels(world, kind):
                 trace_perm(gw, kind, d, perm, n, out)
                 gw.sync()
                 assert np.array_equal(out.cpu().numpy(), plain.view(np.uint8))
+                # the one-call form: same order, same hits
+                out2 = torch.zeros(n * 16, dtype=torch.uint8, device="cuda")
+                perm2 = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+                m.trace_sorted(kind, d, n, perm2, out2, buckets=nb, lo=lo, hi=hi)
+                gw.sync()
+                assert torch.equal(perm2, perm) and np.array_equal(out2.cpu().numpy(), plain.view(np.uint8))
     finally:
         m.close()
 
