@@ -47,7 +47,11 @@ namespace {
 constexpr uint32_t kLabNone = 0xFFFFu;
 constexpr uint32_t kFlagAnchor = 1u;
 constexpr uint32_t kFlagHit = 2u;
-constexpr uint32_t kEditBlocks = 8192;  // persistent grid: 32 single-wave blocks per CU (256 CUs)
+// Persistent grid of the edit kernels: 32 single-wave blocks per CU (256
+// CUs).  Measured on the 64-impact island pass: 6144 blocks (what
+// k_local_ccl<0>'s LDS keeps resident) 0.84 ms, 8192 0.77 ms, 16384 0.77 ms
+// with the extraction kernels, which walk every block's list, 1.5x slower.
+constexpr uint32_t kEditBlocks = 8192;
 constexpr uint32_t kPartWords = 8;      // per-block partials
 constexpr uint32_t kCountWords = 4;     // per-block list lengths
 
