@@ -697,9 +697,10 @@ def learned_leg(args):
     the GPU with Adam (lr 1e-3, batch 65,536, 12 epochs) on the 16.6M
     (ray, step count) pairs that the plain DDA produces for 8 seeded cameras,
     then used on a held-out camera: features + prediction + a stable global
-    sort into 32 cost buckets (one fused launch, a per-bucket scan and a
-    scatter) + the permuted launch, submitted by one call, against the
-    unsorted launch of the same rays."""
+    sort by (direction face, one of 32 cost buckets) (one fused launch, a
+    per-key scan and a scatter) + the permuted launch, submitted by one call,
+    against the unsorted launch of the same rays; with the cost key alone and
+    the face alone as ablations."""
     import torch
     from pixels_amd import GpuWorld, Scene, Simulation
     from pixels_amd import _native as N
@@ -801,11 +802,11 @@ def learned_leg(args):
                 last = model.train_step(tf, tt, bs, lr=1e-3, want_loss=True)
                 train_s = time.time() - t0
                 train_gpu_ms = ev0.elapsed_time(ev1)
-                model.trace_sorted(kind, vd, nv, v_perm, v_out)
+                model.trace_sorted(kind, vd, nv, v_perm, v_out, faces=True)
                 gw.sync()
                 gw.profile_begin()
                 for _ in range(3):
-                    model.trace_sorted(kind, vd, nv, v_perm, v_out)
+                    model.trace_sorted(kind, vd, nv, v_perm, v_out, faces=True)
                 vp = gw.profile_end()
                 v_ms = (vp[N.PROF_TRACE][0] + vp.get(N.PROF_MLP, (0.0, 0))[0]) / 3
                 cands.append((v_ms, init, model, first, last, train_s, train_gpu_ms))
@@ -825,22 +826,27 @@ def learned_leg(args):
             def run_plain():
                 sim.trace_device(kind, d, n, plain)
 
-            def run_binned():
-                model.trace_sorted(kind, d, n, perm, binned, buckets=32, lo=0.0, hi=9.0)
+            def run_binned(faces=True, buckets=32):
+                model.trace_sorted(kind, d, n, perm, binned, buckets=buckets, lo=0.0, hi=9.0, faces=faces)
             run_plain()
             run_binned()
             gw.sync()
-            # the separate kernels give the same order (prediction read back for the MAE)
+            # without the face key the fused sort gives the separate kernels'
+            # order (prediction read back for the MAE)
             model.features(kind, d, n, f)
             model.predict(f, n, pred)
             ref_perm = torch.empty(n, dtype=torch.int32, device=dev)
             bin_rays(gw, pred, n, ref_perm, window=n, buckets=32, lo=0.0, hi=9.0)
+            run_binned(faces=False)
             gw.sync()
             same_order = bool(torch.equal(ref_perm, perm))
+            run_binned()
+            gw.sync()
             # interleaved rounds, medians: an occasional slow round (seen
             # either way) does not decide the comparison
             reps, rounds = 3, 7
             p_r, t_r, s_r = [], [], []
+            abl = {"cost_only": ([], []), "faces_only": ([], [])}
             for _ in range(rounds):
                 gw.profile_begin()
                 for _ in range(reps):
@@ -852,8 +858,20 @@ def learned_leg(args):
                 prof = gw.profile_end()
                 t_r.append(prof[N.PROF_TRACE][0] / reps)
                 s_r.append(prof.get(N.PROF_MLP, (0.0, 0))[0] / reps)
+                # ablations: the learned cost key alone, the direction alone
+                for name, kw in (("cost_only", dict(faces=False)), ("faces_only", dict(buckets=1))):
+                    gw.profile_begin()
+                    for _ in range(reps):
+                        run_binned(**kw)
+                    prof = gw.profile_end()
+                    abl[name][0].append(prof[N.PROF_TRACE][0] / reps)
+                    abl[name][1].append(prof.get(N.PROF_MLP, (0.0, 0))[0] / reps)
+            run_binned()  # leave the main order in perm / binned
             p_ms, b_trace_ms, sort_ms = float(np.median(p_r)), float(np.median(t_r)), float(np.median(s_r))
             b_ms = b_trace_ms + sort_ms
+            ablation = {name: {"speedup": round(p_ms / (float(np.median(t)) + float(np.median(sr))), 3),
+                               "speedup_trace_only": round(p_ms / float(np.median(t)), 3)}
+                        for name, (t, sr) in abl.items()}
             hits = plain.cpu().numpy().view(N.HIT_DTYPE)
             steps = hits["steps"].astype(np.int64)
             pr = pred.cpu().numpy()
@@ -874,7 +892,8 @@ def learned_leg(args):
                 "mae_steps": round(float(np.mean(np.abs(np.exp2(pr) - np.maximum(steps, 1)))), 2),
                 "mean_steps": round(float(steps.mean()), 2),
                 "wave_cost_plain": wave_cost(steps), "wave_cost_binned": wave_cost(steps[pm]),
-                "bit_identical": bool(torch.equal(plain, binned)), "order_equals_separate_kernels": same_order}
+                "bit_identical": bool(torch.equal(plain, binned)), "order_equals_separate_kernels": same_order,
+                "keys": "direction face x 32 cost buckets", "ablation": ablation}
             model.close()
         del data
         out["scenes"].append(res)

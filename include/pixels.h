@@ -484,19 +484,25 @@ int px_mlp_train_step(px_mlp* m, const uint16_t* d_feat, const float* d_target, 
  * buckets over [lo, hi) of the prediction; d_perm[k] = source index. */
 int px_bin_rays(px_ctx* ctx, const float* d_pred, uint32_t n, uint32_t window, uint32_t buckets, float lo, float hi,
                 uint32_t* d_perm);
-/* Features, prediction and a stable counting sort of ALL n rays into
- * `buckets` (1..64) cost buckets over [lo, hi), in three launches: d_perm is
- * the order px_bin_rays(window = n) gives for px_mlp_predict's output on
+/* Sort keys of px_mlp_sort_rays: the cost bucket alone (0), or the major
+ * axis and sign of the ray direction (6 faces), then the cost bucket. */
+#define PX_SORT_FACES 1u
+/* Features, prediction and a stable counting sort of ALL n rays by key, in
+ * three launches.  The key is the cost bucket (`buckets` of the predicted
+ * log2 steps over [lo, hi), 1..256), or with PX_SORT_FACES (face, bucket)
+ * with face = 2 * major axis + (component < 0) of the direction (the first of
+ * equal magnitudes wins; buckets 1..42).  Without flags d_perm is the order
+ * px_bin_rays(window = n) gives for px_mlp_predict's output on
  * px_mlp_features(kind, d_rays).  Device scratch (keys, per-block counts)
  * grows with n and lives with the model.  Calls on one model are ordered on
  * its context's stream. */
 int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo, float hi,
-                     uint32_t* d_perm);
+                     uint32_t flags, uint32_t* d_perm);
 /* The learned-binned traversal in one call: px_mlp_sort_rays, then
  * px_trace_perm_device in that order (d_perm is the scratch order).  The
  * hits equal px_trace_device's for the same rays, bit for bit. */
 int px_mlp_trace_sorted(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo,
-                        float hi, uint32_t* d_perm, px_hit* d_out);
+                        float hi, uint32_t flags, uint32_t* d_perm, px_hit* d_out);
 /* Traces d_rays[d_perm[k]] for k = 0..n-1 in that order and writes the hit to
  * d_out[d_perm[k]] (kinds as px_trace_device). */
 int px_trace_perm_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,

@@ -129,8 +129,8 @@ SIGNATURES = {
     "px_mlp_grad": (C.c_int, [_P, _P, _P, _U32, _FP, _FP]),
     "px_mlp_train_step": (C.c_int, [_P, _P, _P, _U32, C.c_float, _FP]),
     "px_bin_rays": (C.c_int, [_P, _P, _U32, _U32, _U32, C.c_float, C.c_float, _P]),
-    "px_mlp_sort_rays": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_float, C.c_float, _P]),
-    "px_mlp_trace_sorted": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_float, C.c_float, _P, _P]),
+    "px_mlp_sort_rays": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_float, C.c_float, _U32, _P]),
+    "px_mlp_trace_sorted": (C.c_int, [_P, _U32, _P, _U32, _U32, C.c_float, C.c_float, _U32, _P, _P]),
     "px_particles_init": (C.c_int, [_P, _U32]),
     "px_frame_begin": (C.c_int, [_P]),
     "px_spawn": (C.c_int, [_P, _P, _U32]),
@@ -149,6 +149,7 @@ SIGNATURES = {
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
+SORT_FACES = 1  # px_mlp_sort_rays flag: keys (direction face, cost bucket)
 PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_PARTICLES = 0, 1, 2, 3, 4, 5, 6
 PROF_PARTICLE_SIM, PROF_WRITES, PROF_CHUNKS = 7, 8, 9
 PROF_DENSITY, PROF_RENDER_PRIMARY, PROF_RENDER_SHADOW, PROF_SHADE, PROF_REFLECT = 10, 11, 12, 13, 14

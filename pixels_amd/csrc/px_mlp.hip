@@ -555,20 +555,20 @@ __global__ __launch_bounds__(256) void k_bin(const float* __restrict__ pred, uin
 // The rays are cut into tiles of 256, 64 per wavefront; block g owns tiles
 // [g R, (g + 1) R).  Each wavefront computes its 64 rays' features, stages
 // them in LDS as MFMA fragments, runs the network, and turns the prediction
-// into a bucket (bucket_of, as the windowed sort).  A lane's rank among the
-// wavefront's rays of its bucket comes from one ballot per bucket bit; the
-// block keeps running bucket counts over its tiles in LDS, so each ray gets
-// its rank inside (block range, bucket): key = bucket << 24 | rank.  The
-// block's bucket totals go to part[b * G + g] (bucket-major).
-// k_sort_offsets scans each bucket's row of part (one block per bucket), and
-// k_sort_scatter adds the totals of the earlier buckets, which gives each
-// (bucket, block) run's first sorted position, and writes
-// perm[position + rank] = i.  The order is by bucket, then by ray index: the
-// stable sort of the whole launch.  (A
+// into a bucket (bucket_of, as the windowed sort); the sort key is the bucket,
+// or (face of the direction, bucket) with PX_SORT_FACES.  A lane's rank among
+// the wavefront's rays of its key comes from one ballot per key bit; the
+// block keeps running key counts over its tiles in LDS, so each ray gets its
+// rank inside (block range, key): keys[i] = key << 24 | rank.  The block's
+// key totals go to part[key * G + g] (key-major).  k_sort_offsets scans each
+// key's row of part (one block per key), and k_sort_scatter adds the totals
+// of the earlier keys, which gives each (key, block) run's first sorted
+// position, and writes perm[position + rank] = i.  The order is by key, then
+// by ray index: the stable sort of the whole launch.  (A
 // last-block-done scan inside the first kernel needs a device-scope release
 // per block; on gfx950 each is an L2 write-back, and that form measured
 // 0.20 ms per 2M rays against 0.14 ms for separate launches.)
-constexpr uint32_t kSortMaxBuckets = 64;
+constexpr uint32_t kSortMaxKeys = 256;  // 8 key bits; the block size of the sort kernels
 
 PX_DEV void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -576,10 +576,27 @@ PX_DEV void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Major axis and sign of a ray direction (0..5 = +x, -x, +y, -y, +z, -z; the
+// first of equal magnitudes wins).  r0.w, r1.x, r1.y are the direction.
+PX_DEV uint32_t ray_face(float4 r0, float4 r1) {
+  uint32_t a = 0u;
+  float m = fabsf(r0.w), d = r0.w;
+  if (fabsf(r1.x) > m) {
+    a = 1u;
+    m = fabsf(r1.x);
+    d = r1.x;
+  }
+  if (fabsf(r1.y) > m) {
+    a = 2u;
+    d = r1.y;
+  }
+  return a * 2u + (d < 0.0f ? 1u : 0u);
+}
+
 template <bool SUP_LDS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_sort_keys(
     PxWorldView w, const px_ray* __restrict__ rays, uint32_t n, FeatK K, const uint16_t* __restrict__ wbf,
-    const float* __restrict__ par, uint32_t nb, float lo, float hi, uint32_t tiles_per_block,
+    const float* __restrict__ par, uint32_t nb, uint32_t faces, float lo, float hi, uint32_t tiles_per_block,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ part) {
   extern __shared__ uint32_t s_mem[];
   __shared__ MlpFwdLds L;
@@ -588,10 +605,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   constexpr int kLdX = 24;
   __shared__ alignas(16) uint16_t s_x[4][64 * kLdX];
   __shared__ uint32_t s_b[4][64];
-  __shared__ uint32_t s_wc[2][4][kSortMaxBuckets];  // by tile parity
-  __shared__ uint32_t s_run[kSortMaxBuckets];
+  __shared__ uint32_t s_wc[2][4][kSortMaxKeys];  // per-wavefront key counts, by tile parity
+  __shared__ uint32_t s_run[kSortMaxKeys];
   const uint32_t tid = threadIdx.x, l = __lane_id(), wave = tid >> 6;
-  if (tid < kSortMaxBuckets) s_run[tid] = 0u;
+  const uint32_t nkeys = faces ? 6u * nb : nb;
+  s_run[tid] = 0u;  // blockDim == kSortMaxKeys
   const px::MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   load_fwd_weights(L, wbf, par);
   const float scale = (float)nb / (hi - lo);
@@ -628,30 +646,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (l < 16u)
       for (int c = 0; c < 4; c++) s_b[wave][16 * c + l] = bucket_of(F.y[c], lo, scale, nb);
     wave_lds_sync();
+    // sort key: the cost bucket, after the direction's face when asked
     const uint32_t b = s_b[wave][l];
-    // lanes of the same bucket as this lane (mo), and of the bucket numbered
-    // by this lane's index (ml), from one ballot per bucket bit
-    const uint64_t vm = __ballot(valid);
-    uint64_t mo = vm, ml = vm;
-    for (uint32_t k = 0; k < 6u; k++) {
-      const uint64_t v = __ballot((b >> k) & 1u);
-      mo &= ((b >> k) & 1u) ? v : ~v;
-      ml &= ((l >> k) & 1u) ? v : ~v;
+    const uint32_t key = faces ? ray_face(r0, r1) * nb + b : b;
+    // lanes with this lane's key, from one ballot per key bit
+    uint64_t mo = __ballot(valid);
+    for (uint32_t k = 0; k < 8u; k++) {
+      const uint64_t v = __ballot((key >> k) & 1u);
+      mo &= ((key >> k) & 1u) ? v : ~v;
     }
     const uint32_t rank = (uint32_t)__popcll(mo & ((1ull << l) - 1ull));
-    uint32_t(*wc)[kSortMaxBuckets] = s_wc[tile & 1u];
-    if (l < nb) wc[wave][l] = (uint32_t)__popcll(ml);
+    // the wavefront's count of each key, written by the key's first lane
+    uint32_t(*wc)[kSortMaxKeys] = s_wc[tile & 1u];
+    for (uint32_t j = l; j < nkeys; j += 64u) wc[wave][j] = 0u;
+    wave_lds_sync();
+    if (valid && rank == 0u) wc[wave][key] = (uint32_t)__popcll(mo);
     __syncthreads();
     if (valid) {
-      uint32_t r = s_run[b] + rank;
-      for (uint32_t v = 0; v < wave; v++) r += wc[v][b];
-      keys[i] = (b << 24) | r;
+      uint32_t r = s_run[key] + rank;
+      for (uint32_t v = 0; v < wave; v++) r += wc[v][key];
+      keys[i] = (key << 24) | r;
     }
     __syncthreads();
-    if (tid < nb) s_run[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+    if (tid < nkeys) s_run[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
   }
   __syncthreads();
-  if (tid < nb) part[tid * gridDim.x + blockIdx.x] = s_run[tid];
+  if (tid < nkeys) part[tid * gridDim.x + blockIdx.x] = s_run[tid];
 }
 
 // Per bucket b (one block each): the exclusive prefix of part[b][0..G) over
@@ -696,29 +716,34 @@ __global__ __launch_bounds__(256) void k_sort_offsets(const uint32_t* __restrict
   }
 }
 
-// perm[base[b] + off[b][g] + rank] = i, base[b] = sum of the totals of the
-// buckets before b.
+// perm[base[key] + off[key][g] + rank] = i, base[key] = sum of the totals of
+// the keys before it.
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t* __restrict__ keys,
                                                       const uint32_t* __restrict__ off,
-                                                      const uint32_t* __restrict__ tot, uint32_t nb, uint32_t n,
+                                                      const uint32_t* __restrict__ tot, uint32_t nkeys, uint32_t n,
                                                       uint32_t nblk, uint32_t rays_per_block,
                                                       uint32_t* __restrict__ perm) {
-  __shared__ uint32_t s_base[kSortMaxBuckets];
-  if (threadIdx.x < 64u) {
-    const uint32_t l = threadIdx.x;
-    const uint32_t t = l < nb ? tot[l] : 0u;
-    uint32_t x = t;
+  __shared__ uint32_t s_base[kSortMaxKeys];
+  __shared__ uint32_t s_tmp[4];
+  {
+    const uint32_t t = threadIdx.x, l = __lane_id(), wv = t >> 6;
+    const uint32_t v = t < nkeys ? tot[t] : 0u;
+    uint32_t x = v;
     for (uint32_t o = 1; o < 64u; o <<= 1) {
       const uint32_t y = __shfl_up(x, o);
       if (l >= o) x += y;
     }
-    s_base[l] = x - t;
+    if (l == 63u) s_tmp[wv] = x;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < wv; k++) base += s_tmp[k];
+    s_base[t] = base + x - v;
   }
   __syncthreads();
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= n) return;
-  const uint32_t k = keys[i], b = k >> 24;
-  const uint32_t dst = s_base[b] + off[b * nblk + i / rays_per_block] + (k & 0xFFFFFFu);
+  const uint32_t k = keys[i], key = k >> 24;
+  const uint32_t dst = s_base[key] + off[key * nblk + i / rays_per_block] + (k & 0xFFFFFFu);
   if (dst < n) perm[dst] = i;  // always true for consistent counts
 }
 
@@ -735,7 +760,7 @@ struct px_mlp {
   float* d_part = nullptr;
   uint32_t part_cap = 0;  // blocks
   uint32_t step = 0;
-  // global sort: part and off (nb * G each), bucket totals (64), keys (n)
+  // global sort: part and off (keys * G each), key totals (256), keys (n)
   uint32_t* d_sort = nullptr;
   size_t sort_cap = 0;  // words
 };
@@ -963,16 +988,20 @@ int px_bin_rays(px_ctx* c, const float* d_pred, uint32_t n, uint32_t window, uin
 }
 
 int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo,
-                     float hi, uint32_t* d_perm) {
+                     float hi, uint32_t flags, uint32_t* d_perm) {
   if (!m || (n && (!d_rays || !d_perm))) return PX_ERR_ARG;
   px_ctx* c = m->c;
   if (kind > 3u) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown ray kind");
-  if (buckets == 0u || buckets > kSortMaxBuckets) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "buckets: 1..64");
+  if (flags & ~PX_SORT_FACES) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown flags");
+  const uint32_t faces = (flags & PX_SORT_FACES) ? 1u : 0u;
+  if (buckets == 0u || buckets * (faces ? 6u : 1u) > kSortMaxKeys)
+    return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "buckets: 1..256, or 1..42 with PX_SORT_FACES");
   if (!(hi > lo)) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "need hi > lo");
   if (n == 0) return PX_OK;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
   int rc = px_refresh_masks(c);
   if (rc) return rc;
+  const uint32_t nkeys = buckets * (faces ? 6u : 1u);
   // block ranges: about four blocks per CU, a whole number of 256-ray tiles
   // each; the block count is padded to a multiple of 4 (16-byte scan groups)
   const uint32_t ntiles = (n + 255u) / 256u;
@@ -980,7 +1009,7 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   const uint32_t per = (ntiles + target - 1u) / target;
   const uint32_t nblk = ((ntiles + per - 1u) / per + 3u) & ~3u;
   if ((size_t)per * 256u >= (size_t)1 << 24) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "too many rays");
-  const size_t need = 2u * (size_t)buckets * nblk + kSortMaxBuckets + n;
+  const size_t need = 2u * (size_t)nkeys * nblk + kSortMaxKeys + n;
   if (need > m->sort_cap) {
     MLP_HIP(m, hipStreamSynchronize(c->stream));
     (void)hipFree(m->d_sort);
@@ -990,32 +1019,32 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
     m->sort_cap = need;
   }
   uint32_t* part = m->d_sort;
-  uint32_t* off = part + (size_t)buckets * nblk;
-  uint32_t* tot = off + (size_t)buckets * nblk;
-  uint32_t* keys = tot + kSortMaxBuckets;
+  uint32_t* off = part + (size_t)nkeys * nblk;
+  uint32_t* tot = off + (size_t)nkeys * nblk;
+  uint32_t* keys = tot + kSortMaxKeys;
   PxWorldView w = px_view(c);
   const size_t full = (8u + (c->nsuper + 31u) / 32u) * 4u;
   const FeatK K = feat_consts(c, kind);
   PxProfScope prof(c, PX_PROF_MLP);
   if (full <= 24u * 1024u)
     hipLaunchKernelGGL(k_mlp_sort_keys<true>, dim3(nblk), dim3(256), full, c->stream, w, d_rays, n, K, m->d_wbf,
-                       m->d_par, buckets, lo, hi, per, keys, part);
+                       m->d_par, buckets, faces, lo, hi, per, keys, part);
   else
     hipLaunchKernelGGL(k_mlp_sort_keys<false>, dim3(nblk), dim3(256), 32, c->stream, w, d_rays, n, K, m->d_wbf,
-                       m->d_par, buckets, lo, hi, per, keys, part);
+                       m->d_par, buckets, faces, lo, hi, per, keys, part);
   PX_HIP_CHECK(c, hipGetLastError());
-  hipLaunchKernelGGL(k_sort_offsets, dim3(buckets), dim3(256), 0, c->stream, part, nblk / 4u, off, tot);
+  hipLaunchKernelGGL(k_sort_offsets, dim3(nkeys), dim3(256), 0, c->stream, part, nblk / 4u, off, tot);
   PX_HIP_CHECK(c, hipGetLastError());
-  hipLaunchKernelGGL(k_sort_scatter, dim3((n + 255u) / 256u), dim3(256), 0, c->stream, keys, off, tot, buckets, n,
+  hipLaunchKernelGGL(k_sort_scatter, dim3((n + 255u) / 256u), dim3(256), 0, c->stream, keys, off, tot, nkeys, n,
                      nblk, per * 256u, d_perm);
   PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
 }
 
 int px_mlp_trace_sorted(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n, uint32_t buckets, float lo,
-                        float hi, uint32_t* d_perm, px_hit* d_out) {
+                        float hi, uint32_t flags, uint32_t* d_perm, px_hit* d_out) {
   if (!m) return PX_ERR_ARG;
-  int rc = px_mlp_sort_rays(m, kind, d_rays, n, buckets, lo, hi, d_perm);
+  int rc = px_mlp_sort_rays(m, kind, d_rays, n, buckets, lo, hi, flags, d_perm);
   if (rc) return rc;
   return px_trace_perm_device(m->c, kind, d_rays, d_perm, n, d_out);
 }

@@ -7,7 +7,8 @@ oracle (oracle/mlp.py):
 * features are in [-1, 1] with the occupancy samples at exactly +-1;
 * binning is a stable permutation sorted by bucket inside every window, and
   the permuted launch gives the plain launch's hits bit for bit;
-* the fused prediction + global sort equals the separate kernels' order;
+* the fused prediction + global sort equals the separate kernels' order,
+  and with (direction face, bucket) keys numpy's stable sort of them;
 * training on the DDA's own step counts cuts the predictor's held-out error
   to under a quarter of the untrained network's (the gain the training
   buys); the wavefront cost of its bins is printed beside the plain order
@@ -205,6 +206,16 @@ def test_fused_global_sort_equals_separate_kernels(world, kind):
             b = np.clip(np.floor((pr - np.float32(lo)) * scale), 0, nb - 1).astype(np.int64)
             assert np.array_equal(pm, rf)
             assert np.array_equal(pm, np.lexsort((np.arange(n), b)))
+            # (direction face, bucket) keys: 6 * nb = 192 keys
+            rd = rays["rd"].astype(np.float32)
+            ad = np.abs(rd)
+            ax = np.where(ad[:, 1] > ad[:, 0], 1, 0)
+            ax = np.where(ad[:, 2] > ad[np.arange(n), ax], 2, ax)
+            face = 2 * ax + (rd[np.arange(n), ax] < 0)
+            permf = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+            m.sort_rays(kind, d, n, permf, buckets=nb, lo=lo, hi=hi, faces=True)
+            gw.sync()
+            assert np.array_equal(permf.cpu().numpy().astype(np.int64), np.lexsort((np.arange(n), b, face)))
             if n > 1000:
                 assert len(np.unique(b)) > nb // 2  # the buckets are really used
                 plain = _trace_steps(sim, gw, kind, rays)
@@ -218,6 +229,9 @@ def test_fused_global_sort_equals_separate_kernels(world, kind):
                 m.trace_sorted(kind, d, n, perm2, out2, buckets=nb, lo=lo, hi=hi)
                 gw.sync()
                 assert torch.equal(perm2, perm) and np.array_equal(out2.cpu().numpy(), plain.view(np.uint8))
+                m.trace_sorted(kind, d, n, perm2, out2, buckets=nb, lo=lo, hi=hi, faces=True)
+                gw.sync()
+                assert torch.equal(perm2, permf) and np.array_equal(out2.cpu().numpy(), plain.view(np.uint8))
     finally:
         m.close()
 
