@@ -93,7 +93,11 @@ class GpuWorld:
         """Orders the library's stream after the work torch has queued on its
         current stream (an event wait, no host stall): device buffers that
         torch just wrote are complete before a kernel of the library reads
-        them.  The library's results are read back after px_sync()."""
+        them.  The library's results are read back after px_sync().  Tensors
+        handed to the library must stay alive until then: torch's allocator
+        does not see the library's stream, and marking them with
+        record_stream would make torch record events on that stream after
+        close() has destroyed it."""
         import torch
         if not torch.cuda.is_available():
             return
