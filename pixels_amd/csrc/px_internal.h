@@ -29,6 +29,7 @@ struct PxWorldView {
   const uint32_t* super_mask;  // 1 bit per super cell: count != 0
   const uint32_t* brick_mask;  // 1 bit per brick cell: grid != 0
   const uint64_t* super_bm;    // per super cell: 64-bit brick occupancy
+  const uint64_t* row_mask;    // per brick cell with a pool brick: bit y + 8z = x-row (y, z) has a voxel
   int32_t dims[3];
   int32_t bd[3];
   int32_t sd[3];
@@ -67,6 +68,10 @@ struct px_ctx {
   uint32_t* d_super_mask = nullptr;
   uint32_t* d_brick_mask = nullptr;
   uint64_t* d_super_bm = nullptr;
+  // x-row occupancy of each pool brick, by grid cell; rebuilt before a march
+  // when the masks have been rebuilt since (mask_gen != row_mask_gen)
+  uint64_t* d_row_mask = nullptr;
+  uint32_t mask_gen = 0, row_mask_gen = 0xFFFFFFFFu;
   bool masks_dirty = true;
   // frame
   uint32_t* d_shadow_queue = nullptr;  // output slot of each queued shadow ray
@@ -181,6 +186,9 @@ struct PxProfScope {
 };
 PxWorldView px_view(const px_ctx* c);
 int px_refresh_masks(px_ctx* c);
+// px_refresh_masks, then the row masks march() reads; every launcher that
+// runs march() calls this instead.
+int px_refresh_march(px_ctx* c);
 // Dynamic LDS of march()'s tables: 8 glass words, then the super-cell bits
 // when they fit in 64 KB (*sup_lds; else the kernels read the global copy).
 inline size_t px_table_lds(const px_ctx* c, bool* sup_lds) {

@@ -558,7 +558,7 @@ int px_refresh_density(px_ctx* c) {
 
 int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba) {
   if (n == 0) return PX_OK;
-  int rc = px_refresh_masks(c);
+  int rc = px_refresh_march(c);
   if (rc) return rc;
   if ((rc = px_refresh_density(c))) return rc;
   const PxShadeDesc S = shade_desc(c);

@@ -437,6 +437,23 @@ __global__ void k_super_bm(const uint32_t* __restrict__ grid, int32_t bx, int32_
   out[si] = m;
 }
 
+// x-row occupancy of every pool brick, stored by grid cell: one wavefront per
+// cell, lane l reads row l (y = l & 7, z = l >> 3: 8 voxels, 16 bytes) and the
+// ballot of the non-empty rows is the mask.  Cells without a pool brick are
+// skipped (march() reads the mask of pool-brick cells only).
+__global__ __launch_bounds__(256) void k_row_mask(const uint32_t* __restrict__ grid, const uint32_t* __restrict__ pool,
+                                                  uint32_t ncell, uint64_t* __restrict__ rows) {
+  const uint32_t lane = __lane_id();
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t ci = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); ci < ncell; ci += waves) {
+    const uint32_t g = grid[ci];
+    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
+    const uint4 w = reinterpret_cast<const uint4*>(pool + (size_t)(g - 1u) * 256u)[lane];
+    const uint64_t m = __ballot((w.x | w.y | w.z | w.w) != 0u);
+    if (lane == 0u) rows[ci] = m;
+  }
+}
+
 }  // namespace
 
 PxWorldView px_view(const px_ctx* c) {
@@ -448,6 +465,7 @@ PxWorldView px_view(const px_ctx* c) {
   w.super_mask = c->d_super_mask;
   w.brick_mask = c->d_brick_mask;
   w.super_bm = c->d_super_bm;
+  w.row_mask = c->d_row_mask;
   for (int a = 0; a < 3; a++) {
     w.dims[a] = (int32_t)c->dims[a];
     w.bd[a] = (int32_t)c->bd[a];
@@ -469,10 +487,23 @@ int px_launch_masks(px_ctx* c) {
                      c->d_super_bm);
   PX_HIP_CHECK(c, hipGetLastError());
   c->masks_dirty = false;
+  c->mask_gen++;
   return PX_OK;
 }
 
 int px_refresh_masks(px_ctx* c) { return c->masks_dirty ? px_launch_masks(c) : PX_OK; }
+
+int px_refresh_march(px_ctx* c) {
+  int rc = px_refresh_masks(c);
+  if (rc) return rc;
+  if (c->row_mask_gen == c->mask_gen) return PX_OK;
+  const uint32_t blocks = std::min<uint32_t>((c->ncell + 3u) / 4u, c->n_cu * 16u);
+  hipLaunchKernelGGL(k_row_mask, dim3(std::max(blocks, 1u)), dim3(256), 0, c->stream, c->d_grid, c->d_pool, c->ncell,
+                     c->d_row_mask);
+  PX_HIP_CHECK(c, hipGetLastError());
+  c->row_mask_gen = c->mask_gen;
+  return PX_OK;
+}
 
 
 template <int KIND>
@@ -486,7 +517,7 @@ static void launch_trace_kind(px_ctx* c, const PxWorldView& w, const px_ray* d_r
 
 int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, px_hit* d_out) {
   if (n == 0) return PX_OK;
-  int rc = px_refresh_masks(c);
+  int rc = px_refresh_march(c);
   if (rc) return rc;
   PxWorldView w = px_view(c);
   PxProfScope prof(c, PX_PROF_TRACE);
@@ -514,7 +545,7 @@ static void launch_perm_kind(px_ctx* c, const PxWorldView& w, const px_ray* d_ra
 int px_launch_trace_perm(px_ctx* c, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
                          px_hit* d_out) {
   if (n == 0) return PX_OK;
-  int rc = px_refresh_masks(c);
+  int rc = px_refresh_march(c);
   if (rc) return rc;
   PxWorldView w = px_view(c);
   PxProfScope prof(c, PX_PROF_TRACE);
@@ -534,7 +565,7 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
     if (n_cast) *n_cast = 0;
     return PX_OK;
   }
-  int rc = px_refresh_masks(c);
+  int rc = px_refresh_march(c);
   if (rc) return rc;
   // shadow-ray queue: one region of 64 * slots entries per wavefront of primaries
   const uint32_t n_regions = (n + 63u) / 64u;
@@ -621,7 +652,7 @@ static uint32_t batch_blocks(const px_ctx* c) {
 // Captures [cursor reset, k_trace_batch] into a graph for this query set and
 // shape (kept while they stay the same) and launches it.
 int px_launch_batch(px_ctx* c, const px_query_set* q) {
-  int rc = px_refresh_masks(c);
+  int rc = px_refresh_march(c);
   if (rc) return rc;
   const BatchArgs a = batch_args(c, q);
   if (a.total_chunks == 0u) return PX_OK;
