@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) void k_bin(const float* __restrict__ pred, uin
 }
 
 // ---------------------------------------------------------------------------
-// Prediction and a stable global counting sort by cost bucket, fused.
+// Prediction and a stable global counting sort by cost key, fused.
 //
 // The rays are cut into tiles of 256, 64 per wavefront; block g owns tiles
 // [g R, (g + 1) R).  Each wavefront computes its 64 rays' features, stages
@@ -994,7 +994,7 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   if (kind > 3u) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown ray kind");
   if (flags & ~PX_SORT_FACES) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown flags");
   const uint32_t faces = (flags & PX_SORT_FACES) ? 1u : 0u;
-  if (buckets == 0u || buckets * (faces ? 6u : 1u) > kSortMaxKeys)
+  if (buckets == 0u || (uint64_t)buckets * (faces ? 6u : 1u) > kSortMaxKeys)
     return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "buckets: 1..256, or 1..42 with PX_SORT_FACES");
   if (!(hi > lo)) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "need hi > lo");
   if (n == 0) return PX_OK;

@@ -236,6 +236,38 @@ def test_fused_global_sort_equals_separate_kernels(world, kind):
         m.close()
 
 
+def test_sort_rays_rejects_bad_arguments(world):
+    """Unknown flags, more than 256 keys (42 buckets with faces; a bucket
+    count whose 6x overflows 32 bits), an empty bucket range and an unknown
+    kind are errors, and nothing is written."""
+    import torch
+    from pixels_amd import PxError
+    from pixels_amd.learned import CostModel
+    s, gw, sim = world
+    m = CostModel(gw, seed=2)
+    try:
+        rays = s.camera_rays(64, 32, 90.0, seed=3)
+        n = rays.shape[0]
+        d = _dev(rays.view(np.uint8))
+        perm = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+        bad = [dict(buckets=43, faces=True), dict(buckets=257), dict(buckets=0), dict(buckets=0x2AAAAAAB, faces=True),
+               dict(lo=3.0, hi=3.0)]
+        for kw in bad:
+            with pytest.raises(PxError):
+                m.sort_rays(1, d, n, perm, **kw)
+        with pytest.raises(PxError):
+            m.sort_rays(7, d, n, perm)
+        with pytest.raises(PxError):
+            gw.check(gw.lib.px_mlp_sort_rays(m.h, 1, d.data_ptr(), n, 32, 0.0, 9.0, 2, perm.data_ptr()))
+        gw.sync()
+        assert bool((perm == -1).all())
+        m.sort_rays(1, d, n, perm, buckets=42, faces=True)  # the largest face-keyed sort
+        gw.sync()
+        assert np.array_equal(np.sort(perm.cpu().numpy()), np.arange(n))
+    finally:
+        m.close()
+
+
 def test_training_gain_over_untrained(world):
     """Training on the DDA's own step counts makes the predictor far better on
     a held-out camera than the network it started from; the wavefront cost
