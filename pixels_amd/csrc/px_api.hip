@@ -88,7 +88,7 @@ static void release(px_ctx* c) {
   for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
   if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
   void* bufs[] = {c->d_grid, c->d_pool, c->d_super, c->d_mats, c->d_ctrl, c->d_args, c->d_camera, c->d_free,
-                  c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_row_mask, c->d_shadow_queue,
+                  c->d_super_mask, c->d_brick_mask, c->d_super_bm, c->d_sub_mask, c->d_shadow_queue,
                   c->d_wave_count, c->d_counters,
                   c->d_batch_cursors, c->d_density, c->d_rprim, c->d_rtg, c->d_rsh, c->d_rrq, c->d_rrc,
                   c->b_grid, c->b_pool, c->b_super, c->b_ctrl, c->b_free};
@@ -170,7 +170,7 @@ static int ctx_world(px_ctx* c, const px_world_desc* d, uint32_t used, const uin
   TRY(hipMalloc(&c->d_brick_mask, (size_t)((c->ncell + 31u) / 32u) * 4 + 256));
   TRY(hipMalloc(&c->d_super_mask, (size_t)((c->nsuper + 31u) / 32u) * 4 + 256));
   TRY(hipMalloc(&c->d_super_bm, (size_t)c->nsuper * 8 + 256));
-  TRY(hipMalloc(&c->d_row_mask, (size_t)c->ncell * 8 + 256));
+  TRY(hipMalloc(&c->d_sub_mask, (size_t)c->ncell * 8 + 256));
   TRY(hipMalloc(&c->d_counters, 256));
   TRY(hipMalloc(&c->d_batch_cursors, 8 * 256));
   {

@@ -29,7 +29,7 @@ struct PxWorldView {
   const uint32_t* super_mask;  // 1 bit per super cell: count != 0
   const uint32_t* brick_mask;  // 1 bit per brick cell: grid != 0
   const uint64_t* super_bm;    // per super cell: 64-bit brick occupancy
-  const uint64_t* row_mask;    // per brick cell with a pool brick: bit y + 8z = x-row (y, z) has a voxel
+  const uint64_t* sub_mask;    // per brick cell with a pool brick: bit (x/2) + 4 (y/2) + 16 (z/2) = that 2x2x2 block has a voxel
   int32_t dims[3];
   int32_t bd[3];
   int32_t sd[3];
@@ -68,10 +68,10 @@ struct px_ctx {
   uint32_t* d_super_mask = nullptr;
   uint32_t* d_brick_mask = nullptr;
   uint64_t* d_super_bm = nullptr;
-  // x-row occupancy of each pool brick, by grid cell; rebuilt before a march
-  // when the masks have been rebuilt since (mask_gen != row_mask_gen)
-  uint64_t* d_row_mask = nullptr;
-  uint32_t mask_gen = 0, row_mask_gen = 0xFFFFFFFFu;
+  // 2x2x2-block occupancy of each pool brick, by grid cell; rebuilt before a
+  // march when the masks have been rebuilt since (mask_gen != sub_mask_gen)
+  uint64_t* d_sub_mask = nullptr;
+  uint32_t mask_gen = 0, sub_mask_gen = 0xFFFFFFFFu;
   bool masks_dirty = true;
   // frame
   uint32_t* d_shadow_queue = nullptr;  // output slot of each queued shadow ray
@@ -186,7 +186,7 @@ struct PxProfScope {
 };
 PxWorldView px_view(const px_ctx* c);
 int px_refresh_masks(px_ctx* c);
-// px_refresh_masks, then the row masks march() reads; every launcher that
+// px_refresh_masks, then the sub-block masks march() reads; every launcher that
 // runs march() calls this instead.
 int px_refresh_march(px_ctx* c);
 // Dynamic LDS of march()'s tables: 8 glass words, then the super-cell bits

@@ -241,9 +241,9 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
                          bcz = (uint32_t)(sc[2] * 4 + lb[2]);
           const uint32_t cidx = bcx + (uint32_t)w.bd[0] * (bcy + (uint32_t)w.bd[1] * bcz);
           const uint32_t g = w.grid[cidx];
-          // x-row occupancy of a pool brick (read beside the grid word): a
-          // voxel in an empty row is 0 without a pool read
-          const uint64_t rows = w.row_mask[cidx];
+          // 2x2x2-block occupancy of a pool brick (read beside the grid
+          // word): a voxel in an empty block is 0 without a pool read
+          const uint64_t subs = w.sub_mask[cidx];
           const bool uni = (g & PX_UNIFORM_BIT) != 0u;
           if (uni && !bit_of(T.glass, g & 0xffu)) {
             o.key = key_of(w, (int32_t)bcx * 8 + lv[0], (int32_t)bcy * 8 + lv[1], (int32_t)bcz * 8 + lv[2]);
@@ -265,7 +265,8 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
             const uint32_t li = (uint32_t)lv[0] | ((uint32_t)lv[1] << 3) | ((uint32_t)lv[2] << 6);
             uint32_t v = 0u;
             if (uni) v = g & 0xffffu;
-            else if ((rows >> (li >> 3)) & 1ull)
+            else if ((subs >> (((uint32_t)lv[0] >> 1) | (((uint32_t)lv[1] << 1) & 12u) | (((uint32_t)lv[2] << 3) & 48u))) &
+                     1ull)
               v = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(w.pool) + (uint64_t)(bbase + li * 2u));
             if (v != 0u) {
               const uint32_t id = v & 0xffu;

@@ -437,20 +437,32 @@ __global__ void k_super_bm(const uint32_t* __restrict__ grid, int32_t bx, int32_
   out[si] = m;
 }
 
-// x-row occupancy of every pool brick, stored by grid cell: one wavefront per
-// cell, lane l reads row l (y = l & 7, z = l >> 3: 8 voxels, 16 bytes) and the
-// ballot of the non-empty rows is the mask.  Cells without a pool brick are
-// skipped (march() reads the mask of pool-brick cells only).
-__global__ __launch_bounds__(256) void k_row_mask(const uint32_t* __restrict__ grid, const uint32_t* __restrict__ pool,
-                                                  uint32_t ncell, uint64_t* __restrict__ rows) {
+// 2x2x2-block occupancy of every pool brick, stored by grid cell: bit
+// x2 + 4 y2 + 16 z2 is set when the block of voxels (2 x2 .. 2 x2 + 1, ...)
+// has a non-zero voxel.  One wavefront per cell: lane l reads row l (y = l & 7,
+// z = l >> 3; 4 words of 2 voxels), and one ballot per word gives the rows in
+// which that x pair has a voxel; OR-ing neighbouring rows in y and z gives
+// the blocks.  Cells without a pool brick are skipped (march() reads the
+// mask of pool-brick cells only).
+__global__ __launch_bounds__(256) void k_sub_mask(const uint32_t* __restrict__ grid, const uint32_t* __restrict__ pool,
+                                                  uint32_t ncell, uint64_t* __restrict__ subs) {
   const uint32_t lane = __lane_id();
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   for (uint32_t ci = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); ci < ncell; ci += waves) {
     const uint32_t g = grid[ci];
     if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
     const uint4 w = reinterpret_cast<const uint4*>(pool + (size_t)(g - 1u) * 256u)[lane];
-    const uint64_t m = __ballot((w.x | w.y | w.z | w.w) != 0u);
-    if (lane == 0u) rows[ci] = m;
+    // word k of row (y, z) holds voxels x = 2k, 2k + 1: sub-block column x2 = k
+    const uint64_t b[4] = {__ballot(w.x != 0u), __ballot(w.y != 0u), __ballot(w.z != 0u), __ballot(w.w != 0u)};
+    uint64_t m = 0ull;
+    for (uint32_t x2 = 0; x2 < 4u; x2++) {
+      uint64_t t = b[x2] | (b[x2] >> 1);  // rows y, y + 1 (bit at even y)
+      t |= t >> 8;                         // rows z, z + 1 (bit at even z)
+      for (uint32_t z2 = 0; z2 < 4u; z2++)
+        for (uint32_t y2 = 0; y2 < 4u; y2++)
+          if ((t >> (2u * y2 + 16u * z2)) & 1ull) m |= 1ull << (x2 | (y2 << 2) | (z2 << 4));
+    }
+    if (lane == 0u) subs[ci] = m;
   }
 }
 
@@ -465,7 +477,7 @@ PxWorldView px_view(const px_ctx* c) {
   w.super_mask = c->d_super_mask;
   w.brick_mask = c->d_brick_mask;
   w.super_bm = c->d_super_bm;
-  w.row_mask = c->d_row_mask;
+  w.sub_mask = c->d_sub_mask;
   for (int a = 0; a < 3; a++) {
     w.dims[a] = (int32_t)c->dims[a];
     w.bd[a] = (int32_t)c->bd[a];
@@ -496,12 +508,12 @@ int px_refresh_masks(px_ctx* c) { return c->masks_dirty ? px_launch_masks(c) : P
 int px_refresh_march(px_ctx* c) {
   int rc = px_refresh_masks(c);
   if (rc) return rc;
-  if (c->row_mask_gen == c->mask_gen) return PX_OK;
+  if (c->sub_mask_gen == c->mask_gen) return PX_OK;
   const uint32_t blocks = std::min<uint32_t>((c->ncell + 3u) / 4u, c->n_cu * 16u);
-  hipLaunchKernelGGL(k_row_mask, dim3(std::max(blocks, 1u)), dim3(256), 0, c->stream, c->d_grid, c->d_pool, c->ncell,
-                     c->d_row_mask);
+  hipLaunchKernelGGL(k_sub_mask, dim3(std::max(blocks, 1u)), dim3(256), 0, c->stream, c->d_grid, c->d_pool, c->ncell,
+                     c->d_sub_mask);
   PX_HIP_CHECK(c, hipGetLastError());
-  c->row_mask_gen = c->mask_gen;
+  c->sub_mask_gen = c->mask_gen;
   return PX_OK;
 }
 
