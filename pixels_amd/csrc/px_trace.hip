@@ -437,32 +437,61 @@ __global__ void k_super_bm(const uint32_t* __restrict__ grid, int32_t bx, int32_
   out[si] = m;
 }
 
-// 2x2x2-block occupancy of every pool brick, stored by grid cell: bit
-// x2 + 4 y2 + 16 z2 is set when the block of voxels (2 x2 .. 2 x2 + 1, ...)
-// has a non-zero voxel.  One wavefront per cell: lane l reads row l (y = l & 7,
-// z = l >> 3; 4 words of 2 voxels), and one ballot per word gives the rows in
-// which that x pair has a voxel; OR-ing neighbouring rows in y and z gives
-// the blocks.  Cells without a pool brick are skipped (march() reads the
-// mask of pool-brick cells only).
+// 2x2x2-block occupancy of a pool brick: bit x2 + 4 y2 + 16 z2 is set when
+// the block of voxels (2 x2 .. 2 x2 + 1, 2 y2 .., 2 z2 ..) has a non-zero
+// voxel.  w is row l = lane of the brick (y = l & 7, z = l >> 3; 4 words of 2
+// voxels); one ballot per word gives the rows in which that x pair has a
+// voxel, and OR-ing neighbouring rows in y and z gives the blocks.  The mask
+// is stored by grid cell; cells without a pool brick are skipped (march()
+// reads the mask of pool-brick cells only).
+PX_DEV uint64_t sub_bits(uint4 w) {
+  // word k of row (y, z) holds voxels x = 2k, 2k + 1: sub-block column x2 = k
+  const uint64_t b[4] = {__ballot(w.x != 0u), __ballot(w.y != 0u), __ballot(w.z != 0u), __ballot(w.w != 0u)};
+  uint64_t m = 0ull;
+  for (uint32_t x2 = 0; x2 < 4u; x2++) {
+    uint64_t t = b[x2] | (b[x2] >> 1);  // rows y, y + 1 (bit at even y)
+    t |= t >> 8;                         // rows z, z + 1 (bit at even z)
+    // compact the bits at 2 y2 + 16 z2 into bit y2 + 4 z2 ...
+    t &= 0x0055005500550055ull;
+    t = (t | (t >> 1)) & 0x0033003300330033ull;
+    t = (t | (t >> 2)) & 0x000F000F000F000Full;
+    t = (t | (t >> 12)) & 0x000000FF000000FFull;
+    t = (t | (t >> 24)) & 0xFFFFull;
+    // ... then spread bit y2 + 4 z2 to bit 4 (y2 + 4 z2) and add x2
+    t = (t | (t << 24)) & 0x000000FF000000FFull;
+    t = (t | (t << 12)) & 0x000F000F000F000Full;
+    t = (t | (t << 6)) & 0x0303030303030303ull;
+    t = (t | (t << 3)) & 0x1111111111111111ull;
+    m |= t << x2;
+  }
+  return m;
+}
+
+// Each wavefront owns 64 consecutive cells: one coalesced read of their grid
+// words, then their pool bricks four at a time (loads first, so they
+// overlap).  Lane l reads row l of a brick (y = l & 7, z = l >> 3; 4 words).
 __global__ __launch_bounds__(256) void k_sub_mask(const uint32_t* __restrict__ grid, const uint32_t* __restrict__ pool,
                                                   uint32_t ncell, uint64_t* __restrict__ subs) {
   const uint32_t lane = __lane_id();
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t ci = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); ci < ncell; ci += waves) {
-    const uint32_t g = grid[ci];
-    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
-    const uint4 w = reinterpret_cast<const uint4*>(pool + (size_t)(g - 1u) * 256u)[lane];
-    // word k of row (y, z) holds voxels x = 2k, 2k + 1: sub-block column x2 = k
-    const uint64_t b[4] = {__ballot(w.x != 0u), __ballot(w.y != 0u), __ballot(w.z != 0u), __ballot(w.w != 0u)};
-    uint64_t m = 0ull;
-    for (uint32_t x2 = 0; x2 < 4u; x2++) {
-      uint64_t t = b[x2] | (b[x2] >> 1);  // rows y, y + 1 (bit at even y)
-      t |= t >> 8;                         // rows z, z + 1 (bit at even z)
-      for (uint32_t z2 = 0; z2 < 4u; z2++)
-        for (uint32_t y2 = 0; y2 < 4u; y2++)
-          if ((t >> (2u * y2 + 16u * z2)) & 1ull) m |= 1ull << (x2 | (y2 << 2) | (z2 << 4));
+  const uint32_t c0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+  if (c0 >= ncell) return;
+  const uint32_t gl = c0 + lane < ncell ? grid[c0 + lane] : 0u;
+  uint64_t todo = __ballot(gl != 0u && (gl & PX_UNIFORM_BIT) == 0u);
+  const uint4* p4 = reinterpret_cast<const uint4*>(pool);
+  while (todo) {
+    uint32_t k[4];
+    uint4 w[4];
+    for (int j = 0; j < 4; j++) {
+      k[j] = todo ? (uint32_t)(__ffsll((unsigned long long)todo) - 1) : 64u;
+      todo &= todo - 1ull;
     }
-    if (lane == 0u) subs[ci] = m;
+    for (int j = 0; j < 4; j++)
+      if (k[j] < 64u) w[j] = p4[(size_t)(__shfl(gl, (int)k[j]) - 1u) * 64u + lane];
+    for (int j = 0; j < 4; j++)
+      if (k[j] < 64u) {
+        const uint64_t m = sub_bits(w[j]);
+        if (lane == 0u) subs[c0 + k[j]] = m;
+      }
   }
 }
 
@@ -509,9 +538,8 @@ int px_refresh_march(px_ctx* c) {
   int rc = px_refresh_masks(c);
   if (rc) return rc;
   if (c->sub_mask_gen == c->mask_gen) return PX_OK;
-  const uint32_t blocks = std::min<uint32_t>((c->ncell + 3u) / 4u, c->n_cu * 16u);
-  hipLaunchKernelGGL(k_sub_mask, dim3(std::max(blocks, 1u)), dim3(256), 0, c->stream, c->d_grid, c->d_pool, c->ncell,
-                     c->d_sub_mask);
+  hipLaunchKernelGGL(k_sub_mask, dim3((c->ncell + 255u) / 256u), dim3(256), 0, c->stream, c->d_grid, c->d_pool,
+                     c->ncell, c->d_sub_mask);
   PX_HIP_CHECK(c, hipGetLastError());
   c->sub_mask_gen = c->mask_gen;
   return PX_OK;
