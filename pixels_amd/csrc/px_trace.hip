@@ -538,9 +538,11 @@ int px_refresh_march(px_ctx* c) {
   int rc = px_refresh_masks(c);
   if (rc) return rc;
   if (c->sub_mask_gen == c->mask_gen) return PX_OK;
-  hipLaunchKernelGGL(k_sub_mask, dim3((c->ncell + 255u) / 256u), dim3(256), 0, c->stream, c->d_grid, c->d_pool,
-                     c->ncell, c->d_sub_mask);
-  PX_HIP_CHECK(c, hipGetLastError());
+  if (c->ncell) {
+    hipLaunchKernelGGL(k_sub_mask, dim3((c->ncell + 255u) / 256u), dim3(256), 0, c->stream, c->d_grid, c->d_pool,
+                       c->ncell, c->d_sub_mask);
+    PX_HIP_CHECK(c, hipGetLastError());
+  }
   c->sub_mask_gen = c->mask_gen;
   return PX_OK;
 }
