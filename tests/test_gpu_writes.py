@@ -100,6 +100,52 @@ def test_writes_match_oracle(floor_scene, case):
         gw.close()
 
 
+def test_trace_sees_writes_inside_pool_bricks(floor_scene):
+    """Traversal reads a per-brick mask of its non-empty 2x2x2 voxel blocks.
+    Writes that fill air voxels of existing pool bricks (no brick changes
+    occupancy) must reach it: rays traced before and after the writes equal
+    the oracle's on the world before and after, bit for bit."""
+    from pixels_amd import GpuWorld, Scene, Simulation
+    s = floor_scene
+    gw = GpuWorld(s)
+    sim = Simulation(gw)
+    try:
+        rays = s.camera_rays(192, 108, 90.0, seed=11)
+        print("case: trace, writes inside pool bricks, trace", flush=True)
+        before = sim.trace(1, rays)
+        ow0 = oracle.World.from_scene(s)
+        assert np.array_equal(before.view(np.uint8), ow0.trace(1, rays).view(np.uint8))
+        # air voxels of pool bricks, two voxels in front of primary hits
+        hits = sim.trace(0, rays)
+        ok = hits["face"] < 6
+        p = rays["ro"][ok] + rays["rd"][ok] * (hits["t"][ok, None] - 2.0)
+        v = np.floor(p).astype(np.int64)
+        dx, dy, dz = s.dims
+        inb = (v >= 0).all(1) & (v[:, 0] < dx) & (v[:, 1] < dy) & (v[:, 2] < dz)
+        v = v[inb]
+        dense = s.dense()
+        air = dense[v[:, 2], v[:, 1], v[:, 0]] == 0
+        cell = (v[:, 0] // 8) + s.bricks[0] * ((v[:, 1] // 8) + s.bricks[1] * (v[:, 2] // 8))
+        g = s.grid[cell]
+        pool_brick = (g != 0) & ((g & 0x80000000) == 0)
+        v = v[air & pool_brick]
+        assert v.shape[0] > 100
+        key = (v[:, 0] + dx * (v[:, 1] + dy * v[:, 2])).astype(np.uint32)
+        w = np.stack([key, np.full(key.shape[0], 5, np.uint32)], 1)
+        sim.push_writes(w)
+        sim.apply_writes()
+        grid, _ = gw.read_grid()
+        assert np.array_equal(kinds_of(grid), kinds_of(s.grid))  # no brick changed occupancy
+        vox = sim.probe((0, 0, 0), s.dims).reshape(dz, dy, dx)
+        ow1 = oracle.World.from_scene(Scene.from_dense(vox.astype(np.uint16), s.mats))
+        after = sim.trace(1, rays)
+        ref = ow1.trace(1, rays)
+        assert np.array_equal(after.view(np.uint8), ref.view(np.uint8))
+        assert not np.array_equal(after.view(np.uint8), before.view(np.uint8))
+    finally:
+        gw.close()
+
+
 def test_debris_settles_into_voxels(floor_scene):
     """frameBegin + particleSim + writes over frames: the settled deposit
     particles become rubble voxels exactly as the oracle chain gives."""
