@@ -343,9 +343,11 @@ __global__ __launch_bounds__(256) void k_render_shadow(PxWorldView w, PxShadowDe
                                                        float4* __restrict__ sh_out) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
-  const uint32_t region = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // regions last first, as in k_trace_shadow_queue (costly rows first)
   const uint32_t n_regions = (n + 63u) / 64u;
-  if (region >= n_regions) return;
+  const uint32_t lw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (lw >= n_regions) return;
+  const uint32_t region = n_regions - 1u - lw;
   const uint32_t slots = sd.slots;
   const uint32_t first = region * 64u * slots, last = min(region * 64u + 64u, n) * slots;
   const uint32_t lane = __lane_id();

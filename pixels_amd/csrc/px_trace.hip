@@ -260,9 +260,13 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
                                                             uint32_t n, px_hit* __restrict__ sh_out) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
-  const uint32_t region = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // wavefronts take the regions last first: regions follow the image rows,
+  // and the lower rows (near ground, whose shadow rays cross the scene) cost
+  // the most, so issuing them first shortens the tail of the launch
   const uint32_t n_regions = (n + 63u) / 64u;
-  if (region >= n_regions) return;
+  const uint32_t lw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (lw >= n_regions) return;
+  const uint32_t region = n_regions - 1u - lw;
   const uint32_t slots = sd.slots;
   const uint32_t first = region * 64u * slots, last = min(region * 64u + 64u, n) * slots;
   const uint32_t lane = __lane_id();
