@@ -308,9 +308,11 @@ __global__ __launch_bounds__(256) void k_render_reflect(PxWorldView w, PxShadeDe
                                                         const uint8_t* __restrict__ dens, float4* __restrict__ rc) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= *total) return;
-  const uint32_t i = dense[j];
+  // the dense list last first, as the shadow regions (costly rows first)
+  const uint32_t tot = *total;
+  const uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jj >= tot) return;
+  const uint32_t i = dense[tot - 1u - jj];
   float ro[3], rd[3], maxT;
   uint32_t flags;
   px::load_ray(rays + i, ro, rd, maxT, flags);
