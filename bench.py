@@ -70,6 +70,9 @@ def main() -> int:
     ap.add_argument("--no-debris", action="store_true", help="skip the debris-particle leg")
     ap.add_argument("--no-scene", action="store_true", help="skip the GPU scene-build leg")
     ap.add_argument("--no-render", action="store_true", help="skip the shading-to-an-image leg")
+    ap.add_argument("--layout", choices=("planar", "pixel"), default="planar",
+                    help="shadow outputs of the frame: slot-planar (s * n + i) or pixel-major (i * slots + s)")
+    ap.add_argument("--compare-layout", action="store_true", help="also time the frame with the other layout")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the hot path is per-GPU work (rays shard with the brick map "
@@ -101,6 +104,8 @@ def main() -> int:
     d_sh = torch.empty(n * slots * 16, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(gw.stream)
+    planar = args.layout == "planar"
+    sim.set_frame_layout(planar)
 
     for _ in range(args.warmup):
         sim.trace_frame_device(d_rays, n, d_prim, d_sh)
@@ -132,6 +137,23 @@ def main() -> int:
 
     prim = d_prim.cpu().numpy().view(N.HIT_DTYPE)
     sh = d_sh.cpu().numpy().view(N.HIT_DTYPE)
+    if planar:  # to pixel-major (slot s of ray i at i * slots + s), as the oracle writes it
+        sh = np.ascontiguousarray(sh.reshape(slots, n).T).reshape(-1)
+    other_ms = None
+    if args.compare_layout:
+        # the other layout, timed the same way (same records)
+        sim.set_frame_layout(not planar)
+        for _ in range(args.warmup):
+            sim.trace_frame_device(d_rays, n, d_prim, d_sh)
+        gw.sync()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            sim.trace_frame_device(d_rays, n, d_prim, d_sh)
+        ev1.record(stream)
+        ev1.synchronize()
+        other_ms = round(ev0.elapsed_time(ev1) / args.steps, 4)
+        sim.set_frame_layout(planar)
+        log("other layout: %.3f ms/frame" % other_ms)
     cast = sh["face"] != N.FACE_NOT_CAST
     n_cast = int(cast.sum())
     total = n + n_cast
@@ -200,7 +222,9 @@ def main() -> int:
         "config": {"workload": "frame: 1920x1080 primary + sun + 8 point-light shadow rays, 1280x352x960 grid",
                    "grid": list(scene.dims), "primitives": scene.n_prims, "pool_bricks": scene.pool_used,
                    "fill": round(scene.fill, 4), "primary_rays": n, "shadow_rays": n_cast,
-                   "bit_exact_vs_oracle": parity},
+                   "bit_exact_vs_oracle": parity,
+                   "shadow_layout": "slot-planar" if planar else "pixel-major",
+                   "other_layout_ms_per_step": other_ms},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "island_pass": island,
@@ -555,43 +579,6 @@ def debris_leg(args, scene, gw, u, prim_rays, prim_hits):
     return res
 
 
-def shadow_queries(scene, prim_rays, prim_hits, u, lights, n_max):
-    """Shadow rays of shade() (raymarch.wgsl:323-356) for primary hits, in
-    pixel-then-slot order, repeated or cut to exactly n_max (host arithmetic;
-    they are inputs)."""
-    ok = (prim_hits["face"] != 0xFF) & ~(scene.mats.reshape(-1, 20)[prim_hits["mat"], 3] > 0)
-    r, h = prim_rays[ok], prim_hits[ok]
-    n = np.zeros((r.shape[0], 3), np.float32)
-    f = h["face"].astype(np.int64)
-    sel = f < 6
-    n[np.nonzero(sel)[0], f[sel] >> 1] = np.where(f[sel] & 1, -1.0, 1.0)
-    p = r["ro"] + r["rd"] * h["t"][:, None]
-    o = (p + n * np.float32(0.5)).astype(np.float32)
-    vs, sun = u[3], u[4:7]
-    cand = []
-    c = (n @ sun) > 0
-    cand.append((c, np.broadcast_to(sun, n.shape), np.full(n.shape[0], 1e9, np.float32)))
-    l4 = lights.reshape(-1, 4)
-    for i in range(int(l4[0, 0])):
-        lp = l4[1 + 2 * i]
-        d = lp[:3] / vs - p
-        dist = np.sqrt((d * d).sum(1))
-        dn = d / np.maximum(dist, 1e-12)[:, None]
-        cand.append(((dist <= lp[3] / vs) & ((n * dn).sum(1) > 0), dn, dist - 2.0))
-    sel = np.stack([cc[0] for cc in cand], 1)  # (hits, slots)
-    rows, slots = np.nonzero(sel)
-    rows, slots = np.resize(rows, n_max), np.resize(slots, n_max)
-    from pixels_amd import _native as N
-    out = np.zeros(rows.shape[0], N.RAY_DTYPE)
-    out["ro"] = o[rows]
-    dirs = np.stack([cc[1] for cc in cand], 1)
-    mts = np.stack([cc[2] for cc in cand], 1)
-    out["rd"] = dirs[rows, slots]
-    out["max_t"] = mts[rows, slots]
-    out["flags"] = 1
-    return out
-
-
 def batch_leg(args):
     """Queries of very uneven counts on the 2560x704x1920 scene (the
     full-scale generator scaled x2, 160 voxels per metre): 2M primary, 3M
@@ -603,6 +590,7 @@ def batch_leg(args):
     from pixels_amd import GpuWorld, Scene, Simulation
     from pixels_amd import _native as N
     from pixels_amd.scene import LARGE_SCALED
+    from pixels_amd.workloads import mixed_queries
 
     t0 = time.time()
     scene = Scene.build(LARGE_SCALED, seed=args.seed, dims=(2560, 704, 1920), scale=2.0)
@@ -612,32 +600,7 @@ def batch_leg(args):
     gw = GpuWorld(scene, voxel_size=vs)
     sim = Simulation(gw)
     dev = torch.device("cuda:0")
-    rng = np.random.default_rng(args.seed + 100)
-    W, H = 2048, 1024
-    prim = scene.camera_rays(W, H, 90.0, seed=args.seed + 1, tile=8)
-    u, lights = scene.frame_inputs(W, H, seed=args.seed + 1, voxel_size=vs, light_radius_vox=8192.0)
-    d_prim = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
-    d_out0 = torch.empty(prim.shape[0] * 16, dtype=torch.uint8, device=dev)
-    sim.trace_device(0, d_prim, prim.shape[0], d_out0)
-    gw.sync()
-    ph = d_out0.cpu().numpy().view(N.HIT_DTYPE)
-    shadow = shadow_queries(scene, prim, ph, u, lights, 3 * (1 << 20))
-    n_shoot, n_seg = 50000, 262144
-    shoot = np.zeros(n_shoot, N.RAY_DTYPE)
-    origins = np.zeros((64, 3), np.float32)
-    for i in range(64):
-        origins[i] = scene.camera(seed=args.seed + 200 + i)[:3]
-    shoot["ro"] = origins[rng.integers(0, 64, n_shoot)]
-    d = rng.normal(size=(n_shoot, 3))
-    shoot["rd"] = d / np.linalg.norm(d, axis=1, keepdims=True)
-    shoot["max_t"] = 80.0 / vs
-    shoot["flags"] = 2
-    seg = np.zeros(n_seg, N.RAY_DTYPE)
-    seg["ro"] = rng.uniform([0, 0, 0], scene.dims, size=(n_seg, 3))
-    d = rng.normal(size=(n_seg, 3))
-    seg["rd"] = d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(1, 200, size=(n_seg, 1))
-    seg["flags"] = 3
-    host = {0: prim, 1: shadow, 2: shoot, 3: seg}
+    host = mixed_queries(scene, sim, gw, args.seed, vs=vs)
     rays = {k: torch.from_numpy(np.ascontiguousarray(v).view(np.uint8).copy()).to(dev) for k, v in host.items()}
     outb = {k: torch.empty(v.shape[0] * 16, dtype=torch.uint8, device=dev) for k, v in host.items()}
     outp = {k: torch.empty(v.shape[0] * 16, dtype=torch.uint8, device=dev) for k, v in host.items()}
@@ -706,6 +669,7 @@ def learned_leg(args):
     from pixels_amd import _native as N
     from pixels_amd.learned import CostModel, bin_rays, log2_steps, wave_cost
     from pixels_amd.scene import LARGE, SMALL, SMALL_FLOOR
+    from pixels_amd.workloads import shadow_queries
 
     dev = torch.device("cuda:0")
     W, H = 1920, 1080

@@ -104,7 +104,16 @@ typedef struct px_limits {
   uint32_t max_dirty, max_alloc, max_writes, max_impacts, max_jobs, max_flood, max_seeds;
   uint32_t hash_size, max_chunks, max_arena, max_lights, max_anchors, max_flashes;
   uint32_t spawn_budget, component_budget, search_iters_per_frame, flood_iters_per_frame;
+  /* Beyond the reference's LIMITS: which components the island pass
+   * reports (PX_ISLANDS_ALL or PX_ISLANDS_SEEDED, see px_island_pass), and
+   * the step bound of its union-find walks (0 = the node capacity of the
+   * pass, which no walk over well-formed nodes reaches; a walk that passes
+   * it, or meets a parent above its child, stops and the labels are
+   * refused with PX_ERR_STATE). */
+  uint32_t island_mode, uf_walk_cap;
 } px_limits;
+#define PX_ISLANDS_ALL 0u
+#define PX_ISLANDS_SEEDED 1u
 
 /* ---- world description: what `new GpuWorld(device, world, scene)` takes
  * (GpuWorld.ts:46-137).  All host pointers; the library copies them. */
@@ -229,6 +238,16 @@ int px_render(px_ctx* ctx, const px_ray* rays, uint32_t n, float* rgba);
 int px_read_density(px_ctx* ctx, uint8_t* out);
 /* Slots per primary ray that px_trace_frame_device writes (1 + light count). */
 uint32_t px_frame_slots(const px_ctx* ctx);
+/* Layout of d_shadow_out for the frame entry points of this context:
+ * PX_FRAME_PIXEL_MAJOR (the default) puts slot s of primary ray i at
+ * i * slots + s; PX_FRAME_SLOT_PLANAR puts it at s * n + i (one plane of n
+ * records per slot), which lets the shadow kernel store the records of
+ * neighbouring rays as one contiguous run.  The records are the same.  New
+ * entry: the reference consumes its shadow rays inside shade()
+ * (raymarch.wgsl:323-356) and stores no per-ray result. */
+#define PX_FRAME_PIXEL_MAJOR 0u
+#define PX_FRAME_SLOT_PLANAR 1u
+int px_set_frame_layout(px_ctx* ctx, uint32_t layout);
 
 /* ---- batched queries ------------------------------------------------------
  * New entry the hot path needs (the reference runs each query kind inside its

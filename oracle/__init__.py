@@ -26,6 +26,12 @@ P_SPAWNED, P_PCURSOR, P_WRITES, P_ALIVE = range(4)
 _lib = None
 
 
+def default_threads() -> int:
+    """Host threads of a parallel oracle call: every core, at most 16 (the
+    CPU share of one GPU's host)."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 class OWorld(C.Structure):
     _fields_ = [("dx", C.c_int32), ("dy", C.c_int32), ("dz", C.c_int32), ("grid", C.c_void_p),
                 ("pool", C.c_void_p), ("super_grid", C.c_void_p), ("mats", C.c_void_p)]
@@ -119,7 +125,7 @@ class World:
         rays = np.ascontiguousarray(rays, RAY_DTYPE)
         out = np.zeros(rays.shape[0], HIT_DTYPE)
         load().o_trace_batch(C.byref(self.w), kind, rays.ctypes.data, rays.shape[0], out.ctypes.data,
-                             threads or os.cpu_count() or 1)
+                             threads or default_threads())
         return out
 
     def frame(self, u52, lights, rays, threads: int = 0):
@@ -134,7 +140,7 @@ class World:
         sh = np.zeros(n * slots, HIT_DTYPE)
         nc = C.c_uint32(0)
         load().o_frame(C.byref(self.w), u52.ctypes.data, lights.ctypes.data, lights.size // 4, rays.ctypes.data, n,
-                       prim.ctypes.data, sh.ctypes.data, C.byref(nc), threads or os.cpu_count() or 1)
+                       prim.ctypes.data, sh.ctypes.data, C.byref(nc), threads or default_threads())
         return prim, sh, int(nc.value)
 
 
@@ -154,7 +160,7 @@ class World:
         dens = self.density() if density is None else np.ascontiguousarray(density, np.uint8)
         out = np.zeros((rays.shape[0], 4), np.float32)
         load().o_render(C.byref(self.w), u52.ctypes.data, lights.ctypes.data, lights.size // 4, dens.ctypes.data,
-                        rays.ctypes.data, rays.shape[0], out.ctypes.data, threads or os.cpu_count() or 1)
+                        rays.ctypes.data, rays.shape[0], out.ctypes.data, threads or default_threads())
         return out
 
     def march_ex(self, u52, ro, rd, max_t=1e9, shadow=False):

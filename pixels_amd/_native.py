@@ -26,6 +26,8 @@ RAY_PRIMARY, RAY_SHADOW, RAY_SHOOT, RAY_SEGMENT, RAY_MIXED = 0, 1, 2, 3, 4
 STAGE_EDIT, STAGE_PARTICLES, STAGE_FRAME_BEGIN, STAGE_WRITES, STAGE_RESOLVE, STAGE_CHUNKS = 1, 2, 4, 8, 16, 32
 P_SETTLED, P_DEPOSIT, P_DUST = 0x10000, 0x20000, 0x40000
 FACE_ZERO, FACE_NOT_CAST, FACE_MISS = 6, 0xFE, 0xFF
+FRAME_PIXEL_MAJOR, FRAME_SLOT_PLANAR = 0, 1
+ISLANDS_ALL, ISLANDS_SEEDED = 0, 1
 
 # CTRL_NAMES order, src/gpu/layout.ts:48-58
 CTRL_NAMES = [
@@ -55,7 +57,8 @@ class PxLimits(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "max_dirty", "max_alloc", "max_writes", "max_impacts", "max_jobs", "max_flood", "max_seeds",
         "hash_size", "max_chunks", "max_arena", "max_lights", "max_anchors", "max_flashes",
-        "spawn_budget", "component_budget", "search_iters_per_frame", "flood_iters_per_frame")]
+        "spawn_budget", "component_budget", "search_iters_per_frame", "flood_iters_per_frame",
+        "island_mode", "uf_walk_cap")]
 
 
 class PxWorldDesc(C.Structure):
@@ -97,6 +100,7 @@ SIGNATURES = {
     "px_trace": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_device": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_frame_device": (C.c_int, [_P, _P, _U32, _P, _P, _U32P]),
+    "px_set_frame_layout": (C.c_int, [_P, _U32]),
     "px_render_frame_device": (C.c_int, [_P, _P, _U32, _P]),
     "px_render": (C.c_int, [_P, _P, _U32, _P]),
     "px_read_density": (C.c_int, [_P, _P]),

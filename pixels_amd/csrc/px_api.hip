@@ -70,6 +70,8 @@ void px_default_limits(px_limits* l) {
   l->component_budget = 400000;
   l->search_iters_per_frame = 16;
   l->flood_iters_per_frame = 12;
+  l->island_mode = PX_ISLANDS_ALL;
+  l->uf_walk_cap = 0;
 }
 
 const char* px_last_error(const px_ctx* ctx) {
@@ -120,6 +122,10 @@ static int ctx_begin(const px_world_desc* d, const char* what, px_ctx** out) {
   if (hipSetDevice(c->device) != hipSuccess) { release(c); return px_fail(nullptr, PX_ERR_HIP, what, "hipSetDevice"); }
   if (d->limits) c->lim = *d->limits;
   else px_default_limits(&c->lim);
+  if (c->lim.island_mode > PX_ISLANDS_SEEDED) {
+    release(c);
+    return px_fail(nullptr, PX_ERR_ARG, what, "unknown island_mode");
+  }
   c->vs = d->voxel_size > 0.0f ? d->voxel_size : 1.0f;
   for (int a = 0; a < 3; a++) {
     c->dims[a] = d->dims[a];
@@ -375,6 +381,14 @@ int px_trace_frame_device(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit
   if (!c || (n && (!d_primary || !d_prim || !d_shadow))) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
   return px_launch_frame(c, d_primary, n, d_prim, d_shadow, n_cast);
+}
+
+int px_set_frame_layout(px_ctx* c, uint32_t layout) {
+  if (!c) return PX_ERR_ARG;
+  if (layout != PX_FRAME_PIXEL_MAJOR && layout != PX_FRAME_SLOT_PLANAR)
+    return px_fail(c, PX_ERR_ARG, "px_set_frame_layout", "unknown layout");
+  c->frame_layout = layout;
+  return PX_OK;
 }
 
 int px_render_frame_device(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba) {
@@ -706,7 +720,7 @@ int px_run_to_quiescence(px_ctx* c) {
     if (rc) return rc;
   }
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  return PX_OK;
+  return c->labels_valid ? px_check_island_pass(c, "px_run_to_quiescence") : PX_OK;
 }
 
 int px_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n) {

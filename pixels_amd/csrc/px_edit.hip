@@ -98,6 +98,7 @@ struct EditView {
   uint16_t* lab;
   uint32_t* nodes;  // kEditBlocks ranges of slots_node
   uint32_t slots_cell, slots_node;
+  uint32_t walk_cap;  // step bound of the union-find walks (>= the node capacity)
   uint32_t* part;
   uint32_t* bcount;
   uint32_t* out;
@@ -128,24 +129,44 @@ struct EditView {
   float* dev_impacts;  // impacts pushed by falling chunks
 };
 
+// Error bits of ec[EC_OVERFLOW]: a node list overflowed, a brick label was
+// not a local index, a union-find walk met a parent above its child or ran
+// past its step bound (the pass's labels are then not trusted).
+constexpr uint32_t kErrNodes = 1u;
+constexpr uint32_t kErrLabel = 2u;
+constexpr uint32_t kErrWalk = 4u;
+// Steps of an LDS walk inside one brick: a chain of strictly decreasing
+// local indices has at most 512 entries.
+constexpr uint32_t kLdsWalkCap = 512u;
+
 // Union-find over voxel keys.  par[x] <= x always (links go to smaller keys),
-// so chains strictly decrease and every walk ends.  Plain loads are enough:
-// a stale parent is still an ancestor, and a stale root only makes the CAS in
-// uf_union fail and return the current parent.
-PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x) {
-  for (;;) {
-    uint32_t p = par[x];
+// so chains strictly decrease.  Plain loads are enough: a stale parent is
+// still an ancestor, and a stale root only makes the CAS in uf_union fail and
+// return the current parent.  Every walk checks that order (a parent above
+// its child can only be memory no node wrote in this pass) and stops after
+// `cap` steps, which no chain of distinct nodes reaches (cap >= the node
+// capacity of the pass); either way it sets kErrWalk and returns the node it
+// stands on, a key below the one it started from, so every access stays
+// inside par[] and every walk ends.
+PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x, uint32_t cap, uint32_t* err) {
+  for (uint32_t it = 0; it < cap; it++) {
+    const uint32_t p = par[x];
     if (p == x) return x;
-    uint32_t gp = par[p];
+    if (p > x) break;
+    const uint32_t gp = par[p];
+    if (gp > p) break;
     if (gp != p) par[x] = gp;  // path halving; gp < p < x
     x = gp;
   }
+  atomicOr(err, kErrWalk);
+  return x;
 }
 
-PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b) {
-  for (;;) {
-    a = uf_find(par, a);
-    b = uf_find(par, b);
+PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b, uint32_t cap, uint32_t* err) {
+  // each failed CAS leaves par[a] below a, so a strictly decreases
+  for (uint32_t it = 0; it < cap; it++) {
+    a = uf_find(par, a, cap, err);
+    b = uf_find(par, b, cap, err);
     if (a == b) return;
     if (a < b) {
       uint32_t t = a;
@@ -154,27 +175,33 @@ PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b) {
     }
     uint32_t old = atomicCAS(par + a, a, b);
     if (old == a) return;
+    if (old > a) break;
     a = old;
   }
+  atomicOr(err, kErrWalk);
 }
 
 // LDS union-find over the 512 voxels of one brick (links to smaller indices,
 // path halving; parents only decrease, so a racing halving store still
-// writes an ancestor)
-PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x) {
-  for (;;) {
-    uint32_t p = lab[x];
+// writes an ancestor).  Same order check and step bound as uf_find.
+PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x, uint32_t* err) {
+  for (uint32_t it = 0; it < kLdsWalkCap; it++) {
+    const uint32_t p = lab[x];
     if (p == x) return x;
-    uint32_t gp = lab[p];
+    if (p > x) break;
+    const uint32_t gp = lab[p];
+    if (gp > p) break;
     if (gp != p) lab[x] = gp;
     x = gp;
   }
+  atomicOr(err, kErrWalk);
+  return x;
 }
 
-PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
-  for (;;) {
-    a = lds_find(lab, a);
-    b = lds_find(lab, b);
+PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b, uint32_t* err) {
+  for (uint32_t it = 0; it < kLdsWalkCap; it++) {
+    a = lds_find(lab, a, err);
+    b = lds_find(lab, b, err);
     if (a == b) return;
     if (a < b) {
       uint32_t t = a;
@@ -183,8 +210,10 @@ PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b) {
     }
     uint32_t old = atomicCAS(lab + a, a, b);
     if (old == a) return;
+    if (old > a) break;
     a = old;
   }
+  atomicOr(err, kErrWalk);
 }
 
 // inclusive prefix sum over the wavefront
@@ -458,9 +487,9 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
     for (int x = 0; x < 8; x++) {
       uint32_t b = 1u << x, pb = b >> 1;
       if ((ey & b) && !(x > 0 && (ey & pb) && (lx & b) && (ylx & b)))
-        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x);
+        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x, e.ec + EC_OVERFLOW);
       if ((ez & b) && !(x > 0 && (ez & pb) && (lx & b) && (zlx & b)))
-        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x);
+        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x, e.ec + EC_OVERFLOW);
     }
     __syncthreads();
     // anchors inside the brick (islands): y == 0 or a structural neighbour
@@ -481,7 +510,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
       if (!((rs >> x) & 1u)) continue;
       uint32_t run = 1u << x;
       for (int y = x + 1; y < 8 && ((lx >> y) & 1u); y++) run |= 1u << y;
-      uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x);
+      uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x, e.ec + EC_OVERFLOW);
       atomicAdd(&s_cnt[r], (uint32_t)__popc(run));
       if (run & abits) atomicOr(&s_flg[r], kFlagAnchor);
       for (int y = x; y < 8; y++)
@@ -502,7 +531,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
       e.cnt[node] = s_cnt[li];
       e.flag[node] = s_flg[li];
       if (slot < e.slots_node) my_nodes[slot] = node;
-      else atomicOr(&e.ec[EC_OVERFLOW], 1u);
+      else atomicOr(&e.ec[EC_OVERFLOW], kErrNodes);
       slot++;
     }
     nn += total;
@@ -537,7 +566,7 @@ PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t l
   if ((g & PX_UNIFORM_BIT) != 0u) return okey;
   uint32_t r = e.lab[(size_t)(g - 1u) * 512u + li];
   if (r >= 512u) {
-    atomicOr(&e.ec[EC_OVERFLOW], 2u);
+    atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
     return PX_NONE;
   }
   return okey + loff(e, r);
@@ -581,7 +610,7 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, con
     uint32_t pa = __shfl_up(na, 1), pb = __shfl_up(nb, 1), pj = __shfl_up(jb, 1);
     uint32_t qa = __shfl_up(na, 8), qb = __shfl_up(nb, 8), qj = __shfl_up(jb, 8);
     bool dup = (u > 0u && pj && pa == na && pb == nb) || (v > 0u && qj && qa == na && qb == nb);
-    if (join && !dup) uf_union(e.par, na, nb);
+    if (join && !dup) uf_union(e.par, na, nb, e.walk_cap, e.ec + EC_OVERFLOW);
     if (MODE == 0) {
       uint32_t anc_a = (ca && vb && s_str[vb & 0xffu]) ? 1u : 0u;
       uint32_t anc_b = (cb && va && s_str[va & 0xffu]) ? 1u : 0u;
@@ -620,11 +649,17 @@ __global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
       uint32_t x = nodes[i];
       uint32_t r = x;
-      for (;;) {
+      bool ok = false;
+      for (uint32_t it = 0; it < e.walk_cap; it++) {
         uint32_t p = __hip_atomic_load(e.par + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (p == r) break;
+        if (p == r) {
+          ok = true;
+          break;
+        }
+        if (p > r) break;  // never written by a node of this pass
         r = p;
       }
+      if (!ok) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
       __hip_atomic_store(e.par + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
@@ -723,7 +758,7 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
         if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
           uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
           if (r < 512u) hit_root = e.par[okey + loff(e, r)];
-          else atomicOr(&e.ec[EC_OVERFLOW], 2u);
+          else atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
         }
       }
       // the pane component this voxel belongs to is hit (flooded later)
@@ -770,7 +805,7 @@ __global__ __launch_bounds__(64) void k_flood_remove(EditView e) {
         uint32_t r = (ll[j >> 1] >> sh) & 0xffffu;
         if (v == 0u || r == kLabNone) continue;
         if (r >= 512u) {
-          atomicOr(&e.ec[EC_OVERFLOW], 2u);
+          atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
           continue;
         }
         if ((e.flag[e.par[okey + loff(e, r)]] & kFlagHit) == 0u) continue;
@@ -1828,6 +1863,8 @@ EditView make_view(px_ctx* c) {
   e.nodes = c->d_nodes;
   e.slots_cell = (c->ncell + kEditBlocks - 1u) / kEditBlocks;
   e.slots_node = e.slots_cell * 256u;
+  // no chain of distinct nodes is longer than the node lists can hold
+  e.walk_cap = c->lim.uf_walk_cap ? c->lim.uf_walk_cap : e.slots_node * kEditBlocks;
   e.part = c->d_part;
   e.bcount = c->d_bcount;
   e.out = c->d_out_labels;
@@ -1975,14 +2012,31 @@ int px_launch_island_pass(px_ctx* c) {
   return px_refresh_masks(c);
 }
 
+// The error bits of the last island pass as a status (kErrNodes alone is a
+// capacity matter, reported by the label readback).
+int px_edit_errors(px_ctx* c, const char* what, uint32_t bits) {
+  if (bits & kErrWalk)
+    return px_fail(c, PX_ERR_STATE, what,
+                   "a union-find walk met a parent above its child or passed its step bound; labels not trusted");
+  if (bits & kErrLabel) return px_fail(c, PX_ERR_STATE, what, "internal error: a brick label was not a local index");
+  return PX_OK;
+}
+
+int px_check_island_pass(px_ctx* c, const char* what) {
+  if (!c->d_edit_ctr) return PX_OK;
+  uint32_t bits = 0;
+  PX_HIP_CHECK(c, hipMemcpyAsync(&bits, c->d_edit_ctr + EC_OVERFLOW, 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return px_edit_errors(c, what, bits);
+}
+
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n) {
   *n = 0;
   if (!c->labels_valid) return px_fail(c, PX_ERR_STATE, "px_island_labels", "no island pass has run");
   uint32_t ec[EC_WORDS];
   PX_HIP_CHECK(c, hipMemcpyAsync(ec, c->d_edit_ctr, sizeof ec, hipMemcpyDeviceToHost, c->stream));
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  if (ec[EC_OVERFLOW] & 2u)
-    return px_fail(c, PX_ERR_STATE, "px_island_labels", "internal error: a brick label was not a local index");
+  if (int rc = px_edit_errors(c, "px_island_labels", ec[EC_OVERFLOW])) return rc;
   if (ec[EC_OVERFLOW] || ec[EC_LABELS] > c->label_cap)
     return px_fail(c, PX_ERR_CAPACITY, "px_island_labels", "component node list overflowed");
   uint32_t m = ec[EC_LABELS];

@@ -78,6 +78,7 @@ struct px_ctx {
   size_t shadow_cap = 0;
   uint32_t* d_wave_count = nullptr;  // shadow rays queued per wavefront region
   uint32_t wave_count_cap = 0;
+  uint32_t frame_layout = PX_FRAME_PIXEL_MAJOR;  // layout of the frame's shadow outputs
   // batched queries: launch shape, per-XCD claim cursors, the captured graph
   uint32_t n_cu = 256;
   uint32_t batch_block = 256, batch_blocks = 0, batch_chunk = 256;
@@ -215,6 +216,10 @@ int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);
 int px_launch_island_pass(px_ctx* c);
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
+// Error bits of the edit counters as a status; the blocking form reads them
+// from the device after the stream drains.
+int px_edit_errors(px_ctx* c, const char* what, uint32_t bits);
+int px_check_island_pass(px_ctx* c, const char* what);
 // scene rasteriser (px_scene.hip): px_raster_begin rasterises prims into
 // these temporaries and reports the pool bricks the world needs;
 // px_raster_finish writes the grid, pool and super grid of the context,

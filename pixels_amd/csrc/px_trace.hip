@@ -257,7 +257,8 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
                                                             const px_hit* __restrict__ prim,
                                                             const uint32_t* __restrict__ queue,
                                                             const uint32_t* __restrict__ wave_count,
-                                                            uint32_t n, px_hit* __restrict__ sh_out) {
+                                                            uint32_t n, uint32_t planar,
+                                                            px_hit* __restrict__ sh_out) {
   extern __shared__ uint32_t s_mem[];
   const MarchTables T = px::load_march_tables<SUP_LDS>(w, s_mem);
   // wavefronts take the regions last first: regions follow the image rows,
@@ -288,9 +289,20 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (uint32_t k = first + lane; k < last; k += 64u) {
-    const uint32_t local = k - first;
-    if (((bm[local >> 5] >> (local & 31u)) & 1u) == 0u) store_not_cast(sh_out + k);
+  if (planar) {
+    // slot-planar output (slot s of pixel i at s * n + i): lanes walk the
+    // region slot by slot, so each wavefront store covers consecutive pixels
+    const uint32_t p0 = region * 64u, npix = (last - first) / slots;
+    for (uint32_t k = lane; k < npix * slots; k += 64u) {
+      const uint32_t s = k / npix, j = k - s * npix;
+      const uint32_t local = j * slots + s;
+      if (((bm[local >> 5] >> (local & 31u)) & 1u) == 0u) store_not_cast(sh_out + (size_t)s * n + p0 + j);
+    }
+  } else {
+    for (uint32_t k = first + lane; k < last; k += 64u) {
+      const uint32_t local = k - first;
+      if (((bm[local >> 5] >> (local & 31u)) & 1u) == 0u) store_not_cast(sh_out + k);
+    }
   }
   for (uint32_t k = lane; k < cnt; k += 64u) {
     const uint32_t code = rq[k];
@@ -303,7 +315,7 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
     float O[3], dir[3], mt;
     shadow_ray(w, sd, ro, rd, __uint_as_float(hv.y), face, value, s, O, dir, mt);
     MarchOut m = px::march<true>(w, T, O, dir, mt);
-    store_hit(sh_out + code, m);
+    store_hit(sh_out + (planar ? (size_t)s * n + i : (size_t)code), m);
   }
 }
 
@@ -605,6 +617,47 @@ int px_launch_trace_perm(px_ctx* c, uint32_t kind, const px_ray* d_rays, const u
   return PX_OK;
 }
 
+// The frame's two launches over a caller-given queue (n_regions * 64 *
+// slots words), per-region counts (n_regions words) and one word for the
+// cast total.
+static int launch_frame_on(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
+                           uint32_t* n_cast, uint32_t* queue, uint32_t* wave_count, uint32_t* total) {
+  const uint32_t n_regions = (n + 63u) / 64u;
+  PxWorldView w = px_view(c);
+  bool sup_lds;
+  const size_t lds = px_table_lds(c, &sup_lds);
+  {
+    PxProfScope prof(c, PX_PROF_PRIMARY);
+    dim3 grid((n + 255u) / 256u), block(256);
+    if (sup_lds)
+      hipLaunchKernelGGL(k_frame_primary<true>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
+                         queue, wave_count);
+    else
+      hipLaunchKernelGGL(k_frame_primary<false>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
+                         queue, wave_count);
+  }
+  PX_HIP_CHECK(c, hipGetLastError());
+  const uint32_t blocks = (n_regions + 3u) / 4u;
+  const uint32_t planar = c->frame_layout == PX_FRAME_SLOT_PLANAR ? 1u : 0u;
+  {
+    PxProfScope prof(c, PX_PROF_SHADOW);
+    if (sup_lds)
+      hipLaunchKernelGGL(k_trace_shadow_queue<true>, dim3(blocks), dim3(256), lds, c->stream, w, c->shadow,
+                         d_primary, d_prim, queue, wave_count, n, planar, d_shadow);
+    else
+      hipLaunchKernelGGL(k_trace_shadow_queue<false>, dim3(blocks), dim3(256), lds, c->stream, w, c->shadow,
+                         d_primary, d_prim, queue, wave_count, n, planar, d_shadow);
+  }
+  PX_HIP_CHECK(c, hipGetLastError());
+  if (n_cast) {
+    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, c->stream, wave_count, n_regions, total);
+    PX_HIP_CHECK(c, hipGetLastError());
+    PX_HIP_CHECK(c, hipMemcpyAsync(n_cast, total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  }
+  return PX_OK;
+}
+
 int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
                     uint32_t* n_cast) {
   if (n == 0) {
@@ -631,38 +684,8 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
     PX_HIP_CHECK(c, hipMalloc(&c->d_wave_count, (size_t)n_regions * sizeof(uint32_t)));
     c->wave_count_cap = n_regions;
   }
-  PxWorldView w = px_view(c);
-  bool sup_lds;
-  const size_t lds = px_table_lds(c, &sup_lds);
-  {
-    PxProfScope prof(c, PX_PROF_PRIMARY);
-    dim3 grid((n + 255u) / 256u), block(256);
-    if (sup_lds)
-      hipLaunchKernelGGL(k_frame_primary<true>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
-                         c->d_shadow_queue, c->d_wave_count);
-    else
-      hipLaunchKernelGGL(k_frame_primary<false>, grid, block, lds, c->stream, w, c->shadow, d_primary, n, d_prim,
-                         c->d_shadow_queue, c->d_wave_count);
-  }
-  PX_HIP_CHECK(c, hipGetLastError());
-  const uint32_t blocks = (n_regions + 3u) / 4u;
-  {
-    PxProfScope prof(c, PX_PROF_SHADOW);
-    if (sup_lds)
-      hipLaunchKernelGGL(k_trace_shadow_queue<true>, dim3(blocks), dim3(256), lds, c->stream, w, c->shadow,
-                         d_primary, d_prim, c->d_shadow_queue, c->d_wave_count, n, d_shadow);
-    else
-      hipLaunchKernelGGL(k_trace_shadow_queue<false>, dim3(blocks), dim3(256), lds, c->stream, w, c->shadow,
-                         d_primary, d_prim, c->d_shadow_queue, c->d_wave_count, n, d_shadow);
-  }
-  PX_HIP_CHECK(c, hipGetLastError());
-  if (n_cast) {
-    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, c->stream, c->d_wave_count, n_regions, c->d_counters);
-    PX_HIP_CHECK(c, hipGetLastError());
-    PX_HIP_CHECK(c, hipMemcpyAsync(n_cast, c->d_counters, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  }
-  return PX_OK;
+  return launch_frame_on(c, d_primary, n, d_prim, d_shadow, n_cast, c->d_shadow_queue, c->d_wave_count,
+                         c->d_counters);
 }
 
 // Kinds in processing order: shoot rays (long, few), primary, shadow, then

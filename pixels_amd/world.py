@@ -244,6 +244,12 @@ class Simulation:
                                                      C.byref(nc) if count else None))
         return int(nc.value) if count else None
 
+    def set_frame_layout(self, slot_planar: bool) -> None:
+        """Shadow outputs of the frame: pixel-major (slot s of ray i at
+        i * slots + s) or slot-planar (at s * n + i)."""
+        self.gw.check(self.lib.px_set_frame_layout(self.gw.ctx, N.FRAME_SLOT_PLANAR if slot_planar
+                                                   else N.FRAME_PIXEL_MAJOR))
+
     # -- shading to an image (fsMain, raymarch.wgsl:387-450) ------------------
     def render(self, rays: np.ndarray) -> np.ndarray:
         """RGBA float colour of every ray (host buffers, blocking)."""

@@ -223,3 +223,34 @@ def test_sim_step_and_quiescence(floor_scene, floor_impacts):
             sim.encode(2)
     finally:
         gw.close()
+
+
+def test_walk_bound_refuses_labels(floor_scene, floor_impacts):
+    """A union-find walk that passes its step bound stops (every kernel still
+    exits) and the labels are refused; the bound is forced tiny from the
+    host, so the well-formed walks of the 64-impact pass exceed it."""
+    from pixels_amd import GpuWorld, PxError, Simulation, default_limits
+    lim = default_limits()
+    lim.uf_walk_cap = 1
+    gw = GpuWorld(floor_scene, limits=lim)
+    try:
+        sim = Simulation(gw)
+        sim.push_impacts(floor_impacts)
+        print("case: walk bound 1", flush=True)
+        with pytest.raises(PxError):
+            sim.run_to_quiescence()
+        assert "union-find" in gw.lib.px_last_error(gw.ctx).decode()
+        with pytest.raises(PxError):
+            sim.island_labels()
+    finally:
+        gw.close()
+    # the default bound: the same pass is accepted
+    gw = GpuWorld(floor_scene)
+    try:
+        sim = Simulation(gw)
+        sim.push_impacts(floor_impacts)
+        sim.run_to_quiescence()
+        keys, cnt, n = sim.island_labels()
+        assert n > 0
+    finally:
+        gw.close()
