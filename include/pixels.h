@@ -216,6 +216,22 @@ int px_trace_device(px_ctx* ctx, uint32_t kind, const px_ray* d_rays, uint32_t n
 int px_trace_frame_device(px_ctx* ctx, const px_ray* d_primary, uint32_t n, px_hit* d_primary_out,
                           px_hit* d_shadow_out, uint32_t* n_shadow_cast);
 
+/* px_trace_frame_device without any allocation: the same kernels and
+ * results (and the layout of px_set_frame_layout), with the shadow-ray
+ * queue, its per-region counts and the cast total in a fixed scratch array
+ * of 20,971,520 words (80 MiB) that the library holds on each device it
+ * runs on until the process ends (the 1920x1080 frame with the sun and 8
+ * lights needs 18.7M words).  It calls no hipMalloc / hipFree: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops).
+ * Fails with PX_ERR_CAPACITY when 64 * ceil(n / 64) * slots + ceil(n / 64)
+ * + 64 exceeds the array.  Every call on one device shares the array: calls
+ * that overlap on a device (other streams, other contexts, other host
+ * threads) race and must be serialised by the caller.  No counterpart in
+ * the reference, which allocates every buffer once at load
+ * (GpuWorld.ts:46-137). */
+int px_trace_frame_device_noalloc(px_ctx* ctx, const px_ray* d_primary, uint32_t n, px_hit* d_primary_out,
+                                  px_hit* d_shadow_out, uint32_t* n_shadow_cast);
+
 /* fsMain (raymarch.wgsl:387-450) to an image: the colour of every primary
  * ray as RGBA floats (alpha 1; no depth), with shade() (:304-366): material
  * colour (render_common.wgsl:58-140), coarse AO from the density texture and

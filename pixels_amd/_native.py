@@ -100,6 +100,7 @@ SIGNATURES = {
     "px_trace": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_device": (C.c_int, [_P, _U32, _P, _U32, _P]),
     "px_trace_frame_device": (C.c_int, [_P, _P, _U32, _P, _P, _U32P]),
+    "px_trace_frame_device_noalloc": (C.c_int, [_P, _P, _U32, _P, _P, _U32P]),
     "px_set_frame_layout": (C.c_int, [_P, _U32]),
     "px_render_frame_device": (C.c_int, [_P, _P, _U32, _P]),
     "px_render": (C.c_int, [_P, _P, _U32, _P]),

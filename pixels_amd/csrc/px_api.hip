@@ -139,6 +139,13 @@ static int ctx_begin(const px_world_desc* d, const char* what, px_ctx** out) {
     release(c);
     return px_fail(nullptr, PX_ERR_HIP, what, "hipStreamCreateWithFlags");
   }
+  // the fixed scratch of the no-allocation frame entry, once per device,
+  // with the device current
+  if (px_fixed_scratch_init(c) != PX_OK) {
+    const std::string m = c->err;
+    release(c);
+    return px_fail(nullptr, PX_ERR_HIP, what, m.c_str());
+  }
   *out = c;
   return PX_OK;
 }
@@ -381,6 +388,13 @@ int px_trace_frame_device(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit
   if (!c || (n && (!d_primary || !d_prim || !d_shadow))) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
   return px_launch_frame(c, d_primary, n, d_prim, d_shadow, n_cast);
+}
+
+int px_trace_frame_device_noalloc(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim,
+                                  px_hit* d_shadow, uint32_t* n_cast) {
+  if (!c || (n && (!d_primary || !d_prim || !d_shadow))) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_frame_noalloc(c, d_primary, n, d_prim, d_shadow, n_cast);
 }
 
 int px_set_frame_layout(px_ctx* c, uint32_t layout) {

@@ -79,6 +79,9 @@ struct px_ctx {
   uint32_t* d_wave_count = nullptr;  // shadow rays queued per wavefront region
   uint32_t wave_count_cap = 0;
   uint32_t frame_layout = PX_FRAME_PIXEL_MAJOR;  // layout of the frame's shadow outputs
+  // device address of the library's fixed scratch array on this context's
+  // device (queue + counts of px_trace_frame_device_noalloc)
+  uint32_t* d_fixed_scratch = nullptr;
   // batched queries: launch shape, per-XCD claim cursors, the captured graph
   uint32_t n_cu = 256;
   uint32_t batch_block = 256, batch_blocks = 0, batch_chunk = 256;
@@ -203,6 +206,11 @@ int px_launch_trace(px_ctx* c, uint32_t kind, const px_ray* d_rays, uint32_t n, 
 int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
                     uint32_t* n_cast);
 int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t* d_out);
+// Address of the fixed scratch array on the current device, looked up once
+// per device (null + PX_ERR_HIP if the lookup fails).
+int px_fixed_scratch_init(px_ctx* c);
+int px_launch_frame_noalloc(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
+                            uint32_t* n_cast);
 int px_launch_batch(px_ctx* c, const px_query_set* q);
 // px_render.hip
 int px_refresh_density(px_ctx* c);

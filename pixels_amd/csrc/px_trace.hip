@@ -15,6 +15,7 @@
 // fixed as: pow(x,5) = ((x*x)*(x*x))*x, mix(a,b,t) = a*(1-t) + b*t,
 // dot = (x*x' + y*y') + z*z', length = sqrt(dot), zero t reported as +0.
 #include <cstring>
+#include <mutex>
 
 #include "px_internal.h"
 #include "px_march.h"
@@ -686,6 +687,53 @@ int px_launch_frame(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_pr
   }
   return launch_frame_on(c, d_primary, n, d_prim, d_shadow, n_cast, c->d_shadow_queue, c->d_wave_count,
                          c->d_counters);
+}
+
+// ---- the frame without allocation -----------------------------------------
+// Fixed scratch of px_trace_frame_device_noalloc: the cast total (word 0),
+// the per-region counts (from word 64) and the shadow-ray queue, in one
+// array that every call on a device shares; each device holds its own copy
+// for the life of the process.  Host code reaches it only through the
+// address hipGetSymbolAddress gives on that device.
+constexpr size_t kFixedScratchWords = (size_t)20 << 20;  // 80 MiB
+static __device__ __align__(256) uint32_t scratch[kFixedScratchWords];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int px_fixed_scratch_init(px_ctx* c) {
+  if (c->device < 0 || c->device >= kMaxScratchDevices)
+    return px_fail(c, PX_ERR_ARG, "fixed scratch", "device ordinal beyond the supported range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return px_fail(c, PX_ERR_HIP, "hipGetSymbolAddress(scratch)", hipGetErrorString(e));
+    g_scratch_ptr[c->device] = static_cast<uint32_t*>(p);
+  }
+  c->d_fixed_scratch = g_scratch_ptr[c->device];
+  return PX_OK;
+}
+
+int px_launch_frame_noalloc(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
+                            uint32_t* n_cast) {
+  if (n == 0) {
+    if (n_cast) *n_cast = 0;
+    return PX_OK;
+  }
+  if (!c->d_fixed_scratch)
+    return px_fail(c, PX_ERR_STATE, "px_trace_frame_device_noalloc", "fixed scratch not set up");
+  const uint32_t n_regions = (n + 63u) / 64u;
+  const size_t count_words = ((size_t)n_regions + 63u) & ~(size_t)63u;
+  const size_t need = 64u + count_words + (size_t)n_regions * 64u * c->shadow.slots;
+  if (need > kFixedScratchWords)
+    return px_fail(c, PX_ERR_CAPACITY, "px_trace_frame_device_noalloc",
+                   "64 * ceil(n / 64) * slots + ceil(n / 64) exceeds the fixed scratch of 20,971,520 words");
+  int rc = px_refresh_march(c);
+  if (rc) return rc;
+  uint32_t* base = c->d_fixed_scratch;
+  return launch_frame_on(c, d_primary, n, d_prim, d_shadow, n_cast, base + 64 + count_words, base + 64, base);
 }
 
 // Kinds in processing order: shoot rays (long, few), primary, shadow, then
