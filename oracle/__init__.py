@@ -20,6 +20,7 @@ HIT_DTYPE = np.dtype([("key", "<u4"), ("t", "<f4"), ("steps", "<u4"), ("face", "
                       ("value_hi", "<u2")])
 RAY_DTYPE = np.dtype([("ro", "<f4", 3), ("rd", "<f4", 3), ("max_t", "<f4"), ("flags", "<u4")])
 PARTICLE_DTYPE = np.dtype([("pos", "<f4", 3), ("value", "<u4"), ("vel", "<f4", 3), ("life", "<f4")])
+PARTICLE_DEPOSIT = 0x20000  # DEPOSIT flag of a particle's value (shared.wgsl:33-42)
 # counter words of the particle oracle: C_SPAWNED, C_PCURSOR, C_WRITES, alive count
 P_SPAWNED, P_PCURSOR, P_WRITES, P_ALIVE = range(4)
 
@@ -30,6 +31,10 @@ def default_threads() -> int:
     """Host threads of a parallel oracle call: every core, at most 16 (the
     CPU share of one GPU's host)."""
     return max(1, min(os.cpu_count() or 1, 16))
+
+
+class EditParams(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("node_budget", C.c_uint32), ("vs", C.c_float), ("frame_seed", C.c_uint32)]
 
 
 class OWorld(C.Structure):
@@ -44,7 +49,19 @@ def load():
     if not os.path.exists(LIB_PATH):
         from .build_oracle import build
         build()
-    lib = C.CDLL(LIB_PATH)
+    _lib = _bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def load_variant(path: str, defines) -> C.CDLL:
+    """A copy of the oracle built with -D `defines` at `path` (the tests plant
+    known mistakes this way and check that a pin catches each)."""
+    from .build_oracle import build
+    build(out=path, defines=defines)
+    return _bind(C.CDLL(path))
+
+
+def _bind(lib):
     P = C.c_void_p
     W = C.POINTER(OWorld)
     lib.o_march.argtypes = [W, P, P, C.c_float, C.c_int, P, P]
@@ -71,7 +88,7 @@ def load():
                               P]
     lib.o_chunk_frame.restype = C.c_int
     lib.o_chunk_frame.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, P, P, P, P, C.c_float, C.c_float,
-                                  C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, P]
+                                  C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, P, P, C.c_uint32, P]
     lib.o_raster.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, C.c_uint32, P, P]
     lib.o_prim_sdf.restype = C.c_double
     lib.o_prim_sdf.argtypes = [P, C.c_double, C.c_double, C.c_double]
@@ -89,7 +106,9 @@ def load():
     lib.o_frame_begin.argtypes = [P]
     lib.o_spawn.argtypes = [P, C.c_uint32, C.c_uint32, P, P, C.c_uint32]
     lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
-    _lib = lib
+    lib.o_edit_pass.restype = C.c_int
+    lib.o_edit_pass.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, C.POINTER(EditParams), P, C.c_uint32, P,
+                                C.c_uint32, P, P, C.c_uint32, P, P, C.c_uint32, P, P]
     return lib
 
 
@@ -255,6 +274,43 @@ def island_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, cap: int
     return vox, kind, keys[:m].copy(), cnt[:m].copy(), int(n.value), stats
 
 
+class EditResult:
+    """Outcome of the oracle's edit pass: vox / kind after it, the labels
+    (keys, counts; n = their total), the spawn() records in call order
+    (spawn_calls counts them all) and the stats words of o_edit_pass."""
+
+    def __init__(self, vox, kind, keys, counts, n, spawns, spawn_calls, stats):
+        self.vox, self.kind, self.keys, self.counts, self.n = vox, kind, keys, counts, n
+        self.spawns, self.spawn_calls, self.stats = spawns, spawn_calls, stats
+
+
+def edit_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, mode: int = 0,
+              node_budget: int = 0xFFFFFFFF, vs: float = 1.0 / 80.0, frame: float = 0.0, seed_keys=None,
+              cap: int = 1 << 20, spawn_cap: int = 1 << 22, lib=None) -> EditResult:
+    """impactJobs / alloc / carve with carveEffects / pane flood with
+    shardEffects / maint / island search on copies (o_edit_pass)."""
+    vox = np.array(vox, np.uint16, copy=True, order="C")
+    kind = np.array(kind, np.uint8, copy=True, order="C")
+    mats = np.ascontiguousarray(mats, np.float32)
+    imp = np.ascontiguousarray(impacts, np.float32).reshape(-1, 16)
+    sk = np.ascontiguousarray(np.zeros(0, np.uint32) if seed_keys is None else seed_keys, np.uint32)
+    keys = np.zeros(cap, np.uint32)
+    cnt = np.zeros(cap, np.uint32)
+    sp = np.zeros(spawn_cap, PARTICLE_DTYPE)
+    n, ns = C.c_uint32(0), C.c_uint32(0)
+    stats = np.zeros(8, np.uint64)
+    prm = EditParams(mode, min(int(node_budget), 0xFFFFFFFF), float(np.float32(vs)), frame_seed(frame))
+    lib = lib or load()
+    rc = lib.o_edit_pass(dims[0], dims[1], dims[2], vox.ctypes.data, kind.ctypes.data, mats.ctypes.data,
+                         C.byref(prm), imp.ctypes.data, imp.shape[0], sk.ctypes.data, sk.size, keys.ctypes.data,
+                         cnt.ctypes.data, cap, C.byref(n), sp.ctypes.data, spawn_cap, C.byref(ns), stats.ctypes.data)
+    if rc != 0:
+        raise MemoryError("oracle edit pass failed")
+    m = min(cap, n.value)
+    return EditResult(vox, kind, keys[:m].copy(), cnt[:m].copy(), int(n.value), sp[:min(spawn_cap, ns.value)].copy(),
+                      int(ns.value), stats)
+
+
 def apply_writes(dims, vox: np.ndarray, kind: np.ndarray, writes: np.ndarray, max_alloc: int = 65535,
                  free_top: int = 1 << 30):
     """Voxel writes + maint on copies; returns (vox_after, kind_after, stats)."""
@@ -332,15 +388,19 @@ class Chunks:
         kind = np.ascontiguousarray(kind, np.uint8)
         sp = np.zeros(spawn_cap, PARTICLE_DTYPE)
         ns = C.c_uint32(0)
+        seeds = np.zeros(1 << 20, np.uint32)
+        nseed = C.c_uint32(0)
         prm = self.params(vs, frame, 0)
         d = self.dims
         rc = load().o_chunk_frame(d[0], d[1], d[2], vox.ctypes.data, kind.ctypes.data, self.mats.ctypes.data,
                                   self.chunks.ctypes.data, self.arena.ctypes.data, self.ctr.ctypes.data,
                                   C.byref(prm), float(np.float32(dt)), float(np.float32(gravity)), self.max_writes,
                                   self.writes.ctypes.data, self.max_impacts, self.impacts.ctypes.data,
-                                  sp.ctypes.data, spawn_cap, C.byref(ns))
+                                  sp.ctypes.data, spawn_cap, C.byref(ns), seeds.ctypes.data, seeds.size,
+                                  C.byref(nseed))
         if rc != 0:
             raise MemoryError("oracle chunk frame failed")
+        self.land_seeds = seeds[:min(seeds.size, nseed.value)].copy()
         return sp[:min(spawn_cap, ns.value)].copy()
 
     def write_list(self):

@@ -13,12 +13,14 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def build(verbose: bool = False) -> str:
-    out = os.path.join(HERE, "liboracle.so")
+def build(verbose: bool = False, out: str | None = None, defines=()) -> str:
+    """Builds liboracle.so (or `out`).  `defines` are -D macros; the tests use
+    them to plant known mistakes (O_PLANT_*) in a copy built elsewhere."""
+    out = out or os.path.join(HERE, "liboracle.so")
     tmp = out + ".tmp"
-    cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           "-fexcess-precision=standard", "-pthread", os.path.join(HERE, "px_oracle.c"), os.path.join(HERE, "px_shade.c"),
-           "-lm", "-o", tmp]
+    cmd = ["gcc", "-O3", "-std=c11", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+           "-fexcess-precision=standard", "-pthread"] + ["-D" + d for d in defines] + [
+           os.path.join(HERE, "px_oracle.c"), os.path.join(HERE, "px_shade.c"), "-lm", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n%s\n%s" % (r.stdout, r.stderr))

@@ -661,7 +661,9 @@ int o_removes(const float* im, const float* mats, int32_t x, int32_t y, int32_t 
 static float reach(const float* im, const float* mats, uint32_t id) {
   const float* E = matE(mats, id);
   float r = 0.0f;
+#ifndef O_PLANT_REACH_NO_FR
   if (E[1] > 0.5f) r = im[4] * 1.2f;
+#endif
   if (E[0] > 0.0f) r = fmaxf(r, fmaxf(im[3] * fminf(1.4f, E[0]), fminf(im[3], 2.5f)) * 1.2f);
   return r;
 }
@@ -697,170 +699,17 @@ static const int32_t DIRS[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
 int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
                   const float* impacts16, uint32_t n_impacts, uint32_t* lab_key, uint32_t* lab_count,
                   uint32_t cap, uint32_t* n_labels, uint64_t* stats) {
-  ed_world E;
-  E.dx = dx; E.dy = dy; E.dz = dz;
-  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
-  E.vox = vox;
-  E.kind = kind;
-  E.mats = mats;
-  size_t ncell = (size_t)E.bx * E.by * E.bz;
-  E.dirty = (uint8_t*)calloc(ncell, 1);
-  uint8_t* req = (uint8_t*)calloc(ncell, 1);
-  if (!E.dirty || !req) { free(E.dirty); free(req); return -1; }
-  uint64_t destroyed = 0, bill = 0, allocs = 0, njobs = 0, flooded = 0;
-
-  /* impactJobs for every impact on the grid as it is before any alloc
-   * (sim.wgsl:414-443); jobs are collected per impact. */
-  size_t jcap = 1024, jn = 0;
-  uint32_t* jobs = (uint32_t*)malloc(jcap * 2 * sizeof(uint32_t));
-  int32_t bd[3] = {E.bx, E.by, E.bz};
-  for (uint32_t k = 0; k < n_impacts; k++) {
-    const float* im = impacts16 + (size_t)k * 16u;
-    float R = ceilf(fmaxf(im[3], im[4]) * 1.2f) + 1.0f;
-    int32_t bmin[3], bmax[3];
-    for (int a = 0; a < 3; a++) {
-      bmin[a] = iclamp(f2i(floorf((im[a] - R) / 8.0f)), 0, bd[a] - 1);
-      bmax[a] = iclamp(f2i(floorf((im[a] + R) / 8.0f)), 0, bd[a] - 1);
-    }
-    for (int32_t bz = bmin[2]; bz <= bmax[2]; bz++)
-      for (int32_t by = bmin[1]; by <= bmax[1]; by++)
-        for (int32_t bx = bmin[0]; bx <= bmax[0]; bx++) {
-          size_t ci = (size_t)bx + (size_t)E.bx * ((size_t)by + (size_t)E.by * (size_t)bz);
-          if (kind[ci] == 0) continue;
-          float lo[3] = {(float)((uint32_t)bx * 8u), (float)((uint32_t)by * 8u), (float)((uint32_t)bz * 8u)};
-          float e[3];
-          for (int a = 0; a < 3; a++) e[a] = fmaxf(fmaxf(lo[a] - im[a], 0.0f), im[a] - (lo[a] + 8.0f));
-          float near2 = dot3(e, e);
-          if (near2 > R * R) continue;
-          if (kind[ci] == 1) {
-            uint32_t uval = vox[vkey(&E, bx * 8, by * 8, bz * 8)];
-            float rch = reach(im, mats, uval & 0xffu);
-            if (near2 > rch * rch) continue;
-            req[ci] = 1;
-          }
-          if (jn == jcap) {
-            jcap *= 2;
-            jobs = (uint32_t*)realloc(jobs, jcap * 2 * sizeof(uint32_t));
-          }
-          jobs[jn * 2] = k;
-          jobs[jn * 2 + 1] = (uint32_t)ci;
-          jn++;
-        }
-  }
-  njobs = jn;
-  /* allocBricks (sim.wgsl:459-483): uniform cells become pool bricks */
-  for (size_t ci = 0; ci < ncell; ci++)
-    if (req[ci] && kind[ci] == 1) {
-      kind[ci] = 2;
-      E.dirty[ci] = 1;
-      allocs++;
-    }
-  /* carve (sim.wgsl:529-579): pool bricks only; removed pane voxels are kept
-   * as flood triggers (carveEffects :511-523). */
-  size_t tcap = 1024, tn = 0;
-  uint32_t* trig = (uint32_t*)malloc(tcap * 4 * sizeof(uint32_t));
-  for (size_t j = 0; j < jn; j++) {
-    const float* im = impacts16 + (size_t)jobs[j * 2] * 16u;
-    size_t ci = jobs[j * 2 + 1];
-    if (kind[ci] != 2) continue;
-    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8;
-    int32_t oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
-    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
-    int any = 0;
-    for (uint32_t l = 0; l < 512u; l++) {
-      int32_t x = ox + (int32_t)(l & 7u), y = oy + (int32_t)((l >> 3) & 7u), z = oz + (int32_t)(l >> 6);
-      size_t kk = vkey(&E, x, y, z);
-      uint32_t v = vox[kk];
-      if (v == 0u) continue;
-      if (!o_removes(im, mats, x, y, z, v)) continue;
-      vox[kk] = 0;
-      any = 1;
-      destroyed++;
-      bill += price_dimes(mats, v);
-      if (matE(mats, v & 0xffu)[1] > 1.5f) {
-        if (tn == tcap) {
-          tcap *= 2;
-          trig = (uint32_t*)realloc(trig, tcap * 4 * sizeof(uint32_t));
-        }
-        trig[tn * 4] = (uint32_t)x;
-        trig[tn * 4 + 1] = (uint32_t)y;
-        trig[tn * 4 + 2] = (uint32_t)z;
-        trig[tn * 4 + 3] = v & 0xffu;
-        tn++;
-      }
-    }
-    if (any) E.dirty[ci] = 1;
-  }
-  free(jobs);
-  /* Pane flood to completion (carveEffects :511-523 + floodStep :599-618):
-   * breadth-first over same-id voxels that sit in pool bricks (takeVoxel,
-   * :105-124, only removes voxels of pool bricks). */
-  size_t qcap = 1024, qh = 0, qt = 0;
-  uint32_t* q = (uint32_t*)malloc(qcap * 4 * sizeof(uint32_t));
-#define TAKE_PUSH(X, Y, Z, M)                                                        \
-  do {                                                                               \
-    int32_t X_ = (X), Y_ = (Y), Z_ = (Z);                                            \
-    if (in_world(&E, X_, Y_, Z_) && kind[cell_of(&E, X_, Y_, Z_)] == 2) {             \
-      size_t kk_ = vkey(&E, X_, Y_, Z_);                                             \
-      uint32_t v_ = vox[kk_];                                                        \
-      if (v_ != 0u && (v_ & 0xffu) == (M)) {                                         \
-        vox[kk_] = 0;                                                                \
-        E.dirty[cell_of(&E, X_, Y_, Z_)] = 1;                                        \
-        destroyed++;                                                                 \
-        flooded++;                                                                   \
-        bill += price_dimes(mats, v_);                                               \
-        if (qt == qcap) {                                                            \
-          qcap *= 2;                                                                 \
-          q = (uint32_t*)realloc(q, qcap * 4 * sizeof(uint32_t));                    \
-        }                                                                            \
-        q[qt * 4] = (uint32_t)X_;                                                    \
-        q[qt * 4 + 1] = (uint32_t)Y_;                                                \
-        q[qt * 4 + 2] = (uint32_t)Z_;                                                \
-        q[qt * 4 + 3] = (M);                                                         \
-        qt++;                                                                        \
-      }                                                                              \
-    }                                                                                \
-  } while (0)
-  for (size_t i = 0; i < tn; i++) {
-    int32_t x = (int32_t)trig[i * 4], y = (int32_t)trig[i * 4 + 1], z = (int32_t)trig[i * 4 + 2];
-    uint32_t m = trig[i * 4 + 3];
-    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
-  }
-  while (qh < qt) {
-    int32_t x = (int32_t)q[qh * 4], y = (int32_t)q[qh * 4 + 1], z = (int32_t)q[qh * 4 + 2];
-    uint32_t m = q[qh * 4 + 3];
-    qh++;
-    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
-  }
-#undef TAKE_PUSH
-  free(q);
-  free(trig);
-  /* maint (sim.wgsl:1219-1274) on every dirty cell */
-  for (size_t ci = 0; ci < ncell; ci++) {
-    if (!E.dirty[ci] || kind[ci] != 2) continue;
-    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8;
-    int32_t oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
-    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
-    uint32_t first = vox[vkey(&E, ox, oy, oz)];
-    int nz = 0, diff = 0;
-    for (uint32_t l = 0; l < 512u; l++) {
-      uint32_t v = vox[vkey(&E, ox + (int32_t)(l & 7u), oy + (int32_t)((l >> 3) & 7u), oz + (int32_t)(l >> 6))];
-      if (v != 0u) nz = 1;
-      if (v != first) diff = 1;
-    }
-    if (!nz) kind[ci] = 0;
-    else if (!diff) kind[ci] = 1;
-  }
-  free(E.dirty);
-  free(req);
-  if (stats) {
-    stats[0] = destroyed;
-    stats[1] = bill;
-    stats[2] = allocs;
-    stats[3] = njobs;
-    stats[4] = flooded;
-  }
-  return o_islands(dx, dy, dz, vox, mats, lab_key, lab_count, cap, n_labels);
+  o_edit_params P;
+  P.mode = 0u;
+  P.node_budget = 0xFFFFFFFFu;
+  P.vs = 1.0f / 80.0f;
+  P.frame_seed = 0u;
+  uint64_t st[8];
+  int rc = o_edit_pass(dx, dy, dz, vox, kind, mats, &P, impacts16, n_impacts, NULL, 0u, lab_key, lab_count, cap,
+                       n_labels, NULL, 0u, NULL, st);
+  if (stats)
+    for (int k = 0; k < 5; k++) stats[k] = st[k];
+  return rc;
 }
 
 /* Breadth-first labelling of the components of non-air, non-rubble,
@@ -1227,6 +1076,415 @@ static void drop_rubble_above(ed_world* e, int32_t x, int32_t y, int32_t z, floa
   }
 }
 
+/* ---------------------------------------------------------------------- */
+/* The edit pass with its side effects: impactJobs, allocBricks, carve with
+ * carveEffects (sim.wgsl:414-524), the pane flood with shardEffects
+ * (:250-271, 589-618), maint (:1219-1274) and the island search
+ * (:620-882), restated serially in the order the HIP library fixes:
+ *  - a voxel several impacts remove is removed by the lowest-index one;
+ *  - effects run for the carved voxels in (cell, local index) order, then for
+ *    the flooded ones in the same order; a voxel's effects are its
+ *    dropRubbleAbove records (bottom to top), then its own debris record;
+ *  - seeds are the 6-neighbours of carved and flooded voxels that pushSeed
+ *    keeps (not structural, not rubble, not a same-id neighbour of a pane
+ *    voxel), plus the pending seeds of landed chunks; a seed counts when its
+ *    voxel is a node voxel once the edits are done.
+ * In seeded mode only components that hold a seed are searched, and the
+ * floating ones are resolved in label order while their voxel total stays
+ * within node_budget (the rest are assumed supported, as the reference
+ * assumes the unexplored frontier is when its node budget trips). */
+
+static uint32_t bit_get(const uint8_t* b, size_t i) { return (b[i >> 3] >> (i & 7)) & 1u; }
+static void bit_set(uint8_t* b, size_t i) { b[i >> 3] |= (uint8_t)(1u << (i & 7)); }
+
+static int cmp_u64(const void* a, const void* b) {
+  uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* (cell << 9 | local index) of a voxel, the effects order */
+static uint64_t cell_order(const ed_world* e, int32_t x, int32_t y, int32_t z) {
+  return ((uint64_t)cell_of(e, x, y, z) << 9) | (uint64_t)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6));
+}
+
+/* carveEffects' debris record (sim.wgsl:500-508) */
+static void carve_debris(const float* mats, const float* im, int32_t x, int32_t y, int32_t z, uint32_t key, uint32_t v,
+                         uint32_t fs, o_particle* sp, uint32_t cap, uint32_t* ns) {
+  const float r = im[3];
+  const float keep = fminf(1.0f, 30000.0f / fmaxf(1.0f, 4.19f * r * r * r));
+  const uint32_t seed = key ^ fs;
+  if (!(rnd01(seed) < matE(mats, v & 0xffu)[3] * keep)) return;
+  float d[3] = {((float)x + 0.5f) - im[0], ((float)y + 0.5f) - im[1], ((float)z + 0.5f) - im[2]};
+  const float len = sqrtf(dot3(d, d));
+  const float den = len < 1e-3f ? 1e-3f : len;
+  const float s = im[5] * (0.3f + 0.9f * rnd01(seed + 7u));
+  const float off[3] = {0.5f, 0.2f, 0.5f}, m[3] = {0.4f, 0.5f, 0.4f};
+  float vel[3];
+  for (int a = 0; a < 3; a++) {
+    const float dirn = d[a] / den;
+    const float jitter = ((rnd01(seed + 13u + (uint32_t)a) - off[a]) * im[5]) * m[a];
+    vel[a] = ((dirn * 0.7f + im[6 + a] * 0.6f) - im[9 + a] * 0.2f) * s + jitter;
+  }
+  const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+  push_spawn(sp, cap, ns, pos, vel, v | debris_flags(mats, v, seed), 5.0f + 4.0f * rnd01(seed + 11u));
+}
+
+/* shardEffects' debris record (sim.wgsl:265-270) */
+static void shard_debris(const float* mats, int32_t x, int32_t y, int32_t z, uint32_t key, uint32_t v, float vs,
+                         uint32_t fs, o_particle* sp, uint32_t cap, uint32_t* ns) {
+  const uint32_t seed = key ^ fs;
+  if (!(rnd01(seed) < matE(mats, v & 0xffu)[3] * 0.6f)) return;
+  const float s = 1.0f / vs;
+  const float b[3] = {rnd01(seed + 3u) - 0.5f, rnd01(seed + 4u) * 0.4f, rnd01(seed + 5u) - 0.5f};
+  const float m[3] = {1.6f, 1.0f, 1.6f};
+  float vel[3];
+  for (int a = 0; a < 3; a++) vel[a] = (b[a] * m[a]) * s;
+  const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+  push_spawn(sp, cap, ns, pos, vel, v | debris_flags(mats, v, seed), 4.0f + 4.0f * rnd01(seed + 6u));
+}
+
+/* pushSeed candidates around a removed voxel of value v (carveEffects
+ * :511-523, floodStep :606-616): neighbours that are not air, structural or
+ * rubble, and not of v's id when v is a pane; checked on the final world. */
+static void mark_seeds(const ed_world* e, const float* mats, int32_t x, int32_t y, int32_t z, uint32_t v, uint8_t* seed) {
+  const int pane = matE(mats, v & 0xffu)[1] > 1.5f;
+  for (int k = 0; k < 6; k++) {
+    int32_t qx = x + DIRS[k][0], qy = y + DIRS[k][1], qz = z + DIRS[k][2];
+    if (!in_world(e, qx, qy, qz)) continue;
+    uint32_t w = e->vox[vkey(e, qx, qy, qz)];
+    if (w == 0u || structural(mats, w) || is_rubble(w)) continue;
+    if (pane && (w & 0xffu) == (v & 0xffu)) continue;
+    bit_set(seed, vkey(e, qx, qy, qz));
+  }
+}
+
+/* components of node voxels (non-air, non-rubble, non-structural) holding a
+ * seed, in key order; floating ones are kept while their running voxel total
+ * stays within node_budget */
+static int seeded_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats,
+                          const uint8_t* seed, uint32_t node_budget, uint32_t* lab_key, uint32_t* lab_count,
+                          uint32_t cap, uint32_t* n_labels, uint64_t* dropped) {
+  size_t N = (size_t)dx * dy * dz;
+  uint8_t* seen = (uint8_t*)calloc((N + 7) / 8, 1);
+  uint32_t* q = (uint32_t*)malloc(N * sizeof(uint32_t) + 4);
+  if (!seen || !q) { free(seen); free(q); return -1; }
+  uint32_t nl = 0;
+  uint64_t total = 0, drop = 0;
+  int over = 0;
+  size_t plane = (size_t)dx * dy;
+  for (size_t s = 0; s < N; s++) {
+    uint32_t v = vox[s];
+    if (v == 0u || is_rubble(v) || structural(mats, v)) continue;
+    if (bit_get(seen, s)) continue;
+    bit_set(seen, s);
+    size_t qh = 0, qt = 0;
+    q[qt++] = (uint32_t)s;
+    int anchored = 0, seeded = 0;
+    while (qh < qt) {
+      size_t k = q[qh++];
+      if (bit_get(seed, k)) seeded = 1;
+      int32_t x = (int32_t)(k % (size_t)dx), y = (int32_t)((k / (size_t)dx) % (size_t)dy), z = (int32_t)(k / plane);
+      if (y == 0) anchored = 1;
+      for (int d = 0; d < 6; d++) {
+        int32_t nx = x + DIRS[d][0], ny = y + DIRS[d][1], nz = z + DIRS[d][2];
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= dx || ny >= dy || nz >= dz) continue;
+        size_t nk = (size_t)nx + (size_t)dx * ((size_t)ny + (size_t)dy * (size_t)nz);
+        uint32_t w = vox[nk];
+        if (w == 0u || is_rubble(w)) continue;
+        if (structural(mats, w)) { anchored = 1; continue; }
+        if (bit_get(seen, nk)) continue;
+        bit_set(seen, nk);
+        q[qt++] = (uint32_t)nk;
+      }
+    }
+    if (anchored || !seeded) continue;
+    if (!over && total + qt <= node_budget) {
+      total += qt;
+      if (nl < cap) {
+        lab_key[nl] = (uint32_t)s;
+        lab_count[nl] = (uint32_t)qt;
+      }
+      nl++;
+    } else {
+      over = 1;
+      drop++;
+    }
+  }
+  free(seen);
+  free(q);
+  *n_labels = nl;
+  if (dropped) *dropped = drop;
+  return 0;
+}
+
+int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+                const o_edit_params* P, const float* impacts16, uint32_t n_impacts, const uint32_t* seed_keys,
+                uint32_t n_seed_keys, uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels,
+                o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns, uint64_t* stats) {
+  ed_world E;
+  E.dx = dx; E.dy = dy; E.dz = dz;
+  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
+  E.vox = vox;
+  E.kind = kind;
+  E.mats = mats;
+  const size_t ncell = (size_t)E.bx * E.by * E.bz;
+  const size_t N = (size_t)dx * dy * dz;
+  E.dirty = (uint8_t*)calloc(ncell, 1);
+  uint8_t* req = (uint8_t*)calloc(ncell, 1);
+  uint64_t* cmask = (uint64_t*)calloc(ncell, sizeof(uint64_t));
+  uint8_t* seed = (uint8_t*)calloc((N + 7) / 8, 1);
+  uint8_t* flooded = (uint8_t*)calloc((N + 7) / 8, 1);
+  if (!E.dirty || !req || !cmask || !seed || !flooded) {
+    free(E.dirty); free(req); free(cmask); free(seed); free(flooded);
+    return -1;
+  }
+  uint64_t destroyed = 0, bill = 0, allocs = 0, njobs = 0, nflood = 0, ndrop = 0;
+  uint32_t ns = 0;
+  if (n_impacts > 64u) n_impacts = 64u;
+  /* impactJobs (sim.wgsl:414-443) on the grid before any alloc: per cell,
+   * the mask of impacts with a job there */
+  const int32_t bd[3] = {E.bx, E.by, E.bz};
+  for (uint32_t k = 0; k < n_impacts; k++) {
+    const float* im = impacts16 + (size_t)k * 16u;
+#ifdef O_PLANT_AABB_NO_1_2
+    float R = ceilf(fmaxf(im[3], im[4])) + 1.0f;
+#else
+    float R = ceilf(fmaxf(im[3], im[4]) * 1.2f) + 1.0f;
+#endif
+    int32_t bmin[3], bmax[3];
+    for (int a = 0; a < 3; a++) {
+      bmin[a] = iclamp(f2i(floorf((im[a] - R) / 8.0f)), 0, bd[a] - 1);
+      bmax[a] = iclamp(f2i(floorf((im[a] + R) / 8.0f)), 0, bd[a] - 1);
+    }
+    for (int32_t bz = bmin[2]; bz <= bmax[2]; bz++)
+      for (int32_t by = bmin[1]; by <= bmax[1]; by++)
+        for (int32_t bx = bmin[0]; bx <= bmax[0]; bx++) {
+          size_t ci = (size_t)bx + (size_t)E.bx * ((size_t)by + (size_t)E.by * (size_t)bz);
+          if (kind[ci] == 0) continue;
+          float lo[3] = {(float)((uint32_t)bx * 8u), (float)((uint32_t)by * 8u), (float)((uint32_t)bz * 8u)};
+          float ev[3];
+          for (int a = 0; a < 3; a++) ev[a] = fmaxf(fmaxf(lo[a] - im[a], 0.0f), im[a] - (lo[a] + 8.0f));
+          float near2 = dot3(ev, ev);
+          if (near2 > R * R) continue;
+          if (kind[ci] == 1) {
+            uint32_t uval = vox[vkey(&E, bx * 8, by * 8, bz * 8)];
+            float rch = reach(im, mats, uval & 0xffu);
+            if (near2 > rch * rch) continue;
+            req[ci] = 1;
+          }
+          cmask[ci] |= 1ull << k;
+          njobs++;
+        }
+  }
+  /* allocBricks (sim.wgsl:459-483): uniform cells in reach become pool bricks */
+  for (size_t ci = 0; ci < ncell; ci++)
+    if (req[ci] && kind[ci] == 1) {
+      kind[ci] = 2;
+      E.dirty[ci] = 1;
+      allocs++;
+    }
+  /* carve (sim.wgsl:529-579), cell by cell: (order key, remover, value) of
+   * every carved voxel */
+  size_t ccap = 1024, cn = 0;
+  uint64_t* carved = (uint64_t*)malloc(ccap * 2 * sizeof(uint64_t));
+  for (size_t ci = 0; ci < ncell; ci++) {
+    if (!cmask[ci] || kind[ci] != 2) continue;
+    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8;
+    int32_t oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
+    int any = 0;
+    for (uint32_t l = 0; l < 512u; l++) {
+      int32_t x = ox + (int32_t)(l & 7u), y = oy + (int32_t)((l >> 3) & 7u), z = oz + (int32_t)(l >> 6);
+      size_t kk = vkey(&E, x, y, z);
+      uint32_t v = vox[kk];
+      if (v == 0u) continue;
+      uint64_t m = cmask[ci];
+      int rem = -1;
+      while (m) {
+        int k = __builtin_ctzll(m);
+        m &= m - 1ull;
+        if (o_removes(impacts16 + (size_t)k * 16u, mats, x, y, z, v)) {
+          rem = k;
+          break;
+        }
+      }
+      if (rem < 0) continue;
+      vox[kk] = 0;
+      any = 1;
+      destroyed++;
+      bill += price_dimes(mats, v);
+      if (cn == ccap) {
+        ccap *= 2;
+        carved = (uint64_t*)realloc(carved, ccap * 2 * sizeof(uint64_t));
+      }
+      carved[cn * 2] = ((uint64_t)ci << 9) | l;
+      carved[cn * 2 + 1] = ((uint64_t)rem << 32) | v;
+      cn++;
+    }
+    if (any) E.dirty[ci] = 1;
+  }
+  /* carveEffects of every carved voxel, in (cell, local index) order:
+   * dropRubbleAbove, then the carve debris */
+  for (size_t i = 0; i < cn; i++) {
+    size_t ci = (size_t)(carved[i * 2] >> 9);
+    uint32_t l = (uint32_t)(carved[i * 2] & 511u);
+    int32_t x = (int32_t)(ci % (size_t)E.bx) * 8 + (int32_t)(l & 7u);
+    int32_t y = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8 + (int32_t)((l >> 3) & 7u);
+    int32_t z = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8 + (int32_t)(l >> 6);
+    uint32_t k = (uint32_t)(carved[i * 2 + 1] >> 32), v = (uint32_t)(carved[i * 2 + 1] & 0xffffu);
+    uint32_t before = ns;
+    drop_rubble_above(&E, x, y, z, P->vs, P->frame_seed, spawns, spawn_cap, &ns);
+    ndrop += ns - before;
+    carve_debris(mats, impacts16 + (size_t)k * 16u, x, y, z, (uint32_t)vkey(&E, x, y, z), v, P->frame_seed, spawns,
+                 spawn_cap, &ns);
+  }
+  /* the pane flood to completion (carveEffects :511-523 + floodStep
+   * :599-618): breadth-first over same-id voxels in pool bricks (takeVoxel
+   * removes voxels of pool bricks only), from every carved pane voxel */
+  size_t qcap = 1024, qh = 0, qt = 0;
+  uint32_t* q = (uint32_t*)malloc(qcap * 4 * sizeof(uint32_t));
+  size_t fcap = 1024, fn = 0;
+  uint64_t* fl = (uint64_t*)malloc(fcap * 2 * sizeof(uint64_t));
+#define TAKE_PUSH(X, Y, Z, M)                                                        \
+  do {                                                                               \
+    int32_t X_ = (X), Y_ = (Y), Z_ = (Z);                                            \
+    if (in_world(&E, X_, Y_, Z_) && kind[cell_of(&E, X_, Y_, Z_)] == 2) {             \
+      size_t kk_ = vkey(&E, X_, Y_, Z_);                                             \
+      uint32_t v_ = vox[kk_];                                                        \
+      if (v_ != 0u && (v_ & 0xffu) == (M) FLOOD_ID_TEST) {                            \
+        vox[kk_] = 0;                                                                \
+        E.dirty[cell_of(&E, X_, Y_, Z_)] = 1;                                        \
+        destroyed++;                                                                 \
+        nflood++;                                                                    \
+        bill += price_dimes(mats, v_);                                               \
+        bit_set(flooded, kk_);                                                       \
+        if (qt == qcap) {                                                            \
+          qcap *= 2;                                                                 \
+          q = (uint32_t*)realloc(q, qcap * 4 * sizeof(uint32_t));                    \
+        }                                                                            \
+        q[qt * 4] = (uint32_t)X_;                                                    \
+        q[qt * 4 + 1] = (uint32_t)Y_;                                                \
+        q[qt * 4 + 2] = (uint32_t)Z_;                                                \
+        q[qt * 4 + 3] = (M);                                                         \
+        qt++;                                                                        \
+        if (fn == fcap) {                                                            \
+          fcap *= 2;                                                                 \
+          fl = (uint64_t*)realloc(fl, fcap * 2 * sizeof(uint64_t));                  \
+        }                                                                            \
+        fl[fn * 2] = cell_order(&E, X_, Y_, Z_);                                     \
+        fl[fn * 2 + 1] = v_;                                                         \
+        fn++;                                                                        \
+      }                                                                              \
+    }                                                                                \
+  } while (0)
+#ifdef O_PLANT_FLOOD_ANY_PANE
+#define FLOOD_ID_TEST || (matE(mats, v_ & 0xffu)[1] > 1.5f)
+#else
+#define FLOOD_ID_TEST
+#endif
+  for (size_t i = 0; i < cn; i++) {
+    uint32_t v = (uint32_t)(carved[i * 2 + 1] & 0xffffu);
+    if (!(matE(mats, v & 0xffu)[1] > 1.5f)) continue;
+    size_t ci = (size_t)(carved[i * 2] >> 9);
+    uint32_t l = (uint32_t)(carved[i * 2] & 511u);
+    int32_t x = (int32_t)(ci % (size_t)E.bx) * 8 + (int32_t)(l & 7u);
+    int32_t y = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8 + (int32_t)((l >> 3) & 7u);
+    int32_t z = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8 + (int32_t)(l >> 6);
+    uint32_t m = v & 0xffu;
+    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
+  }
+  while (qh < qt) {
+    int32_t x = (int32_t)q[qh * 4], y = (int32_t)q[qh * 4 + 1], z = (int32_t)q[qh * 4 + 2];
+    uint32_t m = q[qh * 4 + 3];
+    qh++;
+    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
+  }
+#undef TAKE_PUSH
+#undef FLOOD_ID_TEST
+  free(q);
+  /* shardEffects of every flooded voxel, in (cell, local index) order */
+  qsort(fl, fn, 2 * sizeof(uint64_t), cmp_u64);
+  for (size_t i = 0; i < fn; i++) {
+    size_t ci = (size_t)(fl[i * 2] >> 9);
+    uint32_t l = (uint32_t)(fl[i * 2] & 511u);
+    int32_t x = (int32_t)(ci % (size_t)E.bx) * 8 + (int32_t)(l & 7u);
+    int32_t y = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8 + (int32_t)((l >> 3) & 7u);
+    int32_t z = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8 + (int32_t)(l >> 6);
+    uint32_t v = (uint32_t)fl[i * 2 + 1];
+    uint32_t before = ns;
+    drop_rubble_above(&E, x, y, z, P->vs, P->frame_seed, spawns, spawn_cap, &ns);
+    ndrop += ns - before;
+    shard_debris(mats, x, y, z, (uint32_t)vkey(&E, x, y, z), v, P->vs, P->frame_seed, spawns, spawn_cap, &ns);
+  }
+  /* maint (sim.wgsl:1219-1274) on every dirty cell */
+  for (size_t ci = 0; ci < ncell; ci++) {
+    if (!E.dirty[ci] || kind[ci] != 2) continue;
+    int32_t ox = (int32_t)(ci % (size_t)E.bx) * 8;
+    int32_t oy = (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8;
+    int32_t oz = (int32_t)(ci / ((size_t)E.bx * E.by)) * 8;
+    uint32_t first = vox[vkey(&E, ox, oy, oz)];
+    int nz = 0, diff = 0;
+    for (uint32_t l = 0; l < 512u; l++) {
+      uint32_t v = vox[vkey(&E, ox + (int32_t)(l & 7u), oy + (int32_t)((l >> 3) & 7u), oz + (int32_t)(l >> 6))];
+      if (v != 0u) nz = 1;
+      if (v != first) diff = 1;
+    }
+    if (!nz) kind[ci] = 0;
+    else if (!diff) kind[ci] = 1;
+  }
+  /* seeds on the final world */
+  uint64_t nseed = 0;
+  if (P->mode == 1u) {
+    for (size_t i = 0; i < cn; i++) {
+      size_t ci = (size_t)(carved[i * 2] >> 9);
+      uint32_t l = (uint32_t)(carved[i * 2] & 511u);
+      mark_seeds(&E, mats, (int32_t)(ci % (size_t)E.bx) * 8 + (int32_t)(l & 7u),
+                 (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8 + (int32_t)((l >> 3) & 7u),
+                 (int32_t)(ci / ((size_t)E.bx * E.by)) * 8 + (int32_t)(l >> 6),
+                 (uint32_t)(carved[i * 2 + 1] & 0xffffu), seed);
+    }
+    for (size_t i = 0; i < fn; i++) {
+      size_t ci = (size_t)(fl[i * 2] >> 9);
+      uint32_t l = (uint32_t)(fl[i * 2] & 511u);
+      mark_seeds(&E, mats, (int32_t)(ci % (size_t)E.bx) * 8 + (int32_t)(l & 7u),
+                 (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8 + (int32_t)((l >> 3) & 7u),
+                 (int32_t)(ci / ((size_t)E.bx * E.by)) * 8 + (int32_t)(l >> 6), (uint32_t)fl[i * 2 + 1], seed);
+    }
+    for (uint32_t i = 0; i < n_seed_keys; i++)
+      if (seed_keys[i] < N) bit_set(seed, seed_keys[i]);
+    for (size_t i = 0; i < N; i++)
+      if (bit_get(seed, i)) {
+        uint32_t w = vox[i];
+        if (w != 0u && !is_rubble(w) && !structural(mats, w)) nseed++;
+      }
+  }
+  free(carved);
+  free(fl);
+  free(E.dirty);
+  free(req);
+  free(cmask);
+  free(flooded);
+  uint64_t dropped = 0;
+  int rc;
+  if (P->mode == 1u)
+    rc = seeded_islands(dx, dy, dz, vox, mats, seed, P->node_budget, lab_key, lab_count, cap, n_labels, &dropped);
+  else
+    rc = o_islands(dx, dy, dz, vox, mats, lab_key, lab_count, cap, n_labels);
+  free(seed);
+  if (n_spawns) *n_spawns = ns;
+  if (stats) {
+    stats[0] = destroyed;
+    stats[1] = bill;
+    stats[2] = allocs;
+    stats[3] = njobs;
+    stats[4] = nflood;
+    stats[5] = nseed;
+    stats[6] = dropped;
+    stats[7] = ndrop;
+  }
+  return rc;
+}
+
 int o_resolve(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
               const uint32_t* lab_key, const uint32_t* lab_count, uint32_t n_labels, const o_resolve_params* prm,
               uint32_t* chunks, uint32_t* arena, uint32_t* ctr, o_particle* spawns, uint32_t spawn_cap,
@@ -1369,7 +1627,8 @@ static inline uint32_t f2u(float x) {
 int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
                   uint32_t* chunks, uint32_t* arena, uint32_t* ctr, const o_resolve_params* prm, float dt,
                   float gravity, uint32_t max_writes, uint32_t* writes, uint32_t max_impacts, float* impacts,
-                  o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns) {
+                  o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns, uint32_t* seeds, uint32_t seed_cap,
+                  uint32_t* n_seeds) {
   ed_world E;
   E.dx = dx; E.dy = dy; E.dz = dz;
   E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
@@ -1494,6 +1753,14 @@ int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* ki
       int32_t p[3] = {px, py, pz};
       writes[2u * k] = (uint32_t)vkey(&E, p[0], p[1], p[2]);
       writes[2u * k + 1u] = v;
+    }
+    /* whatever it landed on is re-checked for support (pushSeed, :1068-1072) */
+    if ((o_hashu(key) & 15u) == 0u) {
+      uint32_t w = read_dense(&E, px, py - 1, pz);
+      if (w != 0u && !structural(mats, w)) {
+        if (n_seeds && *n_seeds < seed_cap) seeds[*n_seeds] = (uint32_t)vkey(&E, px, py - 1, pz);
+        if (n_seeds) (*n_seeds)++;
+      }
     }
   }
   return 0;

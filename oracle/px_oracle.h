@@ -59,6 +59,14 @@ void o_frame(const o_world* w, const float* u52, const float* lights_vec4, uint3
              const float* rays, uint32_t n, o_hit* prim_out, o_hit* shadow_out, uint32_t* n_cast,
              int threads);
 
+/* Debris particle (shared.wgsl:33-38), 32 bytes. */
+typedef struct o_particle {
+  float pos[3];
+  uint32_t value;
+  float vel[3];
+  float life;
+} o_particle;
+
 /* Edit chain: carve + pane flood + maint + island labels (sim.wgsl:194-944,1205-1274).
  * The world is given as a dense voxel array (x fastest) and a per-cell kind
  * (0 empty, 1 uniform, 2 pool brick) taken from the grid before the pass.
@@ -70,6 +78,27 @@ void o_frame(const o_world* w, const float* u52, const float* lights_vec4, uint3
 int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
                   const float* impacts16, uint32_t n_impacts, uint32_t* lab_key, uint32_t* lab_count,
                   uint32_t cap, uint32_t* n_labels, uint64_t* stats);
+
+/* The edit pass with its side effects and the seeded search.  mode 0 labels
+ * every component (o_island_pass is mode 0 with the debris dropped), mode 1
+ * only components that hold a seed (pushSeed, sim.wgsl:186, 524, 615, 1068),
+ * with the floating ones kept in label order while their voxel total stays
+ * within node_budget.  vs = U.dims.w, frame_seed = u32(U.time.y) *
+ * 0x9E3779B9.  seed_keys: pending seeds of landed chunks.  Every spawn()
+ * call (dropRubbleAbove, carve debris, shard debris) is written to spawns in
+ * call order up to spawn_cap; *n_spawns counts them all.  stats: [0..4] as
+ * o_island_pass, [5] seed voxels that are node voxels at the search,
+ * [6] floating components over the node budget, [7] rubble voxels dropped. */
+typedef struct o_edit_params {
+  uint32_t mode;
+  uint32_t node_budget;
+  float vs;
+  uint32_t frame_seed;
+} o_edit_params;
+int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+                const o_edit_params* P, const float* impacts16, uint32_t n_impacts, const uint32_t* seed_keys,
+                uint32_t n_seed_keys, uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels,
+                o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns, uint64_t* stats);
 
 /* Unsupported-component labelling alone (the last stage of o_island_pass). */
 int o_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats,
@@ -102,13 +131,6 @@ float o_rnd(uint32_t seed);
 /* removes() of sim.wgsl:229-239 for one voxel (impact = 16 floats). */
 int o_removes(const float* impact16, const float* mats, int32_t x, int32_t y, int32_t z, uint32_t v);
 
-/* Debris particle (shared.wgsl:33-38), 32 bytes. */
-typedef struct o_particle {
-  float pos[3];
-  uint32_t value;
-  float vel[3];
-  float life;
-} o_particle;
 
 /* Particle counters: C_SPAWNED, C_PCURSOR, C_WRITES (sim.wgsl:167-168,187)
  * and the alive count args[A_DRAWP + 1] (:1083). */
@@ -170,11 +192,14 @@ int o_resolve(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, 
               uint32_t* n_spawns);
 /* One frame of chunkArgs / chunkBegin / chunkCollide / chunkStep / chunkLand:
  * writes (key, value) pushed at C_WRITES, impacts (16 floats) at C_IMPACTS,
- * shattered voxels spawned in arena order.  The world is read, not written. */
+ * shattered voxels spawned in arena order, and the keys of the voxels under
+ * landed voxels that pushSeed re-checks (seeds; n_seeds counts them all).
+ * The world is read, not written. */
 int o_chunk_frame(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
                   uint32_t* chunks, uint32_t* arena, uint32_t* ctr, const o_resolve_params* prm, float dt,
                   float gravity, uint32_t max_writes, uint32_t* writes, uint32_t max_impacts, float* impacts,
-                  o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns);
+                  o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns, uint32_t* seeds, uint32_t seed_cap,
+                  uint32_t* n_seeds);
 
 /* SDF primitive in voxel units, the px_prim layout of include/pixels.h:
  * type 0 box (c, h), 1 rounded box (c, h, r), 2 sphere (c, r), 3 capped
