@@ -564,15 +564,13 @@ int px_world_info(const px_ctx* c, uint32_t dims[3], uint32_t bricks[3], uint32_
 int px_push_impacts(px_ctx* c, const float* impacts16, uint32_t n) {
   if (!c || (n && !impacts16)) return PX_ERR_ARG;
   uint32_t maxi = c->lim.max_impacts < 64u ? c->lim.max_impacts : 64u;
-  if ((uint64_t)c->n_impacts + n > maxi)
+  if (n > maxi)
     return px_fail(c, PX_ERR_CAPACITY, "px_push_impacts", "more impacts than max_impacts (at most 64 per pass)");
   if (!n) return PX_OK;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_impacts + (size_t)c->n_impacts * 16, impacts16, (size_t)n * 64,
-                                 hipMemcpyHostToDevice, c->stream));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_impact_stage, impacts16, (size_t)n * 64, hipMemcpyHostToDevice, c->stream));
+  if (int rc = px_launch_push_impacts(c, n)) return rc;
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  std::memcpy(c->h_impacts + (size_t)c->n_impacts * 16, impacts16, (size_t)n * 64);
-  c->n_impacts += n;
   return PX_OK;
 }
 
@@ -605,9 +603,9 @@ int px_restore(px_ctx* c) {
   PX_HIP_CHECK(c, hipMemcpyAsync(c->d_super, c->b_super, (size_t)c->nsuper * 4, hipMemcpyDeviceToDevice, s));
   PX_HIP_CHECK(c, hipMemcpyAsync(c->d_ctrl, c->b_ctrl, PX_CTRL_WORDS * 4, hipMemcpyDeviceToDevice, s));
   PX_HIP_CHECK(c, hipMemcpyAsync(c->d_free, c->b_free, (size_t)c->pool_cap * 4, hipMemcpyDeviceToDevice, s));
-  c->n_impacts = 0;
   c->labels_valid = false;
   c->masks_dirty = c->density_dirty = true;
+  if (int rc = px_clear_seeds(c)) return rc;  // pending seeds belong to the world being dropped
   return px_refresh_masks(c);
 }
 
@@ -639,8 +637,7 @@ int px_profile_end(px_ctx* c, double out[2 * PX_PROF_SLOTS]) {
 int px_island_pass(px_ctx* c) {
   if (!c) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  if (int rc = px_merge_device_impacts(c)) return rc;
-  return px_launch_island_pass(c);
+  return px_launch_island_pass(c, true);
 }
 
 int px_sim_step(px_ctx* c, uint32_t stages_mask) {
@@ -655,10 +652,8 @@ int px_sim_step(px_ctx* c, uint32_t stages_mask) {
     if (int rc = px_launch_frame_begin(c)) return rc;
   }
   if (stages_mask & PX_STAGE_EDIT) {
-    if (int rc = px_merge_device_impacts(c)) return rc;
-    if (c->n_impacts) {
-      if (int rc = px_launch_island_pass(c)) return rc;
-    }
+    // runs on the device only with queued impacts (or pending seeds, seeded mode)
+    if (int rc = px_launch_island_pass(c, false)) return rc;
   }
   if ((stages_mask & PX_STAGE_RESOLVE) && c->labels_valid) {
     if (int rc = px_launch_resolve(c)) return rc;
@@ -711,9 +706,9 @@ int px_read_device_impacts(px_ctx* c, float* out16, uint32_t cap, uint32_t* n) {
   PX_HIP_CHECK(c, hipMemcpyAsync(&k, c->d_ctrl + PX_C_IMPACTS, 4, hipMemcpyDeviceToHost, c->stream));
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   *n = k;
-  const uint32_t m = std::min(std::min(k, cap), c->lim.max_impacts);
+  const uint32_t m = std::min(std::min(k, cap), std::min<uint32_t>(c->lim.max_impacts, 64u));
   if (m) {
-    PX_HIP_CHECK(c, hipMemcpyAsync(out16, c->d_dev_impacts, (size_t)m * 64, hipMemcpyDeviceToHost, c->stream));
+    PX_HIP_CHECK(c, hipMemcpyAsync(out16, c->d_impacts, (size_t)m * 64, hipMemcpyDeviceToHost, c->stream));
     PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
   }
   return PX_OK;
@@ -728,13 +723,9 @@ int px_apply_writes(px_ctx* c) {
 int px_run_to_quiescence(px_ctx* c) {
   if (!c) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  if (int rc = px_merge_device_impacts(c)) return rc;
-  if (c->n_impacts) {
-    int rc = px_launch_island_pass(c);
-    if (rc) return rc;
-  }
+  if (int rc = px_launch_island_pass(c, false)) return rc;
   PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  return c->labels_valid ? px_check_island_pass(c, "px_run_to_quiescence") : PX_OK;
+  return px_check_island_pass(c, "px_run_to_quiescence");
 }
 
 int px_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n) {

@@ -57,8 +57,8 @@ constexpr uint32_t kCountWords = 4;     // per-block list lengths
 
 // edit-pass counters (ec[])
 enum : uint32_t {
-  EC_JOBCELLS = 0, EC_ALLOCS, EC_UNUSED2, EC_UNUSED3, EC_LABELS, EC_UNUSED5, EC_JOBS, EC_CHUNKS,
-  EC_UNUSED8, EC_OVERFLOW, EC_WORDS = 16,
+  EC_JOBCELLS = 0, EC_ALLOCS, EC_ACTIVE, EC_NIMP, EC_LABELS, EC_LABELS_OK, EC_JOBS, EC_CHUNKS,
+  EC_SEEDS, EC_OVERFLOW, EC_NB_T, EC_NB_DROP, EC_REEMIT, EC_WORDS = 16,
   // voxel-write stage (after the island pass's words)
   EC_W_N = 16, EC_W_DIRTY, EC_W_WRITTEN, EC_W_DROP,
   // extraction and chunk stages
@@ -68,6 +68,11 @@ enum : uint32_t {
 // kept in the world (over the component budget), crumbles into debris, and
 // the chunk id + 1 in bits 8..16 (resolveChunks, sim.wgsl:862-880)
 constexpr uint32_t kFlagKeep = 4u;
+// component flag of the seeded search: the component holds a seed voxel
+constexpr uint32_t kFlagSeed = 16u;
+// impact rows of k_impact_jobs: blocks of 256 cells per impact, grid-stride
+// over the impact's box (MAX_IMPACTS = 64 rows)
+constexpr uint32_t kJobBlocks = 32;
 constexpr uint32_t kFlagCrumb = 8u;
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 constexpr uint32_t kMaxChunkIds = 511;  // chunk id + 1 fits bits 8..16
@@ -104,8 +109,13 @@ struct EditView {
   uint32_t* out;
   uint32_t out_cap;
   uint32_t* ec;
-  const float* impacts;
-  uint32_t n_impacts;
+  float* impacts;     // the impact queue (pushImpact's records), C_IMPACTS of them
+  uint32_t max_imp;   // min(max_impacts, 64)
+  uint32_t* seeds;    // seeded mode: one bit per voxel key, pushSeed (sim.wgsl:180-184)
+  uint32_t seed_words;
+  uint32_t mode;      // PX_ISLANDS_ALL / PX_ISLANDS_SEEDED
+  uint32_t node_budget;
+  uint32_t* etot;     // totals of the effects scan: carve records, flood records
   // voxel writes
   uint32_t* wlist;
   uint32_t max_writes, max_alloc;
@@ -126,7 +136,7 @@ struct EditView {
   uint32_t pmask, spawn_budget;
   uint32_t fseed;    // frameSeed(), sim.wgsl:69
   float vs, dt, grav;
-  float* dev_impacts;  // impacts pushed by falling chunks
+
 };
 
 // Error bits of ec[EC_OVERFLOW]: a node list overflowed, a brick label was
@@ -147,8 +157,9 @@ constexpr uint32_t kLdsWalkCap = 512u;
 // `cap` steps, which no chain of distinct nodes reaches (cap >= the node
 // capacity of the pass); either way it sets kErrWalk and returns the node it
 // stands on, a key below the one it started from, so every access stays
-// inside par[] and every walk ends.
-PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x, uint32_t cap, uint32_t* err) {
+// inside par[] and every walk ends.  The error goes to a register (`err`)
+// that the kernel ORs into ec[EC_OVERFLOW] once, at its end.
+PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x, uint32_t cap, uint32_t& err) {
   for (uint32_t it = 0; it < cap; it++) {
     const uint32_t p = par[x];
     if (p == x) return x;
@@ -158,11 +169,11 @@ PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x, uint32_t cap, uint32_t* err) 
     if (gp != p) par[x] = gp;  // path halving; gp < p < x
     x = gp;
   }
-  atomicOr(err, kErrWalk);
+  err = kErrWalk;
   return x;
 }
 
-PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b, uint32_t cap, uint32_t* err) {
+PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b, uint32_t cap, uint32_t& err) {
   // each failed CAS leaves par[a] below a, so a strictly decreases
   for (uint32_t it = 0; it < cap; it++) {
     a = uf_find(par, a, cap, err);
@@ -178,13 +189,13 @@ PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b, uint32_t cap, uint32
     if (old > a) break;
     a = old;
   }
-  atomicOr(err, kErrWalk);
+  err = kErrWalk;
 }
 
 // LDS union-find over the 512 voxels of one brick (links to smaller indices,
 // path halving; parents only decrease, so a racing halving store still
 // writes an ancestor).  Same order check and step bound as uf_find.
-PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x, uint32_t* err) {
+PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x, uint32_t& err) {
   for (uint32_t it = 0; it < kLdsWalkCap; it++) {
     const uint32_t p = lab[x];
     if (p == x) return x;
@@ -194,11 +205,11 @@ PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x, uint32_t* err) {
     if (gp != p) lab[x] = gp;
     x = gp;
   }
-  atomicOr(err, kErrWalk);
+  err = kErrWalk;
   return x;
 }
 
-PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b, uint32_t* err) {
+PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b, uint32_t& err) {
   for (uint32_t it = 0; it < kLdsWalkCap; it++) {
     a = lds_find(lab, a, err);
     b = lds_find(lab, b, err);
@@ -213,7 +224,7 @@ PX_DEV void lds_union(uint32_t* lab, uint32_t a, uint32_t b, uint32_t* err) {
     if (old > a) break;
     a = old;
   }
-  atomicOr(err, kErrWalk);
+  err = kErrWalk;
 }
 
 // inclusive prefix sum over the wavefront
@@ -303,14 +314,52 @@ PX_DEV void load_tables(const float* mats, uint8_t* s_cls, uint8_t* s_str) {
 
 // ---------------------------------------------------------------------------
 
-__global__ void k_edit_reset(uint32_t* ec) {
-  if (threadIdx.x < EC_WORDS) ec[threadIdx.x] = 0u;
+// Start of a pass (impactArgs + searchBegin, sim.wgsl:408-412, 700-720): the
+// impacts to take (min(C_IMPACTS, MAX_IMPACTS)) and whether the pass runs at
+// all: when forced (px_island_pass), with impacts, or in seeded mode with
+// pending seeds.  Every later kernel of the pass reads this word, so a frame
+// can record the pass whatever the device holds.
+// A pass that does not run keeps the labels, their status and the error
+// bits of the last one that did; only its own work lists are emptied (k_alloc
+// and k_alloc_commit, shared with other stages, run ungated).
+__global__ void k_edit_begin(EditView e, uint32_t force) {
+  if (threadIdx.x != 0) return;
+  const uint32_t nimp = min(e.ctrl[PX_C_IMPACTS], e.max_imp);
+  const uint32_t pend = e.ctrl[PX_C_PENDING];
+  const bool active = force != 0u || nimp != 0u || (e.mode == PX_ISLANDS_SEEDED && pend != 0u);
+  if (active) {
+    for (uint32_t k = 0; k < EC_WORDS; k++) e.ec[k] = 0u;
+    e.ctrl[PX_C_PENDING] = 0u;
+  } else {
+    e.ec[EC_JOBCELLS] = 0u;
+    e.ec[EC_ALLOCS] = 0u;
+    e.ec[EC_JOBS] = 0u;
+  }
+  e.ec[EC_ACTIVE] = active ? 1u : 0u;
+  e.ec[EC_NIMP] = active ? nimp : 0u;
+  e.ec[EC_R_DIRTY] = 0u;
+}
+
+// pushImpact (sim.wgsl:203-211) for n staged records: record j goes to slot
+// C_IMPACTS + j while that is below MAX_IMPACTS; C_IMPACTS grows by n in
+// k_push_impacts_commit.
+__global__ void k_push_impacts(EditView e, const float* __restrict__ stage, uint32_t n) {
+  const uint32_t i = threadIdx.x;  // one block of 16 * 64 threads
+  const uint32_t j = i >> 4;
+  if (j >= n) return;
+  const uint32_t k = e.ctrl[PX_C_IMPACTS] + j;
+  if (k < e.max_imp) e.impacts[(size_t)k * 16u + (i & 15u)] = stage[i];
+}
+
+__global__ void k_push_impacts_commit(EditView e, uint32_t n) {
+  if (threadIdx.x == 0) e.ctrl[PX_C_IMPACTS] += n;
 }
 
 // impactJobs, sim.wgsl:414-443: one block row per impact; list appends are
 // aggregated per wavefront
 __global__ __launch_bounds__(256) void k_impact_jobs(EditView e) {
   const uint32_t k = blockIdx.y;
+  if (k >= e.ec[EC_NIMP]) return;
   const uint32_t lane = __lane_id();
   const float* im = e.impacts + (size_t)k * 16u;
   float R = ceilf(fmaxf(im[3], im[4]) * 1.2f) + 1.0f;
@@ -423,11 +472,13 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
   __shared__ uint8_t s_str[256];
   const uint32_t lane = threadIdx.x;
   const uint32_t ry = lane & 7u, rz = lane >> 3;
+  if (e.ec[MODE == 0 ? EC_ACTIVE : EC_NIMP] == 0u) return;
   load_tables<MODE>(e.mats, s_cls, s_str);
   __syncthreads();
   uint32_t* my_nodes = e.nodes + (size_t)blockIdx.x * e.slots_node;
   uint32_t* my_cells = e.pane_cells + (size_t)blockIdx.x * e.slots_cell;
   uint32_t nn = 0u, nc = 0u;  // wave-uniform list lengths of this block
+  uint32_t walk_err = 0u;
   for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) {
     uint32_t g = e.grid[ci];
     if (g == 0u) continue;
@@ -487,9 +538,9 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
     for (int x = 0; x < 8; x++) {
       uint32_t b = 1u << x, pb = b >> 1;
       if ((ey & b) && !(x > 0 && (ey & pb) && (lx & b) && (ylx & b)))
-        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x, e.ec + EC_OVERFLOW);
+        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x, walk_err);
       if ((ez & b) && !(x > 0 && (ez & pb) && (lx & b) && (zlx & b)))
-        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x, e.ec + EC_OVERFLOW);
+        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x, walk_err);
     }
     __syncthreads();
     // anchors inside the brick (islands): y == 0 or a structural neighbour
@@ -510,7 +561,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
       if (!((rs >> x) & 1u)) continue;
       uint32_t run = 1u << x;
       for (int y = x + 1; y < 8 && ((lx >> y) & 1u); y++) run |= 1u << y;
-      uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x, e.ec + EC_OVERFLOW);
+      uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x, walk_err);
       atomicAdd(&s_cnt[r], (uint32_t)__popc(run));
       if (run & abits) atomicOr(&s_flg[r], kFlagAnchor);
       for (int y = x; y < 8; y++)
@@ -551,6 +602,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
     e.bcount[blockIdx.x * kCountWords + (MODE == 0 ? BC_NODES : BC_PANENODES)] = min(nn, e.slots_node);
     if (MODE == 1) e.bcount[blockIdx.x * kCountWords + BC_PANECELLS] = nc;
   }
+  if (__ballot(walk_err != 0u) != 0ull && lane == 0u) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
 }
 
 PX_DEV uint32_t cell_voxel(const EditView& e, uint32_t g, uint32_t li) {
@@ -575,7 +627,8 @@ PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t l
 // Face merges between a cell and its +x / +y / +z neighbours (64 voxel pairs
 // per face, one per lane), plus anchors across faces (islands).
 template <int MODE>
-PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, const uint8_t* s_str) {
+PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, const uint8_t* s_str,
+                       uint32_t& walk_err) {
   const uint32_t lane = threadIdx.x;
   const uint32_t u = lane & 7u, v = lane >> 3;
   uint32_t g = e.grid[ci];
@@ -610,7 +663,7 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, con
     uint32_t pa = __shfl_up(na, 1), pb = __shfl_up(nb, 1), pj = __shfl_up(jb, 1);
     uint32_t qa = __shfl_up(na, 8), qb = __shfl_up(nb, 8), qj = __shfl_up(jb, 8);
     bool dup = (u > 0u && pj && pa == na && pb == nb) || (v > 0u && qj && qa == na && qb == nb);
-    if (join && !dup) uf_union(e.par, na, nb, e.walk_cap, e.ec + EC_OVERFLOW);
+    if (join && !dup) uf_union(e.par, na, nb, e.walk_cap, walk_err);
     if (MODE == 0) {
       uint32_t anc_a = (ca && vb && s_str[vb & 0xffu]) ? 1u : 0u;
       uint32_t anc_b = (cb && va && s_str[va & 0xffu]) ? 1u : 0u;
@@ -626,23 +679,28 @@ template <int MODE>
 __global__ __launch_bounds__(64) void k_face_merge(EditView e) {
   __shared__ uint8_t s_cls[256];
   __shared__ uint8_t s_str[256];
+  if (e.ec[MODE == 0 ? EC_ACTIVE : EC_NIMP] == 0u) return;
   load_tables<MODE>(e.mats, s_cls, s_str);
   __syncthreads();
+  uint32_t walk_err = 0u;
   if (MODE == 0) {
-    for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) merge_cell<0>(e, ci, s_cls, s_str);
+    for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) merge_cell<0>(e, ci, s_cls, s_str, walk_err);
   } else {
     for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
       uint32_t n = e.bcount[b * kCountWords + BC_PANECELLS];
       const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
-      for (uint32_t i = 0; i < n; i++) merge_cell<1>(e, cells[i], s_cls, s_str);
+      for (uint32_t i = 0; i < n; i++) merge_cell<1>(e, cells[i], s_cls, s_str, walk_err);
     }
   }
+  if (__ballot(walk_err != 0u) != 0ull && threadIdx.x == 0u) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
 }
 
 // After the unions: par[x] = root(x) for every node of block ranges `which`.
 // The walk only reads (no path halving), so each par[x] is written by x's
 // own thread alone and ends as the root.
 __global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
+  if (e.ec[which == BC_NODES ? EC_ACTIVE : EC_NIMP] == 0u) return;
+  uint32_t walk_err = 0u;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t n = e.bcount[b * kCountWords + which];
     const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
@@ -659,173 +717,11 @@ __global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
         if (p > r) break;  // never written by a node of this pass
         r = p;
       }
-      if (!ok) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
+      if (!ok) walk_err = kErrWalk;
       __hip_atomic_store(e.par + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-}
-
-// carve, sim.wgsl:529-579, one wave per job cell over every impact that has a
-// job there.  Impacts are the outer (wave-uniform) loop; a voxel is tested
-// exactly (removes(), :229-239) only if its distance passes a conservative
-// bound for that impact: removes() needs d2 < (fr*jit)^2 or d2 < (rr*jit)^2
-// with jit <= 1.2 and rr <= max(1.4 r, 2.5), so d2 >= (1.2 max(1.4 r, 2.5, fr)
-// (1 + 1e-5))^2 can never remove.  The hash behind jit is computed once per
-// voxel, and only for voxels that pass some bound.  Removed pane voxels mark
-// their pane component for the flood.
-__global__ __launch_bounds__(64) void k_carve(EditView e) {
-  __shared__ float s_im[64 * 16];
-  __shared__ float s_bound2[64];
-  const uint32_t lane = threadIdx.x;
-  for (uint32_t i = lane; i < e.n_impacts * 16u; i += 64u) s_im[i] = e.impacts[i];
-  __syncthreads();
-  if (lane < e.n_impacts) {
-    const float* im = s_im + lane * 16u;
-    float b = fmaxf(fmaxf(im[3] * 1.4f, 2.5f), im[4]) * 1.2f * 1.00001f;
-    s_bound2[lane] = b * b;
-  }
-  __syncthreads();
-  uint32_t acc_d = 0u, acc_b = 0u;
-  const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    uint32_t ci = e.job_list[i];
-    uint64_t mask = e.imask[ci];
-    uint32_t g = e.grid[ci];
-    __syncthreads();
-    if (lane == 0) e.imask[ci] = 0ull;
-    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
-    int32_t bx, by, bz;
-    cell_coord(e, ci, bx, by, bz);
-    uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
-    uint32_t p = g - 1u;
-    uint4* wp = reinterpret_cast<uint4*>(e.pool + (size_t)p * 256u) + lane;
-    uint4 w4 = *wp;
-    uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
-    // lane l owns the row y = l & 7, z = l >> 3 of the brick (x = 0..7)
-    const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
-    const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
-    // per-voxel terms of removes() that do not depend on the impact, loaded
-    // before the impact loop so that loop is arithmetic only
-    uint32_t alive = 0u, frag = 0u, soft = 0u;
-    float jit[8], m14[8];
-    for (int x = 0; x < 8; x++) {
-      uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-      jit[x] = 0.0f;
-      m14[x] = 0.0f;
-      if (!v) continue;
-      alive |= 1u << x;
-      const float* E = e.mats + (v & 0xffu) * 20u + 16u;
-      float sx = E[0], fy0 = E[1];
-      if (fy0 > 0.5f) frag |= 1u << x;
-      if (sx > 0.0f) soft |= 1u << x;
-      m14[x] = fminf(1.4f, sx);
-      jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
-    }
-    uint32_t rem = 0u;
-    uint64_t m = mask;
-    while (m) {
-      uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-      m &= m - 1ull;
-      const float* im = s_im + k * 16u;
-      const float b2 = s_bound2[k];
-      const float r = im[3], fr = im[4];
-      float dy = fy - im[1], dz = fz - im[2];
-      for (int x = 0; x < 8; x++) {
-        uint32_t bit = 1u << x;
-        if (!(alive & bit) || (rem & bit)) continue;
-        float dx = ((float)(bx * 8 + x) + 0.5f) - im[0];
-        float d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < b2)) continue;
-        float j = jit[x];
-        bool hit = false;
-        if ((frag & bit) && d2 < fr * fr * j * j) hit = true;
-        else if (soft & bit) {
-          float rr = fmaxf(r * m14[x], fminf(r, 2.5f)) * j;
-          hit = d2 < rr * rr;
-        }
-        if (hit) rem |= bit;
-      }
-    }
-    uint32_t destroyed = 0, bill = 0;
-    for (int x = 0; x < 8; x++) {
-      uint32_t hit_root = PX_NONE;
-      if ((rem >> x) & 1u) {
-        uint32_t sh = (uint32_t)(x & 1) * 16u;
-        uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
-        ww[x >> 1] &= ~(0xffffu << sh);
-        destroyed++;
-        bill += price_dimes(e.mats, v);
-        if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
-          uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
-          if (r < 512u) hit_root = e.par[okey + loff(e, r)];
-          else atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
-        }
-      }
-      // the pane component this voxel belongs to is hit (flooded later)
-      wave_flag_unique(e.flag, hit_root, kFlagHit);
-    }
-    if (rem) *wp = make_uint4(ww[0], ww[1], ww[2], ww[3]);
-    destroyed = wave_sum(destroyed);
-    bill = wave_sum(bill);
-    acc_d += destroyed;
-    acc_b += bill;
-    if (lane == 0 && destroyed) set_dirty(e, ci);
-  }
-  if (lane == 0) {
-    e.part[blockIdx.x * kPartWords + PT_CARVE_D] = acc_d;
-    e.part[blockIdx.x * kPartWords + PT_CARVE_B] = acc_b;
-  }
-}
-
-// The pane flood's closure: every pane voxel whose component was hit by carve.
-__global__ __launch_bounds__(64) void k_flood_remove(EditView e) {
-  const uint32_t lane = threadIdx.x;
-  uint32_t acc_d = 0u, acc_b = 0u;
-  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
-    uint32_t nc = e.bcount[b * kCountWords + BC_PANECELLS];
-    const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
-    for (uint32_t i = 0; i < nc; i++) {
-      uint32_t ci = cells[i];
-      uint32_t g = e.grid[ci];
-      if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
-      int32_t bx, by, bz;
-      cell_coord(e, ci, bx, by, bz);
-      uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
-      uint32_t p = g - 1u;
-      uint4* wp = reinterpret_cast<uint4*>(e.pool + (size_t)p * 256u) + lane;
-      uint4 w4 = *wp;
-      uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
-      uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
-      uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
-      uint32_t destroyed = 0, bill = 0;
-      bool changed = false;
-      for (int j = 0; j < 8; j++) {
-        uint32_t sh = (uint32_t)(j & 1) * 16u;
-        uint32_t v = (ww[j >> 1] >> sh) & 0xffffu;
-        uint32_t r = (ll[j >> 1] >> sh) & 0xffffu;
-        if (v == 0u || r == kLabNone) continue;
-        if (r >= 512u) {
-          atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
-          continue;
-        }
-        if ((e.flag[e.par[okey + loff(e, r)]] & kFlagHit) == 0u) continue;
-        ww[j >> 1] &= ~(0xffffu << sh);
-        changed = true;
-        destroyed++;
-        bill += price_dimes(e.mats, v);
-      }
-      if (changed) *wp = make_uint4(ww[0], ww[1], ww[2], ww[3]);
-      destroyed = wave_sum(destroyed);
-      bill = wave_sum(bill);
-      acc_d += destroyed;
-      acc_b += bill;
-      if (lane == 0 && destroyed) set_dirty(e, ci);
-    }
-  }
-  if (lane == 0) {
-    e.part[blockIdx.x * kPartWords + PT_FLOOD_D] = acc_d;
-    e.part[blockIdx.x * kPartWords + PT_FLOOD_B] = acc_b;
-  }
+  if (__ballot(walk_err != 0u) != 0ull && threadIdx.x == 0u) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
 }
 
 // maint of one cell (sim.wgsl:1219-1274; the density output is shading only)
@@ -865,16 +761,25 @@ PX_DEV uint32_t maint_cell(const EditView& e, uint32_t ci) {
   return 1u;
 }
 
-// Every dirty cell is a job cell (alloc, carve) or a pane cell (flood).
+// Every dirty cell is a job cell (alloc, carve), a pane cell (flood) or a
+// cell that dropped rubble lost voxels in (the extraction's dirty list).  The
+// job cells' impact masks are cleared here, after the effects read them.
 __global__ __launch_bounds__(64) void k_maint(EditView e) {
+  if (e.ec[EC_NIMP] == 0u) return;
   uint32_t dirty = 0u;
   const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) dirty += maint_cell(e, e.job_list[i]);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t ci = e.job_list[i];
+    if (threadIdx.x == 0) e.imask[ci] = 0ull;
+    dirty += maint_cell(e, ci);
+  }
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t nc = e.bcount[b * kCountWords + BC_PANECELLS];
     const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
     for (uint32_t i = 0; i < nc; i++) dirty += maint_cell(e, cells[i]);
   }
+  const uint32_t nd = min(e.ec[EC_R_DIRTY], e.ncell);
+  for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) dirty += maint_cell(e, e.rdirty[i]);
   if (threadIdx.x == 0) e.part[blockIdx.x * kPartWords + PT_DIRTY] = dirty;
 }
 
@@ -885,8 +790,11 @@ template <int PHASE>
 __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
   // PHASE 0: node flags -> root flags; PHASE 1: node counts -> root counts,
   // for roots left unanchored only (anchored components need no count, and
-  // the largest of them, the floor, would take an atomic from every block)
+  // the largest of them, the floor, would take an atomic from every block;
+  // in seeded mode only searched roots are counted)
   const uint32_t lane = threadIdx.x;
+  if (e.ec[EC_ACTIVE] == 0u) return;
+  const uint32_t need = e.mode == PX_ISLANDS_SEEDED ? kFlagSeed : 0u;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t n = e.bcount[b * kCountWords + BC_NODES];
     const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
@@ -901,7 +809,8 @@ __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
           v = e.flag[x];
           pending = v != 0u;
         } else {
-          pending = (__hip_atomic_load(e.flag + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFlagAnchor) == 0u;
+          const uint32_t rf = __hip_atomic_load(e.flag + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          pending = (rf & kFlagAnchor) == 0u && (rf & need) == need;
           v = pending ? e.cnt[x] : 0u;
         }
       }
@@ -929,9 +838,15 @@ __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
   }
 }
 
-// Unsupported roots -> (label, count), appended with one atomic per wavefront.
+// Unsupported roots -> (label, count), appended with one atomic per
+// wavefront; in seeded mode searched roots only.  REEMIT: the second emit
+// after the node budget tripped, keeping labels below the threshold.
+template <int REEMIT>
 __global__ __launch_bounds__(64) void k_emit(EditView e) {
   const uint32_t lane = threadIdx.x;
+  if (e.ec[REEMIT ? EC_REEMIT : EC_ACTIVE] == 0u) return;
+  const uint32_t need = e.mode == PX_ISLANDS_SEEDED ? kFlagSeed : 0u;
+  const uint32_t below = REEMIT ? e.ec[EC_NB_T] : 0xFFFFFFFFu;
   uint32_t chunks = 0u;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t n = e.bcount[b * kCountWords + BC_NODES];
@@ -939,7 +854,8 @@ __global__ __launch_bounds__(64) void k_emit(EditView e) {
     for (uint32_t base = 0; base < n; base += 64u) {
       uint32_t i = base + lane;
       uint32_t x = i < n ? nodes[i] : 0u;
-      bool lab = i < n && e.par[x] == x && (e.flag[x] & kFlagAnchor) == 0u;
+      const uint32_t f = i < n ? e.flag[x] : 0u;
+      bool lab = i < n && e.par[x] == x && (f & kFlagAnchor) == 0u && (f & need) == need && (!REEMIT || x < below);
       uint32_t c = lab ? e.cnt[x] : 0u;
       uint64_t m = __ballot(lab);
       if (!m) continue;
@@ -960,9 +876,62 @@ __global__ __launch_bounds__(64) void k_emit(EditView e) {
   if (lane == 0 && chunks) atomicAdd(&e.ec[EC_CHUNKS], chunks);
 }
 
+// Seeded mode: the reference stops its search once the hash holds more than
+// 7/10 of its slots (sim.wgsl:758) and takes the unexplored rest as
+// supported.  Here the floating searched components are resolved in label
+// order while their voxel total stays within that budget: the smallest
+// label T whose prefix total passes it is found by bisection, labels from T
+// on are marked anchored, and k_emit<1> emits the rest again.
+__global__ __launch_bounds__(1024) void k_node_budget(EditView e) {
+  __shared__ unsigned long long s_red[16];
+  if (e.ec[EC_ACTIVE] == 0u || e.mode != PX_ISLANDS_SEEDED) return;
+  const uint32_t n = min(e.ec[EC_LABELS], e.out_cap);
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  auto prefix = [&](uint64_t t) -> uint64_t {  // voxels of labels below t
+    unsigned long long v = 0ull;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+      if ((uint64_t)e.out[2u * i] < t) v += e.out[2u * i + 1u];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) s_red[wave] = v;
+    __syncthreads();
+    unsigned long long tot = 0ull;
+    for (uint32_t k = 0; k < (blockDim.x >> 6); k++) tot += s_red[k];
+    __syncthreads();
+    return tot;
+  };
+  const uint64_t budget = e.node_budget;
+  if (prefix(1ull << 32) <= budget) return;
+  // smallest t with prefix(t + 1) > budget: labels >= t are dropped
+  uint64_t lo = 0ull, hi = 0xFFFFFFFFull;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (prefix(mid + 1ull) > budget) hi = mid;
+    else lo = mid + 1ull;
+  }
+  uint32_t dropped = 0u;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t key = e.out[2u * i];
+    if ((uint64_t)key >= lo) {
+      e.flag[key] |= kFlagAnchor;
+      dropped++;
+    }
+  }
+  dropped = wave_sum(dropped);
+  if (lane == 0 && dropped) atomicAdd(&e.ec[EC_NB_DROP], dropped);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    e.ec[EC_NB_T] = (uint32_t)lo;
+    e.ec[EC_REEMIT] = 1u;
+    e.ec[EC_LABELS] = 0u;
+    e.ec[EC_CHUNKS] = 0u;
+  }
+}
+
 // counters of the pass into the ctrl words the reference exposes
-__global__ __launch_bounds__(1024) void k_edit_finish(EditView e, uint32_t carved) {
+__global__ __launch_bounds__(1024) void k_edit_finish(EditView e) {
   __shared__ uint32_t s[6][1024];
+  if (e.ec[EC_ACTIVE] == 0u) return;
+  const bool carved = e.ec[EC_NIMP] != 0u;
   uint32_t t = threadIdx.x;
   uint32_t a[6] = {0, 0, 0, 0, 0, 0};
   for (uint32_t b = t; b < kEditBlocks; b += blockDim.x) {
@@ -988,8 +957,10 @@ __global__ __launch_bounds__(1024) void k_edit_finish(EditView e, uint32_t carve
   e.ctrl[PX_C_JOBS] = e.ec[EC_JOBS];
   e.ctrl[PX_C_ALLOC_N] = e.ec[EC_ALLOCS];
   e.ctrl[PX_C_ALLOCS] = 0u;
-  e.ctrl[PX_C_IMPACTS] = 0u;
+  e.ctrl[PX_C_IMPACTS] = 0u;  // postCarve, sim.wgsl:581-584
   e.ctrl[PX_S_SEARCHES] += 1u;
+  if (e.mode == PX_ISLANDS_SEEDED) e.ctrl[PX_C_SEED_N] = e.ec[EC_SEEDS];
+  e.ec[EC_LABELS_OK] = 1u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1249,6 +1220,451 @@ PX_DEV void clear_voxel(const EditView& e, int32_t x, int32_t y, int32_t z) {
   }
 }
 
+// removes() of sim.wgsl:229-239 for voxel value v at (x, y, z), same FP32
+// operation order as the reference
+PX_DEV bool removes_voxel(const float* __restrict__ im, const float* __restrict__ mats, int32_t x, int32_t y,
+                          int32_t z, uint32_t v) {
+  const float* E = mats + (v & 0xffu) * 20u + 16u;
+  const float dx = ((float)x + 0.5f) - im[0], dy = ((float)y + 0.5f) - im[1], dz = ((float)z + 0.5f) - im[2];
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float jit = 0.8f + 0.4f * px_rnd(px_hash3i(x, y, z));
+  if (E[1] > 0.5f && d2 < im[4] * im[4] * jit * jit) return true;
+  if (E[0] <= 0.0f) return false;
+  const float rr = fmaxf(im[3] * fminf(1.4f, E[0]), fminf(im[3], 2.5f)) * jit;
+  return d2 < rr * rr;
+}
+
+// Whether carve removes voxel v at (x, y, z): some impact with a job in its
+// cell (the cell's impact mask, kept until maint) removes it.
+PX_DEV bool carve_removes(const EditView& e, const float* __restrict__ s_im, int32_t x, int32_t y, int32_t z,
+                          uint32_t v) {
+  uint64_t m = e.imask[cell_of_pos(e, x, y, z)];
+  while (m) {
+    const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    m &= m - 1ull;
+    if (removes_voxel(s_im + k * 16u, e.mats, x, y, z, v)) return true;
+  }
+  return false;
+}
+
+// dropRubbleAbove (sim.wgsl:274-284) after carve: the consecutive rubble
+// voxels of pool bricks above (x, y, z) that carve leaves, at most 12.  A
+// voxel carve removes counts as air whether or not its clear has landed yet,
+// so the count is the same before and after carve's stores.
+PX_DEV uint32_t rubble_chain(const EditView& e, const float* __restrict__ s_im, int32_t x, int32_t y, int32_t z) {
+  uint32_t n = 0u;
+  for (int32_t d = 1; d <= 12; d++) {
+    bool pool;
+    const uint32_t v = voxel_at(e, x, y + d, z, &pool);
+    if (v == 0u || (v & 0x80u) == 0u || !pool) break;
+    if (carve_removes(e, s_im, x, y + d, z, v)) break;
+    n++;
+  }
+  return n;
+}
+
+// pushSeed (sim.wgsl:180-184) of the neighbours of a removed voxel of value
+// v, as carveEffects (:511-523) and floodStep (:606-616) push them: not air,
+// not structural, not rubble, not of v's id when v is a pane.  Whether the
+// seed is still a node voxel is checked when the search reads it.
+PX_DEV void push_seeds(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t v) {
+  const bool pane = e.mats[(v & 0xffu) * 20u + 17u] > 1.5f;
+  for (int k = 0; k < 6; k++) {
+    const int32_t qx = x + (k == 0 ? 1 : (k == 1 ? -1 : 0));
+    const int32_t qy = y + (k == 2 ? 1 : (k == 3 ? -1 : 0));
+    const int32_t qz = z + (k == 4 ? 1 : (k == 5 ? -1 : 0));
+    bool pool;
+    const uint32_t w = voxel_at(e, qx, qy, qz, &pool);
+    if (w == 0u || (w & 0x80u) != 0u || e.mats[(w & 0xffu) * 20u + 18u] > 0.5f) continue;
+    if (pane && (w & 0xffu) == (v & 0xffu)) continue;
+    const uint32_t key = vkey(e, qx, qy, qz);
+    atomicOr(&e.seeds[key >> 5], 1u << (key & 31u));
+  }
+}
+
+// carveEffects' debris record of voxel v at (x, y, z) removed by impact im
+// (sim.wgsl:500-508): spawned when rnd(seed) < E.w * keep.
+PX_DEV bool carve_spawns(const EditView& e, const float* __restrict__ im, uint32_t key, uint32_t v) {
+  const float r = im[3];
+  const float keep = fminf(1.0f, 30000.0f / fmaxf(1.0f, 4.19f * r * r * r));
+  return rnd01(key ^ e.fseed) < e.mats[(v & 0xffu) * 20u + 19u] * keep;
+}
+
+PX_DEV void carve_record(const EditView& e, const float* __restrict__ im, int32_t x, int32_t y, int32_t z,
+                         uint32_t key, uint32_t v, uint32_t j, uint32_t accepted, uint32_t cursor) {
+  const uint32_t seed = key ^ e.fseed;
+  const float d[3] = {((float)x + 0.5f) - im[0], ((float)y + 0.5f) - im[1], ((float)z + 0.5f) - im[2]};
+  const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const float den = len < 1e-3f ? 1e-3f : len;
+  const float sp = im[5] * (0.3f + 0.9f * rnd01(seed + 7u));
+  const float off[3] = {0.5f, 0.2f, 0.5f}, m[3] = {0.4f, 0.5f, 0.4f};
+  float vel[3];
+  for (int a = 0; a < 3; a++) {
+    const float dirn = d[a] / den;
+    const float jitter = ((rnd01(seed + 13u + (uint32_t)a) - off[a]) * im[5]) * m[a];
+    vel[a] = ((dirn * 0.7f + im[6 + a] * 0.6f) - im[9 + a] * 0.2f) * sp + jitter;
+  }
+  const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+  spawn_at(e, j, accepted, cursor, pos, vel, v | debris_flags(e.mats, v, seed), 5.0f + 4.0f * rnd01(seed + 11u));
+}
+
+// shardEffects' debris record (sim.wgsl:265-270): when rnd(seed) < E.w * 0.6
+PX_DEV bool shard_spawns(const EditView& e, uint32_t key, uint32_t v) {
+  return rnd01(key ^ e.fseed) < e.mats[(v & 0xffu) * 20u + 19u] * 0.6f;
+}
+
+PX_DEV void shard_record(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t key, uint32_t v, uint32_t j,
+                         uint32_t accepted, uint32_t cursor) {
+  const uint32_t seed = key ^ e.fseed;
+  const float s = 1.0f / e.vs;
+  const float b[3] = {rnd01(seed + 3u) - 0.5f, rnd01(seed + 4u) * 0.4f, rnd01(seed + 5u) - 0.5f};
+  const float m[3] = {1.6f, 1.0f, 1.6f};
+  float vel[3];
+  for (int a = 0; a < 3; a++) vel[a] = (b[a] * m[a]) * s;
+  const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
+  spawn_at(e, j, accepted, cursor, pos, vel, v | debris_flags(e.mats, v, seed), 4.0f + 4.0f * rnd01(seed + 6u));
+}
+
+// dropRubbleAbove's records for the n rubble voxels above (x, y, z), bottom
+// to top from rank j: each is taken (one clear per voxel, listed for maint)
+// and tumbles as DEPOSIT debris.
+PX_DEV void drop_rubble(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t n, uint32_t j,
+                        uint32_t accepted, uint32_t cursor) {
+  for (uint32_t d = 1; d <= n; d++) {
+    const int32_t qy = y + (int32_t)d;
+    bool pool;
+    const uint32_t t = voxel_at(e, x, qy, z, &pool);
+    clear_voxel(e, x, qy, z);
+    const uint32_t seed = vkey(e, x, qy, z) ^ e.fseed;
+    const float pos[3] = {(float)x + 0.5f, (float)qy + 0.5f, (float)z + 0.5f};
+    const float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
+    const float m[3] = {0.4f, 0.0f, 0.4f};
+    float vel[3];
+    for (int k = 0; k < 3; k++) vel[k] = ((r[k] - 0.5f) * m[k]) / e.vs;
+    spawn_at(e, j + d - 1u, accepted, cursor, pos, vel, t | PX_P_DEPOSIT, 6.0f + 2.0f * rnd01(seed + 5u));
+  }
+}
+
+PX_DEV void load_impacts(const EditView& e, float* s_im, uint32_t nimp) {
+  for (uint32_t i = threadIdx.x; i < nimp * 16u; i += blockDim.x) s_im[i] = e.impacts[i];
+  __syncthreads();
+}
+
+// carve with carveEffects (sim.wgsl:485-579), one wave per job cell over every
+// impact of the cell's mask (a cell has one writer: no CAS loop).  A voxel is
+// removed by the lowest impact of the mask that removes it.  Lane l owns the
+// row y = l & 7, z = l >> 3 of the brick.
+// PASS 0 (count): the pane components the removed voxels hit, their seeds
+// (seeded mode) and the number of debris records their effects spawn
+// (dropRubbleAbove's rubble, then the carve debris) into cpair[cell].x.
+// PASS 1 (apply): the same removals are cleared with atomic ANDs (rubble that
+// another cell's voxels drop may share these words), and the records are
+// spawned at their rank in (cell, local index) order, carved voxels first.
+template <int PASS>
+__global__ __launch_bounds__(64) void k_carve(EditView e) {
+  __shared__ float s_im[64 * 16];
+  __shared__ float s_bound2[64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nimp = e.ec[EC_NIMP];
+  if (nimp == 0u) return;
+  load_impacts(e, s_im, nimp);
+  if (lane < nimp) {
+    // removes() needs d2 < (fr jit)^2 or d2 < (rr jit)^2 with jit <= 1.2 and
+    // rr <= max(1.4 r, 2.5): beyond this bound no voxel is tested exactly
+    const float* im = s_im + lane * 16u;
+    float b = fmaxf(fmaxf(im[3] * 1.4f, 2.5f), im[4]) * 1.2f * 1.00001f;
+    s_bound2[lane] = b * b;
+  }
+  __syncthreads();
+  const uint32_t total = PASS == 1 ? e.etot[0] + e.etot[1] : 0u;
+  const uint32_t accepted = PASS == 1 ? spawn_accepted(e, total) : 0u;
+  const uint32_t cursor = PASS == 1 ? e.ctrl[PX_C_PCURSOR] : 0u;
+  uint32_t acc_d = 0u, acc_b = 0u;
+  const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t ci = e.job_list[i];
+    const uint64_t mask = e.imask[ci];
+    const uint32_t g = e.grid[ci];
+    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
+    int32_t bx, by, bz;
+    cell_coord(e, ci, bx, by, bz);
+    const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+    const uint32_t p = g - 1u;
+    uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
+    const uint4 w4 = *reinterpret_cast<const uint4*>(row);
+    const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+    const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
+    const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
+    // per-voxel terms of removes() that do not depend on the impact
+    uint32_t alive = 0u, frag = 0u, soft = 0u;
+    float jit[8], m14[8];
+    for (int x = 0; x < 8; x++) {
+      const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+      jit[x] = 0.0f;
+      m14[x] = 0.0f;
+      if (!v) continue;
+      alive |= 1u << x;
+      const float* E = e.mats + (v & 0xffu) * 20u + 16u;
+      if (E[1] > 0.5f) frag |= 1u << x;
+      if (E[0] > 0.0f) soft |= 1u << x;
+      m14[x] = fminf(1.4f, E[0]);
+      jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
+    }
+    uint32_t rem = 0u;
+    uint64_t rk = 0ull;  // remover of removed voxel x in bits 6x .. 6x + 5
+    uint64_t m = mask;
+    while (m) {
+      const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+      m &= m - 1ull;
+      const float* im = s_im + k * 16u;
+      const float b2 = s_bound2[k];
+      const float r = im[3], fr = im[4];
+      const float dy = fy - im[1], dz = fz - im[2];
+      for (int x = 0; x < 8; x++) {
+        const uint32_t bit = 1u << x;
+        if (!(alive & bit) || (rem & bit)) continue;
+        const float dx = ((float)(bx * 8 + x) + 0.5f) - im[0];
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < b2)) continue;
+        const float j = jit[x];
+        bool hit = false;
+        if ((frag & bit) && d2 < fr * fr * j * j) hit = true;
+        else if (soft & bit) {
+          const float rr = fmaxf(r * m14[x], fminf(r, 2.5f)) * j;
+          hit = d2 < rr * rr;
+        }
+        if (hit) {
+          rem |= bit;
+          rk |= (uint64_t)k << (6 * x);
+        }
+      }
+    }
+    if (__ballot(rem != 0u) == 0ull) continue;
+    // records of this lane's removed voxels
+    uint32_t rub[8];
+    uint32_t nrec = 0u, own = 0u;
+    for (int x = 0; x < 8; x++) {
+      rub[x] = 0u;
+      if (!((rem >> x) & 1u)) continue;
+      const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+      const int32_t px = bx * 8 + x;
+      rub[x] = rubble_chain(e, s_im, px, vy, vz);
+      const uint32_t kx = (uint32_t)(rk >> (6 * x)) & 63u;
+      if (carve_spawns(e, s_im + kx * 16u, vkey(e, px, vy, vz), v)) own |= 1u << x;
+      nrec += rub[x] + ((own >> x) & 1u);
+    }
+    if (PASS == 0) {
+      for (int x = 0; x < 8; x++) {
+        uint32_t hit_root = PX_NONE;
+        if ((rem >> x) & 1u) {
+          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+          if (e.mode == PX_ISLANDS_SEEDED) push_seeds(e, bx * 8 + x, vy, vz, v);
+          if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
+            const uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
+            if (r < 512u) hit_root = e.par[okey + loff(e, r)];
+            else atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
+          }
+        }
+        // the pane component this voxel belongs to is hit (flooded later)
+        wave_flag_unique(e.flag, hit_root, kFlagHit);
+      }
+      const uint32_t t = wave_sum(nrec);
+      if (lane == 0u) e.cpair[ci].x = t;
+      continue;
+    }
+    // PASS 1
+    uint32_t wmask[4] = {0u, 0u, 0u, 0u};
+    uint32_t destroyed = 0u, bill = 0u;
+    for (int x = 0; x < 8; x++)
+      if ((rem >> x) & 1u) {
+        const uint32_t sh = (uint32_t)(x & 1) * 16u;
+        wmask[x >> 1] |= 0xffffu << sh;
+        destroyed++;
+        bill += price_dimes(e.mats, (ww[x >> 1] >> sh) & 0xffffu);
+      }
+    for (int q = 0; q < 4; q++)
+      if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
+    uint32_t j = e.coff[ci].x + wave_scan(nrec) - nrec;
+    for (int x = 0; x < 8; x++) {
+      if (!((rem >> x) & 1u)) continue;
+      const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+      const int32_t px = bx * 8 + x;
+      drop_rubble(e, px, vy, vz, rub[x], j, accepted, cursor);
+      j += rub[x];
+      if ((own >> x) & 1u) {
+        const uint32_t kx = (uint32_t)(rk >> (6 * x)) & 63u;
+        carve_record(e, s_im + kx * 16u, px, vy, vz, vkey(e, px, vy, vz), v, j, accepted, cursor);
+        j++;
+      }
+    }
+    destroyed = wave_sum(destroyed);
+    bill = wave_sum(bill);
+    acc_d += destroyed;
+    acc_b += bill;
+    if (lane == 0u && destroyed) set_dirty(e, ci);
+  }
+  if (PASS == 1 && lane == 0u) {
+    e.part[blockIdx.x * kPartWords + PT_CARVE_D] = acc_d;
+    e.part[blockIdx.x * kPartWords + PT_CARVE_B] = acc_b;
+  }
+}
+
+// The pane flood's closure with shardEffects (sim.wgsl:250-271, 589-618):
+// every pane voxel whose component carve hit and that carve did not remove.
+// PASS 0 counts the records (rubble above, shard debris) into cpair[cell].y
+// and pushes the seeds; PASS 1 clears the voxels and spawns the records after
+// all of carve's, in (cell, local index) order.
+template <int PASS>
+__global__ __launch_bounds__(64) void k_flood(EditView e) {
+  __shared__ float s_im[64 * 16];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nimp = e.ec[EC_NIMP];
+  if (nimp == 0u) return;
+  load_impacts(e, s_im, nimp);
+  const uint32_t total = PASS == 1 ? e.etot[0] + e.etot[1] : 0u;
+  const uint32_t accepted = PASS == 1 ? spawn_accepted(e, total) : 0u;
+  const uint32_t cursor = PASS == 1 ? e.ctrl[PX_C_PCURSOR] : 0u;
+  const uint32_t base = PASS == 1 ? e.etot[0] : 0u;
+  uint32_t acc_d = 0u, acc_b = 0u;
+  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
+    const uint32_t nc = e.bcount[b * kCountWords + BC_PANECELLS];
+    const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
+    for (uint32_t i = 0; i < nc; i++) {
+      const uint32_t ci = cells[i];
+      const uint32_t g = e.grid[ci];
+      if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
+      int32_t bx, by, bz;
+      cell_coord(e, ci, bx, by, bz);
+      const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+      const uint32_t p = g - 1u;
+      uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
+      const uint4 w4 = *reinterpret_cast<const uint4*>(row);
+      const uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
+      const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+      const uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
+      const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
+      uint32_t fl = 0u;
+      for (int x = 0; x < 8; x++) {
+        const uint32_t sh = (uint32_t)(x & 1) * 16u;
+        const uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
+        const uint32_t r = (ll[x >> 1] >> sh) & 0xffffu;
+        if (v == 0u || r == kLabNone) continue;
+        if (r >= 512u) {
+          atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
+          continue;
+        }
+        if ((e.flag[e.par[okey + loff(e, r)]] & kFlagHit) == 0u) continue;
+        // carve's own voxels (still present before carve's stores) are not flooded
+        if (PASS == 0 && carve_removes(e, s_im, bx * 8 + x, vy, vz, v)) continue;
+        fl |= 1u << x;
+      }
+      if (__ballot(fl != 0u) == 0ull) continue;
+      uint32_t rub[8];
+      uint32_t nrec = 0u, own = 0u;
+      for (int x = 0; x < 8; x++) {
+        rub[x] = 0u;
+        if (!((fl >> x) & 1u)) continue;
+        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        const int32_t px = bx * 8 + x;
+        rub[x] = rubble_chain(e, s_im, px, vy, vz);
+        if (shard_spawns(e, vkey(e, px, vy, vz), v)) own |= 1u << x;
+        nrec += rub[x] + ((own >> x) & 1u);
+        if (PASS == 0 && e.mode == PX_ISLANDS_SEEDED) push_seeds(e, px, vy, vz, v);
+      }
+      if (PASS == 0) {
+        const uint32_t t = wave_sum(nrec);
+        if (lane == 0u) e.cpair[ci].y = t;
+        continue;
+      }
+      uint32_t wmask[4] = {0u, 0u, 0u, 0u};
+      uint32_t destroyed = 0u, bill = 0u;
+      for (int x = 0; x < 8; x++)
+        if ((fl >> x) & 1u) {
+          const uint32_t sh = (uint32_t)(x & 1) * 16u;
+          wmask[x >> 1] |= 0xffffu << sh;
+          destroyed++;
+          bill += price_dimes(e.mats, (ww[x >> 1] >> sh) & 0xffffu);
+        }
+      for (int q = 0; q < 4; q++)
+        if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
+      uint32_t j = base + e.coff[ci].y + wave_scan(nrec) - nrec;
+      for (int x = 0; x < 8; x++) {
+        if (!((fl >> x) & 1u)) continue;
+        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        const int32_t px = bx * 8 + x;
+        drop_rubble(e, px, vy, vz, rub[x], j, accepted, cursor);
+        j += rub[x];
+        if ((own >> x) & 1u) {
+          shard_record(e, px, vy, vz, vkey(e, px, vy, vz), v, j, accepted, cursor);
+          j++;
+        }
+      }
+      destroyed = wave_sum(destroyed);
+      bill = wave_sum(bill);
+      acc_d += destroyed;
+      acc_b += bill;
+      if (lane == 0u && destroyed) set_dirty(e, ci);
+    }
+  }
+  if (PASS == 1 && lane == 0u) {
+    e.part[blockIdx.x * kPartWords + PT_FLOOD_D] = acc_d;
+    e.part[blockIdx.x * kPartWords + PT_FLOOD_B] = acc_b;
+  }
+}
+
+// cpair of every cell to (0, 0) before the count passes (a pass with impacts)
+__global__ void k_edit_zero_cells(EditView e) {
+  if (e.ec[EC_NIMP] == 0u) return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < e.ncell; i += gridDim.x * blockDim.x)
+    e.cpair[i] = make_uint2(0u, 0u);
+}
+
+// spawn()'s counters after the effects: every call counts, the accepted ones
+// advance the cursor (sim.wgsl:166-170)
+__global__ void k_effects_finish(EditView e) {
+  if (threadIdx.x != 0 || e.ec[EC_NIMP] == 0u) return;
+  const uint32_t total = e.etot[0] + e.etot[1];
+  const uint32_t acc = spawn_accepted(e, total);
+  e.ctrl[PX_C_SPAWNED] += total;
+  e.ctrl[PX_C_PCURSOR] += acc;
+}
+
+// Seeded mode (searchSeed, sim.wgsl:722-731): every seed bit whose voxel is
+// a node voxel marks its component searched; the bits are cleared as they
+// are read.  Seeds are the neighbours of carved and flooded voxels and the
+// voxels under landed chunks; they have no capacity limit here (a bit per
+// voxel), so nothing is dropped (S_SEED_DROP stays 0).
+__global__ __launch_bounds__(256) void k_seed_mark(EditView e) {
+  if (e.ec[EC_ACTIVE] == 0u || e.mode != PX_ISLANDS_SEEDED) return;
+  uint32_t found = 0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < e.seed_words; i += gridDim.x * blockDim.x) {
+    uint32_t w = e.seeds[i];
+    if (w == 0u) continue;
+    e.seeds[i] = 0u;
+    while (w) {
+      const uint32_t b = (uint32_t)__ffs(w) - 1u;
+      w &= w - 1u;
+      int32_t x, y, z;
+      key_pos(e, i * 32u + b, x, y, z);
+      if (z >= e.dims[2]) continue;
+      const uint32_t ci = cell_of_pos(e, x, y, z);
+      const uint32_t g = e.grid[ci];
+      bool pool;
+      const uint32_t v = voxel_at(e, x, y, z, &pool);
+      if (v == 0u || (v & 0x80u) != 0u || e.mats[(v & 0xffu) * 20u + 18u] > 0.5f) continue;
+      const uint32_t okey = vkey(e, x & ~7, y & ~7, z & ~7);
+      const uint32_t node = node_of(e, okey, g, (uint32_t)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6)));
+      if (node == PX_NONE) continue;
+      const uint32_t r = e.par[node];
+      found++;
+      if ((__hip_atomic_load(e.flag + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFlagSeed) == 0u)
+        atomicOr(&e.flag[r], kFlagSeed);
+    }
+  }
+  found = wave_sum(found);
+  if (__lane_id() == 0u && found) atomicAdd(&e.ec[EC_SEEDS], found);
+}
+
 // R1: chunk ids for the big unsupported components in label order (one
 // workgroup).  Labels over the component budget stay (kFlagKeep, S_BUDGET);
 // of the big ones (>= 48 voxels) the smallest labels get the free chunk ids,
@@ -1257,6 +1673,7 @@ __global__ __launch_bounds__(1024) void k_resolve_chunks(EditView e) {
   __shared__ uint32_t s_red[3][16];
   __shared__ uint32_t s_roots[512];
   __shared__ uint32_t s_nroots;
+  if (e.ec[EC_LABELS_OK] == 0u) return;  // the last island pass did not run
   const uint32_t n = min(e.ec[EC_LABELS], e.out_cap);
   const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
   auto block_sum = [&](uint32_t v, int slot) -> uint32_t {
@@ -1335,16 +1752,25 @@ __global__ __launch_bounds__(1024) void k_resolve_chunks(EditView e) {
   }
 }
 
+// A component stays in the world when it is supported, over the component
+// budget, or (seeded mode) was not searched: the reference only resolves
+// the nodes its search visited (sim.wgsl:841-923).
+PX_DEV bool stays(const EditView& e, uint32_t f) {
+  if ((f & (kFlagAnchor | kFlagKeep)) != 0u) return true;
+  return e.mode == PX_ISLANDS_SEEDED && (f & kFlagSeed) == 0u;
+}
+
 // R2: cells holding voxels of extracted components; uniform ones need a pool
 // brick first (resolveExpand)
 __global__ __launch_bounds__(64) void k_resolve_mark(EditView e) {
+  if (e.ec[EC_LABELS_OK] == 0u) return;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     const uint32_t n = e.bcount[b * kCountWords + BC_NODES];
     const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
       const uint32_t x = nodes[i];
       const uint32_t f = e.flag[e.par[x]];
-      if ((f & (kFlagAnchor | kFlagKeep)) != 0u) continue;
+      if (stays(e, f)) continue;
       int32_t px, py, pz;
       key_pos(e, x, px, py, pz);
       const uint32_t ci = cell_of_pos(e, px, py, pz);
@@ -1355,6 +1781,7 @@ __global__ __launch_bounds__(64) void k_resolve_mark(EditView e) {
 
 // R3: extract list and alloc list in cell order
 __global__ __launch_bounds__(256) void k_resolve_lists(EditView e, const uint32_t* tot) {
+  if (e.ec[EC_LABELS_OK] == 0u) return;
   for (uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x; ci < e.ncell; ci += gridDim.x * blockDim.x) {
     const uint2 f = e.cpair[ci];
     if (f.x == 0u) continue;
@@ -1383,7 +1810,7 @@ PX_DEV uint32_t extract_class(const EditView& e, uint32_t g, bool was_uniform, u
     node = okey + loff(e, r);
   }
   const uint32_t f = e.flag[e.par[node]];
-  if ((f & (kFlagAnchor | kFlagKeep)) != 0u) return 0u;
+  if (stays(e, f)) return 0u;
   const uint32_t cid = (f >> 8) & 0x1ffu;
   if (cid != 0u && (f & kFlagCrumb) == 0u) return cid;
   return 0xFFFFFFFFu;
@@ -1542,6 +1969,8 @@ __global__ __launch_bounds__(256) void k_arena_tally(EditView e, const uint32_t*
 
 __global__ void k_resolve_begin(EditView e) {
   if (threadIdx.x != 0) return;
+  e.ec[EC_R_CELLS] = 0u;
+  e.ec[EC_ALLOCS] = 0u;
   e.ec[EC_R_DIRTY] = 0u;
   e.ec[EC_R_BILL] = 0u;
   e.ec[EC_R_DESTROYED] = 0u;
@@ -1550,6 +1979,7 @@ __global__ void k_resolve_begin(EditView e) {
 
 __global__ void k_resolve_finish(EditView e, const uint32_t* arena_tot, const uint32_t* spawn_tot) {
   if (threadIdx.x != 0) return;
+  e.ec[EC_LABELS_OK] = 0u;  // the labels are consumed
   const uint32_t total = spawn_tot[1];
   const uint32_t acc = spawn_accepted(e, total);
   e.ctrl[PX_C_SPAWNED] += total;
@@ -1712,7 +2142,7 @@ __global__ __launch_bounds__(256) void k_cand_impacts(EditView e, const uint32_t
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < na; j += gridDim.x * blockDim.x) {
     if (e.cpair[j].x == 0u) continue;
     const uint32_t k = k0 + e.coff[j].x;
-    if (k >= e.max_impacts) continue;
+    if (k >= e.max_imp) continue;
     const uint32_t vv = e.arena[2u * j + 1u];
     const uint32_t* b = e.chunks + (size_t)(vv >> 16) * 16u;
     int32_t px, py, pz;
@@ -1726,7 +2156,7 @@ __global__ __launch_bounds__(256) void k_cand_impacts(EditView e, const uint32_t
       if (qy >= 0 && voxel_at(e, px, qy, pz, &pool) != 0u) break;
     }
     const float speed = u2f(b[1]) * e.vs;
-    float* im = e.dev_impacts + (size_t)k * 16u;
+    float* im = e.impacts + (size_t)k * 16u;
     im[0] = (float)px + 0.5f;
     im[1] = (float)qy + 0.5f;
     im[2] = (float)pz + 0.5f;
@@ -1814,6 +2244,16 @@ __global__ __launch_bounds__(256) void k_land_emit(EditView e, const uint32_t* t
       e.wlist[2u * k] = vkey(e, px, py, pz);
       e.wlist[2u * k + 1u] = v;
     }
+    // whatever it landed on is re-checked for support (pushSeed, :1068-1072)
+    if (e.mode == PX_ISLANDS_SEEDED && (px_hashu(key) & 15u) == 0u) {
+      bool pool;
+      const uint32_t w = voxel_at(e, px, py - 1, pz, &pool);
+      if (w != 0u && !(e.mats[(w & 0xffu) * 20u + 18u] > 0.5f)) {
+        const uint32_t b = vkey(e, px, py - 1, pz);
+        atomicOr(&e.seeds[b >> 5], 1u << (b & 31u));
+        atomicAdd(&e.ctrl[PX_C_PENDING], 1u);
+      }
+    }
   }
   bill = wave_sum(bill);
   destroyed = wave_sum(destroyed);
@@ -1871,7 +2311,13 @@ EditView make_view(px_ctx* c) {
   e.out_cap = c->label_cap;
   e.ec = c->d_edit_ctr;
   e.impacts = c->d_impacts;
-  e.n_impacts = c->n_impacts;
+  e.max_imp = std::min<uint32_t>(c->lim.max_impacts, 64u);
+  e.seeds = c->d_seeds;
+  e.seed_words = (uint32_t)((c->vox_total + 31u) / 32u);
+  e.mode = c->lim.island_mode;
+  // hash slots the reference's search may fill before it stops (sim.wgsl:758)
+  e.node_budget = (uint32_t)(((uint64_t)c->lim.hash_size * 7u) / 10u);
+  e.etot = c->d_rtot + 16;
   e.wlist = c->d_write_list;
   e.max_writes = c->lim.max_writes;
   e.max_alloc = c->lim.max_alloc;
@@ -1903,7 +2349,6 @@ EditView make_view(px_ctx* c) {
     if (fy > 0.0f) f = fy >= 4294967040.0f ? 0xFFFFFFFFu : (uint32_t)fy;
     e.fseed = f * 0x9E3779B9u;
   }
-  e.dev_impacts = c->d_dev_impacts;
   return e;
 }
 
@@ -1913,6 +2358,7 @@ int px_edit_alloc(px_ctx* c) {
   size_t words = (c->ncell + 31u) / 32u;
   size_t slots_cell = (c->ncell + kEditBlocks - 1u) / kEditBlocks;
   PX_HIP_CHECK(c, hipMalloc(&c->d_impacts, 64 * 16 * sizeof(float)));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_impact_stage, 64 * 16 * sizeof(float)));
   PX_HIP_CHECK(c, hipMalloc(&c->d_cell_imask, (size_t)c->ncell * 8));
   PX_HIP_CHECK(c, hipMalloc(&c->d_alloc_flags, words * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_dirty_flags, words * 4));
@@ -1932,16 +2378,16 @@ int px_edit_alloc(px_ctx* c) {
 }
 
 void px_edit_free(px_ctx* c) {
-  void* bufs[] = {c->d_impacts, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags, c->d_job_list, c->d_alloc_list,
-                  c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par, c->d_ncnt, c->d_nflag, c->d_lab,
-                  c->d_nodes, c->d_out_labels};
+  void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags,
+                  c->d_job_list, c->d_alloc_list, c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par,
+                  c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds};
   for (void* p : bufs)
     if (p) hipFree(p);
-  c->d_impacts = nullptr;
+  c->d_impacts = c->d_impact_stage = nullptr;
   c->d_cell_imask = nullptr;
   c->d_par = c->d_ncnt = c->d_nflag = nullptr;
   c->d_lab = nullptr;
-  c->d_nodes = c->d_out_labels = nullptr;
+  c->d_nodes = c->d_out_labels = c->d_seeds = nullptr;
 }
 
 // the labelling arrays are sized by the voxel count and allocated on first use
@@ -1962,58 +2408,80 @@ static int ensure_ccl(px_ctx* c) {
   return PX_OK;
 }
 
-int px_launch_island_pass(px_ctx* c) {
-  int rc = ensure_ccl(c);
-  if (rc) return rc;
+// the seed bits (one per voxel key), cleared once; the search clears what it reads
+int px_seeds_alloc(px_ctx* c) {
+  if (c->d_seeds || c->lim.island_mode != PX_ISLANDS_SEEDED) return PX_OK;
+  const size_t words = (c->vox_total + 31u) / 32u;
+  PX_HIP_CHECK(c, hipMalloc(&c->d_seeds, words * 4));
+  hipLaunchKernelGGL(k_zero32, dim3(1024), dim3(256), 0, c->stream, c->d_seeds, words);
+  PX_HIP_CHECK(c, hipGetLastError());
+  return PX_OK;
+}
+
+int px_clear_seeds(px_ctx* c) {
+  if (!c->d_seeds) return PX_OK;
+  hipLaunchKernelGGL(k_zero32, dim3(1024), dim3(256), 0, c->stream, c->d_seeds, (c->vox_total + 31u) / 32u);
+  PX_HIP_CHECK(c, hipGetLastError());
+  return PX_OK;
+}
+
+int px_launch_push_impacts(px_ctx* c, uint32_t n) {
+  EditView e = make_view(c);
+  hipLaunchKernelGGL(k_push_impacts, dim3(1), dim3(1024), 0, c->stream, e, (const float*)c->d_impact_stage, n);
+  hipLaunchKernelGGL(k_push_impacts_commit, dim3(1), dim3(64), 0, c->stream, e, n);
+  PX_HIP_CHECK(c, hipGetLastError());
+  return PX_OK;
+}
+
+// One island pass, every launch recorded whatever the device holds: the
+// kernels read the impact count and the run decision k_edit_begin writes
+// (force = run the labelling even without impacts or seeds), so the host
+// waits for nothing and a frame of the chain can be captured in a graph.
+int px_launch_island_pass(px_ctx* c, bool force) {
+  if (int rc = ensure_ccl(c)) return rc;
+  if (int rc = px_chunks_alloc(c)) return rc;  // effects scan buffers, dirty list, particle ring
+  if (int rc = px_seeds_alloc(c)) return rc;
   PxProfScope prof(c, PX_PROF_EDIT);
   EditView e = make_view(c);
   hipStream_t s = c->stream;
   const dim3 G(kEditBlocks), W(64);
-  hipLaunchKernelGGL(k_edit_reset, dim3(1), dim3(64), 0, s, c->d_edit_ctr);
-  const uint32_t carved = c->n_impacts ? 1u : 0u;
-  if (c->n_impacts) {
-    // blocks per impact row from the largest impact AABB (host copy of the impacts)
-    uint32_t maxcells = 1;
-    for (uint32_t k = 0; k < c->n_impacts; k++) {
-      const float* im = &c->h_impacts[k * 16];
-      float R = std::ceil(std::max(im[3], im[4]) * 1.2f) + 1.0f;
-      uint32_t cells = 1;
-      for (int a = 0; a < 3; a++) {
-        int32_t lo = (int32_t)std::floor((im[a] - R) / 8.0f), hi = (int32_t)std::floor((im[a] + R) / 8.0f);
-        lo = std::min(std::max(lo, 0), (int32_t)c->bd[a] - 1);
-        hi = std::min(std::max(hi, 0), (int32_t)c->bd[a] - 1);
-        cells *= (uint32_t)(hi - lo + 1);
-      }
-      maxcells = std::max(maxcells, cells);
-    }
-    uint32_t bx = std::min<uint32_t>((maxcells + 255u) / 256u, 256u);
-    hipLaunchKernelGGL(k_impact_jobs, dim3(bx, c->n_impacts), dim3(256), 0, s, e);
-    hipLaunchKernelGGL(k_alloc, G, W, 0, s, e);
-    hipLaunchKernelGGL(k_alloc_commit, dim3(1), dim3(64), 0, s, e);
-    // pane components on the world as it is after alloc and before carve
-    hipLaunchKernelGGL(k_local_ccl<1>, G, W, 0, s, e);
-    hipLaunchKernelGGL(k_face_merge<1>, G, W, 0, s, e);
-    hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_PANENODES);
-    hipLaunchKernelGGL(k_carve, G, W, 0, s, e);
-    hipLaunchKernelGGL(k_flood_remove, G, W, 0, s, e);
-    hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
-    c->masks_dirty = c->density_dirty = true;
-  }
+  hipLaunchKernelGGL(k_edit_begin, dim3(1), dim3(64), 0, s, e, force ? 1u : 0u);
+  hipLaunchKernelGGL(k_impact_jobs, dim3(kJobBlocks, 64), dim3(256), 0, s, e);
+  hipLaunchKernelGGL(k_alloc, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_alloc_commit, dim3(1), dim3(64), 0, s, e);
+  // pane components on the world as it is after alloc and before carve
+  hipLaunchKernelGGL(k_local_ccl<1>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_face_merge<1>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_PANENODES);
+  // carve + flood: count the effects, rank them in cell order, apply
+  hipLaunchKernelGGL(k_edit_zero_cells, dim3(1024), dim3(256), 0, s, e);
+  hipLaunchKernelGGL(k_carve<0>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_flood<0>, G, W, 0, s, e);
+  scan2_launch(s, e.cpair, e.coff, e.tiles, c->scan_cap, nullptr, c->ncell, nullptr, nullptr, e.etot);
+  hipLaunchKernelGGL(k_carve<1>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_flood<1>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_effects_finish, dim3(1), dim3(64), 0, s, e);
+  hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
+  // labelling
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
+  if (c->lim.island_mode == PX_ISLANDS_SEEDED)
+    hipLaunchKernelGGL(k_seed_mark, dim3(2048), dim3(256), 0, s, e);
   hipLaunchKernelGGL(k_aggregate<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_aggregate<1>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_emit, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_edit_finish, dim3(1), dim3(1024), 0, s, e, carved);
+  hipLaunchKernelGGL(k_emit<0>, G, W, 0, s, e);
+  if (c->lim.island_mode == PX_ISLANDS_SEEDED) {
+    hipLaunchKernelGGL(k_node_budget, dim3(1), dim3(1024), 0, s, e);
+    hipLaunchKernelGGL(k_emit<1>, G, W, 0, s, e);
+  }
+  hipLaunchKernelGGL(k_edit_finish, dim3(1), dim3(1024), 0, s, e);
   PX_HIP_CHECK(c, hipGetLastError());
-  c->n_impacts = 0;
   c->labels_valid = true;
+  c->masks_dirty = c->density_dirty = true;
   return px_refresh_masks(c);
 }
 
-// The error bits of the last island pass as a status (kErrNodes alone is a
-// capacity matter, reported by the label readback).
 int px_edit_errors(px_ctx* c, const char* what, uint32_t bits) {
   if (bits & kErrWalk)
     return px_fail(c, PX_ERR_STATE, what,
@@ -2148,9 +2616,8 @@ int px_chunks_alloc(px_ctx* c) {
   PX_HIP_CHECK(c, hipMalloc(&c->d_tiles, tiles * 8));
   PX_HIP_CHECK(c, hipMalloc(&c->d_rlist, (size_t)c->ncell * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_rdirty, (size_t)c->ncell * 4));
-  PX_HIP_CHECK(c, hipMalloc(&c->d_rtot, 64));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_rtot, 128));
   PX_HIP_CHECK(c, hipMalloc(&c->d_best, nch * 4));
-  PX_HIP_CHECK(c, hipMalloc(&c->d_dev_impacts, (size_t)std::max<uint32_t>(c->lim.max_impacts, 1u) * 64));
   PX_HIP_CHECK(c, hipMemsetAsync(c->d_chunks, 0, nch * 16 * 4, c->stream));
   PX_HIP_CHECK(c, hipMemsetAsync(c->d_arena, 0, na * 8, c->stream));
   PX_HIP_CHECK(c, hipMemsetAsync(c->d_best, 0xFF, nch * 4, c->stream));
@@ -2161,12 +2628,11 @@ int px_chunks_alloc(px_ctx* c) {
 
 void px_chunks_free(px_ctx* c) {
   void* bufs[] = {c->d_chunks, c->d_arena, c->d_cpair, c->d_coff, c->d_tiles, c->d_rlist,
-                  c->d_rdirty, c->d_rtot, c->d_best, c->d_dev_impacts};
+                  c->d_rdirty, c->d_rtot, c->d_best};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   c->d_chunks = c->d_arena = c->d_rlist = c->d_rdirty = c->d_rtot = c->d_best = nullptr;
   c->d_cpair = c->d_coff = c->d_tiles = nullptr;
-  c->d_dev_impacts = nullptr;
 }
 
 // exclusive (x, y) offsets of e.cpair[0..n) into e.coff, totals into tot[0..1]
@@ -2212,6 +2678,7 @@ int px_launch_resolve(px_ctx* c) {
 
 int px_launch_chunk_frame(px_ctx* c) {
   if (int rc = px_chunks_alloc(c)) return rc;
+  if (int rc = px_seeds_alloc(c)) return rc;  // landings push seeds in seeded mode
   PxProfScope prof(c, PX_PROF_CHUNKS);
   EditView e = make_view(c);
   hipStream_t s = c->stream;
@@ -2234,31 +2701,6 @@ int px_launch_chunk_frame(px_ctx* c) {
   hipLaunchKernelGGL(k_land_emit, GA, WA, 0, s, e, t);
   hipLaunchKernelGGL(k_land_finish, dim3(1), dim3(64), 0, s, e, t);
   PX_HIP_CHECK(c, hipGetLastError());
-  c->dev_impacts_pending = true;
   return PX_OK;
 }
 
-// Impacts pushed on the device by falling chunks join the host-side queue
-// of the next island pass (a short blocking read).
-int px_merge_device_impacts(px_ctx* c) {
-  if (!c->dev_impacts_pending || !c->d_dev_impacts) return PX_OK;
-  c->dev_impacts_pending = false;
-  uint32_t n = 0;
-  PX_HIP_CHECK(c, hipMemcpyAsync(&n, c->d_ctrl + PX_C_IMPACTS, 4, hipMemcpyDeviceToHost, c->stream));
-  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  const uint32_t maxi = std::min<uint32_t>(c->lim.max_impacts, 64u);
-  n = std::min(n, maxi);
-  const uint32_t take = std::min<uint32_t>(n, maxi - std::min(maxi, c->n_impacts));
-  if (take) {
-    PX_HIP_CHECK(c, hipMemcpyAsync(c->h_impacts + (size_t)c->n_impacts * 16, c->d_dev_impacts, (size_t)take * 64,
-                                   hipMemcpyDeviceToHost, c->stream));
-    PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-    PX_HIP_CHECK(c, hipMemcpyAsync(c->d_impacts + (size_t)c->n_impacts * 16, c->h_impacts + (size_t)c->n_impacts * 16,
-                                   (size_t)take * 64, hipMemcpyHostToDevice, c->stream));
-    c->n_impacts += take;
-  }
-  const uint32_t zero = 0u;
-  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_ctrl + PX_C_IMPACTS, &zero, 4, hipMemcpyHostToDevice, c->stream));
-  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
-  return PX_OK;
-}

@@ -90,8 +90,9 @@ struct px_ctx {
   unsigned char batch_key[160] = {0};
   uint32_t* d_counters = nullptr;  // small scratch counters
   // edit chain
-  float* d_impacts = nullptr;
-  uint32_t n_impacts = 0;
+  float* d_impacts = nullptr;       // the impact queue: 64 records of 16 floats, C_IMPACTS of them used
+  float* d_impact_stage = nullptr;  // host impacts on their way into the queue
+  uint32_t* d_seeds = nullptr;      // seeded search: one bit per voxel key
   uint64_t* d_cell_imask = nullptr;  // per cell: bit k = impact k has a job here
   uint32_t* d_alloc_flags = nullptr; // per cell bitset: alloc requested
   uint32_t* d_dirty_flags = nullptr; // per cell bitset: dirty
@@ -100,7 +101,6 @@ struct px_ctx {
   uint32_t* d_part = nullptr;        // per-block partial sums of the edit pass
   uint32_t* d_bcount = nullptr;      // per-block list lengths of the edit pass
   uint32_t* d_pane_cells = nullptr;  // pool cells holding pane voxels
-  float h_impacts[64 * 16] = {0};     // host copy of the queued impacts
   bool labels_valid = false;
   uint32_t* d_edit_ctr = nullptr;    // edit-pass counters
   // component labelling
@@ -131,8 +131,6 @@ struct px_ctx {
   uint32_t* d_rdirty = nullptr;       // cells the extraction dirtied
   uint32_t* d_rtot = nullptr;         // scan totals
   uint32_t* d_best = nullptr;         // per chunk: smallest candidate index of a selection round
-  float* d_dev_impacts = nullptr;     // impacts pushed by falling chunks (MAX_IMPACTS x 16)
-  bool dev_impacts_pending = false;   // a chunk frame ran since the last island pass
   size_t scan_cap = 0;                // elements d_cpair / d_coff hold
   // debris particles
   uint32_t max_particles = 0;
@@ -222,7 +220,11 @@ int px_launch_masks(px_ctx* c);
 // edit chain (px_edit.hip)
 int px_edit_alloc(px_ctx* c);
 void px_edit_free(px_ctx* c);
-int px_launch_island_pass(px_ctx* c);
+// force: label even without impacts or pending seeds (px_island_pass)
+int px_launch_island_pass(px_ctx* c, bool force);
+int px_launch_push_impacts(px_ctx* c, uint32_t n);
+int px_seeds_alloc(px_ctx* c);
+int px_clear_seeds(px_ctx* c);
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);
 // Error bits of the edit counters as a status; the blocking form reads them
 // from the device after the stream drains.
@@ -268,7 +270,6 @@ int px_chunks_alloc(px_ctx* c);
 void px_chunks_free(px_ctx* c);
 int px_launch_resolve(px_ctx* c);
 int px_launch_chunk_frame(px_ctx* c);
-int px_merge_device_impacts(px_ctx* c);
 // debris particles (px_particles.hip)
 void px_particles_free(px_ctx* c);
 int px_launch_frame_begin(px_ctx* c);

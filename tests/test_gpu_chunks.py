@@ -172,18 +172,20 @@ def test_floor_scene_resolve(floor_scene):
         sim.push_impacts(impacts)
         sim.encode(N.STAGE_FRAME_BEGIN | N.STAGE_EDIT | N.STAGE_RESOLVE)
         P.frame_begin()
-        ovox, okind, okeys, ocnt, on, stats = oracle.island_pass(s.dims, s.dense(), kinds_of(s.grid), s.mats,
-                                                                 impacts)
+        r = oracle.edit_pass(s.dims, s.dense(), kinds_of(s.grid), s.mats, impacts, vs=VS, frame=5.0)
+        P.spawn(r.spawns)  # carve and shard debris, rubble drops
+        stats = r.stats
         ch = oracle.Chunks(s.dims, s.mats)
-        vox, kind, sp = ch.resolve(ovox, okind, okeys, ocnt, VS, 5.0)
+        vox, kind, sp = ch.resolve(r.vox, r.kind, r.keys, r.counts, VS, 5.0)
         P.spawn(sp)
         ctrl = _compare(gw, sim, s.dims, ch, vox, kind, P, "floor resolve", 256)
         c = N.CTRL
         assert int(ctrl[c["S_DESTROYED"]]) == int(stats[0]) + int(ch.ctr[oracle.R_S_DESTROYED])
         assert int(ctrl[c["S_BILL"]]) == int(stats[1]) + int(ch.ctr[oracle.R_S_BILL])
         assert len(sp) > 1000
-        print("floor scene: %d labels, %d chunks, %d arena voxels, %d debris records" %
-              (on, int(ch.ctr[oracle.R_S_ISLANDS]), int(ch.ctr[oracle.R_C_ARENA]), len(sp)), flush=True)
+        print("floor scene: %d labels, %d chunks, %d arena voxels, %d + %d debris records" %
+              (r.n, int(ch.ctr[oracle.R_S_ISLANDS]), int(ch.ctr[oracle.R_C_ARENA]), len(r.spawns), len(sp)),
+              flush=True)
     finally:
         gw.close()
 

@@ -220,7 +220,7 @@ def test_sim_step_and_quiescence(floor_scene, floor_impacts):
         grid2, _ = gw.read_grid()
         assert np.array_equal(grid2, grid)
         with pytest.raises(PxError):
-            sim.encode(2)
+            sim.encode(1 << 10)  # not a stage bit
     finally:
         gw.close()
 
