@@ -455,6 +455,29 @@ int px_read_device_impacts(px_ctx* ctx, float* out16, uint32_t cap, uint32_t* n)
 #define PX_STAGE_RESOLVE 16u
 #define PX_STAGE_CHUNKS 32u
 
+/* ---- shooting (sim.wgsl:371-403) ------------------------------------------
+ * Replaces the player pass's writes of the Camera block (shared.wgsl:22-30,
+ * player.wgsl:150-190), which this library does not run: 64 floats,
+ * invViewProj, viewProj, eye (m) at [32..35], look at [36..39], feet, vel,
+ * state.  The shoot stage reads eye and look.  Blocking copy. */
+int px_set_camera(px_ctx* ctx, const float cam[PX_CAMERA_FLOATS]);
+/* PX_STAGE_SHOOT runs shoot() after PX_STAGE_FRAME_BEGIN and before
+ * PX_STAGE_EDIT (Simulation.ts:223-224): u32(U.fire.x) pellets (at most 16,
+ * the reference's workgroup) trace traceSolid from C.eye / VS along C.look,
+ * spread by U.fire2.x with seed U.fire2.y, up to 80 m.  Every hit queues an
+ * impact (c = voxel + 0.5 - n * min(0.3 r, 2), r = U.fire.y, fr = U.fire.z,
+ * impulse = U.fire.w); a pane voxel (fragile 2) in a pool brick is taken with
+ * shardEffects and floods its pane in the next island pass; pellet 0's hit
+ * writes a flash.  The pellets are traced on the world before any of them
+ * takes a voxel and applied in pellet order (the reference's threads race;
+ * this order fixes the impact slots and the debris order). */
+#define PX_STAGE_SHOOT 64u
+#define PX_MAX_FLASHES 8 /* layout.ts:24 */
+/* The flash ring (H_FLASHES, sim.wgsl:395-401): PX_MAX_FLASHES records of 8
+ * floats, world position (m) at [0..2], radius (m) at [3], U.time.x at [4];
+ * record C_FLASH % PX_MAX_FLASHES is the next one written.  Blocking. */
+int px_read_flashes(px_ctx* ctx, float out[PX_MAX_FLASHES * 8]);
+
 /* ---- checkpoint of the device world (no counterpart in the reference, which
  * rebuilds its world from the seed on every load and never saves GPU state).
  * px_snapshot copies grid, pool, super grid, ctrl words and free list to a

@@ -108,7 +108,10 @@ def _bind(lib):
     lib.o_particle_step.argtypes = [W, P, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint32, P, P, P]
     lib.o_edit_pass.restype = C.c_int
     lib.o_edit_pass.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, C.POINTER(EditParams), P, C.c_uint32, P,
-                                C.c_uint32, P, P, C.c_uint32, P, P, C.c_uint32, P, P]
+                                C.c_uint32, P, C.c_uint32, P, P, P, C.c_uint32, P, P, C.c_uint32, P, P]
+    lib.o_shoot.restype = C.c_int
+    lib.o_shoot.argtypes = [C.c_int32, C.c_int32, C.c_int32, P, P, P, C.POINTER(ShootParams), P, P, P, P, P, P, P, P,
+                            C.c_uint32, P, P]
     return lib
 
 
@@ -286,14 +289,22 @@ class EditResult:
 
 def edit_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, mode: int = 0,
               node_budget: int = 0xFFFFFFFF, vs: float = 1.0 / 80.0, frame: float = 0.0, seed_keys=None,
-              cap: int = 1 << 20, spawn_cap: int = 1 << 22, lib=None) -> EditResult:
+              cap: int = 1 << 20, spawn_cap: int = 1 << 22, lib=None, flood=None, dirty=None) -> EditResult:
     """impactJobs / alloc / carve with carveEffects / pane flood with
-    shardEffects / maint / island search on copies (o_edit_pass)."""
+    shardEffects / maint / island search on copies (o_edit_pass).  flood:
+    (key, id) entries a shot pushed; dirty: one byte per cell, the cells its
+    takes touched (both from shoot())."""
     vox = np.array(vox, np.uint16, copy=True, order="C")
     kind = np.array(kind, np.uint8, copy=True, order="C")
     mats = np.ascontiguousarray(mats, np.float32)
     imp = np.ascontiguousarray(impacts, np.float32).reshape(-1, 16)
     sk = np.ascontiguousarray(np.zeros(0, np.uint32) if seed_keys is None else seed_keys, np.uint32)
+    fl = np.ascontiguousarray(np.zeros((0, 2), np.uint32) if flood is None else flood, np.uint32).reshape(-1, 2)
+    dirty_p = None
+    if dirty is not None:
+        dirty = np.ascontiguousarray(dirty, np.uint8)
+        assert dirty.size == kind.size
+        dirty_p = dirty.ctypes.data
     keys = np.zeros(cap, np.uint32)
     cnt = np.zeros(cap, np.uint32)
     sp = np.zeros(spawn_cap, PARTICLE_DTYPE)
@@ -302,13 +313,60 @@ def edit_pass(dims, vox: np.ndarray, kind: np.ndarray, mats, impacts, mode: int 
     prm = EditParams(mode, min(int(node_budget), 0xFFFFFFFF), float(np.float32(vs)), frame_seed(frame))
     lib = lib or load()
     rc = lib.o_edit_pass(dims[0], dims[1], dims[2], vox.ctypes.data, kind.ctypes.data, mats.ctypes.data,
-                         C.byref(prm), imp.ctypes.data, imp.shape[0], sk.ctypes.data, sk.size, keys.ctypes.data,
-                         cnt.ctypes.data, cap, C.byref(n), sp.ctypes.data, spawn_cap, C.byref(ns), stats.ctypes.data)
+                         C.byref(prm), imp.ctypes.data, imp.shape[0], sk.ctypes.data, sk.size, fl.ctypes.data,
+                         fl.shape[0], dirty_p, keys.ctypes.data, cnt.ctypes.data, cap, C.byref(n), sp.ctypes.data,
+                         spawn_cap, C.byref(ns), stats.ctypes.data)
     if rc != 0:
         raise MemoryError("oracle edit pass failed")
     m = min(cap, n.value)
     return EditResult(vox, kind, keys[:m].copy(), cnt[:m].copy(), int(n.value), sp[:min(spawn_cap, ns.value)].copy(),
                       int(ns.value), stats)
+
+
+class ShootParams(C.Structure):
+    _fields_ = [("fire", C.c_float * 4), ("fire2", C.c_float * 2), ("time_x", C.c_float), ("vs", C.c_float),
+                ("eye", C.c_float * 3), ("look", C.c_float * 3), ("frame_seed", C.c_uint32),
+                ("max_impacts", C.c_uint32)]
+
+
+class ShootResult:
+    """Outcome of o_shoot: the world after the takes, the impact queue
+    (impacts[:min(n_impacts, 64)], n_impacts = C_IMPACTS), the flood entries
+    and the dirtied cells for edit_pass, the flash ring and its counter, the
+    spawn records in call order and [destroyed, bill, pellets that hit]."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def shoot(dims, vox: np.ndarray, kind: np.ndarray, mats, u52, cam64, impacts=None, n_impacts: int = 0,
+          flashes=None, flash_ctr: int = 0, max_impacts: int = 64, lib=None) -> ShootResult:
+    """shoot() (sim.wgsl:371-403) on copies of the dense world: the fire /
+    fire2 / time words of the uniforms u52 and eye / look of the camera
+    block cam64."""
+    vox = np.array(vox, np.uint16, copy=True, order="C")
+    kind = np.array(kind, np.uint8, copy=True, order="C")
+    mats = np.ascontiguousarray(mats, np.float32)
+    u = np.asarray(u52, np.float32)
+    cam = np.asarray(cam64, np.float32)
+    imp = np.zeros((64, 16), np.float32) if impacts is None else np.array(impacts, np.float32).reshape(64, 16)
+    fl = np.zeros((64, 2), np.uint32)
+    dirty = np.zeros(kind.size, np.uint8)
+    fla = np.zeros((8, 8), np.float32) if flashes is None else np.array(flashes, np.float32).reshape(8, 8)
+    ni, nf, fc, ns = C.c_uint32(n_impacts), C.c_uint32(0), C.c_uint32(flash_ctr), C.c_uint32(0)
+    spawn_cap = 1 << 16
+    sp = np.zeros(spawn_cap, PARTICLE_DTYPE)
+    stats = np.zeros(3, np.uint64)
+    prm = ShootParams((C.c_float * 4)(*u[40:44]), (C.c_float * 2)(*u[44:46]), float(u[28]), float(u[3]),
+                      (C.c_float * 3)(*cam[32:35]), (C.c_float * 3)(*cam[36:39]), frame_seed(float(u[29])),
+                      max_impacts)
+    lib = lib or load()
+    lib.o_shoot(dims[0], dims[1], dims[2], vox.ctypes.data, kind.ctypes.data, mats.ctypes.data, C.byref(prm),
+                imp.ctypes.data, C.byref(ni), fl.ctypes.data, C.byref(nf), dirty.ctypes.data, fla.ctypes.data,
+                C.byref(fc), sp.ctypes.data, spawn_cap, C.byref(ns), stats.ctypes.data)
+    return ShootResult(vox=vox, kind=kind, impacts=imp, n_impacts=int(ni.value), flood=fl[:min(64, nf.value)].copy(),
+                       n_flood=int(nf.value), dirty=dirty, flashes=fla, flash_ctr=int(fc.value),
+                       spawns=sp[:min(spawn_cap, ns.value)].copy(), spawn_calls=int(ns.value), stats=stats)
 
 
 def apply_writes(dims, vox: np.ndarray, kind: np.ndarray, writes: np.ndarray, max_alloc: int = 65535,

@@ -356,10 +356,19 @@ static void march_n(const o_world* w, const float ro[3], const float rd[3], floa
 /* traceSolid: sim.wgsl:296-357.  steps = brick iterations + voxel
  * iterations; t = the voxel-level t of the hit (traceSolid itself returns no t). */
 
-void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], float maxT, o_hit* out) {
+/* The world as traceSolid reads it: the brick-cell word (0 = empty) and the
+ * voxel value at p (0 outside), over either the brick map or a dense copy. */
+typedef struct o_src {
+  int32_t dx, dy, dz;
+  const void* ctx;
+  uint32_t (*cell)(const void* ctx, uint32_t ci);
+  uint32_t (*voxel)(const void* ctx, const int32_t* p);
+} o_src;
+
+static void trace_solid_src(const o_src* S, const float ro[3], const float rdIn[3], float maxT, o_hit* out) {
   uint32_t steps = 0;
   float rd[3], inv[3], t0[3], t1[3], tmn[3], tmx[3];
-  float dims[3] = {(float)w->dx, (float)w->dy, (float)w->dz};
+  float dims[3] = {(float)S->dx, (float)S->dy, (float)S->dz};
   for (int a = 0; a < 3; a++) {
     rd[a] = fabsf(rdIn[a]) < 1e-6f ? fsign(rdIn[a] + 1e-30f) * 1e-6f : rdIn[a];
     inv[a] = 1.0f / rd[a];
@@ -384,7 +393,7 @@ void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], flo
     else if (tmn[1] >= tmn[2]) normal[1] = -stpF[1];
     else normal[2] = -stpF[2];
   }
-  int32_t bd[3] = {w->dx >> 3, w->dy >> 3, w->dz >> 3};
+  int32_t bd[3] = {S->dx >> 3, S->dy >> 3, S->dz >> 3};
   int32_t bc[3];
   float tDeltaB[3], tMaxB[3], tDeltaV[3];
   for (int a = 0; a < 3; a++) {
@@ -395,7 +404,7 @@ void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], flo
   }
   for (int i = 0; i < 1024; i++) {
     steps++;
-    uint32_t g = w->grid[(uint32_t)(bc[0] + bd[0] * (bc[1] + bd[1] * bc[2]))];
+    uint32_t g = S->cell(S->ctx, (uint32_t)(bc[0] + bd[0] * (bc[1] + bd[1] * bc[2])));
     if (g != 0u) {
       int32_t lo[3], vc[3];
       float tMaxV[3];
@@ -408,8 +417,14 @@ void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], flo
       float n[3] = {normal[0], normal[1], normal[2]};
       for (int k = 0; k < 25; k++) {
         steps++;
-        uint32_t v = readVoxel(w, vc);
-        if (v != 0u) { set_hit(w, out, tv, vc, n, v, steps); return; }
+        uint32_t v = S->voxel(S->ctx, vc);
+        if (v != 0u) {
+          o_world kw;
+          kw.dx = S->dx;
+          kw.dy = S->dy;
+          set_hit(&kw, out, tv, vc, n, v, steps);
+          return;
+        }
         if (tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2]) {
           tv = tMaxV[0]; vc[0] += stp[0]; tMaxV[0] += tDeltaV[0];
           n[0] = -stpF[0]; n[1] = 0.0f; n[2] = 0.0f;
@@ -439,6 +454,14 @@ void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], flo
     if (t > tExit || bc[0] < 0 || bc[1] < 0 || bc[2] < 0 || bc[0] >= bd[0] || bc[1] >= bd[1] || bc[2] >= bd[2]) break;
   }
   set_miss(out, steps);
+}
+
+static uint32_t src_world_cell(const void* ctx, uint32_t ci) { return ((const o_world*)ctx)->grid[ci]; }
+static uint32_t src_world_voxel(const void* ctx, const int32_t* p) { return readVoxel((const o_world*)ctx, p); }
+
+void o_trace_solid(const o_world* w, const float ro[3], const float rdIn[3], float maxT, o_hit* out) {
+  o_src S = {w->dx, w->dy, w->dz, w, src_world_cell, src_world_voxel};
+  trace_solid_src(&S, ro, rdIn, maxT, out);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -705,8 +728,8 @@ int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* ki
   P.vs = 1.0f / 80.0f;
   P.frame_seed = 0u;
   uint64_t st[8];
-  int rc = o_edit_pass(dx, dy, dz, vox, kind, mats, &P, impacts16, n_impacts, NULL, 0u, lab_key, lab_count, cap,
-                       n_labels, NULL, 0u, NULL, st);
+  int rc = o_edit_pass(dx, dy, dz, vox, kind, mats, &P, impacts16, n_impacts, NULL, 0u, NULL, 0u, NULL, lab_key,
+                       lab_count, cap, n_labels, NULL, 0u, NULL, st);
   if (stats)
     for (int k = 0; k < 5; k++) stats[k] = st[k];
   return rc;
@@ -1219,8 +1242,9 @@ static int seeded_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vo
 
 int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
                 const o_edit_params* P, const float* impacts16, uint32_t n_impacts, const uint32_t* seed_keys,
-                uint32_t n_seed_keys, uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels,
-                o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns, uint64_t* stats) {
+                uint32_t n_seed_keys, const uint32_t* flood_in, uint32_t n_flood_in, const uint8_t* dirty_in,
+                uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels, o_particle* spawns,
+                uint32_t spawn_cap, uint32_t* n_spawns, uint64_t* stats) {
   ed_world E;
   E.dx = dx; E.dy = dy; E.dz = dz;
   E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
@@ -1241,6 +1265,10 @@ int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind
   uint64_t destroyed = 0, bill = 0, allocs = 0, njobs = 0, nflood = 0, ndrop = 0;
   uint32_t ns = 0;
   if (n_impacts > 64u) n_impacts = 64u;
+  /* cells shoot's takes dirtied are maintained by this pass */
+  if (dirty_in)
+    for (size_t ci = 0; ci < ncell; ci++)
+      if (dirty_in[ci]) E.dirty[ci] = 1;
   /* impactJobs (sim.wgsl:414-443) on the grid before any alloc: per cell,
    * the mask of impacts with a job there */
   const int32_t bd[3] = {E.bx, E.by, E.bz};
@@ -1393,6 +1421,17 @@ int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind
     uint32_t m = v & 0xffu;
     for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
   }
+  /* the entries shoot pushed (pushFlood of a struck pane voxel): floodStep
+   * takes that voxel's same-id neighbours */
+#ifdef O_PLANT_SHOT_FLOOD_SKIPPED
+  n_flood_in = 0u;
+#endif
+  for (uint32_t i = 0; i < n_flood_in; i++) {
+    uint32_t key = flood_in[2 * i], m = flood_in[2 * i + 1];
+    int32_t x = (int32_t)(key % (uint32_t)dx), y = (int32_t)((key / (uint32_t)dx) % (uint32_t)dy);
+    int32_t z = (int32_t)(key / ((uint32_t)dx * (uint32_t)dy));
+    for (int k = 0; k < 6; k++) TAKE_PUSH(x + DIRS[k][0], y + DIRS[k][1], z + DIRS[k][2], m);
+  }
   while (qh < qt) {
     int32_t x = (int32_t)q[qh * 4], y = (int32_t)q[qh * 4 + 1], z = (int32_t)q[qh * 4 + 2];
     uint32_t m = q[qh * 4 + 3];
@@ -1450,6 +1489,11 @@ int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind
                  (int32_t)((ci / (size_t)E.bx) % (size_t)E.by) * 8 + (int32_t)((l >> 3) & 7u),
                  (int32_t)(ci / ((size_t)E.bx * E.by)) * 8 + (int32_t)(l >> 6), (uint32_t)fl[i * 2 + 1], seed);
     }
+    for (uint32_t i = 0; i < n_flood_in; i++) {
+      uint32_t key = flood_in[2 * i];
+      mark_seeds(&E, mats, (int32_t)(key % (uint32_t)dx), (int32_t)((key / (uint32_t)dx) % (uint32_t)dy),
+                 (int32_t)(key / ((uint32_t)dx * (uint32_t)dy)), flood_in[2 * i + 1], seed);
+    }
     for (uint32_t i = 0; i < n_seed_keys; i++)
       if (seed_keys[i] < N) bit_set(seed, seed_keys[i]);
     for (size_t i = 0; i < N; i++)
@@ -1483,6 +1527,105 @@ int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind
     stats[7] = ndrop;
   }
   return rc;
+}
+
+static uint32_t src_dense_cell(const void* ctx, uint32_t ci) { return ((const ed_world*)ctx)->kind[ci]; }
+static uint32_t src_dense_voxel(const void* ctx, const int32_t* p) {
+  return read_dense((const ed_world*)ctx, p[0], p[1], p[2]);
+}
+
+static inline uint32_t f2u_sat(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967040.0f) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+int o_shoot(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+            const o_shoot_params* P, float* impacts16, uint32_t* n_impacts, uint32_t* flood, uint32_t* n_flood,
+            uint8_t* dirty, float* flashes, uint32_t* flash_ctr, o_particle* spawns, uint32_t spawn_cap,
+            uint32_t* n_spawns, uint64_t* stats) {
+  ed_world E;
+  E.dx = dx; E.dy = dy; E.dz = dz;
+  E.bx = dx >> 3; E.by = dy >> 3; E.bz = dz >> 3;
+  E.vox = vox;
+  E.kind = kind;
+  E.mats = mats;
+  E.dirty = dirty;
+  o_src S = {dx, dy, dz, &E, src_dense_cell, src_dense_voxel};
+  uint32_t pellets = f2u_sat(P->fire[0]);
+  if (pellets > 16u) pellets = 16u;
+  /* every pellet is traced on the world before any pellet's effects */
+  o_hit hits[16];
+  float dirs[16][3];
+  for (uint32_t li = 0; li < pellets; li++) {
+    float dir[3] = {P->look[0], P->look[1], P->look[2]};
+    if (P->fire2[0] > 0.0f) {
+      uint32_t seed = f2u_sat(P->fire2[1]) * 7919u + li * 104729u;
+      float d[3];
+      for (int k = 0; k < 3; k++) d[k] = dir[k] + ((rnd01(seed + (uint32_t)k) - 0.5f) * 2.0f) * P->fire2[0];
+      float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+      for (int k = 0; k < 3; k++) dir[k] = d[k] / len;
+    }
+    float ro[3] = {P->eye[0] / P->vs, P->eye[1] / P->vs, P->eye[2] / P->vs};
+    trace_solid_src(&S, ro, dir, 80.0f / P->vs, &hits[li]);
+    for (int k = 0; k < 3; k++) dirs[li][k] = dir[k];
+  }
+  uint32_t ns = 0;
+  uint64_t destroyed = 0, bill = 0, nhit = 0;
+  for (uint32_t i = 0; i < pellets; i++) {
+    const o_hit* h = &hits[i];
+    if (h->key == NONE) continue;
+    nhit++;
+    int32_t x = (int32_t)(h->key % (uint32_t)dx), y = (int32_t)((h->key / (uint32_t)dx) % (uint32_t)dy);
+    int32_t z = (int32_t)(h->key / ((uint32_t)dx * (uint32_t)dy));
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    if (h->face < 6) n[h->face >> 1] = (h->face & 1) ? -1.0f : 1.0f;
+    float r = P->fire[1];
+    float off = fminf(r * 0.3f, 2.0f);
+#ifdef O_PLANT_SHOOT_CENTRE_SIGN
+    float c[3] = {((float)x + 0.5f) + n[0] * off, ((float)y + 0.5f) + n[1] * off, ((float)z + 0.5f) + n[2] * off};
+#else
+    float c[3] = {((float)x + 0.5f) - n[0] * off, ((float)y + 0.5f) - n[1] * off, ((float)z + 0.5f) - n[2] * off};
+#endif
+    /* pushImpact (sim.wgsl:203-211) */
+    uint32_t k = (*n_impacts)++;
+    if (k < P->max_impacts && k < 64u) {
+      float* im = impacts16 + (size_t)k * 16u;
+      float rec[12] = {c[0], c[1], c[2], r, P->fire[2], P->fire[3], n[0], n[1], n[2],
+                       dirs[i][0], dirs[i][1], dirs[i][2]};
+      for (int q = 0; q < 12; q++) im[q] = rec[q];
+    }
+    /* a struck pane voxel: takeVoxel, shardEffects, pushFlood (:386-394) */
+    uint32_t id = h->mat;
+    if (matE(mats, id)[1] > 1.5f) {
+      uint32_t v = take_voxel(&E, x, y, z, id);
+      if (v != 0u) {
+        drop_rubble_above(&E, x, y, z, P->vs, P->frame_seed, spawns, spawn_cap, &ns);
+        bill += price_dimes(mats, v);
+        destroyed++;
+        shard_debris(mats, x, y, z, h->key, v, P->vs, P->frame_seed, spawns, spawn_cap, &ns);
+        uint32_t q = (*n_flood)++;
+        if (q < 64u) {
+          flood[2 * q] = h->key;
+          flood[2 * q + 1] = id;
+        }
+      }
+    }
+    /* pellet 0's flash (:395-401) */
+    if (i == 0u) {
+      float* b = flashes + ((*flash_ctr)++ % 8u) * 8u;
+      for (int q = 0; q < 3; q++) b[q] = (c[q] + n[q] * 4.0f) * P->vs;
+      b[3] = r * P->vs;
+      b[4] = P->time_x;
+    }
+  }
+  *n_spawns = ns;
+  if (stats) {
+    stats[0] = destroyed;
+    stats[1] = bill;
+    stats[2] = nhit;
+  }
+  return 0;
 }
 
 int o_resolve(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,

@@ -84,7 +84,12 @@ int o_island_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* ki
  * only components that hold a seed (pushSeed, sim.wgsl:186, 524, 615, 1068),
  * with the floating ones kept in label order while their voxel total stays
  * within node_budget.  vs = U.dims.w, frame_seed = u32(U.time.y) *
- * 0x9E3779B9.  seed_keys: pending seeds of landed chunks.  Every spawn()
+ * 0x9E3779B9.  seed_keys: pending seeds of landed chunks.  flood_in:
+ * n_flood_in (key, id) entries o_shoot pushed for struck pane voxels; the
+ * flood takes each one's same-id neighbours (floodStep, sim.wgsl:599-618)
+ * after carve, and in mode 1 its other neighbours are seeds.  dirty_in (one
+ * byte per cell, may be NULL): cells o_shoot's takes touched, maintained
+ * here with the pass's own.  Every spawn()
  * call (dropRubbleAbove, carve debris, shard debris) is written to spawns in
  * call order up to spawn_cap; *n_spawns counts them all.  stats: [0..4] as
  * o_island_pass, [5] seed voxels that are node voxels at the search,
@@ -97,8 +102,32 @@ typedef struct o_edit_params {
 } o_edit_params;
 int o_edit_pass(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
                 const o_edit_params* P, const float* impacts16, uint32_t n_impacts, const uint32_t* seed_keys,
-                uint32_t n_seed_keys, uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels,
-                o_particle* spawns, uint32_t spawn_cap, uint32_t* n_spawns, uint64_t* stats);
+                uint32_t n_seed_keys, const uint32_t* flood_in, uint32_t n_flood_in, const uint8_t* dirty_in,
+                uint32_t* lab_key, uint32_t* lab_count, uint32_t cap, uint32_t* n_labels, o_particle* spawns,
+                uint32_t spawn_cap, uint32_t* n_spawns, uint64_t* stats);
+
+/* shoot (sim.wgsl:371-403) on the dense world of o_edit_pass, run before it:
+ * min(u32(fire[0]), 16) pellets along look (spread by fire2[0] with seed
+ * fire2[1]: rnd3 of u32(seed) * 7919 + i * 104729, then normalised) traced
+ * with traceSolid from eye / vs to 80 / vs, every pellet on the world before
+ * any pellet's effects, which then run in pellet order: pushImpact (slot
+ * *n_impacts, stored below max_impacts), a struck pane voxel of a pool brick
+ * taken with shardEffects (spawns in call order) and its flood entry (key,
+ * id; 64 kept, *n_flood counts all), pellet 0's flash into record
+ * (*flash_ctr)++ % 8 of flashes.  Cells the takes touch are set in dirty
+ * (one byte per cell) for o_edit_pass's dirty_in.  stats: [0] destroyed,
+ * [1] bill (dimes), [2] pellets that hit. */
+typedef struct o_shoot_params {
+  float fire[4];  /* pellets, radius, fragile radius, impulse */
+  float fire2[2]; /* spread, seed */
+  float time_x, vs;
+  float eye[3], look[3];
+  uint32_t frame_seed, max_impacts;
+} o_shoot_params;
+int o_shoot(int32_t dx, int32_t dy, int32_t dz, uint16_t* vox, uint8_t* kind, const float* mats,
+            const o_shoot_params* P, float* impacts16, uint32_t* n_impacts, uint32_t* flood, uint32_t* n_flood,
+            uint8_t* dirty, float* flashes, uint32_t* flash_ctr, o_particle* spawns, uint32_t spawn_cap,
+            uint32_t* n_spawns, uint64_t* stats);
 
 /* Unsupported-component labelling alone (the last stage of o_island_pass). */
 int o_islands(int32_t dx, int32_t dy, int32_t dz, const uint16_t* vox, const float* mats,

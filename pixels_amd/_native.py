@@ -24,6 +24,8 @@ PX_CAMERA_FLOATS = 64
 
 RAY_PRIMARY, RAY_SHADOW, RAY_SHOOT, RAY_SEGMENT, RAY_MIXED = 0, 1, 2, 3, 4
 STAGE_EDIT, STAGE_PARTICLES, STAGE_FRAME_BEGIN, STAGE_WRITES, STAGE_RESOLVE, STAGE_CHUNKS = 1, 2, 4, 8, 16, 32
+STAGE_SHOOT = 64
+PX_MAX_FLASHES = 8  # layout.ts:24
 P_SETTLED, P_DEPOSIT, P_DUST = 0x10000, 0x20000, 0x40000
 FACE_ZERO, FACE_NOT_CAST, FACE_MISS = 6, 0xFE, 0xFF
 FRAME_PIXEL_MAJOR, FRAME_SLOT_PLANAR = 0, 1
@@ -150,6 +152,8 @@ SIGNATURES = {
     "px_chunk_step": (C.c_int, [_P]),
     "px_read_chunks": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
     "px_read_device_impacts": (C.c_int, [_P, _FP, _U32, _U32P]),
+    "px_set_camera": (C.c_int, [_P, _FP]),
+    "px_read_flashes": (C.c_int, [_P, _FP]),
     "px_push_writes": (C.c_int, [_P, _P, _U32]),
     "px_profile_begin": (C.c_int, [_P]),
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),

@@ -335,6 +335,22 @@ int px_set_lights(px_ctx* c, const float* lights, uint32_t n_vec4) {
 
 uint32_t px_frame_slots(const px_ctx* c) { return c ? c->shadow.slots : 0u; }
 
+int px_set_camera(px_ctx* c, const float cam[PX_CAMERA_FLOATS]) {
+  if (!c || !cam) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipMemcpyAsync(c->d_camera, cam, PX_CAMERA_FLOATS * 4, hipMemcpyHostToDevice, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
+int px_read_flashes(px_ctx* c, float out[PX_MAX_FLASHES * 8]) {
+  if (!c || !out) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipMemcpyAsync(out, c->d_flashes, PX_MAX_FLASHES * 8 * 4, hipMemcpyDeviceToHost, c->stream));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  return PX_OK;
+}
+
 int px_sync(px_ctx* c) {
   if (!c) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
@@ -606,6 +622,7 @@ int px_restore(px_ctx* c) {
   c->labels_valid = false;
   c->masks_dirty = c->density_dirty = true;
   if (int rc = px_clear_seeds(c)) return rc;  // pending seeds belong to the world being dropped
+  if (int rc = px_clear_shots(c)) return rc;  // so do shoot's flood entries and dirty cells
   return px_refresh_masks(c);
 }
 
@@ -643,13 +660,16 @@ int px_island_pass(px_ctx* c) {
 int px_sim_step(px_ctx* c, uint32_t stages_mask) {
   if (!c) return PX_ERR_ARG;
   const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN | PX_STAGE_WRITES |
-                         PX_STAGE_RESOLVE | PX_STAGE_CHUNKS;
+                         PX_STAGE_RESOLVE | PX_STAGE_CHUNKS | PX_STAGE_SHOOT;
   if (stages_mask & ~known) return px_fail(c, PX_ERR_ARG, "px_sim_step", "unknown stage bits");
   if ((stages_mask & PX_STAGE_PARTICLES) && !c->d_particles)
     return px_fail(c, PX_ERR_STATE, "px_sim_step", "particle stage without px_particles_init");
   PX_HIP_CHECK(c, hipSetDevice(c->device));
   if (stages_mask & PX_STAGE_FRAME_BEGIN) {
     if (int rc = px_launch_frame_begin(c)) return rc;
+  }
+  if (stages_mask & PX_STAGE_SHOOT) {
+    if (int rc = px_launch_shoot(c)) return rc;
   }
   if (stages_mask & PX_STAGE_EDIT) {
     // runs on the device only with queued impacts (or pending seeds, seeded mode)

@@ -38,6 +38,7 @@
 #include <algorithm>
 
 #include "px_internal.h"
+#include "px_march.h"
 #include "px_scan.h"
 
 #pragma clang fp contract(off)
@@ -110,6 +111,8 @@ struct EditView {
   uint32_t out_cap;
   uint32_t* ec;
   float* impacts;     // the impact queue (pushImpact's records), C_IMPACTS of them
+  uint32_t* shots;    // shoot's pane flood entries and dirtied cells (kShotWords)
+  float* flashes;     // flash ring, PX_MAX_FLASHES records of 8 floats
   uint32_t max_imp;   // min(max_impacts, 64)
   uint32_t* seeds;    // seeded mode: one bit per voxel key, pushSeed (sim.wgsl:180-184)
   uint32_t seed_words;
@@ -780,6 +783,8 @@ __global__ __launch_bounds__(64) void k_maint(EditView e) {
   }
   const uint32_t nd = min(e.ec[EC_R_DIRTY], e.ncell);
   for (uint32_t i = blockIdx.x; i < nd; i += gridDim.x) dirty += maint_cell(e, e.rdirty[i]);
+  const uint32_t ns = min(e.shots[1], kShotDirtyCap);
+  for (uint32_t i = blockIdx.x; i < ns; i += gridDim.x) dirty += maint_cell(e, e.shots[kShotDirtyBase + i]);
   if (threadIdx.x == 0) e.part[blockIdx.x * kPartWords + PT_DIRTY] = dirty;
 }
 
@@ -958,6 +963,10 @@ __global__ __launch_bounds__(1024) void k_edit_finish(EditView e) {
   e.ctrl[PX_C_ALLOC_N] = e.ec[EC_ALLOCS];
   e.ctrl[PX_C_ALLOCS] = 0u;
   e.ctrl[PX_C_IMPACTS] = 0u;  // postCarve, sim.wgsl:581-584
+  if (carved) {               // shoot's flood entries and dirty cells are consumed
+    e.shots[0] = 0u;
+    e.shots[1] = 0u;
+  }
   e.ctrl[PX_S_SEARCHES] += 1u;
   if (e.mode == PX_ISLANDS_SEEDED) e.ctrl[PX_C_SEED_N] = e.ec[EC_SEEDS];
   e.ec[EC_LABELS_OK] = 1u;
@@ -1313,16 +1322,42 @@ PX_DEV bool shard_spawns(const EditView& e, uint32_t key, uint32_t v) {
   return rnd01(key ^ e.fseed) < e.mats[(v & 0xffu) * 20u + 19u] * 0.6f;
 }
 
-PX_DEV void shard_record(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t key, uint32_t v, uint32_t j,
-                         uint32_t accepted, uint32_t cursor) {
+// the particle of that record
+PX_DEV void shard_particle(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t key, uint32_t v, float pos[3],
+                           float vel[3], uint32_t& value, float& life) {
   const uint32_t seed = key ^ e.fseed;
   const float s = 1.0f / e.vs;
   const float b[3] = {rnd01(seed + 3u) - 0.5f, rnd01(seed + 4u) * 0.4f, rnd01(seed + 5u) - 0.5f};
   const float m[3] = {1.6f, 1.0f, 1.6f};
-  float vel[3];
   for (int a = 0; a < 3; a++) vel[a] = (b[a] * m[a]) * s;
-  const float pos[3] = {(float)x + 0.5f, (float)y + 0.5f, (float)z + 0.5f};
-  spawn_at(e, j, accepted, cursor, pos, vel, v | debris_flags(e.mats, v, seed), 4.0f + 4.0f * rnd01(seed + 6u));
+  pos[0] = (float)x + 0.5f;
+  pos[1] = (float)y + 0.5f;
+  pos[2] = (float)z + 0.5f;
+  value = v | debris_flags(e.mats, v, seed);
+  life = 4.0f + 4.0f * rnd01(seed + 6u);
+}
+
+PX_DEV void shard_record(const EditView& e, int32_t x, int32_t y, int32_t z, uint32_t key, uint32_t v, uint32_t j,
+                         uint32_t accepted, uint32_t cursor) {
+  float pos[3], vel[3], life;
+  uint32_t value;
+  shard_particle(e, x, y, z, key, v, pos, vel, value, life);
+  spawn_at(e, j, accepted, cursor, pos, vel, value, life);
+}
+
+// dropRubbleAbove's particle for rubble voxel t taken at (x, qy, z)
+// (sim.wgsl:281-282)
+PX_DEV void rubble_particle(const EditView& e, int32_t x, int32_t qy, int32_t z, uint32_t t, float pos[3],
+                            float vel[3], uint32_t& value, float& life) {
+  const uint32_t seed = vkey(e, x, qy, z) ^ e.fseed;
+  pos[0] = (float)x + 0.5f;
+  pos[1] = (float)qy + 0.5f;
+  pos[2] = (float)z + 0.5f;
+  const float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
+  const float m[3] = {0.4f, 0.0f, 0.4f};
+  for (int k = 0; k < 3; k++) vel[k] = ((r[k] - 0.5f) * m[k]) / e.vs;
+  value = t | PX_P_DEPOSIT;
+  life = 6.0f + 2.0f * rnd01(seed + 5u);
 }
 
 // dropRubbleAbove's records for the n rubble voxels above (x, y, z), bottom
@@ -1335,13 +1370,10 @@ PX_DEV void drop_rubble(const EditView& e, int32_t x, int32_t y, int32_t z, uint
     bool pool;
     const uint32_t t = voxel_at(e, x, qy, z, &pool);
     clear_voxel(e, x, qy, z);
-    const uint32_t seed = vkey(e, x, qy, z) ^ e.fseed;
-    const float pos[3] = {(float)x + 0.5f, (float)qy + 0.5f, (float)z + 0.5f};
-    const float r[3] = {rnd01(seed), rnd01(seed + 1u), rnd01(seed + 2u)};
-    const float m[3] = {0.4f, 0.0f, 0.4f};
-    float vel[3];
-    for (int k = 0; k < 3; k++) vel[k] = ((r[k] - 0.5f) * m[k]) / e.vs;
-    spawn_at(e, j + d - 1u, accepted, cursor, pos, vel, t | PX_P_DEPOSIT, 6.0f + 2.0f * rnd01(seed + 5u));
+    float pos[3], vel[3], life;
+    uint32_t value;
+    rubble_particle(e, x, qy, z, t, pos, vel, value, life);
+    spawn_at(e, j + d - 1u, accepted, cursor, pos, vel, value, life);
   }
 }
 
@@ -1627,6 +1659,167 @@ __global__ void k_effects_finish(EditView e) {
   const uint32_t acc = spawn_accepted(e, total);
   e.ctrl[PX_C_SPAWNED] += total;
   e.ctrl[PX_C_PCURSOR] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// shoot (sim.wgsl:371-403)
+
+struct ShootArgs {
+  float fire[4];   // U.fire: pellets, radius, fragile radius, impulse
+  float fire2[2];  // U.fire2: spread, seed
+  float time_x;    // U.time.x
+};
+
+// spawn() (sim.wgsl:166-170) of one record: every call counts, the ones
+// within the frame's budget take the next ring slot.
+PX_DEV void spawn_one(const EditView& e, const float pos[3], const float vel[3], uint32_t value, float life) {
+  const uint32_t k = e.ctrl[PX_C_SPAWNED]++;
+  if (k >= e.spawn_budget) return;
+  const uint32_t slot = (e.ctrl[PX_C_PCURSOR]++) & e.pmask;
+  e.parts[2u * slot] = make_float4(pos[0], pos[1], pos[2], __uint_as_float(value));
+  e.parts[2u * slot + 1u] = make_float4(vel[0], vel[1], vel[2], life);
+}
+
+// takeVoxel (sim.wgsl:105-124) by the shoot stage's single thread: clears the
+// voxel of a pool brick; the first clear of a cell lists it for the next
+// island pass's maint.
+PX_DEV void shot_clear(const EditView& e, int32_t x, int32_t y, int32_t z) {
+  const uint32_t ci = cell_of_pos(e, x, y, z);
+  const uint32_t g = e.grid[ci];
+  const uint32_t idx = (g - 1u) * 512u + (uint32_t)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6));
+  atomicAnd(e.pool + (idx >> 1), ~(0xffffu << ((idx & 1u) * 16u)));
+  const uint32_t bit = 1u << (ci & 31u);
+  if ((atomicOr(&e.dirty_flags[ci >> 5], bit) & bit) == 0u) {
+    const uint32_t k = e.shots[1]++;
+    if (k < kShotDirtyCap) e.shots[kShotDirtyBase + k] = ci;
+  }
+}
+
+// Lane li < pellets traces one pellet (sim.wgsl:373-379) on the world as it
+// is before any pellet's effects; lane 0 then applies the hits in pellet
+// order (:380-402): pushImpact, the pane take with shardEffects
+// (dropRubbleAbove, bill, debris) and its flood entry, pellet 0's flash.
+__global__ __launch_bounds__(64) void k_shoot(EditView e, PxWorldView w, const float* __restrict__ cam,
+                                              ShootArgs a) {
+  __shared__ uint32_t s_key[16], s_face[16], s_val[16];
+  __shared__ float s_dir[16][3];
+  const uint32_t li = threadIdx.x;
+  const uint32_t pellets = min(px::f2u_sat(a.fire[0]), 16u);
+  if (li < pellets) {
+    float dir[3] = {cam[36], cam[37], cam[38]};
+    if (a.fire2[0] > 0.0f) {
+      const uint32_t seed = px::f2u_sat(a.fire2[1]) * 7919u + li * 104729u;
+      float d[3];
+      for (int k = 0; k < 3; k++) d[k] = dir[k] + ((rnd01(seed + (uint32_t)k) - 0.5f) * 2.0f) * a.fire2[0];
+      const float len = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+      for (int k = 0; k < 3; k++) dir[k] = d[k] / len;
+    }
+    const float ro[3] = {cam[32] / e.vs, cam[33] / e.vs, cam[34] / e.vs};
+    const px::MarchOut m = px::trace_solid(w, ro, dir, 80.0f / e.vs);
+    s_key[li] = m.key;
+    s_face[li] = m.face;
+    s_val[li] = m.value;
+    for (int k = 0; k < 3; k++) s_dir[li][k] = dir[k];
+  }
+  __syncthreads();
+  if (li != 0u) return;
+  for (uint32_t i = 0; i < pellets; i++) {
+    const uint32_t key = s_key[i];
+    if (key == PX_NONE) continue;
+    int32_t x, y, z;
+    key_pos(e, key, x, y, z);
+    const uint32_t f = s_face[i];
+    float n[3] = {0.0f, 0.0f, 0.0f};
+    if (f < 6u) n[f >> 1] = (f & 1u) ? -1.0f : 1.0f;
+    const float r = a.fire[1];
+    const float off = fminf(r * 0.3f, 2.0f);
+    const float c[3] = {((float)x + 0.5f) - n[0] * off, ((float)y + 0.5f) - n[1] * off,
+                        ((float)z + 0.5f) - n[2] * off};
+    // pushImpact (sim.wgsl:203-211)
+    const uint32_t k = e.ctrl[PX_C_IMPACTS]++;
+    if (k < e.max_imp) {
+      float* im = e.impacts + (size_t)k * 16u;
+      const float rec[12] = {c[0], c[1], c[2], r, a.fire[2], a.fire[3], n[0], n[1], n[2],
+                             s_dir[i][0], s_dir[i][1], s_dir[i][2]};
+      for (int q = 0; q < 12; q++) im[q] = rec[q];
+    }
+    // a pane shatters as a whole: take the voxel, its effects, flood the rest
+    const uint32_t id = s_val[i] & 0xffu;
+    if (e.mats[id * 20u + 17u] > 1.5f) {
+      bool pool;
+      const uint32_t v = voxel_at(e, x, y, z, &pool);
+      if (pool && v != 0u && (v & 0xffu) == id) {
+        shot_clear(e, x, y, z);
+        // shardEffects (sim.wgsl:261-271): dropRubbleAbove (:274-284) first
+        for (int32_t d = 1; d <= 12; d++) {
+          bool qpool;
+          const uint32_t t = voxel_at(e, x, y + d, z, &qpool);
+          if (t == 0u || (t & 0x80u) == 0u || !qpool) break;
+          shot_clear(e, x, y + d, z);
+          float pos[3], vel[3], life;
+          uint32_t value;
+          rubble_particle(e, x, y + d, z, t, pos, vel, value, life);
+          spawn_one(e, pos, vel, value, life);
+        }
+        e.ctrl[PX_S_BILL] += price_dimes(e.mats, v);
+        e.ctrl[PX_S_DESTROYED] += 1u;
+        if (shard_spawns(e, key, v)) {
+          float pos[3], vel[3], life;
+          uint32_t value;
+          shard_particle(e, x, y, z, key, v, pos, vel, value, life);
+          spawn_one(e, pos, vel, value, life);
+        }
+        // pushFlood (sim.wgsl:250-258)
+        const uint32_t q = e.shots[0]++;
+        if (q < kShotFloodCap) {
+          e.shots[kShotFloodBase + 2u * q] = key;
+          e.shots[kShotFloodBase + 2u * q + 1u] = id;
+        }
+      }
+    }
+    if (i == 0u) {
+      const uint32_t fk = (e.ctrl[PX_C_FLASH]++) % (uint32_t)PX_MAX_FLASHES;
+      float* b = e.flashes + fk * 8u;
+      for (int q = 0; q < 3; q++) b[q] = (c[q] + n[q] * 4.0f) * e.vs;
+      b[3] = r * e.vs;
+      b[4] = a.time_x;
+    }
+  }
+}
+
+// The pane flood of shoot's entries (floodStep, sim.wgsl:599-618, on the
+// entries shoot pushed): the pane components of each struck voxel's
+// same-id neighbours are flooded whole, which is what the breadth-first flood
+// from the struck voxel reaches; in seeded mode the struck voxel's other
+// neighbours are seeds.  After the pane labelling, before the flood passes.
+__global__ __launch_bounds__(64) void k_shot_flood(EditView e) {
+  if (e.ec[EC_NIMP] == 0u) return;
+  const uint32_t n = min(e.shots[0], kShotFloodCap);
+  uint32_t err = 0u;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint32_t key = e.shots[kShotFloodBase + 2u * i], id = e.shots[kShotFloodBase + 2u * i + 1u];
+    int32_t x, y, z;
+    key_pos(e, key, x, y, z);
+    for (int k = 0; k < 6; k++) {
+      const int32_t qx = x + (k == 0 ? 1 : (k == 1 ? -1 : 0));
+      const int32_t qy = y + (k == 2 ? 1 : (k == 3 ? -1 : 0));
+      const int32_t qz = z + (k == 4 ? 1 : (k == 5 ? -1 : 0));
+      bool pool;
+      const uint32_t v = voxel_at(e, qx, qy, qz, &pool);
+      if (!pool || v == 0u || (v & 0xffu) != id) continue;
+      const uint32_t g = e.grid[cell_of_pos(e, qx, qy, qz)];
+      const uint32_t li = (uint32_t)((qx & 7) | ((qy & 7) << 3) | ((qz & 7) << 6));
+      const uint32_t r = e.lab[(size_t)(g - 1u) * 512u + li];
+      if (r >= 512u) {
+        err = kErrLabel;
+        continue;
+      }
+      const uint32_t okey = vkey(e, qx & ~7, qy & ~7, qz & ~7);
+      atomicOr(&e.flag[e.par[okey + loff(e, r)]], kFlagHit);
+    }
+    if (e.mode == PX_ISLANDS_SEEDED) push_seeds(e, x, y, z, id);
+  }
+  if (err) atomicOr(&e.ec[EC_OVERFLOW], err);
 }
 
 // Seeded mode (searchSeed, sim.wgsl:722-731): every seed bit whose voxel is
@@ -2311,6 +2504,8 @@ EditView make_view(px_ctx* c) {
   e.out_cap = c->label_cap;
   e.ec = c->d_edit_ctr;
   e.impacts = c->d_impacts;
+  e.shots = c->d_shots;
+  e.flashes = c->d_flashes;
   e.max_imp = std::min<uint32_t>(c->lim.max_impacts, 64u);
   e.seeds = c->d_seeds;
   e.seed_words = (uint32_t)((c->vox_total + 31u) / 32u);
@@ -2368,6 +2563,11 @@ int px_edit_alloc(px_ctx* c) {
   PX_HIP_CHECK(c, hipMalloc(&c->d_part, (size_t)kEditBlocks * kPartWords * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_bcount, (size_t)kEditBlocks * kCountWords * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_edit_ctr, 64 * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_shots, kShotWords * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_flashes, PX_MAX_FLASHES * 8 * 4));
+  hipLaunchKernelGGL(k_zero32, dim3(1), dim3(256), 0, c->stream, c->d_shots, (size_t)kShotWords);
+  hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<uint32_t*>(c->d_flashes),
+                     (size_t)PX_MAX_FLASHES * 8);
   hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, c->stream, c->d_cell_imask, (size_t)c->ncell);
   hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, c->stream, c->d_alloc_flags, words);
   hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, c->stream, c->d_dirty_flags, words);
@@ -2380,14 +2580,16 @@ int px_edit_alloc(px_ctx* c) {
 void px_edit_free(px_ctx* c) {
   void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags,
                   c->d_job_list, c->d_alloc_list, c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par,
-                  c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds};
+                  c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds, c->d_shots,
+                  c->d_flashes};
   for (void* p : bufs)
     if (p) hipFree(p);
   c->d_impacts = c->d_impact_stage = nullptr;
   c->d_cell_imask = nullptr;
   c->d_par = c->d_ncnt = c->d_nflag = nullptr;
   c->d_lab = nullptr;
-  c->d_nodes = c->d_out_labels = c->d_seeds = nullptr;
+  c->d_nodes = c->d_out_labels = c->d_seeds = c->d_shots = nullptr;
+  c->d_flashes = nullptr;
 }
 
 // the labelling arrays are sized by the voxel count and allocated on first use
@@ -2425,6 +2627,26 @@ int px_clear_seeds(px_ctx* c) {
   return PX_OK;
 }
 
+int px_launch_shoot(px_ctx* c) {
+  if (int rc = px_chunks_alloc(c)) return rc;  // the particle ring takes the debris
+  EditView e = make_view(c);
+  ShootArgs a;
+  for (int k = 0; k < 4; k++) a.fire[k] = c->h_uniforms[40 + k];
+  a.fire2[0] = c->h_uniforms[44];
+  a.fire2[1] = c->h_uniforms[45];
+  a.time_x = c->h_uniforms[28];
+  hipLaunchKernelGGL(k_shoot, dim3(1), dim3(64), 0, c->stream, e, px_view(c), (const float*)c->d_camera, a);
+  PX_HIP_CHECK(c, hipGetLastError());
+  c->masks_dirty = c->density_dirty = true;
+  return PX_OK;
+}
+
+int px_clear_shots(px_ctx* c) {
+  hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, c->d_shots, (size_t)2);
+  PX_HIP_CHECK(c, hipGetLastError());
+  return PX_OK;
+}
+
 int px_launch_push_impacts(px_ctx* c, uint32_t n) {
   EditView e = make_view(c);
   hipLaunchKernelGGL(k_push_impacts, dim3(1), dim3(1024), 0, c->stream, e, (const float*)c->d_impact_stage, n);
@@ -2453,6 +2675,7 @@ int px_launch_island_pass(px_ctx* c, bool force) {
   hipLaunchKernelGGL(k_local_ccl<1>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<1>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_PANENODES);
+  hipLaunchKernelGGL(k_shot_flood, dim3(1), dim3(64), 0, s, e);
   // carve + flood: count the effects, rank them in cell order, apply
   hipLaunchKernelGGL(k_edit_zero_cells, dim3(1024), dim3(256), 0, s, e);
   hipLaunchKernelGGL(k_carve<0>, G, W, 0, s, e);

@@ -93,6 +93,8 @@ struct px_ctx {
   float* d_impacts = nullptr;       // the impact queue: 64 records of 16 floats, C_IMPACTS of them used
   float* d_impact_stage = nullptr;  // host impacts on their way into the queue
   uint32_t* d_seeds = nullptr;      // seeded search: one bit per voxel key
+  uint32_t* d_shots = nullptr;      // shoot's pane flood entries and dirtied cells (kShotWords)
+  float* d_flashes = nullptr;       // muzzle flashes: PX_MAX_FLASHES records of 8 floats
   uint64_t* d_cell_imask = nullptr;  // per cell: bit k = impact k has a job here
   uint32_t* d_alloc_flags = nullptr; // per cell bitset: alloc requested
   uint32_t* d_dirty_flags = nullptr; // per cell bitset: dirty
@@ -223,6 +225,18 @@ void px_edit_free(px_ctx* c);
 // force: label even without impacts or pending seeds (px_island_pass)
 int px_launch_island_pass(px_ctx* c, bool force);
 int px_launch_push_impacts(px_ctx* c, uint32_t n);
+int px_launch_shoot(px_ctx* c);
+int px_clear_shots(px_ctx* c);
+
+// Shot buffer (d_shots): word 0 = pane flood entries shoot queued, word 1 =
+// cells its takes dirtied, then kShotFloodCap (voxel key, material id) pairs,
+// then kShotDirtyCap cell indices.  The next island pass that runs consumes
+// both lists and empties them.
+constexpr uint32_t kShotFloodCap = 64u;
+constexpr uint32_t kShotDirtyCap = 256u;
+constexpr uint32_t kShotFloodBase = 4u;
+constexpr uint32_t kShotDirtyBase = kShotFloodBase + 2u * kShotFloodCap;
+constexpr uint32_t kShotWords = kShotDirtyBase + kShotDirtyCap;
 int px_seeds_alloc(px_ctx* c);
 int px_clear_seeds(px_ctx* c);
 int px_read_island_labels(px_ctx* c, uint32_t* min_key, uint32_t* count, uint32_t cap, uint32_t* n);

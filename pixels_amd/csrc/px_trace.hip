@@ -37,99 +37,7 @@ using px::shadow_ray;
 using px::step_face;
 using px::store_hit;
 using px::store_not_cast;
-
-// traceSolid() of sim.wgsl:296-357.  steps = brick + voxel iterations, t = the
-// voxel-level t of the hit.
-PX_DEV MarchOut trace_solid(const PxWorldView& w, const float ro[3], const float rdIn[3], float maxT) {
-  MarchOut o;
-  o.key = PX_NONE;
-  o.t = 0.0f;
-  o.face = 0xFFu;
-  o.value = 0u;
-  o.steps = 0u;
-  uint32_t steps = 0;
-  float rd[3], inv[3], tmn[3], tmx[3];
-  for (int a = 0; a < 3; a++) {
-    rd[a] = fabsf(rdIn[a]) < 1e-6f ? px_sign(rdIn[a] + 1e-30f) * 1e-6f : rdIn[a];
-    inv[a] = 1.0f / rd[a];
-    float t0 = (0.0f - ro[a]) * inv[a];
-    float t1 = (w.fdims[a] - ro[a]) * inv[a];
-    tmn[a] = fminf(t0, t1);
-    tmx[a] = fmaxf(t0, t1);
-  }
-  float t = fmaxf(fmaxf(tmn[0], tmn[1]), fmaxf(tmn[2], 0.0f));
-  float tExit = fminf(fminf(tmx[0], tmx[1]), fminf(tmx[2], maxT));
-  if (t >= tExit) return o;
-  int32_t stp[3];
-  float pos[3];
-  for (int a = 0; a < 3; a++) {
-    stp[a] = rd[a] > 0.0f ? 1 : (rd[a] < 0.0f ? -1 : 0);
-    pos[a] = rd[a] >= 0.0f ? 1.0f : 0.0f;
-  }
-  uint32_t normal = 6u;
-  if (t > 0.0f) {
-    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) normal = step_face(0, stp[0]);
-    else if (tmn[1] >= tmn[2]) normal = step_face(1, stp[1]);
-    else normal = step_face(2, stp[2]);
-  }
-  int32_t bc[3];
-  float tDeltaB[3], tMaxB[3], tDeltaV[3];
-  for (int a = 0; a < 3; a++) {
-    bc[a] = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), 0, w.bd[a] - 1);
-    tDeltaB[a] = fabsf(inv[a]) * 8.0f;
-    tMaxB[a] = (((float)bc[a] + pos[a]) * 8.0f - ro[a]) * inv[a];
-    tDeltaV[a] = fabsf(inv[a]);
-  }
-  for (int i = 0; i < 1024; i++) {
-    steps++;
-    uint32_t g = w.grid[(uint32_t)(bc[0] + w.bd[0] * (bc[1] + w.bd[1] * bc[2]))];
-    if (g != 0u) {
-      int32_t lo[3], vc[3];
-      float tMaxV[3];
-      for (int a = 0; a < 3; a++) {
-        lo[a] = bc[a] * 8;
-        vc[a] = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (t + 1e-3f))), lo[a], lo[a] + 7);
-        tMaxV[a] = ((float)vc[a] + pos[a] - ro[a]) * inv[a];
-      }
-      float tv = t;
-      uint32_t n = normal;
-      for (int k = 0; k < 25; k++) {
-        steps++;
-        uint32_t v = read_voxel(w, vc[0], vc[1], vc[2]);
-        if (v != 0u) {
-          o.key = key_of(w, vc[0], vc[1], vc[2]);
-          o.t = px_canon0(tv);
-          o.face = n;
-          o.value = v;
-          o.steps = steps;
-          return o;
-        }
-        if (tMaxV[0] < tMaxV[1] && tMaxV[0] < tMaxV[2]) {
-          tv = tMaxV[0]; vc[0] += stp[0]; tMaxV[0] += tDeltaV[0]; n = step_face(0, stp[0]);
-        } else if (tMaxV[1] < tMaxV[2]) {
-          tv = tMaxV[1]; vc[1] += stp[1]; tMaxV[1] += tDeltaV[1]; n = step_face(1, stp[1]);
-        } else {
-          tv = tMaxV[2]; vc[2] += stp[2]; tMaxV[2] += tDeltaV[2]; n = step_face(2, stp[2]);
-        }
-        if (tv > tExit) { o.steps = steps; return o; }
-        if (vc[0] < lo[0] || vc[1] < lo[1] || vc[2] < lo[2] || vc[0] > lo[0] + 7 || vc[1] > lo[1] + 7 ||
-            vc[2] > lo[2] + 7)
-          break;
-      }
-    }
-    if (tMaxB[0] < tMaxB[1] && tMaxB[0] < tMaxB[2]) {
-      t = tMaxB[0]; bc[0] += stp[0]; tMaxB[0] += tDeltaB[0]; normal = step_face(0, stp[0]);
-    } else if (tMaxB[1] < tMaxB[2]) {
-      t = tMaxB[1]; bc[1] += stp[1]; tMaxB[1] += tDeltaB[1]; normal = step_face(1, stp[1]);
-    } else {
-      t = tMaxB[2]; bc[2] += stp[2]; tMaxB[2] += tDeltaB[2]; normal = step_face(2, stp[2]);
-    }
-    if (t > tExit || bc[0] < 0 || bc[1] < 0 || bc[2] < 0 || bc[0] >= w.bd[0] || bc[1] >= w.bd[1] || bc[2] >= w.bd[2])
-      break;
-  }
-  o.steps = steps;
-  return o;
-}
+using px::trace_solid;
 
 // Segment DDA of particleSim, sim.wgsl:1110-1145.  A start inside a solid
 // reports face 6; a hit reports the solid cell entered, t of the crossing and

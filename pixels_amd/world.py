@@ -309,6 +309,27 @@ class Simulation:
     def island_pass(self) -> None:
         self.gw.check(self.lib.px_island_pass(self.gw.ctx))
 
+    def set_camera(self, cam: np.ndarray) -> None:
+        """The Camera block the player pass owns (shared.wgsl:22-30): 64
+        floats, eye (m) at [32:35], look at [36:39]; the shoot stage reads them."""
+        cam = np.ascontiguousarray(cam, np.float32)
+        assert cam.size == N.PX_CAMERA_FLOATS
+        self.gw.check(self.lib.px_set_camera(self.gw.ctx, N.ptr(cam, C.c_float)))
+
+    def read_flashes(self) -> np.ndarray:
+        """The flash ring (sim.wgsl:395-401): PX_MAX_FLASHES x 8 floats."""
+        out = np.zeros((N.PX_MAX_FLASHES, 8), np.float32)
+        self.gw.check(self.lib.px_read_flashes(self.gw.ctx, N.ptr(out, C.c_float)))
+        return out
+
+    def read_device_impacts(self, cap: int = 64):
+        """Impacts queued on the device and not yet taken by an island pass
+        (16 floats each) and C_IMPACTS."""
+        out = np.zeros((cap, 16), np.float32)
+        n = C.c_uint32(0)
+        self.gw.check(self.lib.px_read_device_impacts(self.gw.ctx, N.ptr(out, C.c_float), cap, C.byref(n)))
+        return out[:min(cap, int(n.value))].copy(), int(n.value)
+
     def encode(self, stages: int = N.STAGE_EDIT) -> None:
         """Simulation.encode (Simulation.ts:200-273): the hot-path stages, non-blocking."""
         self.gw.check(self.lib.px_sim_step(self.gw.ctx, stages))

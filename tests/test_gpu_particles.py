@@ -201,6 +201,6 @@ def test_particle_errors(small_scene):
         with pytest.raises(PxError):
             gw.check(gw.lib.px_read_particles(gw.ctx, (1 << 18) - 1, 2, np.zeros(2, N.PARTICLE_DTYPE).ctypes.data))
         with pytest.raises(PxError):
-            sim.encode(64)
+            sim.encode(128)  # no such stage
     finally:
         gw.close()
