@@ -338,6 +338,23 @@ int px_island_pass(px_ctx* ctx);
 #define PX_STAGE_PARTICLES 2u
 #define PX_STAGE_FRAME_BEGIN 4u
 int px_sim_step(px_ctx* ctx, uint32_t stages_mask);
+
+/* ---- a frame of stages as a HIP graph (no counterpart in the reference,
+ * whose WebGPU command buffers are encoded anew every frame; for callers that
+ * replay the same frame without host work per launch).
+ * px_capture_sim_step records px_sim_step(stages_mask) on the context's
+ * stream into a graph and instantiates it.  Each stage must have run once
+ * through px_sim_step (its buffers are allocated on first use), else
+ * PX_ERR_STATE; a recording that holds a memset node is refused.  The
+ * uniforms, the limits and the island mode are those at capture (kernel
+ * arguments); everything else is read from device memory, so each launch
+ * continues from the world, queues and counters the device holds.  Blocking.
+ * px_graph_launch enqueues one replay on the context's stream
+ * (asynchronous); px_graph_destroy frees the graph. */
+typedef struct px_graph px_graph;
+int px_capture_sim_step(px_ctx* ctx, uint32_t stages_mask, px_graph** out);
+int px_graph_launch(px_ctx* ctx, px_graph* graph);
+int px_graph_destroy(px_graph* graph);
 /* The edit stages run until no work is pending (the flood and the island
  * search of the reference iterate over frames until their queues empty and
  * C_S_STATE is 0; here one island pass reaches that state), then waits for

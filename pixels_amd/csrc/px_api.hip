@@ -657,7 +657,7 @@ int px_island_pass(px_ctx* c) {
   return px_launch_island_pass(c, true);
 }
 
-int px_sim_step(px_ctx* c, uint32_t stages_mask) {
+static int sim_step(px_ctx* c, uint32_t stages_mask) {
   if (!c) return PX_ERR_ARG;
   const uint32_t known = PX_STAGE_EDIT | PX_STAGE_PARTICLES | PX_STAGE_FRAME_BEGIN | PX_STAGE_WRITES |
                          PX_STAGE_RESOLVE | PX_STAGE_CHUNKS | PX_STAGE_SHOOT;
@@ -685,6 +685,100 @@ int px_sim_step(px_ctx* c, uint32_t stages_mask) {
     if (int rc = px_launch_particle_step(c)) return rc;
   }
   if (stages_mask & PX_STAGE_WRITES) return px_launch_writes(c);
+  return PX_OK;
+}
+
+int px_sim_step(px_ctx* c, uint32_t stages_mask) {
+  const int rc = sim_step(c, stages_mask);
+  if (rc == PX_OK && !c->capturing) c->stages_warm |= stages_mask;
+  return rc;
+}
+
+struct px_graph {
+  px_ctx* ctx;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  bool labels_valid;  // the host's label state after the recorded stages
+};
+
+int px_capture_sim_step(px_ctx* c, uint32_t stages_mask, px_graph** out) {
+  if (!c || !out) return PX_ERR_ARG;
+  *out = nullptr;
+  if (stages_mask & ~c->stages_warm)
+    return px_fail(c, PX_ERR_STATE, "px_capture_sim_step",
+                   "run these stages once with px_sim_step first (their buffers are allocated on first use)");
+  if (c->prof) return px_fail(c, PX_ERR_STATE, "px_capture_sim_step", "profiling is on");
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
+  // host state the recording changes although nothing runs yet
+  const bool lv = c->labels_valid, md = c->masks_dirty, dd = c->density_dirty;
+  const uint32_t mg = c->mask_gen, smg = c->sub_mask_gen;
+  PX_HIP_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  const int rc = sim_step(c, stages_mask);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  const bool lv_after = c->labels_valid;
+  c->labels_valid = lv;
+  c->masks_dirty = md;
+  c->density_dirty = dd;
+  c->mask_gen = mg;
+  c->sub_mask_gen = smg;
+  if (rc != PX_OK || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != PX_OK) return rc;
+    return px_fail(c, PX_ERR_HIP, "px_capture_sim_step", hipGetErrorString(ec));
+  }
+  // memset nodes are refused: every clear of the recorded stages is a kernel
+  size_t nn = 0;
+  (void)hipGraphGetNodes(g, nullptr, &nn);
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn) (void)hipGraphGetNodes(g, nodes.data(), &nn);
+  for (size_t k = 0; k < nn; k++) {
+    hipGraphNodeType t;
+    if (hipGraphNodeGetType(nodes[k], &t) != hipSuccess || t == hipGraphNodeTypeMemset) {
+      (void)hipGraphDestroy(g);
+      return px_fail(c, PX_ERR_STATE, "px_capture_sim_step", "the recorded stages hold a memset node");
+    }
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return px_fail(c, PX_ERR_HIP, "px_capture_sim_step", hipGetErrorString(ei));
+  }
+  px_graph* G = new (std::nothrow) px_graph;
+  if (!G) {
+    (void)hipGraphExecDestroy(x);
+    (void)hipGraphDestroy(g);
+    return px_fail(c, PX_ERR_HIP, "px_capture_sim_step", "out of host memory");
+  }
+  G->ctx = c;
+  G->graph = g;
+  G->exec = x;
+  G->labels_valid = lv_after;
+  *out = G;
+  return PX_OK;
+}
+
+int px_graph_launch(px_ctx* c, px_graph* G) {
+  if (!c || !G || G->ctx != c) return PX_ERR_ARG;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  PX_HIP_CHECK(c, hipGraphLaunch(G->exec, c->stream));
+  // the frame changed the world: masks and density are rebuilt before the
+  // next eager use
+  c->labels_valid = G->labels_valid;
+  c->masks_dirty = c->density_dirty = true;
+  return PX_OK;
+}
+
+int px_graph_destroy(px_graph* G) {
+  if (!G) return PX_OK;
+  (void)hipSetDevice(G->ctx->device);
+  (void)hipGraphExecDestroy(G->exec);
+  (void)hipGraphDestroy(G->graph);
+  delete G;
   return PX_OK;
 }
 

@@ -2792,7 +2792,8 @@ int px_launch_writes(px_ctx* c) {
   const uint64_t ncol = (uint64_t)c->dims[0] * c->dims[2];
   uint32_t bits = 1;
   while (bits < 32 && (1ull << bits) < ncol) bits++;
-  if (int rc = px_sort_u64(c, c->d_wkeys, c->d_wsorted, nw, 32u + bits)) return rc;
+  // from bit 32: the keys are in list order already and the sort is stable
+  if (int rc = px_sort_u64(c, c->d_wkeys, c->d_wsorted, nw, 32u, 32u + bits)) return rc;
   hipLaunchKernelGGL(k_write_apply, G, W, 0, s, e);
   hipLaunchKernelGGL(k_write_maint, dim3(kEditBlocks), dim3(64), 0, s, e);
   hipLaunchKernelGGL(k_write_finish, dim3(1), dim3(64), 0, s, e);

@@ -104,6 +104,8 @@ struct px_ctx {
   uint32_t* d_bcount = nullptr;      // per-block list lengths of the edit pass
   uint32_t* d_pane_cells = nullptr;  // pool cells holding pane voxels
   bool labels_valid = false;
+  uint32_t stages_warm = 0;  // px_sim_step stages run eagerly at least once (buffers exist)
+  bool capturing = false;    // px_capture_sim_step is recording the stream
   uint32_t* d_edit_ctr = nullptr;    // edit-pass counters
   // component labelling
   uint32_t* d_par = nullptr;     // union-find parent per voxel key (node ids are keys)
@@ -278,7 +280,9 @@ int px_raster_finish(px_ctx* c, PxRaster* R);
 int px_writes_alloc(px_ctx* c);
 void px_writes_free(px_ctx* c);
 int px_launch_writes(px_ctx* c);
-int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, uint32_t end_bit);
+// stable radix sort of n 64-bit keys by bits [begin_bit, end_bit) (px_sort.hip)
+int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, uint32_t begin_bit,
+                uint32_t end_bit);
 // island extraction and falling chunks (px_edit.hip)
 int px_chunks_alloc(px_ctx* c);
 void px_chunks_free(px_ctx* c);

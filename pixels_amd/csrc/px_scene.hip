@@ -8,7 +8,7 @@
 //   k_prim_pairs<0/1>  one workgroup per primitive: count, then emit the
 //                      (brick, primitive) pairs its brick-centre test keeps;
 //   radix sort         pairs by (brick, primitive): each brick's primitives
-//                      in order (rocPRIM, px_sort.hip);
+//                      in order (stable LSD radix sort, px_sort.hip);
 //   k_raster           one wavefront per brick (launched per pair, the first
 //                      pair of each brick does the work), the 512 voxels in registers
 //                      (lane = row y | z<<3, 8 voxels in x): the brick test,
@@ -403,7 +403,7 @@ int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, ui
       PX_HIP_CHECK(c, hipGetLastError());
       uint32_t cbits = 1;
       while (cbits < 32 && (1ull << cbits) < ncell) cbits++;
-      if (int rc = px_sort_u64(c, R->d_pairs, R->d_sorted, (uint32_t)npairs, 32u + cbits)) return rc;
+      if (int rc = px_sort_u64(c, R->d_pairs, R->d_sorted, (uint32_t)npairs, 0u, 32u + cbits)) return rc;
     }
   }
   // one staging brick per pair: the brick at its first pair's index

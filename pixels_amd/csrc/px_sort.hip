@@ -1,17 +1,128 @@
-// px_sort.hip - device radix sort of 64-bit keys (rocPRIM), used to group
-// the voxel write list by world column in list order.  Kept in its own
-// translation unit so the rocPRIM templates compile once.
-#include <rocprim/device/device_radix_sort.hpp>
+// px_sort.hip - stable LSD radix sort of 64-bit keys on gfx950, 8 bits per
+// pass over bits [begin_bit, end_bit):
+//   k_radix_hist     per tile of 1024 keys, the count of each digit, stored
+//                    digit-major (hist[digit * tiles + tile]);
+//   k_radix_scan     one exclusive scan of that table: the output position of
+//                    each (digit, tile) run;
+//   k_radix_scatter  each tile again, in key order: a key's rank among the
+//                    earlier keys of its tile with the same digit comes from
+//                    eight wavefront ballots (lanes of equal digit) plus the
+//                    counts of the earlier wavefronts and chunks, so equal
+//                    digits keep their input order.
+// Used to group the voxel write list by world column (keys column << 32 |
+// list index, sorted from bit 32 so that the list order stays inside a
+// column) and the scene builder's (cell, primitive) pairs.  Only kernel
+// launches once the temporary buffer has its size: no memset, no host wait,
+// so a caller may capture the sort in a graph.
+#include <algorithm>
 
 #include "px_internal.h"
 
-// Sorts n keys from d_in into d_out over bits [0, end_bit), on the context's
-// stream.  The temporary storage is the context's, grown on demand (the
-// first call for a size allocates; later calls do not).
-int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, uint32_t end_bit) {
+namespace {
+
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortChunks = 4;
+constexpr uint32_t kSortTile = kSortThreads * kSortChunks;  // keys per tile
+constexpr uint32_t kScanThreads = 1024;
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t* __restrict__ keys, uint32_t n,
+                                                              uint32_t shift, uint32_t* __restrict__ hist,
+                                                              uint32_t tiles) {
+  __shared__ uint32_t s_h[256];
+  const uint32_t t = threadIdx.x;
+  s_h[t] = 0u;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (uint32_t c = 0; c < kSortChunks; c++) {
+    const uint32_t i = base + c * kSortThreads + t;
+    if (i < n) atomicAdd(&s_h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[t * tiles + blockIdx.x] = s_h[t];
+}
+
+// exclusive prefix sum of hist[0 .. total) in place, one workgroup: each
+// thread sums a contiguous run, the run sums are scanned in LDS, then each
+// thread writes its run's offsets
+__global__ __launch_bounds__(kScanThreads) void k_radix_scan(uint32_t* __restrict__ hist, uint32_t total) {
+  __shared__ uint32_t s[kScanThreads];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (total + kScanThreads - 1u) / kScanThreads;
+  const uint32_t lo = min(t * per, total), hi = min(lo + per, total);
+  uint32_t sum = 0u;
+  for (uint32_t i = lo; i < hi; i++) sum += hist[i];
+  s[t] = sum;
+  __syncthreads();
+  for (uint32_t o = 1; o < kScanThreads; o <<= 1) {
+    const uint32_t v = t >= o ? s[t - o] : 0u;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = s[t] - sum;
+  for (uint32_t i = lo; i < hi; i++) {
+    const uint32_t v = hist[i];
+    hist[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t* __restrict__ in,
+                                                                 uint64_t* __restrict__ out, uint32_t n,
+                                                                 uint32_t shift, const uint32_t* __restrict__ off,
+                                                                 uint32_t tiles) {
+  constexpr uint32_t kWaves = kSortThreads / 64u;
+  __shared__ uint32_t s_base[256];        // next output position of each digit for this tile
+  __shared__ uint32_t s_wcnt[kWaves][256];  // keys of each digit per wavefront, current chunk
+  const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  s_base[t] = off[t * tiles + blockIdx.x];
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (uint32_t c = 0; c < kSortChunks; c++) {
+    for (uint32_t w = 0; w < kWaves; w++) s_wcnt[w][t] = 0u;
+    const uint32_t i = base + c * kSortThreads + t;
+    const bool valid = i < n;
+    const uint64_t k = valid ? in[i] : 0ull;
+    const uint32_t d = (uint32_t)(k >> shift) & 255u;
+    uint64_t same = __ballot(valid);
+    for (int b = 0; b < 8; b++) {
+      const bool bit = ((d >> b) & 1u) != 0u;
+      const uint64_t m = __ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(same & lt);
+    __syncthreads();  // s_wcnt cleared, s_base of the previous chunk advanced
+    if (valid && rank == (uint32_t)__popcll(same) - 1u) s_wcnt[wave][d] = (uint32_t)__popcll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0u;
+      for (uint32_t w = 0; w < wave; w++) before += s_wcnt[w][d];
+      out[s_base[d] + before + rank] = k;
+    }
+    __syncthreads();
+    uint32_t add = 0u;
+    for (uint32_t w = 0; w < kWaves; w++) add += s_wcnt[w][t];
+    s_base[t] += add;
+  }
+}
+
+__global__ void k_copy_u64(const uint64_t* __restrict__ in, uint64_t* __restrict__ out, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[i];
+}
+
+}  // namespace
+
+// Sorts n keys from d_in into d_out by bits [begin_bit, end_bit), stably, on
+// the context's stream; d_in is left as it was.  The temporary storage (n
+// keys + the digit table) is the context's, grown on demand: the first call
+// for a size allocates and waits, later calls only launch kernels.
+int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, uint32_t begin_bit,
+                uint32_t end_bit) {
   if (n == 0) return PX_OK;
-  size_t bytes = 0;
-  PX_HIP_CHECK(c, rocprim::radix_sort_keys(nullptr, bytes, d_in, d_out, n, 0u, end_bit, c->stream));
+  const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
+  const uint32_t passes = end_bit > begin_bit ? (end_bit - begin_bit + 7u) / 8u : 0u;
+  const size_t key_bytes = ((size_t)n * 8 + 255) & ~(size_t)255;
+  const size_t bytes = key_bytes + (size_t)256 * tiles * 4;
   if (bytes > c->sort_tmp_bytes) {
     PX_HIP_CHECK(c, hipStreamSynchronize(c->stream));
     if (c->d_sort_tmp) (void)hipFree(c->d_sort_tmp);
@@ -20,7 +131,26 @@ int px_sort_u64(px_ctx* c, const uint64_t* d_in, uint64_t* d_out, uint32_t n, ui
     PX_HIP_CHECK(c, hipMalloc(&c->d_sort_tmp, bytes));
     c->sort_tmp_bytes = bytes;
   }
-  size_t have = c->sort_tmp_bytes;
-  PX_HIP_CHECK(c, rocprim::radix_sort_keys(c->d_sort_tmp, have, d_in, d_out, n, 0u, end_bit, c->stream));
+  uint64_t* tmp = reinterpret_cast<uint64_t*>(c->d_sort_tmp);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(c->d_sort_tmp) + key_bytes);
+  hipStream_t s = c->stream;
+  if (passes == 0) {
+    hipLaunchKernelGGL(k_copy_u64, dim3(std::min<uint32_t>((n + 255u) / 256u, 4096u)), dim3(256), 0, s, d_in, d_out,
+                       n);
+    PX_HIP_CHECK(c, hipGetLastError());
+    return PX_OK;
+  }
+  const uint64_t* src = d_in;
+  for (uint32_t p = 0; p < passes; p++) {
+    // the last pass lands in d_out
+    uint64_t* dst = ((passes - 1u - p) & 1u) == 0u ? d_out : tmp;
+    const uint32_t shift = begin_bit + 8u * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, src, n, shift, hist, tiles);
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(kScanThreads), 0, s, hist, 256u * tiles);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, src, dst, n, shift,
+                       (const uint32_t*)hist, tiles);
+    src = dst;
+  }
+  PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
 }

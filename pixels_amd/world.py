@@ -334,6 +334,13 @@ class Simulation:
         """Simulation.encode (Simulation.ts:200-273): the hot-path stages, non-blocking."""
         self.gw.check(self.lib.px_sim_step(self.gw.ctx, stages))
 
+    def capture(self, stages: int) -> "FrameGraph":
+        """px_capture_sim_step: the stages as a replayable HIP graph (each
+        stage must have run once through encode())."""
+        g = C.c_void_p()
+        self.gw.check(self.lib.px_capture_sim_step(self.gw.ctx, stages, C.byref(g)))
+        return FrameGraph(self, g)
+
     def run_to_quiescence(self) -> None:
         self.gw.check(self.lib.px_run_to_quiescence(self.gw.ctx))
 
@@ -400,3 +407,18 @@ class Simulation:
     def apply_writes(self) -> None:
         """writeArgs + writeRequest + alloc + writeApply + maint of the touched cells."""
         self.gw.check(self.lib.px_apply_writes(self.gw.ctx))
+
+
+class FrameGraph:
+    """A frame of px_sim_step stages recorded as a HIP graph."""
+
+    def __init__(self, sim: "Simulation", handle):
+        self.sim, self.handle = sim, handle
+
+    def launch(self) -> None:
+        self.sim.gw.check(self.sim.lib.px_graph_launch(self.sim.gw.ctx, self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.sim.lib.px_graph_destroy(self.handle)
+            self.handle = None
