@@ -658,21 +658,27 @@ def learned_leg(args):
     the structural floor) and on the 1280x352x960 frame scene.  Per scene
     and query kind (primary, shadow), a 16-64-64-1 BF16 MLP is trained on
     the GPU with Adam (lr 1e-3, batch 65,536, 12 epochs) on the 16.6M
-    (ray, step count) pairs that the plain DDA produces for 8 seeded cameras,
-    then used on a held-out camera: features + prediction + a stable global
-    sort by (direction face, one of 32 cost buckets) (one fused launch, a
-    per-key scan and a scatter) + the permuted launch, submitted by one call,
-    against the unsorted launch of the same rays; with the cost key alone and
-    the face alone as ablations."""
+    (ray, step count) pairs that the plain DDA produces for 128 seeded
+    480x270 cameras, then used on a held-out 1920x1080 camera: features +
+    prediction + a stable global sort by (direction face, one of 32 cost
+    buckets) + the permuted launch, against the unsorted launch of the same
+    rays; with the cost key alone and the face alone as ablations.  Shadow
+    rays are compared in two plain orders: pixel then slot, and the frame's
+    own queue order (64-pixel region, slot, pixel).  On the frame scene the
+    held-out camera's every cast shadow ray (12.1M) is also timed through the
+    frame launch itself, with the predictor's order inside each region, and
+    the primary + shadow rays through the batched launch with and without
+    the learned orders."""
     import torch
     from pixels_amd import GpuWorld, Scene, Simulation
     from pixels_amd import _native as N
-    from pixels_amd.learned import CostModel, bin_rays, log2_steps, wave_cost
+    from pixels_amd.learned import CostModel, bin_rays, log2_steps, trace_perm, wave_cost
     from pixels_amd.scene import LARGE, SMALL, SMALL_FLOOR
-    from pixels_amd.workloads import shadow_queries
+    from pixels_amd.workloads import frame_order, region_wave_cost, shadow_queries
 
     dev = torch.device("cuda:0")
     W, H = 1920, 1080
+    TW, TH, N_TRAIN_CAMS = 480, 270, 128
     out = {"scenes": [], "train_pairs": 0}
 
     def tr(sim, gw, kind, rays_np, d_rays):
@@ -682,42 +688,41 @@ def learned_leg(args):
         gw.sync()
         return o
 
+    def prof_ms(gw, fn, slots, reps):
+        gw.profile_begin()
+        for _ in range(reps):
+            fn()
+        p = gw.profile_end()
+        return sum(p.get(k, (0.0, 0))[0] for k in slots) / reps
+
     t_all = time.time()
     for kind_scene, seed in ((SMALL, 1), (SMALL_FLOOR, 3), (LARGE, 2)):
         scene = Scene.build(kind_scene, seed=seed)
         gw = GpuWorld(scene)
         sim = Simulation(gw)
+        frame_scene = kind_scene is LARGE
 
-        def rays_of(cam):
-            prim = scene.camera_rays(W, H, 90.0, seed=cam, tile=8)
+        def rays_of(cam, w, h, all_shadow=False):
+            prim = scene.camera_rays(w, h, 90.0, seed=cam, tile=8)
             dp = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
             ph = tr(sim, gw, 0, prim, dp).cpu().numpy().view(N.HIT_DTYPE)
-            u, lights = scene.frame_inputs(W, H, seed=cam)
-            sh = shadow_queries(scene, prim, ph, u, lights, W * H)
+            u, lights = scene.frame_inputs(w, h, seed=cam)
+            sh, pix, slot = shadow_queries(scene, prim, ph, u, lights, None if all_shadow else w * h, codes=True)
             ds = torch.from_numpy(sh.view(np.uint8).copy()).to(dev)
-            return {0: (prim, dp), 1: (sh, ds)}
+            return {0: (prim, dp), 1: (sh, ds)}, (pix, slot, u, lights)
 
-        # cameras: the first 9 seeded ones (the held-out one excluded) whose
-        # primaries hit something that casts shadow rays; 8 train, the 9th
-        # validates the choice between networks of different initialisation
-        held_out = seed * 100 + 9
+        # training cameras: 128 seeded 480x270 views (many viewpoints, so the
+        # network learns the scene rather than a few cameras); one 1920x1080
+        # validation camera picks between initialisations; the held-out
+        # 1920x1080 camera is measured
+        held_out = seed * 1000 + 999
         data = {0: [], 1: []}
         probe = CostModel(gw, seed=args.seed)
-        cams = 0
-        val = None
-        for cam in range(seed * 100 + 1, seed * 100 + 40):
-            if cams == 8:
-                if val is None:
-                    try:
-                        val = rays_of(cam) if cam != held_out else None
-                    except IndexError:
-                        pass
-                    continue
-                break
-            if cam == held_out:
-                continue
+        cams, cam = 0, seed * 1000
+        while cams < N_TRAIN_CAMS and cam < seed * 1000 + 4 * N_TRAIN_CAMS:
+            cam += 1
             try:
-                rk = rays_of(cam)
+                rk, _ = rays_of(cam, TW, TH)
             except IndexError:  # no shadow rays from this camera
                 continue
             cams += 1
@@ -728,8 +733,10 @@ def learned_leg(args):
                 steps = tr(sim, gw, kind, r, d).cpu().numpy().view(N.HIT_DTYPE)["steps"]
                 data[kind].append((f, torch.from_numpy(log2_steps(steps)).to(dev)))
         probe.close()
-        res = {"scene": list(scene.dims), "train_cameras": cams}
-        held = rays_of(held_out)
+        val, _ = rays_of(seed * 1000 + 998, W, H)
+        res = {"scene": list(scene.dims), "train_cameras": cams, "train_camera_size": [TW, TH]}
+        held, (h_pix, h_slot, h_u, h_lights) = rays_of(held_out, W, H, all_shadow=frame_scene)
+        models = {}
         for kind in (0, 1):
             # one network per query kind, trained on that kind's pairs; of
             # four initialisations, the one whose binned launch is fastest on
@@ -751,9 +758,6 @@ def learned_leg(args):
             for init in range(4):
                 model = CostModel(gw, seed=args.seed + 1000 * init)
                 gw.sync()
-                # one pair of events around the training loop on the library's
-                # stream (no per-step timing scopes: thousands of live events
-                # slow the later event records)
                 ext = torch.cuda.ExternalStream(gw.stream)
                 ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 t0 = time.time()
@@ -768,11 +772,8 @@ def learned_leg(args):
                 train_gpu_ms = ev0.elapsed_time(ev1)
                 model.trace_sorted(kind, vd, nv, v_perm, v_out, faces=True)
                 gw.sync()
-                gw.profile_begin()
-                for _ in range(3):
-                    model.trace_sorted(kind, vd, nv, v_perm, v_out, faces=True)
-                vp = gw.profile_end()
-                v_ms = (vp[N.PROF_TRACE][0] + vp.get(N.PROF_MLP, (0.0, 0))[0]) / 3
+                v_ms = prof_ms(gw, lambda: model.trace_sorted(kind, vd, nv, v_perm, v_out, faces=True),
+                               (N.PROF_TRACE, N.PROF_MLP), 3)
                 cands.append((v_ms, init, model, first, last, train_s, train_gpu_ms))
             cands.sort(key=lambda c: c[0])
             v_ms, init, model, first, last, train_s, train_gpu_ms = cands[0]
@@ -787,16 +788,14 @@ def learned_leg(args):
             pred = torch.empty(n, dtype=torch.float32, device=dev)
             perm = torch.empty(n, dtype=torch.int32, device=dev)
 
-            def run_plain():
-                sim.trace_device(kind, d, n, plain)
+            def run_plain(dd=d):
+                sim.trace_device(kind, dd, n, plain)
 
             def run_binned(faces=True, buckets=32):
                 model.trace_sorted(kind, d, n, perm, binned, buckets=buckets, lo=0.0, hi=9.0, faces=faces)
             run_plain()
             run_binned()
             gw.sync()
-            # without the face key the fused sort gives the separate kernels'
-            # order (prediction read back for the MAE)
             model.features(kind, d, n, f)
             model.predict(f, n, pred)
             ref_perm = torch.empty(n, dtype=torch.int32, device=dev)
@@ -806,22 +805,36 @@ def learned_leg(args):
             same_order = bool(torch.equal(ref_perm, perm))
             run_binned()
             gw.sync()
+            hits = plain.cpu().numpy().view(N.HIT_DTYPE)
+            steps = hits["steps"].astype(np.int64)
+            pr = pred.cpu().numpy()
+            pm = perm.cpu().numpy().astype(np.int64)
+            # the frame's queue order of the same shadow rays, and the
+            # predictor's order inside each of its regions (stable by bucket)
+            orders = {}
+            if kind == 1:
+                fo = frame_order(h_pix, h_slot)
+                bucket = np.clip(np.floor(pr * (32.0 / 9.0)), 0, 31).astype(np.int64)
+                reg = h_pix // 64
+                fr = fo[np.lexsort((np.arange(fo.size), bucket[fo], reg[fo]))]
+                orders = {"frame": fo, "frame_region_binned": fr}
+            d_orders = {k: torch.from_numpy(v.astype(np.int32)).to(dev) for k, v in orders.items()}
             # interleaved rounds, medians: an occasional slow round (seen
             # either way) does not decide the comparison
             reps, rounds = 3, 7
             p_r, t_r, s_r = [], [], []
+            o_r = {k: [] for k in orders}
             abl = {"cost_only": ([], []), "faces_only": ([], [])}
             for _ in range(rounds):
-                gw.profile_begin()
-                for _ in range(reps):
-                    run_plain()
-                p_r.append(gw.profile_end()[N.PROF_TRACE][0] / reps)
+                p_r.append(prof_ms(gw, run_plain, (N.PROF_TRACE,), reps))
                 gw.profile_begin()
                 for _ in range(reps):
                     run_binned()
                 prof = gw.profile_end()
                 t_r.append(prof[N.PROF_TRACE][0] / reps)
                 s_r.append(prof.get(N.PROF_MLP, (0.0, 0))[0] / reps)
+                for k, dperm in d_orders.items():
+                    o_r[k].append(prof_ms(gw, lambda: trace_perm(gw, kind, d, dperm, n, plain), (N.PROF_TRACE,), reps))
                 # ablations: the learned cost key alone, the direction alone
                 for name, kw in (("cost_only", dict(faces=False)), ("faces_only", dict(buckets=1))):
                     gw.profile_begin()
@@ -836,16 +849,14 @@ def learned_leg(args):
             ablation = {name: {"speedup": round(p_ms / (float(np.median(t)) + float(np.median(sr))), 3),
                                "speedup_trace_only": round(p_ms / float(np.median(t)), 3)}
                         for name, (t, sr) in abl.items()}
-            hits = plain.cpu().numpy().view(N.HIT_DTYPE)
-            steps = hits["steps"].astype(np.int64)
-            pr = pred.cpu().numpy()
-            pm = perm.cpu().numpy().astype(np.int64)
-            res["primary" if kind == 0 else "shadow"] = {
+            rec = {
                 "rays": n, "train_pairs": n_train, "epochs": epochs, "batch": bs, "inits_tried": len(cands),
                 "validation_binned_ms": [round(c[0], 4) for c in sorted(cands, key=lambda c: c[1])],
                 "init_chosen": init,
                 "loss_first": round(first, 4), "loss_last": round(last, 4), "train_wall_s": round(train_s, 3),
                 "train_gpu_ms": round(train_gpu_ms, 3),
+                "held_out_log2_mse": round(float(np.mean((pr - log2_steps(steps)) ** 2)), 4),
+                "plain_order": "pixel, then slot" if kind == 1 else "camera tiles of 8x8",
                 "plain_grays_s": round(n / (p_ms * 1e-3) / 1e9, 4),
                 "binned_grays_s": round(n / (b_ms * 1e-3) / 1e9, 4),
                 "binned_trace_only_grays_s": round(n / (b_trace_ms * 1e-3) / 1e9, 4),
@@ -854,11 +865,31 @@ def learned_leg(args):
                 "binned_trace_ms_range": [round(min(t_r), 4), round(max(t_r), 4)],
                 "speedup": round(p_ms / b_ms, 3), "speedup_trace_only": round(p_ms / b_trace_ms, 3),
                 "mae_steps": round(float(np.mean(np.abs(np.exp2(pr) - np.maximum(steps, 1)))), 2),
+                "mae_steps_constant_median": round(float(np.mean(np.abs(np.median(np.maximum(steps, 1)) -
+                                                                        np.maximum(steps, 1)))), 2),
                 "mean_steps": round(float(steps.mean()), 2),
                 "wave_cost_plain": wave_cost(steps), "wave_cost_binned": wave_cost(steps[pm]),
                 "bit_identical": bool(torch.equal(plain, binned)), "order_equals_separate_kernels": same_order,
                 "keys": "direction face x 32 cost buckets", "ablation": ablation}
-            model.close()
+            for k, o in orders.items():
+                ms = float(np.median(o_r[k]))
+                rec["order_" + k] = {"trace_ms": round(ms, 4), "grays_s": round(n / (ms * 1e-3) / 1e9, 4),
+                                     "wave_cost": wave_cost(steps[o]),
+                                     "region_wave_cost": region_wave_cost(steps[o], (h_pix // 64)[o])}
+            if orders:
+                rec["order_frame_region_binned"]["sort_ms_not_included"] = True
+                # the ideal in-region order (true step counts) bounds what any predictor can give there
+                reg = h_pix // 64
+                fo = orders["frame"]
+                ideal = fo[np.lexsort((np.arange(fo.size), -steps[fo], reg[fo]))]
+                rec["region_wave_cost_true_cost_order"] = region_wave_cost(steps[ideal], reg[ideal])
+            res["primary" if kind == 0 else "shadow"] = rec
+            models[kind] = model
+        if frame_scene:
+            res["frame_and_batch"] = frame_and_batch_learned(args, scene, gw, sim, held, (h_pix, h_slot, h_u, h_lights),
+                                                            models)
+        for m in models.values():
+            m.close()
         del data
         out["scenes"].append(res)
         log("learned binning %s: %s" % (scene.dims, json.dumps(res)))
@@ -866,6 +897,79 @@ def learned_leg(args):
     out["wall_s"] = round(time.time() - t_all, 2)
     return out
 
+
+def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
+    """On the frame scene's held-out camera: its every cast shadow ray
+    through the frame launch itself (k_frame_primary + k_trace_shadow_queue),
+    and the primary + shadow rays through the batched launch in index order
+    and in the learned orders (px_mlp_sort_rays per kind, then
+    px_trace_batch_perm)."""
+    import torch
+    from pixels_amd import _native as N
+    from pixels_amd.workloads import frame_order
+    dev = torch.device("cuda:0")
+    pix, slot, u, lights = codes
+    (prim, d_prim), (sh, d_sh) = held[0], held[1]
+    n = prim.shape[0]
+    gw.set_uniforms(u)
+    gw.set_lights(lights)
+    sim.set_frame_layout(True)  # the measured frame's slot-planar output
+    slots = sim.frame_slots()
+    dp = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+    ds = torch.empty(n * slots * 16, dtype=torch.uint8, device=dev)
+    nc = sim.trace_frame_device(d_prim, n, dp, ds, count=True)
+    gw.sync()
+    fr = []
+    for _ in range(7):
+        gw.profile_begin()
+        for _ in range(3):
+            sim.trace_frame_device(d_prim, n, dp, ds)
+        p = gw.profile_end()
+        fr.append((p[N.PROF_PRIMARY][0] / 3, p[N.PROF_SHADOW][0] / 3))
+    f_prim, f_sh = float(np.median([a for a, _ in fr])), float(np.median([b for _, b in fr]))
+    res = {"frame": {"shadow_rays": int(nc), "materialized_shadow_rays": int(sh.shape[0]),
+                     "shadow_kernel_ms": round(f_sh, 4), "shadow_grays_s": round(nc / (f_sh * 1e-3) / 1e9, 4),
+                     "primary_kernel_ms": round(f_prim, 4)}}
+    # batched: the held-out primaries + the shadow rays in the frame's order
+    fo = torch.from_numpy(frame_order(pix, slot).astype(np.int64)).to(dev)
+    sh_rays = d_sh.view(-1, 32)[fo].reshape(-1).contiguous()
+    ns = sh.shape[0]
+    rays = {0: (d_prim, n), 1: (sh_rays, ns)}
+    outs = {0: torch.empty(n * 16, dtype=torch.uint8, device=dev), 1: torch.empty(ns * 16, dtype=torch.uint8, device=dev)}
+    outs_b = {0: torch.empty(n * 16, dtype=torch.uint8, device=dev), 1: torch.empty(ns * 16, dtype=torch.uint8, device=dev)}
+    q = sim.query_set(rays, outs)
+    qb = sim.query_set(rays, outs_b)
+    perms = {k: torch.empty(rays[k][1], dtype=torch.int32, device=dev) for k in (0, 1)}
+
+    def learned_batch():
+        for k in (0, 1):
+            models[k].sort_rays(k, rays[k][0], rays[k][1], perms[k], buckets=32, lo=0.0, hi=9.0, faces=True)
+        sim.trace_batch_perm(qb, perms)
+    sim.trace_batch(q)
+    learned_batch()
+    gw.sync()
+    same = all(torch.equal(outs[k], outs_b[k]) for k in (0, 1))
+    pb, lb, ls = [], [], []
+    for _ in range(7):
+        gw.profile_begin()
+        for _ in range(3):
+            sim.trace_batch(q)
+        pb.append(gw.profile_end()[N.PROF_BATCH][0] / 3)
+        gw.profile_begin()
+        for _ in range(3):
+            learned_batch()
+        p = gw.profile_end()
+        lb.append(p[N.PROF_BATCH][0] / 3)
+        ls.append(p.get(N.PROF_MLP, (0.0, 0))[0] / 3)
+    p_ms, l_ms, s_ms = float(np.median(pb)), float(np.median(lb)), float(np.median(ls))
+    tot = n + ns
+    res["batch"] = {"queries": {"primary": n, "shadow": int(ns)}, "shadow_order": "the frame's queue order",
+                    "plain_ms": round(p_ms, 4), "learned_trace_ms": round(l_ms, 4), "learned_sort_ms": round(s_ms, 4),
+                    "plain_grays_s": round(tot / (p_ms * 1e-3) / 1e9, 4),
+                    "learned_grays_s": round(tot / ((l_ms + s_ms) * 1e-3) / 1e9, 4),
+                    "speedup": round(p_ms / (l_ms + s_ms), 3), "speedup_trace_only": round(p_ms / l_ms, 3),
+                    "bit_identical": bool(same)}
+    return res
 
 if __name__ == "__main__":
     sys.exit(main())

@@ -280,6 +280,12 @@ typedef struct px_query_set {
   uint32_t n[4];
 } px_query_set;
 int px_trace_batch(px_ctx* ctx, const px_query_set* q);
+/* The same launch with kind k traced in the order d_perm[k] (n[k] source
+ * indices, a permutation; NULL = index order): position j of the kind's
+ * chunk space traces ray d_perm[k][j] and writes its hit to out[k] at that
+ * index, so the hits equal px_trace_batch's.  With the orders
+ * px_mlp_sort_rays gives, this is the learned-binned batched traversal. */
+int px_trace_batch_perm(px_ctx* ctx, const px_query_set* q, const uint32_t* const d_perm[4]);
 /* Launch shape of px_trace_batch: block = threads per workgroup (64, 128 or
  * 256), blocks = workgroups (0 = six wavefronts per SIMD of the device),
  * chunk = rays per claim (>= 1). */

@@ -153,6 +153,7 @@ SIGNATURES = {
     "px_read_chunks": (C.c_int, [_P, _U32P, _U32P, _U32, _U32P]),
     "px_read_device_impacts": (C.c_int, [_P, _FP, _U32, _U32P]),
     "px_set_camera": (C.c_int, [_P, _FP]),
+    "px_trace_batch_perm": (C.c_int, [_P, _P, _P]),
     "px_capture_sim_step": (C.c_int, [_P, _U32, C.POINTER(_P)]),
     "px_graph_launch": (C.c_int, [_P, _P]),
     "px_graph_destroy": (C.c_int, [_P]),

@@ -478,7 +478,15 @@ int px_trace_batch(px_ctx* c, const px_query_set* q) {
   int rc = check_query_set(c, q, "px_trace_batch");
   if (rc) return rc;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  return px_launch_batch(c, q);
+  return px_launch_batch(c, q, nullptr);
+}
+
+int px_trace_batch_perm(px_ctx* c, const px_query_set* q, const uint32_t* const d_perm[4]) {
+  if (!c) return PX_ERR_ARG;
+  int rc = check_query_set(c, q, "px_trace_batch_perm");
+  if (rc) return rc;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_launch_batch(c, q, d_perm);
 }
 
 int px_set_batch_shape(px_ctx* c, uint32_t block, uint32_t blocks, uint32_t chunk) {

@@ -87,7 +87,7 @@ struct px_ctx {
   uint32_t batch_block = 256, batch_blocks = 0, batch_chunk = 256;
   uint32_t* d_batch_cursors = nullptr;
   hipGraphExec_t batch_exec = nullptr;
-  unsigned char batch_key[160] = {0};
+  unsigned char batch_key[192] = {0};
   uint32_t* d_counters = nullptr;  // small scratch counters
   // edit chain
   float* d_impacts = nullptr;       // the impact queue: 64 records of 16 floats, C_IMPACTS of them used
@@ -213,7 +213,7 @@ int px_launch_probe(px_ctx* c, const int32_t o[3], const int32_t s[3], uint32_t*
 int px_fixed_scratch_init(px_ctx* c);
 int px_launch_frame_noalloc(px_ctx* c, const px_ray* d_primary, uint32_t n, px_hit* d_prim, px_hit* d_shadow,
                             uint32_t* n_cast);
-int px_launch_batch(px_ctx* c, const px_query_set* q);
+int px_launch_batch(px_ctx* c, const px_query_set* q, const uint32_t* const* perm);
 // px_render.hip
 int px_refresh_density(px_ctx* c);
 int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba);

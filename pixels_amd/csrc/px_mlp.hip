@@ -436,14 +436,24 @@ PX_DEV uint32_t pack_bf16(float a, float b) {
   return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 
-// Features of a ray (16, in [-1, 1]), bf16: origin / dims, unit direction,
-// super-cell occupancy at 8 points evenly spread over the ray's clipped
-// extent (capped at `horizon` voxels), log2 extent, and the kind.  r0, r1 are
-// the two halves of the 32-byte ray record.  Returned packed two per word,
-// feature 2k in the low half of word k.  The arithmetic is the hardware's
-// (v_rcp_f32, v_rsq_f32, v_log_f32, v_cvt_pk_bf16_f32) and branch-free:
-// features are inputs to the network, not outputs, and the same function
-// makes the training features and the ones prediction uses.
+// Features of a ray (16, in [-1, 1]), bf16.  None depends on where the
+// camera stands except the origin's height, so a network trained on some
+// cameras carries over to others:
+//   f0  log2(1 + entry t) over the world (0 when the origin is inside),
+//   f1  origin height y / dims.y,
+//   f2  log2(1 + clipped extent) (to the exit or max t, capped at `horizon`),
+//   f3..f5  unit direction,
+//   f6..f10  brick occupancy (grid != 0) at 0.5, 4, 8, 16, 32 voxels past the
+//            entry, f11..f14 super-cell occupancy at 64, 128, 256, 512
+//            voxels: +1 occupied, -1 empty or past the extent,
+//   f15 the kind.
+// The near samples tell how soon the ray meets geometry, the far ones how
+// cluttered its path stays - what the step count of march() follows.
+// r0, r1 are the two halves of the 32-byte ray record.  Returned packed two
+// per word, feature 2k in the low half of word k.  The arithmetic is the
+// hardware's (v_rcp_f32, v_rsq_f32, v_log_f32, v_cvt_pk_bf16_f32) and
+// branch-free: features are inputs to the network, not outputs, and the same
+// function makes the training features and the ones prediction uses.
 PX_DEV void ray_features(const PxWorldView& w, const px::MarchTables& T, float4 r0, float4 r1, const FeatK& K,
                          uint4& o0, uint4& o1) {
   const float ro[3] = {r0.x, r0.y, r0.z};
@@ -454,7 +464,6 @@ PX_DEV void ray_features(const PxWorldView& w, const px::MarchTables& T, float4 
   const float rl = len2 > 0.0f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
   float tmn = 0.0f, tmx = maxT;
   for (int a = 0; a < 3; a++) {
-    f[a] = __builtin_amdgcn_fmed3f(__builtin_fmaf(ro[a], K.sx[a], -1.0f), -1.0f, 1.0f);
     f[3 + a] = __builtin_amdgcn_fmed3f(rd[a] * rl, -1.0f, 1.0f);
     const float d = fabsf(rd[a]) < 1e-6f ? (rd[a] < 0.0f ? -1e-6f : 1e-6f) : rd[a];
     const float inv = __builtin_amdgcn_rcpf(d);
@@ -462,27 +471,35 @@ PX_DEV void ray_features(const PxWorldView& w, const px::MarchTables& T, float4 
     tmn = fmaxf(tmn, fminf(t0, t1));
     tmx = fminf(tmx, fmaxf(t0, t1));
   }
-  const float ext = fmaxf(fminf(tmx, tmn + K.horizon) - tmn, 0.0f);
-  // sample k at tmn + ext (k + 1/2) / 8, in super-cell units: A + B c_k
-  const bool any = ext > 0.0f;
-  float A[3], B[3];
-  for (int a = 0; a < 3; a++) {
-    A[a] = any ? __builtin_fmaf(rd[a], tmn, ro[a]) * 0.03125f : -1.0f;
-    B[a] = any ? rd[a] * ext * 0.03125f : 0.0f;
-  }
-  for (int k = 0; k < 8; k++) {
-    const float ck = ((float)k + 0.5f) * 0.125f;
-    uint32_t sc[3];
-    bool in = true;
+  const float len = len2 * rl;  // |rd|
+  const float ext = fmaxf(fminf(tmx, tmn + K.horizon) - tmn, 0.0f) * len;  // voxels
+  const float ent = tmn * len;
+  f[0] = fminf(__builtin_amdgcn_logf(1.0f + ent) * K.inv_log_h, 1.0f) * 2.0f - 1.0f;
+  f[1] = -1.0f;
+  f[2] = fminf(__builtin_amdgcn_logf(1.0f + ext) * K.inv_log_h, 1.0f) * 2.0f - 1.0f;
+  // sample k at `dist[k]` voxels past the entry: point = ro + rd (tmn + dist rl)
+  const float dist[10] = {0.5f, 4.0f, 8.0f, 16.0f, 32.0f, 64.0f, 128.0f, 256.0f, 512.0f, 2.0f};
+  for (int k = 0; k < 10; k++) {
+    const float t = __builtin_fmaf(dist[k], rl, tmn);
+    const bool along = ext > 0.0f && dist[k] <= ext;
+    const bool brick = k < 5 || k == 9;
+    const float sh = brick ? 0.125f : 0.03125f;  // brick or super-cell units
+    uint32_t c[3];
+    bool in = along;
     for (int a = 0; a < 3; a++) {
-      sc[a] = (uint32_t)(int32_t)floorf(__builtin_fmaf(B[a], ck, A[a]));
-      in = in && sc[a] < (uint32_t)w.sd[a];
+      c[a] = (uint32_t)(int32_t)floorf(__builtin_fmaf(rd[a], t, ro[a]) * sh);
+      in = in && c[a] < (uint32_t)(brick ? w.bd[a] : w.sd[a]);
     }
-    const uint32_t idx =
-        in ? sc[0] + __umul24((uint32_t)w.sd[0], sc[1] + __umul24((uint32_t)w.sd[1], sc[2])) : 0u;
-    f[6 + k] = in && ((T.sup[idx >> 5] >> (idx & 31u)) & 1u) ? 1.0f : -1.0f;
+    bool occ = false;
+    if (brick) {
+      const uint32_t idx = in ? c[0] + __umul24((uint32_t)w.bd[0], c[1] + __umul24((uint32_t)w.bd[1], c[2])) : 0u;
+      occ = in && ((w.brick_mask[idx >> 5] >> (idx & 31u)) & 1u);
+    } else {
+      const uint32_t idx = in ? c[0] + __umul24((uint32_t)w.sd[0], c[1] + __umul24((uint32_t)w.sd[1], c[2])) : 0u;
+      occ = in && ((T.sup[idx >> 5] >> (idx & 31u)) & 1u);
+    }
+    f[k == 9 ? 1 : 6 + k] = occ ? 1.0f : -1.0f;
   }
-  f[14] = fminf(__builtin_amdgcn_logf(1.0f + ext) * K.inv_log_h, 1.0f) * 2.0f - 1.0f;
   f[15] = K.kind;
   o0 = uint4{pack_bf16(f[0], f[1]), pack_bf16(f[2], f[3]), pack_bf16(f[4], f[5]), pack_bf16(f[6], f[7])};
   o1 = uint4{pack_bf16(f[8], f[9]), pack_bf16(f[10], f[11]), pack_bf16(f[12], f[13]), pack_bf16(f[14], f[15])};

@@ -242,6 +242,7 @@ constexpr uint32_t kCursorStride = 64;  // words between cursors (256 bytes)
 struct BatchArgs {
   const px_ray* rays[4];
   px_hit* out[4];
+  const uint32_t* perm[4];  // per kind, NULL = index order
   uint32_t n[4];
   uint32_t first_chunk[5];  // chunk index where each kind of `order` starts
   uint32_t order[4];        // processing order -> kind
@@ -271,7 +272,10 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_batch(PxWorldView 
       const uint32_t end = min(start + a.chunk, a.n[kind]);
       const px_ray* __restrict__ rays = a.rays[kind];
       px_hit* __restrict__ out = a.out[kind];
-      for (uint32_t i = start + lane; i < end; i += 64u) {
+      const uint32_t* __restrict__ perm = a.perm[kind];
+      for (uint32_t k = start + lane; k < end; k += 64u) {
+        // position k of the kind's order traces ray perm[k] (k without one)
+        const uint32_t i = perm ? perm[k] : k;
         float ro[3], rd[3], maxT;
         uint32_t flags;
         load_ray(rays + i, ro, rd, maxT, flags);
@@ -648,7 +652,7 @@ int px_launch_frame_noalloc(px_ctx* c, const px_ray* d_primary, uint32_t n, px_h
 // the short debris segments.
 static const uint32_t kBatchOrder[4] = {PX_RAY_SHOOT, PX_RAY_PRIMARY, PX_RAY_SHADOW, PX_RAY_SEGMENT};
 
-static BatchArgs batch_args(const px_ctx* c, const px_query_set* q) {
+static BatchArgs batch_args(const px_ctx* c, const px_query_set* q, const uint32_t* const* perm) {
   BatchArgs a;
   memset(&a, 0, sizeof(a));
   a.chunk = c->batch_chunk;
@@ -664,6 +668,7 @@ static BatchArgs batch_args(const px_ctx* c, const px_query_set* q) {
   for (int k = 0; k < 4; k++) {
     a.rays[k] = q->rays[k];
     a.out[k] = q->out[k];
+    a.perm[k] = perm ? perm[k] : nullptr;
     a.n[k] = q->n[k];
   }
   return a;
@@ -676,10 +681,10 @@ static uint32_t batch_blocks(const px_ctx* c) {
 
 // Captures [cursor reset, k_trace_batch] into a graph for this query set and
 // shape (kept while they stay the same) and launches it.
-int px_launch_batch(px_ctx* c, const px_query_set* q) {
+int px_launch_batch(px_ctx* c, const px_query_set* q, const uint32_t* const* perm) {
   int rc = px_refresh_march(c);
   if (rc) return rc;
-  const BatchArgs a = batch_args(c, q);
+  const BatchArgs a = batch_args(c, q, perm);
   if (a.total_chunks == 0u) return PX_OK;
   unsigned char key[sizeof(c->batch_key)];
   static_assert(sizeof(BatchArgs) + 3 * sizeof(uint32_t) <= sizeof(key), "batch key size");
@@ -745,10 +750,10 @@ int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, u
         c->batch_block = blocks_opt[bi];
         c->batch_blocks = c->n_cu * waves_opt[wi] * 64u / blocks_opt[bi];
         c->batch_chunk = chunk_opt[ci];
-        rc = px_launch_batch(c, q);  // capture + warm-up
+        rc = px_launch_batch(c, q, nullptr);  // capture + warm-up
         if (rc) break;
         if (hipEventRecord(e0, c->stream) != hipSuccess) rc = px_fail(c, PX_ERR_HIP, "px_autotune_batch", "event");
-        for (uint32_t r = 0; r < repeats && rc == PX_OK; r++) rc = px_launch_batch(c, q);
+        for (uint32_t r = 0; r < repeats && rc == PX_OK; r++) rc = px_launch_batch(c, q, nullptr);
         if (rc) break;
         if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
           rc = px_fail(c, PX_ERR_HIP, "px_autotune_batch", "event");

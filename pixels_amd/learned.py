@@ -88,6 +88,10 @@ class CostModel:
                                                    N.SORT_FACES if faces else 0, _dp(perm_t), _dp(out_t)))
 
     def train_step(self, feat_t, target_t, n: int, lr: float = 1e-3, want_loss: bool = False):
+        """One Adam step, asynchronous on the library's stream unless
+        want_loss: keep feat_t / target_t alive (or call gw.sync()) until it
+        has run, since torch may reuse their memory as soon as they are
+        freed."""
         self.gw.after_torch()
         loss = C.c_float(0.0)
         self.gw.check(self.lib.px_mlp_train_step(self.h, _dp(feat_t), _dp(target_t), n, lr,

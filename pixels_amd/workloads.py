@@ -19,13 +19,15 @@ from ._native import HIT_DTYPE, RAY_DTYPE
 
 
 def shadow_queries(scene, prim_rays: np.ndarray, prim_hits: np.ndarray, u: np.ndarray, lights: np.ndarray,
-                   n_max: int) -> np.ndarray:
+                   n_max, codes: bool = False):
     """Shadow rays of shade() (raymarch.wgsl:323-356) for the primary hits
     that cast them (hit, non-emissive), in pixel-then-slot order, repeated or
-    cut to exactly n_max.  Origin P + N * 0.5 (P = ro + rd * t), sun
-    direction with max t 1e9 when dot(N, sun) > 0; per light the unit vector
-    to it with max t d - 2 when d is within its radius and dot(N, L) > 0."""
+    cut to exactly n_max (None: every cast ray once).  Origin P + N * 0.5
+    (P = ro + rd * t), sun direction with max t 1e9 when dot(N, sun) > 0; per
+    light the unit vector to it with max t d - 2 when d is within its radius
+    and dot(N, L) > 0.  codes=True also returns each ray's pixel and slot."""
     ok = (prim_hits["face"] != 0xFF) & ~(scene.mats.reshape(-1, 20)[prim_hits["mat"], 3] > 0)
+    pix = np.nonzero(ok)[0]
     r, h = prim_rays[ok], prim_hits[ok]
     n = np.zeros((r.shape[0], 3), np.float32)
     f = h["face"].astype(np.int64)
@@ -48,7 +50,8 @@ def shadow_queries(scene, prim_rays: np.ndarray, prim_hits: np.ndarray, u: np.nd
     rows, slots = np.nonzero(sel)
     if rows.size == 0:
         raise IndexError("no primary hit casts a shadow ray")
-    rows, slots = np.resize(rows, n_max), np.resize(slots, n_max)
+    if n_max is not None:
+        rows, slots = np.resize(rows, n_max), np.resize(slots, n_max)
     out = np.zeros(rows.shape[0], RAY_DTYPE)
     out["ro"] = o[rows]
     dirs = np.stack([cc[1] for cc in cand], 1)
@@ -56,7 +59,28 @@ def shadow_queries(scene, prim_rays: np.ndarray, prim_hits: np.ndarray, u: np.nd
     out["rd"] = dirs[rows, slots]
     out["max_t"] = mts[rows, slots]
     out["flags"] = 1
+    if codes:
+        return out, pix[rows], slots
     return out
+
+
+def frame_order(pixels: np.ndarray, slots: np.ndarray) -> np.ndarray:
+    """The order in which the frame's shadow stage takes the rays: the queue
+    region of 64 pixels (one wavefront of the primary launch), then the slot,
+    then the pixel (k_frame_primary's per-region, slot-major queue)."""
+    return np.lexsort((pixels, slots, pixels // 64))
+
+
+def region_wave_cost(steps_in_order: np.ndarray, regions_in_order: np.ndarray, wave: int = 64) -> int:
+    """Issue cost of the frame's shadow stage for rays in queue order: each
+    region's rays are taken by one wavefront in chunks of 64, and a chunk
+    costs its longest ray's steps."""
+    s = np.asarray(steps_in_order, np.int64)
+    reg = np.asarray(regions_in_order, np.int64)
+    start = np.r_[0, np.nonzero(np.diff(reg))[0] + 1]
+    pos = np.arange(s.size) - np.repeat(start, np.diff(np.r_[start, s.size]))
+    chunk = np.cumsum(np.r_[True, (np.diff(reg) != 0) | (pos[1:] % wave == 0)]) - 1
+    return int(np.maximum.reduceat(s, np.r_[0, np.nonzero(np.diff(chunk))[0] + 1]).sum())
 
 
 def mixed_queries(scene, sim, gw, seed: int, vs: float = 1.0 / 160.0, width: int = 2048, height: int = 1024,

@@ -287,6 +287,14 @@ class Simulation:
         self.gw.after_torch()
         self.gw.check(self.lib.px_trace_batch(self.gw.ctx, C.byref(q)))
 
+    def trace_batch_perm(self, q: N.PxQuerySet, perms: dict) -> None:
+        """px_trace_batch_perm: {kind: int32 tensor of source indices}."""
+        arr = (C.c_void_p * 4)()
+        for k, t in perms.items():
+            arr[k] = _devptr(t)
+        self.gw.after_torch()
+        self.gw.check(self.lib.px_trace_batch_perm(self.gw.ctx, C.byref(q), C.cast(arr, C.c_void_p)))
+
     def set_batch_shape(self, block: int, blocks: int = 0, chunk: int = 256) -> None:
         self.gw.check(self.lib.px_set_batch_shape(self.gw.ctx, block, blocks, chunk))
 

@@ -4,15 +4,20 @@ oracle (oracle/mlp.py):
 * forward predictions and batch gradients of the BF16/MFMA kernels equal
   the oracle's BF16-rounding restatement to FP32-accumulation-order noise;
 * one Adam step on the GPU equals the oracle's Adam on the GPU gradient;
-* features are in [-1, 1] with the occupancy samples at exactly +-1;
+* features are in [-1, 1]; the occupancy samples are exactly +-1 and agree
+  with the grid as the host reads it;
 * binning is a stable permutation sorted by bucket inside every window, and
   the permuted launch gives the plain launch's hits bit for bit;
 * the fused prediction + global sort equals the separate kernels' order,
   and with (direction face, bucket) keys numpy's stable sort of them;
-* training on the DDA's own step counts cuts the predictor's held-out error
-  to under a quarter of the untrained network's (the gain the training
-  buys); the wavefront cost of its bins is printed beside the plain order
-  and the bins of the true step counts.
+* training on 48 cameras cuts the predictor's held-out error to under a
+  quarter of the untrained network's, with a primary MAE below the mean step
+  count, and the learned shadow order has a lower wavefront cost than the
+  pixel-then-slot order and than the direction sort alone (the frame's own
+  queue order, which beats it, is printed beside);
+* the batched launch in per-kind orders (px_trace_batch_perm) gives
+  px_trace_batch's hits bit for bit, with random orders and with the
+  predictor's.
 """
 import numpy as np
 import pytest
@@ -106,12 +111,42 @@ def test_features_range(world):
         gw.sync()
         f = mlp.bf16_bits_to_f32(feat.cpu().numpy().view(np.uint16)).reshape(n, 16)
         assert np.all(f >= -1.0) and np.all(f <= 1.0)
-        assert np.all(np.isin(f[:, 6:14], (-1.0, 1.0)))
+        occ = np.c_[f[:, 6:15], f[:, 1]]  # samples at 0.5, 4, .., 512 voxels, then at 2
+        assert np.all(np.isin(occ, (-1.0, 1.0)))
         assert np.all(f[:, 15] == -1.0)
         d = rays["rd"] / np.linalg.norm(rays["rd"], axis=1, keepdims=True)
         assert np.allclose(f[:, 3:6], d, atol=1e-2)
-        # rays inside occupied space see some occupied super cells
-        assert (f[:, 6:14] > 0).mean() > 0.2
+        # the occupancy samples, recomputed on the host from the grid: a
+        # sample is +1 when the brick (or super cell) at that distance past
+        # the entry is non-empty; rounding of the hardware rcp / rsq moves a
+        # few points across a cell face
+        dims = np.array(s.dims, np.float64)
+        bd, sd = dims.astype(np.int64) // 8, (dims.astype(np.int64) // 8 + 3) // 4
+        grid = s.grid.reshape(bd[2], bd[1], bd[0]) != 0
+        sup = np.zeros(sd[::-1], bool)
+        zz, yy, xx = np.nonzero(grid)
+        sup[zz // 4, yy // 4, xx // 4] = True
+        ro, rd = rays["ro"].astype(np.float64), rays["rd"].astype(np.float64)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t0, t1 = -ro / rd, (dims - ro) / rd
+        tmn = np.maximum(np.nanmax(np.minimum(t0, t1), 1), 0.0)
+        tmx = np.minimum(np.nanmin(np.maximum(t0, t1), 1), rays["max_t"])
+        ln = np.linalg.norm(rd, axis=1)
+        ext = np.maximum(np.minimum(tmx, tmn + dims.max()) - tmn, 0.0) * ln
+        want = np.empty((n, 10))
+        for k, dist in enumerate((0.5, 4, 8, 16, 32, 64, 128, 256, 512, 2)):
+            cell = 8.0 if (k < 5 or k == 9) else 32.0
+            p = ro + rd * (tmn + dist / ln)[:, None]
+            c = np.floor(p / cell).astype(np.int64)
+            lim = bd if cell == 8.0 else sd
+            inb = (ext > 0) & (dist <= ext) & (c >= 0).all(1) & (c < lim).all(1)
+            cc = np.where(inb[:, None], c, 0)
+            o = (grid if cell == 8.0 else sup)[cc[:, 2], cc[:, 1], cc[:, 0]] & inb
+            want[:, k] = np.where(o, 1.0, -1.0)
+        agree = (want == occ).mean(0)
+        assert agree.min() > 0.995, agree
+        # rays inside occupied space see some occupied cells
+        assert (occ > 0).mean() > 0.2
     finally:
         m.close()
 
@@ -269,65 +304,151 @@ def test_sort_rays_rejects_bad_arguments(world):
 
 
 def test_training_gain_over_untrained(world):
-    """Training on the DDA's own step counts makes the predictor far better on
-    a held-out camera than the network it started from; the wavefront cost
-    of the resulting bins is reported beside the plain order and the bins of
-    the true step counts."""
+    """Training on the DDA's own step counts over 48 seeded 160x90 cameras
+    makes the predictor carry over to a held-out 640x360 camera: its held-out
+    log2 error is under a quarter of the untrained network's and its MAE in
+    steps is below the mean step count.  On that camera's shadow rays the
+    learned order (direction face, then predicted cost) has a lower
+    wavefront cost (the sum over 64-ray wavefronts of the longest ray's
+    steps, deterministic) than the pixel-then-slot order and than the
+    direction sort alone; the frame's own queue order is printed beside it
+    (per-region, slot-major; it beats the learned order, as the measured
+    frame shows)."""
     import torch
-    from pixels_amd.learned import CostModel, bin_rays, log2_steps, wave_cost
+    from pixels_amd.learned import CostModel, log2_steps, wave_cost
+    from pixels_amd.workloads import frame_order, region_wave_cost, shadow_queries
     s, gw, sim = world
 
-    def data(seed, w=640, h=360):
-        rays = s.camera_rays(w, h, 90.0, seed=seed)
-        hits = _trace_steps(sim, gw, 0, rays)
-        n = rays.shape[0]
-        feat = torch.zeros(n * 16, dtype=torch.int16, device="cuda")
-        m.features(0, _dev(rays.view(np.uint8)), n, feat)
-        return rays, hits["steps"].astype(np.int64), feat
+    def data(seed, w, h):
+        prim = s.camera_rays(w, h, 90.0, seed=seed, tile=8)
+        ph = _trace_steps(sim, gw, 0, prim)
+        u, lights = s.frame_inputs(w, h, seed=seed)
+        sh, pix, slot = shadow_queries(s, prim, ph, u, lights, None, codes=True)
+        hs = _trace_steps(sim, gw, 1, sh)
+        return {0: (prim, ph["steps"].astype(np.int64)), 1: (sh, hs["steps"].astype(np.int64))}, (pix, slot)
 
-    m = CostModel(gw, seed=3)
+    def feats(m, kind, rays):
+        n = rays.shape[0]
+        f = torch.zeros(n * 16, dtype=torch.int16, device="cuda")
+        m.features(kind, _dev(rays.view(np.uint8)), n, f)
+        return f
+
+    models = {k: CostModel(gw, seed=3 + k) for k in (0, 1)}
     try:
-        untrained = m.params()
-        feats, targets = [], []
-        for cam in range(4):
-            _, st, f = data(31 + cam)
-            feats.append(f)
-            targets.append(_dev(log2_steps(st)))
-        tf, tt = torch.cat(feats), torch.cat(targets)
-        n = tt.numel()
-        order = torch.randperm(n, device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
-        tf = tf.view(n, 16)[order].reshape(-1).contiguous()
-        tt = tt[order].contiguous()
+        untrained = {k: m.params() for k, m in models.items()}
+        train = {0: ([], []), 1: ([], [])}
+        cams = 0
+        for cam in range(200, 400):
+            try:
+                d, _ = data(cam, 160, 90)
+            except IndexError:  # no shadow rays from this camera
+                continue
+            for k in (0, 1):
+                train[k][0].append(feats(models[k], k, d[k][0]))
+                train[k][1].append(_dev(log2_steps(d[k][1])))
+            cams += 1
+            if cams == 48:
+                break
+        held, (pix, slot) = data(47, 640, 360)
         bs = 65536
-        print("case: training on %d rays" % n, flush=True)
-        for ep in range(10):
-            for off in range(0, n - bs + 1, bs):
-                m.train_step(tf[off * 16:], tt[off:], bs, lr=1e-3)
-        trained = m.params()
-        # held-out camera
-        rays, steps, f = data(47)
-        nh = rays.shape[0]
-        target = _dev(log2_steps(steps))
-        nb = nh // 64 * 64
-        losses = {}
-        for name, params in (("untrained", untrained), ("trained", trained)):
-            m.set_params(params)
-            losses[name], _ = m.grad(f, target, nb)
-        assert losses["trained"] < 0.25 * losses["untrained"], losses
-        pred = torch.zeros(nh, dtype=torch.float32, device="cuda")
+        for k in (0, 1):
+            tf, tt = torch.cat(train[k][0]), torch.cat(train[k][1])
+            n = tt.numel()
+            order = torch.randperm(n, device="cuda", generator=torch.Generator(device="cuda").manual_seed(k))
+            tf = tf.view(n, 16)[order].reshape(-1).contiguous()
+            tt = tt[order].contiguous()
+            print("case: training kind %d on %d rays from %d cameras" % (k, n, cams), flush=True)
+            for ep in range(12):
+                for off in range(0, n - bs + 1, bs):
+                    models[k].train_step(tf[off * 16:], tt[off:], bs, lr=1e-3)
+            # the steps run on the library's stream: wait before torch may
+            # hand tf / tt's memory to the next tensors
+            gw.sync()
+        report = {}
+        for k in (0, 1):
+            m = models[k]
+            rays, steps = held[k]
+            nh = rays.shape[0]
+            f = feats(m, k, rays)
+            target = _dev(log2_steps(steps))
+            nb = nh // 64 * 64
+            trained = m.params()
+            losses = {}
+            for name, params in (("untrained", untrained[k]), ("trained", trained)):
+                m.set_params(params)
+                losses[name], _ = m.grad(f, target, nb)
+            pred = torch.zeros(nh, dtype=torch.float32, device="cuda")
+            m.predict(f, nh, pred)
+            gw.sync()
+            mae = float(np.mean(np.abs(np.exp2(pred.cpu().numpy()) - np.maximum(steps, 1))))
+            report[k] = (losses, mae, float(steps.mean()))
+            assert losses["trained"] < 0.25 * losses["untrained"], (k, losses)
+            if k == 0:
+                assert mae < steps.mean(), (mae, steps.mean())
+        # wavefront cost of the held-out shadow rays in each order
+        sh, steps = held[1]
+        nh = sh.shape[0]
+        d = _dev(sh.view(np.uint8))
         perm = torch.zeros(nh, dtype=torch.int32, device="cuda")
-        m.set_params(trained)
-        m.predict(f, nh, pred)
-        bin_rays(gw, pred, nh, perm, window=256, buckets=32, lo=0.0, hi=9.0)
+        models[1].sort_rays(1, d, nh, perm, buckets=32, faces=True)
         gw.sync()
-        p_trained = pred.cpu().numpy()
-        c_trained = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
-        bin_rays(gw, target, nh, perm, window=256, buckets=32, lo=0.0, hi=9.0)
+        c_learned = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+        models[1].sort_rays(1, d, nh, perm, buckets=1, faces=True)
         gw.sync()
-        c_ideal = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
-        mae = float(np.mean(np.abs(np.exp2(p_trained) - np.maximum(steps, 1))))
-        print("held-out log2 MSE: untrained %.3f trained %.3f; MAE %.2f steps (mean %.1f); wave cost: plain %d, "
-              "trained bins %d, true-cost bins %d" % (losses["untrained"], losses["trained"], mae, steps.mean(),
-                                                      wave_cost(steps), c_trained, c_ideal), flush=True)
+        c_faces = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+        c_plain = wave_cost(steps)
+        fo = frame_order(pix, slot)
+        c_frame = region_wave_cost(steps[fo], (pix // 64)[fo])
+        print("held-out: primary log2 MSE %.3f -> %.3f, MAE %.2f steps (mean %.1f); shadow log2 MSE %.3f -> %.3f, "
+              "MAE %.2f (mean %.1f); shadow wave cost: pixel-then-slot %d, faces only %d, learned %d, frame queue "
+              "order %d" % (report[0][0]["untrained"], report[0][0]["trained"], report[0][1], report[0][2],
+                            report[1][0]["untrained"], report[1][0]["trained"], report[1][1], report[1][2], c_plain,
+                            c_faces, c_learned, c_frame), flush=True)
+        assert c_learned < 0.9 * c_plain
+        assert c_learned < c_faces
+    finally:
+        for m in models.values():
+            m.close()
+
+
+def test_batch_in_learned_order_equals_batch(world):
+    import torch
+    from pixels_amd.learned import CostModel
+    from pixels_amd.workloads import shadow_queries
+    s, gw, sim = world
+    prim = s.camera_rays(320, 180, 90.0, seed=5, tile=8)
+    ph = _trace_steps(sim, gw, 0, prim)
+    u, lights = s.frame_inputs(320, 180, seed=5)
+    sh = shadow_queries(s, prim, ph, u, lights, None)
+    rng = np.random.default_rng(6)
+    seg = np.zeros(5000, prim.dtype)
+    seg["ro"] = rng.uniform((0, 0, 0), s.dims, (5000, 3))
+    seg["rd"] = rng.normal(size=(5000, 3)) * 20
+    seg["flags"] = 3
+    host = {0: prim, 1: sh, 3: seg}
+    rays = {k: (_dev(v.view(np.uint8)), v.shape[0]) for k, v in host.items()}
+    outs = {k: torch.full((v.shape[0] * 16,), 0xAB, dtype=torch.uint8, device="cuda") for k, v in host.items()}
+    q = sim.query_set(rays, outs)
+    sim.trace_batch(q)
+    gw.sync()
+    ref = {k: o.clone() for k, o in outs.items()}
+    m = CostModel(gw, seed=9)
+    try:
+        for how in ("random", "learned"):
+            perms = {}
+            for k in (0, 1, 3):
+                n = host[k].shape[0]
+                if how == "random" or k == 3:
+                    perms[k] = torch.from_numpy(rng.permutation(n).astype(np.int32)).cuda()
+                else:
+                    perms[k] = torch.zeros(n, dtype=torch.int32, device="cuda")
+                    m.sort_rays(k, rays[k][0], n, perms[k], buckets=32, faces=True)
+            for o in outs.values():
+                o.fill_(0xCD)
+            print("case: batch in %s order" % how, flush=True)
+            sim.trace_batch_perm(q, perms)
+            gw.sync()
+            for k in host:
+                assert torch.equal(outs[k], ref[k]), (how, k)
     finally:
         m.close()
