@@ -240,165 +240,208 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const uint16_t* __restrict_
   }
 }
 
-// Forward + backward of 64 samples per block (one wavefront) for the loss
-// (1/n_batch) sum (y - target)^2; writes this block's gradient partials and
-// loss sum to part[blockIdx.x * kPartFloats ...].
-__global__ __launch_bounds__(64) void k_mlp_grad(const uint16_t* __restrict__ wbf, const float* __restrict__ par,
+// Forward + backward for the loss (1/n_batch) sum (y - target)^2.  Blocks of
+// kGradWaves wavefronts, one per SIMD; wavefront w of block b takes the
+// 64-sample tiles v, v + W, v + 2W .. (v = b kGradWaves + w, W = all
+// wavefronts) in that order and keeps its gradient in the MFMA accumulators
+// (each tile's products are accumulated onto the running sums as the C
+// operand).  The wavefronts of a block add their sums in LDS in wavefront
+// order and the block writes one record (gradient + loss sum) to
+// part[blockIdx.x]; k_mlp_sum adds the records in a fixed order.  Every sum
+// runs in an order fixed by the indices, so the result does not depend on
+// scheduling.
+constexpr uint32_t kGradWaves = 4;
+
+PX_DEV void grad_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(64 * kGradWaves) void k_mlp_grad(const uint16_t* __restrict__ wbf, const float* __restrict__ par,
                                                  const uint16_t* __restrict__ feat, const float* __restrict__ target,
                                                  uint32_t n, float inv_batch, float* __restrict__ part) {
   __shared__ MlpLds L;
-  // fragments read 8 bytes per lane: every array is 16-byte aligned
-  __shared__ alignas(16) uint16_t sA[kH * 64];    // feature-major [f][s]: H1, then dH1
-  __shared__ alignas(16) uint16_t sB[kH * 64];    // feature-major [f][s]: H2, then dH2
-  __shared__ alignas(16) uint16_t sX[kIn * 64];   // X^T feature-major [f][s]
-  __shared__ alignas(16) uint16_t sD[16 * 64];    // row 0 = dy (bf16), rows 1..15 zero
-  __shared__ alignas(16) uint16_t sOne[16 * 64];  // row 0 = 1.0: the B fragment of a column of ones
+  // per wavefront; fragments read 8 bytes per lane: every array is 16-byte aligned
+  __shared__ alignas(16) uint16_t s_a[kGradWaves][kH * 64];    // feature-major [f][s]: H1, then dH1
+  __shared__ alignas(16) uint16_t s_b[kGradWaves][kH * 64];    // feature-major [f][s]: H2, then dH2
+  __shared__ alignas(16) uint16_t s_x[kGradWaves][kIn * 64];   // X^T feature-major [f][s]
+  __shared__ alignas(16) uint16_t s_d[kGradWaves][16 * 64];    // row 0 = dy (bf16), rows 1..15 zero
+  __shared__ alignas(16) uint16_t s_one[kGradWaves][16 * 64];  // row 0 = 1.0: B fragment of a column of ones
+  __shared__ float s_red[kPartFloats];                          // the block's record
   load_weights(L, wbf, par);
-  const uint32_t l = __lane_id();
+  const uint32_t l = __lane_id(), wave = threadIdx.x >> 6;
+  uint16_t* sA = s_a[wave];
+  uint16_t* sB = s_b[wave];
+  uint16_t* sX = s_x[wave];
+  uint16_t* sD = s_d[wave];
+  uint16_t* sOne = s_one[wave];
   const int g = (int)(l >> 4), col = (int)(l & 15u);
-  const uint32_t s0 = blockIdx.x * 64u;
-  Fwd F;
-  forward(L, feat, n, s0, F);
-  float* out = part + (size_t)blockIdx.x * kPartFloats;
-
-  // dy = 2 (y - target) / n_batch, zero for padding samples
-  float loss = 0.0f;
-  for (int c = 0; c < 4; c++) {
-    const uint32_t s = s0 + 16u * c + (uint32_t)col;
-    float dy = 0.0f;
-    if (l < 16u && s < n) {
-      const float e = F.y[c] - target[s];
-      loss += e * e;
-      dy = 2.0f * e * inv_batch;
-    }
-    // broadcast dy of sample 16c + col to every lane of that column
-    const float dyb = __shfl(dy, col);
-    if (l < 16u) sD[16 * c + col] = f2bf(dy);
-    F.y[c] = dyb;  // reuse: dy per column, every lane
-  }
   for (int i = (int)l; i < 15 * 64; i += 64) sD[64 + i] = 0;
   for (int i = (int)l; i < 16 * 64; i += 64) sOne[i] = i < 64 ? (uint16_t)0x3F80u : (uint16_t)0;  // 1.0 bf16
-  // stage X^T, H1, H2 feature-major
-  for (int c = 0; c < 4; c++) {
-    const int s = 16 * c + col;
-    for (int j = 0; j < 4; j++) sX[(4 * g + j) * 64 + s] = (uint16_t)(j == 0 ? F.x[c].x : j == 1 ? F.x[c].y : j == 2 ? F.x[c].z : F.x[c].w);
-    for (int t = 0; t < 4; t++) {
-      const s16x4 a = F.h1(t, c), b = F.h2(t, c);
-      sA[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)a.x;
-      sA[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)a.y;
-      sA[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)a.z;
-      sA[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)a.w;
-      sB[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)b.x;
-      sB[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)b.y;
-      sB[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)b.z;
-      sB[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)b.w;
-    }
+  const f32x4 z = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  f32x4 aW3[4], aB3 = z, aW2[4][4], aB2[4], aW1[4], aB1[4];
+  for (int a = 0; a < 4; a++) {
+    aW3[a] = aB2[a] = aW1[a] = aB1[a] = z;
+    for (int b = 0; b < 4; b++) aW2[a][b] = z;
   }
-  __syncthreads();
-
-  // dW3[k] = sum_s dy[s] H2[k][s]  (M = 1 padded, K = samples, N = k)
-  for (int t = 0; t < 4; t++) {
-    f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kc = 0; kc < 4; kc++) d = mfma(frag(sD, 64, 0, 16 * kc), frag(sB, 64, 16 * t, 16 * kc), d);
-    if (l < 16u) out[kOffW3 + 16 * t + col] = d.x;
-  }
-  // db3 = sum_s dy[s]
-  {
-    f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kc = 0; kc < 4; kc++) d = mfma(frag(sD, 64, 0, 16 * kc), frag(sOne, 64, 0, 16 * kc), d);
-    if (l == 0u) out[kOffB3] = d.x;
-  }
-  // dH2 = w3[k] dy[s] where H2 > 0 (output-tile layout), then feature-major in sB
-  s16x4 dh2[4][4];
-  for (int t = 0; t < 4; t++) {
-    float w[4];
-    for (int j = 0; j < 4; j++) w[j] = bf2f(L.w3p[16 * t + 4 * g + j]);
+  float loss = 0.0f;
+  const uint32_t tiles = (n + 63u) / 64u;
+  for (uint32_t tile = blockIdx.x * kGradWaves + wave; tile < tiles; tile += gridDim.x * kGradWaves) {
+    const uint32_t s0 = tile * 64u;
+    Fwd F;
+    forward(L, feat, n, s0, F);
+    // dy = 2 (y - target) / n_batch, zero for padding samples
     for (int c = 0; c < 4; c++) {
-      const s16x4 h = F.h2(t, c);
-      const float dy = F.y[c];
-      dh2[t][c] = pack4(h.x > 0 ? w[0] * dy : 0.0f, h.y > 0 ? w[1] * dy : 0.0f, h.z > 0 ? w[2] * dy : 0.0f,
-                        h.w > 0 ? w[3] * dy : 0.0f);
+      const uint32_t s = s0 + 16u * c + (uint32_t)col;
+      float dy = 0.0f;
+      if (l < 16u && s < n) {
+        const float e = F.y[c] - target[s];
+        loss += e * e;
+        dy = 2.0f * e * inv_batch;
+      }
+      // broadcast dy of sample 16c + col to every lane of that column
+      const float dyb = __shfl(dy, col);
+      if (l < 16u) sD[16 * c + col] = f2bf(dy);
+      F.y[c] = dyb;  // reuse: dy per column, every lane
     }
-  }
-  __syncthreads();
-  for (int c = 0; c < 4; c++) {
-    const int s = 16 * c + col;
+    // stage X^T, H1, H2 feature-major
+    for (int c = 0; c < 4; c++) {
+      const int s = 16 * c + col;
+      for (int j = 0; j < 4; j++)
+        sX[(4 * g + j) * 64 + s] = (uint16_t)(j == 0 ? F.x[c].x : j == 1 ? F.x[c].y : j == 2 ? F.x[c].z : F.x[c].w);
+      for (int t = 0; t < 4; t++) {
+        const s16x4 a = F.h1(t, c), b = F.h2(t, c);
+        sA[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)a.x;
+        sA[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)a.y;
+        sA[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)a.z;
+        sA[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)a.w;
+        sB[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)b.x;
+        sB[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)b.y;
+        sB[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)b.z;
+        sB[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)b.w;
+      }
+    }
+    grad_wave_sync();
+    // dW3[k] += sum_s dy[s] H2[k][s] (M = 1 padded, K = samples, N = k); db3 += sum_s dy[s]
+    for (int t = 0; t < 4; t++)
+      for (int kc = 0; kc < 4; kc++) aW3[t] = mfma(frag(sD, 64, 0, 16 * kc), frag(sB, 64, 16 * t, 16 * kc), aW3[t]);
+    for (int kc = 0; kc < 4; kc++) aB3 = mfma(frag(sD, 64, 0, 16 * kc), frag(sOne, 64, 0, 16 * kc), aB3);
+    // dH2 = w3[k] dy[s] where H2 > 0 (output-tile layout), then feature-major in sB
+    s16x4 dh2[4][4];
     for (int t = 0; t < 4; t++) {
-      const s16x4 a = dh2[t][c];
-      sB[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)a.x;
-      sB[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)a.y;
-      sB[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)a.z;
-      sB[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)a.w;
+      float w[4];
+      for (int j = 0; j < 4; j++) w[j] = bf2f(L.w3p[16 * t + 4 * g + j]);
+      for (int c = 0; c < 4; c++) {
+        const s16x4 h = F.h2(t, c);
+        const float dy = F.y[c];
+        dh2[t][c] = pack4(h.x > 0 ? w[0] * dy : 0.0f, h.y > 0 ? w[1] * dy : 0.0f, h.z > 0 ? w[2] * dy : 0.0f,
+                          h.w > 0 ? w[3] * dy : 0.0f);
+      }
     }
-  }
-  __syncthreads();
-  // dW2[i][o] = sum_s H1[i][s] dH2[o][s]: tile (M = o, K = s, N = i) gives dW2T[o][i]
-  for (int to = 0; to < 4; to++)
+    grad_wave_sync();
+    for (int c = 0; c < 4; c++) {
+      const int s = 16 * c + col;
+      for (int t = 0; t < 4; t++) {
+        const s16x4 a = dh2[t][c];
+        sB[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)a.x;
+        sB[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)a.y;
+        sB[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)a.z;
+        sB[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)a.w;
+      }
+    }
+    grad_wave_sync();
+    // dW2[i][o] += sum_s H1[i][s] dH2[o][s]: tile (M = o, K = s, N = i) gives dW2T[o][i];
+    // db2[o] += sum_s dH2[o][s]
+    for (int to = 0; to < 4; to++) {
+      for (int ti = 0; ti < 4; ti++)
+        for (int kc = 0; kc < 4; kc++)
+          aW2[to][ti] = mfma(frag(sB, 64, 16 * to, 16 * kc), frag(sA, 64, 16 * ti, 16 * kc), aW2[to][ti]);
+      for (int kc = 0; kc < 4; kc++)
+        aB2[to] = mfma(frag(sB, 64, 16 * to, 16 * kc), frag(sOne, 64, 0, 16 * kc), aB2[to]);
+    }
+    // dH1 = W2 . dH2 where H1 > 0 (M = i, K = o, N = samples; B = dh2 in registers)
+    s16x4 dh1[4][4];
     for (int ti = 0; ti < 4; ti++) {
-      f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-      for (int kc = 0; kc < 4; kc++) d = mfma(frag(sB, 64, 16 * to, 16 * kc), frag(sA, 64, 16 * ti, 16 * kc), d);
-      // element (o = 16 to + 4g + j, i = 16 ti + col) -> W2[i][o]
-      for (int j = 0; j < 4; j++) out[kOffW2 + (16 * ti + col) * kH + 16 * to + 4 * g + j] = d[j];
+      s16x4 a[4];
+      for (int ko = 0; ko < 4; ko++) a[ko] = frag(L.w2, kLdH, 16 * ti, 16 * ko);
+      for (int c = 0; c < 4; c++) {
+        f32x4 d = z;
+        for (int ko = 0; ko < 4; ko++) d = mfma(a[ko], dh2[ko][c], d);
+        const s16x4 h = F.h1(ti, c);
+        dh1[ti][c] = pack4(h.x > 0 ? d.x : 0.0f, h.y > 0 ? d.y : 0.0f, h.z > 0 ? d.z : 0.0f, h.w > 0 ? d.w : 0.0f);
+      }
     }
-  // db2[o] = sum_s dH2[o][s]
-  for (int to = 0; to < 4; to++) {
-    f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kc = 0; kc < 4; kc++) d = mfma(frag(sB, 64, 16 * to, 16 * kc), frag(sOne, 64, 0, 16 * kc), d);
-    if (col == 0)
-      for (int j = 0; j < 4; j++) out[kOffB2 + 16 * to + 4 * g + j] = d[j];
-  }
-  // dH1 = W2 . dH2 where H1 > 0 (M = i, K = o, N = samples; B = dh2 in registers)
-  s16x4 dh1[4][4];
-  for (int ti = 0; ti < 4; ti++) {
-    s16x4 a[4];
-    for (int ko = 0; ko < 4; ko++) a[ko] = frag(L.w2, kLdH, 16 * ti, 16 * ko);
+    grad_wave_sync();
     for (int c = 0; c < 4; c++) {
-      f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-      for (int ko = 0; ko < 4; ko++) d = mfma(a[ko], dh2[ko][c], d);
-      const s16x4 h = F.h1(ti, c);
-      dh1[ti][c] = pack4(h.x > 0 ? d.x : 0.0f, h.y > 0 ? d.y : 0.0f, h.z > 0 ? d.z : 0.0f, h.w > 0 ? d.w : 0.0f);
+      const int s = 16 * c + col;
+      for (int t = 0; t < 4; t++) {
+        const s16x4 a = dh1[t][c];
+        sA[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)a.x;
+        sA[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)a.y;
+        sA[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)a.z;
+        sA[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)a.w;
+      }
     }
-  }
-  __syncthreads();
-  for (int c = 0; c < 4; c++) {
-    const int s = 16 * c + col;
-    for (int t = 0; t < 4; t++) {
-      const s16x4 a = dh1[t][c];
-      sA[(16 * t + 4 * g + 0) * 64 + s] = (uint16_t)a.x;
-      sA[(16 * t + 4 * g + 1) * 64 + s] = (uint16_t)a.y;
-      sA[(16 * t + 4 * g + 2) * 64 + s] = (uint16_t)a.z;
-      sA[(16 * t + 4 * g + 3) * 64 + s] = (uint16_t)a.w;
+    grad_wave_sync();
+    // dW1[f][o] += sum_s X[s][f] dH1[o][s]: tile (M = o, K = s, N = f); db1[o] += sum_s dH1[o][s]
+    for (int to = 0; to < 4; to++) {
+      for (int kc = 0; kc < 4; kc++)
+        aW1[to] = mfma(frag(sA, 64, 16 * to, 16 * kc), frag(sX, 64, 0, 16 * kc), aW1[to]);
+      for (int kc = 0; kc < 4; kc++)
+        aB1[to] = mfma(frag(sA, 64, 16 * to, 16 * kc), frag(sOne, 64, 0, 16 * kc), aB1[to]);
     }
+    grad_wave_sync();  // the next tile overwrites sA, sB, sX, sD
   }
-  __syncthreads();
-  // dW1[f][o] = sum_s X[s][f] dH1[o][s]: tile (M = o, K = s, N = f)
-  for (int to = 0; to < 4; to++) {
-    f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kc = 0; kc < 4; kc++) d = mfma(frag(sA, 64, 16 * to, 16 * kc), frag(sX, 64, 0, 16 * kc), d);
-    for (int j = 0; j < 4; j++) out[kOffW1 + col * kH + 16 * to + 4 * g + j] = d[j];
-  }
-  // db1[o] = sum_s dH1[o][s]
-  for (int to = 0; to < 4; to++) {
-    f32x4 d = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kc = 0; kc < 4; kc++) d = mfma(frag(sA, 64, 16 * to, 16 * kc), frag(sOne, 64, 0, 16 * kc), d);
-    if (col == 0)
-      for (int j = 0; j < 4; j++) out[kOffB1 + 16 * to + 4 * g + j] = d[j];
-  }
-  // loss sum of the block
+  // this block's record: the wavefronts' sums added in wavefront order
   for (int o = 32; o > 0; o >>= 1) loss += __shfl_xor(loss, o);
-  if (l == 0u) out[kParams] = loss;
+  __syncthreads();
+  for (uint32_t w = 0; w < kGradWaves; w++) {
+    if (wave == w) {
+      float* out = s_red;
+      const bool first = w == 0u;
+#define PX_ACC(idx, v) (out[idx] = first ? (v) : out[idx] + (v))
+      if (l < 16u)
+        for (int t = 0; t < 4; t++) PX_ACC(kOffW3 + 16 * t + col, aW3[t].x);
+      if (l == 0u) PX_ACC(kOffB3, aB3.x);
+      for (int to = 0; to < 4; to++) {
+        // element (o = 16 to + 4g + j, i = 16 ti + col) of dW2T -> W2[i][o]
+        for (int ti = 0; ti < 4; ti++)
+          for (int j = 0; j < 4; j++) PX_ACC(kOffW2 + (16 * ti + col) * kH + 16 * to + 4 * g + j, aW2[to][ti][j]);
+        if (col == 0)
+          for (int j = 0; j < 4; j++) PX_ACC(kOffB2 + 16 * to + 4 * g + j, aB2[to][j]);
+        for (int j = 0; j < 4; j++) PX_ACC(kOffW1 + col * kH + 16 * to + 4 * g + j, aW1[to][j]);
+        if (col == 0)
+          for (int j = 0; j < 4; j++) PX_ACC(kOffB1 + 16 * to + 4 * g + j, aB1[to][j]);
+      }
+      if (l == 0u) PX_ACC(kParams, loss);
+#undef PX_ACC
+    }
+    __syncthreads();
+  }
+  float* rec = part + (size_t)blockIdx.x * kPartFloats;
+  for (uint32_t p = threadIdx.x; p < (uint32_t)kPartFloats; p += blockDim.x) rec[p] = s_red[p];
 }
 
-// Sums the partials of nb records in a fixed order (deterministic), one
-// parameter (or the loss) per thread; blockIdx.y takes records
-// [y * per, (y + 1) * per) and writes record y of `out`.
-__global__ void k_mlp_reduce(const float* __restrict__ part, uint32_t nb, uint32_t per, float* __restrict__ out) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (uint32_t)kPartFloats) return;
-  const uint32_t b0 = blockIdx.y * per, b1 = min(b0 + per, nb);
+// Sum of the nb block records of k_mlp_grad: a block takes 64 parameters (or
+// the loss) per lane, its wavefront w the records w, w + 16, w + 32 ..; the
+// 16 wavefront sums are then added in wavefront order.  A fixed order.
+constexpr uint32_t kSumWaves = 16;
+__global__ __launch_bounds__(64 * kSumWaves) void k_mlp_sum(const float* __restrict__ part, uint32_t nb,
+                                                            float* __restrict__ out) {
+  __shared__ float s_part[kSumWaves][64];
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint32_t p = blockIdx.x * 64u + lane;
   float s = 0.0f;
-  for (uint32_t b = b0; b < b1; b++) s += part[(size_t)b * kPartFloats + p];
-  out[(size_t)blockIdx.y * kPartFloats + p] = s;
+  if (p < (uint32_t)kPartFloats)
+    for (uint32_t b = wave; b < nb; b += kSumWaves) s += part[(size_t)b * kPartFloats + p];
+  s_part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0u && p < (uint32_t)kPartFloats) {
+    float t = 0.0f;
+    for (uint32_t w = 0; w < kSumWaves; w++) t += s_part[w][lane];
+    out[p] = t;
+  }
 }
 
 // Adam (Kingma & Ba 2015, bias-corrected), then the bf16 copy of every weight.
@@ -929,26 +972,21 @@ int px_mlp_predict(px_mlp* m, const uint16_t* d_feat, uint32_t n, float* d_pred)
 
 static int mlp_grad(px_mlp* m, const uint16_t* d_feat, const float* d_target, uint32_t n) {
   px_ctx* c = m->c;
-  const uint32_t nb = (n + 63u) / 64u;
-  if (nb > m->part_cap) {
+  const uint32_t tiles = (n + 63u) / 64u;
+  // one block of kGradWaves wavefronts per CU (one wavefront per SIMD)
+  const uint32_t G = std::min<uint32_t>((tiles + kGradWaves - 1u) / kGradWaves, c->n_cu);
+  if (G > m->part_cap) {
     (void)hipFree(m->d_part);
     m->d_part = nullptr;
     m->part_cap = 0;
-    // per-block records, then the first-pass sums (one per 32 records)
-    MLP_HIP(m, hipMalloc(&m->d_part, ((size_t)nb + (nb + 31u) / 32u) * kPartFloats * 4));
-    m->part_cap = nb;
+    MLP_HIP(m, hipMalloc(&m->d_part, (size_t)G * kPartFloats * 4));
+    m->part_cap = G;
   }
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(64), 0, c->stream, m->d_wbf, m->d_par, d_feat, d_target, n,
-                     1.0f / (float)n, m->d_part);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(G), dim3(64 * kGradWaves), 0, c->stream, m->d_wbf, m->d_par, d_feat, d_target,
+                     n, 1.0f / (float)n, m->d_part);
   MLP_HIP(m, hipGetLastError());
-  // two fixed-order passes: nb records -> splits records -> 1
-  const uint32_t per = 32u, splits = (nb + per - 1u) / per;
-  float* second = m->d_part + (size_t)nb * kPartFloats;
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartFloats + 255) / 256, splits), dim3(256), 0, c->stream, m->d_part, nb,
-                     per, second);
-  MLP_HIP(m, hipGetLastError());
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kPartFloats + 255) / 256, 1), dim3(256), 0, c->stream, second, splits,
-                     splits, m->d_grad);
+  hipLaunchKernelGGL(k_mlp_sum, dim3((kPartFloats + 63) / 64), dim3(64 * kSumWaves), 0, c->stream, m->d_part, G,
+                     m->d_grad);
   MLP_HIP(m, hipGetLastError());
   return PX_OK;
 }
