@@ -59,7 +59,7 @@ constexpr uint32_t kCountWords = 4;     // per-block list lengths
 // edit-pass counters (ec[])
 enum : uint32_t {
   EC_JOBCELLS = 0, EC_ALLOCS, EC_ACTIVE, EC_NIMP, EC_LABELS, EC_LABELS_OK, EC_JOBS, EC_CHUNKS,
-  EC_SEEDS, EC_OVERFLOW, EC_NB_T, EC_NB_DROP, EC_REEMIT, EC_WORDS = 16,
+  EC_SEEDS, EC_OVERFLOW, EC_NB_T, EC_NB_DROP, EC_REEMIT, EC_CARVE_N, EC_FLOOD_N, EC_WORDS = 16,
   // voxel-write stage (after the island pass's words)
   EC_W_N = 16, EC_W_DIRTY, EC_W_WRITTEN, EC_W_DROP,
   // extraction and chunk stages
@@ -112,6 +112,7 @@ struct EditView {
   uint32_t* ec;
   float* impacts;     // the impact queue (pushImpact's records), C_IMPACTS of them
   uint32_t* shots;    // shoot's pane flood entries and dirtied cells (kShotWords)
+  uint32_t* work;     // cells the count passes found work in: carve at [0, ncell), flood at [ncell, 2 ncell)
   float* flashes;     // flash ring, PX_MAX_FLASHES records of 8 floats
   uint32_t max_imp;   // min(max_impacts, 64)
   uint32_t* seeds;    // seeded mode: one bit per voxel key, pushSeed (sim.wgsl:180-184)
@@ -628,7 +629,9 @@ PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t l
 }
 
 // Face merges between a cell and its +x / +y / +z neighbours (64 voxel pairs
-// per face, one per lane), plus anchors across faces (islands).
+// per face, one per lane), plus anchors across faces (islands).  The three
+// faces' voxel and label reads are issued before any union, so their
+// latencies overlap.
 template <int MODE>
 PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, const uint8_t* s_str,
                        uint32_t& walk_err) {
@@ -640,7 +643,12 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, con
   int32_t bc[3];
   cell_coord(e, ci, bc[0], bc[1], bc[2]);
   uint32_t okey = vkey(e, bc[0] * 8, bc[1] * 8, bc[2] * 8);
+  uint32_t va[3], vb[3], na[3], nb[3];
+  bool live[3];
   for (int d = 0; d < 3; d++) {
+    va[d] = vb[d] = 0u;
+    na[d] = nb[d] = PX_NONE;
+    live[d] = false;
     if (bc[d] + 1 >= e.bd[d]) continue;
     int32_t nc[3] = {bc[0], bc[1], bc[2]};
     nc[d] += 1;
@@ -648,32 +656,38 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, con
     uint32_t gn = e.grid[cj];
     if (gn == 0u) continue;
     if (MODE == 1 && (gn & PX_UNIFORM_BIT) != 0u) continue;
+    live[d] = true;
     uint32_t la, lb;
     if (d == 0) { la = 7u | (u << 3) | (v << 6); lb = 0u | (u << 3) | (v << 6); }
     else if (d == 1) { la = u | (7u << 3) | (v << 6); lb = u | (0u << 3) | (v << 6); }
     else { la = u | (v << 3) | (7u << 6); lb = u | (v << 3) | (0u << 6); }
-    uint32_t va = cell_voxel(e, g, la), vb = cell_voxel(e, gn, lb);
-    uint32_t ca = va ? s_cls[va & 0xffu] : 0u, cb = vb ? s_cls[vb & 0xffu] : 0u;
-    uint32_t okn = vkey(e, nc[0] * 8, nc[1] * 8, nc[2] * 8);
-    uint32_t na = ca ? node_of(e, okey, g, la) : PX_NONE;
-    uint32_t nb = cb ? node_of(e, okn, gn, lb) : PX_NONE;
-    if (na == PX_NONE) ca = 0u;
-    if (nb == PX_NONE) cb = 0u;
-    bool join = ca != 0u && ca == cb;
+    va[d] = cell_voxel(e, g, la);
+    vb[d] = cell_voxel(e, gn, lb);
+    const uint32_t ca = va[d] ? s_cls[va[d] & 0xffu] : 0u, cb = vb[d] ? s_cls[vb[d] & 0xffu] : 0u;
+    const uint32_t okn = vkey(e, nc[0] * 8, nc[1] * 8, nc[2] * 8);
+    na[d] = ca ? node_of(e, okey, g, la) : PX_NONE;
+    nb[d] = cb ? node_of(e, okn, gn, lb) : PX_NONE;
+  }
+  for (int d = 0; d < 3; d++) {
+    if (!live[d]) continue;
+    const uint32_t ca = va[d] ? s_cls[va[d] & 0xffu] : 0u, cb = vb[d] ? s_cls[vb[d] & 0xffu] : 0u;
+    const uint32_t A = na[d], B = nb[d];
+    const uint32_t cA = A == PX_NONE ? 0u : ca, cB = B == PX_NONE ? 0u : cb;
+    bool join = cA != 0u && cA == cB;
     // skip a pair that the lane to the left or above on the face also has:
     // in raster order the first lane of every such region does the work
     uint32_t jb = join ? 1u : 0u;
-    uint32_t pa = __shfl_up(na, 1), pb = __shfl_up(nb, 1), pj = __shfl_up(jb, 1);
-    uint32_t qa = __shfl_up(na, 8), qb = __shfl_up(nb, 8), qj = __shfl_up(jb, 8);
-    bool dup = (u > 0u && pj && pa == na && pb == nb) || (v > 0u && qj && qa == na && qb == nb);
-    if (join && !dup) uf_union(e.par, na, nb, e.walk_cap, walk_err);
+    uint32_t pa = __shfl_up(A, 1), pb = __shfl_up(B, 1), pj = __shfl_up(jb, 1);
+    uint32_t qa = __shfl_up(A, 8), qb = __shfl_up(B, 8), qj = __shfl_up(jb, 8);
+    bool dup = (u > 0u && pj && pa == A && pb == B) || (v > 0u && qj && qa == A && qb == B);
+    if (join && !dup) uf_union(e.par, A, B, e.walk_cap, walk_err);
     if (MODE == 0) {
-      uint32_t anc_a = (ca && vb && s_str[vb & 0xffu]) ? 1u : 0u;
-      uint32_t anc_b = (cb && va && s_str[va & 0xffu]) ? 1u : 0u;
+      uint32_t anc_a = (cA && vb[d] && s_str[vb[d] & 0xffu]) ? 1u : 0u;
+      uint32_t anc_b = (cB && va[d] && s_str[va[d] & 0xffu]) ? 1u : 0u;
       uint32_t pna = __shfl_up(anc_a, 1), qna = __shfl_up(anc_a, 8);
       uint32_t pnb = __shfl_up(anc_b, 1), qnb = __shfl_up(anc_b, 8);
-      if (anc_a && !((u > 0u && pna && pa == na) || (v > 0u && qna && qa == na))) atomicOr(&e.flag[na], kFlagAnchor);
-      if (anc_b && !((u > 0u && pnb && pb == nb) || (v > 0u && qnb && qb == nb))) atomicOr(&e.flag[nb], kFlagAnchor);
+      if (anc_a && !((u > 0u && pna && pa == A) || (v > 0u && qna && qa == A))) atomicOr(&e.flag[A], kFlagAnchor);
+      if (anc_b && !((u > 0u && pnb && pb == B) || (v > 0u && qnb && qb == B))) atomicOr(&e.flag[B], kFlagAnchor);
     }
   }
 }
@@ -1272,6 +1286,21 @@ PX_DEV uint32_t rubble_chain(const EditView& e, const float* __restrict__ s_im, 
   return n;
 }
 
+// rubble_chain with its first probe answered from the brick's own rows: the
+// voxel above a row's voxel x is in the row of the next lane (y + 1, same z)
+// unless the row is the brick's top one.  `up` is that row's word x / 2.
+// Only a rubble voxel there (rare) or the top row reads the world; no other
+// cell's wavefront changes the voxel right above a removed one (a rubble
+// chain stops at a removed or pane voxel), so the brick's copy is current.
+PX_DEV uint32_t rubble_chain_row(const EditView& e, const float* __restrict__ s_im, int32_t x, int32_t y, int32_t z,
+                                 uint32_t ry, uint32_t up) {
+  if (ry < 7u) {
+    const uint32_t va = (up >> ((uint32_t)(x & 1) * 16u)) & 0xffffu;
+    if (va == 0u || (va & 0x80u) == 0u) return 0u;
+  }
+  return rubble_chain(e, s_im, x, y, z);
+}
+
 // pushSeed (sim.wgsl:180-184) of the neighbours of a removed voxel of value
 // v, as carveEffects (:511-523) and floodStep (:606-616) push them: not air,
 // not structural, not rubble, not of v's id when v is a pane.  Whether the
@@ -1412,9 +1441,11 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
   const uint32_t accepted = PASS == 1 ? spawn_accepted(e, total) : 0u;
   const uint32_t cursor = PASS == 1 ? e.ctrl[PX_C_PCURSOR] : 0u;
   uint32_t acc_d = 0u, acc_b = 0u;
-  const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
+  // the count pass visits every job cell, the apply pass the cells it found
+  // removals in
+  const uint32_t n = PASS == 0 ? min(e.ec[EC_JOBCELLS], e.ncell) : min(e.ec[EC_CARVE_N], e.ncell);
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const uint32_t ci = e.job_list[i];
+    const uint32_t ci = PASS == 0 ? e.job_list[i] : e.work[i];
     const uint64_t mask = e.imask[ci];
     const uint32_t g = e.grid[ci];
     if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
@@ -1425,6 +1456,7 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
     uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
     const uint4 w4 = *reinterpret_cast<const uint4*>(row);
     const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+    const uint32_t up[4] = {__shfl_down(w4.x, 1), __shfl_down(w4.y, 1), __shfl_down(w4.z, 1), __shfl_down(w4.w, 1)};
     const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
     const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
     // per-voxel terms of removes() that do not depend on the impact
@@ -1472,6 +1504,10 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
       }
     }
     if (__ballot(rem != 0u) == 0ull) continue;
+    if (PASS == 0 && lane == 0u) {
+      const uint32_t k = atomicAdd(&e.ec[EC_CARVE_N], 1u);
+      if (k < e.ncell) e.work[k] = ci;
+    }
     // records of this lane's removed voxels
     uint32_t rub[8];
     uint32_t nrec = 0u, own = 0u;
@@ -1480,7 +1516,7 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
       if (!((rem >> x) & 1u)) continue;
       const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
       const int32_t px = bx * 8 + x;
-      rub[x] = rubble_chain(e, s_im, px, vy, vz);
+      rub[x] = rubble_chain_row(e, s_im, px, vy, vz, lane & 7u, up[x >> 1]);
       const uint32_t kx = (uint32_t)(rk >> (6 * x)) & 63u;
       if (carve_spawns(e, s_im + kx * 16u, vkey(e, px, vy, vz), v)) own |= 1u << x;
       nrec += rub[x] + ((own >> x) & 1u);
@@ -1558,9 +1594,12 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
   const uint32_t cursor = PASS == 1 ? e.ctrl[PX_C_PCURSOR] : 0u;
   const uint32_t base = PASS == 1 ? e.etot[0] : 0u;
   uint32_t acc_d = 0u, acc_b = 0u;
-  for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
-    const uint32_t nc = e.bcount[b * kCountWords + BC_PANECELLS];
-    const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
+  // the count pass visits every pane cell (the labelling's per-block lists),
+  // the apply pass the cells it found voxels to flood in
+  const uint32_t nb = PASS == 0 ? kEditBlocks : min(e.ec[EC_FLOOD_N], e.ncell);
+  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const uint32_t nc = PASS == 0 ? e.bcount[b * kCountWords + BC_PANECELLS] : 1u;
+    const uint32_t* cells = PASS == 0 ? e.pane_cells + (size_t)b * e.slots_cell : e.work + e.ncell + b;
     for (uint32_t i = 0; i < nc; i++) {
       const uint32_t ci = cells[i];
       const uint32_t g = e.grid[ci];
@@ -1573,9 +1612,14 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
       const uint4 w4 = *reinterpret_cast<const uint4*>(row);
       const uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
       const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+      const uint32_t up[4] = {__shfl_down(w4.x, 1), __shfl_down(w4.y, 1), __shfl_down(w4.z, 1),
+                              __shfl_down(w4.w, 1)};
       const uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
       const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
       uint32_t fl = 0u;
+      // the voxels of a row mostly share a label: one root lookup per run
+      uint32_t last_r = kLabNone;
+      bool last_hit = false;
       for (int x = 0; x < 8; x++) {
         const uint32_t sh = (uint32_t)(x & 1) * 16u;
         const uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
@@ -1585,12 +1629,20 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
           atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
           continue;
         }
-        if ((e.flag[e.par[okey + loff(e, r)]] & kFlagHit) == 0u) continue;
+        if (r != last_r) {
+          last_r = r;
+          last_hit = (e.flag[e.par[okey + loff(e, r)]] & kFlagHit) != 0u;
+        }
+        if (!last_hit) continue;
         // carve's own voxels (still present before carve's stores) are not flooded
         if (PASS == 0 && carve_removes(e, s_im, bx * 8 + x, vy, vz, v)) continue;
         fl |= 1u << x;
       }
       if (__ballot(fl != 0u) == 0ull) continue;
+      if (PASS == 0 && lane == 0u) {
+        const uint32_t k = atomicAdd(&e.ec[EC_FLOOD_N], 1u);
+        if (k < e.ncell) e.work[e.ncell + k] = ci;
+      }
       uint32_t rub[8];
       uint32_t nrec = 0u, own = 0u;
       for (int x = 0; x < 8; x++) {
@@ -1598,7 +1650,7 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
         if (!((fl >> x) & 1u)) continue;
         const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
         const int32_t px = bx * 8 + x;
-        rub[x] = rubble_chain(e, s_im, px, vy, vz);
+        rub[x] = rubble_chain_row(e, s_im, px, vy, vz, lane & 7u, up[x >> 1]);
         if (shard_spawns(e, vkey(e, px, vy, vz), v)) own |= 1u << x;
         nrec += rub[x] + ((own >> x) & 1u);
         if (PASS == 0 && e.mode == PX_ISLANDS_SEEDED) push_seeds(e, px, vy, vz, v);
@@ -2505,6 +2557,7 @@ EditView make_view(px_ctx* c) {
   e.ec = c->d_edit_ctr;
   e.impacts = c->d_impacts;
   e.shots = c->d_shots;
+  e.work = c->d_edit_work;
   e.flashes = c->d_flashes;
   e.max_imp = std::min<uint32_t>(c->lim.max_impacts, 64u);
   e.seeds = c->d_seeds;
@@ -2564,6 +2617,7 @@ int px_edit_alloc(px_ctx* c) {
   PX_HIP_CHECK(c, hipMalloc(&c->d_bcount, (size_t)kEditBlocks * kCountWords * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_edit_ctr, 64 * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_shots, kShotWords * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_edit_work, (size_t)c->ncell * 2 * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_flashes, PX_MAX_FLASHES * 8 * 4));
   hipLaunchKernelGGL(k_zero32, dim3(1), dim3(256), 0, c->stream, c->d_shots, (size_t)kShotWords);
   hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<uint32_t*>(c->d_flashes),
@@ -2581,7 +2635,7 @@ void px_edit_free(px_ctx* c) {
   void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags,
                   c->d_job_list, c->d_alloc_list, c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par,
                   c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds, c->d_shots,
-                  c->d_flashes};
+                  c->d_flashes, c->d_edit_work};
   for (void* p : bufs)
     if (p) hipFree(p);
   c->d_impacts = c->d_impact_stage = nullptr;
@@ -2590,6 +2644,7 @@ void px_edit_free(px_ctx* c) {
   c->d_lab = nullptr;
   c->d_nodes = c->d_out_labels = c->d_seeds = c->d_shots = nullptr;
   c->d_flashes = nullptr;
+  c->d_edit_work = nullptr;
 }
 
 // the labelling arrays are sized by the voxel count and allocated on first use
