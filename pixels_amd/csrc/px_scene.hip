@@ -403,7 +403,9 @@ int px_raster_begin(px_ctx* c, const px_prim* prims, uint32_t n, PxRaster* R, ui
       PX_HIP_CHECK(c, hipGetLastError());
       uint32_t cbits = 1;
       while (cbits < 32 && (1ull << cbits) < ncell) cbits++;
-      if (int rc = px_sort_u64(c, R->d_pairs, R->d_sorted, (uint32_t)npairs, 0u, 32u + cbits)) return rc;
+      // by cell only: each primitive's pairs are emitted at its offset, in
+      // primitive order, and the sort is stable
+      if (int rc = px_sort_u64(c, R->d_pairs, R->d_sorted, (uint32_t)npairs, 32u, 32u + cbits)) return rc;
     }
   }
   // one staging brick per pair: the brick at its first pair's index
