@@ -243,13 +243,23 @@ int px_create(const px_world_desc* d, px_ctx** out) {
     if (ok && d->pool_used)
       ok = hipMemcpy(c->d_pool, d->pool, (size_t)d->pool_used * 1024, hipMemcpyHostToDevice) == hipSuccess;
     std::vector<uint32_t> sc(c->nsuper, 0u);
+    bool grid_ok = true;
     for (uint32_t z = 0; z < c->bd[2]; z++)
       for (uint32_t y = 0; y < c->bd[1]; y++)
-        for (uint32_t x = 0; x < c->bd[0]; x++)
-          if (d->grid[x + c->bd[0] * (y + c->bd[1] * z)] != 0u)
-            sc[(x >> 2) + c->sd[0] * ((y >> 2) + c->sd[1] * (z >> 2))]++;
+        for (uint32_t x = 0; x < c->bd[0]; x++) {
+          const uint32_t g = d->grid[x + c->bd[0] * (y + c->bd[1] * z)];
+          if (g == 0u) continue;
+          // a pool brick index must name an uploaded brick
+          if ((g & PX_UNIFORM_BIT) == 0u && g - 1u >= d->pool_used) grid_ok = false;
+          sc[(x >> 2) + c->sd[0] * ((y >> 2) + c->sd[1] * (z >> 2))]++;
+        }
+    for (uint32_t i = 0; i < d->n_holes; i++)
+      if (d->holes[i] >= d->pool_used) grid_ok = false;
     if (ok) ok = hipMemcpy(c->d_super, sc.data(), (size_t)c->nsuper * 4, hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok) rc = px_fail(nullptr, PX_ERR_HIP, "px_create", "world upload failed");
+    if (!grid_ok)
+      rc = px_fail(nullptr, PX_ERR_ARG, "px_create", "the grid or a hole names a pool brick at or above pool_used");
+    else if (!ok)
+      rc = px_fail(nullptr, PX_ERR_HIP, "px_create", "world upload failed");
   }
   if (rc == PX_OK) rc = ctx_finish(c, "px_create");
   if (rc != PX_OK) {

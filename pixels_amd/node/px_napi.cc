@@ -34,13 +34,36 @@ bool get_args(napi_env env, napi_callback_info info, size_t n, napi_value* argv)
   return true;
 }
 
-px_ctx* get_ctx(napi_env env, napi_value v) {
+// A context handle: the external holds this box, so destroy() can clear it
+// and any later call fails instead of touching freed memory; the finalizer
+// frees the box (and the context, if the script never destroyed it).
+struct CtxBox {
+  px_ctx* ctx;
+};
+
+void finalize_box(napi_env, void* data, void*) {
+  CtxBox* b = static_cast<CtxBox*>(data);
+  if (b->ctx) px_destroy(b->ctx);
+  delete b;
+}
+
+CtxBox* get_box(napi_env env, napi_value v) {
   void* p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     napi_throw_type_error(env, nullptr, "expected a pixels context");
     return nullptr;
   }
-  return static_cast<px_ctx*>(p);
+  return static_cast<CtxBox*>(p);
+}
+
+px_ctx* get_ctx(napi_env env, napi_value v) {
+  CtxBox* b = get_box(env, v);
+  if (!b) return nullptr;
+  if (!b->ctx) {
+    napi_throw_error(env, nullptr, "the pixels context was destroyed");
+    return nullptr;
+  }
+  return b->ctx;
 }
 
 // pointer + byte length of a TypedArray / ArrayBuffer / Buffer
@@ -96,6 +119,18 @@ napi_value new_typed(napi_env env, napi_typedarray_type t, size_t count, size_t 
   return out;
 }
 
+napi_value wrap_ctx(napi_env env, px_ctx* ctx) {
+  CtxBox* b = new CtxBox{ctx};
+  napi_value out;
+  if (napi_create_external(env, b, finalize_box, nullptr, &out) != napi_ok) {
+    px_destroy(ctx);
+    delete b;
+    napi_throw_error(env, nullptr, "napi_create_external failed");
+    return nullptr;
+  }
+  return out;
+}
+
 // create({dims, grid, pool, poolUsed, holes, mats, poolCapacity, voxelSize, device}) -> context
 napi_value Create(napi_env env, napi_callback_info info) {
   napi_value argv[1];
@@ -111,10 +146,19 @@ napi_value Create(napi_env env, napi_callback_info info) {
   }
   void* p = nullptr;
   size_t nb = 0;
-  if (!get_bytes(env, prop(env, d, "grid"), &p, &nb)) { napi_throw_type_error(env, nullptr, "grid: Uint32Array"); return nullptr; }
+  const size_t ncell = (size_t)(desc.dims[0] / 8u) * (desc.dims[1] / 8u) * (desc.dims[2] / 8u);
+  if (!get_bytes(env, prop(env, d, "grid"), &p, &nb) || nb < ncell * 4) {
+    napi_throw_type_error(env, nullptr, "grid: Uint32Array of (dims/8) cells");
+    return nullptr;
+  }
   desc.grid = static_cast<const uint32_t*>(p);
-  if (get_bytes(env, prop(env, d, "pool"), &p, &nb)) desc.pool = static_cast<const uint16_t*>(p);
   desc.pool_used = get_u32(env, prop(env, d, "poolUsed"), 0);
+  size_t pool_bytes = 0;
+  if (get_bytes(env, prop(env, d, "pool"), &p, &pool_bytes)) desc.pool = static_cast<const uint16_t*>(p);
+  if (desc.pool_used && (!desc.pool || pool_bytes < (size_t)desc.pool_used * 1024)) {
+    napi_throw_type_error(env, nullptr, "pool: Uint16Array of poolUsed * 512 voxels");
+    return nullptr;
+  }
   if (get_bytes(env, prop(env, d, "holes"), &p, &nb)) {
     desc.holes = static_cast<const uint32_t*>(p);
     desc.n_holes = (uint32_t)(nb / 4);
@@ -131,17 +175,16 @@ napi_value Create(napi_env env, napi_callback_info info) {
   px_ctx* ctx = nullptr;
   int rc = px_create(&desc, &ctx);
   if (rc != PX_OK) return throw_px(env, nullptr, rc);
-  napi_value out;
-  NAPI_CALL(env, napi_create_external(env, ctx, nullptr, nullptr, &out));
-  return out;
+  return wrap_ctx(env, ctx);
 }
 
 napi_value Destroy(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return nullptr;
-  px_ctx* c = get_ctx(env, argv[0]);
-  if (!c) return nullptr;
-  px_destroy(c);
+  CtxBox* b = get_box(env, argv[0]);
+  if (!b) return nullptr;
+  if (b->ctx) px_destroy(b->ctx);
+  b->ctx = nullptr;  // later calls throw instead of using freed memory
   return nullptr;
 }
 
@@ -318,9 +361,7 @@ napi_value CreateFromPrims(napi_env env, napi_callback_info info) {
   px_ctx* ctx = nullptr;
   int rc = px_create_from_prims(&desc, static_cast<const px_prim*>(pp), (uint32_t)(pb / sizeof(px_prim)), &ctx);
   if (rc != PX_OK) return throw_px(env, nullptr, rc);
-  napi_value out;
-  NAPI_CALL(env, napi_create_external(env, ctx, nullptr, nullptr, &out));
-  return out;
+  return wrap_ctx(env, ctx);
 }
 
 // simStep(ctx, stagesMask): Simulation.encode's stages in the reference's order

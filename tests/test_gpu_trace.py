@@ -176,3 +176,34 @@ def test_empty_batch_and_bad_kind(small):
     assert out.shape == (0,)
     with pytest.raises(PxError):
         gw.check(gw.lib.px_trace(gw.ctx, 9, None, 0, None))
+
+
+def test_create_rejects_grid_beyond_the_uploaded_pool():
+    """px_create checks every pool brick index of the grid, and every hole,
+    against pool_used: a bad index would send the traversal's pool reads out
+    of bounds."""
+    import ctypes as C
+    from pixels_amd import _native as N
+    from pixels_amd.scene import Scene
+    mats = np.ascontiguousarray(Scene.from_dense(np.zeros((8, 8, 8), np.uint16)).mats, np.float32)
+    lib = N.load()
+    pool = np.zeros(512, np.uint16)
+    for grid, holes, ok in ((np.array([1, 0], np.uint32), None, True),
+                            (np.array([2, 0], np.uint32), None, False),
+                            (np.array([1, 0], np.uint32), np.array([1], np.uint32), False)):
+        desc = N.PxWorldDesc()
+        desc.dims[:] = (16, 8, 8)
+        desc.grid = N.ptr(grid)
+        desc.pool = N.ptr(pool, C.c_uint16)
+        desc.pool_used = 1
+        if holes is not None:
+            desc.holes = N.ptr(holes)
+            desc.n_holes = holes.size
+        desc.mats = N.ptr(mats, C.c_float)
+        ctx = C.c_void_p()
+        rc = lib.px_create(C.byref(desc), C.byref(ctx))
+        if ok:
+            assert rc == 0
+            lib.px_destroy(ctx)
+        else:
+            assert rc == -1, rc

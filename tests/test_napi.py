@@ -43,6 +43,22 @@ def test_addon_error_without_device_or_bad_args():
 
 
 @needs_node
+def test_addon_rejects_short_buffers():
+    """create() checks the grid and pool lengths against dims and poolUsed
+    before the library reads them (no device needed: it throws first)."""
+    js = ("const m = require(%r); const mats = new Float32Array(256 * 20); const out = [];"
+          "try { m.addon.create({dims:[16,16,16], grid:new Uint32Array(7), mats}); out.push('no-throw'); }"
+          " catch (e) { out.push(e.message); }"
+          "try { m.addon.create({dims:[16,16,16], grid:new Uint32Array(8), pool:new Uint16Array(600), poolUsed:2,"
+          " mats}); out.push('no-throw'); } catch (e) { out.push(e.message); }"
+          "console.log(JSON.stringify(out));" % os.path.join(ROOT, "pixels_amd", "node", "index.js"))
+    r = subprocess.run([NODE, "-e", js], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out[0].startswith("grid:") and out[1].startswith("pool:"), out
+
+
+@needs_node
 @pytest.mark.gpu
 def test_node_host_traces_and_edits(tmp_path):
     import oracle
@@ -71,8 +87,11 @@ const lab = sim.islandLabels();
 const st = sim.afterSubmit();
 sim.probe([0, 0, 0], [128, 64, 128]).then((vox) => {
   fs.writeFileSync(d + '/vox', Buffer.from(vox.buffer));
-  console.log(JSON.stringify({ n: lab.keys.length, keys: Array.from(lab.keys), counts: Array.from(lab.counts), destroyed: st.destroyed }));
+  const raw = gw.ctx;
   gw.destroy();
+  let after = 'no-throw';  // a call on a destroyed context throws, it does not touch freed memory
+  try { m.addon.trace(raw, 0, rd('rays', Uint8Array)); } catch (e) { after = e.message; }
+  console.log(JSON.stringify({ n: lab.keys.length, keys: Array.from(lab.keys), counts: Array.from(lab.counts), destroyed: st.destroyed, after: after }));
 });
 """ % (os.path.join(ROOT, "pixels_amd", "node", "index.js"), str(tmp_path), s.pool_used)
     print("case: node host", flush=True)
@@ -88,6 +107,7 @@ sim.probe([0, 0, 0], [128, 64, 128]).then((vox) => {
     assert np.array_equal(vox, ovox.astype(np.uint32))
     assert out["n"] == on and out["keys"] == okeys.tolist() and out["counts"] == ocnt.tolist()
     assert out["destroyed"] == int(stats[0])
+    assert "destroyed" in out["after"]
 
 
 @needs_node
