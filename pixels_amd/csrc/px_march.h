@@ -91,14 +91,18 @@ PX_DEV MarchTables load_march_tables(const PxWorldView& w, uint32_t* s_mem) {
 }
 
 // glassSurface (raymarch.wgsl:26-38): Fresnel term for camera rays, then the
-// tinted absorption.  dot(n, rd) with an axis-aligned unit normal is
-// +-rd[axis] (0 for the zero normal).
+// tinted absorption.  The normal is -sign(rd) along `axis` (zero for axis
+// 3), and cos = |dot(n, rd)| is taken over all three components, as the
+// WGSL does, so a NaN component of rd gives a NaN cosine.
 PX_DEV void glass_surface(const float* __restrict__ mats, float trans[3], uint32_t id, uint32_t axis,
-                          const float rd[3], bool shadow) {
+                          const float rd[3], const int32_t stp[3], bool shadow) {
   const float* A = mats + id * 20u;
   float g = mats[id * 20u + 13u];
   if (!shadow) {
-    float cosT = axis < 3u ? fabsf(axis == 0u ? rd[0] : (axis == 1u ? rd[1] : rd[2])) : 0.0f;
+    // (selects, not n[axis]: a runtime index would put the arrays in scratch)
+    const float n[3] = {axis == 0u ? -(float)stp[0] : 0.0f, axis == 1u ? -(float)stp[1] : 0.0f,
+                        axis == 2u ? -(float)stp[2] : 0.0f};
+    const float cosT = fabsf(dot3f(n, rd));
     float F = 0.04f + 0.96f * px_pow5(1.0f - cosT);
     for (int a = 0; a < 3; a++) trans[a] = trans[a] * (1.0f - F);
   }
@@ -116,8 +120,8 @@ PX_DEV void glass_surface_glow(const float* __restrict__ mats, const PxSky& sky,
   const float* A = mats + id * 20u;
   const float* Cm = A + 8;
   const float g = mats[id * 20u + 13u];
-  float n[3] = {0.0f, 0.0f, 0.0f};
-  if (axis < 3u) n[axis] = -(float)stp[axis];
+  const float n[3] = {axis == 0u ? -(float)stp[0] : 0.0f, axis == 1u ? -(float)stp[1] : 0.0f,
+                      axis == 2u ? -(float)stp[2] : 0.0f};
   const float cosT = fabsf(dot3f(n, rd));
   const float F = 0.04f + 0.96f * px_pow5(1.0f - cosT);
   const float dn = dot3f(n, rd);
@@ -144,10 +148,78 @@ PX_DEV uint32_t face_of_axis(uint32_t axis, const float rd[3]) {
   return axis * 2u + (r > 0.0f ? 1u : 0u);
 }
 
-// Every DDA step below is the WGSL's if / else-if chain written as selects:
-// axis 0 when m0 < m1 && m0 < m2, else axis 1 when m1 < m2, else axis 2
-// (ties go to the later axis); that axis's tMax advances by |inv| * scale
-// and its cell coordinate by stp.
+// Cell coordinates inside a super cell (brick level) or a brick (voxel
+// level) are packed as x | y << 10 | z << 20, so one DDA step moves them
+// with a single add of the taken axis's packed step (+-1, +-1024,
+// +-1048576).  A coordinate that leaves its range sets a guard bit of its
+// field (a step below 0 borrows, which sets every high bit of the field), so
+// one AND tests all three axes.
+constexpr uint32_t kPkY = 10u, kPkZ = 20u;
+constexpr uint32_t kVoxGuard = 0x3F8u | (0x3F8u << kPkY) | (0x3F8u << kPkZ);    // fields outside 0..7
+constexpr uint32_t kBrickGuard = 0x3FCu | (0x3FCu << kPkY) | (0x3FCu << kPkZ);  // fields outside 0..3
+
+// The zero normal (a ray starting inside the world) as a packed step; it is
+// only ever read as a face, never added to a coordinate.
+constexpr int32_t kPkNoAxis = 1 << 30;
+
+// Axis code (0..2, 3 = zero normal) of a packed step.  A step of 0 (a NaN
+// direction component, whose sign is 0) gives the zero normal -sign(rd) = 0.
+PX_DEV uint32_t pk_axis(int32_t d) {
+  const uint32_t a = (uint32_t)(d < 0 ? -d : d);
+  return a == 0u || d == kPkNoAxis ? 3u : (a >= (1u << kPkZ) ? 2u : (a >= (1u << kPkY) ? 1u : 0u));
+}
+
+// Bit x + 4 y + 16 z of a brick-level packed coordinate (fields 0..3): one
+// 24-bit multiply gathers the three fields at bit 16 without carries.
+PX_DEV uint32_t pk_brick_bit(uint32_t pb) { return (__umul24(pb, 0x10101u) >> 16) & 63u; }
+// Bit (x/2) + 4 (y/2) + 16 (z/2) of a voxel-level packed coordinate (fields
+// 0..7): the 2x2x2-block index of the sub-block mask, gathered at bit 17.
+PX_DEV uint32_t pk_block_bit(uint32_t p) {
+  return (__umul24(p & ((6u) | (6u << kPkY) | (6u << kPkZ)), 0x10101u) >> 17) & 63u;
+}
+// Local voxel index x | y << 3 | z << 6 of a voxel-level packed coordinate,
+// gathered at bit 14.
+// (The mask keeps the product a 24-bit multiply.)
+PX_DEV uint32_t pk_local_index(uint32_t p) {
+  return (__umul24(p & (7u | (7u << kPkY) | (7u << kPkZ)), 0x4081u) >> 14) & 511u;
+}
+
+// One DDA step: the WGSL's if / else-if chain as selects - axis 0 when
+// m0 < m1 && m0 < m2, else axis 1 when m1 < m2, else axis 2 (ties go to the
+// later axis).  The taken axis's crossing time advances by its increment
+// dl (|inv| scaled by the level: an exact power-of-two scaling, so the add
+// rounds once as the WGSL's tMax += tDelta does); t receives the crossing
+// time and d the packed step of the axis.
+PX_DEV void dda_step(float tm[3], float dl0, float dl1, float dl2, int32_t ps0, int32_t ps1, int32_t ps2, float& t,
+                     int32_t& d) {
+  const float m0 = tm[0], m1 = tm[1], m2 = tm[2];
+  const bool a0 = m0 < m1 && m0 < m2;
+  const bool b12 = m1 < m2;
+  t = a0 ? m0 : (b12 ? m1 : m2);
+  d = a0 ? ps0 : (b12 ? ps1 : ps2);
+  tm[0] = a0 ? m0 + dl0 : m0;
+  tm[1] = (!a0 && b12) ? m1 + dl1 : m1;
+  tm[2] = (!a0 && !b12) ? m2 + dl2 : m2;
+}
+
+// dda_step for crossing times that are never NaN: the minimum is the time
+// the if / else-if chain selects (ties go to the later axis: axis 2 when m2
+// is a minimum, else axis 1 when m1 is, else axis 0).  A tie between +0 and
+// -0 may return either zero; every use of t (comparisons, t + 1e-3, the
+// reported t with +0 for zero) is the same for both.
+PX_DEV void dda_min(float tm[3], float dl0, float dl1, float dl2, int32_t ps0, int32_t ps1, int32_t ps2, float& t,
+                    int32_t& d) {
+  const float m0 = tm[0], m1 = tm[1], m2 = tm[2];
+  const float mn = fminf(m0, fminf(m1, m2));
+  const bool e2 = m2 == mn;
+  const bool e1 = !e2 && m1 == mn;
+  t = mn;
+  d = e2 ? ps2 : (e1 ? ps1 : ps0);
+  tm[0] = (!e2 && !e1) ? m0 + dl0 : m0;
+  tm[1] = e1 ? m1 + dl1 : m1;
+  tm[2] = e2 ? m2 + dl2 : m2;
+}
+
 // With SHADE, `ms` receives the transmittance, glow (camera rays, with `sky`)
 // and inner step count of the march.
 template <bool SHADOW, bool SHADE = false>
@@ -193,11 +265,16 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
     stp[a] = rd[a] > 0.0f ? 1 : (rd[a] < 0.0f ? -1 : 0);
     pos[a] = rd[a] >= 0.0f ? 1.0f : 0.0f;
   }
-  uint32_t axisS = 3u;  // normal as an axis code
+  // (three scalars, not an array: a select among array elements may be
+  // turned into a runtime-indexed scratch load)
+  const int32_t ps0 = stp[0], ps1 = stp[1] * (1 << kPkY), ps2 = stp[2] * (1 << kPkZ);
+  const float dv0 = fabsf(inv[0]), dv1 = fabsf(inv[1]), dv2 = fabsf(inv[2]);
+  // the normal of the entry face as a packed step (only its axis is used)
+  int32_t dS = kPkNoAxis;
   if (t > 0.0f) {
-    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) axisS = 0u;
-    else if (tmn[1] >= tmn[2]) axisS = 1u;
-    else axisS = 2u;
+    if (tmn[0] >= tmn[1] && tmn[0] >= tmn[2]) dS = ps0;
+    else if (tmn[1] >= tmn[2]) dS = ps1;
+    else dS = ps2;
   }
   int32_t sc[3];
   float tMaxS[3];
@@ -208,81 +285,237 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
   }
   uint32_t glassState = 0u;  // 0x100 | id while inside glass `id`, else 0
   uint32_t steps = 0u;
+  const char* __restrict__ poolb = reinterpret_cast<const char*>(w.pool);
+
+  // The voxel at packed coordinate p of brick (bx, by, bz) as the march's
+  // hit: key, t, face of the step that entered it, value.
+  auto hit_out = [&](uint32_t bx, uint32_t by, uint32_t bz, uint32_t p, float th, int32_t d, uint32_t v) {
+    o.key = key_of(w, (int32_t)(bx * 8u + (p & 1023u)), (int32_t)(by * 8u + ((p >> kPkY) & 1023u)),
+                   (int32_t)(bz * 8u + ((p >> kPkZ) & 1023u)));
+    o.t = px_canon0(th);
+    o.face = face_of_axis(pk_axis(d), rd);
+    o.value = v;
+    o.steps = steps;
+    done();
+  };
+
+  if (__builtin_isfinite(ro[0]) && __builtin_isfinite(ro[1]) && __builtin_isfinite(ro[2]) &&
+      __builtin_isfinite(rd[0]) && __builtin_isfinite(rd[1]) && __builtin_isfinite(rd[2])) {
+    // Finite origin and direction: every crossing time is a number (never
+    // NaN) and every axis steps by +-1.  So a brick loop ends within 10 turns
+    // and a voxel loop within 22 (a coordinate leaves its range after at most
+    // 3 + 3 + 3 + 1 or 7 + 7 + 7 + 1 steps): the WGSL's caps of 14 and 25
+    // never bind and are not counted, and each DDA step can take the
+    // minimum crossing time as t (dda_min).  The glass state is kept as the
+    // value of the last voxel visited (0 after air or an empty brick or
+    // super cell): a non-zero one is always glass, since any other solid
+    // voxel ends the march.
+    uint32_t prev = 0u;
+    for (int i = 0; i < 256; i++) {
+      steps++;
+      const uint32_t sidx =
+          (uint32_t)sc[0] + __umul24((uint32_t)w.sd[0], (uint32_t)sc[1] + __umul24((uint32_t)w.sd[1], (uint32_t)sc[2]));
+      if (bit_of(T.sup, sidx)) {
+        const uint64_t bm = w.super_bm[sidx];
+        uint32_t pb = 0u, badd = 0u;
+        float tMaxB[3];
+        for (int a = 0; a < 3; a++) {
+          const int32_t sLo = sc[a] * 4;
+          const int32_t sHi = min(sLo + 3, w.bd[a] - 1);
+          const int32_t bc = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo, sHi);
+          pb |= (uint32_t)(bc - sLo) << (kPkY * a);
+          badd |= (uint32_t)(3 - (sHi - sLo)) << (kPkY * a);
+          tMaxB[a] = (((float)bc + pos[a]) * 8.0f - ro[a]) * inv[a];
+        }
+        float tb = t;
+        int32_t dB = dS;
+        for (;;) {
+          steps++;
+          if ((bm >> pk_brick_bit(pb)) & 1ull) {
+            const uint32_t bcx = (uint32_t)sc[0] * 4u + (pb & 1023u);
+            const uint32_t bcy = (uint32_t)sc[1] * 4u + ((pb >> kPkY) & 1023u);
+            const uint32_t bcz = (uint32_t)sc[2] * 4u + (pb >> kPkZ);
+            const uint32_t cidx = bcx + __umul24((uint32_t)w.bd[0], bcy + __umul24((uint32_t)w.bd[1], bcz));
+            const uint32_t g = w.grid[cidx];
+            const uint64_t subs = w.sub_mask[cidx];
+            const uint32_t bco[3] = {bcx, bcy, bcz};
+            uint32_t p = 0u;
+            float tMaxV[3];
+            for (int a = 0; a < 3; a++) {
+              const int32_t lo = (int32_t)bco[a] * 8;
+              const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo, lo + 7);
+              p |= (uint32_t)(vc - lo) << (kPkY * a);
+              tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
+            }
+            const bool uni = (g & PX_UNIFORM_BIT) != 0u;
+            if (uni && !bit_of(T.glass, g & 0xffu)) {
+              hit_out(bcx, bcy, bcz, p, tb, dB, g & 0xffffu);
+              return o;
+            }
+            const uint64_t blocks = uni ? 0ull : subs;
+            const uint32_t vuni = uni ? (g & 0xffffu) : 0u;
+            const uint32_t bbase = (g - 1u) * 1024u;
+            float tv = tb;
+            int32_t dV = dB;
+            uint32_t vhit = 0u;
+            for (;;) {
+              steps++;
+              if constexpr (SHADE) inner++;
+              uint32_t v = vuni;
+              if ((blocks >> pk_block_bit(p)) & 1ull)
+                v = *reinterpret_cast<const uint16_t*>(poolb + (bbase + pk_local_index(p) * 2u));
+              if (v != 0u) {
+                const uint32_t id = v & 0xffu;
+                bool hit = true;
+                if (bit_of(T.glass, id)) {
+                  if (prev == 0u || (prev & 0xffu) != id) {
+                    if constexpr (SHADE && !SHADOW)
+                      glass_surface_glow(w.mats, *sky, trans, glow, id, pk_axis(dV), rd, stp);
+                    else
+                      glass_surface(w.mats, trans, id, pk_axis(dV), rd, stp, SHADOW);
+                  }
+                  hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
+                }
+                if (hit) {
+                  vhit = v;
+                  break;
+                }
+              }
+              prev = v;
+              dda_min(tMaxV, dv0, dv1, dv2, ps0, ps1, ps2, tv, dV);
+              p += (uint32_t)dV;
+              if (tv > tExit || (p & kVoxGuard)) break;
+            }
+            if (vhit) {
+              hit_out(bcx, bcy, bcz, p, tv, dV, vhit);
+              return o;
+            }
+            if (tv > tExit) {
+              o.steps = steps;
+              done();
+              return o;
+            }
+          } else {
+            prev = 0u;
+          }
+          dda_min(tMaxB, dv0 * 8.0f, dv1 * 8.0f, dv2 * 8.0f, ps0, ps1, ps2, tb, dB);
+          pb += (uint32_t)dB;
+          if (tb > tExit) {
+            o.steps = steps;
+            done();
+            return o;
+          }
+          if ((pb | (pb + badd)) & kBrickGuard) break;
+        }
+      } else {
+        prev = 0u;
+      }
+      {
+        const float m0 = tMaxS[0], m1 = tMaxS[1], m2 = tMaxS[2];
+        const float mn = fminf(m0, fminf(m1, m2));
+        const bool e2 = m2 == mn;
+        const bool e1 = !e2 && m1 == mn;
+        t = mn;
+        dS = e2 ? ps2 : (e1 ? ps1 : ps0);
+        tMaxS[0] = (!e2 && !e1) ? __builtin_fmaf(dv0, 32.0f, m0) : m0;
+        tMaxS[1] = e1 ? __builtin_fmaf(dv1, 32.0f, m1) : m1;
+        tMaxS[2] = e2 ? __builtin_fmaf(dv2, 32.0f, m2) : m2;
+        sc[0] = (!e2 && !e1) ? sc[0] + stp[0] : sc[0];
+        sc[1] = e1 ? sc[1] + stp[1] : sc[1];
+        sc[2] = e2 ? sc[2] + stp[2] : sc[2];
+      }
+      if (t > tExit || (uint32_t)sc[0] >= (uint32_t)w.sd[0] || (uint32_t)sc[1] >= (uint32_t)w.sd[1] ||
+          (uint32_t)sc[2] >= (uint32_t)w.sd[2])
+        break;
+    }
+    o.steps = steps;
+    done();
+    return o;
+  }
+
+  // A NaN or infinite origin or direction: the WGSL's loops with their caps
+  // (a NaN component steps by 0, so only the caps end those loops).
 
   for (int i = 0; i < 256; i++) {
     steps++;
-    const uint32_t sidx = (uint32_t)(sc[0] + w.sd[0] * (sc[1] + w.sd[1] * sc[2]));
+    const uint32_t sidx = (uint32_t)sc[0] + __umul24((uint32_t)w.sd[0], (uint32_t)sc[1] + __umul24((uint32_t)w.sd[1], (uint32_t)sc[2]));
     if (bit_of(T.sup, sidx)) {
       const uint64_t bm = w.super_bm[sidx];
-      int32_t hiL[3], lb[3];
+      uint32_t pb = 0u, badd = 0u;  // packed brick coordinate; 3 - (last index) per axis
       float tMaxB[3];
       for (int a = 0; a < 3; a++) {
         const int32_t sLo = sc[a] * 4;
         const int32_t sHi = min(sLo + 3, w.bd[a] - 1);
-        hiL[a] = sHi - sLo;
         const int32_t bc = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo, sHi);
-        lb[a] = bc - sLo;
+        pb |= (uint32_t)(bc - sLo) << (kPkY * a);
+        badd |= (uint32_t)(3 - (sHi - sLo)) << (kPkY * a);
         tMaxB[a] = (((float)bc + pos[a]) * 8.0f - ro[a]) * inv[a];
       }
       float tb = t;
-      uint32_t axisB = axisS;
+      int32_t dB = dS;
       for (int j = 0; j < 14; j++) {
         steps++;
-        if ((bm >> (uint32_t)(lb[0] + 4 * (lb[1] + 4 * lb[2]))) & 1ull) {
-          int32_t lv[3];
-          float tMaxV[3];
-          for (int a = 0; a < 3; a++) {
-            const int32_t lo = (sc[a] * 4 + lb[a]) * 8;
-            const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo, lo + 7);
-            lv[a] = vc - lo;
-            tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
-          }
-          const uint32_t bcx = (uint32_t)(sc[0] * 4 + lb[0]), bcy = (uint32_t)(sc[1] * 4 + lb[1]),
-                         bcz = (uint32_t)(sc[2] * 4 + lb[2]);
-          const uint32_t cidx = bcx + (uint32_t)w.bd[0] * (bcy + (uint32_t)w.bd[1] * bcz);
+        if ((bm >> pk_brick_bit(pb)) & 1ull) {
+          const uint32_t bcx = (uint32_t)sc[0] * 4u + (pb & 1023u);
+          const uint32_t bcy = (uint32_t)sc[1] * 4u + ((pb >> kPkY) & 1023u);
+          const uint32_t bcz = (uint32_t)sc[2] * 4u + (pb >> kPkZ);
+          const uint32_t cidx = bcx + __umul24((uint32_t)w.bd[0], bcy + __umul24((uint32_t)w.bd[1], bcz));
           const uint32_t g = w.grid[cidx];
           // 2x2x2-block occupancy of a pool brick (read beside the grid
           // word): a voxel in an empty block is 0 without a pool read
           const uint64_t subs = w.sub_mask[cidx];
+          const uint32_t bco[3] = {bcx, bcy, bcz};
+          uint32_t p = 0u;
+          float tMaxV[3];
+          for (int a = 0; a < 3; a++) {
+            const int32_t lo = (int32_t)bco[a] * 8;
+            const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo, lo + 7);
+            p |= (uint32_t)(vc - lo) << (kPkY * a);
+            tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
+          }
           const bool uni = (g & PX_UNIFORM_BIT) != 0u;
           if (uni && !bit_of(T.glass, g & 0xffu)) {
-            o.key = key_of(w, (int32_t)bcx * 8 + lv[0], (int32_t)bcy * 8 + lv[1], (int32_t)bcz * 8 + lv[2]);
+            o.key = key_of(w, (int32_t)(bcx * 8u + (p & 1023u)), (int32_t)(bcy * 8u + ((p >> kPkY) & 1023u)),
+                           (int32_t)(bcz * 8u + (p >> kPkZ)));
             o.t = px_canon0(tb);
-            o.face = face_of_axis(axisB, rd);
+            o.face = face_of_axis(pk_axis(dB), rd);
             o.value = g & 0xffffu;
             o.steps = steps;
             done();
             return o;
           }
-          // byte offset of the brick in the pool (pool bricks only; the pool is < 4 GiB)
+          // a uniform (glass) brick reads its value from the grid word and
+          // no pool voxel; a pool brick starts at byte (g - 1) * 1024 of
+          // the pool (the pool is < 4 GiB)
+          const uint64_t blocks = uni ? 0ull : subs;
+          const uint32_t vuni = uni ? (g & 0xffffu) : 0u;
           const uint32_t bbase = (g - 1u) * 1024u;
           float tv = tb;
-          uint32_t axisV = axisB;
-          const float d0 = fabsf(inv[0]), d1 = fabsf(inv[1]), d2 = fabsf(inv[2]);
+          int32_t dV = dB;
           for (int k = 0; k < 25; k++) {
             steps++;
             if constexpr (SHADE) inner++;
-            const uint32_t li = (uint32_t)lv[0] | ((uint32_t)lv[1] << 3) | ((uint32_t)lv[2] << 6);
-            uint32_t v = 0u;
-            if (uni) v = g & 0xffffu;
-            else if ((subs >> (((uint32_t)lv[0] >> 1) | (((uint32_t)lv[1] << 1) & 12u) | (((uint32_t)lv[2] << 3) & 48u))) &
-                     1ull)
-              v = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(w.pool) + (uint64_t)(bbase + li * 2u));
+            uint32_t v = vuni;
+            if ((blocks >> pk_block_bit(p)) & 1ull)
+              v = *reinterpret_cast<const uint16_t*>(poolb + (bbase + pk_local_index(p) * 2u));
             if (v != 0u) {
               const uint32_t id = v & 0xffu;
               bool hit = true;
               if (bit_of(T.glass, id)) {
                 if (glassState != (0x100u | id)) {
-                  if constexpr (SHADE && !SHADOW) glass_surface_glow(w.mats, *sky, trans, glow, id, axisV, rd, stp);
-                  else glass_surface(w.mats, trans, id, axisV, rd, SHADOW);
+                  if constexpr (SHADE && !SHADOW)
+                    glass_surface_glow(w.mats, *sky, trans, glow, id, pk_axis(dV), rd, stp);
+                  else
+                    glass_surface(w.mats, trans, id, pk_axis(dV), rd, stp, SHADOW);
                 }
                 glassState = 0x100u | id;
                 hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
               }
               if (hit) {
-                o.key = key_of(w, (int32_t)bcx * 8 + lv[0], (int32_t)bcy * 8 + lv[1], (int32_t)bcz * 8 + lv[2]);
+                o.key = key_of(w, (int32_t)(bcx * 8u + (p & 1023u)), (int32_t)(bcy * 8u + ((p >> kPkY) & 1023u)),
+                               (int32_t)(bcz * 8u + (p >> kPkZ)));
                 o.t = px_canon0(tv);
-                o.face = face_of_axis(axisV, rd);
+                o.face = face_of_axis(pk_axis(dV), rd);
                 o.value = v;
                 o.steps = steps;
                 done();
@@ -291,48 +524,28 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
             } else {
               glassState = 0u;
             }
-            const float m0 = tMaxV[0], m1 = tMaxV[1], m2 = tMaxV[2];
-            const bool a0 = m0 < m1 && m0 < m2;
-            const bool b12 = m1 < m2;
-            tv = a0 ? m0 : (b12 ? m1 : m2);
-            axisV = a0 ? 0u : (b12 ? 1u : 2u);
-            tMaxV[0] = a0 ? m0 + d0 : m0;
-            tMaxV[1] = (!a0 && b12) ? m1 + d1 : m1;
-            tMaxV[2] = (!a0 && !b12) ? m2 + d2 : m2;
-            lv[0] = a0 ? lv[0] + stp[0] : lv[0];
-            lv[1] = (!a0 && b12) ? lv[1] + stp[1] : lv[1];
-            lv[2] = (!a0 && !b12) ? lv[2] + stp[2] : lv[2];
+            dda_step(tMaxV, dv0, dv1, dv2, ps0, ps1, ps2, tv, dV);
+            p += (uint32_t)dV;
             if (tv > tExit) {
               o.steps = steps;
               done();
               return o;
             }
-            if ((uint32_t)(lv[0] | lv[1] | lv[2]) > 7u) break;
+            if (p & kVoxGuard) break;
           }
         } else {
           glassState = 0u;
         }
-        {
-          const float m0 = tMaxB[0], m1 = tMaxB[1], m2 = tMaxB[2];
-          const bool a0 = m0 < m1 && m0 < m2;
-          const bool b12 = m1 < m2;
-          tb = a0 ? m0 : (b12 ? m1 : m2);
-          axisB = a0 ? 0u : (b12 ? 1u : 2u);
-          tMaxB[0] = a0 ? __builtin_fmaf(fabsf(inv[0]), 8.0f, m0) : m0;
-          tMaxB[1] = (!a0 && b12) ? __builtin_fmaf(fabsf(inv[1]), 8.0f, m1) : m1;
-          tMaxB[2] = (!a0 && !b12) ? __builtin_fmaf(fabsf(inv[2]), 8.0f, m2) : m2;
-          lb[0] = a0 ? lb[0] + stp[0] : lb[0];
-          lb[1] = (!a0 && b12) ? lb[1] + stp[1] : lb[1];
-          lb[2] = (!a0 && !b12) ? lb[2] + stp[2] : lb[2];
-        }
+        dda_step(tMaxB, dv0 * 8.0f, dv1 * 8.0f, dv2 * 8.0f, ps0, ps1, ps2, tb, dB);
+        pb += (uint32_t)dB;
         if (tb > tExit) {
           o.steps = steps;
           done();
           return o;
         }
-        if ((uint32_t)lb[0] > (uint32_t)hiL[0] || (uint32_t)lb[1] > (uint32_t)hiL[1] ||
-            (uint32_t)lb[2] > (uint32_t)hiL[2])
-          break;
+        // outside the super cell: a field below 0 or above 3 (pb's own guard
+        // bits), or above the last brick of a super cell the world cuts
+        if ((pb | (pb + badd)) & kBrickGuard) break;
       }
     } else {
       glassState = 0u;
@@ -342,10 +555,10 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
       const bool a0 = m0 < m1 && m0 < m2;
       const bool b12 = m1 < m2;
       t = a0 ? m0 : (b12 ? m1 : m2);
-      axisS = a0 ? 0u : (b12 ? 1u : 2u);
-      tMaxS[0] = a0 ? __builtin_fmaf(fabsf(inv[0]), 32.0f, m0) : m0;
-      tMaxS[1] = (!a0 && b12) ? __builtin_fmaf(fabsf(inv[1]), 32.0f, m1) : m1;
-      tMaxS[2] = (!a0 && !b12) ? __builtin_fmaf(fabsf(inv[2]), 32.0f, m2) : m2;
+      dS = a0 ? ps0 : (b12 ? ps1 : ps2);
+      tMaxS[0] = a0 ? __builtin_fmaf(dv0, 32.0f, m0) : m0;
+      tMaxS[1] = (!a0 && b12) ? __builtin_fmaf(dv1, 32.0f, m1) : m1;
+      tMaxS[2] = (!a0 && !b12) ? __builtin_fmaf(dv2, 32.0f, m2) : m2;
       sc[0] = a0 ? sc[0] + stp[0] : sc[0];
       sc[1] = (!a0 && b12) ? sc[1] + stp[1] : sc[1];
       sc[2] = (!a0 && !b12) ? sc[2] + stp[2] : sc[2];

@@ -143,6 +143,41 @@ def test_random_worlds_bit_exact(seed):
     gw.close()
 
 
+def test_degenerate_directions_bit_exact():
+    """Directions with NaN, zero and tiny components, origins on cell faces
+    and inside glass, on a world whose last super cells are cut by its edge
+    (136 x 40 x 72 voxels: 17 x 5 x 9 bricks).  A NaN component makes every
+    comparison with its crossing time false and its normal zero; the
+    transmittance of a camera ray through glass then reads a NaN cosine."""
+    from pixels_amd import GpuWorld, Scene, Simulation
+    rng = np.random.default_rng(77)
+    dims = (136, 40, 72)
+    vox = rand_world(rng, dims=dims, fill=0.06, glass_ids=(9, 10, 11, 12), blobs=16)
+    s = Scene.from_dense(vox)
+    print("case: degenerate directions", dims, flush=True)
+    gw = GpuWorld(s)
+    sim = Simulation(gw)
+    ow = oracle.World.from_scene(s)
+    n = 40000
+    rays = np.zeros(n, oracle.RAY_DTYPE)
+    d = np.asarray(dims, np.float32)
+    ro = rng.uniform(-0.3, 1.3, (n, 3)).astype(np.float32) * d
+    face = rng.random(n) < 0.3
+    ro[face] = np.round(ro[face] * 2) / 2
+    rd = rng.normal(size=(n, 3)).astype(np.float32)
+    rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    rd[rng.random((n, 3)) < 0.1] = 0.0
+    rd[rng.random((n, 3)) < 0.05] = 3e-7
+    rd[rng.random((n, 3)) < 0.08] = np.nan
+    rd[(rd == 0).all(1)] = [0, 0, 1]
+    rays["ro"] = ro
+    rays["rd"] = rd
+    rays["max_t"] = np.where(rng.random(n) < 0.3, rng.uniform(1, 300, n), 1e9)
+    for kind in (0, 1):
+        _assert_same(sim.trace(kind, rays), ow.trace(kind, rays), "degenerate kind %d" % kind)
+    gw.close()
+
+
 @pytest.mark.parametrize("n", [1, 63, 65, 257, 1000])
 def test_ragged_batches_bit_exact(small, n):
     """Ray counts that leave partial wavefronts and workgroups: every kind of
