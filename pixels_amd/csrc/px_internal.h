@@ -318,6 +318,19 @@ PX_DEV float px_rnd(uint32_t seed) { return (float)(px_hashu(seed) & 0xffffffu) 
 // NaN gives 0).
 PX_DEV int32_t px_f2i(float x) { return (int32_t)__builtin_amdgcn_fmed3f(x, -2147483648.0f, 2147483520.0f); }
 PX_DEV int32_t px_clampi(int32_t v, int32_t lo, int32_t hi) { return min(max(v, lo), hi); }
+// Makes the compiler forget what it knows of a VGPR value (emits nothing):
+// used after a loop so that tests of its live-out values are evaluated there
+// once, not tracked as masks on every turn of the loop.
+#define PX_OPAQUE(x) asm volatile("" : "+v"(x))
+
+// a * b + c for a, b < 2^24 as one v_mad_u32_u24 (written out: the compiler
+// otherwise widens an index that addresses memory into a 64-bit multiply).
+// b must be wave-uniform (an SGPR operand).
+PX_DEV uint32_t px_mad24(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
 PX_DEV float px_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 PX_DEV float px_canon0(float t) { return t == 0.0f ? 0.0f : t; }
 PX_DEV float px_pow5(float x) {

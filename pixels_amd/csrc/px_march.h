@@ -171,7 +171,9 @@ PX_DEV uint32_t pk_axis(int32_t d) {
 
 // Bit x + 4 y + 16 z of a brick-level packed coordinate (fields 0..3): one
 // 24-bit multiply gathers the three fields at bit 16 without carries.
-PX_DEV uint32_t pk_brick_bit(uint32_t pb) { return (__umul24(pb, 0x10101u) >> 16) & 63u; }
+PX_DEV uint32_t pk_brick_bit(uint32_t pb) {
+  return (__umul24(pb & (3u | (3u << kPkY) | (3u << kPkZ)), 0x10101u) >> 16) & 63u;
+}
 // Bit (x/2) + 4 (y/2) + 16 (z/2) of a voxel-level packed coordinate (fields
 // 0..7): the 2x2x2-block index of the sub-block mask, gathered at bit 17.
 PX_DEV uint32_t pk_block_bit(uint32_t p) {
@@ -202,22 +204,24 @@ PX_DEV void dda_step(float tm[3], float dl0, float dl1, float dl2, int32_t ps0, 
   tm[2] = (!a0 && !b12) ? m2 + dl2 : m2;
 }
 
-// dda_step for crossing times that are never NaN: the minimum is the time
+// dda_step for crossing times that are never NaN, with the increments
+// |inv| * scale (scale 1, 8 or 32: the product is exact, so the fma rounds
+// once as the WGSL's tMax += tDelta does): the minimum is the time
 // the if / else-if chain selects (ties go to the later axis: axis 2 when m2
 // is a minimum, else axis 1 when m1 is, else axis 0).  A tie between +0 and
 // -0 may return either zero; every use of t (comparisons, t + 1e-3, the
 // reported t with +0 for zero) is the same for both.
-PX_DEV void dda_min(float tm[3], float dl0, float dl1, float dl2, int32_t ps0, int32_t ps1, int32_t ps2, float& t,
-                    int32_t& d) {
+PX_DEV void dda_min(float tm[3], float dv0, float dv1, float dv2, float scale, int32_t ps0, int32_t ps1, int32_t ps2,
+                    float& t, int32_t& d) {
   const float m0 = tm[0], m1 = tm[1], m2 = tm[2];
   const float mn = fminf(m0, fminf(m1, m2));
   const bool e2 = m2 == mn;
   const bool e1 = !e2 && m1 == mn;
   t = mn;
   d = e2 ? ps2 : (e1 ? ps1 : ps0);
-  tm[0] = (!e2 && !e1) ? m0 + dl0 : m0;
-  tm[1] = e1 ? m1 + dl1 : m1;
-  tm[2] = e2 ? m2 + dl2 : m2;
+  tm[0] = (!e2 && !e1) ? __builtin_fmaf(dv0, scale, m0) : m0;
+  tm[1] = e1 ? __builtin_fmaf(dv1, scale, m1) : m1;
+  tm[2] = e2 ? __builtin_fmaf(dv2, scale, m2) : m2;
 }
 
 // With SHADE, `ms` receives the transmittance, glow (camera rays, with `sky`)
@@ -309,23 +313,34 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
     // minimum crossing time as t (dda_min).  The glass state is kept as the
     // value of the last voxel visited (0 after air or an empty brick or
     // super cell): a non-zero one is always glass, since any other solid
-    // voxel ends the march.
+    // voxel ends the march.  A cell index clamped into [lo, hi] is taken as
+    // lo + med3(floor(x) - lo, 0, hi - lo) in FP32: floor(x) - lo is exact
+    // whenever the result is not clamped, and x is a number or +-inf.
+    const char* __restrict__ gridb = reinterpret_cast<const char*>(w.grid);
+    const char* __restrict__ subb = reinterpret_cast<const char*>(w.sub_mask);
+    const char* __restrict__ sbmb = reinterpret_cast<const char*>(w.super_bm);
     uint32_t prev = 0u;
+    // the hit: brick, packed voxel, t, entry step, value
+    uint32_t hx = 0u, hy = 0u, hz = 0u, hp = 0u, hv = 0u;
+    float ht = 0.0f;
+    int32_t hd = 0;
     for (int i = 0; i < 256; i++) {
       steps++;
-      const uint32_t sidx =
-          (uint32_t)sc[0] + __umul24((uint32_t)w.sd[0], (uint32_t)sc[1] + __umul24((uint32_t)w.sd[1], (uint32_t)sc[2]));
-      if (bit_of(T.sup, sidx)) {
-        const uint64_t bm = w.super_bm[sidx];
+      const uint32_t sidx = px_mad24((uint32_t)sc[2], (uint32_t)w.sd[1], (uint32_t)sc[1]);
+      const uint32_t sidx2 = px_mad24(sidx, (uint32_t)w.sd[0], (uint32_t)sc[0]);
+      if (bit_of(T.sup, sidx2)) {
+        const uint64_t bm = *reinterpret_cast<const uint64_t*>(sbmb + sidx2 * 8u);
         uint32_t pb = 0u, badd = 0u;
         float tMaxB[3];
+        const float tt = t + 1e-3f;
         for (int a = 0; a < 3; a++) {
           const int32_t sLo = sc[a] * 4;
-          const int32_t sHi = min(sLo + 3, w.bd[a] - 1);
-          const int32_t bc = px_clampi(px_f2i(floorf((ro[a] + rd[a] * (t + 1e-3f)) / 8.0f)), sLo, sHi);
-          pb |= (uint32_t)(bc - sLo) << (kPkY * a);
-          badd |= (uint32_t)(3 - (sHi - sLo)) << (kPkY * a);
-          tMaxB[a] = (((float)bc + pos[a]) * 8.0f - ro[a]) * inv[a];
+          const int32_t hiL = min(3, w.bd[a] - 1 - sLo);
+          const float lbf = __builtin_amdgcn_fmed3f(floorf((ro[a] + rd[a] * tt) / 8.0f) - (float)sLo, 0.0f,
+                                                    (float)hiL);
+          pb |= (uint32_t)(int32_t)lbf << (kPkY * a);
+          badd |= (uint32_t)(3 - hiL) << (kPkY * a);
+          tMaxB[a] = (((float)sLo + lbf + pos[a]) * 8.0f - ro[a]) * inv[a];
         }
         float tb = t;
         int32_t dB = dS;
@@ -335,38 +350,43 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
             const uint32_t bcx = (uint32_t)sc[0] * 4u + (pb & 1023u);
             const uint32_t bcy = (uint32_t)sc[1] * 4u + ((pb >> kPkY) & 1023u);
             const uint32_t bcz = (uint32_t)sc[2] * 4u + (pb >> kPkZ);
-            const uint32_t cidx = bcx + __umul24((uint32_t)w.bd[0], bcy + __umul24((uint32_t)w.bd[1], bcz));
-            const uint32_t g = w.grid[cidx];
-            const uint64_t subs = w.sub_mask[cidx];
+            const uint32_t cidx = px_mad24(px_mad24(bcz, (uint32_t)w.bd[1], bcy), (uint32_t)w.bd[0], bcx);
+            const uint32_t g = *reinterpret_cast<const uint32_t*>(gridb + cidx * 4u);
+            const uint64_t subs = *reinterpret_cast<const uint64_t*>(subb + cidx * 8u);
             const uint32_t bco[3] = {bcx, bcy, bcz};
+            const float tt2 = tb + 1e-3f;
             uint32_t p = 0u;
             float tMaxV[3];
             for (int a = 0; a < 3; a++) {
-              const int32_t lo = (int32_t)bco[a] * 8;
-              const int32_t vc = px_clampi(px_f2i(floorf(ro[a] + rd[a] * (tb + 1e-3f))), lo, lo + 7);
-              p |= (uint32_t)(vc - lo) << (kPkY * a);
-              tMaxV[a] = ((float)vc + pos[a] - ro[a]) * inv[a];
+              const float lo = (float)(bco[a] * 8u);
+              const float lvf = __builtin_amdgcn_fmed3f(floorf(ro[a] + rd[a] * tt2) - lo, 0.0f, 7.0f);
+              p |= (uint32_t)(int32_t)lvf << (kPkY * a);
+              tMaxV[a] = ((lo + lvf) + pos[a] - ro[a]) * inv[a];
             }
+            float tv = tb;
+            int32_t dV = dB;
             const bool uni = (g & PX_UNIFORM_BIT) != 0u;
             if (uni && !bit_of(T.glass, g & 0xffu)) {
-              hit_out(bcx, bcy, bcz, p, tb, dB, g & 0xffffu);
-              return o;
+              hx = bcx, hy = bcy, hz = bcz, hp = p, ht = tv, hd = dV, hv = g & 0xffffu;
+              goto hit;
             }
             const uint64_t blocks = uni ? 0ull : subs;
             const uint32_t vuni = uni ? (g & 0xffffu) : 0u;
             const uint32_t bbase = (g - 1u) * 1024u;
-            float tv = tb;
-            int32_t dV = dB;
-            uint32_t vhit = 0u;
+            // one exit: a lane that hits keeps its voxel (no step), so after
+            // the loop tv > tExit means a miss, a coordinate outside the brick
+            // means the next brick, and anything else the hit of voxel p
+            uint32_t v;
             for (;;) {
               steps++;
               if constexpr (SHADE) inner++;
-              uint32_t v = vuni;
+              v = vuni;
               if ((blocks >> pk_block_bit(p)) & 1ull)
                 v = *reinterpret_cast<const uint16_t*>(poolb + (bbase + pk_local_index(p) * 2u));
+              bool hit = false;
               if (v != 0u) {
                 const uint32_t id = v & 0xffu;
-                bool hit = true;
+                hit = true;
                 if (bit_of(T.glass, id)) {
                   if (prev == 0u || (prev & 0xffu) != id) {
                     if constexpr (SHADE && !SHADOW)
@@ -376,35 +396,27 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
                   }
                   hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
                 }
-                if (hit) {
-                  vhit = v;
-                  break;
-                }
               }
-              prev = v;
-              dda_min(tMaxV, dv0, dv1, dv2, ps0, ps1, ps2, tv, dV);
-              p += (uint32_t)dV;
-              if (tv > tExit || (p & kVoxGuard)) break;
+              if (!hit) {
+                prev = v;
+                dda_min(tMaxV, dv0, dv1, dv2, 1.0f, ps0, ps1, ps2, tv, dV);
+                p += (uint32_t)dV;
+              }
+              if (hit || tv > tExit || (p & kVoxGuard)) break;
             }
-            if (vhit) {
-              hit_out(bcx, bcy, bcz, p, tv, dV, vhit);
-              return o;
-            }
-            if (tv > tExit) {
-              o.steps = steps;
-              done();
-              return o;
+            PX_OPAQUE(tv);
+            PX_OPAQUE(p);
+            if (tv > tExit) goto miss;
+            if (!(p & kVoxGuard)) {
+              hx = bcx, hy = bcy, hz = bcz, hp = p, ht = tv, hd = dV, hv = v;
+              goto hit;
             }
           } else {
             prev = 0u;
           }
-          dda_min(tMaxB, dv0 * 8.0f, dv1 * 8.0f, dv2 * 8.0f, ps0, ps1, ps2, tb, dB);
+          dda_min(tMaxB, dv0, dv1, dv2, 8.0f, ps0, ps1, ps2, tb, dB);
           pb += (uint32_t)dB;
-          if (tb > tExit) {
-            o.steps = steps;
-            done();
-            return o;
-          }
+          if (tb > tExit) goto miss;
           if ((pb | (pb + badd)) & kBrickGuard) break;
         }
       } else {
@@ -428,8 +440,12 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
           (uint32_t)sc[2] >= (uint32_t)w.sd[2])
         break;
     }
+  miss:
     o.steps = steps;
     done();
+    return o;
+  hit:
+    hit_out(hx, hy, hz, hp, ht, hd, hv);
     return o;
   }
 
