@@ -322,7 +322,7 @@ static void decode_shadow(px_ctx* c) {
   s.n_lights = (uint32_t)count;
   s.slots = 1u + (uint32_t)count;
   for (int i = 0; i < count; i++) {
-    for (int a = 0; a < 4; a++) s.lp[i][a] = c->h_lights[(1 + 2 * i) * 4 + a];
+    for (int a = 0; a < 4; a++) s.lpv[i][a] = c->h_lights[(1 + 2 * i) * 4 + a] / s.vs;
     s.lc_w[i] = c->h_lights[(2 + 2 * i) * 4 + 3];
   }
 }

@@ -44,7 +44,7 @@ struct PxShadowDesc {
   float lamp_shadows;  // U.flags.y
   uint32_t n_lights;
   uint32_t slots;
-  float lp[32][4];  // light pos (m), radius (m)
+  float lpv[32][4];  // light position and radius in voxels: lp / vs, divided on the host (IEEE FP32, as the WGSL)
   float lc_w[32];   // light shadow flag
 };
 

@@ -638,10 +638,10 @@ PX_DEV bool shadow_ray(const PxWorldView& w, const PxShadowDesc& sd, const float
     mt = 1e9f;
     return ndl > 0.0f && sd.sun_w > 0.0f;
   }
-  const float* lp = sd.lp[s - 1u];
+  const float* lp = sd.lpv[s - 1u];  // lp / vs
   float L[3];
-  for (int a = 0; a < 3; a++) L[a] = lp[a] / sd.vs - P[a];
-  const float radius = lp[3] / sd.vs;
+  for (int a = 0; a < 3; a++) L[a] = lp[a] - P[a];
+  const float radius = lp[3];
   const float d = sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
   if (d > radius) return false;
   for (int a = 0; a < 3; a++) L[a] = L[a] / d;

@@ -201,11 +201,12 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
   if (planar) {
     // slot-planar output (slot s of pixel i at s * n + i): lanes walk the
     // region slot by slot, so each wavefront store covers consecutive pixels
-    const uint32_t p0 = region * 64u, npix = (last - first) / slots;
-    for (uint32_t k = lane; k < npix * slots; k += 64u) {
-      const uint32_t s = k / npix, j = k - s * npix;
-      const uint32_t local = j * slots + s;
-      if (((bm[local >> 5] >> (local & 31u)) & 1u) == 0u) store_not_cast(sh_out + (size_t)s * n + p0 + j);
+    // (a region has at most 64 pixels: lane j owns pixel j)
+    const uint32_t p0 = region * 64u, npix = min(64u, n - p0);
+    for (uint32_t s = 0; s < slots; s++) {
+      const uint32_t local = lane * slots + s;
+      if (lane < npix && ((bm[local >> 5] >> (local & 31u)) & 1u) == 0u)
+        store_not_cast(sh_out + (size_t)s * n + p0 + lane);
     }
   } else {
     for (uint32_t k = first + lane; k < last; k += 64u) {
