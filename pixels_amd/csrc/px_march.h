@@ -271,7 +271,12 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
   }
   // (three scalars, not an array: a select among array elements may be
   // turned into a runtime-indexed scratch load)
-  const int32_t ps0 = stp[0], ps1 = stp[1] * (1 << kPkY), ps2 = stp[2] * (1 << kPkZ);
+  int32_t ps0 = stp[0], ps1 = stp[1] * (1 << kPkY), ps2 = stp[2] * (1 << kPkZ);
+  // (opaque, so that the fast path's unit steps are taken from these by a
+  // shift instead of keeping stp[] live, and spilled, beside them)
+  PX_OPAQUE(ps0);
+  PX_OPAQUE(ps1);
+  PX_OPAQUE(ps2);
   const float dv0 = fabsf(inv[0]), dv1 = fabsf(inv[1]), dv2 = fabsf(inv[2]);
   // the normal of the entry face as a packed step (only its axis is used)
   int32_t dS = kPkNoAxis;
@@ -389,10 +394,11 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
                 hit = true;
                 if (bit_of(T.glass, id)) {
                   if (prev == 0u || (prev & 0xffu) != id) {
+                    const int32_t st[3] = {ps0, ps1 >> kPkY, ps2 >> kPkZ};
                     if constexpr (SHADE && !SHADOW)
-                      glass_surface_glow(w.mats, *sky, trans, glow, id, pk_axis(dV), rd, stp);
+                      glass_surface_glow(w.mats, *sky, trans, glow, id, pk_axis(dV), rd, st);
                     else
-                      glass_surface(w.mats, trans, id, pk_axis(dV), rd, stp, SHADOW);
+                      glass_surface(w.mats, trans, id, pk_axis(dV), rd, st, SHADOW);
                   }
                   hit = fmaxf(trans[0], fmaxf(trans[1], trans[2])) < 0.02f;
                 }
@@ -432,9 +438,10 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
         tMaxS[0] = (!e2 && !e1) ? __builtin_fmaf(dv0, 32.0f, m0) : m0;
         tMaxS[1] = e1 ? __builtin_fmaf(dv1, 32.0f, m1) : m1;
         tMaxS[2] = e2 ? __builtin_fmaf(dv2, 32.0f, m2) : m2;
-        sc[0] = (!e2 && !e1) ? sc[0] + stp[0] : sc[0];
-        sc[1] = e1 ? sc[1] + stp[1] : sc[1];
-        sc[2] = e2 ? sc[2] + stp[2] : sc[2];
+        // (the unit steps come from the packed ones, which stay live anyway)
+        sc[0] = (!e2 && !e1) ? sc[0] + ps0 : sc[0];
+        sc[1] = e1 ? sc[1] + (ps1 >> kPkY) : sc[1];
+        sc[2] = e2 ? sc[2] + (ps2 >> kPkZ) : sc[2];
       }
       if (t > tExit || (uint32_t)sc[0] >= (uint32_t)w.sd[0] || (uint32_t)sc[1] >= (uint32_t)w.sd[1] ||
           (uint32_t)sc[2] >= (uint32_t)w.sd[2])
@@ -450,7 +457,12 @@ PX_DEV MarchOut march(const PxWorldView& w, const MarchTables& T, const float ro
   }
 
   // A NaN or infinite origin or direction: the WGSL's loops with their caps
-  // (a NaN component steps by 0, so only the caps end those loops).
+  // (a NaN component steps by 0, so only the caps end those loops).  The
+  // unit steps are taken back from the packed ones, so that stp[] is not
+  // live (and spilled) through the setup both paths share.
+  stp[0] = ps0;
+  stp[1] = ps1 >> kPkY;
+  stp[2] = ps2 >> kPkZ;
 
   for (int i = 0; i < 256; i++) {
     steps++;

@@ -173,8 +173,11 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
   // wavefronts take the regions last first: regions follow the image rows,
   // and the lower rows (near ground, whose shadow rays cross the scene) cost
   // the most, so issuing them first shortens the tail of the launch
+  // (the wavefront's region and its derived values are wave-uniform: taken
+  // through readfirstlane they live in scalar registers, not in VGPRs that
+  // would stay live through the march)
   const uint32_t n_regions = (n + 63u) / 64u;
-  const uint32_t lw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lw = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (lw >= n_regions) return;
   const uint32_t region = n_regions - 1u - lw;
   const uint32_t slots = sd.slots;
@@ -214,9 +217,17 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
       if (((bm[local >> 5] >> (local & 31u)) & 1u) == 0u) store_not_cast(sh_out + k);
     }
   }
-  for (uint32_t k = lane; k < cnt; k += 64u) {
+  // (a wave-uniform chunk base, so no per-lane counter stays live through
+  // the march: the lane's entry is kb + lane)
+  for (uint32_t kb = 0; kb < cnt; kb += 64u) {
+    const uint32_t k = kb + lane;
+    if (k >= cnt) break;
     const uint32_t code = rq[k];
     const uint32_t i = code / slots, s = code - i * slots;
+    // output slot as one 32-bit index (codes, i * slots + s, are u32 too),
+    // the only value kept through the march
+    uint32_t oidx = planar ? s * n + i : code;
+    PX_OPAQUE(oidx);
     float ro[3], rd[3], maxT;
     uint32_t flags;
     load_ray(rays + i, ro, rd, maxT, flags);
@@ -225,7 +236,7 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_shadow_queue(PxWor
     float O[3], dir[3], mt;
     shadow_ray(w, sd, ro, rd, __uint_as_float(hv.y), face, value, s, O, dir, mt);
     MarchOut m = px::march<true>(w, T, O, dir, mt);
-    store_hit(sh_out + (planar ? (size_t)s * n + i : (size_t)code), m);
+    store_hit(sh_out + oidx, m);
   }
 }
 
