@@ -274,7 +274,10 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_batch(PxWorldView 
     for (;;) {
       uint32_t j = 0u;
       if (lane == 0u) j = atomicAdd(cursors + g * kCursorStride, 1u);
-      j = __shfl(j, 0);
+      // (readfirstlane: the chunk and everything derived from it - kind,
+      // range, array pointers - are wave-uniform and live in scalar
+      // registers instead of VGPRs kept through the traversal)
+      j = __builtin_amdgcn_readfirstlane(j);
       const uint32_t c = j * kBatchGroups + g;
       if (c >= a.total_chunks) break;
       uint32_t pos = 0u;
@@ -285,7 +288,9 @@ __global__ __launch_bounds__(256) PX_MARCH_WAVES void k_trace_batch(PxWorldView 
       const px_ray* __restrict__ rays = a.rays[kind];
       px_hit* __restrict__ out = a.out[kind];
       const uint32_t* __restrict__ perm = a.perm[kind];
-      for (uint32_t k = start + lane; k < end; k += 64u) {
+      for (uint32_t kb = start; kb < end; kb += 64u) {
+        const uint32_t k = kb + lane;
+        if (k >= end) break;
         // position k of the kind's order traces ray perm[k] (k without one)
         const uint32_t i = perm ? perm[k] : k;
         float ro[3], rd[3], maxT;
