@@ -441,8 +441,10 @@ __global__ __launch_bounds__(64) void k_alloc(EditView e) {
     uint32_t p = e.free_list[top - 1u - i];
     uint32_t fill = g != 0u ? (g & 0xffffu) : 0u;
     fill |= fill << 16;
+    // no fence before publishing: nothing in this kernel reads the new brick,
+    // and the kernels after it on the stream see every store (a device-scope
+    // release per request cost 38 us of the 64-impact pass)
     reinterpret_cast<uint4*>(e.pool + (size_t)p * 256u)[lane] = make_uint4(fill, fill, fill, fill);
-    __threadfence();
     if (lane == 0) {
       e.grid[ci] = p + 1u;
       if (g == 0u) atomicAdd(&e.sgrid[super_of(e, ci)], 1u);
