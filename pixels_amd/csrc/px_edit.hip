@@ -48,11 +48,15 @@ namespace {
 constexpr uint32_t kLabNone = 0xFFFFu;
 constexpr uint32_t kFlagAnchor = 1u;
 constexpr uint32_t kFlagHit = 2u;
-// Persistent grid of the edit kernels: 32 single-wave blocks per CU (256
-// CUs).  Measured on the 64-impact island pass: 6144 blocks (what
-// k_local_ccl<0>'s LDS keeps resident) 0.84 ms, 8192 0.77 ms, 16384 0.77 ms
-// with the extraction kernels, which walk every block's list, 1.5x slower.
-constexpr uint32_t kEditBlocks = 8192;
+// Grid of the edit kernels: 16384 single-wave blocks, 64 per CU (256 CUs),
+// each taking every 16384th cell or list entry, with one node / pane-cell
+// list per block.  More blocks than stay resident lets the dispatcher even
+// out the blocks' uneven work: on the 64-impact island pass of the
+// 512 x 256 x 512 floor scene 8192 blocks took 0.95 ms and 16384 0.91 ms
+// (k_local_ccl<0> 180 -> 142 us, k_flood<0> 83 -> 56 us), 32768 0.95 ms.
+// k_face_merge<0>, which walks no list, keeps 8192 (208 -> 203 us).
+constexpr uint32_t kEditBlocks = 16384;
+constexpr uint32_t kMergeBlocks = 8192;
 constexpr uint32_t kPartWords = 8;      // per-block partials
 constexpr uint32_t kCountWords = 4;     // per-block list lengths
 
@@ -198,15 +202,23 @@ PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b, uint32_t cap, uint32
 
 // LDS union-find over the 512 voxels of one brick (links to smaller indices,
 // path halving; parents only decrease, so a racing halving store still
-// writes an ancestor).  Same order check and step bound as uf_find.
-PX_DEV uint32_t lds_find(volatile uint32_t* lab, uint32_t x, uint32_t& err) {
+// writes an ancestor).  Same order check and step bound as uf_find.  The
+// walk's reads and stores are relaxed workgroup-scope atomics: each read
+// sees other lanes' links, and they stay LDS instructions (a volatile
+// access through the generic pointer would be a flat access that also
+// waits for every global load in flight).
+PX_DEV uint32_t lds_get(uint32_t* lab, uint32_t i) {
+  return __hip_atomic_load(lab + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+PX_DEV uint32_t lds_find(uint32_t* lab, uint32_t x, uint32_t& err) {
   for (uint32_t it = 0; it < kLdsWalkCap; it++) {
-    const uint32_t p = lab[x];
+    const uint32_t p = lds_get(lab, x);
     if (p == x) return x;
     if (p > x) break;
-    const uint32_t gp = lab[p];
+    const uint32_t gp = lds_get(lab, p);
     if (gp > p) break;
-    if (gp != p) lab[x] = gp;
+    if (gp != p) __hip_atomic_store(lab + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     x = gp;
   }
   err = kErrWalk;
@@ -267,6 +279,30 @@ PX_DEV uint32_t super_of(const EditView& e, uint32_t ci) {
   return (uint32_t)((bx >> 2) + e.sd[0] * ((by >> 2) + e.sd[1] * (bz >> 2)));
 }
 
+// Cell of step k of this block in a pass over every cell: step k covers
+// cells [k G, (k + 1) G) (G = gridDim.x), so neighbouring cells are visited
+// at about the same time, and block b takes cell k G + (b + k R) mod G of
+// it.  The rotation R (odd, far from a multiple of a row or a slice) gives
+// each block cells at many heights and columns: a world's solid cells sit in
+// a few y layers, and with R = 0 the blocks whose column is in them take up
+// to 3.4x the mean number of pool bricks.
+constexpr uint32_t kStepRot = 1031u;
+PX_DEV uint32_t step_cell(uint32_t k) { return k * gridDim.x + (blockIdx.x + k * kStepRot) % gridDim.x; }
+
+// lowest set lane of a wave-uniform lane mask, and lane l's value of v
+PX_DEV uint32_t first_lane(uint64_t m) { return (uint32_t)__ffsll((unsigned long long)m) - 1u; }
+PX_DEV uint32_t lane_value(uint32_t v, uint32_t l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l); }
+PX_DEV uint64_t lane_value64(uint64_t v, uint32_t l) {
+  return ((uint64_t)lane_value((uint32_t)(v >> 32), l) << 32) | lane_value((uint32_t)v, l);
+}
+
+// lane's 16 bytes (8 voxels) of the brick grid word g names, zero unless g
+// is a pool brick
+PX_DEV uint4 brick_row(const EditView& e, uint32_t g, uint32_t lane) {
+  if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) return make_uint4(0u, 0u, 0u, 0u);
+  return reinterpret_cast<const uint4*>(e.pool + (size_t)(g - 1u) * 256u)[lane];
+}
+
 PX_DEV void set_dirty(const EditView& e, uint32_t ci) { atomicOr(&e.dirty_flags[ci >> 5], 1u << (ci & 31u)); }
 
 // flag[node] |= bits for every lane with node != PX_NONE, one atomic per
@@ -285,6 +321,27 @@ PX_DEV void wave_flag_unique(uint32_t* flag, uint32_t node, uint32_t bits) {
       if ((f & bits) != bits) atomicOr(flag + n0, bits);
     }
     if (node == n0) node = PX_NONE;
+  }
+}
+
+// wave_flag_unique over eight nodes per lane: one atomic per distinct node
+// of the wavefront.
+PX_DEV void wave_flag_unique8(uint32_t* flag, uint32_t node[8], uint32_t bits) {
+  const uint32_t lane = __lane_id();
+  for (;;) {
+    uint32_t mine = PX_NONE;
+    for (int x = 7; x >= 0; x--)
+      if (node[x] != PX_NONE) mine = node[x];
+    const uint64_t act = __ballot(mine != PX_NONE);
+    if (act == 0ull) return;
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)act) - 1u;
+    const uint32_t n0 = __shfl(mine, (int)leader);
+    if (lane == leader) {
+      const uint32_t f = __hip_atomic_load(flag + n0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((f & bits) != bits) atomicOr(flag + n0, bits);
+    }
+    for (int x = 0; x < 8; x++)
+      if (node[x] == n0) node[x] = PX_NONE;
   }
 }
 
@@ -485,124 +542,139 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
   uint32_t* my_cells = e.pane_cells + (size_t)blockIdx.x * e.slots_cell;
   uint32_t nn = 0u, nc = 0u;  // wave-uniform list lengths of this block
   uint32_t walk_err = 0u;
-  for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) {
-    uint32_t g = e.grid[ci];
-    if (g == 0u) continue;
-    int32_t bx, by, bz;
-    cell_coord(e, ci, bx, by, bz);
-    uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
-    if ((g & PX_UNIFORM_BIT) != 0u) {
-      if (MODE == 1) continue;
-      if (!s_cls[g & 0xffu]) continue;
-      if (lane == 0) {
-        e.par[okey] = okey;
-        e.cnt[okey] = 512u;
-        e.flag[okey] = by == 0 ? kFlagAnchor : 0u;
-        if (nn < e.slots_node) my_nodes[nn] = okey;
+  // Lane l reads the grid word of the block's cell of the l-th of the next
+  // 64 steps (step_cell) at once, and the brick of the next
+  // pool cell is loaded while the current one is labelled, so the loop does
+  // not wait on two dependent loads per cell.
+  for (uint32_t k0 = 0; k0 * gridDim.x < e.ncell; k0 += 64u) {
+    const uint32_t mc = step_cell(k0 + lane);
+    const uint32_t mg = mc < e.ncell ? e.grid[mc] : 0u;
+    const bool want = mg != 0u && ((mg & PX_UNIFORM_BIT) == 0u || (MODE == 0 && s_cls[mg & 0xffu]));
+    uint64_t live = __ballot(want);
+    uint4 wn = make_uint4(0u, 0u, 0u, 0u);
+    if (live) wn = brick_row(e, lane_value(mg, first_lane(live)), lane);
+    while (live) {
+      const uint32_t l = first_lane(live);
+      live &= live - 1ull;
+      const uint32_t ci = lane_value(mc, l);
+      const uint32_t g = lane_value(mg, l);
+      const uint4 w = wn;
+      if (live) wn = brick_row(e, lane_value(mg, first_lane(live)), lane);
+      int32_t bx, by, bz;
+      cell_coord(e, ci, bx, by, bz);
+      uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+      if ((g & PX_UNIFORM_BIT) != 0u) {  // a node-class uniform brick (MODE 0)
+        if (lane == 0) {
+          e.par[okey] = okey;
+          e.cnt[okey] = 512u;
+          e.flag[okey] = by == 0 ? kFlagAnchor : 0u;
+          if (nn < e.slots_node) my_nodes[nn] = okey;
+        }
+        nn++;
+        continue;
       }
-      nn++;
-      continue;
+      uint32_t p = g - 1u;
+      uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+      uint32_t cl[8];
+      uint32_t m = 0u, sm = 0u;
+      for (int x = 0; x < 8; x++) {
+        uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        cl[x] = v ? s_cls[v & 0xffu] : 0u;
+        if (cl[x]) m |= 1u << x;
+        if (v && s_str[v & 0xffu]) sm |= 1u << x;
+      }
+      if (__ballot(m != 0u) == 0ull) continue;  // no node voxel in this brick
+      uint32_t c_lo = cl[0] | (cl[1] << 8) | (cl[2] << 16) | (cl[3] << 24);
+      uint32_t c_hi = cl[4] | (cl[5] << 8) | (cl[6] << 16) | (cl[7] << 24);
+      uint32_t lx = 0u;  // x-links: voxel x continues the run of x-1
+      for (int x = 1; x < 8; x++)
+        if (cl[x] && cl[x] == cl[x - 1]) lx |= 1u << x;
+      uint32_t rs = m & ~lx;  // run starts
+      uint32_t ylo = __shfl_down(c_lo, 1), yhi = __shfl_down(c_hi, 1), ylx = __shfl_down(lx, 1);
+      uint32_t zlo = __shfl_down(c_lo, 8), zhi = __shfl_down(c_hi, 8), zlx = __shfl_down(lx, 8);
+      uint32_t sm_yp = __shfl_down(sm, 1), sm_zp = __shfl_down(sm, 8);
+      uint32_t sm_ym = __shfl_up(sm, 1), sm_zm = __shfl_up(sm, 8);
+      bool has_y = ry < 7u, has_z = rz < 7u;
+      uint32_t ey = 0u, ez = 0u;
+      for (int x = 0; x < 8; x++) {
+        uint32_t cy = ((x < 4 ? ylo : yhi) >> ((x & 3) * 8)) & 0xffu;
+        uint32_t cz = ((x < 4 ? zlo : zhi) >> ((x & 3) * 8)) & 0xffu;
+        if (has_y && cl[x] && cl[x] == cy) ey |= 1u << x;
+        if (has_z && cl[x] && cl[x] == cz) ez |= 1u << x;
+      }
+      for (int x = 0; x < 8; x++) {
+        uint32_t li = lane * 8u + (uint32_t)x;
+        uint32_t start = 31u - (uint32_t)__clz(rs & ((2u << x) - 1u));
+        s_lab[li] = ((m >> x) & 1u) ? lane * 8u + start : PX_NONE;
+        s_cnt[li] = 0u;
+        s_flg[li] = 0u;
+      }
+      __syncthreads();
+      for (int x = 0; x < 8; x++) {
+        uint32_t b = 1u << x, pb = b >> 1;
+        if ((ey & b) && !(x > 0 && (ey & pb) && (lx & b) && (ylx & b)))
+          lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x, walk_err);
+        if ((ez & b) && !(x > 0 && (ez & pb) && (lx & b) && (zlx & b)))
+          lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x, walk_err);
+      }
+      __syncthreads();
+      // anchors inside the brick (islands): y == 0 or a structural neighbour
+      uint32_t abits = 0u;
+      if (MODE == 0) {
+        uint32_t adj = ((sm << 1) | (sm >> 1)) & 0xffu;
+        if (has_y) adj |= sm_yp;
+        if (ry > 0u) adj |= sm_ym;
+        if (has_z) adj |= sm_zp;
+        if (rz > 0u) adj |= sm_zm;
+        if (by == 0 && ry == 0u) adj = 0xffu;
+        abits = m & adj;
+      }
+      uint32_t rootx[8];
+      uint32_t nroots = 0u;
+      for (int x = 0; x < 8; x++) rootx[x] = kLabNone;
+      // unrolled, so rootx[] stays in registers
+#pragma unroll
+      for (int x = 0; x < 8; x++) {
+        if (!((rs >> x) & 1u)) continue;
+        uint32_t run = 1u << x;
+        for (int y = x + 1; y < 8 && ((lx >> y) & 1u); y++) run |= 1u << y;
+        uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x, walk_err);
+        atomicAdd(&s_cnt[r], (uint32_t)__popc(run));
+        if (run & abits) atomicOr(&s_flg[r], kFlagAnchor);
+        for (int y = x; y < 8; y++)
+          if ((run >> y) & 1u) rootx[y] = r;
+        if (r == lane * 8u + (uint32_t)x) nroots++;
+      }
+      __syncthreads();
+      // roots become union-find nodes in this block's range
+      uint32_t incl = wave_scan(nroots);
+      uint32_t total = __shfl(incl, 63);
+      uint32_t slot = nn + incl - nroots;
+      for (int x = 0; x < 8; x++) {
+        if (!((rs >> x) & 1u)) continue;
+        uint32_t li = lane * 8u + (uint32_t)x;
+        if (rootx[x] != li) continue;
+        uint32_t node = okey + loff(e, li);
+        e.par[node] = node;
+        e.cnt[node] = s_cnt[li];
+        e.flag[node] = s_flg[li];
+        if (slot < e.slots_node) my_nodes[slot] = node;
+        else atomicOr(&e.ec[EC_OVERFLOW], kErrNodes);
+        slot++;
+      }
+      nn += total;
+      uint4 lw;
+      lw.x = rootx[0] | (rootx[1] << 16);
+      lw.y = rootx[2] | (rootx[3] << 16);
+      lw.z = rootx[4] | (rootx[5] << 16);
+      lw.w = rootx[6] | (rootx[7] << 16);
+      reinterpret_cast<uint4*>(e.lab + (size_t)p * 512u)[lane] = lw;
+      if (MODE == 1 && total) {
+        if (lane == 0u) my_cells[nc] = ci;
+        nc++;
+      }
+      __syncthreads();
     }
-    uint32_t p = g - 1u;
-    uint4 w = reinterpret_cast<const uint4*>(e.pool + (size_t)p * 256u)[lane];
-    uint32_t ww[4] = {w.x, w.y, w.z, w.w};
-    uint32_t cl[8];
-    uint32_t m = 0u, sm = 0u;
-    for (int x = 0; x < 8; x++) {
-      uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-      cl[x] = v ? s_cls[v & 0xffu] : 0u;
-      if (cl[x]) m |= 1u << x;
-      if (v && s_str[v & 0xffu]) sm |= 1u << x;
-    }
-    if (__ballot(m != 0u) == 0ull) continue;  // no node voxel in this brick
-    uint32_t c_lo = cl[0] | (cl[1] << 8) | (cl[2] << 16) | (cl[3] << 24);
-    uint32_t c_hi = cl[4] | (cl[5] << 8) | (cl[6] << 16) | (cl[7] << 24);
-    uint32_t lx = 0u;  // x-links: voxel x continues the run of x-1
-    for (int x = 1; x < 8; x++)
-      if (cl[x] && cl[x] == cl[x - 1]) lx |= 1u << x;
-    uint32_t rs = m & ~lx;  // run starts
-    uint32_t ylo = __shfl_down(c_lo, 1), yhi = __shfl_down(c_hi, 1), ylx = __shfl_down(lx, 1);
-    uint32_t zlo = __shfl_down(c_lo, 8), zhi = __shfl_down(c_hi, 8), zlx = __shfl_down(lx, 8);
-    uint32_t sm_yp = __shfl_down(sm, 1), sm_zp = __shfl_down(sm, 8);
-    uint32_t sm_ym = __shfl_up(sm, 1), sm_zm = __shfl_up(sm, 8);
-    bool has_y = ry < 7u, has_z = rz < 7u;
-    uint32_t ey = 0u, ez = 0u;
-    for (int x = 0; x < 8; x++) {
-      uint32_t cy = ((x < 4 ? ylo : yhi) >> ((x & 3) * 8)) & 0xffu;
-      uint32_t cz = ((x < 4 ? zlo : zhi) >> ((x & 3) * 8)) & 0xffu;
-      if (has_y && cl[x] && cl[x] == cy) ey |= 1u << x;
-      if (has_z && cl[x] && cl[x] == cz) ez |= 1u << x;
-    }
-    for (int x = 0; x < 8; x++) {
-      uint32_t li = lane * 8u + (uint32_t)x;
-      uint32_t start = 31u - (uint32_t)__clz(rs & ((2u << x) - 1u));
-      s_lab[li] = ((m >> x) & 1u) ? lane * 8u + start : PX_NONE;
-      s_cnt[li] = 0u;
-      s_flg[li] = 0u;
-    }
-    __syncthreads();
-    for (int x = 0; x < 8; x++) {
-      uint32_t b = 1u << x, pb = b >> 1;
-      if ((ey & b) && !(x > 0 && (ey & pb) && (lx & b) && (ylx & b)))
-        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 1u) * 8u + (uint32_t)x, walk_err);
-      if ((ez & b) && !(x > 0 && (ez & pb) && (lx & b) && (zlx & b)))
-        lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x, walk_err);
-    }
-    __syncthreads();
-    // anchors inside the brick (islands): y == 0 or a structural neighbour
-    uint32_t abits = 0u;
-    if (MODE == 0) {
-      uint32_t adj = ((sm << 1) | (sm >> 1)) & 0xffu;
-      if (has_y) adj |= sm_yp;
-      if (ry > 0u) adj |= sm_ym;
-      if (has_z) adj |= sm_zp;
-      if (rz > 0u) adj |= sm_zm;
-      if (by == 0 && ry == 0u) adj = 0xffu;
-      abits = m & adj;
-    }
-    uint32_t rootx[8];
-    uint32_t nroots = 0u;
-    for (int x = 0; x < 8; x++) rootx[x] = kLabNone;
-    for (int x = 0; x < 8; x++) {
-      if (!((rs >> x) & 1u)) continue;
-      uint32_t run = 1u << x;
-      for (int y = x + 1; y < 8 && ((lx >> y) & 1u); y++) run |= 1u << y;
-      uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x, walk_err);
-      atomicAdd(&s_cnt[r], (uint32_t)__popc(run));
-      if (run & abits) atomicOr(&s_flg[r], kFlagAnchor);
-      for (int y = x; y < 8; y++)
-        if ((run >> y) & 1u) rootx[y] = r;
-      if (r == lane * 8u + (uint32_t)x) nroots++;
-    }
-    __syncthreads();
-    // roots become union-find nodes in this block's range
-    uint32_t incl = wave_scan(nroots);
-    uint32_t total = __shfl(incl, 63);
-    uint32_t slot = nn + incl - nroots;
-    for (int x = 0; x < 8; x++) {
-      if (!((rs >> x) & 1u)) continue;
-      uint32_t li = lane * 8u + (uint32_t)x;
-      if (rootx[x] != li) continue;
-      uint32_t node = okey + loff(e, li);
-      e.par[node] = node;
-      e.cnt[node] = s_cnt[li];
-      e.flag[node] = s_flg[li];
-      if (slot < e.slots_node) my_nodes[slot] = node;
-      else atomicOr(&e.ec[EC_OVERFLOW], kErrNodes);
-      slot++;
-    }
-    nn += total;
-    uint4 lw;
-    lw.x = rootx[0] | (rootx[1] << 16);
-    lw.y = rootx[2] | (rootx[3] << 16);
-    lw.z = rootx[4] | (rootx[5] << 16);
-    lw.w = rootx[6] | (rootx[7] << 16);
-    reinterpret_cast<uint4*>(e.lab + (size_t)p * 512u)[lane] = lw;
-    if (MODE == 1 && total) {
-      if (lane == 0u) my_cells[nc] = ci;
-      nc++;
-    }
-    __syncthreads();
   }
   if (lane == 0u) {
     e.bcount[blockIdx.x * kCountWords + (MODE == 0 ? BC_NODES : BC_PANENODES)] = min(nn, e.slots_node);
@@ -618,11 +690,19 @@ PX_DEV uint32_t cell_voxel(const EditView& e, uint32_t g, uint32_t li) {
   return (e.pool[idx >> 1u] >> ((idx & 1u) * 16u)) & 0xffffu;
 }
 
-// Node of a voxel known to belong to a component; PX_NONE (and the internal
-// error bit in ec[EC_OVERFLOW]) if its brick label is not a local index.
-PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t li) {
+// Brick label of local index li in the brick grid word g names, read
+// whatever the voxel holds (the caller decides whether it is a node), so
+// that the read need not wait for the voxel's; 0 for a uniform brick.
+PX_DEV uint32_t raw_label(const EditView& e, uint32_t g, uint32_t li) {
+  if ((g & PX_UNIFORM_BIT) != 0u) return 0u;
+  return e.lab[(size_t)(g - 1u) * 512u + li];
+}
+
+// Node of a voxel known to belong to a component, from its brick label r;
+// PX_NONE (and the internal error bit in ec[EC_OVERFLOW]) if r is not a
+// local index.
+PX_DEV uint32_t node_from(const EditView& e, uint32_t okey, uint32_t g, uint32_t r) {
   if ((g & PX_UNIFORM_BIT) != 0u) return okey;
-  uint32_t r = e.lab[(size_t)(g - 1u) * 512u + li];
   if (r >= 512u) {
     atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
     return PX_NONE;
@@ -630,45 +710,60 @@ PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t l
   return okey + loff(e, r);
 }
 
-// Face merges between a cell and its +x / +y / +z neighbours (64 voxel pairs
-// per face, one per lane), plus anchors across faces (islands).  The three
-// faces' voxel and label reads are issued before any union, so their
+PX_DEV uint32_t node_of(const EditView& e, uint32_t okey, uint32_t g, uint32_t li) {
+  return node_from(e, okey, g, raw_label(e, g, li));
+}
+
+// grid words of cell ci and of its +x / +y / +z neighbours (0 past the
+// world's edge, and all 0 unless `in`)
+PX_DEV void cell_and_neighbours(const EditView& e, uint32_t ci, bool in, uint32_t g[4]) {
+  g[0] = g[1] = g[2] = g[3] = 0u;
+  if (!in) return;
+  int32_t bx, by, bz;
+  cell_coord(e, ci, bx, by, bz);
+  const uint32_t sy = (uint32_t)e.bd[0], sz = (uint32_t)e.bd[0] * (uint32_t)e.bd[1];
+  g[0] = e.grid[ci];
+  if (bx + 1 < e.bd[0]) g[1] = e.grid[ci + 1u];
+  if (by + 1 < e.bd[1]) g[2] = e.grid[ci + sy];
+  if (bz + 1 < e.bd[2]) g[3] = e.grid[ci + sz];
+}
+
+// Face merges between cell ci (grid word g != 0) and its +x / +y / +z
+// neighbours (grid words gn[], 0 past the world's edge): 64 voxel pairs per
+// face, one per lane, plus anchors across faces (islands).  The three faces'
+// voxel and label reads are issued together before any union, so their
 // latencies overlap.
 template <int MODE>
-PX_DEV void merge_cell(const EditView& e, uint32_t ci, const uint8_t* s_cls, const uint8_t* s_str,
-                       uint32_t& walk_err) {
+PX_DEV void merge_cell(const EditView& e, uint32_t ci, uint32_t g, const uint32_t gn[3], const uint8_t* s_cls,
+                       const uint8_t* s_str, uint32_t& walk_err) {
   const uint32_t lane = threadIdx.x;
   const uint32_t u = lane & 7u, v = lane >> 3;
-  uint32_t g = e.grid[ci];
-  if (g == 0u) return;
-  if (MODE == 1 && (g & PX_UNIFORM_BIT) != 0u) return;
+  if (g == 0u || (MODE == 1 && (g & PX_UNIFORM_BIT) != 0u)) return;
   int32_t bc[3];
   cell_coord(e, ci, bc[0], bc[1], bc[2]);
   uint32_t okey = vkey(e, bc[0] * 8, bc[1] * 8, bc[2] * 8);
-  uint32_t va[3], vb[3], na[3], nb[3];
+  uint32_t va[3], vb[3], ra[3], rb[3], na[3], nb[3];
   bool live[3];
   for (int d = 0; d < 3; d++) {
-    va[d] = vb[d] = 0u;
-    na[d] = nb[d] = PX_NONE;
-    live[d] = false;
-    if (bc[d] + 1 >= e.bd[d]) continue;
-    int32_t nc[3] = {bc[0], bc[1], bc[2]};
-    nc[d] += 1;
-    uint32_t cj = (uint32_t)(nc[0] + e.bd[0] * (nc[1] + e.bd[1] * nc[2]));
-    uint32_t gn = e.grid[cj];
-    if (gn == 0u) continue;
-    if (MODE == 1 && (gn & PX_UNIFORM_BIT) != 0u) continue;
-    live[d] = true;
+    va[d] = vb[d] = ra[d] = rb[d] = 0u;
+    live[d] = gn[d] != 0u && !(MODE == 1 && (gn[d] & PX_UNIFORM_BIT) != 0u);
+    if (!live[d]) continue;
     uint32_t la, lb;
     if (d == 0) { la = 7u | (u << 3) | (v << 6); lb = 0u | (u << 3) | (v << 6); }
     else if (d == 1) { la = u | (7u << 3) | (v << 6); lb = u | (0u << 3) | (v << 6); }
     else { la = u | (v << 3) | (7u << 6); lb = u | (v << 3) | (0u << 6); }
     va[d] = cell_voxel(e, g, la);
-    vb[d] = cell_voxel(e, gn, lb);
+    vb[d] = cell_voxel(e, gn[d], lb);
+    ra[d] = raw_label(e, g, la);
+    rb[d] = raw_label(e, gn[d], lb);
+  }
+  for (int d = 0; d < 3; d++) {
+    na[d] = nb[d] = PX_NONE;
+    if (!live[d]) continue;
     const uint32_t ca = va[d] ? s_cls[va[d] & 0xffu] : 0u, cb = vb[d] ? s_cls[vb[d] & 0xffu] : 0u;
-    const uint32_t okn = vkey(e, nc[0] * 8, nc[1] * 8, nc[2] * 8);
-    na[d] = ca ? node_of(e, okey, g, la) : PX_NONE;
-    nb[d] = cb ? node_of(e, okn, gn, lb) : PX_NONE;
+    const uint32_t okn = okey + 8u * (d == 0 ? 1u : d == 1 ? (uint32_t)e.dims[0] : (uint32_t)e.dims[0] * e.dims[1]);
+    na[d] = ca ? node_from(e, okey, g, ra[d]) : PX_NONE;
+    nb[d] = cb ? node_from(e, okn, gn[d], rb[d]) : PX_NONE;
   }
   for (int d = 0; d < 3; d++) {
     if (!live[d]) continue;
@@ -702,13 +797,39 @@ __global__ __launch_bounds__(64) void k_face_merge(EditView e) {
   load_tables<MODE>(e.mats, s_cls, s_str);
   __syncthreads();
   uint32_t walk_err = 0u;
+  const uint32_t lane = threadIdx.x;
+  // Lane l reads the grid words of the l-th of the next 64 cells and of its
+  // three neighbours at once; the cells are then merged in order, skipping
+  // air and cells whose neighbours are all air without another read.
   if (MODE == 0) {
-    for (uint32_t ci = blockIdx.x; ci < e.ncell; ci += gridDim.x) merge_cell<0>(e, ci, s_cls, s_str, walk_err);
+    for (uint32_t k0 = 0; k0 * gridDim.x < e.ncell; k0 += 64u) {
+      // plain round robin here: rotating the steps (step_cell) balances the
+      // blocks but made this kernel slower (244 against 213 us on the
+      // 512 x 256 x 512 floor scene)
+      const uint32_t mc = blockIdx.x + (k0 + lane) * gridDim.x;
+      uint32_t mg[4];
+      cell_and_neighbours(e, mc, mc < e.ncell, mg);
+      for (uint64_t live = __ballot(mg[0] != 0u && (mg[1] | mg[2] | mg[3]) != 0u); live; live &= live - 1ull) {
+        const uint32_t l = first_lane(live);
+        const uint32_t gn[3] = {lane_value(mg[1], l), lane_value(mg[2], l), lane_value(mg[3], l)};
+        merge_cell<0>(e, lane_value(mc, l), lane_value(mg[0], l), gn, s_cls, s_str, walk_err);
+      }
+    }
   } else {
     for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
-      uint32_t n = e.bcount[b * kCountWords + BC_PANECELLS];
+      const uint32_t n = e.bcount[b * kCountWords + BC_PANECELLS];
       const uint32_t* cells = e.pane_cells + (size_t)b * e.slots_cell;
-      for (uint32_t i = 0; i < n; i++) merge_cell<1>(e, cells[i], s_cls, s_str, walk_err);
+      for (uint32_t i0 = 0; i0 < n; i0 += 64u) {
+        const bool in = i0 + lane < n;
+        const uint32_t mc = in ? cells[i0 + lane] : 0u;
+        uint32_t mg[4];
+        cell_and_neighbours(e, mc, in, mg);
+        for (uint64_t live = __ballot(mg[0] != 0u && (mg[1] | mg[2] | mg[3]) != 0u); live; live &= live - 1ull) {
+          const uint32_t l = first_lane(live);
+          const uint32_t gn[3] = {lane_value(mg[1], l), lane_value(mg[2], l), lane_value(mg[3], l)};
+          merge_cell<1>(e, lane_value(mc, l), lane_value(mg[0], l), gn, s_cls, s_str, walk_err);
+        }
+      }
     }
   }
   if (__ballot(walk_err != 0u) != 0ull && threadIdx.x == 0u) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
@@ -1288,19 +1409,52 @@ PX_DEV uint32_t rubble_chain(const EditView& e, const float* __restrict__ s_im, 
   return n;
 }
 
-// rubble_chain with its first probe answered from the brick's own rows: the
-// voxel above a row's voxel x is in the row of the next lane (y + 1, same z)
-// unless the row is the brick's top one.  `up` is that row's word x / 2.
-// Only a rubble voxel there (rare) or the top row reads the world; no other
-// cell's wavefront changes the voxel right above a removed one (a rubble
-// chain stops at a removed or pane voxel), so the brick's copy is current.
+// rubble_chain with its first probe answered from a copy of the row above:
+// the voxel above a row's voxel x is in the row of the next lane (y + 1,
+// same z), or for the brick's top row in the bottom row of the cell above.
+// `up` is word x / 2 of that row and `up_pool` whether it lies in a pool
+// brick (rubble_chain stops at once otherwise).  Only a rubble voxel there
+// (rare) reads the world.  The copy may predate another wavefront's clear of
+// that voxel, and the result is the same either way: no wavefront but this
+// one drops the rubble right above a removed voxel (a chain from further
+// down stops at the removed voxel), and a rubble voxel that carve removes
+// ends the chain whether its clear has landed or not.
 PX_DEV uint32_t rubble_chain_row(const EditView& e, const float* __restrict__ s_im, int32_t x, int32_t y, int32_t z,
-                                 uint32_t ry, uint32_t up) {
-  if (ry < 7u) {
-    const uint32_t va = (up >> ((uint32_t)(x & 1) * 16u)) & 0xffffu;
-    if (va == 0u || (va & 0x80u) == 0u) return 0u;
-  }
+                                 bool up_pool, uint32_t up) {
+  if (!up_pool) return 0u;
+  const uint32_t va = (up >> ((uint32_t)(x & 1) * 16u)) & 0xffffu;
+  if (va == 0u || (va & 0x80u) == 0u) return 0u;
   return rubble_chain(e, s_im, x, y, z);
+}
+
+// grid word of the cell above cell ci (0 on the top layer or unless `in`)
+PX_DEV uint32_t grid_above(const EditView& e, uint32_t ci, bool in) {
+  if (!in) return 0u;
+  const uint32_t sy = (uint32_t)e.bd[0];
+  if ((ci / sy) % (uint32_t)e.bd[1] + 1u >= (uint32_t)e.bd[1]) return 0u;
+  return e.grid[ci + sy];
+}
+
+// For the lanes of a brick's top row (ry = 7, lane = 7 + 8 z): the bottom
+// row (y = 0, same z, row 8 z) of the brick that grid word g_up names, if
+// it is a pool brick; zero otherwise and on the other lanes.
+PX_DEV uint4 top_row_above(const EditView& e, uint32_t g_up, uint32_t lane) {
+  if ((lane & 7u) != 7u || g_up == 0u || (g_up & PX_UNIFORM_BIT) != 0u) return make_uint4(0u, 0u, 0u, 0u);
+  return reinterpret_cast<const uint4*>(e.pool + (size_t)(g_up - 1u) * 256u)[lane - 7u];
+}
+
+// The row above this lane's row of a pool brick (own = the lane's row):
+// the next lane's row, or on the brick's top row the row top_row_above()
+// read from the brick above (grid word g_up).  `pool` says whether that row
+// lies in a pool brick.
+PX_DEV void row_above(uint4 own, uint4 above, uint32_t g_up, uint32_t lane, uint32_t up[4], bool& pool) {
+  const uint4 n = make_uint4(__shfl_down(own.x, 1), __shfl_down(own.y, 1), __shfl_down(own.z, 1), __shfl_down(own.w, 1));
+  const bool top = (lane & 7u) == 7u;
+  pool = !top || (g_up != 0u && (g_up & PX_UNIFORM_BIT) == 0u);
+  up[0] = top ? above.x : n.x;
+  up[1] = top ? above.y : n.y;
+  up[2] = top ? above.z : n.z;
+  up[3] = top ? above.w : n.w;
 }
 
 // pushSeed (sim.wgsl:180-184) of the neighbours of a removed voxel of value
@@ -1446,132 +1600,172 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
   // the count pass visits every job cell, the apply pass the cells it found
   // removals in
   const uint32_t n = PASS == 0 ? min(e.ec[EC_JOBCELLS], e.ncell) : min(e.ec[EC_CARVE_N], e.ncell);
-  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
-    const uint32_t ci = PASS == 0 ? e.job_list[i] : e.work[i];
-    const uint64_t mask = e.imask[ci];
-    const uint32_t g = e.grid[ci];
-    if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
-    int32_t bx, by, bz;
-    cell_coord(e, ci, bx, by, bz);
-    const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
-    const uint32_t p = g - 1u;
-    uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
-    const uint4 w4 = *reinterpret_cast<const uint4*>(row);
-    const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
-    const uint32_t up[4] = {__shfl_down(w4.x, 1), __shfl_down(w4.y, 1), __shfl_down(w4.z, 1), __shfl_down(w4.w, 1)};
-    const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
-    const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
-    // per-voxel terms of removes() that do not depend on the impact
-    uint32_t alive = 0u, frag = 0u, soft = 0u;
-    float jit[8], m14[8];
-    for (int x = 0; x < 8; x++) {
-      const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-      jit[x] = 0.0f;
-      m14[x] = 0.0f;
-      if (!v) continue;
-      alive |= 1u << x;
-      const float* E = e.mats + (v & 0xffu) * 20u + 16u;
-      if (E[1] > 0.5f) frag |= 1u << x;
-      if (E[0] > 0.0f) soft |= 1u << x;
-      m14[x] = fminf(1.4f, E[0]);
-      jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
+  // Lane l reads list entry, impact mask and grid word of the block's l-th
+  // of the next 64 cells at once.  The count pass also loads the brick of
+  // the next pool cell while the current one is carved (the apply pass reads
+  // each brick in turn, after its own earlier stores).
+  for (uint32_t i0 = blockIdx.x; i0 < n; i0 += 64u * gridDim.x) {
+    const uint32_t mi = i0 + lane * gridDim.x;
+    const uint32_t mc = mi < n ? (PASS == 0 ? e.job_list[mi] : e.work[mi]) : 0u;
+    const uint32_t mg = mi < n ? e.grid[mc] : 0u;
+    const uint64_t mm = mi < n ? e.imask[mc] : 0ull;
+    const uint32_t mu = grid_above(e, mc, mi < n);
+    uint64_t live = __ballot(mg != 0u && (mg & PX_UNIFORM_BIT) == 0u);
+    uint4 wn = make_uint4(0u, 0u, 0u, 0u), an = wn;
+    if (PASS == 0 && live) {
+      wn = brick_row(e, lane_value(mg, first_lane(live)), lane);
+      an = top_row_above(e, lane_value(mu, first_lane(live)), lane);
     }
-    uint32_t rem = 0u;
-    uint64_t rk = 0ull;  // remover of removed voxel x in bits 6x .. 6x + 5
-    uint64_t m = mask;
-    while (m) {
-      const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-      m &= m - 1ull;
-      const float* im = s_im + k * 16u;
-      const float b2 = s_bound2[k];
-      const float r = im[3], fr = im[4];
-      const float dy = fy - im[1], dz = fz - im[2];
-      for (int x = 0; x < 8; x++) {
-        const uint32_t bit = 1u << x;
-        if (!(alive & bit) || (rem & bit)) continue;
-        const float dx = ((float)(bx * 8 + x) + 0.5f) - im[0];
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < b2)) continue;
-        const float j = jit[x];
-        bool hit = false;
-        if ((frag & bit) && d2 < fr * fr * j * j) hit = true;
-        else if (soft & bit) {
-          const float rr = fmaxf(r * m14[x], fminf(r, 2.5f)) * j;
-          hit = d2 < rr * rr;
+    while (live) {
+      const uint32_t l = first_lane(live);
+      live &= live - 1ull;
+      const uint32_t ci = lane_value(mc, l);
+      const uint32_t g = lane_value(mg, l);
+      const uint32_t g_up = lane_value(mu, l);
+      const uint64_t mask = lane_value64(mm, l);
+      int32_t bx, by, bz;
+      cell_coord(e, ci, bx, by, bz);
+      const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+      const uint32_t p = g - 1u;
+      uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
+      uint4 w4, a4;
+      if (PASS == 0) {
+        w4 = wn;
+        a4 = an;
+        if (live) {
+          wn = brick_row(e, lane_value(mg, first_lane(live)), lane);
+          an = top_row_above(e, lane_value(mu, first_lane(live)), lane);
         }
-        if (hit) {
-          rem |= bit;
-          rk |= (uint64_t)k << (6 * x);
-        }
+      } else {
+        w4 = *reinterpret_cast<const uint4*>(row);
+        a4 = top_row_above(e, g_up, lane);
       }
-    }
-    if (__ballot(rem != 0u) == 0ull) continue;
-    if (PASS == 0 && lane == 0u) {
-      const uint32_t k = atomicAdd(&e.ec[EC_CARVE_N], 1u);
-      if (k < e.ncell) e.work[k] = ci;
-    }
-    // records of this lane's removed voxels
-    uint32_t rub[8];
-    uint32_t nrec = 0u, own = 0u;
-    for (int x = 0; x < 8; x++) {
-      rub[x] = 0u;
-      if (!((rem >> x) & 1u)) continue;
-      const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-      const int32_t px = bx * 8 + x;
-      rub[x] = rubble_chain_row(e, s_im, px, vy, vz, lane & 7u, up[x >> 1]);
-      const uint32_t kx = (uint32_t)(rk >> (6 * x)) & 63u;
-      if (carve_spawns(e, s_im + kx * 16u, vkey(e, px, vy, vz), v)) own |= 1u << x;
-      nrec += rub[x] + ((own >> x) & 1u);
-    }
-    if (PASS == 0) {
+      const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+      uint32_t up[4];
+      bool up_pool;
+      row_above(w4, a4, g_up, lane, up, up_pool);
+      const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
+      const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
+      // per-voxel terms of removes() that do not depend on the impact
+      uint32_t alive = 0u, frag = 0u, soft = 0u;
+      float jit[8], m14[8];
       for (int x = 0; x < 8; x++) {
-        uint32_t hit_root = PX_NONE;
-        if ((rem >> x) & 1u) {
-          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-          if (e.mode == PX_ISLANDS_SEEDED) push_seeds(e, bx * 8 + x, vy, vz, v);
-          if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) {
-            const uint32_t r = e.lab[(size_t)p * 512u + lane * 8u + (uint32_t)x];
-            if (r < 512u) hit_root = e.par[okey + loff(e, r)];
-            else atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
+        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        jit[x] = 0.0f;
+        m14[x] = 0.0f;
+        if (!v) continue;
+        alive |= 1u << x;
+        const float* E = e.mats + (v & 0xffu) * 20u + 16u;
+        if (E[1] > 0.5f) frag |= 1u << x;
+        if (E[0] > 0.0f) soft |= 1u << x;
+        m14[x] = fminf(1.4f, E[0]);
+        jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
+      }
+      uint32_t rem = 0u;
+      uint64_t rk = 0ull;  // remover of removed voxel x in bits 6x .. 6x + 5
+      uint64_t m = mask;
+      while (m) {
+        const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+        m &= m - 1ull;
+        const float* im = s_im + k * 16u;
+        const float b2 = s_bound2[k];
+        const float r = im[3], fr = im[4];
+        const float dy = fy - im[1], dz = fz - im[2];
+        for (int x = 0; x < 8; x++) {
+          const uint32_t bit = 1u << x;
+          if (!(alive & bit) || (rem & bit)) continue;
+          const float dx = ((float)(bx * 8 + x) + 0.5f) - im[0];
+          const float d2 = dx * dx + dy * dy + dz * dz;
+          if (!(d2 < b2)) continue;
+          const float j = jit[x];
+          bool hit = false;
+          if ((frag & bit) && d2 < fr * fr * j * j) hit = true;
+          else if (soft & bit) {
+            const float rr = fmaxf(r * m14[x], fminf(r, 2.5f)) * j;
+            hit = d2 < rr * rr;
+          }
+          if (hit) {
+            rem |= bit;
+            rk |= (uint64_t)k << (6 * x);
           }
         }
-        // the pane component this voxel belongs to is hit (flooded later)
-        wave_flag_unique(e.flag, hit_root, kFlagHit);
       }
-      const uint32_t t = wave_sum(nrec);
-      if (lane == 0u) e.cpair[ci].x = t;
-      continue;
-    }
-    // PASS 1
-    uint32_t wmask[4] = {0u, 0u, 0u, 0u};
-    uint32_t destroyed = 0u, bill = 0u;
-    for (int x = 0; x < 8; x++)
-      if ((rem >> x) & 1u) {
-        const uint32_t sh = (uint32_t)(x & 1) * 16u;
-        wmask[x >> 1] |= 0xffffu << sh;
-        destroyed++;
-        bill += price_dimes(e.mats, (ww[x >> 1] >> sh) & 0xffffu);
+      if (__ballot(rem != 0u) == 0ull) continue;
+      if (PASS == 0 && lane == 0u) {
+        const uint32_t k = atomicAdd(&e.ec[EC_CARVE_N], 1u);
+        if (k < e.ncell) e.work[k] = ci;
       }
-    for (int q = 0; q < 4; q++)
-      if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
-    uint32_t j = e.coff[ci].x + wave_scan(nrec) - nrec;
-    for (int x = 0; x < 8; x++) {
-      if (!((rem >> x) & 1u)) continue;
-      const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-      const int32_t px = bx * 8 + x;
-      drop_rubble(e, px, vy, vz, rub[x], j, accepted, cursor);
-      j += rub[x];
-      if ((own >> x) & 1u) {
+      // records of this lane's removed voxels
+      uint32_t rub[8];
+      uint32_t nrec = 0u, own = 0u;
+      for (int x = 0; x < 8; x++) {
+        rub[x] = 0u;
+        if (!((rem >> x) & 1u)) continue;
+        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        const int32_t px = bx * 8 + x;
+        rub[x] = rubble_chain_row(e, s_im, px, vy, vz, up_pool, up[x >> 1]);
         const uint32_t kx = (uint32_t)(rk >> (6 * x)) & 63u;
-        carve_record(e, s_im + kx * 16u, px, vy, vz, vkey(e, px, vy, vz), v, j, accepted, cursor);
-        j++;
+        if (carve_spawns(e, s_im + kx * 16u, vkey(e, px, vy, vz), v)) own |= 1u << x;
+        nrec += rub[x] + ((own >> x) & 1u);
       }
+      if (PASS == 0) {
+        uint32_t pane = 0u;
+        for (int x = 0; x < 8; x++) {
+          if (!((rem >> x) & 1u)) continue;
+          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+          if (e.mode == PX_ISLANDS_SEEDED) push_seeds(e, bx * 8 + x, vy, vz, v);
+          if (e.mats[(v & 0xffu) * 20u + 17u] > 1.5f) pane |= 1u << x;
+        }
+        // the pane components of the removed pane voxels are hit (flooded
+        // later): the row's labels in one read, their roots read together
+        if (__ballot(pane != 0u) != 0ull) {
+          const uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
+          const uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
+          uint32_t hit_root[8];
+          for (int x = 0; x < 8; x++) {
+            hit_root[x] = PX_NONE;
+            if (!((pane >> x) & 1u)) continue;
+            const uint32_t r = (ll[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+            if (r < 512u) hit_root[x] = e.par[okey + loff(e, r)];
+            else atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
+          }
+          wave_flag_unique8(e.flag, hit_root, kFlagHit);
+        }
+        const uint32_t t = wave_sum(nrec);
+        if (lane == 0u) e.cpair[ci].x = t;
+        continue;
+      }
+      // PASS 1
+      uint32_t wmask[4] = {0u, 0u, 0u, 0u};
+      uint32_t destroyed = 0u, bill = 0u;
+      for (int x = 0; x < 8; x++)
+        if ((rem >> x) & 1u) {
+          const uint32_t sh = (uint32_t)(x & 1) * 16u;
+          wmask[x >> 1] |= 0xffffu << sh;
+          destroyed++;
+          bill += price_dimes(e.mats, (ww[x >> 1] >> sh) & 0xffffu);
+        }
+      for (int q = 0; q < 4; q++)
+        if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
+      uint32_t j = e.coff[ci].x + wave_scan(nrec) - nrec;
+      for (int x = 0; x < 8; x++) {
+        if (!((rem >> x) & 1u)) continue;
+        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+        const int32_t px = bx * 8 + x;
+        drop_rubble(e, px, vy, vz, rub[x], j, accepted, cursor);
+        j += rub[x];
+        if ((own >> x) & 1u) {
+          const uint32_t kx = (uint32_t)(rk >> (6 * x)) & 63u;
+          carve_record(e, s_im + kx * 16u, px, vy, vz, vkey(e, px, vy, vz), v, j, accepted, cursor);
+          j++;
+        }
+      }
+      destroyed = wave_sum(destroyed);
+      bill = wave_sum(bill);
+      acc_d += destroyed;
+      acc_b += bill;
+      if (lane == 0u && destroyed) set_dirty(e, ci);
     }
-    destroyed = wave_sum(destroyed);
-    bill = wave_sum(bill);
-    acc_d += destroyed;
-    acc_b += bill;
-    if (lane == 0u && destroyed) set_dirty(e, ci);
   }
   if (PASS == 1 && lane == 0u) {
     e.part[blockIdx.x * kPartWords + PT_CARVE_D] = acc_d;
@@ -1598,98 +1792,121 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
   uint32_t acc_d = 0u, acc_b = 0u;
   // the count pass visits every pane cell (the labelling's per-block lists),
   // the apply pass the cells it found voxels to flood in
-  const uint32_t nb = PASS == 0 ? kEditBlocks : min(e.ec[EC_FLOOD_N], e.ncell);
-  for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-    const uint32_t nc = PASS == 0 ? e.bcount[b * kCountWords + BC_PANECELLS] : 1u;
-    const uint32_t* cells = PASS == 0 ? e.pane_cells + (size_t)b * e.slots_cell : e.work + e.ncell + b;
-    for (uint32_t i = 0; i < nc; i++) {
-      const uint32_t ci = cells[i];
-      const uint32_t g = e.grid[ci];
-      if (g == 0u || (g & PX_UNIFORM_BIT) != 0u) continue;
-      int32_t bx, by, bz;
-      cell_coord(e, ci, bx, by, bz);
-      const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
-      const uint32_t p = g - 1u;
-      uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
-      const uint4 w4 = *reinterpret_cast<const uint4*>(row);
-      const uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
-      const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
-      const uint32_t up[4] = {__shfl_down(w4.x, 1), __shfl_down(w4.y, 1), __shfl_down(w4.z, 1),
-                              __shfl_down(w4.w, 1)};
-      const uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
-      const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
-      uint32_t fl = 0u;
-      // the voxels of a row mostly share a label: one root lookup per run
-      uint32_t last_r = kLabNone;
-      bool last_hit = false;
-      for (int x = 0; x < 8; x++) {
-        const uint32_t sh = (uint32_t)(x & 1) * 16u;
-        const uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
-        const uint32_t r = (ll[x >> 1] >> sh) & 0xffffu;
-        if (v == 0u || r == kLabNone) continue;
-        if (r >= 512u) {
-          atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
+  // The count pass walks every block's pane-cell list, the apply pass the
+  // list of cells the count pass found voxels to flood in.  Lane l reads the
+  // l-th of the next 64 list entries and its grid word at once.
+  const uint32_t nb = PASS == 0 ? kEditBlocks : 1u;
+  for (uint32_t b = PASS == 0 ? blockIdx.x : 0u; b < nb; b += gridDim.x) {
+    const uint32_t nc = PASS == 0 ? e.bcount[b * kCountWords + BC_PANECELLS] : min(e.ec[EC_FLOOD_N], e.ncell);
+    const uint32_t* cells = PASS == 0 ? e.pane_cells + (size_t)b * e.slots_cell : e.work + e.ncell;
+    const uint32_t first = PASS == 0 ? 0u : blockIdx.x, step = PASS == 0 ? 1u : gridDim.x;
+    for (uint32_t i0 = first; i0 < nc; i0 += 64u * step) {
+      const uint32_t mi = i0 + lane * step;
+      const uint32_t mc = mi < nc ? cells[mi] : 0u;
+      const uint32_t mg = mi < nc ? e.grid[mc] : 0u;
+      const uint32_t mu = grid_above(e, mc, mi < nc);
+      for (uint64_t live = __ballot(mg != 0u && (mg & PX_UNIFORM_BIT) == 0u); live; live &= live - 1ull) {
+        const uint32_t l = first_lane(live);
+        const uint32_t ci = lane_value(mc, l);
+        const uint32_t g = lane_value(mg, l);
+        const uint32_t g_up = lane_value(mu, l);
+        int32_t bx, by, bz;
+        cell_coord(e, ci, bx, by, bz);
+        const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+        const uint32_t p = g - 1u;
+        uint32_t* row = e.pool + (size_t)p * 256u + lane * 4u;
+        const uint4 w4 = *reinterpret_cast<const uint4*>(row);
+        const uint4 l4 = reinterpret_cast<const uint4*>(e.lab + (size_t)p * 512u)[lane];
+        const uint4 a4 = top_row_above(e, g_up, lane);
+        const uint32_t ww[4] = {w4.x, w4.y, w4.z, w4.w};
+        uint32_t up[4];
+        bool up_pool;
+        row_above(w4, a4, g_up, lane, up, up_pool);
+        const uint32_t ll[4] = {l4.x, l4.y, l4.z, l4.w};
+        const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
+        // The voxels of a row mostly share a label: one root lookup per run of
+        // a label (bit x of `look`), all issued together, then their flags.
+        uint32_t lab[8], look = 0u, valid = 0u, last_r = kLabNone;
+        for (int x = 0; x < 8; x++) {
+          const uint32_t sh = (uint32_t)(x & 1) * 16u;
+          const uint32_t v = (ww[x >> 1] >> sh) & 0xffffu;
+          const uint32_t r = (ll[x >> 1] >> sh) & 0xffffu;
+          lab[x] = r;
+          if (v == 0u || r == kLabNone) continue;
+          if (r >= 512u) {
+            atomicOr(&e.ec[EC_OVERFLOW], kErrLabel);
+            continue;
+          }
+          valid |= 1u << x;
+          if (r != last_r) look |= 1u << x;
+          last_r = r;
+        }
+        uint32_t root[8], hit = 0u;
+        for (int x = 0; x < 8; x++) root[x] = ((look >> x) & 1u) ? e.par[okey + loff(e, lab[x])] : 0u;
+        for (int x = 0; x < 8; x++)
+          if (((look >> x) & 1u) && (e.flag[root[x]] & kFlagHit) != 0u) hit |= 1u << x;
+        uint32_t fl = 0u;
+        bool last_hit = false;
+        for (int x = 0; x < 8; x++) {
+          if (!((valid >> x) & 1u)) continue;
+          if ((look >> x) & 1u) last_hit = ((hit >> x) & 1u) != 0u;
+          if (!last_hit) continue;
+          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+          // carve's own voxels (still present before carve's stores) are not flooded
+          if (PASS == 0 && carve_removes(e, s_im, bx * 8 + x, vy, vz, v)) continue;
+          fl |= 1u << x;
+        }
+        if (__ballot(fl != 0u) == 0ull) continue;
+        if (PASS == 0 && lane == 0u) {
+          const uint32_t k = atomicAdd(&e.ec[EC_FLOOD_N], 1u);
+          if (k < e.ncell) e.work[e.ncell + k] = ci;
+        }
+        uint32_t rub[8];
+        uint32_t nrec = 0u, own = 0u;
+        for (int x = 0; x < 8; x++) {
+          rub[x] = 0u;
+          if (!((fl >> x) & 1u)) continue;
+          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+          const int32_t px = bx * 8 + x;
+          rub[x] = rubble_chain_row(e, s_im, px, vy, vz, up_pool, up[x >> 1]);
+          if (shard_spawns(e, vkey(e, px, vy, vz), v)) own |= 1u << x;
+          nrec += rub[x] + ((own >> x) & 1u);
+          if (PASS == 0 && e.mode == PX_ISLANDS_SEEDED) push_seeds(e, px, vy, vz, v);
+        }
+        if (PASS == 0) {
+          const uint32_t t = wave_sum(nrec);
+          if (lane == 0u) e.cpair[ci].y = t;
           continue;
         }
-        if (r != last_r) {
-          last_r = r;
-          last_hit = (e.flag[e.par[okey + loff(e, r)]] & kFlagHit) != 0u;
+        uint32_t wmask[4] = {0u, 0u, 0u, 0u};
+        uint32_t destroyed = 0u, bill = 0u;
+        for (int x = 0; x < 8; x++)
+          if ((fl >> x) & 1u) {
+            const uint32_t sh = (uint32_t)(x & 1) * 16u;
+            wmask[x >> 1] |= 0xffffu << sh;
+            destroyed++;
+            bill += price_dimes(e.mats, (ww[x >> 1] >> sh) & 0xffffu);
+          }
+        for (int q = 0; q < 4; q++)
+          if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
+        uint32_t j = base + e.coff[ci].y + wave_scan(nrec) - nrec;
+        for (int x = 0; x < 8; x++) {
+          if (!((fl >> x) & 1u)) continue;
+          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+          const int32_t px = bx * 8 + x;
+          drop_rubble(e, px, vy, vz, rub[x], j, accepted, cursor);
+          j += rub[x];
+          if ((own >> x) & 1u) {
+            shard_record(e, px, vy, vz, vkey(e, px, vy, vz), v, j, accepted, cursor);
+            j++;
+          }
         }
-        if (!last_hit) continue;
-        // carve's own voxels (still present before carve's stores) are not flooded
-        if (PASS == 0 && carve_removes(e, s_im, bx * 8 + x, vy, vz, v)) continue;
-        fl |= 1u << x;
+        destroyed = wave_sum(destroyed);
+        bill = wave_sum(bill);
+        acc_d += destroyed;
+        acc_b += bill;
+        if (lane == 0u && destroyed) set_dirty(e, ci);
       }
-      if (__ballot(fl != 0u) == 0ull) continue;
-      if (PASS == 0 && lane == 0u) {
-        const uint32_t k = atomicAdd(&e.ec[EC_FLOOD_N], 1u);
-        if (k < e.ncell) e.work[e.ncell + k] = ci;
-      }
-      uint32_t rub[8];
-      uint32_t nrec = 0u, own = 0u;
-      for (int x = 0; x < 8; x++) {
-        rub[x] = 0u;
-        if (!((fl >> x) & 1u)) continue;
-        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-        const int32_t px = bx * 8 + x;
-        rub[x] = rubble_chain_row(e, s_im, px, vy, vz, lane & 7u, up[x >> 1]);
-        if (shard_spawns(e, vkey(e, px, vy, vz), v)) own |= 1u << x;
-        nrec += rub[x] + ((own >> x) & 1u);
-        if (PASS == 0 && e.mode == PX_ISLANDS_SEEDED) push_seeds(e, px, vy, vz, v);
-      }
-      if (PASS == 0) {
-        const uint32_t t = wave_sum(nrec);
-        if (lane == 0u) e.cpair[ci].y = t;
-        continue;
-      }
-      uint32_t wmask[4] = {0u, 0u, 0u, 0u};
-      uint32_t destroyed = 0u, bill = 0u;
-      for (int x = 0; x < 8; x++)
-        if ((fl >> x) & 1u) {
-          const uint32_t sh = (uint32_t)(x & 1) * 16u;
-          wmask[x >> 1] |= 0xffffu << sh;
-          destroyed++;
-          bill += price_dimes(e.mats, (ww[x >> 1] >> sh) & 0xffffu);
-        }
-      for (int q = 0; q < 4; q++)
-        if (wmask[q]) atomicAnd(row + q, ~wmask[q]);
-      uint32_t j = base + e.coff[ci].y + wave_scan(nrec) - nrec;
-      for (int x = 0; x < 8; x++) {
-        if (!((fl >> x) & 1u)) continue;
-        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-        const int32_t px = bx * 8 + x;
-        drop_rubble(e, px, vy, vz, rub[x], j, accepted, cursor);
-        j += rub[x];
-        if ((own >> x) & 1u) {
-          shard_record(e, px, vy, vz, vkey(e, px, vy, vz), v, j, accepted, cursor);
-          j++;
-        }
-      }
-      destroyed = wave_sum(destroyed);
-      bill = wave_sum(bill);
-      acc_d += destroyed;
-      acc_b += bill;
-      if (lane == 0u && destroyed) set_dirty(e, ci);
     }
   }
   if (PASS == 1 && lane == 0u) {
@@ -2744,7 +2961,7 @@ int px_launch_island_pass(px_ctx* c, bool force) {
   hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
   // labelling
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_face_merge<0>, dim3(kMergeBlocks), W, 0, s, e);
   hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
   if (c->lim.island_mode == PX_ISLANDS_SEEDED)
     hipLaunchKernelGGL(k_seed_mark, dim3(2048), dim3(256), 0, s, e);
