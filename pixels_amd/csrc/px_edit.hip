@@ -57,6 +57,10 @@ constexpr uint32_t kFlagHit = 2u;
 // k_face_merge<0>, which walks no list, keeps 8192 (208 -> 203 us).
 constexpr uint32_t kEditBlocks = 16384;
 constexpr uint32_t kMergeBlocks = 8192;
+// Carve's count pass keeps each row's removals (the impact loop, most of a
+// carved cell's time) for the first kCarveRowCache job entries, 512 B each
+// (16 MiB); the apply pass reads them instead of testing the impacts again.
+constexpr uint32_t kCarveRowCache = 32768;
 constexpr uint32_t kPartWords = 8;      // per-block partials
 constexpr uint32_t kCountWords = 4;     // per-block list lengths
 
@@ -117,6 +121,7 @@ struct EditView {
   float* impacts;     // the impact queue (pushImpact's records), C_IMPACTS of them
   uint32_t* shots;    // shoot's pane flood entries and dirtied cells (kShotWords)
   uint32_t* work;     // cells the count passes found work in: carve at [0, ncell), flood at [ncell, 2 ncell)
+  uint2* carve_rows;  // per job entry i < kCarveRowCache, per lane: removed_row()'s rem | rk << 8
   float* flashes;     // flash ring, PX_MAX_FLASHES records of 8 floats
   uint32_t max_imp;   // min(max_impacts, 64)
   uint32_t* seeds;    // seeded mode: one bit per voxel key, pushSeed (sim.wgsl:180-184)
@@ -1393,6 +1398,64 @@ PX_DEV bool carve_removes(const EditView& e, const float* __restrict__ s_im, int
   return false;
 }
 
+// carve_removes() for the 8 voxels x = 0..7 of one row of a pool brick
+// (row words ww, at y = vy, z = vz in the column of brick x index bx),
+// against the impacts of the row's cell mask: bit x set if some impact
+// removes voxel x; rk gets the first such impact (index order, the one whose
+// effects the voxel takes) in bits 6x .. 6x + 5.  Only alive fragile or soft
+// voxels can go (`can`), so the loop ends once no lane has such a voxel
+// left.  A row whose nearest voxel lies beyond an impact's reach bound is
+// skipped for that impact (every exact test fails there), and the 8 tests
+// run without branches, in the operation order of removes_voxel().
+PX_DEV uint32_t removed_row(const EditView& e, const float* __restrict__ s_im, uint64_t mask, int32_t bx, int32_t vy,
+                            int32_t vz, const uint32_t ww[4], uint64_t& rk) {
+  uint32_t alive = 0u, frag = 0u, soft = 0u;
+  float jit[8], m14[8], fx[8];
+  for (int x = 0; x < 8; x++) {
+    const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
+    fx[x] = (float)(bx * 8 + x) + 0.5f;
+    jit[x] = 0.0f;
+    m14[x] = 0.0f;
+    if (!v) continue;
+    alive |= 1u << x;
+    const float* E = e.mats + (v & 0xffu) * 20u + 16u;
+    if (E[1] > 0.5f) frag |= 1u << x;
+    if (E[0] > 0.0f) soft |= 1u << x;
+    m14[x] = fminf(1.4f, E[0]);
+    jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
+  }
+  const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
+  const uint32_t can = alive & (frag | soft);
+  uint32_t rem = 0u;
+  rk = 0ull;
+  for (uint64_t m = mask; m != 0ull && __ballot((can & ~rem) != 0u) != 0ull; m &= m - 1ull) {
+    const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+    const float* im = s_im + k * 16u;
+    const float ix = im[0], r = im[3], fr = im[4];
+    // removes() needs d2 < (fr jit)^2 or d2 < (rr jit)^2 with jit < 1.2 and
+    // rr <= max(1.4 r, 2.5): nothing beyond b is removed
+    const float b = fmaxf(fmaxf(r * 1.4f, 2.5f), fr) * 1.2f * 1.00001f;
+    const float dy = fy - im[1], dz = fz - im[2];
+    const float dy2 = dy * dy, dz2 = dz * dz;
+    const float dxn = fmaxf(fmaxf(fx[0] - ix, ix - fx[7]), 0.0f);
+    const uint32_t open = can & ~rem;
+    if (__ballot(open != 0u && dxn * dxn + dy2 + dz2 < b * b) == 0ull) continue;
+    const float fr2 = fr * fr, rlo = fminf(r, 2.5f);
+#pragma unroll
+    for (int x = 0; x < 8; x++) {
+      const float dx = fx[x] - ix;
+      const float d2 = dx * dx + dy2 + dz2;
+      const float j = jit[x];
+      const float rr = fmaxf(r * m14[x], rlo) * j;
+      const uint32_t hit = ((frag >> x) & (uint32_t)(d2 < fr2 * j * j)) | ((soft >> x) & (uint32_t)(d2 < rr * rr));
+      const bool take = (hit & (open >> x) & 1u) != 0u;
+      rem |= take ? 1u << x : 0u;
+      rk |= take ? (uint64_t)k << (6 * x) : 0ull;
+    }
+  }
+  return rem;
+}
+
 // dropRubbleAbove (sim.wgsl:274-284) after carve: the consecutive rubble
 // voxels of pool bricks above (x, y, z) that carve leaves, at most 12.  A
 // voxel carve removes counts as air whether or not its clear has landed yet,
@@ -1580,37 +1643,40 @@ PX_DEV void load_impacts(const EditView& e, float* s_im, uint32_t nimp) {
 template <int PASS>
 __global__ __launch_bounds__(64) void k_carve(EditView e) {
   __shared__ float s_im[64 * 16];
-  __shared__ float s_bound2[64];
   const uint32_t lane = threadIdx.x;
   const uint32_t nimp = e.ec[EC_NIMP];
   if (nimp == 0u) return;
-  load_impacts(e, s_im, nimp);
-  if (lane < nimp) {
-    // removes() needs d2 < (fr jit)^2 or d2 < (rr jit)^2 with jit <= 1.2 and
-    // rr <= max(1.4 r, 2.5): beyond this bound no voxel is tested exactly
-    const float* im = s_im + lane * 16u;
-    float b = fmaxf(fmaxf(im[3] * 1.4f, 2.5f), im[4]) * 1.2f * 1.00001f;
-    s_bound2[lane] = b * b;
+  const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
+  if (blockIdx.x >= n) {  // no job entry for this block: only the partials
+    if (PASS == 1 && lane == 0u) {
+      e.part[blockIdx.x * kPartWords + PT_CARVE_D] = 0u;
+      e.part[blockIdx.x * kPartWords + PT_CARVE_B] = 0u;
+    }
+    return;
   }
-  __syncthreads();
+  load_impacts(e, s_im, nimp);
   const uint32_t total = PASS == 1 ? e.etot[0] + e.etot[1] : 0u;
   const uint32_t accepted = PASS == 1 ? spawn_accepted(e, total) : 0u;
   const uint32_t cursor = PASS == 1 ? e.ctrl[PX_C_PCURSOR] : 0u;
   uint32_t acc_d = 0u, acc_b = 0u;
-  // the count pass visits every job cell, the apply pass the cells it found
-  // removals in
-  const uint32_t n = PASS == 0 ? min(e.ec[EC_JOBCELLS], e.ncell) : min(e.ec[EC_CARVE_N], e.ncell);
+  // The count pass visits every job cell and writes work[i] = the cell of
+  // job entry i if it found removals there, PX_NONE otherwise (an indexed
+  // list: an appended one costs a same-address atomic per cell); the apply
+  // pass visits the cells so listed.
   // Lane l reads list entry, impact mask and grid word of the block's l-th
-  // of the next 64 cells at once.  The count pass also loads the brick of
+  // of the next 64 entries at once.  The count pass also loads the brick of
   // the next pool cell while the current one is carved (the apply pass reads
   // each brick in turn, after its own earlier stores).
   for (uint32_t i0 = blockIdx.x; i0 < n; i0 += 64u * gridDim.x) {
     const uint32_t mi = i0 + lane * gridDim.x;
-    const uint32_t mc = mi < n ? (PASS == 0 ? e.job_list[mi] : e.work[mi]) : 0u;
-    const uint32_t mg = mi < n ? e.grid[mc] : 0u;
-    const uint64_t mm = mi < n ? e.imask[mc] : 0ull;
-    const uint32_t mu = grid_above(e, mc, mi < n);
-    uint64_t live = __ballot(mg != 0u && (mg & PX_UNIFORM_BIT) == 0u);
+    const uint32_t mc = mi < n ? (PASS == 0 ? e.job_list[mi] : e.work[mi]) : PX_NONE;
+    const bool in = mc != PX_NONE;
+    const uint32_t mg = in ? e.grid[mc] : 0u;
+    const uint64_t mm = in ? e.imask[mc] : 0ull;
+    const uint32_t mu = grid_above(e, mc, in);
+    const bool pool = mg != 0u && (mg & PX_UNIFORM_BIT) == 0u;
+    if (PASS == 0 && mi < n && !pool) e.work[mi] = PX_NONE;
+    uint64_t live = __ballot(pool);
     uint4 wn = make_uint4(0u, 0u, 0u, 0u), an = wn;
     if (PASS == 0 && live) {
       wn = brick_row(e, lane_value(mg, first_lane(live)), lane);
@@ -1645,56 +1711,21 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
       bool up_pool;
       row_above(w4, a4, g_up, lane, up, up_pool);
       const int32_t vy = by * 8 + (int32_t)(lane & 7u), vz = bz * 8 + (int32_t)(lane >> 3);
-      const float fy = (float)vy + 0.5f, fz = (float)vz + 0.5f;
-      // per-voxel terms of removes() that do not depend on the impact
-      uint32_t alive = 0u, frag = 0u, soft = 0u;
-      float jit[8], m14[8];
-      for (int x = 0; x < 8; x++) {
-        const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-        jit[x] = 0.0f;
-        m14[x] = 0.0f;
-        if (!v) continue;
-        alive |= 1u << x;
-        const float* E = e.mats + (v & 0xffu) * 20u + 16u;
-        if (E[1] > 0.5f) frag |= 1u << x;
-        if (E[0] > 0.0f) soft |= 1u << x;
-        m14[x] = fminf(1.4f, E[0]);
-        jit[x] = 0.8f + 0.4f * px_rnd(px_hash3i(bx * 8 + x, vy, vz));
+      uint64_t rk;  // remover of removed voxel x in bits 6x .. 6x + 5
+      uint32_t rem;
+      const uint32_t ie = i0 + l * gridDim.x;  // job entry
+      if (PASS == 1 && ie < kCarveRowCache) {
+        const uint2 c = e.carve_rows[(size_t)ie * 64u + lane];
+        rem = c.x & 0xffu;
+        rk = ((uint64_t)c.y << 24) | (c.x >> 8);
+      } else {
+        rem = removed_row(e, s_im, mask, bx, vy, vz, ww, rk);
+        if (PASS == 0 && ie < kCarveRowCache)
+          e.carve_rows[(size_t)ie * 64u + lane] = make_uint2(rem | ((uint32_t)rk << 8), (uint32_t)(rk >> 24));
       }
-      uint32_t rem = 0u;
-      uint64_t rk = 0ull;  // remover of removed voxel x in bits 6x .. 6x + 5
-      uint64_t m = mask;
-      while (m) {
-        const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-        m &= m - 1ull;
-        const float* im = s_im + k * 16u;
-        const float b2 = s_bound2[k];
-        const float r = im[3], fr = im[4];
-        const float dy = fy - im[1], dz = fz - im[2];
-        for (int x = 0; x < 8; x++) {
-          const uint32_t bit = 1u << x;
-          if (!(alive & bit) || (rem & bit)) continue;
-          const float dx = ((float)(bx * 8 + x) + 0.5f) - im[0];
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          if (!(d2 < b2)) continue;
-          const float j = jit[x];
-          bool hit = false;
-          if ((frag & bit) && d2 < fr * fr * j * j) hit = true;
-          else if (soft & bit) {
-            const float rr = fmaxf(r * m14[x], fminf(r, 2.5f)) * j;
-            hit = d2 < rr * rr;
-          }
-          if (hit) {
-            rem |= bit;
-            rk |= (uint64_t)k << (6 * x);
-          }
-        }
-      }
-      if (__ballot(rem != 0u) == 0ull) continue;
-      if (PASS == 0 && lane == 0u) {
-        const uint32_t k = atomicAdd(&e.ec[EC_CARVE_N], 1u);
-        if (k < e.ncell) e.work[k] = ci;
-      }
+      const bool any = __ballot(rem != 0u) != 0ull;
+      if (PASS == 0 && lane == 0u) e.work[ie] = any ? ci : PX_NONE;
+      if (!any) continue;
       // records of this lane's removed voxels
       uint32_t rub[8];
       uint32_t nrec = 0u, own = 0u;
@@ -1784,6 +1815,21 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
   const uint32_t lane = threadIdx.x;
   const uint32_t nimp = e.ec[EC_NIMP];
   if (nimp == 0u) return;
+  // a block with no list entry writes its partials and leaves before the
+  // impacts are read
+  bool work = false;
+  if (PASS == 0) {
+    for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) work |= e.bcount[b * kCountWords + BC_PANECELLS] != 0u;
+  } else {
+    work = blockIdx.x < min(e.ec[EC_FLOOD_N], e.ncell);
+  }
+  if (!work) {
+    if (PASS == 1 && lane == 0u) {
+      e.part[blockIdx.x * kPartWords + PT_FLOOD_D] = 0u;
+      e.part[blockIdx.x * kPartWords + PT_FLOOD_B] = 0u;
+    }
+    return;
+  }
   load_impacts(e, s_im, nimp);
   const uint32_t total = PASS == 1 ? e.etot[0] + e.etot[1] : 0u;
   const uint32_t accepted = PASS == 1 ? spawn_accepted(e, total) : 0u;
@@ -1805,6 +1851,7 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
       const uint32_t mc = mi < nc ? cells[mi] : 0u;
       const uint32_t mg = mi < nc ? e.grid[mc] : 0u;
       const uint32_t mu = grid_above(e, mc, mi < nc);
+      const uint64_t mm = PASS == 0 && mi < nc ? e.imask[mc] : 0ull;
       for (uint64_t live = __ballot(mg != 0u && (mg & PX_UNIFORM_BIT) == 0u); live; live &= live - 1ull) {
         const uint32_t l = first_lane(live);
         const uint32_t ci = lane_value(mc, l);
@@ -1850,11 +1897,13 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
         for (int x = 0; x < 8; x++) {
           if (!((valid >> x) & 1u)) continue;
           if ((look >> x) & 1u) last_hit = ((hit >> x) & 1u) != 0u;
-          if (!last_hit) continue;
-          const uint32_t v = (ww[x >> 1] >> ((x & 1) * 16)) & 0xffffu;
-          // carve's own voxels (still present before carve's stores) are not flooded
-          if (PASS == 0 && carve_removes(e, s_im, bx * 8 + x, vy, vz, v)) continue;
-          fl |= 1u << x;
+          if (last_hit) fl |= 1u << x;
+        }
+        // carve's own voxels (still present before carve's stores) are not
+        // flooded: the count pass takes carve's removals of the row out
+        if (PASS == 0 && __ballot(fl != 0u) != 0ull) {
+          uint64_t rk;
+          fl &= ~removed_row(e, s_im, lane_value64(mm, l), bx, vy, vz, ww, rk);
         }
         if (__ballot(fl != 0u) == 0ull) continue;
         if (PASS == 0 && lane == 0u) {
@@ -2777,6 +2826,7 @@ EditView make_view(px_ctx* c) {
   e.impacts = c->d_impacts;
   e.shots = c->d_shots;
   e.work = c->d_edit_work;
+  e.carve_rows = c->d_carve_rows;
   e.flashes = c->d_flashes;
   e.max_imp = std::min<uint32_t>(c->lim.max_impacts, 64u);
   e.seeds = c->d_seeds;
@@ -2837,6 +2887,7 @@ int px_edit_alloc(px_ctx* c) {
   PX_HIP_CHECK(c, hipMalloc(&c->d_edit_ctr, 64 * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_shots, kShotWords * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_edit_work, (size_t)c->ncell * 2 * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_carve_rows, (size_t)kCarveRowCache * 64 * sizeof(uint2)));
   PX_HIP_CHECK(c, hipMalloc(&c->d_flashes, PX_MAX_FLASHES * 8 * 4));
   hipLaunchKernelGGL(k_zero32, dim3(1), dim3(256), 0, c->stream, c->d_shots, (size_t)kShotWords);
   hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<uint32_t*>(c->d_flashes),
@@ -2854,7 +2905,7 @@ void px_edit_free(px_ctx* c) {
   void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags,
                   c->d_job_list, c->d_alloc_list, c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par,
                   c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds, c->d_shots,
-                  c->d_flashes, c->d_edit_work};
+                  c->d_flashes, c->d_edit_work, c->d_carve_rows};
   for (void* p : bufs)
     if (p) hipFree(p);
   c->d_impacts = c->d_impact_stage = nullptr;
@@ -2864,6 +2915,7 @@ void px_edit_free(px_ctx* c) {
   c->d_nodes = c->d_out_labels = c->d_seeds = c->d_shots = nullptr;
   c->d_flashes = nullptr;
   c->d_edit_work = nullptr;
+  c->d_carve_rows = nullptr;
 }
 
 // the labelling arrays are sized by the voxel count and allocated on first use

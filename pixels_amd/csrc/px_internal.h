@@ -95,6 +95,7 @@ struct px_ctx {
   uint32_t* d_seeds = nullptr;      // seeded search: one bit per voxel key
   uint32_t* d_shots = nullptr;      // shoot's pane flood entries and dirtied cells (kShotWords)
   uint32_t* d_edit_work = nullptr;  // cells the carve / flood count passes found work in (2 x ncell)
+  uint2* d_carve_rows = nullptr;     // carve's removals per row of its first job entries (count -> apply pass)
   float* d_flashes = nullptr;       // muzzle flashes: PX_MAX_FLASHES records of 8 floats
   uint64_t* d_cell_imask = nullptr;  // per cell: bit k = impact k has a job here
   uint32_t* d_alloc_flags = nullptr; // per cell bitset: alloc requested
