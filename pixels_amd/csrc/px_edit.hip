@@ -54,9 +54,7 @@ constexpr uint32_t kFlagHit = 2u;
 // out the blocks' uneven work: on the 64-impact island pass of the
 // 512 x 256 x 512 floor scene 8192 blocks took 0.95 ms and 16384 0.91 ms
 // (k_local_ccl<0> 180 -> 142 us, k_flood<0> 83 -> 56 us), 32768 0.95 ms.
-// k_face_merge<0>, which walks no list, keeps 8192 (208 -> 203 us).
 constexpr uint32_t kEditBlocks = 16384;
-constexpr uint32_t kMergeBlocks = 8192;
 // Carve's count pass keeps each row's removals (the impact loop, most of a
 // carved cell's time) for the first kCarveRowCache job entries, 512 B each
 // (16 MiB); the apply pass reads them instead of testing the impacts again.
@@ -164,51 +162,76 @@ constexpr uint32_t kLdsWalkCap = 512u;
 
 // Union-find over voxel keys.  par[x] <= x always (links go to smaller keys),
 // so chains strictly decrease.  Plain loads are enough: a stale parent is
-// still an ancestor, and a stale root only makes the CAS in uf_union fail and
+// still an ancestor, and a stale root only makes the linking CAS fail and
 // return the current parent.  Every walk checks that order (a parent above
-// its child can only be memory no node wrote in this pass) and stops after
-// `cap` steps, which no chain of distinct nodes reaches (cap >= the node
-// capacity of the pass); either way it sets kErrWalk and returns the node it
-// stands on, a key below the one it started from, so every access stays
+// its child can only be memory no node wrote in this pass) and stops after a
+// step bound that no chain of distinct nodes reaches (the bound derives from
+// walk_cap >= the node capacity of the pass); either way it sets kErrWalk
+// and stops on a key below the one it started from, so every access stays
 // inside par[] and every walk ends.  The error goes to a register (`err`)
 // that the kernel ORs into ec[EC_OVERFLOW] once, at its end.
-PX_DEV uint32_t uf_find(uint32_t* par, uint32_t x, uint32_t cap, uint32_t& err) {
-  for (uint32_t it = 0; it < cap; it++) {
-    const uint32_t p = par[x];
-    if (p == x) return x;
-    if (p > x) break;
-    const uint32_t gp = par[p];
-    if (gp > p) break;
-    if (gp != p) par[x] = gp;  // path halving; gp < p < x
-    x = gp;
-  }
-  err = kErrWalk;
-  return x;
-}
 
-PX_DEV void uf_union(uint32_t* par, uint32_t a, uint32_t b, uint32_t cap, uint32_t& err) {
-  // each failed CAS leaves par[a] below a, so a strictly decreases
-  for (uint32_t it = 0; it < cap; it++) {
-    a = uf_find(par, a, cap, err);
-    b = uf_find(par, b, cap, err);
-    if (a == b) return;
-    if (a < b) {
-      uint32_t t = a;
-      a = b;
-      b = t;
-    }
-    uint32_t old = atomicCAS(par + a, a, b);
-    if (old == a) return;
-    if (old > a) break;
-    a = old;
+// Unions of up to three pairs per lane (pair d when bit d of act is set),
+// walked together: each round reads the parents of both nodes of every open
+// pair at once, so the walks share their load latencies, and links two
+// roots with one CAS (the larger root under the smaller).  Each round moves
+// a pair's a or b (or both) to a smaller key, so a pair ends within the
+// lengths of its two chains; the bound of 2 cap + 2 rounds (cap per walk)
+// is never reached by chains of distinct nodes.  Once a pair is joined under
+// root r, its two start nodes point at r (a store of an ancestor, like path
+// halving).
+PX_DEV void uf_union3(uint32_t* par, uint32_t a[3], uint32_t b[3], uint32_t act, uint32_t cap, uint32_t& err) {
+  uint32_t a0[3], b0[3];
+  for (int d = 0; d < 3; d++) {
+    a0[d] = a[d];
+    b0[d] = b[d];
   }
-  err = kErrWalk;
+  const uint64_t rounds = 2ull * cap + 2ull;
+  for (uint64_t it = 0; act != 0u && it < rounds; it++) {
+    uint32_t pa[3], pb[3];
+    for (int d = 0; d < 3; d++) {
+      pa[d] = ((act >> d) & 1u) ? par[a[d]] : a[d];
+      pb[d] = ((act >> d) & 1u) ? par[b[d]] : b[d];
+    }
+    for (int d = 0; d < 3; d++) {
+      if (!((act >> d) & 1u)) continue;
+      if (pa[d] > a[d] || pb[d] > b[d]) {
+        err = kErrWalk;
+        act &= ~(1u << d);
+        continue;
+      }
+      if (pa[d] != a[d] || pb[d] != b[d]) {  // not both roots yet: one step up
+        a[d] = pa[d];
+        b[d] = pb[d];
+        continue;
+      }
+      uint32_t r = a[d];
+      if (a[d] != b[d]) {
+        const uint32_t hi = max(a[d], b[d]), lo = min(a[d], b[d]);
+        const uint32_t old = atomicCAS(par + hi, hi, lo);
+        if (old != hi) {  // hi gained a parent meanwhile: go on from it
+          if (old > hi) {
+            err = kErrWalk;
+            act &= ~(1u << d);
+          }
+          a[d] = old;
+          b[d] = lo;
+          continue;
+        }
+        r = lo;
+      }
+      if (a0[d] > r) par[a0[d]] = r;
+      if (b0[d] > r) par[b0[d]] = r;
+      act &= ~(1u << d);
+    }
+  }
+  if (act != 0u) err = kErrWalk;
 }
 
 // LDS union-find over the 512 voxels of one brick (links to smaller indices,
 // path halving; parents only decrease, so a racing halving store still
-// writes an ancestor).  Same order check and step bound as uf_find.  The
-// walk's reads and stores are relaxed workgroup-scope atomics: each read
+// writes an ancestor).  Same order check as the global walks, step bound
+// 512.  The walk's reads and stores are relaxed workgroup-scope atomics: each read
 // sees other lanes' links, and they stay LDS instructions (a volatile
 // access through the generic pointer would be a flat access that also
 // waits for every global load in flight).
@@ -770,7 +793,9 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, uint32_t g, const uint32_
     na[d] = ca ? node_from(e, okey, g, ra[d]) : PX_NONE;
     nb[d] = cb ? node_from(e, okn, gn[d], rb[d]) : PX_NONE;
   }
+  uint32_t ua[3], ub[3], act = 0u;  // the unions of the three faces, run together
   for (int d = 0; d < 3; d++) {
+    ua[d] = ub[d] = 0u;
     if (!live[d]) continue;
     const uint32_t ca = va[d] ? s_cls[va[d] & 0xffu] : 0u, cb = vb[d] ? s_cls[vb[d] & 0xffu] : 0u;
     const uint32_t A = na[d], B = nb[d];
@@ -782,7 +807,11 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, uint32_t g, const uint32_
     uint32_t pa = __shfl_up(A, 1), pb = __shfl_up(B, 1), pj = __shfl_up(jb, 1);
     uint32_t qa = __shfl_up(A, 8), qb = __shfl_up(B, 8), qj = __shfl_up(jb, 8);
     bool dup = (u > 0u && pj && pa == A && pb == B) || (v > 0u && qj && qa == A && qb == B);
-    if (join && !dup) uf_union(e.par, A, B, e.walk_cap, walk_err);
+    if (join && !dup) {
+      ua[d] = A;
+      ub[d] = B;
+      act |= 1u << d;
+    }
     if (MODE == 0) {
       uint32_t anc_a = (cA && vb[d] && s_str[vb[d] & 0xffu]) ? 1u : 0u;
       uint32_t anc_b = (cB && va[d] && s_str[va[d] & 0xffu]) ? 1u : 0u;
@@ -792,6 +821,7 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, uint32_t g, const uint32_
       if (anc_b && !((u > 0u && pnb && pb == B) || (v > 0u && qnb && qb == B))) atomicOr(&e.flag[B], kFlagAnchor);
     }
   }
+  uf_union3(e.par, ua, ub, act, e.walk_cap, walk_err);
 }
 
 template <int MODE>
@@ -810,7 +840,7 @@ __global__ __launch_bounds__(64) void k_face_merge(EditView e) {
     for (uint32_t k0 = 0; k0 * gridDim.x < e.ncell; k0 += 64u) {
       // plain round robin here: rotating the steps (step_cell) balances the
       // blocks but made this kernel slower (244 against 213 us on the
-      // 512 x 256 x 512 floor scene)
+      // 512 x 256 x 512 floor scene, with one union walk per face)
       const uint32_t mc = blockIdx.x + (k0 + lane) * gridDim.x;
       uint32_t mg[4];
       cell_and_neighbours(e, mc, mc < e.ncell, mg);
@@ -3013,7 +3043,7 @@ int px_launch_island_pass(px_ctx* c, bool force) {
   hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
   // labelling
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_face_merge<0>, dim3(kMergeBlocks), W, 0, s, e);
+  hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
   if (c->lim.island_mode == PX_ISLANDS_SEEDED)
     hipLaunchKernelGGL(k_seed_mark, dim3(2048), dim3(256), 0, s, e);
