@@ -557,8 +557,9 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
   // neighbouring rows (y+1: lane l+1, z+1: lane l+8) at the start of every
   // overlap, so a full brick needs ~100 LDS unions instead of ~1300.
   __shared__ uint32_t s_lab[512];
+  // per brick-local root: voxel count (bits 0..15) and anchored runs (bits
+  // 16..31), one LDS add per run; 4.5 KiB of LDS per block in all
   __shared__ uint32_t s_cnt[512];
-  __shared__ uint32_t s_flg[512];
   __shared__ uint8_t s_cls[256];
   __shared__ uint8_t s_str[256];
   const uint32_t lane = threadIdx.x;
@@ -635,7 +636,6 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         uint32_t start = 31u - (uint32_t)__clz(rs & ((2u << x) - 1u));
         s_lab[li] = ((m >> x) & 1u) ? lane * 8u + start : PX_NONE;
         s_cnt[li] = 0u;
-        s_flg[li] = 0u;
       }
       __syncthreads();
       for (int x = 0; x < 8; x++) {
@@ -667,8 +667,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         uint32_t run = 1u << x;
         for (int y = x + 1; y < 8 && ((lx >> y) & 1u); y++) run |= 1u << y;
         uint32_t r = lds_find(s_lab, lane * 8u + (uint32_t)x, walk_err);
-        atomicAdd(&s_cnt[r], (uint32_t)__popc(run));
-        if (run & abits) atomicOr(&s_flg[r], kFlagAnchor);
+        atomicAdd(&s_cnt[r], (uint32_t)__popc(run) + ((run & abits) != 0u ? 0x10000u : 0u));
         for (int y = x; y < 8; y++)
           if ((run >> y) & 1u) rootx[y] = r;
         if (r == lane * 8u + (uint32_t)x) nroots++;
@@ -684,8 +683,8 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         if (rootx[x] != li) continue;
         uint32_t node = okey + loff(e, li);
         e.par[node] = node;
-        e.cnt[node] = s_cnt[li];
-        e.flag[node] = s_flg[li];
+        e.cnt[node] = s_cnt[li] & 0xffffu;
+        e.flag[node] = (s_cnt[li] >> 16) != 0u ? kFlagAnchor : 0u;
         if (slot < e.slots_node) my_nodes[slot] = node;
         else atomicOr(&e.ec[EC_OVERFLOW], kErrNodes);
         slot++;
