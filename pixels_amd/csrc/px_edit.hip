@@ -79,7 +79,7 @@ constexpr uint32_t kFlagKeep = 4u;
 constexpr uint32_t kFlagSeed = 16u;
 // impact rows of k_impact_jobs: blocks of 256 cells per impact, grid-stride
 // over the impact's box (MAX_IMPACTS = 64 rows)
-constexpr uint32_t kJobBlocks = 32;
+constexpr uint32_t kJobBlocks = 128;
 constexpr uint32_t kFlagCrumb = 8u;
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 constexpr uint32_t kMaxChunkIds = 511;  // chunk id + 1 fits bits 8..16
@@ -99,7 +99,6 @@ struct EditView {
   int32_t dims[3], bd[3], sd[3];
   uint32_t ncell;
   uint64_t* imask;
-  uint32_t* alloc_flags;
   uint32_t* dirty_flags;
   uint32_t* job_list;
   uint32_t* alloc_list;
@@ -444,64 +443,101 @@ __global__ void k_push_impacts_commit(EditView e, uint32_t n) {
   if (threadIdx.x == 0) e.ctrl[PX_C_IMPACTS] += n;
 }
 
-// impactJobs, sim.wgsl:414-443: one block row per impact; list appends are
-// aggregated per wavefront
+// impactJobs, sim.wgsl:414-443, with one owner per cell and the impacts
+// across the lanes.  Block row k walks impact k's box of bricks; a cell
+// that an earlier impact's box also holds is left to that impact's row, so
+// every cell is decided once.  Lane j of every wave holds impact j (its box
+// of bricks, centre, radii and R = ceil(max(r, fr) 1.2) + 1), so one cell is
+// tested against all impacts at once: box, distance from the impact centre
+// to the brick against R, and for a uniform brick reach(); the ballot is the
+// cell's impact mask.  A wave takes 64 cells at a time: lane l reads cell
+// l's grid word and material, the 64 cells are decided in turn, and lane l
+// then writes cell l's mask (a plain store: one writer per cell) and its
+// list entries, one atomic add per list per wave.
 __global__ __launch_bounds__(256) void k_impact_jobs(EditView e) {
+  const uint32_t nimp = e.ec[EC_NIMP];
   const uint32_t k = blockIdx.y;
-  if (k >= e.ec[EC_NIMP]) return;
+  if (k >= nimp) return;
   const uint32_t lane = __lane_id();
-  const float* im = e.impacts + (size_t)k * 16u;
-  float R = ceilf(fmaxf(im[3], im[4]) * 1.2f) + 1.0f;
-  int32_t bmin[3], bmax[3];
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+  // impact `lane`
+  const bool has = lane < nimp;
+  const float* im = e.impacts + (size_t)(has ? lane : 0u) * 16u;
+  const float ic[3] = {im[0], im[1], im[2]}, ir = im[3], ifr = im[4];
+  const float R = ceilf(fmaxf(ir, ifr) * 1.2f) + 1.0f;
+  int32_t blo[3], bhi[3];
   for (int a = 0; a < 3; a++) {
-    bmin[a] = px_clampi(px_f2i(floorf((im[a] - R) / 8.0f)), 0, e.bd[a] - 1);
-    bmax[a] = px_clampi(px_f2i(floorf((im[a] + R) / 8.0f)), 0, e.bd[a] - 1);
+    blo[a] = px_clampi(px_f2i(floorf((ic[a] - R) / 8.0f)), 0, e.bd[a] - 1);
+    bhi[a] = px_clampi(px_f2i(floorf((ic[a] + R) / 8.0f)), 0, e.bd[a] - 1);
   }
-  uint32_t ex = (uint32_t)(bmax[0] - bmin[0] + 1), ey = (uint32_t)(bmax[1] - bmin[1] + 1);
-  uint32_t ez = (uint32_t)(bmax[2] - bmin[2] + 1);
-  uint32_t total = ex * ey * ez;
+  // impact k's box, from its lane
+  int32_t kb[6];
+  for (int a = 0; a < 3; a++) {
+    kb[a] = __shfl(blo[a], (int)k);
+    kb[3 + a] = __shfl(bhi[a], (int)k);
+  }
+  const uint32_t ex = (uint32_t)(kb[3] - kb[0] + 1), ey = (uint32_t)(kb[4] - kb[1] + 1);
+  const uint32_t ez = (uint32_t)(kb[5] - kb[2] + 1);
+  const uint32_t total = ex * ey * ez;
   uint32_t jobs = 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
-    uint32_t j = base + threadIdx.x;
-    bool job = false, alloc_new = false, cell_new = false;
-    uint32_t ci = 0;
-    if (j < total) {
-      uint32_t b0 = (uint32_t)bmin[0] + j % ex, b1 = (uint32_t)bmin[1] + (j / ex) % ey;
-      uint32_t b2 = (uint32_t)bmin[2] + j / (ex * ey);
-      ci = b0 + (uint32_t)e.bd[0] * (b1 + (uint32_t)e.bd[1] * b2);
-      uint32_t g = e.grid[ci];
-      if (g != 0u) {
-        float lo[3] = {(float)(b0 * 8u), (float)(b1 * 8u), (float)(b2 * 8u)};
+  for (uint32_t c0 = wave * 64u; c0 < total; c0 += nwaves * 64u) {
+    // cell c0 + lane of the box: coordinates, grid word, material terms
+    const uint32_t t = c0 + lane;
+    const int32_t cx = kb[0] + (int32_t)(t % ex), cy = kb[1] + (int32_t)((t / ex) % ey);
+    const int32_t cz = kb[2] + (int32_t)(t / (ex * ey));
+    const uint32_t ci = (uint32_t)cx + (uint32_t)e.bd[0] * ((uint32_t)cy + (uint32_t)e.bd[1] * (uint32_t)cz);
+    const uint32_t g = t < total ? e.grid[ci] : 0u;
+    const bool uni = (g & PX_UNIFORM_BIT) != 0u;
+    const float* E = e.mats + (g & 0xffu) * 20u + 16u;
+    const uint32_t efrag = uni && E[1] > 0.5f ? 1u : 0u, esoft = uni && E[0] > 0.0f ? 1u : 0u;
+    const float em14 = uni ? fminf(1.4f, E[0]) : 0.0f;
+    uint64_t my_mask = 0ull;
+    bool my_alloc = false;
+    for (uint64_t todo = __ballot(g != 0u); todo; todo &= todo - 1ull) {
+      const uint32_t c = first_lane(todo);
+      const int32_t bx = (int32_t)lane_value((uint32_t)cx, c), by = (int32_t)lane_value((uint32_t)cy, c);
+      const int32_t bz = (int32_t)lane_value((uint32_t)cz, c);
+      const bool inbox = has && bx >= blo[0] && bx <= bhi[0] && by >= blo[1] && by <= bhi[1] && bz >= blo[2] &&
+                         bz <= bhi[2];
+      if (__ballot(inbox && lane < k) != 0ull) continue;  // an earlier impact's row decides it
+      bool job = false;
+      if (inbox) {
+        const float lo[3] = {(float)(bx * 8), (float)(by * 8), (float)(bz * 8)};
         float ev[3];
-        for (int a = 0; a < 3; a++) ev[a] = fmaxf(fmaxf(lo[a] - im[a], 0.0f), im[a] - (lo[a] + 8.0f));
-        float near2 = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
+        for (int a = 0; a < 3; a++) ev[a] = fmaxf(fmaxf(lo[a] - ic[a], 0.0f), ic[a] - (lo[a] + 8.0f));
+        const float near2 = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2];
         job = !(near2 > R * R);
-        if (job && (g & PX_UNIFORM_BIT) != 0u) {
-          float rch = reach(im, e.mats, g & 0xffu);
-          if (near2 > rch * rch) {
-            job = false;
-          } else {
-            uint32_t bit = 1u << (ci & 31u);
-            alloc_new = (atomicOr(&e.alloc_flags[ci >> 5], bit) & bit) == 0u;
-          }
+        if (job && lane_value(uni ? 1u : 0u, c) != 0u) {
+          // reach() of sim.wgsl:242-248 with cell c's material terms
+          float rch = lane_value(efrag, c) != 0u ? ifr * 1.2f : 0.0f;
+          if (lane_value(esoft, c) != 0u)
+            rch = fmaxf(rch, fmaxf(ir * __int_as_float((int)lane_value((uint32_t)__float_as_int(em14), c)),
+                                   fminf(ir, 2.5f)) * 1.2f);
+          job = !(near2 > rch * rch);
         }
-        if (job) cell_new = atomicOr((unsigned long long*)&e.imask[ci], 1ull << k) == 0ull;
+      }
+      const uint64_t m = __ballot(job);
+      if (lane == c) {
+        my_mask = m;
+        my_alloc = uni && m != 0ull;
       }
     }
-    jobs += job ? 1u : 0u;
-    uint64_t am = __ballot(alloc_new), cm = __ballot(cell_new);
-    uint64_t below = (1ull << lane) - 1ull;
+    if (my_mask) e.imask[ci] = my_mask;
+    jobs += (uint32_t)__popcll(my_mask);
+    const uint64_t am = __ballot(my_alloc), cm = __ballot(my_mask != 0ull);
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t abase = 0, cbase = 0;
     if (lane == 0 && am) abase = atomicAdd(&e.ec[EC_ALLOCS], (uint32_t)__popcll(am));
     if (lane == 0 && cm) cbase = atomicAdd(&e.ec[EC_JOBCELLS], (uint32_t)__popcll(cm));
     abase = __shfl(abase, 0);
     cbase = __shfl(cbase, 0);
-    if (alloc_new) {
-      uint32_t q = abase + (uint32_t)__popcll(am & below);
+    if (my_alloc) {
+      const uint32_t q = abase + (uint32_t)__popcll(am & below);
       if (q < e.ncell) e.alloc_list[q] = ci;
     }
-    if (cell_new) {
-      uint32_t q = cbase + (uint32_t)__popcll(cm & below);
+    if (my_mask) {
+      const uint32_t q = cbase + (uint32_t)__popcll(cm & below);
       if (q < e.ncell) e.job_list[q] = ci;
     }
   }
@@ -520,7 +556,6 @@ __global__ __launch_bounds__(64) void k_alloc(EditView e) {
   for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
     uint32_t ci = e.alloc_list[i];
     uint32_t g = e.grid[ci];
-    if (lane == 0) atomicAnd(&e.alloc_flags[ci >> 5], ~(1u << (ci & 31u)));
     if (!(g == 0u || (g & PX_UNIFORM_BIT) != 0u)) continue;
     if (i >= top || top > e.pool_cap) continue;
     uint32_t p = e.free_list[top - 1u - i];
@@ -2833,7 +2868,6 @@ EditView make_view(px_ctx* c) {
   }
   e.ncell = c->ncell;
   e.imask = c->d_cell_imask;
-  e.alloc_flags = c->d_alloc_flags;
   e.dirty_flags = c->d_dirty_flags;
   e.job_list = c->d_job_list;
   e.alloc_list = c->d_alloc_list;
@@ -2906,7 +2940,6 @@ int px_edit_alloc(px_ctx* c) {
   PX_HIP_CHECK(c, hipMalloc(&c->d_impacts, 64 * 16 * sizeof(float)));
   PX_HIP_CHECK(c, hipMalloc(&c->d_impact_stage, 64 * 16 * sizeof(float)));
   PX_HIP_CHECK(c, hipMalloc(&c->d_cell_imask, (size_t)c->ncell * 8));
-  PX_HIP_CHECK(c, hipMalloc(&c->d_alloc_flags, words * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_dirty_flags, words * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_job_list, (size_t)c->ncell * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_alloc_list, (size_t)c->ncell * 4));
@@ -2922,7 +2955,6 @@ int px_edit_alloc(px_ctx* c) {
   hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<uint32_t*>(c->d_flashes),
                      (size_t)PX_MAX_FLASHES * 8);
   hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, c->stream, c->d_cell_imask, (size_t)c->ncell);
-  hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, c->stream, c->d_alloc_flags, words);
   hipLaunchKernelGGL(k_zero32, dim3(256), dim3(256), 0, c->stream, c->d_dirty_flags, words);
   hipLaunchKernelGGL(k_zero32, dim3(1), dim3(64), 0, c->stream, c->d_edit_ctr, (size_t)64);
   hipLaunchKernelGGL(k_zero32, dim3(64), dim3(256), 0, c->stream, c->d_bcount, (size_t)kEditBlocks * kCountWords);
@@ -2931,7 +2963,7 @@ int px_edit_alloc(px_ctx* c) {
 }
 
 void px_edit_free(px_ctx* c) {
-  void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_alloc_flags, c->d_dirty_flags,
+  void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_dirty_flags,
                   c->d_job_list, c->d_alloc_list, c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par,
                   c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds, c->d_shots,
                   c->d_flashes, c->d_edit_work, c->d_carve_rows};

@@ -98,7 +98,6 @@ struct px_ctx {
   uint2* d_carve_rows = nullptr;     // carve's removals per row of its first job entries (count -> apply pass)
   float* d_flashes = nullptr;       // muzzle flashes: PX_MAX_FLASHES records of 8 floats
   uint64_t* d_cell_imask = nullptr;  // per cell: bit k = impact k has a job here
-  uint32_t* d_alloc_flags = nullptr; // per cell bitset: alloc requested
   uint32_t* d_dirty_flags = nullptr; // per cell bitset: dirty
   uint32_t* d_job_list = nullptr;    // cells with at least one job
   uint32_t* d_alloc_list = nullptr;
