@@ -667,6 +667,65 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         if (has_y && cl[x] && cl[x] == cy) ey |= 1u << x;
         if (has_z && cl[x] && cl[x] == cz) ez |= 1u << x;
       }
+      // anchors inside the brick (islands): y == 0 or a structural neighbour
+      uint32_t abits = 0u;
+      if (MODE == 0) {
+        uint32_t adj = ((sm << 1) | (sm >> 1)) & 0xffu;
+        if (has_y) adj |= sm_yp;
+        if (ry > 0u) adj |= sm_ym;
+        if (has_z) adj |= sm_zp;
+        if (rz > 0u) adj |= sm_zm;
+        if (by == 0 && ry == 0u) adj = 0xffu;
+        abits = m & adj;
+      }
+      // Every row holds at most one run (most bricks): the runs are then
+      // joined exactly when their rows' runs touch (ey / ez), and the
+      // components are the connected sets of rows, found with bit operations
+      // on the wavefront's row masks instead of LDS walks.  A component's
+      // root is its lowest row's run start, its minimum local index, and the
+      // roots come in increasing order, as from the walks.
+      if (__ballot(__popc(rs) > 1) == 0ull) {
+        const uint64_t rows = __ballot(m != 0u), yj = __ballot(ey != 0u), zj = __ballot(ez != 0u);
+        const uint32_t cnt_l = (uint32_t)__popc(m);
+        const bool anc_l = (m & abits) != 0u;
+        uint32_t myroot = kLabNone, total = 0u;
+        for (uint64_t rem = rows; rem != 0ull;) {
+          uint64_t comp = rem & (~rem + 1ull);
+          for (;;) {
+            const uint64_t nxt =
+                comp | ((comp & yj) << 1) | ((comp >> 1) & yj) | ((comp & zj) << 8) | ((comp >> 8) & zj);
+            if (nxt == comp) break;
+            comp = nxt;
+          }
+          rem &= ~comp;
+          const uint32_t rl = first_lane(comp);
+          const uint32_t rli = rl * 8u + (uint32_t)__ffs(lane_value(m, rl)) - 1u;
+          const bool in = ((comp >> lane) & 1ull) != 0ull;
+          const uint32_t cnt = wave_sum(in ? cnt_l : 0u);
+          const bool anch = __ballot(in && anc_l) != 0ull;
+          if (in) myroot = rli;
+          if (lane == 0u) {
+            const uint32_t node = okey + loff(e, rli);
+            e.par[node] = node;
+            e.cnt[node] = cnt;
+            e.flag[node] = anch ? kFlagAnchor : 0u;
+            if (nn + total < e.slots_node) my_nodes[nn + total] = node;
+            else atomicOr(&e.ec[EC_OVERFLOW], kErrNodes);
+          }
+          total++;
+        }
+        nn += total;
+        uint32_t rx[8];
+        for (int x = 0; x < 8; x++) rx[x] = ((m >> x) & 1u) ? myroot : kLabNone;
+        const uint4 lw = make_uint4(rx[0] | (rx[1] << 16), rx[2] | (rx[3] << 16), rx[4] | (rx[5] << 16),
+                                    rx[6] | (rx[7] << 16));
+        reinterpret_cast<uint4*>(e.lab + (size_t)p * 512u)[lane] = lw;
+        if (MODE == 1) {
+          if (lane == 0u) my_cells[nc] = ci;
+          nc++;
+        }
+        continue;  // no LDS touched: the next brick needs no barrier
+      }
       for (int x = 0; x < 8; x++) {
         uint32_t li = lane * 8u + (uint32_t)x;
         uint32_t start = 31u - (uint32_t)__clz(rs & ((2u << x) - 1u));
@@ -682,17 +741,6 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
           lds_union(s_lab, lane * 8u + (uint32_t)x, (lane + 8u) * 8u + (uint32_t)x, walk_err);
       }
       __syncthreads();
-      // anchors inside the brick (islands): y == 0 or a structural neighbour
-      uint32_t abits = 0u;
-      if (MODE == 0) {
-        uint32_t adj = ((sm << 1) | (sm >> 1)) & 0xffu;
-        if (has_y) adj |= sm_yp;
-        if (ry > 0u) adj |= sm_ym;
-        if (has_z) adj |= sm_zp;
-        if (rz > 0u) adj |= sm_zm;
-        if (by == 0 && ry == 0u) adj = 0xffu;
-        abits = m & adj;
-      }
       uint32_t rootx[8];
       uint32_t nroots = 0u;
       for (int x = 0; x < 8; x++) rootx[x] = kLabNone;
@@ -1630,7 +1678,8 @@ PX_DEV uint4 top_row_above(const EditView& e, uint32_t g_up, uint32_t lane) {
 // read from the brick above (grid word g_up).  `pool` says whether that row
 // lies in a pool brick.
 PX_DEV void row_above(uint4 own, uint4 above, uint32_t g_up, uint32_t lane, uint32_t up[4], bool& pool) {
-  const uint4 n = make_uint4(__shfl_down(own.x, 1), __shfl_down(own.y, 1), __shfl_down(own.z, 1), __shfl_down(own.w, 1));
+  const uint4 n =
+      make_uint4(__shfl_down(own.x, 1), __shfl_down(own.y, 1), __shfl_down(own.z, 1), __shfl_down(own.w, 1));
   const bool top = (lane & 7u) == 7u;
   pool = !top || (g_up != 0u && (g_up & PX_UNIFORM_BIT) == 0u);
   up[0] = top ? above.x : n.x;
@@ -1938,7 +1987,8 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
   // impacts are read
   bool work = false;
   if (PASS == 0) {
-    for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) work |= e.bcount[b * kCountWords + BC_PANECELLS] != 0u;
+    for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x)
+      work |= e.bcount[b * kCountWords + BC_PANECELLS] != 0u;
   } else {
     work = blockIdx.x < min(e.ec[EC_FLOOD_N], e.ncell);
   }
