@@ -60,6 +60,7 @@ constexpr uint32_t kEditBlocks = 16384;
 // (16 MiB); the apply pass reads them instead of testing the impacts again.
 constexpr uint32_t kCarveRowCache = 32768;
 constexpr uint32_t kPartWords = 8;      // per-block partials
+constexpr uint32_t kEditFinishBlocks = 64;
 constexpr uint32_t kCountWords = 4;     // per-block list lengths
 
 // edit-pass counters (ec[])
@@ -69,7 +70,9 @@ enum : uint32_t {
   // voxel-write stage (after the island pass's words)
   EC_W_N = 16, EC_W_DIRTY, EC_W_WRITTEN, EC_W_DROP,
   // extraction and chunk stages
-  EC_R_CELLS = 24, EC_R_DIRTY, EC_R_BILL, EC_R_DESTROYED, EC_R_SPAWN_ACC, EC_C_LAND, EC_C_CAND, EC_R_ARENA0
+  EC_R_CELLS = 24, EC_R_DIRTY, EC_R_BILL, EC_R_DESTROYED, EC_R_SPAWN_ACC, EC_C_LAND, EC_C_CAND, EC_R_ARENA0,
+  // k_edit_finish's block sums (destroyed, bill, dirty, nodes) and its count of finished blocks
+  EC_F_SUM = 32, EC_F_DONE = EC_F_SUM + 4
 };
 // component flags written by the extraction (beside kFlagAnchor / kFlagHit):
 // kept in the world (over the component budget), crumbles into debris, and
@@ -427,6 +430,7 @@ __global__ void k_edit_begin(EditView e, uint32_t force) {
   e.ec[EC_ACTIVE] = active ? 1u : 0u;
   e.ec[EC_NIMP] = active ? nimp : 0u;
   e.ec[EC_R_DIRTY] = 0u;
+  for (uint32_t k = EC_F_SUM; k <= EC_F_DONE; k++) e.ec[k] = 0u;
 }
 
 // pushImpact (sim.wgsl:203-211) for n staged records: record j goes to slot
@@ -1242,13 +1246,17 @@ __global__ __launch_bounds__(1024) void k_node_budget(EditView e) {
 }
 
 // counters of the pass into the ctrl words the reference exposes
-__global__ __launch_bounds__(1024) void k_edit_finish(EditView e) {
-  __shared__ uint32_t s[6][1024];
+__global__ __launch_bounds__(256) void k_edit_finish(EditView e) {
+  // One partial per thread over kEditFinishBlocks blocks; each block adds its
+  // sums to ec[EC_F_SUM ..], and the last block to finish writes the totals
+  // (one block reading all 16384 partials was latency-bound at 17 us).
+  __shared__ uint32_t s[4][4];
+  __shared__ uint32_t s_last;
   if (e.ec[EC_ACTIVE] == 0u) return;
   const bool carved = e.ec[EC_NIMP] != 0u;
-  uint32_t t = threadIdx.x;
-  uint32_t a[6] = {0, 0, 0, 0, 0, 0};
-  for (uint32_t b = t; b < kEditBlocks; b += blockDim.x) {
+  const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
+  uint32_t a[4] = {0, 0, 0, 0};
+  for (uint32_t b = blockIdx.x * blockDim.x + t; b < kEditBlocks; b += gridDim.x * blockDim.x) {
     if (carved) {
       a[0] += e.part[b * kPartWords + PT_CARVE_D] + e.part[b * kPartWords + PT_FLOOD_D];
       a[1] += e.part[b * kPartWords + PT_CARVE_B] + e.part[b * kPartWords + PT_FLOOD_B];
@@ -1256,18 +1264,30 @@ __global__ __launch_bounds__(1024) void k_edit_finish(EditView e) {
     }
     a[3] += e.bcount[b * kCountWords + BC_NODES];
   }
-  for (int k = 0; k < 4; k++) s[k][t] = a[k];
-  __syncthreads();
-  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
-    if (t < o)
-      for (int k = 0; k < 4; k++) s[k][t] += s[k][t + o];
-    __syncthreads();
+  for (int k = 0; k < 4; k++) {
+    a[k] = wave_sum(a[k]);
+    if (lane == 0u) s[k][wave] = a[k];
   }
-  if (t != 0) return;
-  e.ctrl[PX_S_DESTROYED] += s[0][0];
-  e.ctrl[PX_S_BILL] += s[1][0];
-  e.ctrl[PX_C_DIRTY] = s[2][0];
-  e.ctrl[PX_C_NODES] = s[3][0];
+  __syncthreads();
+  if (t == 0u) {
+    for (int k = 0; k < 4; k++) {
+      uint32_t sum = 0u;
+      for (uint32_t w = 0; w < blockDim.x / 64u; w++) sum += s[k][w];
+      if (sum) atomicAdd(&e.ec[EC_F_SUM + k], sum);
+    }
+    __threadfence();
+    s_last = atomicAdd(&e.ec[EC_F_DONE], 1u) == gridDim.x - 1u ? 1u : 0u;
+  }
+  __syncthreads();
+  if (t != 0u || !s_last) return;
+  __threadfence();
+  uint32_t tot[4];
+  for (int k = 0; k < 4; k++)
+    tot[k] = __hip_atomic_load(&e.ec[EC_F_SUM + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  e.ctrl[PX_S_DESTROYED] += tot[0];
+  e.ctrl[PX_S_BILL] += tot[1];
+  e.ctrl[PX_C_DIRTY] = tot[2];
+  e.ctrl[PX_C_NODES] = tot[3];
   e.ctrl[PX_C_JOBS] = e.ec[EC_JOBS];
   e.ctrl[PX_C_ALLOC_N] = e.ec[EC_ALLOCS];
   e.ctrl[PX_C_ALLOCS] = 0u;
@@ -3190,7 +3210,7 @@ int px_launch_island_pass(px_ctx* c, bool force) {
     hipLaunchKernelGGL(k_node_budget, dim3(1), dim3(1024), 0, s, e);
     hipLaunchKernelGGL(k_emit<1>, G, W, 0, s, e);
   }
-  hipLaunchKernelGGL(k_edit_finish, dim3(1), dim3(1024), 0, s, e);
+  hipLaunchKernelGGL(k_edit_finish, dim3(kEditFinishBlocks), dim3(256), 0, s, e);
   PX_HIP_CHECK(c, hipGetLastError());
   c->labels_valid = true;
   c->masks_dirty = c->density_dirty = true;

@@ -338,49 +338,40 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-// one bit per super cell (count != 0) and per brick cell (entry != 0)
+// one bit per super cell (count != 0) and per brick cell (entry != 0):
+// lane l of wave w reads cell 64 w + l (coalesced), and the wave's ballot is
+// two mask words; the super cells' words follow the cells' waves
 __global__ void k_masks(const uint32_t* __restrict__ grid, uint32_t ncell, const uint32_t* __restrict__ sgrid,
                         uint32_t nsuper, uint32_t* __restrict__ bmask, uint32_t* __restrict__ smask) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t lane = threadIdx.x & 63u;
-  uint32_t cells = (ncell + 31u) / 32u;
-  if (i < cells) {
-    uint32_t m = 0;
-    for (uint32_t b = 0; b < 32u; b++) {
-      uint32_t c = i * 32u + b;
-      if (c < ncell && grid[c] != 0u) m |= 1u << b;
-    }
-    bmask[i] = m;
-  }
-  uint32_t sw = (nsuper + 31u) / 32u;
-  if (i < sw) {
-    uint32_t m = 0;
-    for (uint32_t b = 0; b < 32u; b++) {
-      uint32_t c = i * 32u + b;
-      if (c < nsuper && sgrid[c] != 0u) m |= 1u << b;
-    }
-    smask[i] = m;
-  }
-  (void)lane;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t cw = (ncell + 63u) / 64u;  // waves over the cells
+  const bool cells = wave < cw;
+  const uint32_t base = (cells ? wave : wave - cw) * 64u;
+  const uint32_t n = cells ? ncell : nsuper;
+  if (base >= n) return;
+  const uint32_t i = base + lane;
+  const uint64_t m = __ballot(i < n && (cells ? grid[i] : sgrid[i]) != 0u);
+  uint32_t* out = cells ? bmask : smask;
+  const uint32_t words = (n + 31u) / 32u;
+  if (lane < 2u && base / 32u + lane < words) out[base / 32u + lane] = (uint32_t)(m >> (32u * lane));
 }
 
-// Per-super-cell 64-bit brick occupancy (bit = lx + 4 (ly + 4 lz)).
+// Per-super-cell 64-bit brick occupancy (bit = lx + 4 (ly + 4 lz)): one
+// wave per super cell, lane = the brick's bit.
 __global__ void k_super_bm(const uint32_t* __restrict__ grid, int32_t bx, int32_t by, int32_t bz, int32_t sx,
                            int32_t sy, int32_t sz, uint64_t* __restrict__ out) {
-  uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t ns = (uint32_t)(sx * sy * sz);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t si = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t ns = (uint32_t)(sx * sy * sz);
   if (si >= ns) return;
-  int32_t s0 = (int32_t)(si % (uint32_t)sx), s1 = (int32_t)((si / (uint32_t)sx) % (uint32_t)sy);
-  int32_t s2 = (int32_t)(si / ((uint32_t)sx * (uint32_t)sy));
-  uint64_t m = 0ull;
-  for (int32_t lz = 0; lz < 4; lz++)
-    for (int32_t ly = 0; ly < 4; ly++)
-      for (int32_t lx = 0; lx < 4; lx++) {
-        int32_t x = s0 * 4 + lx, y = s1 * 4 + ly, z = s2 * 4 + lz;
-        if (x >= bx || y >= by || z >= bz) continue;
-        if (grid[(uint32_t)(x + bx * (y + by * z))] != 0u) m |= 1ull << (lx + 4 * (ly + 4 * lz));
-      }
-  out[si] = m;
+  const int32_t s0 = (int32_t)(si % (uint32_t)sx), s1 = (int32_t)((si / (uint32_t)sx) % (uint32_t)sy);
+  const int32_t s2 = (int32_t)(si / ((uint32_t)sx * (uint32_t)sy));
+  const int32_t x = s0 * 4 + (int32_t)(lane & 3u), y = s1 * 4 + (int32_t)((lane >> 2) & 3u);
+  const int32_t z = s2 * 4 + (int32_t)(lane >> 4);
+  const bool in = x < bx && y < by && z < bz;
+  const uint64_t m = __ballot(in && grid[(uint32_t)(x + bx * (y + by * z))] != 0u);
+  if (lane == 0u) out[si] = m;
 }
 
 // 2x2x2-block occupancy of a pool brick: bit x2 + 4 y2 + 16 z2 is set when
@@ -463,13 +454,12 @@ PxWorldView px_view(const px_ctx* c) {
 }
 
 int px_launch_masks(px_ctx* c) {
-  uint32_t words = (c->ncell + 31u) / 32u;
-  uint32_t blocks = (words + 255u) / 256u;
-  hipLaunchKernelGGL(k_masks, dim3(blocks), dim3(256), 0, c->stream, c->d_grid, c->ncell, c->d_super, c->nsuper,
-                     c->d_brick_mask, c->d_super_mask);
+  const uint32_t waves = (c->ncell + 63u) / 64u + (c->nsuper + 63u) / 64u;
+  hipLaunchKernelGGL(k_masks, dim3((waves + 3u) / 4u), dim3(256), 0, c->stream, c->d_grid, c->ncell, c->d_super,
+                     c->nsuper, c->d_brick_mask, c->d_super_mask);
   PX_HIP_CHECK(c, hipGetLastError());
   const uint32_t ns = c->nsuper;
-  hipLaunchKernelGGL(k_super_bm, dim3((ns + 255u) / 256u), dim3(256), 0, c->stream, c->d_grid, (int32_t)c->bd[0],
+  hipLaunchKernelGGL(k_super_bm, dim3((ns + 3u) / 4u), dim3(256), 0, c->stream, c->d_grid, (int32_t)c->bd[0],
                      (int32_t)c->bd[1], (int32_t)c->bd[2], (int32_t)c->sd[0], (int32_t)c->sd[1], (int32_t)c->sd[2],
                      c->d_super_bm);
   PX_HIP_CHECK(c, hipGetLastError());
