@@ -85,6 +85,12 @@ constexpr uint32_t kFlagSeed = 16u;
 // blocks were slower on the 64-impact island pass: 32 / 26 against 24 us)
 constexpr uint32_t kJobBlocks = 64;
 constexpr uint32_t kFlagCrumb = 8u;
+// per pool brick (labelling): bnode = its island node if its node-class voxels
+// form one component, kNoNode if it has none, kManyNodes otherwise; bface =
+// 12 masks of 64 bits, node-class voxels then structural voxels, each of the
+// faces x = 0, y = 0, z = 0 then x = 7, y = 7, z = 7 (face_masks())
+constexpr uint32_t kNoNode = 0xFFFFFFFFu, kManyNodes = 0xFFFFFFFEu;
+constexpr uint32_t kFaceMasks = 12;
 constexpr uint32_t kNoBlock = 0xFFFFFFFFu;
 constexpr uint32_t kMaxChunkIds = 511;  // chunk id + 1 fits bits 8..16
 // per-block partials (part[b * kPartWords + k])
@@ -111,6 +117,8 @@ struct EditView {
   uint32_t* cnt;
   uint32_t* flag;
   uint16_t* lab;
+  uint32_t* bnode;  // per pool brick: its node if its island voxels are one component, else kNoNode / kManyNodes
+  uint64_t* bface;  // per pool brick: kFaceMasks face masks (face_masks())
   uint32_t* nodes;  // kEditBlocks ranges of slots_node
   uint32_t slots_cell, slots_node;
   uint32_t walk_cap;  // step bound of the union-find walks (>= the node capacity)
@@ -404,6 +412,23 @@ PX_DEV void load_tables(const float* mats, uint8_t* s_cls, uint8_t* s_str) {
   }
 }
 
+// The voxels of a brick's six faces as 64-bit masks, from its rows (lane l
+// holds row y = l & 7, z = l >> 3, bit x of `bits`): out[0..2] the faces
+// x = 0, y = 0, z = 0 and out[3..5] the faces x = 7, y = 7, z = 7, with bit
+// y + 8 z on x faces, x + 8 z on y faces and x + 8 y on z faces, so the
+// touching faces of two neighbouring bricks index their voxel pairs alike.
+PX_DEV void face_masks(uint32_t bits, uint64_t out[6]) {
+  // lane t of a y or z face stands for (x = t & 7, the other axis t >> 3):
+  // it fetches that row's bits and the ballot packs the face
+  const uint32_t t = __lane_id(), x = t & 7u, q = t >> 3;
+  out[0] = __ballot((bits & 1u) != 0u);
+  out[3] = __ballot(((bits >> 7) & 1u) != 0u);
+  out[1] = __ballot(((__shfl(bits, (int)(8u * q)) >> x) & 1u) != 0u);        // y = 0, z = q
+  out[4] = __ballot(((__shfl(bits, (int)(8u * q + 7u)) >> x) & 1u) != 0u);   // y = 7, z = q
+  out[2] = __ballot(((__shfl(bits, (int)q) >> x) & 1u) != 0u);               // z = 0, y = q
+  out[5] = __ballot(((__shfl(bits, (int)(56u + q)) >> x) & 1u) != 0u);       // z = 7, y = q
+}
+
 // ---------------------------------------------------------------------------
 
 // Start of a pass (impactArgs + searchBegin, sim.wgsl:408-412, 700-720): the
@@ -652,7 +677,19 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         if (cl[x]) m |= 1u << x;
         if (v && s_str[v & 0xffu]) sm |= 1u << x;
       }
-      if (__ballot(m != 0u) == 0ull) continue;  // no node voxel in this brick
+      if (MODE == 0) {  // the brick's face masks for the face merge
+        uint64_t fm[kFaceMasks];
+        face_masks(m, fm);
+        face_masks(sm, fm + 6);
+        uint64_t mine = 0ull;
+        for (uint32_t k = 0; k < kFaceMasks; k++)
+          if (lane == k) mine = fm[k];
+        if (lane < kFaceMasks) e.bface[(size_t)p * kFaceMasks + lane] = mine;
+      }
+      if (__ballot(m != 0u) == 0ull) {  // no node voxel in this brick
+        if (MODE == 0 && lane == 0u) e.bnode[p] = kNoNode;
+        continue;
+      }
       uint32_t c_lo = cl[0] | (cl[1] << 8) | (cl[2] << 16) | (cl[3] << 24);
       uint32_t c_hi = cl[4] | (cl[5] << 8) | (cl[6] << 16) | (cl[7] << 24);
       uint32_t lx = 0u;  // x-links: voxel x continues the run of x-1
@@ -692,7 +729,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         const uint64_t rows = __ballot(m != 0u), yj = __ballot(ey != 0u), zj = __ballot(ez != 0u);
         const uint32_t cnt_l = (uint32_t)__popc(m);
         const bool anc_l = (m & abits) != 0u;
-        uint32_t myroot = kLabNone, total = 0u;
+        uint32_t myroot = kLabNone, total = 0u, one = kNoNode;
         for (uint64_t rem = rows; rem != 0ull;) {
           uint64_t comp = rem & (~rem + 1ull);
           for (;;) {
@@ -708,6 +745,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
           const uint32_t cnt = wave_sum(in ? cnt_l : 0u);
           const bool anch = __ballot(in && anc_l) != 0ull;
           if (in) myroot = rli;
+          if (total == 0u) one = okey + loff(e, rli);
           if (lane == 0u) {
             const uint32_t node = okey + loff(e, rli);
             e.par[node] = node;
@@ -719,6 +757,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
           total++;
         }
         nn += total;
+        if (MODE == 0 && lane == 0u) e.bnode[p] = total == 1u ? one : kManyNodes;
         uint32_t rx[8];
         for (int x = 0; x < 8; x++) rx[x] = ((m >> x) & 1u) ? myroot : kLabNone;
         const uint4 lw = make_uint4(rx[0] | (rx[1] << 16), rx[2] | (rx[3] << 16), rx[4] | (rx[5] << 16),
@@ -778,6 +817,7 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
         slot++;
       }
       nn += total;
+      if (MODE == 0 && lane == 0u) e.bnode[p] = kManyNodes;  // the face merge takes this brick voxel by voxel
       uint4 lw;
       lw.x = rootx[0] | (rootx[1] << 16);
       lw.y = rootx[2] | (rootx[3] << 16);
@@ -911,6 +951,70 @@ PX_DEV void merge_cell(const EditView& e, uint32_t ci, uint32_t g, const uint32_
   uf_union3(e.par, ua, ub, act, e.walk_cap, walk_err);
 }
 
+// What the face merge needs of one side of a face: the island node of the
+// cell's brick (kNoNode / kManyNodes as in bnode) and the node-class and
+// structural voxels of its face `f` (face_masks() order, 0..5).  A uniform
+// brick of a node class is one node (its origin key) and fills its faces.
+PX_DEV void face_side(const EditView& e, uint32_t g, uint32_t okey, uint32_t f, const uint8_t* s_cls,
+                      const uint8_t* s_str, uint32_t& node, uint64_t& cls, uint64_t& str) {
+  if ((g & PX_UNIFORM_BIT) != 0u) {
+    const uint32_t v = g & 0xffu;
+    node = s_cls[v] ? okey : kNoNode;
+    cls = s_cls[v] ? ~0ull : 0ull;
+    str = s_str[v] ? ~0ull : 0ull;
+    return;
+  }
+  const uint32_t p = g - 1u;
+  node = e.bnode[p];
+  cls = e.bface[(size_t)p * kFaceMasks + f];
+  str = e.bface[(size_t)p * kFaceMasks + 6u + f];
+}
+
+// The island face merge of the cell each lane holds (grid words mg: the
+// cell, then its +x / +y / +z neighbours), one cell per lane, for faces
+// whose two bricks are each one node (or none): the pair joins when the
+// faces' node-class masks meet, and a node is anchored when its class voxels
+// meet structural ones across the face.  Returns the lanes whose cell has a
+// brick of several nodes on some face; merge_cell() takes those voxel by
+// voxel.
+PX_DEV uint64_t merge_simple(const EditView& e, uint32_t ci, const uint32_t mg[4], const uint8_t* s_cls,
+                             const uint8_t* s_str, uint32_t& walk_err) {
+  bool hard = false;
+  uint32_t ua[3] = {0u, 0u, 0u}, ub[3] = {0u, 0u, 0u}, act = 0u;
+  if (mg[0] != 0u && (mg[1] | mg[2] | mg[3]) != 0u) {
+    int32_t bx, by, bz;
+    cell_coord(e, ci, bx, by, bz);
+    const uint32_t okey = vkey(e, bx * 8, by * 8, bz * 8);
+    const uint32_t step[3] = {8u, 8u * (uint32_t)e.dims[0], 8u * (uint32_t)e.dims[0] * (uint32_t)e.dims[1]};
+    uint32_t na[3], nb[3];
+    uint64_t ca[3], sa[3], cb[3], sb[3];
+    for (int d = 0; d < 3; d++) {
+      na[d] = nb[d] = kNoNode;
+      ca[d] = sa[d] = cb[d] = sb[d] = 0ull;
+      if (mg[1 + d] == 0u) continue;
+      face_side(e, mg[0], okey, 3u + (uint32_t)d, s_cls, s_str, na[d], ca[d], sa[d]);
+      face_side(e, mg[1 + d], okey + step[d], (uint32_t)d, s_cls, s_str, nb[d], cb[d], sb[d]);
+    }
+    for (int d = 0; d < 3; d++) {
+      if (mg[1 + d] == 0u) continue;
+      if (na[d] == kManyNodes || nb[d] == kManyNodes) {
+        hard = true;
+        continue;
+      }
+      if (na[d] != kNoNode && nb[d] != kNoNode && (ca[d] & cb[d]) != 0ull) {
+        ua[d] = na[d];
+        ub[d] = nb[d];
+        act |= 1u << d;
+      }
+      if (na[d] != kNoNode && (ca[d] & sb[d]) != 0ull) atomicOr(&e.flag[na[d]], kFlagAnchor);
+      if (nb[d] != kNoNode && (sa[d] & cb[d]) != 0ull) atomicOr(&e.flag[nb[d]], kFlagAnchor);
+    }
+  }
+  if (hard) act = 0u;
+  uf_union3(e.par, ua, ub, act, e.walk_cap, walk_err);
+  return __ballot(hard);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(64) void k_face_merge(EditView e) {
   __shared__ uint8_t s_cls[256];
@@ -931,7 +1035,9 @@ __global__ __launch_bounds__(64) void k_face_merge(EditView e) {
       const uint32_t mc = blockIdx.x + (k0 + lane) * gridDim.x;
       uint32_t mg[4];
       cell_and_neighbours(e, mc, mc < e.ncell, mg);
-      for (uint64_t live = __ballot(mg[0] != 0u && (mg[1] | mg[2] | mg[3]) != 0u); live; live &= live - 1ull) {
+      // one cell per lane where each face's bricks are one node or none,
+      // then the rest a cell at a time, voxel pair by voxel pair
+      for (uint64_t live = merge_simple(e, mc, mg, s_cls, s_str, walk_err); live; live &= live - 1ull) {
         const uint32_t l = first_lane(live);
         const uint32_t gn[3] = {lane_value(mg[1], l), lane_value(mg[2], l), lane_value(mg[3], l)};
         merge_cell<0>(e, lane_value(mc, l), lane_value(mg[0], l), gn, s_cls, s_str, walk_err);
@@ -3001,6 +3107,8 @@ EditView make_view(px_ctx* c) {
   e.cnt = c->d_ncnt;
   e.flag = c->d_nflag;
   e.lab = c->d_lab;
+  e.bnode = c->d_bnode;
+  e.bface = c->d_bface;
   e.nodes = c->d_nodes;
   e.slots_cell = (c->ncell + kEditBlocks - 1u) / kEditBlocks;
   e.slots_node = e.slots_cell * 256u;
@@ -3090,14 +3198,16 @@ int px_edit_alloc(px_ctx* c) {
 void px_edit_free(px_ctx* c) {
   void* bufs[] = {c->d_impacts, c->d_impact_stage, c->d_cell_imask, c->d_dirty_flags,
                   c->d_job_list, c->d_alloc_list, c->d_pane_cells, c->d_part, c->d_bcount, c->d_edit_ctr, c->d_par,
-                  c->d_ncnt, c->d_nflag, c->d_lab, c->d_nodes, c->d_out_labels, c->d_seeds, c->d_shots,
-                  c->d_flashes, c->d_edit_work, c->d_carve_rows};
+                  c->d_ncnt, c->d_nflag, c->d_lab, c->d_bnode, c->d_bface, c->d_nodes, c->d_out_labels,
+                  c->d_seeds, c->d_shots, c->d_flashes, c->d_edit_work, c->d_carve_rows};
   for (void* p : bufs)
     if (p) hipFree(p);
   c->d_impacts = c->d_impact_stage = nullptr;
   c->d_cell_imask = nullptr;
   c->d_par = c->d_ncnt = c->d_nflag = nullptr;
   c->d_lab = nullptr;
+  c->d_bnode = nullptr;
+  c->d_bface = nullptr;
   c->d_nodes = c->d_out_labels = c->d_seeds = c->d_shots = nullptr;
   c->d_flashes = nullptr;
   c->d_edit_work = nullptr;
@@ -3115,6 +3225,8 @@ static int ensure_ccl(px_ctx* c) {
   PX_HIP_CHECK(c, hipMalloc(&c->d_ncnt, nv * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_nflag, nv * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_lab, (size_t)c->pool_cap * 1024));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_bnode, (size_t)c->pool_cap * 4));
+  PX_HIP_CHECK(c, hipMalloc(&c->d_bface, (size_t)c->pool_cap * kFaceMasks * 8));
   PX_HIP_CHECK(c, hipMalloc(&c->d_nodes, node_cap * 4));
   PX_HIP_CHECK(c, hipMalloc(&c->d_out_labels, label_cap * 8));
   c->node_cap = (uint32_t)std::min<size_t>(node_cap, 0xFFFFFFF0u);

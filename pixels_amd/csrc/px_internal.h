@@ -113,6 +113,8 @@ struct px_ctx {
   uint32_t* d_ncnt = nullptr;    // per node: voxel count, then per-root totals
   uint32_t* d_nflag = nullptr;   // per node flags
   uint16_t* d_lab = nullptr;     // per pool brick: local root of each voxel (0xFFFF = none)
+  uint32_t* d_bnode = nullptr;   // per pool brick: its one island node, none, or several (labelling)
+  uint64_t* d_bface = nullptr;   // per pool brick: node-class and structural voxels of its 6 faces
   uint32_t* d_nodes = nullptr;   // node list
   uint32_t node_cap = 0;
   uint32_t* d_out_labels = nullptr;  // (key, count) pairs
