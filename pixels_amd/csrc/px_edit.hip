@@ -1066,6 +1066,21 @@ __global__ __launch_bounds__(64) void k_face_merge(EditView e) {
 // After the unions: par[x] = root(x) for every node of block ranges `which`.
 // The walk only reads (no path halving), so each par[x] is written by x's
 // own thread alone and ends as the root.
+// The root of node x after the unions, by a read-only walk (same order
+// check and step bound as the union walks; kErrWalk into err when either
+// fails, stopping on a key below x).
+PX_DEV uint32_t root_of(const EditView& e, uint32_t x, uint32_t& err) {
+  uint32_t r = x;
+  for (uint32_t it = 0; it < e.walk_cap; it++) {
+    const uint32_t p = __hip_atomic_load(e.par + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p == r) return r;
+    if (p > r) break;  // never written by a node of this pass
+    r = p;
+  }
+  err = kErrWalk;
+  return r;
+}
+
 __global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
   if (e.ec[which == BC_NODES ? EC_ACTIVE : EC_NIMP] == 0u) return;
   uint32_t walk_err = 0u;
@@ -1073,20 +1088,8 @@ __global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
     uint32_t n = e.bcount[b * kCountWords + which];
     const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-      uint32_t x = nodes[i];
-      uint32_t r = x;
-      bool ok = false;
-      for (uint32_t it = 0; it < e.walk_cap; it++) {
-        uint32_t p = __hip_atomic_load(e.par + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (p == r) {
-          ok = true;
-          break;
-        }
-        if (p > r) break;  // never written by a node of this pass
-        r = p;
-      }
-      if (!ok) walk_err = kErrWalk;
-      __hip_atomic_store(e.par + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t x = nodes[i];
+      __hip_atomic_store(e.par + x, root_of(e, x, walk_err), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   if (__ballot(walk_err != 0u) != 0ull && threadIdx.x == 0u) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
@@ -1210,7 +1213,10 @@ __global__ __launch_bounds__(64) void k_maint(EditView e) {
 // Per-root totals.  Lanes of a wavefront that share a root are reduced first
 // (one loop turn per distinct root), so a component spanning many bricks
 // takes one atomic per wavefront instead of one per brick-local node.
-template <int PHASE>
+// FLATTEN (PHASE 0 only): the kernel also flattens its nodes, as k_flatten
+// would have just before it (each node is in one list, so par[x] has one
+// writer; other nodes' stores only shorten the walks).
+template <int PHASE, bool FLATTEN>
 __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
   // PHASE 0: node flags -> root flags; PHASE 1: node counts -> root counts,
   // for roots left unanchored only (anchored components need no count, and
@@ -1219,13 +1225,22 @@ __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
   const uint32_t lane = threadIdx.x;
   if (e.ec[EC_ACTIVE] == 0u) return;
   const uint32_t need = e.mode == PX_ISLANDS_SEEDED ? kFlagSeed : 0u;
+  uint32_t walk_err = 0u;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t n = e.bcount[b * kCountWords + BC_NODES];
     const uint32_t* nodes = e.nodes + (size_t)b * e.slots_node;
     for (uint32_t base = 0; base < n; base += 64u) {
       uint32_t i = base + lane;
       uint32_t x = i < n ? nodes[i] : 0u;
-      uint32_t r = i < n ? e.par[x] : 0u;
+      uint32_t r = 0u;
+      if (i < n) {
+        if (PHASE == 0 && FLATTEN) {  // k_flatten's walk and store, done here
+          r = root_of(e, x, walk_err);
+          __hip_atomic_store(e.par + x, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+          r = e.par[x];
+        }
+      }
       bool pending = i < n && r != x;
       uint32_t v = 0u;
       if (pending) {
@@ -1260,6 +1275,7 @@ __global__ __launch_bounds__(64) void k_aggregate(EditView e) {
       }
     }
   }
+  if (__ballot(walk_err != 0u) != 0ull && lane == 0u) atomicOr(&e.ec[EC_OVERFLOW], kErrWalk);
 }
 
 // Unsupported roots -> (label, count), appended with one atomic per
@@ -3312,11 +3328,14 @@ int px_launch_island_pass(px_ctx* c, bool force) {
   // labelling
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
-  if (c->lim.island_mode == PX_ISLANDS_SEEDED)
+  if (c->lim.island_mode == PX_ISLANDS_SEEDED) {  // the seeds find their roots through flat parents
+    hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_NODES);
     hipLaunchKernelGGL(k_seed_mark, dim3(2048), dim3(256), 0, s, e);
-  hipLaunchKernelGGL(k_aggregate<0>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_aggregate<1>, G, W, 0, s, e);
+    hipLaunchKernelGGL((k_aggregate<0, false>), G, W, 0, s, e);
+  } else {
+    hipLaunchKernelGGL((k_aggregate<0, true>), G, W, 0, s, e);
+  }
+  hipLaunchKernelGGL((k_aggregate<1, false>), G, W, 0, s, e);
   hipLaunchKernelGGL(k_emit<0>, G, W, 0, s, e);
   if (c->lim.island_mode == PX_ISLANDS_SEEDED) {
     hipLaunchKernelGGL(k_node_budget, dim3(1), dim3(1024), 0, s, e);
