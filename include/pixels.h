@@ -530,6 +530,7 @@ int px_restore(px_ctx* ctx);
 #define PX_PROF_RENDER_SHADOW 12  /* render: queued shadow rays with transmittance */
 #define PX_PROF_SHADE 13   /* render: shading (or the debug-view kernel) */
 #define PX_PROF_REFLECT 14 /* render: queued reflection rays with their quality-0 shading */
+#define PX_PROF_GRAPH 15   /* px_graph_launch: a recorded frame of stages, whole */
 #define PX_PROF_SLOTS 16
 int px_profile_begin(px_ctx* ctx);
 int px_profile_end(px_ctx* ctx, double out[2 * PX_PROF_SLOTS]);

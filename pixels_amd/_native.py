@@ -166,6 +166,7 @@ SORT_FACES = 1  # px_mlp_sort_rays flag: keys (direction face, cost bucket)
 PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_PARTICLES = 0, 1, 2, 3, 4, 5, 6
 PROF_PARTICLE_SIM, PROF_WRITES, PROF_CHUNKS = 7, 8, 9
 PROF_DENSITY, PROF_RENDER_PRIMARY, PROF_RENDER_SHADOW, PROF_SHADE, PROF_REFLECT = 10, 11, 12, 13, 14
+PROF_GRAPH = 15
 PROF_SLOTS = 16
 
 _lib = None

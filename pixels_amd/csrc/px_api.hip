@@ -783,7 +783,10 @@ int px_capture_sim_step(px_ctx* c, uint32_t stages_mask, px_graph** out) {
 int px_graph_launch(px_ctx* c, px_graph* G) {
   if (!c || !G || G->ctx != c) return PX_ERR_ARG;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  PX_HIP_CHECK(c, hipGraphLaunch(G->exec, c->stream));
+  {
+    PxProfScope prof(c, PX_PROF_GRAPH);
+    PX_HIP_CHECK(c, hipGraphLaunch(G->exec, c->stream));
+  }
   // the frame changed the world: masks and density are rebuilt before the
   // next eager use
   c->labels_valid = G->labels_valid;
