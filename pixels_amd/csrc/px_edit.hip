@@ -429,6 +429,11 @@ PX_DEV void face_masks(uint32_t bits, uint64_t out[6]) {
   out[5] = __ballot(((__shfl(bits, (int)(56u + q)) >> x) & 1u) != 0u);       // z = 7, y = q
 }
 
+// defined with the stages they belong to, run inside other kernels of the
+// island pass
+PX_DEV void effects_finish(const EditView& e);
+PX_DEV void shot_flood(const EditView& e);
+
 // ---------------------------------------------------------------------------
 
 // Start of a pass (impactArgs + searchBegin, sim.wgsl:408-412, 700-720): the
@@ -603,13 +608,18 @@ __global__ __launch_bounds__(64) void k_alloc(EditView e) {
   }
 }
 
-__global__ void k_alloc_commit(EditView e) {
-  if (threadIdx.x != 0) return;
+// The free-list top after k_alloc's requests, and the failed ones (one
+// thread).
+PX_DEV void alloc_commit(const EditView& e) {
   uint32_t n = min(e.ec[EC_ALLOCS], e.ncell);
   uint32_t top = e.ctrl[PX_C_FREE_TOP];
   uint32_t used = top > e.pool_cap ? 0u : min(n, top);
   e.ctrl[PX_C_FREE_TOP] = top - used;
   e.ctrl[PX_S_ALLOC_FAIL] += n - used;
+}
+
+__global__ void k_alloc_commit(EditView e) {
+  if (threadIdx.x == 0) alloc_commit(e);
 }
 
 // Per-brick labelling.  MODE 0: islands (uniform bricks of a node class are
@@ -629,6 +639,9 @@ __global__ __launch_bounds__(64) void k_local_ccl(EditView e) {
   __shared__ uint8_t s_str[256];
   const uint32_t lane = threadIdx.x;
   const uint32_t ry = lane & 7u, rz = lane >> 3;
+  // the island pass's k_alloc_commit, whether the pass runs or not (nothing
+  // here reads the free-list top)
+  if (MODE == 1 && blockIdx.x == 0u && lane == 0u) alloc_commit(e);
   if (e.ec[MODE == 0 ? EC_ACTIVE : EC_NIMP] == 0u) return;
   load_tables<MODE>(e.mats, s_cls, s_str);
   __syncthreads();
@@ -1083,6 +1096,10 @@ PX_DEV uint32_t root_of(const EditView& e, uint32_t x, uint32_t& err) {
 
 __global__ __launch_bounds__(64) void k_flatten(EditView e, uint32_t which) {
   if (e.ec[which == BC_NODES ? EC_ACTIVE : EC_NIMP] == 0u) return;
+  if (which == BC_PANENODES) {  // cpair of every cell to (0, 0) before the count passes
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < e.ncell; i += gridDim.x * blockDim.x)
+      e.cpair[i] = make_uint2(0u, 0u);
+  }
   uint32_t walk_err = 0u;
   for (uint32_t b = blockIdx.x; b < kEditBlocks; b += gridDim.x) {
     uint32_t n = e.bcount[b * kCountWords + which];
@@ -1183,6 +1200,9 @@ PX_DEV uint32_t maint_batch(const EditView& e, uint32_t c, bool in) {
 __global__ __launch_bounds__(64) void k_maint(EditView e) {
   if (e.ec[EC_NIMP] == 0u) return;
   const uint32_t lane = threadIdx.x;
+  // spawn()'s counters after the carve and flood effects (nothing here
+  // reads them)
+  if (blockIdx.x == 0u && lane == 0u) effects_finish(e);
   uint32_t dirty = 0u;
   const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
   for (uint32_t i0 = blockIdx.x; i0 < n; i0 += 64u * gridDim.x) {
@@ -1956,6 +1976,9 @@ __global__ __launch_bounds__(64) void k_carve(EditView e) {
   const uint32_t lane = threadIdx.x;
   const uint32_t nimp = e.ec[EC_NIMP];
   if (nimp == 0u) return;
+  // the last block also marks the pane components shoot's flood entries hit
+  // (the count pass only reads the world; flags are only OR-ed in)
+  if (PASS == 0 && blockIdx.x == gridDim.x - 1u) shot_flood(e);
   const uint32_t n = min(e.ec[EC_JOBCELLS], e.ncell);
   if (blockIdx.x >= n) {  // no job entry for this block: only the partials
     if (PASS == 1 && lane == 0u) {
@@ -2276,16 +2299,10 @@ __global__ __launch_bounds__(64) void k_flood(EditView e) {
 }
 
 // cpair of every cell to (0, 0) before the count passes (a pass with impacts)
-__global__ void k_edit_zero_cells(EditView e) {
-  if (e.ec[EC_NIMP] == 0u) return;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < e.ncell; i += gridDim.x * blockDim.x)
-    e.cpair[i] = make_uint2(0u, 0u);
-}
 
 // spawn()'s counters after the effects: every call counts, the accepted ones
 // advance the cursor (sim.wgsl:166-170)
-__global__ void k_effects_finish(EditView e) {
-  if (threadIdx.x != 0 || e.ec[EC_NIMP] == 0u) return;
+PX_DEV void effects_finish(const EditView& e) {
   const uint32_t total = e.etot[0] + e.etot[1];
   const uint32_t acc = spawn_accepted(e, total);
   e.ctrl[PX_C_SPAWNED] += total;
@@ -2423,8 +2440,7 @@ __global__ __launch_bounds__(64) void k_shoot(EditView e, PxWorldView w, const f
 // same-id neighbours are flooded whole, which is what the breadth-first flood
 // from the struck voxel reaches; in seeded mode the struck voxel's other
 // neighbours are seeds.  After the pane labelling, before the flood passes.
-__global__ __launch_bounds__(64) void k_shot_flood(EditView e) {
-  if (e.ec[EC_NIMP] == 0u) return;
+PX_DEV void shot_flood(const EditView& e) {
   const uint32_t n = min(e.shots[0], kShotFloodCap);
   uint32_t err = 0u;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
@@ -3310,21 +3326,19 @@ int px_launch_island_pass(px_ctx* c, bool force) {
   hipLaunchKernelGGL(k_edit_begin, dim3(1), dim3(64), 0, s, e, force ? 1u : 0u);
   hipLaunchKernelGGL(k_impact_jobs, dim3(kJobBlocks, 64), dim3(256), 0, s, e);
   hipLaunchKernelGGL(k_alloc, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_alloc_commit, dim3(1), dim3(64), 0, s, e);
   // pane components on the world as it is after alloc and before carve
+  // (k_local_ccl<1> commits the allocations first)
   hipLaunchKernelGGL(k_local_ccl<1>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<1>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_PANENODES);
-  hipLaunchKernelGGL(k_shot_flood, dim3(1), dim3(64), 0, s, e);
-  // carve + flood: count the effects, rank them in cell order, apply
-  hipLaunchKernelGGL(k_edit_zero_cells, dim3(1024), dim3(256), 0, s, e);
+  hipLaunchKernelGGL(k_flatten, G, W, 0, s, e, (uint32_t)BC_PANENODES);  // also clears cpair
+  // carve + flood: count the effects (k_carve<0> also takes shoot's flood
+  // entries), rank them in cell order, apply
   hipLaunchKernelGGL(k_carve<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_flood<0>, G, W, 0, s, e);
   scan2_launch(s, e.cpair, e.coff, e.tiles, c->scan_cap, nullptr, c->ncell, nullptr, nullptr, e.etot);
   hipLaunchKernelGGL(k_carve<1>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_flood<1>, G, W, 0, s, e);
-  hipLaunchKernelGGL(k_effects_finish, dim3(1), dim3(64), 0, s, e);
-  hipLaunchKernelGGL(k_maint, G, W, 0, s, e);
+  hipLaunchKernelGGL(k_maint, G, W, 0, s, e);  // also spawn()'s counters
   // labelling
   hipLaunchKernelGGL(k_local_ccl<0>, G, W, 0, s, e);
   hipLaunchKernelGGL(k_face_merge<0>, G, W, 0, s, e);

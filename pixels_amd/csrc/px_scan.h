@@ -99,11 +99,41 @@ __global__ __launch_bounds__(1024) void k_scan2_apply(const uint2* __restrict__ 
   if (i < n) out[i] = make_uint2(t.x + ex.x, t.y + ex.y);
 }
 
+// k_scan2_apply for at most kScanTile tiles, without k_scan2_top: block b
+// adds the totals of tiles 0 .. b-1 itself (one per thread), and the last
+// tile's block writes the totals.
+__global__ __launch_bounds__(1024) void k_scan2_apply_small(const uint2* __restrict__ in, uint32_t n,
+                                                            const uint2* __restrict__ tiles,
+                                                            const uint32_t* base_x, const uint32_t* base_y,
+                                                            uint2* __restrict__ out, uint32_t* __restrict__ tot) {
+  const uint32_t i = blockIdx.x * kScanTile + threadIdx.x;
+  const uint2 tb = threadIdx.x < blockIdx.x ? tiles[threadIdx.x] : make_uint2(0u, 0u);
+  uint2 before;
+  block_excl2(tb, &before);
+  const uint2 v = i < n ? in[i] : make_uint2(0u, 0u);
+  uint2 own;
+  const uint2 ex = block_excl2(v, &own);
+  const uint32_t bx = base_x ? *base_x : 0u, by = base_y ? *base_y : 0u;
+  if (i < n) out[i] = make_uint2(bx + before.x + ex.x, by + before.y + ex.y);
+  if (blockIdx.x == gridDim.x - 1u && threadIdx.x == 0u) {
+    tot[0] = before.x + own.x;
+    tot[1] = before.y + own.y;
+  }
+}
+
 // Exclusive (x, y) offsets of in[0..n) into out (n = *n_dev, or n_fixed when
 // n_dev is null; cap bounds n), totals into tot[0..1]; tiles holds
 // ceil(cap / 1024) entries.
 inline void scan2_launch(hipStream_t s, const uint2* in, uint2* out, uint2* tiles, size_t cap, const uint32_t* n_dev,
                          uint32_t n_fixed, const uint32_t* base_x, const uint32_t* base_y, uint32_t* tot) {
+  if (!n_dev && n_fixed > 0u && n_fixed <= (size_t)kScanTile * kScanTile && n_fixed <= cap) {
+    // a length known here, in at most kScanTile tiles: two launches
+    const uint32_t nt = (n_fixed + kScanTile - 1u) / kScanTile;
+    hipLaunchKernelGGL(k_scan2_tiles, dim3(nt), dim3(kScanTile), 0, s, in, nullptr, n_fixed, tiles);
+    hipLaunchKernelGGL(k_scan2_apply_small, dim3(nt), dim3(kScanTile), 0, s, in, n_fixed, tiles, base_x, base_y,
+                       out, tot);
+    return;
+  }
   const uint32_t ntiles = (uint32_t)((cap + kScanTile - 1u) / kScanTile);
   hipLaunchKernelGGL(k_scan2_tiles, dim3(ntiles), dim3(kScanTile), 0, s, in, n_dev, n_fixed, tiles);
   hipLaunchKernelGGL(k_scan2_top, dim3(1), dim3(1024), 0, s, tiles, n_dev, n_fixed, base_x, base_y, tot);
