@@ -680,8 +680,123 @@ def batch_leg(args):
             ok = ok and bool(np.array_equal(got.view(np.uint8), exp.view(np.uint8)))
         res["bit_exact_vs_oracle_sample"] = ok
         log("batched sample bit-exact vs oracle: %s" % ok)
+    if not args.no_learned:
+        res["learned"] = batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, plain_ms, batch_ms)
     gw.close()
     return res
+
+
+def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms, batch_ms):
+    """The learned-binned batched launch on the 2560x704x1920 mix.  The 3M
+    shadow queries arrive in the order fsMain makes them (pixel, then sun and
+    lights), which is the plain launches' order.  A 16-64-64-1 network is
+    trained on this scene (Adam, lr 1e-3, batch 65,536, 12 epochs) on the
+    shadow rays' step counts from 128 seeded 480x270 cameras, then orders the
+    measured shadow queries by (direction face, one of 32 predicted-cost
+    buckets) before one px_trace_batch_perm launch; primary rays (8x8 tiles),
+    shoot rays and segments stay in index order.  Against: the per-kind
+    launches and the batched launch in index order, and the same sort with
+    one cost bucket (direction face only, no network)."""
+    import torch
+    from pixels_amd import _native as N
+    from pixels_amd.learned import CostModel, log2_steps, wave_cost
+    from pixels_amd.workloads import shadow_queries
+
+    dev = torch.device("cuda:0")
+    vs = 1.0 / 160.0
+    TW, TH, CAMS = 480, 270, 128
+    t0 = time.time()
+    model = CostModel(gw, seed=args.seed)
+    feats, targets = [], []
+    cam = 5000
+    while len(targets) < CAMS and cam < 5000 + 4 * CAMS:
+        cam += 1
+        prim = scene.camera_rays(TW, TH, 90.0, seed=cam, tile=8)
+        dp = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
+        op = torch.empty(prim.shape[0] * 16, dtype=torch.uint8, device=dev)
+        sim.trace_device(0, dp, prim.shape[0], op)
+        gw.sync()
+        u, lights = scene.frame_inputs(TW, TH, seed=cam, voxel_size=vs, light_radius_vox=8192.0)
+        try:
+            sh = shadow_queries(scene, prim, op.cpu().numpy().view(N.HIT_DTYPE), u, lights, TW * TH)
+        except IndexError:
+            continue
+        ds = torch.from_numpy(sh.view(np.uint8).copy()).to(dev)
+        n = sh.shape[0]
+        f = torch.empty(n * 16, dtype=torch.int16, device=dev)
+        model.features(1, ds, n, f)
+        o = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+        sim.trace_device(1, ds, n, o)
+        gw.sync()
+        feats.append(f)
+        targets.append(torch.from_numpy(log2_steps(o.cpu().numpy().view(N.HIT_DTYPE)["steps"])).to(dev))
+    tf, tt = torch.cat(feats), torch.cat(targets)
+    del feats, targets
+    n_train = tt.numel()
+    order = torch.randperm(n_train, device=dev, generator=torch.Generator(device=dev).manual_seed(args.seed))
+    tf = tf.view(n_train, 16)[order].reshape(-1).contiguous()
+    tt = tt[order].contiguous()
+    bs, epochs = 65536, 12
+    first = model.train_step(tf, tt, bs, lr=1e-3, want_loss=True)
+    for _ in range(epochs):
+        for off in range(0, n_train - bs + 1, bs):
+            model.train_step(tf[off * 16:], tt[off:], bs, lr=1e-3)
+    last = model.train_step(tf, tt, bs, lr=1e-3, want_loss=True)
+    del tf, tt
+    train_s = time.time() - t0
+
+    n_sh = counts[1]
+    perm = torch.empty(n_sh, dtype=torch.int32, device=dev)
+    outl = {k: torch.empty_like(v) for k, v in outb.items()}
+    ql = sim.query_set({k: (rays[k], counts[k]) for k in host}, outl)
+
+    def learned(buckets=32):
+        model.sort_rays(1, rays[1], n_sh, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True)
+        sim.trace_batch_perm(ql, {1: perm})
+
+    learned()
+    gw.sync()
+    same = all(torch.equal(outl[k], outb[k]) for k in host)
+    steps = outb[1].cpu().numpy().view(N.HIT_DTYPE)["steps"].astype(np.int64)
+    pm = perm.cpu().numpy().astype(np.int64)
+    wc_plain, wc_learned = wave_cost(steps), wave_cost(steps[pm])
+    reps, rounds = 3, 7
+    pb, lb, ls, fb, fs = [], [], [], [], []
+    for _ in range(rounds):  # interleaved, medians
+        gw.profile_begin()
+        for _ in range(reps):
+            sim.trace_batch(q)
+        pb.append(gw.profile_end()[N.PROF_BATCH][0] / reps)
+        for buckets, tb, ts in ((32, lb, ls), (1, fb, fs)):
+            gw.profile_begin()
+            for _ in range(reps):
+                learned(buckets)
+            p = gw.profile_end()
+            tb.append(p[N.PROF_BATCH][0] / reps)
+            ts.append(p.get(N.PROF_MLP, (0.0, 0))[0] / reps)
+    learned(1)
+    gw.sync()
+    wc_faces = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+    med = lambda x: float(np.median(x))
+    p_ms, l_ms, l_sort, f_ms, f_sort = med(pb), med(lb), med(ls), med(fb), med(fs)
+    total = sum(counts.values())
+    r = {"shadow_order": "pixel, then sun and lights (fsMain's order)", "train_cameras": CAMS,
+         "train_pairs": int(n_train), "epochs": epochs, "batch": bs, "loss_first": round(first, 4),
+         "loss_last": round(last, 4), "train_wall_s": round(train_s, 2),
+         "per_kind_ms": round(per_kind_ms, 4), "batch_ms": round(p_ms, 4),
+         "learned_trace_ms": round(l_ms, 4), "learned_sort_ms": round(l_sort, 4),
+         "learned_ms": round(l_ms + l_sort, 4), "learned_grays_s": round(total / ((l_ms + l_sort) * 1e-3) / 1e9, 4),
+         "speedup_vs_per_kind": round(per_kind_ms / (l_ms + l_sort), 3),
+         "speedup_vs_batch": round(p_ms / (l_ms + l_sort), 3),
+         "faces_only_ms": round(f_ms + f_sort, 4), "faces_only_speedup_vs_per_kind": round(per_kind_ms / (f_ms + f_sort), 3),
+         "shadow_wave_cost": {"index_order": wc_plain, "learned": wc_learned, "faces_only": wc_faces},
+         "bit_identical": bool(same)}
+    log("learned-binned batch: %.3f + %.3f ms sort vs per-kind %.3f ms (x%.2f), vs batch %.3f ms (x%.2f); "
+        "faces only %.3f ms (x%.2f); same bits %s" %
+        (l_ms, l_sort, per_kind_ms, r["speedup_vs_per_kind"], p_ms, r["speedup_vs_batch"], f_ms + f_sort,
+         r["faces_only_speedup_vs_per_kind"], same))
+    model.close()
+    return r
 
 
 def learned_leg(args):
@@ -1002,6 +1117,61 @@ def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
                     "learned_grays_s": round(tot / ((l_ms + s_ms) * 1e-3) / 1e9, 4),
                     "speedup": round(p_ms / (l_ms + s_ms), 3), "speedup_trace_only": round(p_ms / l_ms, 3),
                     "bit_identical": bool(same)}
+    del sh_rays, outs, outs_b, perms
+    # the same queries with the shadow rays in the order fsMain makes them
+    # (pixel, then sun and lights: the plain launches' order), the primary
+    # rays in 8x8 tiles: per-kind launches, the batched launch, and the
+    # batched launch with the shadow rays in the learned (direction face,
+    # cost) order or in direction-face order alone (no network)
+    rays = {0: (d_prim, n), 1: (d_sh, ns)}
+    outs = {0: torch.empty(n * 16, dtype=torch.uint8, device=dev), 1: torch.empty(ns * 16, dtype=torch.uint8, device=dev)}
+    outs_b = {0: torch.empty(n * 16, dtype=torch.uint8, device=dev), 1: torch.empty(ns * 16, dtype=torch.uint8, device=dev)}
+    q = sim.query_set(rays, outs)
+    qb = sim.query_set(rays, outs_b)
+    perm = torch.empty(ns, dtype=torch.int32, device=dev)
+
+    def per_kind():
+        for k in (0, 1):
+            sim.trace_device(k, rays[k][0], rays[k][1], outs[k])
+
+    def binned(buckets):
+        models[1].sort_rays(1, d_sh, ns, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True)
+        sim.trace_batch_perm(qb, {1: perm})
+    sim.trace_batch(q)
+    binned(32)
+    gw.sync()
+    same = all(torch.equal(outs[k], outs_b[k]) for k in (0, 1))
+    kb, pb, lb, ls, fb, fs = [], [], [], [], [], []
+    for _ in range(7):
+        gw.profile_begin()
+        for _ in range(3):
+            per_kind()
+        kb.append(gw.profile_end()[N.PROF_TRACE][0] / 3)
+        gw.profile_begin()
+        for _ in range(3):
+            sim.trace_batch(q)
+        pb.append(gw.profile_end()[N.PROF_BATCH][0] / 3)
+        for buckets, tb, ts in ((32, lb, ls), (1, fb, fs)):
+            gw.profile_begin()
+            for _ in range(3):
+                binned(buckets)
+            p = gw.profile_end()
+            tb.append(p[N.PROF_BATCH][0] / 3)
+            ts.append(p.get(N.PROF_MLP, (0.0, 0))[0] / 3)
+    same = same and all(torch.equal(outs[k], outs_b[k]) for k in (0, 1))
+    med = lambda x: float(np.median(x))
+    k_ms, p_ms, l_ms, s_ms, f_ms = med(kb), med(pb), med(lb), med(ls), med(fb) + med(fs)
+    res["batch_fsmain_order"] = {
+        "queries": {"primary": n, "shadow": int(ns)}, "shadow_order": "pixel, then sun and lights (fsMain's order)",
+        "per_kind_ms": round(k_ms, 4), "batch_ms": round(p_ms, 4), "learned_trace_ms": round(l_ms, 4),
+        "learned_sort_ms": round(s_ms, 4), "learned_grays_s": round(tot / ((l_ms + s_ms) * 1e-3) / 1e9, 4),
+        "per_kind_grays_s": round(tot / (k_ms * 1e-3) / 1e9, 4),
+        "speedup_vs_per_kind": round(k_ms / (l_ms + s_ms), 3), "speedup_vs_batch": round(p_ms / (l_ms + s_ms), 3),
+        "faces_only_ms": round(f_ms, 4), "faces_only_speedup_vs_per_kind": round(k_ms / f_ms, 3),
+        "bit_identical": bool(same)}
+    log("frame scene, fsMain order: learned-binned batch %.3f + %.3f ms vs per-kind %.3f ms (x%.2f), "
+        "vs batch %.3f ms (x%.2f); faces only %.3f ms; same bits %s" %
+        (l_ms, s_ms, k_ms, k_ms / (l_ms + s_ms), p_ms, p_ms / (l_ms + s_ms), f_ms, same))
     return res
 
 if __name__ == "__main__":

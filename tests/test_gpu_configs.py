@@ -11,7 +11,8 @@ oracle:
   voxels per metre): 2M primary, 3M shadow, 50k shoot rays from 64 origins,
   262,144 debris segments through one graph launch of px_trace_batch, equal
   byte for byte to the four per-kind launches, plus an oracle sample of
-  every kind;
+  every kind, and again with the shadow queries in a learned order
+  (px_mlp_sort_rays, then px_trace_batch_perm);
 * learned binning on the 512x256x512 scene with the structural floor: the
   binned launch (px_mlp_trace_sorted) gives the plain launch's hits bit for
   bit on 2M shadow rays.
@@ -121,6 +122,24 @@ def test_batched_mix_full_size():
         gw.sync()
         for k in host:
             assert np.array_equal(outb[k].cpu().numpy(), outp[k].cpu().numpy()), "replay, kind %d" % k
+        # the learned-binned form: the 3M shadow queries in the predictor's
+        # (direction face, cost bucket) order through px_trace_batch_perm
+        from pixels_amd.learned import CostModel
+        m = CostModel(gw, seed=4)
+        try:
+            perm = torch.zeros(counts[1], dtype=torch.int32, device="cuda")
+            m.sort_rays(1, rays[1], counts[1], perm, buckets=32, faces=True)
+            for t in outb.values():
+                t.fill_(0x22)
+            print("case: batched launch, shadow kind in the learned order", flush=True)
+            sim.trace_batch_perm(q, {1: perm})
+            gw.sync()
+            pm = perm.cpu().numpy()
+            assert np.array_equal(np.sort(pm), np.arange(counts[1], dtype=np.int32)), "not a permutation"
+            for k in host:
+                assert np.array_equal(outb[k].cpu().numpy(), outp[k].cpu().numpy()), "learned order, kind %d" % k
+        finally:
+            m.close()
     finally:
         gw.close()
 
