@@ -27,6 +27,9 @@ METRIC = "Grays/s primary+shadow @1280x352x960 bit-exact; island-pass ms; % HBM 
 HBM_PEAK_GBS = 8000.0
 BYTES_IN, BYTES_OUT, BYTES_STEP = 28, 13, 4
 COPY_BYTES = 256 << 20
+# windows of the learned sort in the batched comparisons: 2^18 consecutive
+# queries, each sorted by (direction face, predicted cost) on its own
+WINDOW_LOG2 = 18
 
 
 TRAFFIC_FILE = "profiles/hbm_traffic_frame_1920x1080.md"
@@ -750,8 +753,8 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
     outl = {k: torch.empty_like(v) for k, v in outb.items()}
     ql = sim.query_set({k: (rays[k], counts[k]) for k in host}, outl)
 
-    def learned(buckets=32):
-        model.sort_rays(1, rays[1], n_sh, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True)
+    def learned(buckets=32, wlog=WINDOW_LOG2):
+        model.sort_rays(1, rays[1], n_sh, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True, window_log2=wlog)
         sim.trace_batch_perm(ql, {1: perm})
 
     learned()
@@ -761,26 +764,35 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
     pm = perm.cpu().numpy().astype(np.int64)
     wc_plain, wc_learned = wave_cost(steps), wave_cost(steps[pm])
     reps, rounds = 3, 7
-    pb, lb, ls, fb, fs = [], [], [], [], []
+    pb, lb, ls, fb, fs, gb, gs = [], [], [], [], [], [], []
     for _ in range(rounds):  # interleaved, medians
         gw.profile_begin()
         for _ in range(reps):
             sim.trace_batch(q)
         pb.append(gw.profile_end()[N.PROF_BATCH][0] / reps)
-        for buckets, tb, ts in ((32, lb, ls), (1, fb, fs)):
+        for buckets, wlog, tb, ts in ((32, WINDOW_LOG2, lb, ls), (1, WINDOW_LOG2, fb, fs), (32, 0, gb, gs)):
             gw.profile_begin()
             for _ in range(reps):
-                learned(buckets)
+                learned(buckets, wlog)
             p = gw.profile_end()
             tb.append(p[N.PROF_BATCH][0] / reps)
             ts.append(p.get(N.PROF_MLP, (0.0, 0))[0] / reps)
     learned(1)
     gw.sync()
     wc_faces = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+    learned(32, 0)
+    gw.sync()
+    wc_global = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+    learned()
+    gw.sync()
+    same = same and all(torch.equal(outl[k], outb[k]) for k in host)
     med = lambda x: float(np.median(x))
     p_ms, l_ms, l_sort, f_ms, f_sort = med(pb), med(lb), med(ls), med(fb), med(fs)
+    g_ms = med(gb) + med(gs)
     total = sum(counts.values())
-    r = {"shadow_order": "pixel, then sun and lights (fsMain's order)", "train_cameras": CAMS,
+    r = {"shadow_order": "pixel, then sun and lights (fsMain's order)",
+         "learned_order": "(direction face, 32 predicted-cost buckets) inside each window of 2^%d queries" % WINDOW_LOG2,
+         "train_cameras": CAMS,
          "train_pairs": int(n_train), "epochs": epochs, "batch": bs, "loss_first": round(first, 4),
          "loss_last": round(last, 4), "train_wall_s": round(train_s, 2),
          "per_kind_ms": round(per_kind_ms, 4), "batch_ms": round(p_ms, 4),
@@ -789,12 +801,14 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
          "speedup_vs_per_kind": round(per_kind_ms / (l_ms + l_sort), 3),
          "speedup_vs_batch": round(p_ms / (l_ms + l_sort), 3),
          "faces_only_ms": round(f_ms + f_sort, 4), "faces_only_speedup_vs_per_kind": round(per_kind_ms / (f_ms + f_sort), 3),
-         "shadow_wave_cost": {"index_order": wc_plain, "learned": wc_learned, "faces_only": wc_faces},
+         "global_learned_ms": round(g_ms, 4), "global_learned_speedup_vs_per_kind": round(per_kind_ms / g_ms, 3),
+         "shadow_wave_cost": {"index_order": wc_plain, "learned": wc_learned, "faces_only": wc_faces,
+                              "global_learned": wc_global},
          "bit_identical": bool(same)}
     log("learned-binned batch: %.3f + %.3f ms sort vs per-kind %.3f ms (x%.2f), vs batch %.3f ms (x%.2f); "
-        "faces only %.3f ms (x%.2f); same bits %s" %
+        "faces only %.3f ms (x%.2f); one global sort %.3f ms; same bits %s" %
         (l_ms, l_sort, per_kind_ms, r["speedup_vs_per_kind"], p_ms, r["speedup_vs_batch"], f_ms + f_sort,
-         r["faces_only_speedup_vs_per_kind"], same))
+         r["faces_only_speedup_vs_per_kind"], g_ms, same))
     model.close()
     return r
 
@@ -1134,14 +1148,14 @@ def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
         for k in (0, 1):
             sim.trace_device(k, rays[k][0], rays[k][1], outs[k])
 
-    def binned(buckets):
-        models[1].sort_rays(1, d_sh, ns, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True)
+    def binned(buckets, wlog=WINDOW_LOG2):
+        models[1].sort_rays(1, d_sh, ns, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True, window_log2=wlog)
         sim.trace_batch_perm(qb, {1: perm})
     sim.trace_batch(q)
     binned(32)
     gw.sync()
     same = all(torch.equal(outs[k], outs_b[k]) for k in (0, 1))
-    kb, pb, lb, ls, fb, fs = [], [], [], [], [], []
+    kb, pb, lb, ls, fb, fs, gb, gs = [], [], [], [], [], [], [], []
     for _ in range(7):
         gw.profile_begin()
         for _ in range(3):
@@ -1151,27 +1165,30 @@ def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
         for _ in range(3):
             sim.trace_batch(q)
         pb.append(gw.profile_end()[N.PROF_BATCH][0] / 3)
-        for buckets, tb, ts in ((32, lb, ls), (1, fb, fs)):
+        for buckets, wlog, tb, ts in ((32, WINDOW_LOG2, lb, ls), (1, WINDOW_LOG2, fb, fs), (32, 0, gb, gs)):
             gw.profile_begin()
             for _ in range(3):
-                binned(buckets)
+                binned(buckets, wlog)
             p = gw.profile_end()
             tb.append(p[N.PROF_BATCH][0] / 3)
             ts.append(p.get(N.PROF_MLP, (0.0, 0))[0] / 3)
     same = same and all(torch.equal(outs[k], outs_b[k]) for k in (0, 1))
     med = lambda x: float(np.median(x))
     k_ms, p_ms, l_ms, s_ms, f_ms = med(kb), med(pb), med(lb), med(ls), med(fb) + med(fs)
+    g_ms = med(gb) + med(gs)
     res["batch_fsmain_order"] = {
         "queries": {"primary": n, "shadow": int(ns)}, "shadow_order": "pixel, then sun and lights (fsMain's order)",
+        "learned_order": "(direction face, 32 predicted-cost buckets) inside each window of 2^%d queries" % WINDOW_LOG2,
         "per_kind_ms": round(k_ms, 4), "batch_ms": round(p_ms, 4), "learned_trace_ms": round(l_ms, 4),
         "learned_sort_ms": round(s_ms, 4), "learned_grays_s": round(tot / ((l_ms + s_ms) * 1e-3) / 1e9, 4),
         "per_kind_grays_s": round(tot / (k_ms * 1e-3) / 1e9, 4),
         "speedup_vs_per_kind": round(k_ms / (l_ms + s_ms), 3), "speedup_vs_batch": round(p_ms / (l_ms + s_ms), 3),
         "faces_only_ms": round(f_ms, 4), "faces_only_speedup_vs_per_kind": round(k_ms / f_ms, 3),
+        "global_learned_ms": round(g_ms, 4), "global_learned_speedup_vs_per_kind": round(k_ms / g_ms, 3),
         "bit_identical": bool(same)}
     log("frame scene, fsMain order: learned-binned batch %.3f + %.3f ms vs per-kind %.3f ms (x%.2f), "
-        "vs batch %.3f ms (x%.2f); faces only %.3f ms; same bits %s" %
-        (l_ms, s_ms, k_ms, k_ms / (l_ms + s_ms), p_ms, p_ms / (l_ms + s_ms), f_ms, same))
+        "vs batch %.3f ms (x%.2f); faces only %.3f ms; one global sort %.3f ms; same bits %s" %
+        (l_ms, s_ms, k_ms, k_ms / (l_ms + s_ms), p_ms, p_ms / (l_ms + s_ms), f_ms, g_ms, same))
     return res
 
 if __name__ == "__main__":

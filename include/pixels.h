@@ -569,11 +569,16 @@ int px_bin_rays(px_ctx* ctx, const float* d_pred, uint32_t n, uint32_t window, u
 /* Sort keys of px_mlp_sort_rays: the cost bucket alone (0), or the major
  * axis and sign of the ray direction (6 faces), then the cost bucket. */
 #define PX_SORT_FACES 1u
+/* Sort each window of 2^log2 consecutive rays (log2 8..30) on its own, the
+ * windows in input order, instead of all n rays at once (0): rays that come
+ * in a spatially coherent order keep part of it. */
+#define PX_SORT_WINDOW(log2) ((uint32_t)(log2) << 8)
 /* Features, prediction and a stable counting sort of ALL n rays by key, in
  * three launches.  The key is the cost bucket (`buckets` of the predicted
  * log2 steps over [lo, hi), 1..256), or with PX_SORT_FACES (face, bucket)
  * with face = 2 * major axis + (component < 0) of the direction (the first of
- * equal magnitudes wins; buckets 1..42).  Without flags d_perm is the order
+ * equal magnitudes wins; buckets 1..42); with PX_SORT_WINDOW(k), by (window
+ * of 2^k rays, key).  Without flags d_perm is the order
  * px_bin_rays(window = n) gives for px_mlp_predict's output on
  * px_mlp_features(kind, d_rays).  Device scratch (keys, per-block counts)
  * grows with n and lives with the model.  Calls on one model are ordered on

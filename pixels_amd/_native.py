@@ -163,6 +163,12 @@ SIGNATURES = {
     "px_profile_end": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 SORT_FACES = 1  # px_mlp_sort_rays flag: keys (direction face, cost bucket)
+
+
+def SORT_WINDOW(log2: int) -> int:
+    """px_mlp_sort_rays flag: sort each window of 2**log2 consecutive rays
+    (log2 8..30) on its own, the windows in input order."""
+    return (int(log2) & 31) << 8
 PROF_PRIMARY, PROF_SHADOW, PROF_TRACE, PROF_EDIT, PROF_BATCH, PROF_MLP, PROF_PARTICLES = 0, 1, 2, 3, 4, 5, 6
 PROF_PARTICLE_SIM, PROF_WRITES, PROF_CHUNKS = 7, 8, 9
 PROF_DENSITY, PROF_RENDER_PRIMARY, PROF_RENDER_SHADOW, PROF_SHADE, PROF_REFLECT = 10, 11, 12, 13, 14

@@ -776,18 +776,36 @@ __global__ __launch_bounds__(256) void k_sort_offsets(const uint32_t* __restrict
   }
 }
 
-// perm[base[key] + off[key][g] + rank] = i, base[key] = sum of the totals of
-// the keys before it.
+// perm[base[key] + off[key][g] - off[key][g0] + rank] = i for the rays of
+// the window of blocks [g0, g1) (all blocks when the sort has no windows):
+// base[key] = the window's first ray + the window's counts of the keys
+// before it, each count off[k][g1] - off[k][g0] (tot[k] stands for
+// off[k][nblk]).  Windows are whole tiles of 256 rays, so a block of this
+// kernel lies in one.
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t* __restrict__ keys,
                                                       const uint32_t* __restrict__ off,
                                                       const uint32_t* __restrict__ tot, uint32_t nkeys, uint32_t n,
-                                                      uint32_t nblk, uint32_t rays_per_block,
+                                                      uint32_t nblk, uint32_t rays_per_block, uint32_t window_log2,
                                                       uint32_t* __restrict__ perm) {
   __shared__ uint32_t s_base[kSortMaxKeys];
+  __shared__ uint32_t s_w0[kSortMaxKeys];
   __shared__ uint32_t s_tmp[4];
+  const uint32_t i0 = blockIdx.x * 256u;
+  uint32_t first = 0u, g0 = 0u, g1 = nblk;
+  if (window_log2) {
+    const uint32_t win = i0 >> window_log2, bpw = (1u << window_log2) / rays_per_block;
+    first = win << window_log2;
+    g0 = win * bpw;
+    g1 = min(g0 + bpw, nblk);
+  }
   {
     const uint32_t t = threadIdx.x, l = __lane_id(), wv = t >> 6;
-    const uint32_t v = t < nkeys ? tot[t] : 0u;
+    uint32_t v = 0u, p0 = 0u;
+    if (t < nkeys) {
+      p0 = g0 < nblk ? off[t * nblk + g0] : tot[t];
+      v = (g1 < nblk ? off[t * nblk + g1] : tot[t]) - p0;
+    }
+    s_w0[t] = p0;
     uint32_t x = v;
     for (uint32_t o = 1; o < 64u; o <<= 1) {
       const uint32_t y = __shfl_up(x, o);
@@ -795,15 +813,15 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t* __restrict
     }
     if (l == 63u) s_tmp[wv] = x;
     __syncthreads();
-    uint32_t base = 0;
+    uint32_t base = first;
     for (uint32_t k = 0; k < wv; k++) base += s_tmp[k];
     s_base[t] = base + x - v;
   }
   __syncthreads();
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t i = i0 + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = keys[i], key = k >> 24;
-  const uint32_t dst = s_base[key] + off[key * nblk + i / rays_per_block] + (k & 0xFFFFFFu);
+  const uint32_t dst = s_base[key] + (off[key * nblk + i / rays_per_block] - s_w0[key]) + (k & 0xFFFFFFu);
   if (dst < n) perm[dst] = i;  // always true for consistent counts
 }
 
@@ -1047,8 +1065,11 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   if (!m || (n && (!d_rays || !d_perm))) return PX_ERR_ARG;
   px_ctx* c = m->c;
   if (kind > 3u) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown ray kind");
-  if (flags & ~PX_SORT_FACES) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown flags");
+  if (flags & ~(PX_SORT_FACES | PX_SORT_WINDOW(31u))) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "unknown flags");
   const uint32_t faces = (flags & PX_SORT_FACES) ? 1u : 0u;
+  const uint32_t wlog = (flags >> 8) & 31u;
+  if (wlog != 0u && (wlog < 8u || wlog > 30u))
+    return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "window: 2^8 .. 2^30 rays");
   if (buckets == 0u || (uint64_t)buckets * (faces ? 6u : 1u) > kSortMaxKeys)
     return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "buckets: 1..256, or 1..42 with PX_SORT_FACES");
   if (!(hi > lo)) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "need hi > lo");
@@ -1061,7 +1082,13 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   // each; the block count is padded to a multiple of 4 (16-byte scan groups)
   const uint32_t ntiles = (n + 255u) / 256u;
   const uint32_t target = std::max(4u, c->n_cu * 4u);
-  const uint32_t per = (ntiles + target - 1u) / target;
+  uint32_t per = (ntiles + target - 1u) / target;
+  if (wlog) {  // a power of two of tiles that divides the window's, so no block straddles two windows
+    const uint32_t tw = 1u << (wlog - 8u);
+    uint32_t p = 1u;
+    while (p * 2u <= per && p * 2u <= tw) p *= 2u;
+    per = p;
+  }
   const uint32_t nblk = ((ntiles + per - 1u) / per + 3u) & ~3u;
   if ((size_t)per * 256u >= (size_t)1 << 24) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "too many rays");
   const size_t need = 2u * (size_t)nkeys * nblk + kSortMaxKeys + n;
@@ -1091,7 +1118,7 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   hipLaunchKernelGGL(k_sort_offsets, dim3(nkeys), dim3(256), 0, c->stream, part, nblk / 4u, off, tot);
   PX_HIP_CHECK(c, hipGetLastError());
   hipLaunchKernelGGL(k_sort_scatter, dim3((n + 255u) / 256u), dim3(256), 0, c->stream, keys, off, tot, nkeys, n,
-                     nblk, per * 256u, d_perm);
+                     nblk, per * 256u, wlog, d_perm);
   PX_HIP_CHECK(c, hipGetLastError());
   return PX_OK;
 }

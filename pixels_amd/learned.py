@@ -70,22 +70,25 @@ class CostModel:
         return float(loss.value), g
 
     def sort_rays(self, kind: int, rays_t, n: int, perm_t, buckets: int = 32, lo: float = 0.0,
-                  hi: float = 9.0, faces: bool = False) -> None:
+                  hi: float = 9.0, faces: bool = False, window_log2: int = 0) -> None:
         """Features, prediction and a stable sort of all n rays by cost
         bucket (three launches), or by (direction face, cost bucket) with
-        faces=True.  Without faces, perm_t gets the order
-        bin_rays(window=n) gives for predict(features(rays))."""
+        faces=True; with window_log2, each window of 2**window_log2
+        consecutive rays is sorted on its own.  Without faces or windows,
+        perm_t gets the order bin_rays(window=n) gives for
+        predict(features(rays))."""
         self.gw.after_torch()
-        self.gw.check(self.lib.px_mlp_sort_rays(self.h, kind, _dp(rays_t), n, buckets, lo, hi,
-                                                N.SORT_FACES if faces else 0, _dp(perm_t)))
+        flags = (N.SORT_FACES if faces else 0) | (N.SORT_WINDOW(window_log2) if window_log2 else 0)
+        self.gw.check(self.lib.px_mlp_sort_rays(self.h, kind, _dp(rays_t), n, buckets, lo, hi, flags, _dp(perm_t)))
 
     def trace_sorted(self, kind: int, rays_t, n: int, perm_t, out_t, buckets: int = 32, lo: float = 0.0,
-                     hi: float = 9.0, faces: bool = False) -> None:
+                     hi: float = 9.0, faces: bool = False, window_log2: int = 0) -> None:
         """sort_rays, then the permuted launch, in one call (one submission
         of four kernels); out_t gets trace_device's hits bit for bit."""
         self.gw.after_torch()
-        self.gw.check(self.lib.px_mlp_trace_sorted(self.h, kind, _dp(rays_t), n, buckets, lo, hi,
-                                                   N.SORT_FACES if faces else 0, _dp(perm_t), _dp(out_t)))
+        flags = (N.SORT_FACES if faces else 0) | (N.SORT_WINDOW(window_log2) if window_log2 else 0)
+        self.gw.check(self.lib.px_mlp_trace_sorted(self.h, kind, _dp(rays_t), n, buckets, lo, hi, flags,
+                                                   _dp(perm_t), _dp(out_t)))
 
     def train_step(self, feat_t, target_t, n: int, lr: float = 1e-3, want_loss: bool = False):
         """One Adam step, asynchronous on the library's stream unless

@@ -123,12 +123,13 @@ def test_batched_mix_full_size():
         for k in host:
             assert np.array_equal(outb[k].cpu().numpy(), outp[k].cpu().numpy()), "replay, kind %d" % k
         # the learned-binned form: the 3M shadow queries in the predictor's
-        # (direction face, cost bucket) order through px_trace_batch_perm
+        # (direction face, cost bucket) order inside windows of 2^18 queries,
+        # through px_trace_batch_perm
         from pixels_amd.learned import CostModel
         m = CostModel(gw, seed=4)
         try:
             perm = torch.zeros(counts[1], dtype=torch.int32, device="cuda")
-            m.sort_rays(1, rays[1], counts[1], perm, buckets=32, faces=True)
+            m.sort_rays(1, rays[1], counts[1], perm, buckets=32, faces=True, window_log2=18)
             for t in outb.values():
                 t.fill_(0x22)
             print("case: batched launch, shadow kind in the learned order", flush=True)

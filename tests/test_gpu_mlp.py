@@ -251,6 +251,17 @@ def test_fused_global_sort_equals_separate_kernels(world, kind):
             m.sort_rays(kind, d, n, permf, buckets=nb, lo=lo, hi=hi, faces=True)
             gw.sync()
             assert np.array_equal(permf.cpu().numpy().astype(np.int64), np.lexsort((np.arange(n), b, face)))
+            # windows of 2^k consecutive rays, each sorted on its own (k = 17
+            # holds all 129,600 rays: the global sort again)
+            for wlog in (8, 12, 17):
+                win = np.arange(n) >> wlog
+                print("case: windowed sort kind=%d n=%d window=2^%d" % (kind, n, wlog), flush=True)
+                for faces, exp in ((True, np.lexsort((np.arange(n), b, face, win))),
+                                   (False, np.lexsort((np.arange(n), b, win)))):
+                    permw = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+                    m.sort_rays(kind, d, n, permw, buckets=nb, lo=lo, hi=hi, faces=faces, window_log2=wlog)
+                    gw.sync()
+                    assert np.array_equal(permw.cpu().numpy().astype(np.int64), exp), (faces, wlog)
             if n > 1000:
                 assert len(np.unique(b)) > nb // 2  # the buckets are really used
                 plain = _trace_steps(sim, gw, kind, rays)
@@ -273,8 +284,9 @@ def test_fused_global_sort_equals_separate_kernels(world, kind):
 
 def test_sort_rays_rejects_bad_arguments(world):
     """Unknown flags, more than 256 keys (42 buckets with faces; a bucket
-    count whose 6x overflows 32 bits), an empty bucket range and an unknown
-    kind are errors, and nothing is written."""
+    count whose 6x overflows 32 bits), an empty bucket range, a window below
+    one 256-ray tile or above 2^30 rays and an unknown kind are errors, and
+    nothing is written."""
     import torch
     from pixels_amd import PxError
     from pixels_amd.learned import CostModel
@@ -286,7 +298,7 @@ def test_sort_rays_rejects_bad_arguments(world):
         d = _dev(rays.view(np.uint8))
         perm = torch.full((n,), -1, dtype=torch.int32, device="cuda")
         bad = [dict(buckets=43, faces=True), dict(buckets=257), dict(buckets=0), dict(buckets=0x2AAAAAAB, faces=True),
-               dict(lo=3.0, hi=3.0)]
+               dict(lo=3.0, hi=3.0), dict(window_log2=7), dict(window_log2=31)]
         for kw in bad:
             with pytest.raises(PxError):
                 m.sort_rays(1, d, n, perm, **kw)
