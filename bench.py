@@ -757,6 +757,12 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
         model.sort_rays(1, rays[1], n_sh, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True, window_log2=wlog)
         sim.trace_batch_perm(ql, {1: perm})
 
+    # each order with the launch shape the autotuner picks for it
+    shapes = {"plain": sim.get_batch_shape()}
+    for name, buckets, wlog in (("faces", 1, WINDOW_LOG2), ("global", 32, 0), ("learned", 32, WINDOW_LOG2)):
+        learned(buckets, wlog)
+        shapes[name], _ = sim.autotune_batch(ql, repeats=2, perms={1: perm})
+    use = lambda name: sim.set_batch_shape(shapes[name]["block"], shapes[name]["blocks"], shapes[name]["chunk"])
     learned()
     gw.sync()
     same = all(torch.equal(outl[k], outb[k]) for k in host)
@@ -766,11 +772,16 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
     reps, rounds = 3, 7
     pb, lb, ls, fb, fs, gb, gs = [], [], [], [], [], [], []
     for _ in range(rounds):  # interleaved, medians
+        use("plain")
+        sim.trace_batch(q)
         gw.profile_begin()
         for _ in range(reps):
             sim.trace_batch(q)
         pb.append(gw.profile_end()[N.PROF_BATCH][0] / reps)
-        for buckets, wlog, tb, ts in ((32, WINDOW_LOG2, lb, ls), (1, WINDOW_LOG2, fb, fs), (32, 0, gb, gs)):
+        for name, buckets, wlog, tb, ts in (("learned", 32, WINDOW_LOG2, lb, ls), ("faces", 1, WINDOW_LOG2, fb, fs),
+                                            ("global", 32, 0, gb, gs)):
+            use(name)
+            learned(buckets, wlog)
             gw.profile_begin()
             for _ in range(reps):
                 learned(buckets, wlog)
@@ -783,9 +794,11 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
     learned(32, 0)
     gw.sync()
     wc_global = wave_cost(steps[perm.cpu().numpy().astype(np.int64)])
+    use("learned")
     learned()
     gw.sync()
     same = same and all(torch.equal(outl[k], outb[k]) for k in host)
+    use("plain")
     med = lambda x: float(np.median(x))
     p_ms, l_ms, l_sort, f_ms, f_sort = med(pb), med(lb), med(ls), med(fb), med(fs)
     g_ms = med(gb) + med(gs)
@@ -802,7 +815,7 @@ def batch_learned(args, scene, sim, gw, host, rays, counts, q, outb, per_kind_ms
          "speedup_vs_batch": round(p_ms / (l_ms + l_sort), 3),
          "faces_only_ms": round(f_ms + f_sort, 4), "faces_only_speedup_vs_per_kind": round(per_kind_ms / (f_ms + f_sort), 3),
          "global_learned_ms": round(g_ms, 4), "global_learned_speedup_vs_per_kind": round(per_kind_ms / g_ms, 3),
-         "shadow_wave_cost": {"index_order": wc_plain, "learned": wc_learned, "faces_only": wc_faces,
+         "launch_shapes": shapes, "shadow_wave_cost": {"index_order": wc_plain, "learned": wc_learned, "faces_only": wc_faces,
                               "global_learned": wc_global},
          "bit_identical": bool(same)}
     log("learned-binned batch: %.3f + %.3f ms sort vs per-kind %.3f ms (x%.2f), vs batch %.3f ms (x%.2f); "
@@ -1151,7 +1164,16 @@ def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
     def binned(buckets, wlog=WINDOW_LOG2):
         models[1].sort_rays(1, d_sh, ns, perm, buckets=buckets, lo=0.0, hi=9.0, faces=True, window_log2=wlog)
         sim.trace_batch_perm(qb, {1: perm})
+    # each batched launch below with the shape the autotuner picks for its order
+    shapes = {}
+    shapes["plain"], _ = sim.autotune_batch(q, repeats=2)
+    for name, buckets, wlog in (("faces", 1, WINDOW_LOG2), ("global", 32, 0), ("learned", 32, WINDOW_LOG2)):
+        binned(buckets, wlog)
+        shapes[name], _ = sim.autotune_batch(qb, repeats=2, perms={1: perm})
+    use = lambda name: sim.set_batch_shape(shapes[name]["block"], shapes[name]["blocks"], shapes[name]["chunk"])
+    use("plain")
     sim.trace_batch(q)
+    use("learned")
     binned(32)
     gw.sync()
     same = all(torch.equal(outs[k], outs_b[k]) for k in (0, 1))
@@ -1161,11 +1183,16 @@ def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
         for _ in range(3):
             per_kind()
         kb.append(gw.profile_end()[N.PROF_TRACE][0] / 3)
+        use("plain")
+        sim.trace_batch(q)
         gw.profile_begin()
         for _ in range(3):
             sim.trace_batch(q)
         pb.append(gw.profile_end()[N.PROF_BATCH][0] / 3)
-        for buckets, wlog, tb, ts in ((32, WINDOW_LOG2, lb, ls), (1, WINDOW_LOG2, fb, fs), (32, 0, gb, gs)):
+        for name, buckets, wlog, tb, ts in (("learned", 32, WINDOW_LOG2, lb, ls), ("faces", 1, WINDOW_LOG2, fb, fs),
+                                            ("global", 32, 0, gb, gs)):
+            use(name)
+            binned(buckets, wlog)
             gw.profile_begin()
             for _ in range(3):
                 binned(buckets, wlog)
@@ -1185,7 +1212,7 @@ def frame_and_batch_learned(args, scene, gw, sim, held, codes, models):
         "speedup_vs_per_kind": round(k_ms / (l_ms + s_ms), 3), "speedup_vs_batch": round(p_ms / (l_ms + s_ms), 3),
         "faces_only_ms": round(f_ms, 4), "faces_only_speedup_vs_per_kind": round(k_ms / f_ms, 3),
         "global_learned_ms": round(g_ms, 4), "global_learned_speedup_vs_per_kind": round(k_ms / g_ms, 3),
-        "bit_identical": bool(same)}
+        "launch_shapes": shapes, "bit_identical": bool(same)}
     log("frame scene, fsMain order: learned-binned batch %.3f + %.3f ms vs per-kind %.3f ms (x%.2f), "
         "vs batch %.3f ms (x%.2f); faces only %.3f ms; one global sort %.3f ms; same bits %s" %
         (l_ms, s_ms, k_ms, k_ms / (l_ms + s_ms), p_ms, p_ms / (l_ms + s_ms), f_ms, g_ms, same))

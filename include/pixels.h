@@ -297,6 +297,11 @@ int px_get_batch_shape(const px_ctx* ctx, uint32_t out[3]);
  * launches each, after one warm-up), leaves the fastest set and writes it to
  * out[3] and its mean time to *best_ms.  Blocking. */
 int px_autotune_batch(px_ctx* ctx, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms);
+/* px_autotune_batch timed on px_trace_batch_perm(ctx, q, d_perm): the shape
+ * for the launch in that order (no counterpart in the reference; for callers
+ * of px_trace_batch_perm, whose best shape differs from the index order's). */
+int px_autotune_batch_perm(px_ctx* ctx, const px_query_set* q, const uint32_t* const d_perm[4], uint32_t repeats,
+                           uint32_t out[3], float* best_ms);
 
 /* Replaces Simulation.probe (Simulation.ts:161-196, probe.wgsl:25-32):
  * decoded voxel values of the box [o, o+s), x fastest.  Host buffer. */

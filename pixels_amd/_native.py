@@ -112,6 +112,7 @@ SIGNATURES = {
     "px_set_batch_shape": (C.c_int, [_P, _U32, _U32, _U32]),
     "px_get_batch_shape": (C.c_int, [_P, _U32P]),
     "px_autotune_batch": (C.c_int, [_P, C.POINTER(PxQuerySet), _U32, _U32P, C.POINTER(C.c_float)]),
+    "px_autotune_batch_perm": (C.c_int, [_P, C.POINTER(PxQuerySet), _P, _U32, _U32P, C.POINTER(C.c_float)]),
     "px_probe": (C.c_int, [_P, _I32P, _I32P, _U32P]),
     "px_probe_device": (C.c_int, [_P, _I32P, _I32P, _P]),
     "px_read_ctrl": (C.c_int, [_P, _U32P, _U32P, _FP]),

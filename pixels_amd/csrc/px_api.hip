@@ -524,7 +524,16 @@ int px_autotune_batch(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32
   int rc = check_query_set(c, q, "px_autotune_batch");
   if (rc) return rc;
   PX_HIP_CHECK(c, hipSetDevice(c->device));
-  return px_autotune_batch_impl(c, q, repeats ? repeats : 3u, out, best_ms);
+  return px_autotune_batch_impl(c, q, nullptr, repeats ? repeats : 3u, out, best_ms);
+}
+
+int px_autotune_batch_perm(px_ctx* c, const px_query_set* q, const uint32_t* const d_perm[4], uint32_t repeats,
+                           uint32_t out[3], float* best_ms) {
+  if (!c || !out) return PX_ERR_ARG;
+  int rc = check_query_set(c, q, "px_autotune_batch_perm");
+  if (rc) return rc;
+  PX_HIP_CHECK(c, hipSetDevice(c->device));
+  return px_autotune_batch_impl(c, q, d_perm, repeats ? repeats : 3u, out, best_ms);
 }
 
 static bool box_ok(const px_ctx* c, const int32_t s[3]) {

@@ -222,7 +222,8 @@ int px_refresh_density(px_ctx* c);
 int px_launch_render(px_ctx* c, const px_ray* d_rays, uint32_t n, float* d_rgba);
 int px_launch_trace_perm(px_ctx* c, uint32_t kind, const px_ray* d_rays, const uint32_t* d_perm, uint32_t n,
                          px_hit* d_out);
-int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms);
+int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, const uint32_t* const* perm, uint32_t repeats,
+                           uint32_t out[3], float* best_ms);
 int px_launch_masks(px_ctx* c);
 // edit chain (px_edit.hip)
 int px_edit_alloc(px_ctx* c);

@@ -737,7 +737,8 @@ int px_launch_batch(px_ctx* c, const px_query_set* q, const uint32_t* const* per
   return PX_OK;
 }
 
-int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, uint32_t out[3], float* best_ms) {
+int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, const uint32_t* const* perm, uint32_t repeats,
+                           uint32_t out[3], float* best_ms) {
   static const uint32_t blocks_opt[3] = {64u, 128u, 256u};
   static const uint32_t waves_opt[3] = {8u, 16u, 24u};  // wavefronts per CU
   static const uint32_t chunk_opt[3] = {64u, 256u, 1024u};
@@ -757,10 +758,10 @@ int px_autotune_batch_impl(px_ctx* c, const px_query_set* q, uint32_t repeats, u
         c->batch_block = blocks_opt[bi];
         c->batch_blocks = c->n_cu * waves_opt[wi] * 64u / blocks_opt[bi];
         c->batch_chunk = chunk_opt[ci];
-        rc = px_launch_batch(c, q, nullptr);  // capture + warm-up
+        rc = px_launch_batch(c, q, perm);  // capture + warm-up
         if (rc) break;
         if (hipEventRecord(e0, c->stream) != hipSuccess) rc = px_fail(c, PX_ERR_HIP, "px_autotune_batch", "event");
-        for (uint32_t r = 0; r < repeats && rc == PX_OK; r++) rc = px_launch_batch(c, q, nullptr);
+        for (uint32_t r = 0; r < repeats && rc == PX_OK; r++) rc = px_launch_batch(c, q, perm);
         if (rc) break;
         if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
           rc = px_fail(c, PX_ERR_HIP, "px_autotune_batch", "event");

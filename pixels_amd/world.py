@@ -303,10 +303,20 @@ class Simulation:
         self.gw.check(self.lib.px_get_batch_shape(self.gw.ctx, N.ptr(out)))
         return dict(block=int(out[0]), blocks=int(out[1]), chunk=int(out[2]))
 
-    def autotune_batch(self, q: N.PxQuerySet, repeats: int = 3):
+    def autotune_batch(self, q: N.PxQuerySet, repeats: int = 3, perms: dict | None = None):
+        """px_autotune_batch, or px_autotune_batch_perm for the launch in the
+        orders `perms` ({kind: int32 tensor}); leaves the best shape set."""
         out = np.zeros(3, np.uint32)
         ms = C.c_float(0.0)
-        self.gw.check(self.lib.px_autotune_batch(self.gw.ctx, C.byref(q), repeats, N.ptr(out), C.byref(ms)))
+        if perms is None:
+            self.gw.check(self.lib.px_autotune_batch(self.gw.ctx, C.byref(q), repeats, N.ptr(out), C.byref(ms)))
+        else:
+            arr = (C.c_void_p * 4)()
+            for k, t in perms.items():
+                arr[k] = _devptr(t)
+            self.gw.after_torch()
+            self.gw.check(self.lib.px_autotune_batch_perm(self.gw.ctx, C.byref(q), C.cast(arr, C.c_void_p), repeats,
+                                                          N.ptr(out), C.byref(ms)))
         return dict(block=int(out[0]), blocks=int(out[1]), chunk=int(out[2])), float(ms.value)
 
     # -- edit chain -----------------------------------------------------------

@@ -134,6 +134,21 @@ def test_autotune_keeps_results(batch):
     got = batch.results()
     for k in ref:
         assert _same(got[k], ref[k])
+    # the shape for the launch in given orders (px_autotune_batch_perm), here
+    # each kind reversed: the tuner leaves its pick set and the permuted
+    # launch still gives the per-kind hits
+    import torch
+    perms = {k: torch.arange(int(batch.rays[k].numel()) // 32 - 1, -1, -1, dtype=torch.int32, device="cuda")
+             for k in range(4) if int(batch.rays[k].numel()) >= 32}
+    shape_p, ms_p = batch.sim.autotune_batch(batch.qs, repeats=1, perms=perms)
+    print("case: autotuned for reversed orders", shape_p, ms_p, flush=True)
+    assert shape_p["block"] in (64, 128, 256) and shape_p["chunk"] in (64, 256, 1024) and ms_p > 0
+    assert batch.sim.get_batch_shape() == shape_p
+    batch.clear()
+    batch.sim.trace_batch_perm(batch.qs, perms)
+    got = batch.results()
+    for k in ref:
+        assert _same(got[k], ref[k])
 
 
 def test_empty_and_one_kind(small_scene):
