@@ -1083,11 +1083,23 @@ int px_mlp_sort_rays(px_mlp* m, uint32_t kind, const px_ray* d_rays, uint32_t n,
   const uint32_t ntiles = (n + 255u) / 256u;
   const uint32_t target = std::max(4u, c->n_cu * 4u);
   uint32_t per = (ntiles + target - 1u) / target;
-  if (wlog) {  // a power of two of tiles that divides the window's, so no block straddles two windows
+  if (wlog) {
+    // a power of two of tiles that divides the window's, so no block
+    // straddles two windows: the one with the fewest turns per block over
+    // the rounds of resident blocks (about `target` at a time), counting a
+    // block's start (network weights into LDS) as 4 tile turns
     const uint32_t tw = 1u << (wlog - 8u);
-    uint32_t p = 1u;
-    while (p * 2u <= per && p * 2u <= tw) p *= 2u;
-    per = p;
+    uint32_t best = 1u;
+    uint64_t best_cost = ~0ull;
+    for (uint32_t p = 1u; p <= per && p <= tw; p *= 2u) {
+      const uint64_t blocks = (ntiles + p - 1u) / p;
+      const uint64_t cost = ((blocks + target - 1u) / target) * (p + 4u);
+      if (cost <= best_cost) {
+        best_cost = cost;
+        best = p;
+      }
+    }
+    per = best;
   }
   const uint32_t nblk = ((ntiles + per - 1u) / per + 3u) & ~3u;
   if ((size_t)per * 256u >= (size_t)1 << 24) return px_fail(c, PX_ERR_ARG, "px_mlp_sort_rays", "too many rays");
